@@ -1,0 +1,121 @@
+// Internal structures of libcgef_hip.so: device-resident grid, footprint plan and workspace.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/cgef.h"
+
+namespace cgef {
+
+// Error plumbing: the C ABI returns a status and keeps a per-thread message.
+void set_error(const std::string &msg);
+struct Error {
+    cgef_status status;
+    std::string msg;
+};
+[[noreturn]] void fail(cgef_status s, const std::string &msg);
+void hip_check(hipError_t e, const char *what);
+#define CGEF_HIP(call) ::cgef::hip_check((call), #call)
+
+// One run of the banded footprint: source row (dy, dz) relative to the target row, columns
+// di = lo..hi relative to the target column, weights at w[woff + (di - lo)].
+struct alignas(32) Run {
+    int32_t dy, dz, lo, hi;
+    int32_t woff, pad0, pad1, pad2;
+};
+
+struct Workspace;
+
+struct Grid {
+    int device = 0;
+    int ndim = 2, geometry = CGEF_CARTESIAN;
+    int64_t n[3] = {1, 1, 1};
+    int periodic[3] = {0, 0, 0}, uniform[3] = {1, 1, 1};
+    double period[3] = {0, 0, 0};
+    double R = 1.0;
+    std::vector<double> coord[3], width[3];
+    std::vector<uint8_t> mask_h;  // empty = all active
+    bool has_mask = false;
+    double active_area = 0.0;
+    int64_t npts = 0, nrows = 0;  // npts = Nx*Ny*Nz, nrows = Ny*Nz
+    // device
+    uint8_t *mask_d = nullptr;    // always allocated (all ones when unmasked)
+    double *coord_d[3] = {nullptr, nullptr, nullptr};
+    double *width_d[3] = {nullptr, nullptr, nullptr};
+    double *trig_d = nullptr;     // sphere: [sinλ(Nx) cosλ(Nx) sinφ(Ny) cosφ(Ny)]
+    std::atomic<int> refs{1};
+    std::mutex ws_mu;
+    Workspace *internal_ws = nullptr;
+    double cell_area(int64_t i, int64_t j, int64_t k) const;
+    ~Grid();
+};
+
+struct Plan {
+    Grid *grid = nullptr;
+    int kind = CGEF_GAUSSIAN;
+    double alpha = 6.0, scale = 0.0, rad = 0.0;
+    int strategy = CGEF_ZEROFILL;
+    int engine = CGEF_ENGINE_BANDED;
+    bool edge_x = false;          // non-periodic x: per-point in-range denominator
+    // banded tables (host)
+    int nbands = 0;
+    std::vector<int32_t> band_of_row, run_ptr, row_order;
+    std::vector<Run> runs;
+    std::vector<double> w64, w32d, D64_row, D32_row;
+    std::vector<float> w32;
+    int64_t nentries = 0;
+    double mean_taps = 0.0;
+    int64_t max_taps = 0;
+    double active_taps = 0.0;     // Σ over active target points of their in-support taps
+    // banded tables (device)
+    Run *runs_d = nullptr;
+    int32_t *run_ptr_d = nullptr, *band_of_row_d = nullptr, *row_order_d = nullptr;
+    double *w64_d = nullptr, *w32d_d = nullptr, *D64_row_d = nullptr, *D32_row_d = nullptr;
+    float *w32_d = nullptr;
+    // plain CSR of the baseline engine (device), built on first use
+    int32_t *csr_di_d = nullptr, *csr_dj_d = nullptr, *csr_dk_d = nullptr;
+    int64_t *csr_ptr_d = nullptr;
+    // separable engine (device): per-axis 1-D weights and ZeroFill profiles / Deformable 1/den
+    int sep_lim[2] = {0, 0};
+    double *sep_g_d[2] = {nullptr, nullptr};
+    double *sep_prof_d[2] = {nullptr, nullptr};
+    double *sep_invden_d = nullptr;
+    // optional kernel timing: event pairs around each dominant-kernel launch
+    bool profiling = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_events;
+    size_t prof_used = 0;
+    void prof_begin(hipStream_t s);
+    void prof_end(hipStream_t s);
+    ~Plan();
+};
+
+struct Workspace {
+    Grid *grid = nullptr;
+    int64_t nbatch = 0;
+    int dtype = CGEF_F32;
+    // fused path
+    void *qpack = nullptr;        // T4 per point per slice
+    double *ubar = nullptr;       // 3 planes per slice (local frame)
+    void *tau = nullptr;          // 6 planes per slice, element type T
+    double *cmean = nullptr;      // 3 per slice
+    double *partials = nullptr;   // reduction scratch
+    int64_t npartials = 0;
+    double *E_dev = nullptr;      // per-slice energies
+    // unfused (baseline / separable) path: FP64 planes, grown on demand
+    double *scratch64 = nullptr;
+    int64_t nscratch64 = 0;
+    void *fbuf = nullptr;         // filtered-field / row-pass scratch, grown on demand
+    size_t nfbuf = 0;
+    ~Workspace();
+};
+
+// Footprint construction (host, FP64), cgef_footprint.cpp
+void build_banded(Plan &p);
+void build_separable(Plan &p);
+
+}  // namespace cgef

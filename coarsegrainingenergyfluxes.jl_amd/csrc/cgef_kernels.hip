@@ -1,0 +1,949 @@
+// Device code of libcgef_hip.so (gfx950 / CDNA4).
+//
+// Hot path: k_pi_filter, the fused footprint filter of the (mean-shifted) velocity components and
+// their centred quadratic products for Π. One workgroup = one target row × 256 consecutive target
+// columns, one target per lane. Per footprint run (one source row) the workgroup stages the source
+// segment (velocity components + mask flag, 16 B per cell in FP32) in LDS once, then every lane
+// walks the run with wave-uniform loop bounds and wave-uniform (scalar-loaded) weights, so a weight
+// costs no VGPR traffic and each staged cell is reused by up to 256 targets.
+//
+// Precision (FP32 path): pass 1 accumulates Σ w·q in FP64 (v_fma_f64 runs at the FP32 VALU rate on
+// CDNA4), so ū carries no accumulation error beyond its FP32 inputs. Pass 2 accumulates the CENTRED
+// products Σ w·(q_a − q̄_a)(q_b − q̄_b) in FP32: positive-definite sums with no cancellation, and an
+// error in the centre enters τ only at second order. τ = F[(q−q̄)(q−q̄)] + (1−m)[…] with m = F[1]
+// (ZeroFill near a mask) is algebraically F[q_a q_b] − q̄_a q̄_b, i.e. the reference's τ.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <type_traits>
+
+#include "cgef_launch.hpp"
+
+namespace cgef {
+
+template <typename T> struct V4;
+template <> struct V4<float> { using type = float4; };
+template <> struct V4<double> { using type = double4; };
+
+__device__ __forceinline__ int64_t wrapi(int64_t a, int64_t n) {
+    int64_t r = a % n;
+    return r < 0 ? r + n : r;
+}
+
+template <typename T>
+__device__ __forceinline__ T ld(const void *p, int64_t i) {
+    return static_cast<const T *>(p)[i];
+}
+
+// Wave/block reductions in FP64 (deterministic tree order).
+__device__ __forceinline__ double warp_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+
+template <int NT>
+__device__ double block_sum(double v, double *sh) {
+    v = warp_sum(v);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0) {
+        for (int q = 0; q < NT / 64; q++) r += sh[q];
+    }
+    __syncthreads();
+    return r;
+}
+
+// ------------------------------------------------------------------ velocity mean (the shift) ----
+
+__device__ __forceinline__ double cell_area(const GeomArgs &G, int64_t i, int64_t j, int64_t k) {
+    if (G.mode == 1) {
+        const double cphi = G.trig[2 * G.n[0] + G.n[1] + j];
+        return G.R * G.R * cphi * G.width[0][i] * G.width[1][j];
+    }
+    double a = G.width[0][i] * G.width[1][j];
+    if (G.mode == 2) a *= G.width[2][k];
+    return a;
+}
+
+// Velocity in the frame the filter works in: planetary Cartesian on the sphere (Diagnostics.jl:
+// 555-573, e_east = (−sinλ, cosλ, 0), e_north = (−sinφcosλ, −sinφsinλ, cosφ)), as given otherwise.
+template <typename T>
+__device__ __forceinline__ void frame_velocity(const GeomArgs &G, const void *u, const void *v, const void *w,
+                                               int64_t off, int64_t t, int64_t i, int64_t j, double U[3]) {
+    const double uu = (double)ld<T>(u, off + t), vv = (double)ld<T>(v, off + t);
+    if (G.mode == 1) {
+        const double sl = G.trig[i], cl = G.trig[G.n[0] + i];
+        const double sp = G.trig[2 * G.n[0] + j], cp = G.trig[2 * G.n[0] + G.n[1] + j];
+        U[0] = -sl * uu - sp * cl * vv;
+        U[1] = cl * uu - sp * sl * vv;
+        U[2] = cp * vv;
+    } else {
+        U[0] = uu;
+        U[1] = vv;
+        U[2] = (G.mode == 2) ? (double)ld<T>(w, off + t) : 0.0;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_mean_partial(GeomArgs G, const void *u, const void *v, const void *w,
+                                                      double *partials, int64_t nblk) {
+    __shared__ double sh[4];
+    const int64_t b = blockIdx.y;
+    double acc[3] = {0.0, 0.0, 0.0};
+    const int64_t Nx = G.n[0], Ny = G.n[1];
+    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < G.npts; t += (int64_t)gridDim.x * 256) {
+        if (!G.mask[t]) continue;
+        const int64_t i = t % Nx, j = (t / Nx) % Ny, k = t / (Nx * Ny);
+        double U[3];
+        frame_velocity<T>(G, u, v, w, b * G.npts, t, i, j, U);
+        const double A = cell_area(G, i, j, k);
+        for (int a = 0; a < 3; a++) acc[a] += A * U[a];
+    }
+    for (int a = 0; a < 3; a++) {
+        double s = block_sum<256>(acc[a], sh);
+        if (threadIdx.x == 0) partials[(b * 3 + a) * nblk + blockIdx.x] = s;
+    }
+}
+
+__global__ void k_mean_final(const double *partials, int64_t nblk, double inv_area, double *cmean) {
+    __shared__ double sh[4];
+    const int64_t b = blockIdx.x;
+    for (int a = 0; a < 3; a++) {
+        double acc = 0.0;
+        for (int64_t q = threadIdx.x; q < nblk; q += 256) acc += partials[(b * 3 + a) * nblk + q];
+        double s = block_sum<256>(acc, sh);
+        if (threadIdx.x == 0) cmean[b * 3 + a] = s * inv_area;
+    }
+}
+
+void launch_mean(const GeomArgs &G, int dtype, const void *u, const void *v, const void *w, int64_t nbatch,
+                 double *partials, int64_t npartials, double *cmean, hipStream_t s) {
+    int64_t nblk = (G.npts + 255) / 256;
+    if (nblk > 1024) nblk = 1024;
+    if (nblk * 3 * nbatch > npartials) nblk = npartials / (3 * nbatch);
+    dim3 grid((unsigned)nblk, (unsigned)nbatch);
+    if (dtype == CGEF_F32)
+        hipLaunchKernelGGL(k_mean_partial<float>, grid, dim3(256), 0, s, G, u, v, w, partials, nblk);
+    else
+        hipLaunchKernelGGL(k_mean_partial<double>, grid, dim3(256), 0, s, G, u, v, w, partials, nblk);
+    hipLaunchKernelGGL(k_mean_final, dim3((unsigned)nbatch), dim3(256), 0, s, partials, nblk, 1.0 / G.active_area, cmean);
+}
+
+// ------------------------------------------------------------- pack: shifted, masked components ----
+
+// qpack cell = (q0, q1, q2, flag): q = U − c at active cells, 0 at inactive; flag 0 active, 1 inactive.
+template <typename T>
+__global__ __launch_bounds__(256) void k_prep(GeomArgs G, const void *u, const void *v, const void *w,
+                                              const double *cmean, typename V4<T>::type *qpack) {
+    const int64_t b = blockIdx.y;
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= G.npts) return;
+    using T4 = typename V4<T>::type;
+    T4 o;
+    if (G.mask[t]) {
+        const int64_t Nx = G.n[0], Ny = G.n[1];
+        const int64_t i = t % Nx, j = (t / Nx) % Ny;
+        double U[3];
+        frame_velocity<T>(G, u, v, w, b * G.npts, t, i, j, U);
+        const double *c = cmean + 3 * b;
+        o.x = (T)(U[0] - c[0]);
+        o.y = (T)(U[1] - c[1]);
+        o.z = (T)(U[2] - c[2]);
+        o.w = (T)0;
+    } else {
+        o.x = o.y = o.z = (T)0;
+        o.w = (T)1;
+    }
+    qpack[b * G.npts + t] = o;
+}
+
+void launch_prep(const GeomArgs &G, int dtype, const void *u, const void *v, const void *w, int64_t nbatch,
+                 const double *cmean, void *qpack, hipStream_t s) {
+    dim3 grid((unsigned)((G.npts + 255) / 256), (unsigned)nbatch);
+    if (dtype == CGEF_F32)
+        hipLaunchKernelGGL(k_prep<float>, grid, dim3(256), 0, s, G, u, v, w, cmean, (float4 *)qpack);
+    else
+        hipLaunchKernelGGL(k_prep<double>, grid, dim3(256), 0, s, G, u, v, w, cmean, (double4 *)qpack);
+}
+
+// ----------------------------------------------------------------------- the fused Π filter ----
+
+// Stage source cells g0 .. g0+L-1 of one row (x wrapped on a periodic axis; flag 2 outside).
+template <typename T4, typename T>
+__device__ __forceinline__ void stage(T4 *seg, const T4 *__restrict__ q, int64_t rowbase, int64_t g0, int L,
+                                      int64_t Nx, int px) {
+    for (int s = threadIdx.x; s < L; s += TI) {
+        int64_t gx = g0 + s;
+        T4 val;
+        if (gx >= 0 && gx < Nx) {
+            val = q[rowbase + gx];
+        } else if (px) {
+            gx = wrapi(gx, Nx);
+            val = q[rowbase + gx];
+        } else {
+            val.x = val.y = val.z = (T)0;
+            val.w = (T)2;
+        }
+        seg[s] = val;
+    }
+}
+
+template <typename T, int NC, bool MASKED, bool EDGE_X, bool SPHERE>
+__global__ __launch_bounds__(TI) void k_pi_filter(GeomArgs G, BandArgs B, const typename V4<T>::type *__restrict__ qpack,
+                                                  const double *__restrict__ cmean, double *__restrict__ ubar,
+                                                  T *__restrict__ tau) {
+    using T4 = typename V4<T>::type;
+    extern __shared__ unsigned char smem_raw[];
+    T4 *seg = reinterpret_cast<T4 *>(smem_raw);
+    const int tid = threadIdx.x;
+    const int64_t Nx = G.n[0], Ny = G.n[1], Nz = G.n[2];
+    const int32_t r = B.row_order[blockIdx.y];
+    const int64_t j = r % Ny, k = r / Ny;
+    const int64_t b = blockIdx.z;
+    const int64_t i0 = (int64_t)blockIdx.x * TI;
+    const int64_t i = i0 + tid;
+    const bool inb = i < Nx;
+    const int64_t tpt = i + Nx * (int64_t)r;
+    const bool tact = inb && (!MASKED || G.mask[tpt] != 0);
+    const T4 *q = qpack + b * G.npts;
+    const int band = B.band_of_row[r];
+    const int rb = B.run_ptr[band], re = B.run_ptr[band + 1];
+    const T *wt = std::is_same<T, float>::value ? reinterpret_cast<const T *>(B.w32) : reinterpret_cast<const T *>(B.wd);
+
+    // ---- pass 1: Σ w q (FP64), Σ w over inactive (mo) and in-range (din) cells
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, mo = 0.0, din = 0.0;
+    for (int ru = rb; ru < re; ++ru) {
+        const Run R = B.runs[ru];
+        int64_t jj = j + R.dy, kk = k + R.dz;
+        if (jj < 0 || jj >= Ny) {
+            if (!G.periodic[1]) continue;
+            jj = wrapi(jj, Ny);
+        }
+        if (kk < 0 || kk >= Nz) {
+            if (!G.periodic[2]) continue;
+            kk = wrapi(kk, Nz);
+        }
+        const int L = TI + R.hi - R.lo;
+        __syncthreads();
+        stage<T4, T>(seg, q, Nx * (jj + Ny * kk), i0 + R.lo, L, Nx, G.periodic[0]);
+        __syncthreads();
+        if (tact) {
+            const double *wd = B.wd + R.woff;
+            const T4 *sp = seg + tid;
+            const int n = R.hi - R.lo + 1;
+#pragma unroll 4
+            for (int e = 0; e < n; ++e) {
+                const double w = wd[e];
+                const T4 c = sp[e];
+                s0 = fma(w, (double)c.x, s0);
+                s1 = fma(w, (double)c.y, s1);
+                if (NC == 3) s2 = fma(w, (double)c.z, s2);
+                if (EDGE_X) {
+                    din += (c.w < (T)1.5) ? w : 0.0;
+                    mo += (c.w > (T)0.5 && c.w < (T)1.5) ? w : 0.0;
+                } else if (MASKED) {
+                    mo = fma(w, (double)c.w, mo);
+                }
+            }
+        }
+    }
+    const double D = EDGE_X ? din : B.D_row[r];
+    const double den = (B.strategy == CGEF_DEFORMABLE) ? D - mo : D;
+    const bool ok = tact && den > 1e-15;
+    const double qb0 = ok ? s0 / den : 0.0, qb1 = ok ? s1 / den : 0.0, qb2 = ok ? s2 / den : 0.0;
+    const T qc0 = (T)qb0, qc1 = (T)qb1, qc2 = (T)qb2;
+
+    // ---- pass 2: centred products Σ w (q_a − q̄_a)(q_b − q̄_b) over active in-range cells
+    T c00 = 0, c01 = 0, c02 = 0, c11 = 0, c12 = 0, c22 = 0;
+    for (int ru = rb; ru < re; ++ru) {
+        const Run R = B.runs[ru];
+        int64_t jj = j + R.dy, kk = k + R.dz;
+        if (jj < 0 || jj >= Ny) {
+            if (!G.periodic[1]) continue;
+            jj = wrapi(jj, Ny);
+        }
+        if (kk < 0 || kk >= Nz) {
+            if (!G.periodic[2]) continue;
+            kk = wrapi(kk, Nz);
+        }
+        const int L = TI + R.hi - R.lo;
+        __syncthreads();
+        stage<T4, T>(seg, q, Nx * (jj + Ny * kk), i0 + R.lo, L, Nx, G.periodic[0]);
+        __syncthreads();
+        if (tact) {
+            const T *wr = wt + R.woff;
+            const T4 *sp = seg + tid;
+            const int n = R.hi - R.lo + 1;
+#pragma unroll 4
+            for (int e = 0; e < n; ++e) {
+                const T4 c = sp[e];
+                T tw = wr[e];
+                if (EDGE_X) tw = (c.w < (T)0.5) ? tw : (T)0;
+                else if (MASKED) tw = tw * ((T)1 - c.w);
+                const T d0 = c.x - qc0, d1 = c.y - qc1;
+                const T t0 = tw * d0, t1 = tw * d1;
+                c00 = fma(t0, d0, c00);
+                c01 = fma(t0, d1, c01);
+                c11 = fma(t1, d1, c11);
+                if (NC == 3) {
+                    const T d2 = c.z - qc2;
+                    const T t2 = tw * d2;
+                    c02 = fma(t0, d2, c02);
+                    c12 = fma(t1, d2, c12);
+                    c22 = fma(t2, d2, c22);
+                }
+            }
+        }
+    }
+    if (!inb) return;
+
+    // ---- epilogue: τ (frame of the filter), ū, then the local frame on the sphere
+    const double *cm = cmean + 3 * b;
+    double U[3] = {0.0, 0.0, 0.0}, tc[6] = {0, 0, 0, 0, 0, 0};
+    if (ok) {
+        const double om = (B.strategy == CGEF_ZEROFILL) ? mo / D : 0.0;  // 1 − m, m = F[1]
+        const double m = 1.0 - om;
+        const double qb[3] = {qb0, qb1, qb2};
+        const double cv[6] = {(double)c00, (double)c01, (double)c02, (double)c11, (double)c12, (double)c22};
+        const int ia[6] = {0, 0, 0, 1, 1, 2}, ib[6] = {0, 1, 2, 1, 2, 2};
+        for (int a = 0; a < 3; a++) U[a] = qb[a] + cm[a] * ((B.strategy == CGEF_ZEROFILL) ? m : 1.0);
+        for (int q6 = 0; q6 < 6; q6++) {
+            const int a = ia[q6], bb = ib[q6];
+            tc[q6] = cv[q6] / den + om * ((qb[a] + cm[a]) * (qb[bb] + cm[bb]) - om * cm[a] * cm[bb]);
+        }
+    }
+    const int64_t P = G.npts;
+    double *ub = ubar + b * 3 * P;
+    T *tb = tau + b * 6 * P;
+    if (SPHERE) {
+        const double sl = G.trig[i], cl = G.trig[Nx + i];
+        const double sp = G.trig[2 * Nx + j], cp = G.trig[2 * Nx + Ny + j];
+        const double ee[3] = {-sl, cl, 0.0}, en[3] = {-sp * cl, -sp * sl, cp};
+        const double er[3] = {cp * cl, cp * sl, sp};
+        // Diagnostics.jl:596-610 (ū back to local) and 619-650 (τ to local east/north).
+        ub[tpt] = ee[0] * U[0] + ee[1] * U[1] + ee[2] * U[2];
+        ub[P + tpt] = en[0] * U[0] + en[1] * U[1] + en[2] * U[2];
+        ub[2 * P + tpt] = er[0] * U[0] + er[1] * U[1] + er[2] * U[2];
+        const double T3[3][3] = {{tc[0], tc[1], tc[2]}, {tc[1], tc[3], tc[4]}, {tc[2], tc[4], tc[5]}};
+        double Te[3], Tn[3];
+        for (int rr = 0; rr < 3; rr++) {
+            Te[rr] = T3[rr][0] * ee[0] + T3[rr][1] * ee[1] + T3[rr][2] * ee[2];
+            Tn[rr] = T3[rr][0] * en[0] + T3[rr][1] * en[1] + T3[rr][2] * en[2];
+        }
+        tb[tpt] = (T)(ee[0] * Te[0] + ee[1] * Te[1] + ee[2] * Te[2]);
+        tb[P + tpt] = (T)(ee[0] * Tn[0] + ee[1] * Tn[1] + ee[2] * Tn[2]);
+        tb[2 * P + tpt] = (T)(en[0] * Tn[0] + en[1] * Tn[1] + en[2] * Tn[2]);
+    } else if (NC == 2) {
+        ub[tpt] = U[0];
+        ub[P + tpt] = U[1];
+        tb[tpt] = (T)tc[0];
+        tb[P + tpt] = (T)tc[1];
+        tb[2 * P + tpt] = (T)tc[3];
+    } else {
+        ub[tpt] = U[0];
+        ub[P + tpt] = U[1];
+        ub[2 * P + tpt] = U[2];
+        for (int q6 = 0; q6 < 6; q6++) tb[q6 * P + tpt] = (T)tc[q6];
+    }
+}
+
+template <typename T, int NC, bool SPHERE>
+static void pi_filter_dispatch(const GeomArgs &G, const BandArgs &B, bool masked, bool edge_x, const void *qpack,
+                               const double *cmean, double *ubar, void *tau, int64_t nbatch, hipStream_t s) {
+    using T4 = typename V4<T>::type;
+    const size_t shmem = (size_t)(TI + B.max_run - 1) * sizeof(T4);
+    dim3 grid((unsigned)((G.n[0] + TI - 1) / TI), (unsigned)(G.n[1] * G.n[2]), (unsigned)nbatch);
+    const T4 *qp = (const T4 *)qpack;
+    T *tp = (T *)tau;
+    if (edge_x)
+        hipLaunchKernelGGL((k_pi_filter<T, NC, true, true, SPHERE>), grid, dim3(TI), shmem, s, G, B, qp, cmean, ubar, tp);
+    else if (masked)
+        hipLaunchKernelGGL((k_pi_filter<T, NC, true, false, SPHERE>), grid, dim3(TI), shmem, s, G, B, qp, cmean, ubar, tp);
+    else
+        hipLaunchKernelGGL((k_pi_filter<T, NC, false, false, SPHERE>), grid, dim3(TI), shmem, s, G, B, qp, cmean, ubar, tp);
+}
+
+void launch_pi_filter(const GeomArgs &G, const BandArgs &B, int dtype, bool masked, bool edge_x, const void *qpack,
+                      const double *cmean, double *ubar, void *tau, int64_t nbatch, hipStream_t s) {
+    if (dtype == CGEF_F32) {
+        if (G.mode == 1) pi_filter_dispatch<float, 3, true>(G, B, masked, edge_x, qpack, cmean, ubar, tau, nbatch, s);
+        else if (G.mode == 0) pi_filter_dispatch<float, 2, false>(G, B, masked, edge_x, qpack, cmean, ubar, tau, nbatch, s);
+        else pi_filter_dispatch<float, 3, false>(G, B, masked, edge_x, qpack, cmean, ubar, tau, nbatch, s);
+    } else {
+        if (G.mode == 1) pi_filter_dispatch<double, 3, true>(G, B, masked, edge_x, qpack, cmean, ubar, tau, nbatch, s);
+        else if (G.mode == 0) pi_filter_dispatch<double, 2, false>(G, B, masked, edge_x, qpack, cmean, ubar, tau, nbatch, s);
+        else pi_filter_dispatch<double, 3, false>(G, B, masked, edge_x, qpack, cmean, ubar, tau, nbatch, s);
+    }
+}
+
+// ------------------------------------------------------------ derivatives, strain, Π, energy ----
+
+__device__ __forceinline__ double lag3(double x, double a, double b, double c, double fa, double fb, double fc) {
+    return (2 * x - b - c) / ((a - b) * (a - c)) * fa + (2 * x - a - c) / ((b - a) * (b - c)) * fb +
+           (2 * x - a - b) / ((c - a) * (c - b)) * fc;
+}
+
+// First derivative along `axis` of f at linear index t w.r.t. the axis coordinate: centred 3-node
+// stencil inside a run of active cells, the window shifted inward at a domain or mask edge, two nodes
+// for a run of two, 0 for an isolated cell (Derivatives.jl:26-31 → FlowGeometries derivative!,
+// nodes = 3, ReduceInRun). Periodic axes wrap index and coordinate.
+template <typename F>
+__device__ double deriv_axis(const GeomArgs &G, int axis, int64_t t, F f) {
+    const int64_t n = G.n[axis];
+    if (n < 2) return 0.0;
+    const int64_t stride = axis == 0 ? 1 : (axis == 1 ? G.n[0] : G.n[0] * G.n[1]);
+    const int64_t p = (t / stride) % n;
+    const int64_t base = t - p * stride;
+    const double *x = G.coord[axis];
+    bool ok[5];
+    int64_t ix[5];
+    double xc[5];
+#pragma unroll
+    for (int o = -2; o <= 2; o++) {
+        int64_t qq = p + o;
+        double sh = 0.0;
+        bool valid = true;
+        if (qq < 0 || qq >= n) {
+            if (!G.periodic[axis] || n < 3) valid = false;
+            else {
+                sh = qq < 0 ? -G.period[axis] : G.period[axis];
+                qq = wrapi(qq, n);
+            }
+        }
+        ix[o + 2] = valid ? base + qq * stride : t;
+        ok[o + 2] = valid && G.mask[ix[o + 2]] != 0;
+        xc[o + 2] = valid ? x[qq] + sh : 0.0;
+    }
+    if (ok[1] && ok[3]) return lag3(xc[2], xc[1], xc[2], xc[3], f(ix[1]), f(t), f(ix[3]));
+    if (ok[3] && ok[4]) return lag3(xc[2], xc[2], xc[3], xc[4], f(t), f(ix[3]), f(ix[4]));
+    if (ok[1] && ok[0]) return lag3(xc[2], xc[0], xc[1], xc[2], f(ix[0]), f(ix[1]), f(t));
+    if (ok[3]) return (f(ix[3]) - f(t)) / (xc[3] - xc[2]);
+    if (ok[1]) return (f(t) - f(ix[1])) / (xc[2] - xc[1]);
+    return 0.0;
+}
+
+// Strain S̄ (with the spherical curvature terms, Diagnostics.jl:652-669), contraction
+// (Diagnostics.jl:460-466, 683-698, 848-855), Π = −ρ₀ S̄:τ, and per-block partial energies
+// ½ |ū|² A (Diagnostics.jl:1125-1134; the radial component is excluded on the sphere).
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void k_strain_pi(GeomArgs G, const double *__restrict__ ubar, const T *__restrict__ tau,
+                                                   T *__restrict__ Pi, double rho0, double *partials, int64_t nblk) {
+    __shared__ double sh[4];
+    const int64_t b = blockIdx.y;
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t P = G.npts;
+    double e = 0.0;
+    if (t < P) {
+        const double *U = ubar + b * 3 * P, *V = U + P, *W = U + 2 * P;
+        const T *tb = tau + b * 6 * P;
+        double out = 0.0;
+        if (G.mask[t]) {
+            const int64_t Nx = G.n[0], Ny = G.n[1];
+            const int64_t i = t % Nx, j = (t / Nx) % Ny, k = t / (Nx * Ny);
+            auto fu = [&](int64_t q) { return U[q]; };
+            auto fv = [&](int64_t q) { return V[q]; };
+            if (MODE == 2) {
+                auto fw = [&](int64_t q) { return W[q]; };
+                const double Sxx = deriv_axis(G, 0, t, fu), Syy = deriv_axis(G, 1, t, fv), Szz = deriv_axis(G, 2, t, fw);
+                const double Sxy = 0.5 * (deriv_axis(G, 1, t, fu) + deriv_axis(G, 0, t, fv));
+                const double Sxz = 0.5 * (deriv_axis(G, 2, t, fu) + deriv_axis(G, 0, t, fw));
+                const double Syz = 0.5 * (deriv_axis(G, 2, t, fv) + deriv_axis(G, 1, t, fw));
+                const double c = Sxx * (double)tb[t] + 2.0 * Sxy * (double)tb[P + t] + Syy * (double)tb[3 * P + t] +
+                                 2.0 * Sxz * (double)tb[2 * P + t] + 2.0 * Syz * (double)tb[4 * P + t] +
+                                 Szz * (double)tb[5 * P + t];
+                out = -c;
+                e = 0.5 * (U[t] * U[t] + V[t] * V[t] + W[t] * W[t]) * cell_area(G, i, j, k);
+            } else {
+                double dudx = deriv_axis(G, 0, t, fu), dudy = deriv_axis(G, 1, t, fu);
+                double dvdx = deriv_axis(G, 0, t, fv), dvdy = deriv_axis(G, 1, t, fv);
+                double Sxx, Sxy, Syy;
+                if (MODE == 1) {
+                    const double sp = G.trig[2 * Nx + j], cp = G.trig[2 * Nx + Ny + j];
+                    const double hx = (fabs(cp) < sqrt(DBL_EPSILON)) ? 0.0 : 1.0 / (G.R * cp);
+                    dudx *= hx;
+                    dvdx *= hx;
+                    dudy /= G.R;
+                    dvdy /= G.R;
+                    const double tf = fabs(cp) > 1e-12 ? sp / (G.R * cp) : 0.0;
+                    Sxx = dudx - V[t] * tf;
+                    Syy = dvdy;
+                    Sxy = 0.5 * (dudy + dvdx + U[t] * tf);
+                } else {
+                    Sxx = dudx;
+                    Syy = dvdy;
+                    Sxy = 0.5 * (dudy + dvdx);
+                }
+                const double c = Sxx * (double)tb[t] + 2.0 * Sxy * (double)tb[P + t] + Syy * (double)tb[2 * P + t];
+                out = -c;
+                e = 0.5 * (U[t] * U[t] + V[t] * V[t]) * cell_area(G, i, j, k);
+            }
+        }
+        Pi[b * P + t] = (T)(rho0 * out);
+    }
+    if (partials) {
+        const double s = block_sum<256>(e, sh);
+        if (threadIdx.x == 0) partials[b * nblk + blockIdx.x] = s;
+    }
+}
+
+int64_t strain_blocks(const GeomArgs &G) { return (G.npts + 255) / 256; }
+
+void launch_strain_pi(const GeomArgs &G, int dtype, const double *ubar, const void *tau, void *Pi, double rho0,
+                      int64_t nbatch, double *partials, int64_t npartials, hipStream_t s) {
+    const int64_t nblk = strain_blocks(G);
+    double *pp = (partials && nblk * nbatch <= npartials) ? partials : nullptr;
+    dim3 grid((unsigned)nblk, (unsigned)nbatch);
+#define CGEF_SP(TT, MM) hipLaunchKernelGGL((k_strain_pi<TT, MM>), grid, dim3(256), 0, s, G, ubar, (const TT *)tau, (TT *)Pi, rho0, pp, nblk)
+    if (dtype == CGEF_F32) {
+        if (G.mode == 0) CGEF_SP(float, 0);
+        else if (G.mode == 1) CGEF_SP(float, 1);
+        else CGEF_SP(float, 2);
+    } else {
+        if (G.mode == 0) CGEF_SP(double, 0);
+        else if (G.mode == 1) CGEF_SP(double, 1);
+        else CGEF_SP(double, 2);
+    }
+#undef CGEF_SP
+}
+
+__global__ void k_energy_final(const double *partials, int64_t nblk, double inv_area, double *E) {
+    __shared__ double sh[4];
+    const int64_t b = blockIdx.x;
+    double acc = 0.0;
+    for (int64_t q = threadIdx.x; q < nblk; q += 256) acc += partials[b * nblk + q];
+    const double s = block_sum<256>(acc, sh);
+    if (threadIdx.x == 0) E[b] = s * inv_area;
+}
+
+void launch_energy_final(int64_t nbatch, int64_t nblocks, const double *partials, double active_area, double *E_dev,
+                         hipStream_t s) {
+    hipLaunchKernelGGL(k_energy_final, dim3((unsigned)nbatch), dim3(256), 0, s, partials, nblocks, 1.0 / active_area, E_dev);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_energy_fields(GeomArgs G, const T *ub, const T *vb, const T *wb, double *partials,
+                                                       int64_t nblk) {
+    __shared__ double sh[4];
+    const int64_t b = blockIdx.y;
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t P = G.npts;
+    double e = 0.0;
+    if (t < P && G.mask[t]) {
+        const int64_t Nx = G.n[0], Ny = G.n[1];
+        const int64_t i = t % Nx, j = (t / Nx) % Ny, k = t / (Nx * Ny);
+        const double u = (double)ub[b * P + t], v = (double)vb[b * P + t];
+        double v2 = u * u + v * v;
+        if (wb) {
+            const double w = (double)wb[b * P + t];
+            v2 += w * w;
+        }
+        e = 0.5 * v2 * cell_area(G, i, j, k);
+    }
+    const double s = block_sum<256>(e, sh);
+    if (threadIdx.x == 0) partials[b * nblk + blockIdx.x] = s;
+}
+
+void launch_energy_fields(const GeomArgs &G, int dtype, const void *ub, const void *vb, const void *wb, int64_t nbatch,
+                          double *partials, int64_t npartials, hipStream_t s) {
+    const int64_t nblk = strain_blocks(G);
+    (void)npartials;
+    dim3 grid((unsigned)nblk, (unsigned)nbatch);
+    if (dtype == CGEF_F32)
+        hipLaunchKernelGGL(k_energy_fields<float>, grid, dim3(256), 0, s, G, (const float *)ub, (const float *)vb,
+                           (const float *)wb, partials, nblk);
+    else
+        hipLaunchKernelGGL(k_energy_fields<double>, grid, dim3(256), 0, s, G, (const double *)ub, (const double *)vb,
+                           (const double *)wb, partials, nblk);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_derivative(GeomArgs G, int axis, const T *in, T *out) {
+    const int64_t b = blockIdx.y;
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t P = G.npts;
+    if (t >= P) return;
+    double v = 0.0;
+    if (G.mask[t]) {
+        const T *f = in + b * P;
+        v = deriv_axis(G, axis, t, [&](int64_t q) { return (double)f[q]; });
+        if (G.mode == 1) {
+            const int64_t j = (t / G.n[0]) % G.n[1];
+            const double cp = G.trig[2 * G.n[0] + G.n[1] + j];
+            if (axis == 0) v = (fabs(cp) < sqrt(DBL_EPSILON)) ? 0.0 : v / (G.R * cp);
+            else v = v / G.R;
+        }
+    }
+    out[b * P + t] = (T)v;
+}
+
+void launch_derivative(const GeomArgs &G, int axis, int dtype, const void *in, void *out, int64_t nbatch, hipStream_t s) {
+    dim3 grid((unsigned)((G.npts + 255) / 256), (unsigned)nbatch);
+    if (dtype == CGEF_F32)
+        hipLaunchKernelGGL(k_derivative<float>, grid, dim3(256), 0, s, G, axis, (const float *)in, (float *)out);
+    else
+        hipLaunchKernelGGL(k_derivative<double>, grid, dim3(256), 0, s, G, axis, (const double *)in, (double *)out);
+}
+
+// ----------------------------------------------------------- linear filter of K given fields ----
+
+// Same run walk as k_pi_filter for plain fields (filter_apply!): the staged cells are f·active and
+// a flag (0 active, 1 inactive, 2 outside a non-periodic x axis). FP64 accumulation.
+template <typename T, int K, bool MASKED, bool EDGE_X>
+__global__ __launch_bounds__(TI) void k_filter_lin(GeomArgs G, BandArgs B, const T *__restrict__ in, T *__restrict__ out,
+                                                   int64_t nslices) {
+    extern __shared__ unsigned char smem_raw[];
+    const int L_max = TI + B.max_run - 1;
+    T *seg = reinterpret_cast<T *>(smem_raw);
+    unsigned char *flag = smem_raw + (size_t)K * L_max * sizeof(T);
+    const int tid = threadIdx.x;
+    const int64_t Nx = G.n[0], Ny = G.n[1], Nz = G.n[2];
+    const int32_t r = B.row_order[blockIdx.y];
+    const int64_t j = r % Ny, k = r / Ny;
+    const int64_t s0 = (int64_t)blockIdx.z * K;
+    const int64_t i0 = (int64_t)blockIdx.x * TI;
+    const int64_t i = i0 + tid;
+    const bool inb = i < Nx;
+    const int64_t tpt = i + Nx * (int64_t)r;
+    const bool tact = inb && (!MASKED || G.mask[tpt] != 0);
+    const int band = B.band_of_row[r];
+    const int rb = B.run_ptr[band], re = B.run_ptr[band + 1];
+    double acc[K];
+#pragma unroll
+    for (int m = 0; m < K; m++) acc[m] = 0.0;
+    double mo = 0.0, din = 0.0;
+    for (int ru = rb; ru < re; ++ru) {
+        const Run R = B.runs[ru];
+        int64_t jj = j + R.dy, kk = k + R.dz;
+        if (jj < 0 || jj >= Ny) {
+            if (!G.periodic[1]) continue;
+            jj = wrapi(jj, Ny);
+        }
+        if (kk < 0 || kk >= Nz) {
+            if (!G.periodic[2]) continue;
+            kk = wrapi(kk, Nz);
+        }
+        const int L = TI + R.hi - R.lo;
+        const int64_t rowbase = Nx * (jj + Ny * kk);
+        __syncthreads();
+        for (int s = tid; s < L; s += TI) {
+            int64_t gx = i0 + R.lo + s;
+            unsigned char fl = 0;
+            if (gx < 0 || gx >= Nx) {
+                if (G.periodic[0]) gx = wrapi(gx, Nx);
+                else fl = 2;
+            }
+            if (fl == 0 && MASKED && !G.mask[rowbase + gx]) fl = 1;
+            flag[s] = fl;
+#pragma unroll
+            for (int m = 0; m < K; m++) {
+                const int64_t sl = s0 + m;
+                seg[m * L_max + s] = (fl == 0 && sl < nslices) ? in[sl * G.npts + rowbase + gx] : (T)0;
+            }
+        }
+        __syncthreads();
+        if (tact) {
+            const double *wd = B.wd + R.woff;
+            const int n = R.hi - R.lo + 1;
+            for (int e = 0; e < n; ++e) {
+                const double w = wd[e];
+#pragma unroll
+                for (int m = 0; m < K; m++) acc[m] = fma(w, (double)seg[m * L_max + tid + e], acc[m]);
+                if (EDGE_X || MASKED) {
+                    const unsigned char fl = flag[tid + e];
+                    if (fl == 1) mo += w;
+                    if (EDGE_X && fl != 2) din += w;
+                }
+            }
+        }
+    }
+    if (!inb) return;
+    const double D = EDGE_X ? din : B.D_row[r];
+    const double den = (B.strategy == CGEF_DEFORMABLE) ? D - mo : D;
+    const bool ok = tact && den > 1e-15;
+#pragma unroll
+    for (int m = 0; m < K; m++) {
+        const int64_t sl = s0 + m;
+        if (sl < nslices) out[sl * G.npts + tpt] = ok ? (T)(acc[m] / den) : (T)0;
+    }
+}
+
+template <typename T, int K>
+static void filter_lin_k(const GeomArgs &G, const BandArgs &B, bool masked, bool edge_x, const void *in, void *out,
+                         int64_t nslices, hipStream_t s) {
+    const size_t L_max = TI + B.max_run - 1;
+    const size_t shmem = K * L_max * sizeof(T) + L_max;
+    dim3 grid((unsigned)((G.n[0] + TI - 1) / TI), (unsigned)(G.n[1] * G.n[2]), (unsigned)((nslices + K - 1) / K));
+    if (edge_x)
+        hipLaunchKernelGGL((k_filter_lin<T, K, true, true>), grid, dim3(TI), shmem, s, G, B, (const T *)in, (T *)out, nslices);
+    else if (masked)
+        hipLaunchKernelGGL((k_filter_lin<T, K, true, false>), grid, dim3(TI), shmem, s, G, B, (const T *)in, (T *)out, nslices);
+    else
+        hipLaunchKernelGGL((k_filter_lin<T, K, false, false>), grid, dim3(TI), shmem, s, G, B, (const T *)in, (T *)out, nslices);
+}
+
+void launch_filter_lin(const GeomArgs &G, const BandArgs &B, int dtype, bool masked, bool edge_x, const void *in,
+                       void *out, int64_t nslices, hipStream_t s) {
+    // up to 4 slices share one neighbour walk (filter_apply_batch!); larger batches take more z-blocks
+    const int K = nslices >= 4 ? 4 : (int)nslices;
+    if (dtype == CGEF_F32) {
+        if (K == 4) filter_lin_k<float, 4>(G, B, masked, edge_x, in, out, nslices, s);
+        else if (K == 3) filter_lin_k<float, 3>(G, B, masked, edge_x, in, out, nslices, s);
+        else if (K == 2) filter_lin_k<float, 2>(G, B, masked, edge_x, in, out, nslices, s);
+        else filter_lin_k<float, 1>(G, B, masked, edge_x, in, out, nslices, s);
+    } else {
+        if (K == 4) filter_lin_k<double, 4>(G, B, masked, edge_x, in, out, nslices, s);
+        else if (K == 3) filter_lin_k<double, 3>(G, B, masked, edge_x, in, out, nslices, s);
+        else if (K == 2) filter_lin_k<double, 2>(G, B, masked, edge_x, in, out, nslices, s);
+        else filter_lin_k<double, 1>(G, B, masked, edge_x, in, out, nslices, s);
+    }
+}
+
+// ----------------------------------------------- plain FP64 direct sum (the KA kernel restated) ----
+
+// KernelAbstractionsExt.jl:37-90 `_cgef_filter_kernel!`: one thread per output point, the CSR
+// footprint (di, dj[, dk], w, ptr) read from global memory, ZeroFill/Deformable, ws/wn > 1e-15.
+__global__ __launch_bounds__(256) void k_baseline(GeomArgs G, const int32_t *__restrict__ di, const int32_t *__restrict__ dj,
+                                                  const int32_t *__restrict__ dk, const double *__restrict__ w,
+                                                  const int64_t *__restrict__ ptr, const int32_t *__restrict__ band_of_row,
+                                                  int strategy, const double *__restrict__ field, double *__restrict__ out) {
+    const int64_t Nx = G.n[0], Ny = G.n[1], Nz = G.n[2];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t r = (int64_t)blockIdx.y * blockDim.y + threadIdx.y;
+    const int64_t b = blockIdx.z;
+    if (i >= Nx || r >= Ny * Nz) return;
+    const int64_t j = r % Ny, k = r / Ny;
+    const int64_t t = i + Nx * r;
+    const double *f = field + b * G.npts;
+    if (!G.mask[t]) {
+        out[b * G.npts + t] = 0.0;
+        return;
+    }
+    const int band = band_of_row[r];
+    double ws = 0.0, wn = 0.0;
+    for (int64_t e = ptr[band]; e < ptr[band + 1]; e++) {
+        int64_t jj = j + dj[e];
+        if (jj < 0 || jj >= Ny) {
+            if (!G.periodic[1]) continue;
+            jj = wrapi(jj, Ny);
+        }
+        int64_t kk = k + dk[e];
+        if (kk < 0 || kk >= Nz) {
+            if (!G.periodic[2]) continue;
+            kk = wrapi(kk, Nz);
+        }
+        int64_t ii = i + di[e];
+        if (ii < 0 || ii >= Nx) {
+            if (!G.periodic[0]) continue;
+            ii = wrapi(ii, Nx);
+        }
+        const int64_t sidx = ii + Nx * (jj + Ny * kk);
+        const bool act = G.mask[sidx] != 0;
+        const double wk = w[e];
+        if (strategy == CGEF_ZEROFILL) {
+            wn += wk;
+            if (act) ws += wk * f[sidx];
+        } else if (act) {
+            wn += wk;
+            ws += wk * f[sidx];
+        }
+    }
+    out[b * G.npts + t] = wn > 1e-15 ? ws / wn : 0.0;
+}
+
+void launch_baseline_filter(const GeomArgs &G, const Plan &p, const double *in, double *out, int64_t nslices,
+                            hipStream_t s) {
+    dim3 block(64, 4);
+    dim3 grid((unsigned)((G.n[0] + 63) / 64), (unsigned)((G.n[1] * G.n[2] + 3) / 4), (unsigned)nslices);
+    hipLaunchKernelGGL(k_baseline, grid, block, 0, s, G, p.csr_di_d, p.csr_dj_d, p.csr_dk_d, p.w64_d, p.csr_ptr_d,
+                       p.band_of_row_d, p.strategy, in, out);
+}
+
+// ------------------------------------------------------------- separable Gaussian (Cartesian) ----
+
+// Filtering.jl:1210-1279 row pass (masked input) then column pass; 1510-1528 normalization.
+template <typename T>
+__global__ __launch_bounds__(256) void k_sep_row(GeomArgs G, const double *__restrict__ gx, int lx, const T *__restrict__ in,
+                                                 double *__restrict__ rp) {
+    const int64_t b = blockIdx.y;
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= G.npts) return;
+    const int64_t Nx = G.n[0];
+    const int64_t i = t % Nx, rowbase = t - i;
+    const T *f = in + b * G.npts;
+    double acc = 0.0;
+    for (int d = -lx; d <= lx; d++) {
+        int64_t ii = i + d;
+        if (ii < 0 || ii >= Nx) {
+            if (!G.periodic[0]) continue;
+            ii = wrapi(ii, Nx);
+        }
+        const int64_t q = rowbase + ii;
+        acc += gx[d + lx] * (G.mask[q] ? (double)f[q] : 0.0);
+    }
+    rp[b * G.npts + t] = acc;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_sep_col(GeomArgs G, const double *__restrict__ gy, int ly, const double *__restrict__ rp,
+                                                 const double *__restrict__ px, const double *__restrict__ py,
+                                                 const double *__restrict__ invden, T *__restrict__ out) {
+    const int64_t b = blockIdx.y;
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= G.npts) return;
+    const int64_t Nx = G.n[0], Ny = G.n[1];
+    const int64_t i = t % Nx, j = t / Nx;
+    const double *r = rp + b * G.npts;
+    double acc = 0.0;
+    for (int d = -ly; d <= ly; d++) {
+        int64_t jj = j + d;
+        if (jj < 0 || jj >= Ny) {
+            if (!G.periodic[1]) continue;
+            jj = wrapi(jj, Ny);
+        }
+        acc += gy[d + ly] * r[i + Nx * jj];
+    }
+    double v;
+    if (invden) {
+        v = acc * invden[t];
+    } else {
+        const double den = px[i] * py[j];
+        v = den > 1e-15 ? acc / den : 0.0;
+    }
+    if (!G.mask[t]) v = 0.0;
+    out[b * G.npts + t] = (T)v;
+}
+
+void launch_separable(const GeomArgs &G, const Plan &p, int dtype, const void *in, void *out, int64_t nslices,
+                      double *rowpass, hipStream_t s) {
+    dim3 grid((unsigned)((G.npts + 255) / 256), (unsigned)nslices);
+    if (dtype == CGEF_F32) {
+        hipLaunchKernelGGL(k_sep_row<float>, grid, dim3(256), 0, s, G, p.sep_g_d[0], p.sep_lim[0], (const float *)in, rowpass);
+        hipLaunchKernelGGL(k_sep_col<float>, grid, dim3(256), 0, s, G, p.sep_g_d[1], p.sep_lim[1], rowpass, p.sep_prof_d[0],
+                           p.sep_prof_d[1], p.sep_invden_d, (float *)out);
+    } else {
+        hipLaunchKernelGGL(k_sep_row<double>, grid, dim3(256), 0, s, G, p.sep_g_d[0], p.sep_lim[0], (const double *)in, rowpass);
+        hipLaunchKernelGGL(k_sep_col<double>, grid, dim3(256), 0, s, G, p.sep_g_d[1], p.sep_lim[1], rowpass, p.sep_prof_d[0],
+                           p.sep_prof_d[1], p.sep_invden_d, (double *)out);
+    }
+}
+
+// --------------------------------------------------- unfused Π (baseline and separable engines) ----
+
+// Diagnostics.jl:489-527 / 550-650 / 805-829 as separate elementwise steps in FP64: the frame
+// velocity (rotated on the sphere, masked cells 0 there), its products, then τ and ū in the local
+// frame from the filtered planes.
+template <typename T>
+__global__ __launch_bounds__(256) void k_unfused_prep(GeomArgs G, const void *u, const void *v, const void *w, double *vel) {
+    const int64_t b = blockIdx.y;
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= G.npts) return;
+    const int64_t P = G.npts, Nx = G.n[0], Ny = G.n[1];
+    const int64_t i = t % Nx, j = (t / Nx) % Ny;
+    double U[3];
+    frame_velocity<T>(G, u, v, w, b * P, t, i, j, U);
+    if (G.mode == 1 && !G.mask[t]) U[0] = U[1] = U[2] = 0.0;
+    double *o = vel + b * 3 * P;
+    o[t] = U[0];
+    o[P + t] = U[1];
+    o[2 * P + t] = U[2];
+}
+
+void launch_unfused_prep(const GeomArgs &G, int dtype, const void *u, const void *v, const void *w, int64_t nbatch,
+                         double *vel, hipStream_t s) {
+    dim3 grid((unsigned)((G.npts + 255) / 256), (unsigned)nbatch);
+    if (dtype == CGEF_F32) hipLaunchKernelGGL(k_unfused_prep<float>, grid, dim3(256), 0, s, G, u, v, w, vel);
+    else hipLaunchKernelGGL(k_unfused_prep<double>, grid, dim3(256), 0, s, G, u, v, w, vel);
+}
+
+__global__ __launch_bounds__(256) void k_products(int64_t P, const double *vel, double *prod) {
+    const int64_t b = blockIdx.y;
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= P) return;
+    const double *U = vel + b * 3 * P;
+    const double a = U[t], c = U[P + t], d = U[2 * P + t];
+    double *o = prod + b * 6 * P;
+    o[t] = a * a;
+    o[P + t] = a * c;
+    o[2 * P + t] = a * d;
+    o[3 * P + t] = c * c;
+    o[4 * P + t] = c * d;
+    o[5 * P + t] = d * d;
+}
+
+void launch_products(const GeomArgs &G, const double *vel, double *prod, int64_t nbatch, hipStream_t s) {
+    dim3 grid((unsigned)((G.npts + 255) / 256), (unsigned)nbatch);
+    hipLaunchKernelGGL(k_products, grid, dim3(256), 0, s, G.npts, vel, prod);
+}
+
+__global__ __launch_bounds__(256) void k_unfused_tau(GeomArgs G, const double *velf, const double *prodf, double *ubar,
+                                                     double *tau) {
+    const int64_t b = blockIdx.y;
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t P = G.npts;
+    if (t >= P) return;
+    const double *Uf = velf + b * 3 * P;
+    const double *F = prodf + b * 6 * P;
+    double *ub = ubar + b * 3 * P;
+    double *tb = tau + b * 6 * P;
+    const double u0 = Uf[t], u1 = Uf[P + t], u2 = Uf[2 * P + t];
+    const double tc[6] = {F[t] - u0 * u0, F[P + t] - u0 * u1, F[2 * P + t] - u0 * u2,
+                          F[3 * P + t] - u1 * u1, F[4 * P + t] - u1 * u2, F[5 * P + t] - u2 * u2};
+    if (G.mode == 1) {
+        if (!G.mask[t]) {
+            ub[t] = ub[P + t] = ub[2 * P + t] = 0.0;
+            tb[t] = tb[P + t] = tb[2 * P + t] = 0.0;
+            return;
+        }
+        const int64_t Nx = G.n[0], Ny = G.n[1];
+        const int64_t i = t % Nx, j = (t / Nx) % Ny;
+        const double sl = G.trig[i], cl = G.trig[Nx + i];
+        const double sp = G.trig[2 * Nx + j], cp = G.trig[2 * Nx + Ny + j];
+        const double ee[3] = {-sl, cl, 0.0}, en[3] = {-sp * cl, -sp * sl, cp}, er[3] = {cp * cl, cp * sl, sp};
+        ub[t] = ee[0] * u0 + ee[1] * u1;
+        ub[P + t] = en[0] * u0 + en[1] * u1 + en[2] * u2;
+        ub[2 * P + t] = er[0] * u0 + er[1] * u1 + er[2] * u2;
+        const double T3[3][3] = {{tc[0], tc[1], tc[2]}, {tc[1], tc[3], tc[4]}, {tc[2], tc[4], tc[5]}};
+        double Te[3], Tn[3];
+        for (int rr = 0; rr < 3; rr++) {
+            Te[rr] = T3[rr][0] * ee[0] + T3[rr][1] * ee[1] + T3[rr][2] * ee[2];
+            Tn[rr] = T3[rr][0] * en[0] + T3[rr][1] * en[1] + T3[rr][2] * en[2];
+        }
+        tb[t] = ee[0] * Te[0] + ee[1] * Te[1] + ee[2] * Te[2];
+        tb[P + t] = ee[0] * Tn[0] + ee[1] * Tn[1] + ee[2] * Tn[2];
+        tb[2 * P + t] = en[0] * Tn[0] + en[1] * Tn[1] + en[2] * Tn[2];
+    } else if (G.mode == 0) {
+        ub[t] = u0;
+        ub[P + t] = u1;
+        ub[2 * P + t] = 0.0;
+        tb[t] = tc[0];
+        tb[P + t] = tc[1];
+        tb[2 * P + t] = tc[3];
+    } else {
+        ub[t] = u0;
+        ub[P + t] = u1;
+        ub[2 * P + t] = u2;
+        for (int q = 0; q < 6; q++) tb[q * P + t] = tc[q];
+    }
+}
+
+void launch_unfused_tau(const GeomArgs &G, const double *velf, const double *prodf, double *ubar, double *tau,
+                        int64_t nbatch, hipStream_t s) {
+    dim3 grid((unsigned)((G.npts + 255) / 256), (unsigned)nbatch);
+    hipLaunchKernelGGL(k_unfused_tau, grid, dim3(256), 0, s, G, velf, prodf, ubar, tau);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_cast(const double *in, T *out, int64_t n) {
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t < n) out[t] = (T)in[t];
+}
+
+void launch_cast(int dtype_out, const double *in, void *out, int64_t n, hipStream_t s) {
+    dim3 grid((unsigned)((n + 255) / 256));
+    if (dtype_out == CGEF_F32) hipLaunchKernelGGL(k_cast<float>, grid, dim3(256), 0, s, in, (float *)out, n);
+    else hipLaunchKernelGGL(k_cast<double>, grid, dim3(256), 0, s, in, (double *)out, n);
+}
+
+}  // namespace cgef

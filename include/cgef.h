@@ -1,0 +1,148 @@
+/*
+ * cgef.h — C ABI of libcgef_hip.so, the MI355X (gfx950) backend for CoarseGrainingEnergyFluxes.jl's
+ * real-space footprint filter and Π pipeline.
+ *
+ * Plain pointers and sizes only. Field pointers are DEVICE pointers (HBM-resident), laid out as the
+ * reference lays out its arrays: column-major (Nx, Ny[, Nz], trailing batch) with i fastest.
+ * Descriptor arrays (axis coordinates, mask) are HOST pointers copied at creation.
+ *
+ * Every entry point returns a cgef_status; CGEF_ERR_ARGUMENT corresponds to the reference's
+ * ArgumentError and CGEF_ERR_DIMENSION to its DimensionMismatch. cgef_last_error() returns the
+ * message of the calling thread's last failure. All compute entry points are stream-ordered on the
+ * given hipStream_t (NULL = the null stream) and return after enqueueing unless stated otherwise;
+ * entry points that return host values (energies) synchronize that stream.
+ */
+#ifndef CGEF_H
+#define CGEF_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    CGEF_OK = 0,
+    CGEF_ERR_ARGUMENT = 1,    /* reference: ArgumentError */
+    CGEF_ERR_DIMENSION = 2,   /* reference: DimensionMismatch */
+    CGEF_ERR_DEVICE = 3,      /* a HIP runtime call failed */
+    CGEF_ERR_ALLOC = 4,       /* device or host allocation failed */
+    CGEF_ERR_UNSUPPORTED = 5  /* valid request this backend has no kernel for */
+} cgef_status;
+
+typedef enum { CGEF_F32 = 0, CGEF_F64 = 1 } cgef_dtype;
+typedef enum { CGEF_CARTESIAN = 0, CGEF_SPHERICAL = 1 } cgef_geometry;
+typedef enum { CGEF_TOPHAT = 0, CGEF_GAUSSIAN = 1 } cgef_kernel_kind;
+typedef enum { CGEF_ZEROFILL = 0, CGEF_DEFORMABLE = 1 } cgef_mask_strategy;
+typedef enum {
+    CGEF_ENGINE_AUTO = 0,      /* the reference's default per grid/kernel */
+    CGEF_ENGINE_BANDED = 1,    /* disk/cap/ball footprint direct sum (fused, LDS-tiled) */
+    CGEF_ENGINE_SEPARABLE = 2, /* square-truncated two-pass Gaussian (Cartesian 2-D) */
+    CGEF_ENGINE_BASELINE = 3   /* plain FP64 direct sum, one thread per point, CSR in HBM */
+} cgef_engine;
+
+/* Grid description. Reference: FlowGeometries.Grids.StructuredGrid (un-vendored), as consumed by
+ * Filtering.jl:636-732 (axes, spacing, periodicity, radius) and Diagnostics.jl:1051-1059 (area). */
+typedef struct {
+    int32_t ndim;             /* 2 or 3 */
+    int32_t geometry;         /* cgef_geometry; spherical requires ndim == 2 */
+    int64_t n[3];             /* Nx, Ny, Nz (Nz ignored for ndim 2) */
+    const double *coord[3];   /* host: axis coordinates; spherical lon/lat in radians */
+    int32_t uniform[3];       /* 1 = the axis is a Range (constant step) */
+    int32_t periodic[3];
+    double period[3];         /* wrap length of a periodic axis */
+    double radius;            /* sphere radius (spherical only) */
+    const uint8_t *mask;      /* host: one byte per cell, nonzero = active; NULL = all active */
+} cgef_grid_desc;
+
+/* Kernels.jl:28, 59-62 — TopHatKernel / GaussianKernel(α). */
+typedef struct {
+    int32_t kind;             /* cgef_kernel_kind */
+    double alpha;             /* Gaussian α (ignored for the top-hat) */
+} cgef_kernel_desc;
+
+typedef struct cgef_grid cgef_grid;
+typedef struct cgef_plan cgef_plan;
+typedef struct cgef_workspace cgef_workspace;
+
+typedef struct {
+    int32_t engine;           /* resolved cgef_engine */
+    int32_t nbands;
+    int64_t nentries;         /* footprint table entries (in-support taps summed over bands) */
+    int64_t nruns;
+    double mean_taps;         /* in-support taps per grid point, averaged over rows */
+    int64_t max_taps;
+    double radius;            /* kernel_radius(kernel, scale) */
+    double active_taps;       /* in-support taps summed over ACTIVE target points */
+} cgef_plan_info;
+
+const char *cgef_version(void);
+const char *cgef_last_error(void);
+
+/* Device-resident grid (mask, axes, cell measures, rotation tables). Replaces the grid argument
+ * the reference threads through plan_filter / compute_Π! (Filtering.jl:2931; Diagnostics.jl:389). */
+cgef_status cgef_grid_create(const cgef_grid_desc *desc, int32_t device, cgef_grid **out);
+cgef_status cgef_grid_destroy(cgef_grid *grid);
+/* Diagnostics.jl:1051-1059 active_area; ArgumentError when every cell is masked. */
+cgef_status cgef_active_area(const cgef_grid *grid, double *out);
+
+/* Filtering.jl:2931-2954 plan_filter (+ the GPU residency hook prepare_workspace, 2910-2912 and
+ * KernelAbstractionsExt:474-571): builds the footprint once on the host and makes it resident.
+ * ArgumentError for a non-finite or non-positive scale (Filtering.jl:2917-2922). */
+cgef_status cgef_plan_create(cgef_grid *grid, const cgef_kernel_desc *kernel, double scale,
+                             int32_t mask_strategy, int32_t engine, cgef_plan **out);
+cgef_status cgef_plan_destroy(cgef_plan *plan);
+cgef_status cgef_plan_query(const cgef_plan *plan, cgef_plan_info *info);
+
+/* ΠWorkspace (Diagnostics.jl:145-194): caller-owned scratch sized for nbatch slices, for the
+ * given element type. Passing NULL to the entry points below uses a scratch the grid owns. */
+cgef_status cgef_workspace_create(const cgef_grid *grid, int64_t nbatch, int32_t dtype, cgef_workspace **out);
+cgef_status cgef_workspace_destroy(cgef_workspace *ws);
+
+/* Filtering.jl:3242-3256 filter_apply!, 3360-3385 filter_apply_batch!, 3286-3305
+ * filter_apply_batched! and the GPU hooks gpu_filter_field!/gpu_filter_field_batched!
+ * (Filtering.jl:195-197, 3342-3344). Filters nfields × nbatch slices of prod(n) elements each,
+ * slice s of `in` at in + s·prod(n) (field-major, then batch), into the same layout of `out`. */
+cgef_status cgef_filter_apply(cgef_plan *plan, void *out, const void *in, int64_t nfields,
+                              int64_t nbatch, int32_t dtype, void *stream);
+
+/* Diagnostics.jl:389-416, 472-702 (2-D, Cartesian and spherical) and 792-857 (3-D Cartesian):
+ * Π = -rho0 · S̄:τ (rho0 = 1 reproduces the reference, which applies no density) for nbatch
+ * slices. w may be NULL for 2-D grids (required for 3-D). ubar/vbar/wbar (device, may be NULL)
+ * receive the filtered velocity the reference leaves in ws.u_filt/v_filt/w_filt (local frame on the
+ * sphere). E_out (host, nbatch entries, may be NULL) receives energy_from_filtered(!)
+ * (Diagnostics.jl:1086-1121) for each slice; requesting it synchronizes the stream. */
+cgef_status cgef_compute_pi(cgef_plan *plan, cgef_workspace *ws, void *Pi, const void *u,
+                            const void *v, const void *w, int64_t nbatch, int32_t dtype,
+                            double rho0, double *E_out, void *ubar, void *vbar, void *wbar,
+                            void *stream);
+
+/* Derivatives.jl:20-144 ddx!/ddy!/ddz!: derivative w.r.t. distance along axis 0/1/2 of nbatch
+ * slices (3-node stencil, masked cells 0). */
+cgef_status cgef_derivative(const cgef_grid *grid, int32_t axis, void *out, const void *in,
+                            int64_t nbatch, int32_t dtype, void *stream);
+
+/* Diagnostics.jl:1125-1134 _energy_over: ½ Σ_active |ū|² A / Σ_active A per slice (wbar may be
+ * NULL). Synchronizes the stream. */
+cgef_status cgef_energy(const cgef_grid *grid, const void *ubar, const void *vbar, const void *wbar,
+                        int64_t nbatch, int32_t dtype, double *E_out, void *stream);
+
+/* Diagnostics.jl:1144-1217 cumulative_energy!: filters the raw (u, v[, w]) with plans[s] and
+ * reduces E per scale; E_out is host (nscales × nbatch, scale fastest). Synchronizes the stream. */
+cgef_status cgef_cumulative_energy(cgef_plan *const *plans, int64_t nscales, cgef_workspace *ws,
+                                   const void *u, const void *v, const void *w, int64_t nbatch,
+                                   int32_t dtype, double *E_out, void *stream);
+
+/* Kernel timing (no counterpart in the reference; for measurement): while enabled, every launch of
+ * the plan's dominant filter kernel is bracketed by HIP events recorded on the launch stream.
+ * cgef_profile_read synchronizes on those events and returns the summed kernel time (ms) and the
+ * number of launches since enabling; enabling again resets the counters. */
+cgef_status cgef_profile_enable(cgef_plan *plan, int32_t on);
+cgef_status cgef_profile_read(cgef_plan *plan, double *total_ms, int64_t *nlaunches);
+
+/* Diagnostics.jl:1319-1333 spectral_density!: host arrays of length n. */
+cgef_status cgef_spectral_density(int64_t n, const double *C, const double *k, double *g);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

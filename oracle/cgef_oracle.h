@@ -1,0 +1,99 @@
+/*
+ * cgef_oracle.h — FP64 CPU restatement of the reference's real-space filter + Π pipeline.
+ *
+ * TEST INFRASTRUCTURE ONLY. This is the checker, never the product: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it. The product path
+ * (libcgef_hip.so) neither links nor calls anything declared here.
+ *
+ * Layout convention (the reference's): column-major, i (x / longitude) fastest, then j
+ * (y / latitude), then k (z), then trailing batch. 0-based indices here.
+ */
+#ifndef CGEF_ORACLE_H
+#define CGEF_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    int32_t ndim;          /* 2 or 3 */
+    int32_t spherical;     /* 0 Cartesian, 1 spherical lon-lat (ndim 2) */
+    int64_t n[3];          /* Nx, Ny, Nz */
+    const double *x[3];    /* axis coordinates; spherical: lon, lat in radians */
+    int32_t uniform[3];    /* 1: axis is a Range (constant step) */
+    int32_t periodic[3];
+    double period[3];
+    double R;              /* sphere radius */
+    const uint8_t *mask;   /* NULL = every cell active; else one byte per cell */
+} og_grid;
+
+enum { OG_TOPHAT = 0, OG_GAUSSIAN = 1 };
+enum { OG_ZEROFILL = 0, OG_DEFORMABLE = 1 };
+
+double og_kernel_weight(int kind, double alpha, double d, double ell);
+double og_kernel_radius(int kind, double alpha, double ell);
+double og_haversine(double R, double lam1, double phi1, double lam2, double phi2);
+double og_area(const og_grid *g, int64_t i, int64_t j, int64_t k);
+double og_cell_width(const og_grid *g, int axis, int64_t i);
+
+/* Banded 2-D footprint (CSR): entries di, dj, w; band b owns ptr[b]..ptr[b+1]-1. */
+typedef struct {
+    int64_t nentries;
+    int64_t nbands;
+    int32_t *di;
+    int32_t *dj;
+    double *w;
+    int64_t *ptr;
+} og_footprint;
+
+og_footprint *og_footprint_build(const og_grid *g, int kind, double alpha, double ell);
+void og_footprint_free(og_footprint *fp);
+int64_t og_footprint_nentries(const og_footprint *fp);
+int64_t og_footprint_nbands(const og_footprint *fp);
+void og_footprint_copy(const og_footprint *fp, int32_t *di, int32_t *dj, double *w, int64_t *ptr);
+
+/* Apply a banded footprint to K fields sharing one neighbour walk (2-D grids). */
+void og_apply_banded(const og_grid *g, const og_footprint *fp, int strategy, int K,
+                     const double *const *in, double *const *out, int nthreads);
+
+/* Separable Gaussian engine (2-D Cartesian, uniform axes): square truncation. */
+void og_apply_separable(const og_grid *g, double alpha, double ell, int strategy, int K,
+                        const double *const *in, double *const *out);
+
+/* 3-D Cartesian ball footprint applied per point (uniform or stretched axes). */
+void og_apply_nd3(const og_grid *g, int kind, double alpha, double ell, int strategy, int K,
+                  const double *const *in, double *const *out, int nthreads);
+
+/* First derivative along `axis` w.r.t. distance (3-node stencil, mask-run aware). */
+void og_deriv(const og_grid *g, int axis, const double *f, double *df);
+
+/* Planetary-Cartesian rotations (spherical). */
+void og_vector_to_cartesian(double u, double v, double w, double lam, double phi, double out[3]);
+void og_vector_from_cartesian(double X, double Y, double Z, double lam, double phi, double out[3]);
+void og_tensor_to_local(const double t[6], double lam, double phi, double out[6]);
+
+/* engine: 0 banded footprint, 1 separable (Cartesian Gaussian only), 2 3-D ball. */
+typedef struct {
+    int32_t kind;
+    double alpha;
+    double ell;
+    int32_t strategy;
+    int32_t engine;
+    int32_t nthreads;
+} og_filter_spec;
+
+/* Π = -S̄:τ for one slice (no ρ₀). Outputs ubar/vbar/wbar (may be NULL) hold the filtered
+ * velocity the reference leaves in the workspace (local frame on the sphere). */
+void og_compute_pi(const og_grid *g, const og_filter_spec *spec, const og_footprint *fp,
+                   const double *u, const double *v, const double *w,
+                   double *Pi, double *ubar, double *vbar, double *wbar);
+
+double og_active_area(const og_grid *g);
+double og_energy(const og_grid *g, const double *ub, const double *vb, const double *wb);
+void og_spectral_density(int64_t n, const double *C, const double *k, double *g);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,282 @@
+"""GPU parity: libcgef_hip.so (through the C ABI, via cgef) against the FP64 oracle on the same seeded
+inputs. Tolerances: FP64 kernels 1e-10 relative (summation order and FMA contraction are the only
+differences); the FP32 fused Π path rel-L2 ≤ 1e-5 against the FP64 oracle (the stated target)."""
+import numpy as np
+import pytest
+import torch
+
+import cgef
+from helpers import (cart_axes, oracle_grid_cart, oracle_grid_sphere, rel_l2, smooth_random_field, sphere_axes)
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+STRATS = {"zerofill": (cgef.ZeroFill(), O.ZEROFILL), "deformable": (cgef.Deformable(), O.DEFORMABLE)}
+
+
+def T(a, dtype=torch.float64):
+    return torch.tensor(np.ascontiguousarray(a), dtype=dtype, device=DEV)
+
+
+def N(t):
+    torch.cuda.synchronize()
+    return t.detach().cpu().numpy().astype(np.float64)
+
+
+def blob_mask(shape, seed, frac=0.25):
+    f = smooth_random_field(shape, seed, 1, smooth=6)[0]
+    return f > np.quantile(f, frac)
+
+
+# ------------------------------------------------------------------------------------------ 2-D --
+
+def cart_case(periodic=True, masked=False, nx=48, ny=40, dx=1000.0):
+    xs, ys = cart_axes(nx, dx), cart_axes(ny, dx * 1.25)
+    mask = blob_mask((ny, nx), 11) if masked else None
+    og = oracle_grid_cart(xs, ys, periodic=(periodic, periodic), mask=mask)
+    g = cgef.StructuredGrid(cgef.CartesianGeometry(), xs, ys, mask=mask, periodic=(periodic, periodic))
+    return og, g
+
+
+@pytest.mark.parametrize("engine", ["banded", "baseline"])
+@pytest.mark.parametrize("strategy", ["zerofill", "deformable"])
+@pytest.mark.parametrize("masked,periodic", [(False, True), (True, True), (True, False)])
+def test_filter_cartesian_fp64(engine, strategy, masked, periodic):
+    og, g = cart_case(periodic, masked)
+    f = smooth_random_field(g.field_shape, 3, 2)
+    s_cg, s_o = STRATS[strategy]
+    ker = cgef.GaussianKernel(6.0)
+    plan = cgef.plan_filter(g, ker, 6000.0, mask_strategy=s_cg, engine=engine)
+    assert plan.engine == engine
+    inp = T(np.stack(f))
+    out = torch.empty_like(inp)
+    cgef.filter_apply_(out, inp, plan)
+    ref = O.filter_fields(og, f, kind=O.GAUSSIAN, alpha=6.0, ell=6000.0, strategy=s_o)
+    got = N(out)
+    for m in range(2):
+        assert np.max(np.abs(got[m] - ref[m])) <= 1e-12 * np.max(np.abs(ref[m]))
+
+
+@pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-12), (torch.float32, 2e-6)])
+def test_filter_banded_fp32_fp64_batch(dtype, tol):
+    og, g = cart_case(True, True)
+    f = smooth_random_field((5,) + g.field_shape, 4, 1)[0]
+    plan = cgef.plan_filter(g, cgef.TopHatKernel(), 9000.0, engine="banded")
+    inp = T(f, dtype)
+    out = torch.empty_like(inp)
+    cgef.filter_apply_(out, inp, plan)
+    fin = N(inp)
+    ref = O.filter_fields(og, list(fin), kind=O.TOPHAT, alpha=0.0, ell=9000.0)
+    got = N(out)
+    for b in range(5):
+        assert np.max(np.abs(got[b] - ref[b])) <= tol * np.max(np.abs(ref[b]))
+
+
+@pytest.mark.parametrize("strategy", ["zerofill", "deformable"])
+@pytest.mark.parametrize("masked,periodic", [(False, True), (True, True), (True, False)])
+def test_filter_separable_fp64(strategy, masked, periodic):
+    og, g = cart_case(periodic, masked)
+    f = smooth_random_field(g.field_shape, 5, 1)[0]
+    s_cg, s_o = STRATS[strategy]
+    plan = cgef.plan_filter(g, cgef.GaussianKernel(6.0), 7000.0, mask_strategy=s_cg)
+    assert plan.engine == "separable"  # the reference's default for a Cartesian Gaussian
+    out = torch.empty(g.field_shape, dtype=torch.float64, device=DEV)
+    cgef.filter_apply_(out, T(f), plan)
+    ref = O.apply_separable(og, 6.0, 7000.0, s_o, [f])[0]
+    assert np.max(np.abs(N(out) - ref)) <= 1e-12 * np.max(np.abs(ref))
+
+
+def sphere_case(masked=True, nlon=96, nlat=48):
+    lam, phi = sphere_axes(nlon, nlat)
+    mask = blob_mask((nlat, nlon), 21, 0.3) if masked else None
+    og = oracle_grid_sphere(lam, phi, mask=mask)
+    g = cgef.StructuredGrid(cgef.SphericalGeometry(6.371e6), lam, phi, mask=mask)
+    return og, g, lam, phi
+
+
+@pytest.mark.parametrize("engine", ["banded", "baseline"])
+@pytest.mark.parametrize("strategy", ["zerofill", "deformable"])
+def test_filter_sphere_fp64(engine, strategy):
+    og, g, lam, phi = sphere_case(True)
+    f = smooth_random_field(g.field_shape, 6, 1)[0]
+    s_cg, s_o = STRATS[strategy]
+    plan = cgef.plan_filter(g, cgef.GaussianKernel(4.0), 1.6e6, mask_strategy=s_cg, engine=engine)
+    fp = O.Footprint(og, O.GAUSSIAN, 4.0, 1.6e6)
+    assert plan.info.nentries == len(fp.w)
+    out = torch.empty(g.field_shape, dtype=torch.float64, device=DEV)
+    cgef.filter_apply_(out, T(f), plan)
+    ref = O.apply_banded(og, fp, s_o, [f])[0]
+    assert np.max(np.abs(N(out) - ref)) <= 1e-12 * np.max(np.abs(ref))
+
+
+def test_derivatives_sphere_and_cartesian():
+    og, g, lam, phi = sphere_case(True)
+    f = smooth_random_field(g.field_shape, 7, 1)[0]
+    for ax, fn in ((0, cgef.ddx_), (1, cgef.ddy_)):
+        out = torch.empty(g.field_shape, dtype=torch.float64, device=DEV)
+        fn(out, T(f), g)
+        ref = O.deriv(og, ax, f)
+        assert np.max(np.abs(N(out) - ref)) <= 1e-12 * np.max(np.abs(ref))
+    og2, g2 = cart_case(False, True)
+    f2 = smooth_random_field(g2.field_shape, 8, 1)[0]
+    for ax, fn in ((0, cgef.ddx_), (1, cgef.ddy_)):
+        out = torch.empty(g2.field_shape, dtype=torch.float64, device=DEV)
+        fn(out, T(f2), g2)
+        ref = O.deriv(og2, ax, f2)
+        assert np.max(np.abs(N(out) - ref)) <= 1e-12 * np.max(np.abs(ref))
+
+
+# ------------------------------------------------------------------------------------ Π, 2-D --
+
+@pytest.mark.parametrize("engine", ["banded", "baseline", "separable"])
+@pytest.mark.parametrize("strategy", ["zerofill", "deformable"])
+@pytest.mark.parametrize("masked", [False, True])
+def test_pi_cartesian_fp64(engine, strategy, masked):
+    og, g = cart_case(True, masked)
+    u, v = smooth_random_field(g.field_shape, 9, 2)
+    s_cg, s_o = STRATS[strategy]
+    ker = cgef.GaussianKernel(6.0)
+    plan = cgef.plan_filter(g, ker, 6000.0, mask_strategy=s_cg, engine=engine)
+    Pi = torch.empty(g.field_shape, dtype=torch.float64, device=DEV)
+    ws = cgef.PiWorkspace(g, (), torch.float64)
+    _, E = cgef.compute_pi_(Pi, T(u), T(v), None, g, ker, 6000.0, workspace=ws, filter_plan=plan, return_energy=True)
+    eng = {"banded": O.ENGINE_BANDED, "baseline": O.ENGINE_BANDED, "separable": O.ENGINE_SEPARABLE}[engine]
+    ref, ub, vb, _ = O.compute_pi(og, u, v, kind=O.GAUSSIAN, alpha=6.0, ell=6000.0, strategy=s_o, engine=eng)
+    assert rel_l2(N(Pi), ref) <= 1e-10
+    assert rel_l2(N(ws.u_filt), ub) <= 1e-12 and rel_l2(N(ws.v_filt), vb) <= 1e-12
+    assert abs(E - O.energy(og, ub, vb)) <= 1e-12 * abs(O.energy(og, ub, vb))
+
+
+@pytest.mark.parametrize("strategy", ["zerofill", "deformable"])
+@pytest.mark.parametrize("masked", [False, True])
+def test_pi_sphere_fp64(strategy, masked):
+    og, g, lam, phi = sphere_case(masked)
+    u, v = smooth_random_field(g.field_shape, 10, 2)
+    s_cg, s_o = STRATS[strategy]
+    ker = cgef.GaussianKernel(4.0)
+    Pi = torch.empty(g.field_shape, dtype=torch.float64, device=DEV)
+    ws = cgef.PiWorkspace(g, (), torch.float64)
+    _, E = cgef.compute_pi_(Pi, T(u), T(v), None, g, ker, 1.2e6, workspace=ws, mask_strategy=s_cg, return_energy=True)
+    ref, ub, vb, _ = O.compute_pi(og, u, v, kind=O.GAUSSIAN, alpha=4.0, ell=1.2e6, strategy=s_o)
+    assert rel_l2(N(Pi), ref) <= 1e-10
+    assert rel_l2(N(ws.u_filt), ub) <= 1e-12
+    assert abs(E - O.energy(og, ub, vb)) <= 1e-12 * abs(O.energy(og, ub, vb))
+
+
+@pytest.mark.parametrize("geom", ["cartesian", "sphere"])
+@pytest.mark.parametrize("strategy", ["zerofill", "deformable"])
+def test_pi_fp32_meets_tolerance(geom, strategy):
+    """FP32 inputs, FP32 fused kernel vs the FP64 oracle on the same (FP32-rounded) inputs."""
+    if geom == "cartesian":
+        og, g = cart_case(True, True, nx=96, ny=80)
+        ell, alpha = 8000.0, 6.0
+    else:
+        og, g, _, _ = sphere_case(True, 144, 72)
+        ell, alpha = 9e5, 4.0
+    u, v = smooth_random_field(g.field_shape, 12, 2, smooth=2)
+    u32, v32 = T(u + 0.3, torch.float32), T(v - 0.2, torch.float32)   # a mean flow exercises the shift
+    s_cg, s_o = STRATS[strategy]
+    Pi = torch.empty(g.field_shape, dtype=torch.float32, device=DEV)
+    cgef.compute_pi_(Pi, u32, v32, None, g, cgef.GaussianKernel(alpha), ell, mask_strategy=s_cg,
+                     filter_plan=cgef.plan_filter(g, cgef.GaussianKernel(alpha), ell, mask_strategy=s_cg,
+                                                  engine="banded"))
+    ref, _, _, _ = O.compute_pi(og, N(u32), N(v32), kind=O.GAUSSIAN, alpha=alpha, ell=ell, strategy=s_o)
+    assert rel_l2(N(Pi), ref) <= 1e-5
+
+
+def test_pi_batch_equals_slices():
+    og, g, _, _ = sphere_case(True)
+    fs = smooth_random_field((3,) + g.field_shape, 13, 2)
+    u, v = T(fs[0], torch.float32), T(fs[1], torch.float32)
+    ker = cgef.GaussianKernel(4.0)
+    plan = cgef.plan_filter(g, ker, 1.0e6)
+    Pb = torch.empty_like(u)
+    _, Eb = cgef.compute_pi_(Pb, u, v, None, g, ker, 1.0e6, filter_plan=plan, return_energy=True)
+    for b in range(3):
+        Ps = torch.empty_like(u[b])
+        _, Es = cgef.compute_pi_(Ps, u[b].contiguous(), v[b].contiguous(), None, g, ker, 1.0e6, filter_plan=plan,
+                                 return_energy=True)
+        assert torch.equal(Ps, Pb[b])
+        assert Es == Eb[b]
+
+
+def test_cumulative_energy_and_spectrum():
+    og, g, _, _ = sphere_case(True)
+    u, v = smooth_random_field(g.field_shape, 14, 2)
+    scales = [6e5, 9e5, 1.3e6, 2e6]
+    E = cgef.cumulative_energy(T(u), T(v), None, g, cgef.GaussianKernel(4.0), scales)
+    Eo = O.cumulative_energy(og, u, v, scales, kind=O.GAUSSIAN, alpha=4.0)
+    assert np.max(np.abs(E - Eo) / np.abs(Eo)) <= 1e-12
+    k, Et = cgef.filtering_spectrum(T(u), T(v), None, g, cgef.GaussianKernel(4.0), scales, L=1.0)
+    assert np.allclose(Et, O.spectral_density(Eo, 1.0 / np.asarray(scales)), rtol=1e-9)
+
+
+# ------------------------------------------------------------------------------------------ 3-D --
+
+def volume_case(stretched=True, masked=False):
+    xs, ys = cart_axes(24, 1.0), cart_axes(20, 1.0)
+    if stretched:
+        zs = np.concatenate([[0.0], np.cumsum(0.8 * 1.15 ** np.arange(11))])
+    else:
+        zs = cart_axes(12, 1.0)
+    shape = (len(zs), len(ys), len(xs))
+    mask = blob_mask(shape, 31, 0.2) if masked else None
+    og = oracle_grid_cart(xs, ys, zs, periodic=(True, True, False), mask=mask)
+    g = cgef.StructuredGrid(cgef.CartesianGeometry(), xs, ys, zs, mask=mask, periodic=(True, True, False))
+    return og, g
+
+
+@pytest.mark.parametrize("stretched", [True, False])
+@pytest.mark.parametrize("kernel", ["tophat", "gaussian"])
+@pytest.mark.parametrize("masked", [False, True])
+def test_pi_3d_fp64(stretched, kernel, masked):
+    og, g = volume_case(stretched, masked)
+    u, v, w = smooth_random_field(g.field_shape, 15, 3)
+    ker = cgef.TopHatKernel() if kernel == "tophat" else cgef.GaussianKernel(6.0)
+    kind, alpha, ell = (O.TOPHAT, 0.0, 6.0) if kernel == "tophat" else (O.GAUSSIAN, 6.0, 3.0)
+    Pi = torch.empty(g.field_shape, dtype=torch.float64, device=DEV)
+    ws = cgef.PiWorkspace(g, (), torch.float64)
+    _, E = cgef.compute_pi_(Pi, T(u), T(v), T(w), g, ker, ell, workspace=ws, return_energy=True)
+    ref, ub, vb, wb = O.compute_pi(og, u, v, w, kind=kind, alpha=alpha, ell=ell, engine=O.ENGINE_ND3)
+    assert rel_l2(N(Pi), ref) <= 1e-10
+    assert rel_l2(N(ws.w_filt), wb) <= 1e-12
+    assert abs(E - O.energy(og, ub, vb, wb)) <= 1e-12 * O.energy(og, ub, vb, wb)
+
+
+def test_pi_3d_fp32():
+    og, g = volume_case(True, False)
+    u, v, w = smooth_random_field(g.field_shape, 16, 3)
+    f32 = [T(a, torch.float32) for a in (u, v, w)]
+    Pi = torch.empty(g.field_shape, dtype=torch.float32, device=DEV)
+    cgef.compute_pi_(Pi, *f32, g, cgef.TopHatKernel(), 6.0)
+    ref, _, _, _ = O.compute_pi(og, *[N(a) for a in f32], kind=O.TOPHAT, alpha=0.0, ell=6.0, engine=O.ENGINE_ND3)
+    assert rel_l2(N(Pi), ref) <= 1e-5
+
+
+def test_derivative_z_stretched():
+    og, g = volume_case(True, True)
+    f = smooth_random_field(g.field_shape, 17, 1)[0]
+    out = torch.empty(g.field_shape, dtype=torch.float64, device=DEV)
+    cgef.ddz_(out, T(f), g)
+    ref = O.deriv(og, 2, f)
+    assert np.max(np.abs(N(out) - ref)) <= 1e-12 * np.max(np.abs(ref))
+
+
+# --------------------------------------------------------------------------- errors on the GPU --
+
+def test_error_paths():
+    og, g = cart_case(True, False)
+    with pytest.raises(cgef.ArgumentError):
+        cgef.plan_filter(g, cgef.GaussianKernel(), -1.0)
+    with pytest.raises(cgef.ArgumentError):
+        cgef.plan_filter(g, cgef.GaussianKernel(), float("nan"))
+    plan = cgef.plan_filter(g, cgef.GaussianKernel(), 5000.0, engine="banded")
+    bad = torch.zeros((3, 3), dtype=torch.float64, device=DEV)
+    with pytest.raises(cgef.DimensionMismatch):
+        cgef.filter_apply_(bad, bad, plan)
+    u = torch.zeros(g.field_shape, dtype=torch.float64, device=DEV)
+    v = torch.zeros((2,) + g.field_shape, dtype=torch.float64, device=DEV)
+    with pytest.raises(cgef.DimensionMismatch):
+        cgef.compute_pi_(torch.zeros_like(u), u, v, None, g, cgef.GaussianKernel(), 5000.0)
+    with pytest.raises(cgef.ArgumentError):
+        cgef.filter_apply_(torch.empty_like(u), u.to(torch.float32), plan)
