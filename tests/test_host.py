@@ -1,0 +1,92 @@
+"""Host-side checks that need no GPU: the C ABI library loads and exports every symbol include/cgef.h
+declares, the host-only entry points behave as the reference's, and the Python mirror's argument
+handling (kernels, spectrum policies, grids) matches the reference."""
+import ctypes as C
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+import cgef
+from cgef import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    src = open(os.path.join(ROOT, "include", "cgef.h")).read()
+    return sorted(set(re.findall(r"\b(cgef_[a-z_]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.load()
+    names = declared_symbols()
+    assert len(names) >= 18
+    for n in names:
+        assert hasattr(L, n), n
+    assert set(names) == set(_lib.EXPORTED)
+    assert cgef.version().startswith("cgef-mi355x")
+
+
+def test_spectral_density_host_entry_point():
+    # Diagnostics.jl:1319-1333: exact for a quadratic on a uniform grid, one-sided at the ends
+    k = np.arange(1.0, 6.0)
+    g = cgef.spectral_density(k ** 2, k)
+    assert g[2] == pytest.approx(2 * k[2])
+    assert g[0] == pytest.approx((4 - 1) / 1.0) and g[-1] == pytest.approx((25 - 16) / 1.0)
+    assert np.all(cgef.spectral_density(np.array([3.0]), np.array([1.0])) == 0.0)
+    with pytest.raises(cgef.DimensionMismatch):
+        cgef.spectral_density_(np.zeros(2), np.zeros(3), np.zeros(3))
+    st = _lib.load().cgef_spectral_density(-1, None, None, None)
+    assert st == _lib.CGEF_ERR_DIMENSION
+
+
+def test_kernels_and_spectrum_policy():
+    g = cgef.GaussianKernel(6.0)
+    assert cgef.kernel_weight(g, 1.0, 1.0) == pytest.approx(math.exp(-6.0))
+    assert cgef.kernel_radius(cgef.TopHatKernel(), 8.0) == 4.0
+    assert cgef.spectral_transfer(g, 0.0, 5.0) == 1.0
+    assert cgef.transfer_monotone(g) and not cgef.transfer_monotone(cgef.TopHatKernel())
+    assert cgef.gate_spectrum(g, cgef.StrictSpectrum())
+    with pytest.raises(cgef.ArgumentError):
+        cgef.gate_spectrum(cgef.TopHatKernel(), cgef.StrictSpectrum())
+    assert not cgef.gate_spectrum(cgef.TopHatKernel(), cgef.NoSpectrum())
+    with pytest.warns(UserWarning):
+        assert cgef.gate_spectrum(cgef.TopHatKernel(), cgef.ForceSpectrum())
+
+
+def test_plan_filter_rejects_bad_scale_before_touching_a_device():
+    x = np.arange(8.0)
+    grid = cgef.StructuredGrid(cgef.CartesianGeometry(), x, x)
+    for bad in (0.0, -1.0, float("nan"), float("inf")):
+        with pytest.raises(cgef.ArgumentError):
+            cgef.plan_filter(grid, cgef.GaussianKernel(), bad)
+
+
+def test_grid_description():
+    lam = (np.arange(16) + 0.5) * (2 * np.pi / 16)
+    phi = -np.pi / 2 + (np.arange(8) + 0.5) * (np.pi / 8)
+    g = cgef.StructuredGrid(cgef.SphericalGeometry(), lam, phi)
+    assert g.periodic == [True, False] and g.uniform == [True, True]
+    assert g.field_shape == (8, 16) and g.shape == (16, 8)
+    with pytest.raises(cgef.DimensionMismatch):
+        cgef.StructuredGrid(cgef.CartesianGeometry(), np.arange(4.0), np.arange(5.0), mask=np.ones((4, 5)))
+    z = np.cumsum(1.03 ** np.arange(6))
+    g3 = cgef.StructuredGrid(cgef.CartesianGeometry(), np.arange(4.0), np.arange(5.0), z)
+    assert g3.uniform == [True, True, False] and g3.field_shape == (6, 5, 4)
+    r = cgef.Range(0.0, 0.25, 4)
+    assert np.allclose(r.values, [0, 0.25, 0.5, 0.75])
+
+
+def test_compute_pi_validates_layout_on_host():
+    x = np.arange(8.0)
+    grid = cgef.StructuredGrid(cgef.CartesianGeometry(), x, x)
+    with pytest.raises(cgef.ArgumentError):
+        cgef.filtering.filter_apply_(np.zeros((8, 8)), np.zeros((8, 8)), _FakePlan(grid))
+
+
+class _FakePlan:
+    def __init__(self, grid):
+        self.grid, self.device = grid, 0
