@@ -37,7 +37,7 @@ class GPUBackend:
 
 
 ENGINES = {"auto": _lib.ENGINE_AUTO, "banded": _lib.ENGINE_BANDED, "separable": _lib.ENGINE_SEPARABLE,
-           "baseline": _lib.ENGINE_BASELINE}
+           "baseline": _lib.ENGINE_BASELINE, "ring": _lib.ENGINE_RING}
 ENGINE_NAMES = {v: k for k, v in ENGINES.items()}
 
 

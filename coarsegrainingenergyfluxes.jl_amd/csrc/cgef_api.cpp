@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "cgef_launch.hpp"
+#include "cgef_ring.hpp"
 
 #define CGEF_VERSION_STRING "cgef-mi355x 0.1.0 (gfx950)"
 
@@ -113,6 +114,7 @@ Plan::~Plan() {
         dfree(sep_prof_d[a]);
     }
     dfree(sep_invden_d);
+    free_ring(ring);
     if (grid && --grid->refs == 0) delete grid;
 }
 
@@ -125,6 +127,9 @@ Workspace::~Workspace() {
     dfree(scratch64);
     dfree(fbuf);
     dfree(E_dev);
+    dfree(rxy);
+    dfree(rxf);
+    dfree(rout);
 }
 
 GeomArgs geom_args(const Grid &g) {
@@ -155,9 +160,7 @@ BandArgs band_args(const Plan &p, bool fp32) {
     b.wd = fp32 ? p.w32d_d : p.w64_d;
     b.D_row = fp32 ? p.D32_row_d : p.D64_row_d;
     b.strategy = p.strategy;
-    int mr = 1;
-    for (const Run &r : p.runs) mr = std::max(mr, r.hi - r.lo + 1);
-    b.max_run = mr;
+    b.max_run = p.max_run;
     return b;
 }
 
@@ -280,6 +283,48 @@ static void build_csr(Plan &p) {
 }
 
 static hipStream_t S(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Ring engine scratch: ncol spatial planes (FP64, input and output share) and two spectra sets.
+static void ring_reserve(Workspace &w, const Plan &p, int64_t ncol) {
+    const Grid &g = *p.grid;
+    const size_t nxy = (size_t)ncol * g.npts;
+    const size_t nxf = (size_t)ncol * p.ring.mh * g.nrows;
+    if (w.nrxy < nxy) {
+        dfree(w.rxy);
+        CGEF_HIP(hipMalloc(&w.rxy, nxy * sizeof(double)));
+        w.nrxy = nxy;
+    }
+    if (w.nrxf < nxf) {
+        dfree(w.rxf);
+        dfree(w.rout);
+        CGEF_HIP(hipMalloc(&w.rxf, nxf * sizeof(double2)));
+        CGEF_HIP(hipMalloc(&w.rout, nxf * sizeof(double2)));
+        w.nrxf = nxf;
+    }
+}
+
+// forward transform of w.rxy (ncol planes), banded contraction, inverse back into w.rxy
+static void ring_filter_planes(Plan &p, Workspace &w, int64_t ncol, hipStream_t s) {
+    ring_forward(p.ring, w.rxy, w.rxf, ncol, s);
+    p.prof_begin(s);
+    ring_contract(p.ring, w.rxf, w.rout, ncol, s);
+    p.prof_end(s);
+    ring_inverse(p.ring, w.rout, w.rxy, ncol, s);
+}
+
+static int ring_nfields(const Grid &g, bool masked) { return (g.ndim == 2 && g.geometry == CGEF_CARTESIAN ? 5 : 9) + (masked ? 1 : 0); }
+
+// filter nsl plain slices with the ring engine
+static void ring_filter_apply(Plan &p, Workspace &w, int dtype, const void *in, void *out, int64_t nsl, hipStream_t s) {
+    Grid &g = *p.grid;
+    const GeomArgs G = geom_args(g);
+    const int withmask = (g.has_mask && p.strategy == CGEF_DEFORMABLE) ? 1 : 0;
+    const int64_t ncol = nsl + withmask;
+    ring_reserve(w, p, ncol);
+    ring_prep_fields(G, dtype, in, nsl, withmask, w.rxy, s);
+    ring_filter_planes(p, w, ncol, s);
+    ring_normalize(G, dtype, w.rxy, p.D64_row_d, nsl, withmask, p.strategy, out, s);
+}
 
 static void export_planes(const Grid &g, const double *ubar, int plane, void *out, int64_t nbatch, int dtype, hipStream_t s) {
     const int64_t P = g.npts;
@@ -472,31 +517,54 @@ cgef_status cgef_plan_create(cgef_grid *grid, const cgef_kernel_desc *kernel, do
             // Kernels.jl:354-359
             p->rad = p->kind == CGEF_TOPHAT ? scale / 2.0 : scale * std::sqrt(-std::log(1e-10) / p->alpha);
             int eng = engine;
+            bool sep = false;
             if (eng == CGEF_ENGINE_AUTO) {
                 // the reference's dispatch: a Gaussian on a uniform 2-D Cartesian grid takes the
-                // separable engine (Filtering.jl:1533-1540), everything else the banded footprint
-                const bool sep = g->ndim == 2 && g->geometry == CGEF_CARTESIAN && p->kind == CGEF_GAUSSIAN &&
-                                 g->uniform[0] && g->uniform[1];
-                eng = sep ? CGEF_ENGINE_SEPARABLE : CGEF_ENGINE_BANDED;
+                // separable engine (Filtering.jl:1533-1540), everything else the banded footprint —
+                // evaluated by the ring engine where x is periodic and factorable
+                sep = g->ndim == 2 && g->geometry == CGEF_CARTESIAN && p->kind == CGEF_GAUSSIAN && g->uniform[0] &&
+                      g->uniform[1];
             }
-            if (eng != CGEF_ENGINE_BANDED && eng != CGEF_ENGINE_SEPARABLE && eng != CGEF_ENGINE_BASELINE)
+            if (eng != CGEF_ENGINE_AUTO && eng != CGEF_ENGINE_BANDED && eng != CGEF_ENGINE_SEPARABLE &&
+                eng != CGEF_ENGINE_BASELINE && eng != CGEF_ENGINE_RING)
                 fail(CGEF_ERR_ARGUMENT, "unknown engine");
-            p->engine = eng;
-            if (eng == CGEF_ENGINE_SEPARABLE) {
+            if (eng == CGEF_ENGINE_SEPARABLE || sep) {
+                p->engine = CGEF_ENGINE_SEPARABLE;
                 build_separable(*p);
             } else {
                 build_banded(*p);
-                split_wide_runs(*p);
-                p->runs_d = upload(p->runs);
-                p->run_ptr_d = upload(p->run_ptr);
+                if (eng == CGEF_ENGINE_AUTO) eng = ring_supported(*p) ? CGEF_ENGINE_RING : CGEF_ENGINE_BANDED;
+                if (eng == CGEF_ENGINE_RING && !ring_supported(*p))
+                    fail(CGEF_ERR_UNSUPPORTED,
+                         "the ring engine needs a periodic, uniform x axis whose length factors into 2,3,5,7,11,13");
+                p->engine = eng;
                 p->band_of_row_d = upload(p->band_of_row);
-                p->row_order_d = upload(p->row_order);
-                p->w64_d = upload(p->w64);
-                p->w32_d = upload(p->w32);
-                p->w32d_d = upload(p->w32d);
                 p->D64_row_d = upload(p->D64_row);
-                p->D32_row_d = upload(p->D32_row);
-                if (eng == CGEF_ENGINE_BASELINE) build_csr(*p);
+                if (eng == CGEF_ENGINE_RING) {
+                    // Ŵ is built on the device from the unsplit runs; the direct-sum tables are then dropped
+                    p->runs_d = upload(p->runs);
+                    p->w64_d = upload(p->w64);
+                    build_ring(*p, nullptr);
+                    dfree(p->runs_d);
+                    dfree(p->w64_d);
+                } else {
+                    split_wide_runs(*p);
+                    p->runs_d = upload(p->runs);
+                    p->run_ptr_d = upload(p->run_ptr);
+                    p->row_order_d = upload(p->row_order);
+                    p->w64_d = upload(p->w64);
+                    p->w32_d = upload(p->w32);
+                    p->w32d_d = upload(p->w32d);
+                    p->D32_row_d = upload(p->D32_row);
+                    if (eng == CGEF_ENGINE_BASELINE) build_csr(*p);
+                }
+                int mr = 1;
+                for (const Run &r : p->runs) mr = std::max(mr, r.hi - r.lo + 1);
+                p->max_run = mr;
+                // the host copies of the weight tables are not needed once resident
+                std::vector<double>().swap(p->w64);
+                std::vector<double>().swap(p->w32d);
+                std::vector<float>().swap(p->w32);
             }
         } catch (...) {
             delete p;
@@ -562,6 +630,8 @@ cgef_status cgef_filter_apply(cgef_plan *plan, void *out, const void *in, int64_
         hipStream_t s = S(stream);
         if (p->engine == CGEF_ENGINE_BANDED) {
             launch_filter_lin(G, band_args(*p, dtype == CGEF_F32), dtype, g.has_mask, p->edge_x, in, out, nsl, s);
+        } else if (p->engine == CGEF_ENGINE_RING) {
+            ring_filter_apply(*p, resolve_ws(g, nullptr, 1, dtype), dtype, in, out, nsl, s);
         } else if (p->engine == CGEF_ENGINE_SEPARABLE) {
             Workspace &w = resolve_ws(g, nullptr, 1, dtype);
             double *rp = (double *)ws_fbuf(w, (size_t)g.npts * nsl * sizeof(double));
@@ -597,6 +667,17 @@ cgef_status cgef_compute_pi(cgef_plan *plan, cgef_workspace *ws, void *Pi, const
             launch_pi_filter(G, band_args(*p, dtype == CGEF_F32), dtype, g.has_mask, p->edge_x, W.qpack, W.cmean, W.ubar,
                              W.tau, nbatch, s);
             p->prof_end(s);
+            launch_strain_pi(G, dtype, W.ubar, W.tau, Pi, rho0, nbatch, W.partials, W.npartials, s);
+            CGEF_HIP(hipGetLastError());
+            if (E_out) fetch_energy(W, nbatch, strain_blocks(G), g.active_area, E_out, s);
+        } else if (p->engine == CGEF_ENGINE_RING) {
+            const int nf = ring_nfields(g, g.has_mask);
+            ring_reserve(W, *p, nf * nbatch);
+            launch_mean(G, dtype, u, v, w, nbatch, W.partials, W.npartials, W.cmean, s);
+            ring_prep(G, dtype, u, v, w, W.cmean, nf, g.has_mask ? 1 : 0, W.rxy, nbatch, s);
+            ring_filter_planes(*p, W, nf * nbatch, s);
+            ring_epilogue(G, dtype, W.rxy, p->D64_row_d, W.cmean, nf, g.has_mask ? 1 : 0, p->strategy, W.ubar, W.tau,
+                          nbatch, s);
             launch_strain_pi(G, dtype, W.ubar, W.tau, Pi, rho0, nbatch, W.partials, W.npartials, s);
             CGEF_HIP(hipGetLastError());
             if (E_out) fetch_energy(W, nbatch, strain_blocks(G), g.active_area, E_out, s);
@@ -667,6 +748,8 @@ cgef_status cgef_cumulative_energy(cgef_plan *const *plans, int64_t nscales, cge
             auto filt = [&](const void *in, void *out) {
                 if (p->engine == CGEF_ENGINE_BANDED) {
                     launch_filter_lin(G, band_args(*p, dtype == CGEF_F32), dtype, g.has_mask, p->edge_x, in, out, nbatch, s);
+                } else if (p->engine == CGEF_ENGINE_RING) {
+                    ring_filter_apply(*p, W, dtype, in, out, nbatch, s);
                 } else if (p->engine == CGEF_ENGINE_SEPARABLE) {
                     double *rp = ws_scratch64(W, P * nbatch);
                     launch_separable(G, *p, dtype, in, out, nbatch, rp, s);

@@ -31,6 +31,39 @@ struct alignas(32) Run {
 
 struct Workspace;
 
+// Ring engine plan (see cgef_ring.hip): mixed-radix factorization of the row length, the (dz, dy)
+// offsets of the footprint and the Fourier weights Ŵ[m][offset][band].
+constexpr int RING_MAX_STAGES = 24;
+
+struct RingFft {
+    int n = 0;
+    int nstage = 0;
+    int radix[RING_MAX_STAGES] = {};
+};
+
+struct RingArgs {
+    int64_t ny = 1, nz = 1, nrows = 1;
+    int32_t py = 0, pz = 0;
+    int32_t noff = 0, nbands = 0;
+    int32_t ndz = 0;
+    int32_t dy_min = 0, dy_max = 0;
+    const int32_t *band_of_row = nullptr;
+    const double *what = nullptr;      // [m][offset][band]
+    const int32_t *dz_val = nullptr;   // per dz group
+    const int32_t *dz_off0 = nullptr;  // offsets of group g: dz_off0[g] .. dz_off0[g+1]-1
+    const int32_t *off_dy = nullptr;   // per offset
+};
+
+struct RingPlan {
+    bool ok = false;
+    RingFft fft;
+    int64_t mh = 0;                    // N/2 + 1 Fourier modes kept
+    double2 *tw_d = nullptr;           // exp(−2πik/N), k < N
+    double *what_d = nullptr;
+    int32_t *dz_val_d = nullptr, *dz_off0_d = nullptr, *off_dy_d = nullptr;
+    RingArgs args;
+};
+
 struct Grid {
     int device = 0;
     int ndim = 2, geometry = CGEF_CARTESIAN;
@@ -71,6 +104,7 @@ struct Plan {
     int64_t nentries = 0;
     double mean_taps = 0.0;
     int64_t max_taps = 0;
+    int32_t max_run = 1;          // widest run after splitting (LDS segment sizing)
     double active_taps = 0.0;     // Σ over active target points of their in-support taps
     // banded tables (device)
     Run *runs_d = nullptr;
@@ -85,6 +119,8 @@ struct Plan {
     double *sep_g_d[2] = {nullptr, nullptr};
     double *sep_prof_d[2] = {nullptr, nullptr};
     double *sep_invden_d = nullptr;
+    // ring engine
+    RingPlan ring;
     // optional kernel timing: event pairs around each dominant-kernel launch
     bool profiling = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_events;
@@ -111,6 +147,10 @@ struct Workspace {
     int64_t nscratch64 = 0;
     void *fbuf = nullptr;         // filtered-field / row-pass scratch, grown on demand
     size_t nfbuf = 0;
+    // ring engine scratch: spatial planes (in and out share), forward spectra, contracted spectra
+    double *rxy = nullptr;
+    double2 *rxf = nullptr, *rout = nullptr;
+    size_t nrxy = 0, nrxf = 0;
     ~Workspace();
 };
 

@@ -1,0 +1,30 @@
+// Ring engine (Fourier-space evaluation of the banded footprint along the periodic x axis): plan
+// structures and launches. See cgef_ring.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "cgef_launch.hpp"
+
+namespace cgef {
+
+bool ring_factor(int n, RingFft &F);
+bool ring_supported(const Plan &p);
+void build_ring(Plan &p, hipStream_t s);
+void free_ring(RingPlan &R);
+void ring_build_what(const RingPlan &R, const Run *runs_d, const int32_t *run_off_d, const double *w64_d, int64_t nruns,
+                     hipStream_t s);
+void ring_forward(const RingPlan &R, const double *xin, double2 *XF, int64_t ncol, hipStream_t s);
+void ring_contract(const RingPlan &R, const double2 *XF, double2 *OUT, int64_t ncol, hipStream_t s);
+void ring_inverse(const RingPlan &R, const double2 *OUT, double *yout, int64_t ncol, hipStream_t s);
+void ring_prep(const GeomArgs &G, int dtype, const void *u, const void *v, const void *w, const double *cmean, int nf,
+               int masked, double *xin, int64_t nbatch, hipStream_t s);
+void ring_epilogue(const GeomArgs &G, int dtype, const double *yout, const double *D_row, const double *cmean, int nf,
+                   int masked, int strategy, double *ubar, void *tau, int64_t nbatch, hipStream_t s);
+void ring_prep_fields(const GeomArgs &G, int dtype, const void *in, int64_t nsl, int withmask, double *xin, hipStream_t s);
+void ring_normalize(const GeomArgs &G, int dtype, const double *yout, const double *D_row, int64_t nsl, int withmask,
+                    int strategy, void *out, hipStream_t s);
+
+}  // namespace cgef

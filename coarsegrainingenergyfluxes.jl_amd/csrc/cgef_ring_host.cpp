@@ -1,0 +1,125 @@
+// Host side of the ring engine: factorization, twiddles, (dz, dy) offset table and the device-built
+// Fourier weights Ŵ[m][offset][band].
+#include <algorithm>
+#include <cmath>
+#include <map>
+
+#include "cgef_ring.hpp"
+
+namespace cgef {
+
+bool ring_factor(int n, RingFft &F) {
+    F = RingFft{};
+    F.n = n;
+    if (n < 2) return false;
+    int m = n;
+    const int order[] = {4, 2, 3, 5, 7, 11, 13};
+    for (int r : order) {
+        while (m % r == 0) {
+            if (F.nstage == RING_MAX_STAGES) return false;
+            F.radix[F.nstage++] = r;
+            m /= r;
+        }
+    }
+    return m == 1;
+}
+
+template <class T>
+static T *up(const std::vector<T> &v) {
+    T *d = nullptr;
+    if (v.empty()) return d;
+    CGEF_HIP(hipMalloc(&d, v.size() * sizeof(T)));
+    CGEF_HIP(hipMemcpy(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return d;
+}
+
+// Whether the ring engine can serve this plan: a periodic, uniform x axis whose length factors into
+// the supported radices, and runs that are symmetric modulo the ring (so Ŵ is real).
+bool ring_supported(const Plan &p) {
+    const Grid &g = *p.grid;
+    if (!g.periodic[0] || !g.uniform[0] || g.n[0] < 2) return false;
+    RingFft F;
+    if (!ring_factor((int)g.n[0], F)) return false;
+    for (const Run &r : p.runs)
+        if (!(r.lo == -r.hi || (int64_t)(r.hi - r.lo + 1) == g.n[0])) return false;
+    return true;
+}
+
+// Requires p.runs / p.run_ptr / p.w64 from build_banded (before any run splitting) and the device
+// copies runs_d / w64_d / band_of_row_d.
+void build_ring(Plan &p, hipStream_t s) {
+    Grid &g = *p.grid;
+    RingPlan &R = p.ring;
+    if (!ring_factor((int)g.n[0], R.fft)) fail(CGEF_ERR_UNSUPPORTED, "row length does not factor into radices 2,3,5,7,11,13");
+    const int n = (int)g.n[0];
+    R.mh = n / 2 + 1;
+    // offsets (dz, dy), sorted
+    std::map<std::pair<int, int>, int> offs;
+    for (const Run &r : p.runs) offs[{r.dz, r.dy}] = 0;
+    std::vector<int32_t> off_dy, dz_val, dz_off0;
+    int idx = 0, dymin = 0, dymax = 0;
+    int last_dz = INT32_MIN;
+    for (auto &kv : offs) {
+        kv.second = idx;
+        const int dz = kv.first.first, dy = kv.first.second;
+        if (dz != last_dz) {
+            dz_val.push_back(dz);
+            dz_off0.push_back(idx);
+            last_dz = dz;
+        }
+        off_dy.push_back(dy);
+        dymin = std::min(dymin, dy);
+        dymax = std::max(dymax, dy);
+        idx++;
+    }
+    dz_off0.push_back(idx);
+    std::vector<int32_t> run_off(2 * p.runs.size());
+    for (int b = 0; b < p.nbands; b++)
+        for (int32_t q = p.run_ptr[b]; q < p.run_ptr[b + 1]; q++) {
+            run_off[2 * q] = b;
+            run_off[2 * q + 1] = offs[{p.runs[q].dz, p.runs[q].dy}];
+        }
+    std::vector<double2> tw(n);
+    for (int k = 0; k < n; k++) {
+        const double a = 2.0 * M_PI * (double)k / (double)n;
+        tw[k] = make_double2(std::cos(a), -std::sin(a));
+    }
+    R.tw_d = up(tw);
+    R.off_dy_d = up(off_dy);
+    R.dz_val_d = up(dz_val);
+    R.dz_off0_d = up(dz_off0);
+    int32_t *run_off_d = up(run_off);
+    RingArgs &A = R.args;
+    A.ny = g.n[1];
+    A.nz = g.n[2];
+    A.nrows = g.n[1] * g.n[2];
+    A.py = g.periodic[1];
+    A.pz = g.ndim == 3 ? g.periodic[2] : 0;
+    A.noff = idx;
+    A.nbands = p.nbands;
+    A.ndz = (int32_t)dz_val.size();
+    A.dy_min = dymin;
+    A.dy_max = dymax;
+    A.band_of_row = p.band_of_row_d;
+    A.dz_val = R.dz_val_d;
+    A.dz_off0 = R.dz_off0_d;
+    A.off_dy = R.off_dy_d;
+    CGEF_HIP(hipMalloc(&R.what_d, (size_t)R.mh * A.noff * A.nbands * sizeof(double)));
+    A.what = R.what_d;
+    ring_build_what(R, p.runs_d, run_off_d, p.w64_d, (int64_t)p.runs.size(), s);
+    CGEF_HIP(hipGetLastError());
+    CGEF_HIP(hipStreamSynchronize(s));
+    (void)hipFree(run_off_d);
+    R.ok = true;
+}
+
+void free_ring(RingPlan &R) {
+    if (R.tw_d) (void)hipFree(R.tw_d);
+    if (R.what_d) (void)hipFree(R.what_d);
+    if (R.dz_val_d) (void)hipFree(R.dz_val_d);
+    if (R.dz_off0_d) (void)hipFree(R.dz_off0_d);
+    if (R.off_dy_d) (void)hipFree(R.off_dy_d);
+    R = RingPlan{};
+}
+
+}  // namespace cgef

@@ -280,3 +280,76 @@ def test_error_paths():
         cgef.compute_pi_(torch.zeros_like(u), u, v, None, g, cgef.GaussianKernel(), 5000.0)
     with pytest.raises(cgef.ArgumentError):
         cgef.filter_apply_(torch.empty_like(u), u.to(torch.float32), plan)
+
+
+# ------------------------------------------------------------------------------ ring engine --
+# The ring engine evaluates the same banded footprint in Fourier space along the periodic x axis
+# (FP64 throughout): parity with the oracle's direct sum to FFT rounding.
+
+@pytest.mark.parametrize("strategy", ["zerofill", "deformable"])
+@pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-11), (torch.float32, 1e-6)])
+def test_ring_filter_sphere(strategy, dtype, tol):
+    og, g, _, _ = sphere_case(True)
+    f = smooth_random_field((3,) + g.field_shape, 18, 1)[0]
+    s_cg, s_o = STRATS[strategy]
+    plan = cgef.plan_filter(g, cgef.GaussianKernel(4.0), 1.6e6, mask_strategy=s_cg)
+    assert plan.engine == "ring"
+    inp = T(f, dtype)
+    out = torch.empty_like(inp)
+    cgef.filter_apply_(out, inp, plan)
+    fin = N(inp)
+    fp = O.Footprint(og, O.GAUSSIAN, 4.0, 1.6e6)
+    got = N(out)
+    for b in range(3):
+        ref = O.apply_banded(og, fp, s_o, [fin[b]])[0]
+        assert np.max(np.abs(got[b] - ref)) <= tol * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("nx,ny,ell", [(45, 30, 9000.0), (56, 24, 8000.0), (32, 32, 40000.0)])
+def test_ring_filter_cartesian_odd_sizes_and_multiwrap(nx, ny, ell):
+    # odd length (radices 3·3·5), a radix-7 length, and a radius larger than the box (runs that
+    # wrap the ring more than once)
+    xs, ys = cart_axes(nx, 1000.0), cart_axes(ny, 1000.0)
+    mask = blob_mask((ny, nx), 41)
+    og = oracle_grid_cart(xs, ys, periodic=(True, True), mask=mask)
+    g = cgef.StructuredGrid(cgef.CartesianGeometry(), xs, ys, mask=mask, periodic=(True, True))
+    f = smooth_random_field(g.field_shape, 19, 1)[0]
+    for ker, kind, alpha in ((cgef.GaussianKernel(6.0), O.GAUSSIAN, 6.0), (cgef.TopHatKernel(), O.TOPHAT, 0.0)):
+        plan = cgef.plan_filter(g, ker, ell, engine="ring")
+        out = torch.empty(g.field_shape, dtype=torch.float64, device=DEV)
+        cgef.filter_apply_(out, T(f), plan)
+        ref = O.filter_fields(og, [f], kind=kind, alpha=alpha, ell=ell)[0]
+        assert np.max(np.abs(N(out) - ref)) <= 1e-11 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("geom", ["cartesian", "sphere", "volume"])
+@pytest.mark.parametrize("strategy", ["zerofill", "deformable"])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_ring_pi(geom, strategy, dtype):
+    s_cg, s_o = STRATS[strategy]
+    if geom == "cartesian":
+        og, g = cart_case(True, True)
+        ker, kw = cgef.GaussianKernel(6.0), dict(kind=O.GAUSSIAN, alpha=6.0, ell=6000.0)
+        fields = smooth_random_field(g.field_shape, 20, 2)
+    elif geom == "sphere":
+        og, g, _, _ = sphere_case(True)
+        ker, kw = cgef.GaussianKernel(4.0), dict(kind=O.GAUSSIAN, alpha=4.0, ell=1.2e6)
+        fields = smooth_random_field(g.field_shape, 21, 2)
+    else:
+        og, g = volume_case(True, True)
+        ker, kw = cgef.TopHatKernel(), dict(kind=O.TOPHAT, alpha=0.0, ell=6.0, engine=O.ENGINE_ND3)
+        fields = smooth_random_field(g.field_shape, 22, 3)
+    fields = [f + 0.5 for f in fields]                         # a mean flow exercises the shift
+    ins = [T(f, dtype) for f in fields]
+    plan = cgef.plan_filter(g, ker, kw["ell"], mask_strategy=s_cg, engine="ring")
+    Pi = torch.empty(g.field_shape, dtype=dtype, device=DEV)
+    ws = cgef.PiWorkspace(g, (), dtype)
+    w = ins[2] if geom == "volume" else None
+    _, E = cgef.compute_pi_(Pi, ins[0], ins[1], w, g, ker, kw["ell"], workspace=ws, filter_plan=plan,
+                            return_energy=True)
+    ref, ub, vb, wb = O.compute_pi(og, *[N(a) for a in ins], strategy=s_o, **kw)
+    tol = 1e-10 if dtype == torch.float64 else 1e-6
+    assert rel_l2(N(Pi), ref) <= tol
+    assert rel_l2(N(ws.u_filt), ub) <= (1e-12 if dtype == torch.float64 else 1e-7)
+    Eo = O.energy(og, ub, vb, wb if geom == "volume" else None)
+    assert abs(E - Eo) <= 1e-12 * abs(Eo)
