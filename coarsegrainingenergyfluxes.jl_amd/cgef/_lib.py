@@ -21,7 +21,8 @@ ENGINE_AUTO, ENGINE_BANDED, ENGINE_SEPARABLE, ENGINE_BASELINE, ENGINE_RING = ran
 EXPORTED = (
     "cgef_version", "cgef_last_error", "cgef_grid_create", "cgef_grid_destroy", "cgef_active_area",
     "cgef_plan_create", "cgef_plan_destroy", "cgef_plan_query", "cgef_workspace_create",
-    "cgef_workspace_destroy", "cgef_filter_apply", "cgef_compute_pi", "cgef_derivative", "cgef_energy",
+    "cgef_workspace_destroy", "cgef_filter_apply", "cgef_compute_pi", "cgef_coarse_grain", "cgef_derivative",
+    "cgef_energy",
     "cgef_cumulative_energy", "cgef_profile_enable", "cgef_profile_read", "cgef_spectral_density",
 )
 
@@ -70,6 +71,9 @@ class PlanInfo(C.Structure):
         ("max_taps", C.c_int64),
         ("radius", C.c_double),
         ("active_taps", C.c_double),
+        ("ring_offsets", C.c_int64),
+        ("ring_modes", C.c_int64),
+        ("ring_pi_fields", C.c_int64),
     ]
 
 
@@ -103,6 +107,7 @@ def load():
         "cgef_workspace_destroy": [vp],
         "cgef_filter_apply": [vp, vp, vp, i64, i64, i32, vp],
         "cgef_compute_pi": [vp, vp, vp, vp, vp, vp, i64, i32, dbl, dp, vp, vp, vp, vp],
+        "cgef_coarse_grain": [C.POINTER(vp), i64, vp, vp, vp, vp, vp, i64, i32, dbl, dp, vp],
         "cgef_derivative": [vp, i32, vp, vp, i64, i32, vp],
         "cgef_energy": [vp, vp, vp, vp, i64, i32, dp, vp],
         "cgef_cumulative_energy": [C.POINTER(vp), i64, vp, vp, vp, vp, i64, i32, dp, vp],

@@ -3,14 +3,15 @@ snapshot (coarse_grain!) or a batch of snapshots computed batch-native on the GP
 (coarse_grain_batch!, the GPU driver of Pipeline.jl:519-563)."""
 from __future__ import annotations
 
+import ctypes as C
 from dataclasses import dataclass
 
 import numpy as np
 import torch
 
 from . import _lib
-from .diagnostics import (PiWorkspace, StrictSpectrum, NoSpectrum, compute_pi_, gate_spectrum, spectral_density_)
-from .filtering import ZeroFill, _check_field, batch_dims, plan_filter
+from .diagnostics import PiWorkspace, StrictSpectrum, NoSpectrum, gate_spectrum, spectral_density_
+from .filtering import ZeroFill, _check_field, _dtype_code, _stream, batch_dims, plan_filter
 from .grids import StructuredGrid
 from .kernels import TopHatKernel
 
@@ -34,6 +35,29 @@ def _allocate_result(grid, nscales, dtype, device):
                              np.zeros(nscales), np.zeros(nscales), np.zeros(nscales))
 
 
+def _run_coarse_grain(Pi, u, v, w, grid, plans, workspace, nbatch, rho0, E_out):
+    """One cgef_coarse_grain call: Π for every plan (scale) over `nbatch` slices, E(ℓ) into E_out
+    (slice-major, scale fastest)."""
+    dev = u.device.index if u.is_cuda else -1
+    for name, t in (("Π", Pi), ("u", u), ("v", v), ("w", w)):
+        if t is not None:
+            _check_field(t, name, grid, dev)
+    if grid.ndim == 2 and w is not None:
+        raise _lib.ArgumentError("the 2.5-D (u, v, w) layer Π is not provided by this backend; pass w=None")
+    if any(p.grid is not grid or p.device != dev for p in plans):
+        raise _lib.ArgumentError("every filter plan must belong to this grid and device")
+    if Pi.dtype != u.dtype or v.dtype != u.dtype or (w is not None and w.dtype != u.dtype):
+        raise _lib.ArgumentError("Π and the velocity components must share a dtype")
+    arr = (C.c_void_p * len(plans))(*[p.h for p in plans])
+    Eb = np.zeros(len(plans) * nbatch, dtype=np.float64)
+    _lib.check(_lib.load().cgef_coarse_grain(
+        arr, len(plans), workspace.h if workspace is not None else None, C.c_void_p(Pi.data_ptr()),
+        C.c_void_p(u.data_ptr()), C.c_void_p(v.data_ptr()), C.c_void_p(w.data_ptr()) if w is not None else None,
+        nbatch, _dtype_code(u), C.c_double(rho0), Eb.ctypes.data_as(C.POINTER(C.c_double)), _stream(dev)))
+    E_out[...] = Eb.reshape(np.shape(E_out))
+    return Pi
+
+
 def _fill_spectrum(g, Cv, k, policy):
     if isinstance(policy, NoSpectrum):
         g[:] = np.nan
@@ -51,11 +75,8 @@ def coarse_grain_(result: CoarseGrainResult, u, v, w, grid: StructuredGrid, *, s
     gate_spectrum(kernel, spectrum)
     plans = filter_plans if filter_plans is not None else [
         plan_filter(grid, kernel, float(s), mask_strategy=mask_strategy, backend=backend) for s in scales]
-    for s_idx, s in enumerate(scales):
-        result.scales[s_idx] = float(s)
-        _, E = compute_pi_(result.Pi[s_idx], u, v, w, grid, kernel, float(s), workspace=workspace,
-                           filter_plan=plans[s_idx], mask_strategy=mask_strategy, rho0=rho0, return_energy=True)
-        result.cumulative_energy[s_idx] = E
+    _run_coarse_grain(result.Pi, u, v, w, grid, plans, workspace, 1, rho0, result.cumulative_energy)
+    result.scales[:] = [float(s) for s in scales]
     result.wavenumber[:] = float(L) / result.scales
     _fill_spectrum(result.filtering_spectrum, result.cumulative_energy, result.wavenumber, spectrum)
     return result
@@ -107,14 +128,11 @@ def coarse_grain_batch_(batch: CoarseGrainBatchResult, u, v, w, grid: Structured
         raise _lib.DimensionMismatch("coarse_grain_batch! needs a batch-sized workspace")
     plans = filter_plans if filter_plans is not None else [
         plan_filter(grid, kernel, float(s), mask_strategy=mask_strategy, backend=backend) for s in scales]
-    tmp = torch.empty(bs + grid.field_shape, dtype=u.dtype, device=u.device)
-    nd = len(bs)
-    for s_idx, s in enumerate(scales):
-        batch.scales[..., s_idx] = float(s)
-        _, E = compute_pi_(tmp, u, v, w, grid, kernel, float(s), workspace=ws, filter_plan=plans[s_idx],
-                           mask_strategy=mask_strategy, rho0=rho0, return_energy=True)
-        batch.Pi.select(nd, s_idx).copy_(tmp)
-        batch.cumulative_energy[..., s_idx] = E
+    nb = int(np.prod(bs)) if bs else 1
+    E = np.zeros(nb * ns)
+    _run_coarse_grain(batch.Pi, u, v, w, grid, plans, ws, nb, rho0, E)
+    batch.cumulative_energy[...] = E.reshape(batch.cumulative_energy.shape)
+    batch.scales[...] = np.asarray([float(s) for s in scales])
     batch.wavenumber[...] = float(L) / batch.scales
     flat_c = batch.cumulative_energy.reshape(-1, ns)
     flat_k = batch.wavenumber.reshape(-1, ns)

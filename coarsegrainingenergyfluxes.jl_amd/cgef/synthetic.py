@@ -107,6 +107,61 @@ def sphere_turbulence(Nx=1440, Ny=720, R=6.371e6, seed=1, lmin=8, lmax=400, rms=
     return u * s, v * s, lam, phi
 
 
+def sphere_turbulence_batch(seeds, Nx=1440, Ny=720, R=6.371e6, lmin=8, lmax=400, rms=0.2):
+    """sphere_turbulence for several seeds, sharing the Legendre recursion (one pass for all)."""
+    seeds = list(seeds)
+    ns = len(seeds)
+    lam, phi = sphere_axes(Nx, Ny)
+    l = np.arange(lmax + 1, dtype=np.float64)
+    amp = np.zeros(lmax + 1)
+    sel = l >= lmin
+    amp[sel] = np.sqrt(l[sel] ** -3.0 / (l[sel] * (l[sel] + 1.0) * (2.0 * l[sel] + 1.0)))
+    coef = np.zeros((ns, lmax + 1, lmax + 1), dtype=np.complex128)
+    for q, sd in enumerate(seeds):
+        rng = np.random.Generator(np.random.PCG64(sd))
+        theta = rng.uniform(0.0, 2.0 * np.pi, size=(lmax + 1, lmax + 1))
+        for m in range(lmax + 1):
+            coef[q, m:, m] = amp[m:] * np.exp(1j * theta[m:, m])
+    h = 1e-5
+    xs = np.concatenate([np.sin(phi + h), np.sin(phi - h), np.sin(phi)])
+    # Legendre synthesis for all seeds at the three latitude sets at once
+    nx = len(xs)
+    fm = np.zeros((ns, lmax + 1, nx), dtype=np.complex128)
+    sx = np.sqrt(np.maximum(0.0, 1.0 - xs * xs))
+    pmm = np.ones(nx)
+    for m in range(lmax + 1):
+        if m > 0:
+            pmm = pmm * sx * np.sqrt((2.0 * m + 1.0) / (2.0 * m))
+        acc = coef[:, m, m][:, None] * pmm[None, :]
+        if m + 1 <= lmax:
+            p1 = xs * np.sqrt(2.0 * m + 3.0) * pmm
+            acc = acc + coef[:, m + 1, m][:, None] * p1[None, :]
+            pl2, pl1 = pmm, p1
+            for ll in range(m + 2, lmax + 1):
+                a = np.sqrt((4.0 * ll * ll - 1.0) / (ll * ll - m * m))
+                b = np.sqrt(((ll - 1.0) ** 2 - m * m) / (4.0 * (ll - 1.0) ** 2 - 1.0))
+                pl = a * (xs * pl1 - b * pl2)
+                acc = acc + coef[:, ll, m][:, None] * pl[None, :]
+                pl2, pl1 = pl1, pl
+        fm[:, m, :] = acc
+    mm = np.arange(lmax + 1)
+    ph = np.exp(1j * np.outer(mm, lam))
+    cphi = np.cos(phi)[:, None]
+    out = []
+    for q in range(ns):
+        f = fm[q]
+        psi_p = np.real(f[:, :Ny].T @ ph)
+        psi_m = np.real(f[:, Ny:2 * Ny].T @ ph)
+        dpsi_dphi = (psi_p - psi_m) / (2.0 * h)
+        dpsi_dlam = np.real((f[:, 2 * Ny:] * (1j * mm)[:, None]).T @ ph)
+        u = -dpsi_dphi / R
+        v = dpsi_dlam / (R * cphi)
+        wts = np.broadcast_to(cphi, u.shape)
+        s = rms / np.sqrt(np.sum((u ** 2 + v ** 2) * wts) / np.sum(wts))
+        out.append((u * s, v * s))
+    return [o[0] for o in out], [o[1] for o in out], lam, phi
+
+
 def sphere_land_mask(Nx=1440, Ny=720, R=6.371e6, seed=2, corr=500e3, land_fraction=0.3, lmax=100):
     rng = np.random.Generator(np.random.PCG64(seed))
     lam, phi = sphere_axes(Nx, Ny)

@@ -72,6 +72,7 @@ Grid::~Grid() {
         dfree(width_d[d]);
     }
     dfree(trig_d);
+    for (int d = 0; d < 3; d++) dfree(dcoef_d[d]);
 }
 
 void Plan::prof_begin(hipStream_t s) {
@@ -132,6 +133,56 @@ Workspace::~Workspace() {
     dfree(rout);
 }
 
+// Lagrange weights of the derivative at x of the quadratic through (a, b, c).
+static void lag3_weights(double x, double a, double b, double c, double *w) {
+    w[0] = (2 * x - b - c) / ((a - b) * (a - c));
+    w[1] = (2 * x - a - c) / ((b - a) * (b - c));
+    w[2] = (2 * x - a - b) / ((c - a) * (c - b));
+}
+
+// Stencil weights of the 3-node first derivative along one axis (Derivatives.jl:26-31 → the
+// FlowGeometries derivative with nodes = 3): which of the five stencils applies at a point depends
+// on the mask and is decided on the device; the weights depend only on the axis index.
+static std::vector<double> deriv_table(const Grid &g, int axis) {
+    const int64_t n = g.n[axis];
+    std::vector<double> tab((size_t)n * DCOEF_STRIDE, 0.0);
+    if (n < 2) return tab;
+    const bool per = g.periodic[axis] && n >= 3;
+    const std::vector<double> &x = g.coord[axis];
+    for (int64_t p = 0; p < n; p++) {
+        double xc[5];
+        bool ok[5];
+        for (int o = -2; o <= 2; o++) {
+            int64_t q = p + o;
+            double sh = 0.0;
+            ok[o + 2] = true;
+            if (q < 0 || q >= n) {
+                if (!per) {
+                    ok[o + 2] = false;
+                    xc[o + 2] = 0.0;
+                    continue;
+                }
+                sh = q < 0 ? -g.period[axis] : g.period[axis];
+                q = q < 0 ? q + n : q - n;
+            }
+            xc[o + 2] = x[q] + sh;
+        }
+        double *w = tab.data() + p * DCOEF_STRIDE;
+        if (ok[1] && ok[3]) lag3_weights(xc[2], xc[1], xc[2], xc[3], w);
+        if (ok[3] && ok[4]) lag3_weights(xc[2], xc[2], xc[3], xc[4], w + 3);
+        if (ok[0] && ok[1]) lag3_weights(xc[2], xc[0], xc[1], xc[2], w + 6);
+        if (ok[3]) {
+            w[10] = 1.0 / (xc[3] - xc[2]);
+            w[9] = -w[10];
+        }
+        if (ok[1]) {
+            w[12] = 1.0 / (xc[2] - xc[1]);
+            w[11] = -w[12];
+        }
+    }
+    return tab;
+}
+
 GeomArgs geom_args(const Grid &g) {
     GeomArgs a{};
     for (int d = 0; d < 3; d++) {
@@ -140,6 +191,7 @@ GeomArgs geom_args(const Grid &g) {
         a.period[d] = g.period[d];
         a.coord[d] = g.coord_d[d];
         a.width[d] = g.width_d[d];
+        a.dcoef[d] = g.dcoef_d[d];
     }
     a.npts = g.npts;
     a.R = g.R;
@@ -186,7 +238,7 @@ static void ws_reserve(Workspace &w, int64_t nbatch, int dtype) {
     CGEF_HIP(hipMalloc(&w.ubar, (size_t)P * nbatch * 3 * sizeof(double)));
     CGEF_HIP(hipMalloc(&w.tau, (size_t)P * nbatch * 6 * es));
     CGEF_HIP(hipMalloc(&w.cmean, (size_t)nbatch * 3 * sizeof(double)));
-    const int64_t nblk = (P + 255) / 256;
+    const int64_t nblk = (g.n[0] + 255) / 256 * g.n[1] * g.n[2];  // strain_blocks
     w.npartials = nbatch * std::max<int64_t>(nblk, 3 * 1024);
     CGEF_HIP(hipMalloc(&w.partials, (size_t)w.npartials * sizeof(double)));
     CGEF_HIP(hipMalloc(&w.E_dev, (size_t)nbatch * sizeof(double)));
@@ -312,7 +364,12 @@ static void ring_filter_planes(Plan &p, Workspace &w, int64_t ncol, hipStream_t 
     ring_inverse(p.ring, w.rout, w.rxy, ncol, s);
 }
 
-static int ring_nfields(const Grid &g, bool masked) { return (g.ndim == 2 && g.geometry == CGEF_CARTESIAN ? 5 : 9) + (masked ? 1 : 0); }
+// Planes the ring engine filters per slice for Π: the frame velocity components and their products,
+// plus the mask when a Deformable plan on a masked grid needs its filtered value as the denominator.
+static int ring_withmask(const Grid &g, int strategy) { return g.has_mask && strategy == CGEF_DEFORMABLE ? 1 : 0; }
+static int ring_nfields(const Grid &g, int withmask) {
+    return (g.ndim == 2 && g.geometry == CGEF_CARTESIAN ? 5 : 9) + withmask;
+}
 
 // filter nsl plain slices with the ring engine
 static void ring_filter_apply(Plan &p, Workspace &w, int dtype, const void *in, void *out, int64_t nsl, hipStream_t s) {
@@ -454,6 +511,7 @@ cgef_status cgef_grid_create(const cgef_grid_desc *desc, int32_t device, cgef_gr
             for (int d = 0; d < 3; d++) {
                 g->coord_d[d] = upload(g->coord[d]);
                 g->width_d[d] = upload(g->width[d]);
+                g->dcoef_d[d] = upload(deriv_table(*g, d));
             }
             if (g->geometry == CGEF_SPHERICAL) {
                 const int64_t Nx = g->n[0], Ny = g->n[1];
@@ -532,8 +590,8 @@ cgef_status cgef_plan_create(cgef_grid *grid, const cgef_kernel_desc *kernel, do
                 p->engine = CGEF_ENGINE_SEPARABLE;
                 build_separable(*p);
             } else {
-                build_banded(*p);
-                if (eng == CGEF_ENGINE_AUTO) eng = ring_supported(*p) ? CGEF_ENGINE_RING : CGEF_ENGINE_BANDED;
+                if (eng == CGEF_ENGINE_AUTO) eng = ring_grid_ok(*g) ? CGEF_ENGINE_RING : CGEF_ENGINE_BANDED;
+                build_banded(*p, eng != CGEF_ENGINE_RING);
                 if (eng == CGEF_ENGINE_RING && !ring_supported(*p))
                     fail(CGEF_ERR_UNSUPPORTED,
                          "the ring engine needs a periodic, uniform x axis whose length factors into 2,3,5,7,11,13");
@@ -590,6 +648,9 @@ cgef_status cgef_plan_query(const cgef_plan *plan, cgef_plan_info *info) {
         info->max_taps = p->max_taps;
         info->active_taps = p->active_taps;
         info->radius = p->rad;
+        info->ring_offsets = p->ring.ok ? p->ring.args.noff : 0;
+        info->ring_modes = p->ring.ok ? p->ring.mh : 0;
+        info->ring_pi_fields = p->ring.ok ? ring_nfields(*p->grid, ring_withmask(*p->grid, p->strategy)) : 0;
     });
 }
 
@@ -671,23 +732,100 @@ cgef_status cgef_compute_pi(cgef_plan *plan, cgef_workspace *ws, void *Pi, const
             CGEF_HIP(hipGetLastError());
             if (E_out) fetch_energy(W, nbatch, strain_blocks(G), g.active_area, E_out, s);
         } else if (p->engine == CGEF_ENGINE_RING) {
-            const int nf = ring_nfields(g, g.has_mask);
+            const int wm = ring_withmask(g, p->strategy), nf = ring_nfields(g, wm);
             ring_reserve(W, *p, nf * nbatch);
-            launch_mean(G, dtype, u, v, w, nbatch, W.partials, W.npartials, W.cmean, s);
-            ring_prep(G, dtype, u, v, w, W.cmean, nf, g.has_mask ? 1 : 0, W.rxy, nbatch, s);
+            ring_prep(G, dtype, u, v, w, nf, wm, W.rxy, nbatch, s);
             ring_filter_planes(*p, W, nf * nbatch, s);
-            ring_epilogue(G, dtype, W.rxy, p->D64_row_d, W.cmean, nf, g.has_mask ? 1 : 0, p->strategy, W.ubar, W.tau,
-                          nbatch, s);
+            ring_epilogue(G, dtype, W.rxy, p->D64_row_d, nf, wm, p->strategy, W.ubar, W.tau, ubar, vbar, wbar, nbatch, s);
             launch_strain_pi(G, dtype, W.ubar, W.tau, Pi, rho0, nbatch, W.partials, W.npartials, s);
             CGEF_HIP(hipGetLastError());
             if (E_out) fetch_energy(W, nbatch, strain_blocks(G), g.active_area, E_out, s);
         } else {
             compute_pi_unfused(*p, W, Pi, u, v, w, nbatch, dtype, rho0, E_out, s);
         }
-        if (ubar) export_planes(g, W.ubar, 0, ubar, nbatch, dtype, s);
-        if (vbar) export_planes(g, W.ubar, 1, vbar, nbatch, dtype, s);
-        if (wbar) export_planes(g, W.ubar, 2, wbar, nbatch, dtype, s);
+        if (p->engine != CGEF_ENGINE_RING) {  // the ring epilogue writes them itself
+            if (ubar) export_planes(g, W.ubar, 0, ubar, nbatch, dtype, s);
+            if (vbar) export_planes(g, W.ubar, 1, vbar, nbatch, dtype, s);
+            if (wbar) export_planes(g, W.ubar, 2, wbar, nbatch, dtype, s);
+        }
         CGEF_HIP(hipGetLastError());
+    });
+}
+
+cgef_status cgef_coarse_grain(cgef_plan *const *plans, int64_t nscales, cgef_workspace *ws, void *Pi, const void *u,
+                              const void *v, const void *w, int64_t nbatch, int32_t dtype, double rho0, double *E_out,
+                              void *stream) {
+    return guard([&] {
+        if (!plans || !Pi || !u || !v) fail(CGEF_ERR_ARGUMENT, "null plans or field pointer");
+        if (nscales < 0 || nbatch < 1) fail(CGEF_ERR_DIMENSION, "bad nscales / nbatch");
+        check_dtype(dtype);
+        if (nscales == 0) return;
+        Grid &g = *reinterpret_cast<Plan *>(plans[0])->grid;
+        bool all_ring = true;
+        for (int64_t q = 0; q < nscales; q++) {
+            Plan *p = reinterpret_cast<Plan *>(plans[q]);
+            if (!p || p->grid != &g) fail(CGEF_ERR_ARGUMENT, "every plan must belong to the same grid");
+            all_ring = all_ring && p->engine == CGEF_ENGINE_RING;
+        }
+        if (g.ndim == 3 && !w) fail(CGEF_ERR_ARGUMENT, "3-D Π needs the vertical velocity w");
+        if (g.ndim == 2 && w)
+            fail(CGEF_ERR_UNSUPPORTED, "the 2.5-D (u, v, w) layer Π is not provided by this backend; pass w = NULL");
+        CGEF_HIP(hipSetDevice(g.device));
+        hipStream_t s = S(stream);
+        Workspace &W = resolve_ws(g, ws, nbatch, dtype);
+        const GeomArgs G = geom_args(g);
+        const int64_t P = g.npts;
+        const size_t es = esize(dtype);
+        double *Eall = (double *)ws_fbuf(W, (size_t)nscales * nbatch * sizeof(double));
+        const int64_t nblk = strain_blocks(G);
+        if (all_ring) {
+            // one forward transform for every scale (Diagnostics.jl:283-301's analyze-once idea, here
+            // for the real-space footprint): only the contraction and the inverse depend on ℓ
+            Plan &p0 = *reinterpret_cast<Plan *>(plans[0]);
+            int wm = 0;
+            for (int64_t q = 0; q < nscales; q++) wm |= ring_withmask(g, reinterpret_cast<Plan *>(plans[q])->strategy);
+            const int nf = ring_nfields(g, wm);
+            ring_reserve(W, p0, nf * nbatch);
+            ring_prep(G, dtype, u, v, w, nf, wm, W.rxy, nbatch, s);
+            ring_forward(p0.ring, W.rxy, W.rxf, nf * nbatch, s);
+            for (int64_t q = 0; q < nscales; q++) {
+                Plan &p = *reinterpret_cast<Plan *>(plans[q]);
+                p.prof_begin(s);
+                ring_contract(p.ring, W.rxf, W.rout, nf * nbatch, s);
+                p.prof_end(s);
+                ring_inverse(p.ring, W.rout, W.rxy, nf * nbatch, s);
+                ring_epilogue(G, dtype, W.rxy, p.D64_row_d, nf, wm, p.strategy, W.ubar, W.tau, nullptr, nullptr, nullptr,
+                              nbatch, s);
+                launch_strain_pi(G, dtype, W.ubar, W.tau, (char *)Pi + (size_t)q * P * es, rho0, nbatch, W.partials,
+                                 W.npartials, s, nscales * P);
+                if (E_out) launch_energy_final(nbatch, nblk, W.partials, g.active_area, Eall + q * nbatch, s);
+            }
+            CGEF_HIP(hipGetLastError());
+        } else {
+            // other engines: one compute_Π! per scale, written into the scale's strided slots
+            for (int64_t q = 0; q < nscales; q++) {
+                std::vector<double> Eq(nbatch);
+                void *pis = nullptr;
+                CGEF_HIP(hipMallocAsync(&pis, (size_t)nbatch * P * es, s));
+                cgef_status st = cgef_compute_pi(plans[q], ws, pis, u, v, w, nbatch, dtype, rho0,
+                                                 E_out ? Eq.data() : nullptr, nullptr, nullptr, nullptr, stream);
+                if (st != CGEF_OK) fail(st, g_last_error);
+                CGEF_HIP(hipMemcpy2DAsync((char *)Pi + (size_t)q * P * es, (size_t)nscales * P * es, pis, (size_t)P * es,
+                                          (size_t)P * es, (size_t)nbatch, hipMemcpyDeviceToDevice, s));
+                CGEF_HIP(hipFreeAsync(pis, s));
+                if (E_out)
+                    for (int64_t b = 0; b < nbatch; b++) E_out[q + nscales * b] = Eq[b];
+            }
+            return;
+        }
+        if (E_out) {
+            std::vector<double> Eh((size_t)nscales * nbatch);
+            CGEF_HIP(hipMemcpyAsync(Eh.data(), Eall, Eh.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+            CGEF_HIP(hipStreamSynchronize(s));
+            if (!(g.active_area > 0)) fail(CGEF_ERR_ARGUMENT, "grid has no active cells (all masked out)");
+            for (int64_t q = 0; q < nscales; q++)
+                for (int64_t b = 0; b < nbatch; b++) E_out[q + nscales * b] = Eh[q * nbatch + b];
+        }
     });
 }
 

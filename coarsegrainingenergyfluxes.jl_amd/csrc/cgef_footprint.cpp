@@ -8,8 +8,10 @@
 // dz, then dy, then di ascending, weight kernel_weight(d)·measure. Runs let a kernel walk the
 // footprint with wave-uniform loop bounds and weights, and let each source row be staged once.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <numeric>
+#include <thread>
 
 #include "cgef_internal.hpp"
 
@@ -33,10 +35,17 @@ static double haversine(double R, double phi1, double dlam, double phi2) {
 static int64_t iceil(double v) { return (int64_t)std::ceil(v); }
 
 namespace {
+// Runs and weights are collected per band (woff relative to the band), so bands can be built in
+// parallel and concatenated in band order afterwards.
 struct Builder {
     Plan &p;
     std::vector<std::vector<Run>> band_runs;
+    std::vector<std::vector<double>> band_w;
     explicit Builder(Plan &pl) : p(pl) {}
+    void resize(int nb) {
+        band_runs.assign(nb, {});
+        band_w.assign(nb, {});
+    }
 
     // Appends the run of columns in [-lim_lo, lim_hi] whose distance passes d <= rad. dist(di) must
     // grow with |di|; the run is found by scanning outward from di = 0 on both sides.
@@ -51,14 +60,33 @@ struct Builder {
         r.dz = dz;
         r.lo = (int32_t)lo;
         r.hi = (int32_t)hi;
-        r.woff = (int32_t)p.w64.size();
-        for (int64_t di = lo; di <= hi; di++) p.w64.push_back(weight(dist(di)));
+        std::vector<double> &w = band_w[band];
+        r.woff = (int32_t)w.size();
+        for (int64_t di = lo; di <= hi; di++) w.push_back(weight(dist(di)));
         band_runs[band].push_back(r);
     }
 };
+
+// Runs fn(band) for band = 0..nb-1 on the host's cores (bands are independent).
+template <class F>
+void parallel_bands(int nb, F fn) {
+    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const int nthreads = (int)std::min<unsigned>(hw, (unsigned)std::max(nb, 1));
+    if (nthreads <= 1 || nb < 8) {
+        for (int b = 0; b < nb; b++) fn(b);
+        return;
+    }
+    std::atomic<int> next{0};
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nthreads; t++)
+        pool.emplace_back([&] {
+            for (int b = next++; b < nb; b = next++) fn(b);
+        });
+    for (auto &th : pool) th.join();
+}
 }  // namespace
 
-void build_banded(Plan &p) {
+void build_banded(Plan &p, bool need_fp32) {
     const Grid &g = *p.grid;
     const int64_t Nx = g.n[0], Ny = g.n[1], Nz = g.n[2];
     Builder B(p);
@@ -72,10 +100,11 @@ void build_banded(Plan &p) {
         const int64_t dj_lim = dphi > 0 ? iceil(p.rad / (R * dphi)) : 0;
         const int64_t half = Nx / 2;
         p.nbands = (int)Ny;
-        B.band_runs.assign(Ny, {});
+        B.resize((int)Ny);
         std::vector<double> area(Ny);
         for (int64_t jj = 0; jj < Ny; jj++) area[jj] = g.cell_area(0, jj, 0);
-        for (int64_t j = 0; j < Ny; j++) {
+        parallel_bands((int)Ny, [&](int jb) {
+            const int64_t j = jb;
             const double phi = g.coord[1][j];
             // A row whose support reaches the antipodal column covers the whole ring; the window is
             // then -Nx÷2 .. Nx÷2-1 on an even ring. A row that does not reach it is unaffected by
@@ -91,7 +120,7 @@ void build_banded(Plan &p) {
                 auto wt = [&](double d) { return kernel_weight(p.kind, p.alpha, d, p.scale) * A; };
                 B.add_run((int)j, (int)ddj, 0, lim_lo, lim_hi, dist, wt);
             }
-        }
+        });
     } else if (g.ndim == 2) {
         // Filtering.jl:653-673 — Cartesian: one translation-invariant band, disk d <= rad.
         if (!g.uniform[0] || !g.uniform[1])
@@ -100,7 +129,7 @@ void build_banded(Plan &p) {
         const double A = g.cell_area(0, 0, 0);
         const int64_t di_lim = dx > 0 ? iceil(p.rad / dx) : 0, dj_lim = dy > 0 ? iceil(p.rad / dy) : 0;
         p.nbands = 1;
-        B.band_runs.assign(1, {});
+        B.resize(1);
         for (int64_t ddj = -dj_lim; ddj <= dj_lim; ddj++) {
             const double b = (double)ddj * dy;
             auto dist = [&](int64_t di) {
@@ -125,8 +154,8 @@ void build_banded(Plan &p) {
         const bool all_uniform = g.uniform[2] != 0;
         const double A = g.cell_area(0, 0, 0);
         p.nbands = all_uniform ? 1 : (int)Nz;
-        B.band_runs.assign(p.nbands, {});
-        for (int band = 0; band < p.nbands; band++) {
+        B.resize(p.nbands);
+        parallel_bands(p.nbands, [&](int band) {
             const int64_t k = band;  // target level (uniform: any level, offsets only)
             for (int64_t ddk = -lz; ddk <= lz; ddk++) {
                 double ddz;
@@ -154,25 +183,38 @@ void build_banded(Plan &p) {
                     B.add_run(band, (int)ddj, (int)ddk, lx, lx, dist, wt);
                 }
             }
-        }
+        });
     }
-    if (p.w64.size() >= (size_t)INT32_MAX) fail(CGEF_ERR_UNSUPPORTED, "footprint table exceeds 2^31 entries");
 
-    // flatten runs, band pointers
+    // flatten runs and weights in band order (woff becomes global), band pointers
+    size_t total = 0;
+    for (int b = 0; b < p.nbands; b++) total += B.band_w[b].size();
+    if (total >= (size_t)INT32_MAX) fail(CGEF_ERR_UNSUPPORTED, "footprint table exceeds 2^31 entries");
+    p.w64.clear();
+    p.w64.reserve(total);
     p.run_ptr.assign(p.nbands + 1, 0);
     p.runs.clear();
     for (int b = 0; b < p.nbands; b++) {
         p.run_ptr[b] = (int32_t)p.runs.size();
-        for (auto &r : B.band_runs[b]) p.runs.push_back(r);
+        const int32_t base = (int32_t)p.w64.size();
+        for (Run r : B.band_runs[b]) {
+            r.woff += base;
+            p.runs.push_back(r);
+        }
+        p.w64.insert(p.w64.end(), B.band_w[b].begin(), B.band_w[b].end());
+        std::vector<double>().swap(B.band_w[b]);
     }
     p.run_ptr[p.nbands] = (int32_t)p.runs.size();
     p.nentries = (int64_t)p.w64.size();
 
     // FP32 weights scaled per band to O(1) magnitude; the FP32 path normalizes with sums of the
-    // rounded weights so that it filters exactly the operator its weights represent.
-    p.w32.resize(p.w64.size());
-    p.w32d.resize(p.w64.size());
-    for (int b = 0; b < p.nbands; b++) {
+    // rounded weights so that it filters exactly the operator its weights represent. (The ring
+    // engine works in FP64 only and skips them.)
+    if (need_fp32) {
+        p.w32.resize(p.w64.size());
+        p.w32d.resize(p.w64.size());
+    }
+    for (int b = 0; b < p.nbands && need_fp32; b++) {
         double tot = 0.0;
         for (int32_t q = p.run_ptr[b]; q < p.run_ptr[b + 1]; q++) {
             const Run &r = p.runs[q];
@@ -206,10 +248,9 @@ void build_banded(Plan &p) {
             const int64_t jj = j + ru.dy, kk = k + ru.dz;
             if ((jj < 0 || jj >= Ny) && !g.periodic[1]) continue;
             if ((kk < 0 || kk >= Nz) && !(g.ndim == 3 && g.periodic[2])) continue;
-            for (int32_t e = 0; e <= ru.hi - ru.lo; e++) {
-                d64 += p.w64[ru.woff + e];
-                d32 += p.w32d[ru.woff + e];
-            }
+            for (int32_t e = 0; e <= ru.hi - ru.lo; e++) d64 += p.w64[ru.woff + e];
+            if (need_fp32)
+                for (int32_t e = 0; e <= ru.hi - ru.lo; e++) d32 += p.w32d[ru.woff + e];
             taps += ru.hi - ru.lo + 1;
         }
         p.D64_row[r] = d64;

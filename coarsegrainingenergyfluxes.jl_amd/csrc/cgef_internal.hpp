@@ -31,6 +31,11 @@ struct alignas(32) Run {
 
 struct Workspace;
 
+// First-derivative stencil weights per axis index p (DCOEF_STRIDE doubles): [0,3) centred nodes
+// p−1, p, p+1; [3,6) forward p, p+1, p+2; [6,9) backward p−2, p−1, p; [9,11) two-node forward p, p+1;
+// [11,13) two-node backward p−1, p. Periodic axes (n ≥ 3) wrap index and coordinate.
+constexpr int DCOEF_STRIDE = 16;
+
 // Ring engine plan (see cgef_ring.hip): mixed-radix factorization of the row length, the (dz, dy)
 // offsets of the footprint and the Fourier weights Ŵ[m][offset][band].
 constexpr int RING_MAX_STAGES = 24;
@@ -39,10 +44,14 @@ struct RingFft {
     int n = 0;
     int nstage = 0;
     int radix[RING_MAX_STAGES] = {};
+    int twoff[RING_MAX_STAGES] = {};  // offset of stage st in the per-stage twiddle table
+    int nseq = 0;                      // complex sequences per block of the register-staged row FFT
+                                       // (0: not supported for this length, the LDS ping-pong FFT serves)
 };
 
 struct RingArgs {
     int64_t ny = 1, nz = 1, nrows = 1;
+    int64_t mh = 0;                    // Fourier modes kept (N/2 + 1)
     int32_t py = 0, pz = 0;
     int32_t noff = 0, nbands = 0;
     int32_t ndz = 0;
@@ -59,6 +68,8 @@ struct RingPlan {
     RingFft fft;
     int64_t mh = 0;                    // N/2 + 1 Fourier modes kept
     double2 *tw_d = nullptr;           // exp(−2πik/N), k < N
+    double2 *tws_d = nullptr;          // per stage (radix R after Ns points): exp(−2πi·r·k/(Ns·R)) at
+                                       // twoff + k·(R−1) + r−1, 0 ≤ k < Ns, 1 ≤ r < R
     double *what_d = nullptr;
     int32_t *dz_val_d = nullptr, *dz_off0_d = nullptr, *off_dy_d = nullptr;
     RingArgs args;
@@ -81,6 +92,7 @@ struct Grid {
     double *coord_d[3] = {nullptr, nullptr, nullptr};
     double *width_d[3] = {nullptr, nullptr, nullptr};
     double *trig_d = nullptr;     // sphere: [sinλ(Nx) cosλ(Nx) sinφ(Ny) cosφ(Ny)]
+    double *dcoef_d[3] = {nullptr, nullptr, nullptr};  // derivative stencil weights, DCOEF_STRIDE per index
     std::atomic<int> refs{1};
     std::mutex ws_mu;
     Workspace *internal_ws = nullptr;
@@ -155,7 +167,8 @@ struct Workspace {
 };
 
 // Footprint construction (host, FP64), cgef_footprint.cpp
-void build_banded(Plan &p);
+void build_banded(Plan &p, bool need_fp32);
+bool ring_grid_ok(const Grid &g);
 void build_separable(Plan &p);
 
 }  // namespace cgef

@@ -380,87 +380,103 @@ void launch_pi_filter(const GeomArgs &G, const BandArgs &B, int dtype, bool mask
 
 // ------------------------------------------------------------ derivatives, strain, Π, energy ----
 
-__device__ __forceinline__ double lag3(double x, double a, double b, double c, double fa, double fb, double fc) {
-    return (2 * x - b - c) / ((a - b) * (a - c)) * fa + (2 * x - a - c) / ((b - a) * (b - c)) * fb +
-           (2 * x - a - b) / ((c - a) * (c - b)) * fc;
+// First-derivative stencil along `axis` at linear index t (axis index p, index stride `stride`):
+// centred 3-node stencil inside a run of active cells, the window shifted inward at a domain or mask
+// edge, two nodes for a run of two, none for an isolated cell (Derivatives.jl:26-31 → FlowGeometries
+// derivative!, nodes = 3, ReduceInRun). Weights come from the grid's per-index table; only the choice
+// of stencil depends on the mask. The derivative is c0·f[q0] + c1·f[q1] + c2·f[q2].
+struct Stencil {
+    int64_t q0, q1, q2;
+    double c0, c1, c2;
+};
+
+__device__ __forceinline__ Stencil stencil_axis(const GeomArgs &G, int axis, int64_t t, int64_t p, int64_t stride) {
+    Stencil S{t, t, t, 0.0, 0.0, 0.0};
+    const int64_t n = G.n[axis];
+    if (n < 2) return S;
+    const bool per = G.periodic[axis] && n >= 3;
+    auto nb = [&](int o, int64_t &ix) {  // neighbour p+o: in range (or wrapped) and active
+        int64_t q = p + o;
+        if (q < 0 || q >= n) {
+            if (!per) return false;
+            q = q < 0 ? q + n : q - n;
+        }
+        ix = t + (q - p) * stride;
+        return G.mask[ix] != 0;
+    };
+    int64_t im1 = t, ip1 = t, im2 = t, ip2 = t;
+    const bool okm1 = nb(-1, im1), okp1 = nb(1, ip1);
+    const double *w = G.dcoef[axis] + p * DCOEF_STRIDE;
+    if (okm1 && okp1) return Stencil{im1, t, ip1, w[0], w[1], w[2]};
+    if (okp1 && nb(2, ip2)) return Stencil{t, ip1, ip2, w[3], w[4], w[5]};
+    if (okm1 && nb(-2, im2)) return Stencil{im2, im1, t, w[6], w[7], w[8]};
+    if (okp1) return Stencil{t, ip1, t, w[9], w[10], 0.0};
+    if (okm1) return Stencil{im1, t, t, w[11], w[12], 0.0};
+    return S;
 }
 
-// First derivative along `axis` of f at linear index t w.r.t. the axis coordinate: centred 3-node
-// stencil inside a run of active cells, the window shifted inward at a domain or mask edge, two nodes
-// for a run of two, 0 for an isolated cell (Derivatives.jl:26-31 → FlowGeometries derivative!,
-// nodes = 3, ReduceInRun). Periodic axes wrap index and coordinate.
 template <typename F>
-__device__ double deriv_axis(const GeomArgs &G, int axis, int64_t t, F f) {
-    const int64_t n = G.n[axis];
-    if (n < 2) return 0.0;
-    const int64_t stride = axis == 0 ? 1 : (axis == 1 ? G.n[0] : G.n[0] * G.n[1]);
-    const int64_t p = (t / stride) % n;
-    const int64_t base = t - p * stride;
-    const double *x = G.coord[axis];
-    bool ok[5];
-    int64_t ix[5];
-    double xc[5];
-#pragma unroll
-    for (int o = -2; o <= 2; o++) {
-        int64_t qq = p + o;
-        double sh = 0.0;
-        bool valid = true;
-        if (qq < 0 || qq >= n) {
-            if (!G.periodic[axis] || n < 3) valid = false;
-            else {
-                sh = qq < 0 ? -G.period[axis] : G.period[axis];
-                qq = wrapi(qq, n);
-            }
-        }
-        ix[o + 2] = valid ? base + qq * stride : t;
-        ok[o + 2] = valid && G.mask[ix[o + 2]] != 0;
-        xc[o + 2] = valid ? x[qq] + sh : 0.0;
-    }
-    if (ok[1] && ok[3]) return lag3(xc[2], xc[1], xc[2], xc[3], f(ix[1]), f(t), f(ix[3]));
-    if (ok[3] && ok[4]) return lag3(xc[2], xc[2], xc[3], xc[4], f(t), f(ix[3]), f(ix[4]));
-    if (ok[1] && ok[0]) return lag3(xc[2], xc[0], xc[1], xc[2], f(ix[0]), f(ix[1]), f(t));
-    if (ok[3]) return (f(ix[3]) - f(t)) / (xc[3] - xc[2]);
-    if (ok[1]) return (f(t) - f(ix[1])) / (xc[2] - xc[1]);
-    return 0.0;
+__device__ __forceinline__ double apply_stencil(const Stencil &S, F f) {
+    return S.c0 * f(S.q0) + S.c1 * f(S.q1) + S.c2 * f(S.q2);
 }
+
+// Launch shape shared by the per-point kernels below: blockIdx.x = row · nchunk + chunk, where a row is
+// one (j, k) line along x and a chunk covers 256 consecutive x indices, so no lane divides by Nx.
+struct RowChunk {
+    int64_t i, j, k, r, t;
+};
+
+__device__ __forceinline__ RowChunk row_chunk(const GeomArgs &G, int64_t nchunk) {
+    RowChunk c;
+    c.r = (int64_t)blockIdx.x / nchunk;
+    c.i = ((int64_t)blockIdx.x - c.r * nchunk) * 256 + threadIdx.x;
+    c.j = c.r % G.n[1];
+    c.k = c.r / G.n[1];
+    c.t = c.r * G.n[0] + c.i;
+    return c;
+}
+
+int64_t row_chunks(const GeomArgs &G) { return (G.n[0] + 255) / 256; }
 
 // Strain S̄ (with the spherical curvature terms, Diagnostics.jl:652-669), contraction
 // (Diagnostics.jl:460-466, 683-698, 848-855), Π = −ρ₀ S̄:τ, and per-block partial energies
 // ½ |ū|² A (Diagnostics.jl:1125-1134; the radial component is excluded on the sphere).
 template <typename T, int MODE>
 __global__ __launch_bounds__(256) void k_strain_pi(GeomArgs G, const double *__restrict__ ubar, const T *__restrict__ tau,
-                                                   T *__restrict__ Pi, double rho0, double *partials, int64_t nblk) {
+                                                   T *__restrict__ Pi, double rho0, double *partials, int64_t nblk,
+                                                   int64_t pi_stride, int64_t nchunk) {
     __shared__ double sh[4];
     const int64_t b = blockIdx.y;
-    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    const int64_t P = G.npts;
+    const RowChunk c = row_chunk(G, nchunk);
+    const int64_t t = c.t, P = G.npts, Nx = G.n[0], Ny = G.n[1];
     double e = 0.0;
-    if (t < P) {
+    if (c.i < Nx) {
         const double *U = ubar + b * 3 * P, *V = U + P, *W = U + 2 * P;
         const T *tb = tau + b * 6 * P;
         double out = 0.0;
         if (G.mask[t]) {
-            const int64_t Nx = G.n[0], Ny = G.n[1];
-            const int64_t i = t % Nx, j = (t / Nx) % Ny, k = t / (Nx * Ny);
             auto fu = [&](int64_t q) { return U[q]; };
             auto fv = [&](int64_t q) { return V[q]; };
+            const Stencil sx = stencil_axis(G, 0, t, c.i, 1);
+            const Stencil sy = stencil_axis(G, 1, t, c.j, Nx);
             if (MODE == 2) {
                 auto fw = [&](int64_t q) { return W[q]; };
-                const double Sxx = deriv_axis(G, 0, t, fu), Syy = deriv_axis(G, 1, t, fv), Szz = deriv_axis(G, 2, t, fw);
-                const double Sxy = 0.5 * (deriv_axis(G, 1, t, fu) + deriv_axis(G, 0, t, fv));
-                const double Sxz = 0.5 * (deriv_axis(G, 2, t, fu) + deriv_axis(G, 0, t, fw));
-                const double Syz = 0.5 * (deriv_axis(G, 2, t, fv) + deriv_axis(G, 1, t, fw));
-                const double c = Sxx * (double)tb[t] + 2.0 * Sxy * (double)tb[P + t] + Syy * (double)tb[3 * P + t] +
-                                 2.0 * Sxz * (double)tb[2 * P + t] + 2.0 * Syz * (double)tb[4 * P + t] +
-                                 Szz * (double)tb[5 * P + t];
-                out = -c;
-                e = 0.5 * (U[t] * U[t] + V[t] * V[t] + W[t] * W[t]) * cell_area(G, i, j, k);
+                const Stencil sz = stencil_axis(G, 2, t, c.k, Nx * Ny);
+                const double Sxx = apply_stencil(sx, fu), Syy = apply_stencil(sy, fv), Szz = apply_stencil(sz, fw);
+                const double Sxy = 0.5 * (apply_stencil(sy, fu) + apply_stencil(sx, fv));
+                const double Sxz = 0.5 * (apply_stencil(sz, fu) + apply_stencil(sx, fw));
+                const double Syz = 0.5 * (apply_stencil(sz, fv) + apply_stencil(sy, fw));
+                const double cc = Sxx * (double)tb[t] + 2.0 * Sxy * (double)tb[P + t] + Syy * (double)tb[3 * P + t] +
+                                  2.0 * Sxz * (double)tb[2 * P + t] + 2.0 * Syz * (double)tb[4 * P + t] +
+                                  Szz * (double)tb[5 * P + t];
+                out = -cc;
+                e = 0.5 * (U[t] * U[t] + V[t] * V[t] + W[t] * W[t]) * cell_area(G, c.i, c.j, c.k);
             } else {
-                double dudx = deriv_axis(G, 0, t, fu), dudy = deriv_axis(G, 1, t, fu);
-                double dvdx = deriv_axis(G, 0, t, fv), dvdy = deriv_axis(G, 1, t, fv);
+                double dudx = apply_stencil(sx, fu), dudy = apply_stencil(sy, fu);
+                double dvdx = apply_stencil(sx, fv), dvdy = apply_stencil(sy, fv);
                 double Sxx, Sxy, Syy;
                 if (MODE == 1) {
-                    const double sp = G.trig[2 * Nx + j], cp = G.trig[2 * Nx + Ny + j];
+                    const double sp = G.trig[2 * Nx + c.j], cp = G.trig[2 * Nx + Ny + c.j];
                     const double hx = (fabs(cp) < sqrt(DBL_EPSILON)) ? 0.0 : 1.0 / (G.R * cp);
                     dudx *= hx;
                     dvdx *= hx;
@@ -475,12 +491,12 @@ __global__ __launch_bounds__(256) void k_strain_pi(GeomArgs G, const double *__r
                     Syy = dvdy;
                     Sxy = 0.5 * (dudy + dvdx);
                 }
-                const double c = Sxx * (double)tb[t] + 2.0 * Sxy * (double)tb[P + t] + Syy * (double)tb[2 * P + t];
-                out = -c;
-                e = 0.5 * (U[t] * U[t] + V[t] * V[t]) * cell_area(G, i, j, k);
+                const double cc = Sxx * (double)tb[t] + 2.0 * Sxy * (double)tb[P + t] + Syy * (double)tb[2 * P + t];
+                out = -cc;
+                e = 0.5 * (U[t] * U[t] + V[t] * V[t]) * cell_area(G, c.i, c.j, c.k);
             }
         }
-        Pi[b * P + t] = (T)(rho0 * out);
+        Pi[b * pi_stride + t] = (T)(rho0 * out);
     }
     if (partials) {
         const double s = block_sum<256>(e, sh);
@@ -488,14 +504,16 @@ __global__ __launch_bounds__(256) void k_strain_pi(GeomArgs G, const double *__r
     }
 }
 
-int64_t strain_blocks(const GeomArgs &G) { return (G.npts + 255) / 256; }
+int64_t strain_blocks(const GeomArgs &G) { return row_chunks(G) * G.n[1] * G.n[2]; }
 
 void launch_strain_pi(const GeomArgs &G, int dtype, const double *ubar, const void *tau, void *Pi, double rho0,
-                      int64_t nbatch, double *partials, int64_t npartials, hipStream_t s) {
+                      int64_t nbatch, double *partials, int64_t npartials, hipStream_t s, int64_t pi_stride) {
     const int64_t nblk = strain_blocks(G);
     double *pp = (partials && nblk * nbatch <= npartials) ? partials : nullptr;
     dim3 grid((unsigned)nblk, (unsigned)nbatch);
-#define CGEF_SP(TT, MM) hipLaunchKernelGGL((k_strain_pi<TT, MM>), grid, dim3(256), 0, s, G, ubar, (const TT *)tau, (TT *)Pi, rho0, pp, nblk)
+    const int64_t ps = pi_stride > 0 ? pi_stride : G.npts;
+    const int64_t nch = row_chunks(G);
+#define CGEF_SP(TT, MM) hipLaunchKernelGGL((k_strain_pi<TT, MM>), grid, dim3(256), 0, s, G, ubar, (const TT *)tau, (TT *)Pi, rho0, pp, nblk, ps, nch)
     if (dtype == CGEF_F32) {
         if (G.mode == 0) CGEF_SP(float, 0);
         else if (G.mode == 1) CGEF_SP(float, 1);
@@ -524,22 +542,20 @@ void launch_energy_final(int64_t nbatch, int64_t nblocks, const double *partials
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_energy_fields(GeomArgs G, const T *ub, const T *vb, const T *wb, double *partials,
-                                                       int64_t nblk) {
+                                                       int64_t nblk, int64_t nchunk) {
     __shared__ double sh[4];
     const int64_t b = blockIdx.y;
-    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    const int64_t P = G.npts;
+    const RowChunk c = row_chunk(G, nchunk);
+    const int64_t P = G.npts, t = c.t;
     double e = 0.0;
-    if (t < P && G.mask[t]) {
-        const int64_t Nx = G.n[0], Ny = G.n[1];
-        const int64_t i = t % Nx, j = (t / Nx) % Ny, k = t / (Nx * Ny);
+    if (c.i < G.n[0] && G.mask[t]) {
         const double u = (double)ub[b * P + t], v = (double)vb[b * P + t];
         double v2 = u * u + v * v;
         if (wb) {
             const double w = (double)wb[b * P + t];
             v2 += w * w;
         }
-        e = 0.5 * v2 * cell_area(G, i, j, k);
+        e = 0.5 * v2 * cell_area(G, c.i, c.j, c.k);
     }
     const double s = block_sum<256>(e, sh);
     if (threadIdx.x == 0) partials[b * nblk + blockIdx.x] = s;
@@ -552,25 +568,26 @@ void launch_energy_fields(const GeomArgs &G, int dtype, const void *ub, const vo
     dim3 grid((unsigned)nblk, (unsigned)nbatch);
     if (dtype == CGEF_F32)
         hipLaunchKernelGGL(k_energy_fields<float>, grid, dim3(256), 0, s, G, (const float *)ub, (const float *)vb,
-                           (const float *)wb, partials, nblk);
+                           (const float *)wb, partials, nblk, row_chunks(G));
     else
         hipLaunchKernelGGL(k_energy_fields<double>, grid, dim3(256), 0, s, G, (const double *)ub, (const double *)vb,
-                           (const double *)wb, partials, nblk);
+                           (const double *)wb, partials, nblk, row_chunks(G));
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_derivative(GeomArgs G, int axis, const T *in, T *out) {
+__global__ __launch_bounds__(256) void k_derivative(GeomArgs G, int axis, const T *in, T *out, int64_t nchunk) {
     const int64_t b = blockIdx.y;
-    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    const int64_t P = G.npts;
-    if (t >= P) return;
+    const RowChunk c = row_chunk(G, nchunk);
+    const int64_t P = G.npts, t = c.t, Nx = G.n[0];
+    if (c.i >= Nx) return;
     double v = 0.0;
     if (G.mask[t]) {
         const T *f = in + b * P;
-        v = deriv_axis(G, axis, t, [&](int64_t q) { return (double)f[q]; });
+        const int64_t p = axis == 0 ? c.i : (axis == 1 ? c.j : c.k);
+        const int64_t stride = axis == 0 ? 1 : (axis == 1 ? Nx : Nx * G.n[1]);
+        v = apply_stencil(stencil_axis(G, axis, t, p, stride), [&](int64_t q) { return (double)f[q]; });
         if (G.mode == 1) {
-            const int64_t j = (t / G.n[0]) % G.n[1];
-            const double cp = G.trig[2 * G.n[0] + G.n[1] + j];
+            const double cp = G.trig[2 * Nx + G.n[1] + c.j];
             if (axis == 0) v = (fabs(cp) < sqrt(DBL_EPSILON)) ? 0.0 : v / (G.R * cp);
             else v = v / G.R;
         }
@@ -579,11 +596,12 @@ __global__ __launch_bounds__(256) void k_derivative(GeomArgs G, int axis, const 
 }
 
 void launch_derivative(const GeomArgs &G, int axis, int dtype, const void *in, void *out, int64_t nbatch, hipStream_t s) {
-    dim3 grid((unsigned)((G.npts + 255) / 256), (unsigned)nbatch);
+    const int64_t nch = row_chunks(G);
+    dim3 grid((unsigned)(nch * G.n[1] * G.n[2]), (unsigned)nbatch);
     if (dtype == CGEF_F32)
-        hipLaunchKernelGGL(k_derivative<float>, grid, dim3(256), 0, s, G, axis, (const float *)in, (float *)out);
+        hipLaunchKernelGGL(k_derivative<float>, grid, dim3(256), 0, s, G, axis, (const float *)in, (float *)out, nch);
     else
-        hipLaunchKernelGGL(k_derivative<double>, grid, dim3(256), 0, s, G, axis, (const double *)in, (double *)out);
+        hipLaunchKernelGGL(k_derivative<double>, grid, dim3(256), 0, s, G, axis, (const double *)in, (double *)out, nch);
 }
 
 // ----------------------------------------------------------- linear filter of K given fields ----

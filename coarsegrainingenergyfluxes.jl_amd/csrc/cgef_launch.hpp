@@ -23,6 +23,7 @@ struct GeomArgs {
     const double *coord[3];
     const double *width[3];
     const double *trig;    // spherical: sinλ[Nx], cosλ[Nx], sinφ[Ny], cosφ[Ny]
+    const double *dcoef[3];  // derivative stencil weights per axis index (DCOEF_STRIDE each)
     double active_area;
 };
 
@@ -50,7 +51,8 @@ void launch_pi_filter(const GeomArgs &G, const BandArgs &B, int dtype, bool mask
                       const void *qpack, const double *cmean, double *ubar, void *tau, int64_t nbatch,
                       hipStream_t s);
 void launch_strain_pi(const GeomArgs &G, int dtype, const double *ubar, const void *tau, void *Pi,
-                      double rho0, int64_t nbatch, double *partials, int64_t npartials, hipStream_t s);
+                      double rho0, int64_t nbatch, double *partials, int64_t npartials, hipStream_t s,
+                      int64_t pi_stride = 0);
 void launch_energy_final(int64_t nbatch, int64_t nblocks, const double *partials, double active_area,
                          double *E_dev, hipStream_t s);
 int64_t strain_blocks(const GeomArgs &G);

@@ -27,9 +27,9 @@ __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_doub
 __device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
 __device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
 
-// Twiddle tw[k] = exp(−2πik/N) (conjugated for the inverse).
-__device__ __forceinline__ double2 twid(const double2 *tw, int64_t k, int n, bool inv) {
-    double2 t = tw[k % n];
+// Twiddle tw[k] = exp(−2πik/N) (conjugated for the inverse); callers guarantee 0 ≤ k < N.
+__device__ __forceinline__ double2 twid(const double2 *tw, int k, bool inv) {
+    const double2 t = tw[k];
     return inv ? cconj(t) : t;
 }
 
@@ -49,6 +49,33 @@ __device__ __forceinline__ void dft_small(double2 *v, const double2 *tw, int n, 
         v[2] = csub(a, c);
         v[1] = cadd(b, dj);
         v[3] = csub(b, dj);
+    } else if (R == 3) {
+        // y1,2 = v0 − (v1+v2)/2 ∓ i·(√3/2)(v1 − v2) (forward; signs swap for the inverse)
+        const double s3 = 0.86602540378443865;  // sin(2π/3)
+        const double2 a = cadd(v[1], v[2]), b = csub(v[1], v[2]);
+        const double2 m = make_double2(v[0].x - 0.5 * a.x, v[0].y - 0.5 * a.y);
+        const double2 d = make_double2(s3 * b.x, s3 * b.y);
+        const double2 dj = inv ? make_double2(-d.y, d.x) : make_double2(d.y, -d.x);  // ∓i·d
+        v[0] = cadd(v[0], a);
+        v[1] = cadd(m, dj);
+        v[2] = csub(m, dj);
+    } else if (R == 5) {
+        // pairs a_k = v_k + v_{5−k}, b_k = v_k − v_{5−k}; cosines mix the a's, sines the b's
+        const double c1 = 0.30901699437494742, c2 = -0.80901699437494742;  // cos(2π/5), cos(4π/5)
+        const double s1 = 0.95105651629515357, s2 = 0.58778525229247313;   // sin(2π/5), sin(4π/5)
+        const double2 a1 = cadd(v[1], v[4]), b1 = csub(v[1], v[4]);
+        const double2 a2 = cadd(v[2], v[3]), b2 = csub(v[2], v[3]);
+        const double2 m1 = make_double2(v[0].x + c1 * a1.x + c2 * a2.x, v[0].y + c1 * a1.y + c2 * a2.y);
+        const double2 m2 = make_double2(v[0].x + c2 * a1.x + c1 * a2.x, v[0].y + c2 * a1.y + c1 * a2.y);
+        const double2 n1 = make_double2(s1 * b1.x + s2 * b2.x, s1 * b1.y + s2 * b2.y);
+        const double2 n2 = make_double2(s2 * b1.x - s1 * b2.x, s2 * b1.y - s1 * b2.y);
+        const double2 j1 = inv ? make_double2(-n1.y, n1.x) : make_double2(n1.y, -n1.x);  // ∓i·n1
+        const double2 j2 = inv ? make_double2(-n2.y, n2.x) : make_double2(n2.y, -n2.x);
+        v[0] = make_double2(v[0].x + a1.x + a2.x, v[0].y + a1.y + a2.y);
+        v[1] = cadd(m1, j1);
+        v[4] = csub(m1, j1);
+        v[2] = cadd(m2, j2);
+        v[3] = csub(m2, j2);
     } else {
         double2 o[R];
         const int step = n / R;
@@ -56,7 +83,7 @@ __device__ __forceinline__ void dft_small(double2 *v, const double2 *tw, int n, 
         for (int a = 0; a < R; a++) {
             double2 s = v[0];
 #pragma unroll
-            for (int b = 1; b < R; b++) s = cadd(s, cmul(v[b], twid(tw, (int64_t)((a * b) % R) * step, n, inv)));
+            for (int b = 1; b < R; b++) s = cadd(s, cmul(v[b], twid(tw, ((a * b) % R) * step, inv)));
             o[a] = s;
         }
 #pragma unroll
@@ -78,7 +105,7 @@ __device__ void stockham_stage(const double2 *src, double2 *dst, int n, int Ns, 
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const double2 a = x[j + r * nb];
-            v[r] = r == 0 ? a : cmul(a, twid(tw, (int64_t)r * k * step, n, inv));
+            v[r] = r == 0 ? a : cmul(a, twid(tw, r * k * step, inv));
         }
         dft_small<R>(v, tw, n, inv);
         const int idx = (j / Ns) * Ns * R + k;
@@ -115,13 +142,25 @@ __device__ double2 *fft_lds(double2 *buf, double2 *buf2, const RingFft &F, const
     return src;
 }
 
+// Work item of a row-pair kernel. Workgroups are dispatched round-robin over the 8 XCDs, each with its
+// own L2; giving every XCD a contiguous range of (column, row pair) items makes neighbouring row pairs
+// — whose spectra share cache lines of the [m][col][row] layout — write through the same L2.
+__device__ __forceinline__ bool xcd_item(int64_t total, int64_t &item) {
+    const int64_t chunk = (total + 7) / 8;
+    item = (int64_t)(blockIdx.x % 8) * chunk + blockIdx.x / 8;
+    return item < total;
+}
+
 // Forward: real rows (r0, r0+1) of column `col` (xin[col][row][i]) → XF[m][col][row], m ≤ N/2.
 __global__ __launch_bounds__(256) void k_fft_rows(RingFft F, const double2 *__restrict__ tw, const double *__restrict__ xin,
                                                   double2 *__restrict__ XF, int64_t nrows, int64_t ncol) {
     extern __shared__ double2 sm[];
     const int n = F.n;
-    const int64_t col = blockIdx.y;
-    const int64_t r0 = 2 * (int64_t)blockIdx.x, r1 = r0 + 1;
+    const int64_t npair = (nrows + 1) / 2;
+    int64_t item;
+    if (!xcd_item(npair * ncol, item)) return;
+    const int64_t col = item / npair;
+    const int64_t r0 = 2 * (item % npair), r1 = r0 + 1;
     const bool has1 = r1 < nrows;
     const double *x0 = xin + (col * nrows + r0) * n;
     const double *x1 = has1 ? x0 + n : nullptr;
@@ -144,8 +183,11 @@ __global__ __launch_bounds__(256) void k_ifft_rows(RingFft F, const double2 *__r
                                                    double *__restrict__ yout, int64_t nrows, int64_t ncol) {
     extern __shared__ double2 sm[];
     const int n = F.n;
-    const int64_t col = blockIdx.y;
-    const int64_t r0 = 2 * (int64_t)blockIdx.x, r1 = r0 + 1;
+    const int64_t npair = (nrows + 1) / 2;
+    int64_t item;
+    if (!xcd_item(npair * ncol, item)) return;
+    const int64_t col = item / npair;
+    const int64_t r0 = 2 * (item % npair), r1 = r0 + 1;
     const bool has1 = r1 < nrows;
     const int mh = n / 2 + 1;
     for (int m = threadIdx.x; m < n; m += blockDim.x) {
@@ -169,28 +211,217 @@ __global__ __launch_bounds__(256) void k_ifft_rows(RingFft F, const double2 *__r
     }
 }
 
+// ---- register-staged row FFT: one LDS buffer per sequence, each stage reads its butterflies into
+// registers, waits, then writes them back in Stockham order (half the LDS of a ping-pong, so twice
+// the transforms in flight per CU), with per-stage twiddles read contiguously.
+
+// Butterflies per thread and stage held in registers: at most 12 complex values for the in-place
+// radices 2, 3, 4, and 16 with the output copy of the generic radices 5, 7 (11 and 13 are not staged).
+__host__ __device__ constexpr int staged_mb_max(int R) {
+    return R == 2 ? 6 : R == 3 ? 4 : R == 4 ? 3 : R == 5 ? 2 : R == 7 ? 1 : 0;
+}
+
+template <int R, int MB, int NSEQ>
+__device__ __forceinline__ void stage_staged(double2 *sm, int n, int Ns, const double2 *__restrict__ tws,
+                                             const double2 *__restrict__ tw, bool inv) {
+    const int nb = n / R;
+    const int tot = nb * NSEQ;
+    double2 v[MB][R];
+    int dst[MB];
+#pragma unroll
+    for (int q = 0; q < MB; q++) {
+        const int p = threadIdx.x + q * 256;
+        if (p < tot) {
+            const int sq = NSEQ == 1 ? 0 : (p >= nb ? 1 : 0);
+            const int j = p - sq * nb;
+            const int k = j % Ns;
+            const double2 *x = sm + sq * n;
+            const double2 *t = tws + k * (R - 1);
+            v[q][0] = x[j];
+            if (Ns == 1) {  // first stage: every twiddle is 1
+#pragma unroll
+                for (int r = 1; r < R; r++) v[q][r] = x[j + r * nb];
+            } else {
+#pragma unroll
+                for (int r = 1; r < R; r++) {
+                    const double2 w = t[r - 1];
+                    v[q][r] = cmul(x[j + r * nb], inv ? cconj(w) : w);
+                }
+            }
+            dft_small<R>(v[q], tw, n, inv);
+            dst[q] = sq * n + (j - k) * R + k;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < MB; q++) {
+        const int p = threadIdx.x + q * 256;
+        if (p < tot) {
+#pragma unroll
+            for (int r = 0; r < R; r++) sm[dst[q] + r * Ns] = v[q][r];
+        }
+    }
+    __syncthreads();
+}
+
+// NSEQ complex sequences of length F.n in sm (transformed in place); the host has checked that every
+// stage fits staged_mb_max.
+template <int NSEQ>
+__device__ void fft_staged(double2 *sm, const RingFft &F, const double2 *tw, const double2 *tws, bool inv) {
+    __syncthreads();
+    int Ns = 1;
+    for (int st = 0; st < F.nstage; st++) {
+        const int R = F.radix[st];
+        const int mb = (F.n / R * NSEQ + 255) / 256;
+        const double2 *t = tws + F.twoff[st];
+        switch (R * 16 + mb) {
+#define CGEF_ST(RR, MM) \
+    case RR * 16 + MM: stage_staged<RR, MM, NSEQ>(sm, F.n, Ns, t, tw, inv); break;
+            CGEF_ST(2, 1) CGEF_ST(2, 2) CGEF_ST(2, 3) CGEF_ST(2, 4) CGEF_ST(2, 5) CGEF_ST(2, 6)
+            CGEF_ST(3, 1) CGEF_ST(3, 2) CGEF_ST(3, 3) CGEF_ST(3, 4)
+            CGEF_ST(4, 1) CGEF_ST(4, 2) CGEF_ST(4, 3)
+            CGEF_ST(5, 1) CGEF_ST(5, 2)
+            CGEF_ST(7, 1)
+#undef CGEF_ST
+            default: break;
+        }
+        Ns *= R;
+    }
+}
+
+// Forward, 2·NSEQ real rows per block: rows r0.. of column `col` → XF[m][col][row], m ≤ N/2; the
+// spectra of the block's rows land contiguously for each m.
+template <int NSEQ>
+__global__ __launch_bounds__(256) void k_fft_rows_staged(RingFft F, const double2 *__restrict__ tw,
+                                                         const double2 *__restrict__ tws, const double *__restrict__ xin,
+                                                         double2 *__restrict__ XF, int64_t nrows, int64_t ncol) {
+    extern __shared__ double2 sm[];
+    constexpr int RPB = 2 * NSEQ;
+    const int n = F.n;
+    const int64_t ngrp = (nrows + RPB - 1) / RPB;
+    int64_t item;
+    if (!xcd_item(ngrp * ncol, item)) return;
+    const int64_t col = item / ngrp, r0 = (item % ngrp) * RPB;
+    const double *x = xin + (col * nrows + r0) * n;
+    for (int e = threadIdx.x; e < n * NSEQ; e += 256) {
+        const int sq = NSEQ == 1 ? 0 : (e >= n ? 1 : 0);
+        const int i = e - sq * n;
+        const bool va = r0 + 2 * sq < nrows, vb = r0 + 2 * sq + 1 < nrows;
+        sm[e] = make_double2(va ? x[(2 * sq) * n + i] : 0.0, vb ? x[(2 * sq + 1) * n + i] : 0.0);
+    }
+    fft_staged<NSEQ>(sm, F, tw, tws, false);
+    const int mh = n / 2 + 1;
+    for (int e = threadIdx.x; e < mh * NSEQ; e += 256) {
+        const int m = e / NSEQ, sq = e - m * NSEQ;
+        const double2 a = sm[sq * n + m], b = cconj(sm[sq * n + (m == 0 ? 0 : n - m)]);
+        const double2 X = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y + b.y));
+        const double2 d = csub(a, b);
+        const double2 Y = make_double2(0.5 * d.y, -0.5 * d.x);  // (a − b)/(2i)
+        const int64_t ra = r0 + 2 * sq;
+        double2 *o = XF + ((int64_t)m * ncol + col) * nrows + ra;
+        if (ra < nrows) o[0] = X;
+        if (ra + 1 < nrows) o[1] = Y;
+    }
+}
+
+// Inverse, 2·NSEQ rows per block: OUT[m][col][row] (Hermitian halves) → yout[col][row][i] / N.
+template <int NSEQ>
+__global__ __launch_bounds__(256) void k_ifft_rows_staged(RingFft F, const double2 *__restrict__ tw,
+                                                          const double2 *__restrict__ tws, const double2 *__restrict__ OUT,
+                                                          double *__restrict__ yout, int64_t nrows, int64_t ncol) {
+    extern __shared__ double2 sm[];
+    constexpr int RPB = 2 * NSEQ;
+    const int n = F.n;
+    const int64_t ngrp = (nrows + RPB - 1) / RPB;
+    int64_t item;
+    if (!xcd_item(ngrp * ncol, item)) return;
+    const int64_t col = item / ngrp, r0 = (item % ngrp) * RPB;
+    const int mh = n / 2 + 1;
+    for (int e = threadIdx.x; e < n * NSEQ; e += 256) {
+        const int m = e / NSEQ, sq = e - m * NSEQ;
+        const bool lo = m < mh;
+        const int mm = lo ? m : n - m;
+        const int64_t ra = r0 + 2 * sq;
+        const double2 *o = OUT + ((int64_t)mm * ncol + col) * nrows + ra;
+        double2 X = ra < nrows ? o[0] : make_double2(0.0, 0.0);
+        double2 Y = ra + 1 < nrows ? o[1] : make_double2(0.0, 0.0);
+        if (!lo) {
+            X = cconj(X);
+            Y = cconj(Y);
+        }
+        sm[sq * n + m] = make_double2(X.x - Y.y, X.y + Y.x);  // X + iY
+    }
+    fft_staged<NSEQ>(sm, F, tw, tws, true);
+    const double sc = 1.0 / n;
+    double *y = yout + (col * nrows + r0) * n;
+    for (int e = threadIdx.x; e < n * NSEQ; e += 256) {
+        const int sq = NSEQ == 1 ? 0 : (e >= n ? 1 : 0);
+        const int i = e - sq * n;
+        const double2 z = sm[e];
+        if (r0 + 2 * sq < nrows) y[(2 * sq) * n + i] = z.x * sc;
+        if (r0 + 2 * sq + 1 < nrows) y[(2 * sq + 1) * n + i] = z.y * sc;
+    }
+}
+
+// One sequence (two real rows) per 256-thread block: 2-row blocks keep the kernels near 100 VGPRs,
+// so four blocks share a CU; the LDS of a sequence must also leave room for them.
+int ring_staged_nseq(const RingFft &F) {
+    if ((size_t)F.n * sizeof(double2) > 32 * 1024 || F.nstage == 0) return 0;
+    for (int st = 0; st < F.nstage; st++) {
+        const int R = F.radix[st];
+        if (staged_mb_max(R) == 0 || (F.n / R + 255) / 256 > staged_mb_max(R)) return 0;
+    }
+    return 1;
+}
+
 // Banded contraction per Fourier mode m. Block: 256 target rows of one plane (j0..j0+255, level k)
-// × a chunk of CC columns; source rows of each dz plane staged in LDS (zero outside a non-periodic
-// axis), Ŵ read along rows (coalesced on the sphere, broadcast on translation-invariant grids).
-template <int CC>
-__global__ __launch_bounds__(256) void k_contract(RingArgs A, const double2 *__restrict__ XF, double2 *__restrict__ OUT,
-                                                  int64_t ncol) {
+// × a chunk of CC columns, 256/RB threads, each accumulating RB consecutive target rows so that every
+// staged source value feeds RB rows (the offsets of a dz group form a contiguous dy range, so source
+// row s serves target row q at offset s − q). Source rows of each dz plane are staged in LDS (zero
+// outside a non-periodic axis) split into RB planes by s mod RB, which keeps the lanes of a read on
+// consecutive addresses; Ŵ is read along rows (coalesced on the sphere, broadcast on
+// translation-invariant grids).
+#ifndef CGEF_CONTRACT_CC
+#define CGEF_CONTRACT_CC 5
+#endif
+#ifndef CGEF_CONTRACT_RB
+#define CGEF_CONTRACT_RB 1
+#endif
+
+template <int CC, int RB>
+__global__ __launch_bounds__(256 / RB) void k_contract(RingArgs A, const double2 *__restrict__ XF, double2 *__restrict__ OUT,
+                                                       int64_t ncol) {
+    constexpr int NT = 256 / RB;
     extern __shared__ double2 st[];
     const int64_t Ny = A.ny, Nz = A.nz;
-    const int64_t c0 = (int64_t)blockIdx.x * CC;
     const int64_t nyb = (Ny + 255) / 256;
-    const int64_t k = blockIdx.y / nyb, j0 = (blockIdx.y % nyb) * 256;
-    const int64_t m = blockIdx.z;
+    const int64_t ncc = (ncol + CC - 1) / CC;
+    // items (m, row block, column chunk), column chunk fastest, in contiguous ranges per XCD: the
+    // chunks of one (m, row block) read the same Ŵ slab and so should share an L2
+    int64_t item;
+    if (!xcd_item((int64_t)A.mh * nyb * Nz * ncc, item)) return;
+    const int64_t c0 = (item % ncc) * CC;
+    const int64_t rbk = (item / ncc) % (nyb * Nz);
+    const int64_t m = item / (ncc * nyb * Nz);
+    const int64_t k = rbk / nyb, j0 = (rbk % nyb) * 256;
     const int tid = threadIdx.x;
-    const int64_t j = j0 + tid;
-    const bool valid = j < Ny;
-    const int64_t r = valid ? j + Ny * k : 0;
-    const int band = valid ? A.band_of_row[r] : 0;
     const int span = 256 + A.dy_max - A.dy_min;
-    const int ncc = (int)((ncol - c0) < CC ? (ncol - c0) : CC);
-    double2 acc[CC];
+    const int sp = (span + RB - 1) / RB;                          // LDS plane length
+    const int nvc = (int)((ncol - c0) < CC ? (ncol - c0) : CC);  // valid columns in this chunk
+    int band[RB];
+    bool valid[RB], anyv = false;
 #pragma unroll
-    for (int c = 0; c < CC; c++) acc[c] = make_double2(0.0, 0.0);
+    for (int q = 0; q < RB; q++) {
+        const int64_t j = j0 + tid * RB + q;
+        valid[q] = j < Ny;
+        band[q] = valid[q] ? A.band_of_row[j + Ny * k] : 0;
+        anyv = anyv || valid[q];
+    }
+    double2 acc[RB][CC];
+#pragma unroll
+    for (int q = 0; q < RB; q++)
+#pragma unroll
+        for (int c = 0; c < CC; c++) acc[q][c] = make_double2(0.0, 0.0);
     const int64_t mrow = (int64_t)m * A.noff;
     for (int g = 0; g < A.ndz; g++) {
         const int dz = A.dz_val[g];
@@ -200,7 +431,7 @@ __global__ __launch_bounds__(256) void k_contract(RingArgs A, const double2 *__r
             kk = ((kk % Nz) + Nz) % Nz;
         }
         __syncthreads();
-        for (int q = tid; q < span * CC; q += 256) {
+        for (int q = tid; q < span * CC; q += NT) {
             const int c = q / span, s = q - c * span;
             int64_t jj = j0 + A.dy_min + s;
             double2 v = make_double2(0.0, 0.0);
@@ -209,26 +440,54 @@ __global__ __launch_bounds__(256) void k_contract(RingArgs A, const double2 *__r
                 if (A.py) jj = ((jj % Ny) + Ny) % Ny;
                 else in = false;
             }
-            if (in && c < ncc) v = XF[(m * ncol + c0 + c) * A.nrows + jj + Ny * kk];
-            st[c * span + s] = v;
+            if (in && c < nvc) v = XF[(m * ncol + c0 + c) * A.nrows + jj + Ny * kk];
+            st[(c * RB + s % RB) * sp + s / RB] = v;
         }
         __syncthreads();
-        if (valid) {
-            for (int o = A.dz_off0[g]; o < A.dz_off0[g + 1]; o++) {
-                const double w = A.what[(mrow + o) * A.nbands + band];
-                if (w == 0.0) continue;
-                const int s = tid + A.off_dy[o] - A.dy_min;
+        if (anyv) {
+            const int o0 = A.dz_off0[g], ng = A.dz_off0[g + 1] - o0;
+            const int e0 = A.off_dy[o0] - A.dy_min;  // staged index of (thread row 0, offset 0) − tid·RB
+            const double *wrow = A.what + (mrow + o0) * A.nbands;
+            // weights in batches of PB source rows, loaded before use so their latency overlaps
+            constexpr int PB = 8;
+            for (int p0 = 0; p0 < ng + RB - 1; p0 += PB) {
+                double w[PB][RB];
 #pragma unroll
-                for (int c = 0; c < CC; c++) {
-                    const double2 v = st[c * span + s];
-                    acc[c].x = fma(w, v.x, acc[c].x);
-                    acc[c].y = fma(w, v.y, acc[c].y);
+                for (int pp = 0; pp < PB; pp++)
+#pragma unroll
+                    for (int q = 0; q < RB; q++) {
+                        const int o = p0 + pp - q;
+                        w[pp][q] = (o >= 0 && o < ng) ? wrow[(int64_t)o * A.nbands + band[q]] : 0.0;
+                    }
+#pragma unroll
+                for (int pp = 0; pp < PB; pp++) {
+                    if (p0 + pp >= ng + RB - 1) break;
+                    // staged row tid·RB + e is the source of target row q at offset p0 + pp − q
+                    const int e = e0 + p0 + pp;
+                    const double2 *src = st + (e % RB) * sp + tid + e / RB;
+#pragma unroll
+                    for (int c = 0; c < CC; c++) {
+                        const double2 v = src[c * RB * sp];
+#pragma unroll
+                        for (int q = 0; q < RB; q++) {
+                            acc[q][c].x = fma(w[pp][q], v.x, acc[q][c].x);
+                            acc[q][c].y = fma(w[pp][q], v.y, acc[q][c].y);
+                        }
+                    }
                 }
             }
         }
     }
-    if (!valid) return;
-    for (int c = 0; c < ncc; c++) OUT[(m * ncol + c0 + c) * A.nrows + r] = acc[c];
+    // fully unrolled with predicates: loops bounded by nvc would index acc dynamically and move the
+    // accumulators to scratch memory
+#pragma unroll
+    for (int q = 0; q < RB; q++) {
+        if (!valid[q]) continue;
+        const int64_t r = j0 + tid * RB + q + Ny * k;
+#pragma unroll
+        for (int c = 0; c < CC; c++)
+            if (c < nvc) OUT[(m * ncol + c0 + c) * A.nrows + r] = acc[q][c];
+    }
 }
 
 // Ŵ for every (band, offset): the run's weights laid out circularly on a length-N row, FFT'd two
@@ -301,7 +560,7 @@ __device__ __forceinline__ void frame_vel(const GeomArgs &G, int dtype, const vo
 
 // Field order per slice: q (NC), products (NP: xx xy [xz] yy [yz zz]), mask (if masked).
 __global__ __launch_bounds__(256) void k_ring_prep(GeomArgs G, int dtype, const void *u, const void *v, const void *w,
-                                                   const double *cmean, int nf, int masked, double *xin) {
+                                                   int nf, int withmask, double *xin) {
     const int64_t b = blockIdx.y;
     const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const int64_t P = G.npts;
@@ -310,11 +569,7 @@ __global__ __launch_bounds__(256) void k_ring_prep(GeomArgs G, int dtype, const 
     const int64_t i = t % Nx, j = (t / Nx) % Ny;
     const bool act = G.mask[t] != 0;
     double q[3] = {0.0, 0.0, 0.0};
-    if (act) {
-        double U[3];
-        frame_vel(G, dtype, u, v, w, b * P, t, i, j, U);
-        for (int a = 0; a < 3; a++) q[a] = U[a] - cmean[3 * b + a];
-    }
+    if (act) frame_vel(G, dtype, u, v, w, b * P, t, i, j, q);
     double *o = xin + b * nf * P;
     if (G.mode == 0) {
         o[t] = q[0];
@@ -322,7 +577,7 @@ __global__ __launch_bounds__(256) void k_ring_prep(GeomArgs G, int dtype, const 
         o[2 * P + t] = q[0] * q[0];
         o[3 * P + t] = q[0] * q[1];
         o[4 * P + t] = q[1] * q[1];
-        if (masked) o[5 * P + t] = act ? 1.0 : 0.0;
+        if (withmask) o[5 * P + t] = act ? 1.0 : 0.0;
     } else {
         o[t] = q[0];
         o[P + t] = q[1];
@@ -333,16 +588,17 @@ __global__ __launch_bounds__(256) void k_ring_prep(GeomArgs G, int dtype, const 
         o[6 * P + t] = q[1] * q[1];
         o[7 * P + t] = q[1] * q[2];
         o[8 * P + t] = q[2] * q[2];
-        if (masked) o[9 * P + t] = act ? 1.0 : 0.0;
+        if (withmask) o[9 * P + t] = act ? 1.0 : 0.0;
     }
 }
 
-// τ and ū from the filtered fields (numerators), the Galilean correction of the mean shift, and the
-// local frame on the sphere (Diagnostics.jl:596-650).
+// τ = F[q_a q_b]/den − ū_a ū_b and ū = F[q]/den from the filtered planes, then the local frame on the
+// sphere (Diagnostics.jl:596-650). FP64 throughout, so no mean shift is needed against cancellation.
 template <typename T>
 __global__ __launch_bounds__(256) void k_ring_epilogue(GeomArgs G, const double *__restrict__ yout, const double *__restrict__ D_row,
-                                                       const double *__restrict__ cmean, int nf, int masked, int strategy,
-                                                       double *__restrict__ ubar, T *__restrict__ tau) {
+                                                       int nf, int withmask, int strategy, double *__restrict__ ubar,
+                                                       T *__restrict__ tau, T *__restrict__ uo, T *__restrict__ vo,
+                                                       T *__restrict__ wo) {
     const int64_t b = blockIdx.y;
     const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const int64_t P = G.npts;
@@ -351,41 +607,33 @@ __global__ __launch_bounds__(256) void k_ring_epilogue(GeomArgs G, const double 
     const int64_t i = t % Nx, r = t / Nx, j = r % Ny;
     const double *y = yout + b * nf * P;
     const int NC = G.mode == 0 ? 2 : 3;
-    const double D = D_row[r];
-    const double msum = masked ? y[(nf - 1) * P + t] : D;     // Σ_active w
-    const double den = strategy == CGEF_DEFORMABLE ? msum : D;
+    // ZeroFill divides by the in-range weight sum of the row; Deformable by the filtered mask
+    // (Filtering.jl:3180-3240)
+    const double den = (strategy == CGEF_DEFORMABLE && withmask) ? y[(nf - 1) * P + t] : D_row[r];
     const bool ok = G.mask[t] != 0 && den > 1e-15;
     double U[3] = {0.0, 0.0, 0.0}, tc[6] = {0, 0, 0, 0, 0, 0};
     if (ok) {
-        const double *c = cmean + 3 * b;
-        double qb[3] = {0.0, 0.0, 0.0};
-        for (int a = 0; a < NC; a++) qb[a] = y[a * P + t] / den;
-        double Pq[6] = {0, 0, 0, 0, 0, 0};
+        const double inv = 1.0 / den;
+        for (int a = 0; a < NC; a++) U[a] = y[a * P + t] * inv;
         if (NC == 2) {
-            Pq[0] = y[2 * P + t] / den;
-            Pq[1] = y[3 * P + t] / den;
-            Pq[3] = y[4 * P + t] / den;
+            tc[0] = y[2 * P + t] * inv - U[0] * U[0];
+            tc[1] = y[3 * P + t] * inv - U[0] * U[1];
+            tc[3] = y[4 * P + t] * inv - U[1] * U[1];
         } else {
-            for (int q6 = 0; q6 < 6; q6++) Pq[q6] = y[(3 + q6) * P + t] / den;
-        }
-        const int ia[6] = {0, 0, 0, 1, 1, 2}, ib[6] = {0, 1, 2, 1, 2, 2};
-        const double om = (strategy == CGEF_ZEROFILL && masked) ? 1.0 - msum / D : 0.0;  // 1 − m
-        const double mm = 1.0 - om;
-        for (int a = 0; a < 3; a++) U[a] = qb[a] + c[a] * (strategy == CGEF_ZEROFILL ? mm : 1.0);
-        for (int q6 = 0; q6 < 6; q6++) {
-            const int a = ia[q6], bb = ib[q6];
-            tc[q6] = Pq[q6] - qb[a] * qb[bb] + om * (c[a] * U[bb] + c[bb] * U[a]) - c[a] * c[bb] * mm * om;
+            const int ia[6] = {0, 0, 0, 1, 1, 2}, ib[6] = {0, 1, 2, 1, 2, 2};
+            for (int q6 = 0; q6 < 6; q6++) tc[q6] = y[(3 + q6) * P + t] * inv - U[ia[q6]] * U[ib[q6]];
         }
     }
     double *ub = ubar + b * 3 * P;
     T *tb = tau + b * 6 * P;
+    double o0, o1, o2;
     if (G.mode == 1) {
         const double sl = G.trig[i], cl = G.trig[Nx + i];
         const double sp = G.trig[2 * Nx + j], cp = G.trig[2 * Nx + Ny + j];
         const double ee[3] = {-sl, cl, 0.0}, en[3] = {-sp * cl, -sp * sl, cp}, er[3] = {cp * cl, cp * sl, sp};
-        ub[t] = ee[0] * U[0] + ee[1] * U[1];
-        ub[P + t] = en[0] * U[0] + en[1] * U[1] + en[2] * U[2];
-        ub[2 * P + t] = er[0] * U[0] + er[1] * U[1] + er[2] * U[2];
+        o0 = ee[0] * U[0] + ee[1] * U[1];
+        o1 = en[0] * U[0] + en[1] * U[1] + en[2] * U[2];
+        o2 = er[0] * U[0] + er[1] * U[1] + er[2] * U[2];
         const double T3[3][3] = {{tc[0], tc[1], tc[2]}, {tc[1], tc[3], tc[4]}, {tc[2], tc[4], tc[5]}};
         double Te[3], Tn[3];
         for (int rr = 0; rr < 3; rr++) {
@@ -396,18 +644,25 @@ __global__ __launch_bounds__(256) void k_ring_epilogue(GeomArgs G, const double 
         tb[P + t] = (T)(ee[0] * Tn[0] + ee[1] * Tn[1] + ee[2] * Tn[2]);
         tb[2 * P + t] = (T)(en[0] * Tn[0] + en[1] * Tn[1] + en[2] * Tn[2]);
     } else if (G.mode == 0) {
-        ub[t] = U[0];
-        ub[P + t] = U[1];
-        ub[2 * P + t] = 0.0;
+        o0 = U[0];
+        o1 = U[1];
+        o2 = 0.0;
         tb[t] = (T)tc[0];
         tb[P + t] = (T)tc[1];
         tb[2 * P + t] = (T)tc[3];
     } else {
-        ub[t] = U[0];
-        ub[P + t] = U[1];
-        ub[2 * P + t] = U[2];
+        o0 = U[0];
+        o1 = U[1];
+        o2 = U[2];
         for (int q6 = 0; q6 < 6; q6++) tb[q6 * P + t] = (T)tc[q6];
     }
+    // the strain reads planes 0, 1 (and 2 in 3-D); the filtered velocities leave in T when asked for
+    ub[t] = o0;
+    ub[P + t] = o1;
+    if (G.mode == 2) ub[2 * P + t] = o2;
+    if (uo) uo[b * P + t] = (T)o0;
+    if (vo) vo[b * P + t] = (T)o1;
+    if (wo) wo[b * P + t] = (T)o2;
 }
 
 // plain fields for filter_apply: x = f·active (FP64), plus the mask when a Deformable denominator is needed
@@ -460,44 +715,63 @@ void ring_build_what(const RingPlan &R, const Run *runs_d, const int32_t *run_of
 }
 
 void ring_forward(const RingPlan &R, const double *xin, double2 *XF, int64_t ncol, hipStream_t s) {
+    const int ns = R.fft.nseq;
+    if (ns > 0) {
+        const size_t sh = (size_t)ns * R.fft.n * sizeof(double2);
+        const int64_t items = (R.args.nrows + 2 * ns - 1) / (2 * ns) * ncol;
+        const dim3 grid((unsigned)(8 * ((items + 7) / 8)));
+        hipLaunchKernelGGL(k_fft_rows_staged<1>, grid, dim3(256), sh, s, R.fft, R.tw_d, R.tws_d, xin, XF, R.args.nrows, ncol);
+        return;
+    }
     const size_t sh = fft_shmem(R.fft);
     set_shmem((const void *)k_fft_rows, sh);
-    hipLaunchKernelGGL(k_fft_rows, dim3((unsigned)((R.args.nrows + 1) / 2), (unsigned)ncol), dim3(256), sh, s, R.fft,
-                       R.tw_d, xin, XF, R.args.nrows, ncol);
+    const int64_t items = (R.args.nrows + 1) / 2 * ncol;
+    hipLaunchKernelGGL(k_fft_rows, dim3((unsigned)(8 * ((items + 7) / 8))), dim3(256), sh, s, R.fft, R.tw_d, xin, XF,
+                       R.args.nrows, ncol);
 }
 
 void ring_inverse(const RingPlan &R, const double2 *OUT, double *yout, int64_t ncol, hipStream_t s) {
+    const int ns = R.fft.nseq;
+    if (ns > 0) {
+        const size_t sh = (size_t)ns * R.fft.n * sizeof(double2);
+        const int64_t items = (R.args.nrows + 2 * ns - 1) / (2 * ns) * ncol;
+        const dim3 grid((unsigned)(8 * ((items + 7) / 8)));
+        hipLaunchKernelGGL(k_ifft_rows_staged<1>, grid, dim3(256), sh, s, R.fft, R.tw_d, R.tws_d, OUT, yout, R.args.nrows, ncol);
+        return;
+    }
     const size_t sh = fft_shmem(R.fft);
     set_shmem((const void *)k_ifft_rows, sh);
-    hipLaunchKernelGGL(k_ifft_rows, dim3((unsigned)((R.args.nrows + 1) / 2), (unsigned)ncol), dim3(256), sh, s, R.fft,
-                       R.tw_d, OUT, yout, R.args.nrows, ncol);
+    const int64_t items = (R.args.nrows + 1) / 2 * ncol;
+    hipLaunchKernelGGL(k_ifft_rows, dim3((unsigned)(8 * ((items + 7) / 8))), dim3(256), sh, s, R.fft, R.tw_d, OUT, yout,
+                       R.args.nrows, ncol);
 }
 
 void ring_contract(const RingPlan &R, const double2 *XF, double2 *OUT, int64_t ncol, hipStream_t s) {
-    constexpr int CC = 10;
+    constexpr int CC = CGEF_CONTRACT_CC, RB = CGEF_CONTRACT_RB;
     const int span = 256 + R.args.dy_max - R.args.dy_min;
-    const size_t sh = (size_t)span * CC * sizeof(double2);
-    set_shmem((const void *)k_contract<CC>, sh);
+    const size_t sh = (size_t)CC * RB * ((span + RB - 1) / RB) * sizeof(double2);
+    set_shmem((const void *)k_contract<CC, RB>, sh);
     const int64_t nyb = (R.args.ny + 255) / 256;
-    dim3 grid((unsigned)((ncol + CC - 1) / CC), (unsigned)(nyb * R.args.nz), (unsigned)R.mh);
-    hipLaunchKernelGGL(k_contract<CC>, grid, dim3(256), sh, s, R.args, XF, OUT, ncol);
+    const int64_t items = (ncol + CC - 1) / CC * nyb * R.args.nz * R.mh;
+    hipLaunchKernelGGL((k_contract<CC, RB>), dim3((unsigned)(8 * ((items + 7) / 8))), dim3(256 / RB), sh, s, R.args, XF,
+                       OUT, ncol);
 }
 
-void ring_prep(const GeomArgs &G, int dtype, const void *u, const void *v, const void *w, const double *cmean, int nf,
-               int masked, double *xin, int64_t nbatch, hipStream_t s) {
+void ring_prep(const GeomArgs &G, int dtype, const void *u, const void *v, const void *w, int nf, int withmask,
+               double *xin, int64_t nbatch, hipStream_t s) {
     dim3 grid((unsigned)((G.npts + 255) / 256), (unsigned)nbatch);
-    hipLaunchKernelGGL(k_ring_prep, grid, dim3(256), 0, s, G, dtype, u, v, w, cmean, nf, masked, xin);
+    hipLaunchKernelGGL(k_ring_prep, grid, dim3(256), 0, s, G, dtype, u, v, w, nf, withmask, xin);
 }
 
-void ring_epilogue(const GeomArgs &G, int dtype, const double *yout, const double *D_row, const double *cmean, int nf,
-                   int masked, int strategy, double *ubar, void *tau, int64_t nbatch, hipStream_t s) {
+void ring_epilogue(const GeomArgs &G, int dtype, const double *yout, const double *D_row, int nf, int withmask,
+                   int strategy, double *ubar, void *tau, void *uo, void *vo, void *wo, int64_t nbatch, hipStream_t s) {
     dim3 grid((unsigned)((G.npts + 255) / 256), (unsigned)nbatch);
     if (dtype == CGEF_F32)
-        hipLaunchKernelGGL(k_ring_epilogue<float>, grid, dim3(256), 0, s, G, yout, D_row, cmean, nf, masked, strategy, ubar,
-                           (float *)tau);
+        hipLaunchKernelGGL(k_ring_epilogue<float>, grid, dim3(256), 0, s, G, yout, D_row, nf, withmask, strategy, ubar,
+                           (float *)tau, (float *)uo, (float *)vo, (float *)wo);
     else
-        hipLaunchKernelGGL(k_ring_epilogue<double>, grid, dim3(256), 0, s, G, yout, D_row, cmean, nf, masked, strategy, ubar,
-                           (double *)tau);
+        hipLaunchKernelGGL(k_ring_epilogue<double>, grid, dim3(256), 0, s, G, yout, D_row, nf, withmask, strategy, ubar,
+                           (double *)tau, (double *)uo, (double *)vo, (double *)wo);
 }
 
 void ring_prep_fields(const GeomArgs &G, int dtype, const void *in, int64_t nsl, int withmask, double *xin, hipStream_t s) {

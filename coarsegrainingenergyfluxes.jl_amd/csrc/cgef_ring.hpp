@@ -11,6 +11,10 @@
 namespace cgef {
 
 bool ring_factor(int n, RingFft &F);
+// Complex sequences per block of the register-staged row FFT (1), or 0 when a stage's radix or
+// butterfly count exceeds its register budget or the row exceeds 32 KiB of LDS (then the LDS
+// ping-pong FFT serves the plan).
+int ring_staged_nseq(const RingFft &F);
 bool ring_supported(const Plan &p);
 void build_ring(Plan &p, hipStream_t s);
 void free_ring(RingPlan &R);
@@ -19,10 +23,12 @@ void ring_build_what(const RingPlan &R, const Run *runs_d, const int32_t *run_of
 void ring_forward(const RingPlan &R, const double *xin, double2 *XF, int64_t ncol, hipStream_t s);
 void ring_contract(const RingPlan &R, const double2 *XF, double2 *OUT, int64_t ncol, hipStream_t s);
 void ring_inverse(const RingPlan &R, const double2 *OUT, double *yout, int64_t ncol, hipStream_t s);
-void ring_prep(const GeomArgs &G, int dtype, const void *u, const void *v, const void *w, const double *cmean, int nf,
-               int masked, double *xin, int64_t nbatch, hipStream_t s);
-void ring_epilogue(const GeomArgs &G, int dtype, const double *yout, const double *D_row, const double *cmean, int nf,
-                   int masked, int strategy, double *ubar, void *tau, int64_t nbatch, hipStream_t s);
+// Π inputs: per slice nf planes (frame velocity components, their products, then the mask when
+// `withmask`); the epilogue turns the filtered planes into ū (FP64) and τ, optionally exporting ū in T.
+void ring_prep(const GeomArgs &G, int dtype, const void *u, const void *v, const void *w, int nf, int withmask,
+               double *xin, int64_t nbatch, hipStream_t s);
+void ring_epilogue(const GeomArgs &G, int dtype, const double *yout, const double *D_row, int nf, int withmask,
+                   int strategy, double *ubar, void *tau, void *uo, void *vo, void *wo, int64_t nbatch, hipStream_t s);
 void ring_prep_fields(const GeomArgs &G, int dtype, const void *in, int64_t nsl, int withmask, double *xin, hipStream_t s);
 void ring_normalize(const GeomArgs &G, int dtype, const double *yout, const double *D_row, int64_t nsl, int withmask,
                     int strategy, void *out, hipStream_t s);

@@ -33,13 +33,18 @@ static T *up(const std::vector<T> &v) {
     return d;
 }
 
-// Whether the ring engine can serve this plan: a periodic, uniform x axis whose length factors into
-// the supported radices, and runs that are symmetric modulo the ring (so Ŵ is real).
-bool ring_supported(const Plan &p) {
-    const Grid &g = *p.grid;
+// Whether the ring engine can serve a grid: a periodic, uniform x axis whose length factors into the
+// supported radices.
+bool ring_grid_ok(const Grid &g) {
     if (!g.periodic[0] || !g.uniform[0] || g.n[0] < 2) return false;
     RingFft F;
-    if (!ring_factor((int)g.n[0], F)) return false;
+    return ring_factor((int)g.n[0], F);
+}
+
+// ... and a plan: its runs must be symmetric modulo the ring (so Ŵ is real).
+bool ring_supported(const Plan &p) {
+    if (!ring_grid_ok(*p.grid)) return false;
+    const Grid &g = *p.grid;
     for (const Run &r : p.runs)
         if (!(r.lo == -r.hi || (int64_t)(r.hi - r.lo + 1) == g.n[0])) return false;
     return true;
@@ -56,6 +61,16 @@ void build_ring(Plan &p, hipStream_t s) {
     // offsets (dz, dy), sorted
     std::map<std::pair<int, int>, int> offs;
     for (const Run &r : p.runs) offs[{r.dz, r.dy}] = 0;
+    // the contraction walks each dz group as one contiguous dy range: fill any gap with an offset
+    // whose Ŵ stays zero
+    std::map<int, std::pair<int, int>> dy_range;
+    for (const auto &kv : offs) {
+        auto it = dy_range.find(kv.first.first);
+        if (it == dy_range.end()) dy_range[kv.first.first] = {kv.first.second, kv.first.second};
+        else it->second = {std::min(it->second.first, kv.first.second), std::max(it->second.second, kv.first.second)};
+    }
+    for (const auto &kv : dy_range)
+        for (int dy = kv.second.first; dy <= kv.second.second; dy++) offs[{kv.first, dy}] = 0;
     std::vector<int32_t> off_dy, dz_val, dz_off0;
     int idx = 0, dymin = 0, dymax = 0;
     int last_dz = INT32_MIN;
@@ -85,6 +100,21 @@ void build_ring(Plan &p, hipStream_t s) {
         tw[k] = make_double2(std::cos(a), -std::sin(a));
     }
     R.tw_d = up(tw);
+    std::vector<double2> tws;
+    int Ns = 1;
+    for (int st = 0; st < R.fft.nstage; st++) {
+        const int Rr = R.fft.radix[st];
+        R.fft.twoff[st] = (int)tws.size();
+        for (int k = 0; k < Ns; k++)
+            for (int r = 1; r < Rr; r++) {
+                const double a = 2.0 * M_PI * (double)r * (double)k / ((double)Ns * (double)Rr);
+                tws.push_back(make_double2(std::cos(a), -std::sin(a)));
+            }
+        Ns *= Rr;
+    }
+    if (tws.empty()) tws.push_back(make_double2(1.0, 0.0));
+    R.tws_d = up(tws);
+    R.fft.nseq = ring_staged_nseq(R.fft);
     R.off_dy_d = up(off_dy);
     R.dz_val_d = up(dz_val);
     R.dz_off0_d = up(dz_off0);
@@ -93,6 +123,7 @@ void build_ring(Plan &p, hipStream_t s) {
     A.ny = g.n[1];
     A.nz = g.n[2];
     A.nrows = g.n[1] * g.n[2];
+    A.mh = R.mh;
     A.py = g.periodic[1];
     A.pz = g.ndim == 3 ? g.periodic[2] : 0;
     A.noff = idx;
@@ -115,6 +146,7 @@ void build_ring(Plan &p, hipStream_t s) {
 
 void free_ring(RingPlan &R) {
     if (R.tw_d) (void)hipFree(R.tw_d);
+    if (R.tws_d) (void)hipFree(R.tws_d);
     if (R.what_d) (void)hipFree(R.what_d);
     if (R.dz_val_d) (void)hipFree(R.dz_val_d);
     if (R.dz_off0_d) (void)hipFree(R.dz_off0_d);

@@ -75,6 +75,9 @@ typedef struct {
     int64_t max_taps;
     double radius;            /* kernel_radius(kernel, scale) */
     double active_taps;       /* in-support taps summed over ACTIVE target points */
+    int64_t ring_offsets;     /* ring engine: distinct (dz, dy) source-row offsets; 0 otherwise */
+    int64_t ring_modes;       /* ring engine: Fourier modes kept along x (N/2 + 1); 0 otherwise */
+    int64_t ring_pi_fields;   /* ring engine: planes filtered per snapshot by compute_pi; 0 otherwise */
 } cgef_plan_info;
 
 const char *cgef_version(void);
@@ -117,6 +120,15 @@ cgef_status cgef_compute_pi(cgef_plan *plan, cgef_workspace *ws, void *Pi, const
                             const void *v, const void *w, int64_t nbatch, int32_t dtype,
                             double rho0, double *E_out, void *ubar, void *vbar, void *wbar,
                             void *stream);
+
+/* Pipeline.jl:412-468 coarse_grain! with the GPU batch driver of Pipeline.jl:519-563: Π for nscales
+ * plans of one grid over nbatch slices in one call. Pi is (nbatch, nscales, grid…) in C order — the
+ * reference's column-major (grid…, Nscales, batch…) — and E_out (host, may be NULL) receives
+ * E(ℓ) as (nbatch × nscales, scale fastest). With ring-engine plans the velocity and its products
+ * are transformed once for all scales. Synchronizes the stream when E_out is requested. */
+cgef_status cgef_coarse_grain(cgef_plan *const *plans, int64_t nscales, cgef_workspace *ws, void *Pi,
+                              const void *u, const void *v, const void *w, int64_t nbatch, int32_t dtype,
+                              double rho0, double *E_out, void *stream);
 
 /* Derivatives.jl:20-144 ddx!/ddy!/ddz!: derivative w.r.t. distance along axis 0/1/2 of nbatch
  * slices (3-node stencil, masked cells 0). */

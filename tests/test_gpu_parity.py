@@ -305,10 +305,11 @@ def test_ring_filter_sphere(strategy, dtype, tol):
         assert np.max(np.abs(got[b] - ref)) <= tol * np.max(np.abs(ref))
 
 
-@pytest.mark.parametrize("nx,ny,ell", [(45, 30, 9000.0), (56, 24, 8000.0), (32, 32, 40000.0)])
+@pytest.mark.parametrize("nx,ny,ell", [(45, 30, 9000.0), (56, 24, 8000.0), (32, 32, 40000.0), (44, 20, 7000.0),
+                                       (26, 18, 5000.0)])
 def test_ring_filter_cartesian_odd_sizes_and_multiwrap(nx, ny, ell):
-    # odd length (radices 3·3·5), a radix-7 length, and a radius larger than the box (runs that
-    # wrap the ring more than once)
+    # odd length (radices 3·3·5), a radix-7 length, a radius larger than the box (runs that wrap the
+    # ring more than once), and radix-11 / radix-13 lengths (served by the LDS ping-pong FFT)
     xs, ys = cart_axes(nx, 1000.0), cart_axes(ny, 1000.0)
     mask = blob_mask((ny, nx), 41)
     og = oracle_grid_cart(xs, ys, periodic=(True, True), mask=mask)
@@ -339,7 +340,7 @@ def test_ring_pi(geom, strategy, dtype):
         og, g = volume_case(True, True)
         ker, kw = cgef.TopHatKernel(), dict(kind=O.TOPHAT, alpha=0.0, ell=6.0, engine=O.ENGINE_ND3)
         fields = smooth_random_field(g.field_shape, 22, 3)
-    fields = [f + 0.5 for f in fields]                         # a mean flow exercises the shift
+    fields = [f + 0.5 for f in fields]        # a mean flow: τ is a small difference of large terms
     ins = [T(f, dtype) for f in fields]
     plan = cgef.plan_filter(g, ker, kw["ell"], mask_strategy=s_cg, engine="ring")
     Pi = torch.empty(g.field_shape, dtype=dtype, device=DEV)
@@ -353,3 +354,48 @@ def test_ring_pi(geom, strategy, dtype):
     assert rel_l2(N(ws.u_filt), ub) <= (1e-12 if dtype == torch.float64 else 1e-7)
     Eo = O.energy(og, ub, vb, wb if geom == "volume" else None)
     assert abs(E - Eo) <= 1e-12 * abs(Eo)
+
+
+# --------------------------------------------------------------------------------- pipeline --
+
+def test_coarse_grain_matches_oracle_and_fallback():
+    og, g, _, _ = sphere_case(True)
+    u, v = smooth_random_field(g.field_shape, 23, 2)
+    scales = [8e5, 1.2e6, 1.8e6]
+    ker = cgef.GaussianKernel(4.0)
+    res = cgef.coarse_grain(T(u), T(v), None, g, scales=scales, kernel=ker)
+    Eo = []
+    for s_idx, ell in enumerate(scales):
+        ref, ub, vb, _ = O.compute_pi(og, u, v, kind=O.GAUSSIAN, alpha=4.0, ell=ell)
+        assert rel_l2(N(res.Pi[s_idx]), ref) <= 1e-10
+        Eo.append(O.energy(og, ub, vb))
+        assert abs(res.cumulative_energy[s_idx] - Eo[-1]) <= 1e-12 * abs(Eo[-1])
+    k = 1.0 / np.asarray(scales)
+    assert np.allclose(res.wavenumber, k)
+    assert np.allclose(res.filtering_spectrum, O.spectral_density(np.asarray(Eo), k), rtol=1e-9)
+    # the reference's default kernel (top-hat) cannot give a spectral density under StrictSpectrum
+    with pytest.raises(cgef.ArgumentError):
+        cgef.coarse_grain(T(u), T(v), None, g, scales=scales)
+    nos = cgef.coarse_grain(T(u), T(v), None, g, scales=scales, spectrum=cgef.NoSpectrum())
+    assert np.all(np.isnan(nos.filtering_spectrum))
+    # non-ring plans take the per-scale path and agree
+    plans = [cgef.plan_filter(g, ker, s, engine="banded") for s in scales]
+    res2 = cgef.pipeline._allocate_result(g, len(scales), torch.float64, DEV)
+    cgef.coarse_grain_(res2, T(u), T(v), None, g, scales=scales, kernel=ker, filter_plans=plans)
+    assert rel_l2(N(res2.Pi), N(res.Pi)) <= 1e-10
+
+
+def test_coarse_grain_batch_equals_slices():
+    og, g, _, _ = sphere_case(True)
+    fs = smooth_random_field((2, 3) + g.field_shape, 24, 2)
+    U, V = T(fs[0], torch.float32), T(fs[1], torch.float32)
+    scales = [9e5, 1.5e6]
+    ker = cgef.GaussianKernel(4.0)
+    batch = cgef.CoarseGrainBatchResult.allocate(g, len(scales), (2, 3), torch.float32, DEV)
+    cgef.coarse_grain_batch_(batch, U, V, None, g, scales=scales, kernel=ker)
+    for b0 in range(2):
+        for b1 in range(3):
+            r = cgef.coarse_grain(U[b0, b1].contiguous(), V[b0, b1].contiguous(), None, g, scales=scales, kernel=ker)
+            assert torch.equal(r.Pi, batch.Pi[b0, b1])
+            assert np.array_equal(r.cumulative_energy, batch.cumulative_energy[b0, b1])
+            assert np.allclose(r.filtering_spectrum, batch.filtering_spectrum[b0, b1])
