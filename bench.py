@@ -128,6 +128,108 @@ def batched_leg(args, grid, mask, lam, phi):
     return out
 
 
+def cartesian_leg(args):
+    """Plain method at small size: 256×256 doubly periodic Cartesian, L = 1000 km, FP64, no mask,
+    Gaussian α = 6, ℓ = 8Δx, ZeroFill, Π with ρ₀ = 1025. The library's default engine here is the
+    reference's (the separable Gaussian); the disk-footprint engines are timed beside it. Parity of
+    each against the oracle's matching engine."""
+    import numpy as np
+    import torch
+
+    import cgef
+    from cgef import synthetic as S
+
+    dev = "cuda:0"
+    N, L = 256, 1.0e6
+    dx = L / N
+    ell = 8 * dx
+    u, v, xs = S.cartesian_turbulence(N=N, L=L, seed=0)
+    grid = cgef.StructuredGrid(cgef.CartesianGeometry(), xs, xs, periodic=(True, True))
+    ker = cgef.GaussianKernel(6.0)
+    U, V = torch.tensor(u, device=dev), torch.tensor(v, device=dev)
+    Pi = torch.empty_like(U)
+    ws = cgef.PiWorkspace(grid, (), torch.float64)
+    out = {"workload": "cartesian 256x256 doubly periodic FP64, Gaussian alpha=6, l=8dx, rho0=1025"}
+    refs = {}
+    if not args.no_cpu:
+        from oracle import oracle as O
+        og = O.Grid([xs, xs], periodic=[True, True], period=[L, L], uniform=[True, True])
+        ncores = min(len(os.sched_getaffinity(0)), 16)
+        for eng, oeng in (("separable", O.ENGINE_SEPARABLE), ("banded", O.ENGINE_BANDED)):
+            t0 = time.perf_counter()
+            refs[eng] = 1025.0 * O.compute_pi(og, u, v, kind=O.GAUSSIAN, alpha=6.0, ell=ell, engine=oeng,
+                                              nthreads=ncores)[0]
+            out[f"cpu_{eng}_s"] = time.perf_counter() - t0
+        refs["ring"] = refs["banded"]
+        out["cpu_cores"] = ncores
+    for eng in ("auto", "banded", "ring"):
+        plan = cgef.plan_filter(grid, ker, ell, engine=eng)
+        steps = 20
+
+        def step():
+            cgef.compute_pi_(Pi, U, V, None, grid, ker, ell, workspace=ws, filter_plan=plan, rho0=1025.0)
+
+        el, _ = timed_steps(step, steps, 3)
+        r = {"engine": plan.engine, "ms_per_step": el * 1e3 / steps, "value": grid.npoints * steps / el,
+             "unit": "point-scales/s"}
+        if plan.engine in refs:
+            ref = refs[plan.engine]
+            got = Pi.cpu().numpy()
+            r["rel_l2_vs_fp64_oracle"] = float(np.sqrt(np.sum((got - ref) ** 2) / np.sum(ref ** 2)))
+        out[eng] = r
+        log(f"cartesian 256^2 {plan.engine}: {r['ms_per_step']:.3f} ms/step")
+    return out
+
+
+def volume_leg(args):
+    """True 3-D: 256×256×64, Δx = Δy = 1, Δz_k = Δz₁·1.03^(k−1) with Δz = 4 at mid-depth (so the
+    top-hat ball of radius ℓ/2 = 8 spans ±2 levels there), x/y periodic, z bounded, FP32, top-hat
+    ℓ = 16Δx, ZeroFill, no mask. Parity against the oracle's 3-D ball engine."""
+    import numpy as np
+    import torch
+
+    import cgef
+    from cgef import synthetic as S
+
+    dev = "cuda:0"
+    u, v, w = S.volume_turbulence()
+    Nz, Ny, Nx = u.shape
+    xs, ys = np.arange(Nx, dtype=np.float64), np.arange(Ny, dtype=np.float64)
+    zs = S.stretched_levels(Nz)
+    grid = cgef.StructuredGrid(cgef.CartesianGeometry(), xs, ys, zs, periodic=(True, True, False))
+    ker, ell = cgef.TopHatKernel(), 16.0
+    U, V, W = (torch.tensor(a, dtype=torch.float32, device=dev) for a in (u, v, w))
+    Pi = torch.empty_like(U)
+    ws = cgef.PiWorkspace(grid, (), torch.float32)
+    plan = cgef.plan_filter(grid, ker, ell)
+    steps = 10
+
+    def step():
+        cgef.compute_pi_(Pi, U, V, W, grid, ker, ell, workspace=ws, filter_plan=plan)
+
+    el, _ = timed_steps(step, steps, 2)
+    out = {"workload": "3-D 256x256x64 stretched z (x1.03), FP32, top-hat l=16dx, x/y periodic",
+           "engine": plan.engine, "ms_per_step": el * 1e3 / steps, "value": grid.npoints * steps / el,
+           "unit": "point-scales/s", "mean_taps": plan.info.mean_taps}
+    log(f"volume 256x256x64 {plan.engine}: {out['ms_per_step']:.3f} ms/step")
+    if not args.no_cpu:
+        from oracle import oracle as O
+        og = O.Grid([xs, ys, zs], periodic=[True, True, False], period=[float(Nx), float(Ny), 0.0],
+                    uniform=[True, True, False])
+        ncores = min(len(os.sched_getaffinity(0)), 16)
+        t0 = time.perf_counter()
+        ref = O.compute_pi(og, *(a.cpu().double().numpy() for a in (U, V, W)), kind=O.TOPHAT, alpha=0.0, ell=ell,
+                           engine=O.ENGINE_ND3, nthreads=ncores)[0]
+        out["cpu_s"] = time.perf_counter() - t0
+        out["cpu_cores"] = ncores
+        got = Pi.cpu().double().numpy()
+        out["rel_l2_vs_fp64_oracle"] = float(np.sqrt(np.sum((got - ref) ** 2) / np.sum(ref ** 2)))
+        out["max_abs_err"] = float(np.max(np.abs(got - ref)))
+    del U, V, W, Pi, ws, plan
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -137,6 +239,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extra", action="store_true", help="skip the secondary GPU engines")
     ap.add_argument("--no-batched", action="store_true", help="skip the 8-snapshot × 24-scale batched leg")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the 256² Cartesian and 256×256×64 volume legs")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 512 MiB → 512 MiB FP32 elementwise copy first, so that hardware counter runs "
                          "can scale FETCH_SIZE / WRITE_SIZE by a kernel of known traffic")
@@ -236,6 +340,9 @@ def main():
 
     if not args.no_batched:
         extra["batched_spectrum"] = batched_leg(args, grid, mask, lam, phi)
+    if not args.no_configs:
+        extra["cartesian_256"] = cartesian_leg(args)
+        extra["volume_256x256x64"] = volume_leg(args)
 
     cpu = None
     if not args.no_cpu:
