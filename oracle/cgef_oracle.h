@@ -7,6 +7,16 @@
  *
  * Layout convention (the reference's): column-major, i (x / longitude) fastest, then j
  * (y / latitude), then k (z), then trailing batch. 0-based indices here.
+ *
+ * Pinning: tests/test_oracle.py checks these functions against the reference's own known answers,
+ * closed forms, invariants and brute force. What only the un-vendored FlowGeometries.jl could fix
+ * is "parity unpinned", with the choice made here (the HIP kernels make the same one):
+ *   - og_deriv at domain and mask edges: pinned only to exactness for linear fields; here a 3-node
+ *     one-sided stencil when two same-side neighbours are active, 2 nodes when one is, 0 if none;
+ *   - og_cell_width / og_area on non-uniform axes: midpoint widths, half cells at non-periodic ends;
+ *   - periodicity: explicit flags (no auto-detection);
+ *   - og_haversine rounding at a spherical top-hat rim;
+ *   - inputs are promoted to FP64 (no mixed-precision promotion rules).
  */
 #ifndef CGEF_ORACLE_H
 #define CGEF_ORACLE_H
