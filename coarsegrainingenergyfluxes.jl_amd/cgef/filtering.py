@@ -166,3 +166,19 @@ def filter_field_(out, field, grid, kernel, scale, *, mask_strategy=ZeroFill(), 
     plan = filter_plan if filter_plan is not None else plan_filter(
         grid, kernel, scale, mask_strategy=mask_strategy, backend=backend, method=method, engine=engine)
     return filter_apply_(out, field, plan)
+
+
+def filter_slices_(outs, fields, plans, *, backend=None):
+    """Filtering.jl:3433-3470 filter_slices!: apply plans[t] to fields[t] into outs[t] over
+    independent slices, each with its own grid and plan. On one GPU the slices run back to back on
+    the current stream, longest first (slice_costs)."""
+    if not (len(outs) == len(fields) == len(plans)):
+        raise DimensionMismatch(f"filter_slices! got {len(outs)} outputs, {len(fields)} fields and {len(plans)} plans")
+    for t in sorted(range(len(plans)), key=lambda q: -slice_costs([plans[q]])[0]):
+        filter_apply_(outs[t], fields[t], plans[t])
+    return outs
+
+
+def slice_costs(plans):
+    """Filtering.jl:3486-3491: per-slice work proxy, the number of target points each plan writes."""
+    return [int(p.grid.npoints) for p in plans]

@@ -142,3 +142,51 @@ def coarse_grain_batch_(batch: CoarseGrainBatchResult, u, v, w, grid: Structured
         _fill_spectrum(row, flat_c[q], flat_k[q], spectrum)
         flat_g[q] = row
     return batch
+
+
+def coarse_grain_profile_(batch: CoarseGrainBatchResult, u, v, w, grid: StructuredGrid, **kwargs):
+    """Pipeline.jl:991-1010 coarse_grain_profile!: independent per-level 2-D sweeps on a shared
+    horizontal grid, i.e. coarse_grain_batch! with the level axis as the batch axis (u, v[, w] are
+    (Nlevels, Ny, Nx); Π is (Nlevels, Nscales, Ny, Nx); per-level energies are not summed)."""
+    if grid.ndim != 2:
+        raise _lib.ArgumentError("coarse_grain_profile! takes a 2-D horizontal grid")
+    for name, t in (("u", u), ("v", v), ("w", w)):
+        if t is not None and t.dim() != 3:
+            raise _lib.DimensionMismatch(f"{name} must be (Nlevels, Ny, Nx), got {tuple(t.shape)}")
+    return coarse_grain_batch_(batch, u, v, w, grid, **kwargs)
+
+
+def coarse_grain_profile(u, v, w, grid: StructuredGrid, *, scales, **kwargs):
+    """Pipeline.jl:1012-1034: allocating coarse_grain_profile!, batch size (Nlevels,)."""
+    batch = CoarseGrainBatchResult.allocate(grid, len(scales), (u.shape[0],), u.dtype, u.device)
+    return coarse_grain_profile_(batch, u, v, w, grid, scales=scales, **kwargs)
+
+
+def slice_pipeline_costs(grids, nscales: int):
+    """Pipeline.jl:838-839: relative per-slice sweep cost (points × scales) for longest-first order."""
+    return [int(g.npoints) * int(nscales) for g in grids]
+
+
+def coarse_grain_slices_(results, us, vs, ws, grids, *, scales, kernel=TopHatKernel(), workspaces=None,
+                         filter_plans=None, deriv_plans=None, backend=None, mask_strategy=ZeroFill(), L: float = 1.0,
+                         spectrum=StrictSpectrum(), rho0: float = 1.0):
+    """Pipeline.jl:866-919 coarse_grain_slices!: a full coarse_grain! sweep per slice, each slice with
+    its own grid; scratch is per slice. Slices run longest-first (slice_pipeline_costs)."""
+    n = len(results)
+    if not (len(us) == n and len(vs) == n and len(grids) == n):
+        raise _lib.DimensionMismatch(
+            f"coarse_grain_slices! got {n} results, {len(us)} u, {len(vs)} v and {len(grids)} grids")
+    if ws is not None and len(ws) != n:
+        raise _lib.DimensionMismatch(f"coarse_grain_slices! got {n} results and {len(ws)} w slices")
+    for name, pool in (("workspaces", workspaces), ("filter_plans", filter_plans), ("deriv_plans", deriv_plans)):
+        if pool is not None and len(pool) != n:
+            raise _lib.DimensionMismatch(
+                f"coarse_grain_slices! got {n} results and {len(pool)} {name} entries — ragged scratch is per slice")
+    gate_spectrum(kernel, spectrum)
+    costs = slice_pipeline_costs(grids, len(scales))
+    for t in sorted(range(n), key=lambda q: -costs[q]):
+        coarse_grain_(results[t], us[t], vs[t], None if ws is None else ws[t], grids[t], scales=scales,
+                      kernel=kernel, workspace=None if workspaces is None else workspaces[t],
+                      filter_plans=None if filter_plans is None else filter_plans[t], backend=backend,
+                      mask_strategy=mask_strategy, L=L, spectrum=spectrum, rho0=rho0)
+    return results

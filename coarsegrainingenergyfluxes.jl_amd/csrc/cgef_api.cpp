@@ -145,7 +145,7 @@ static void lag3_weights(double x, double a, double b, double c, double *w) {
 // on the mask and is decided on the device; the weights depend only on the axis index.
 static std::vector<double> deriv_table(const Grid &g, int axis) {
     const int64_t n = g.n[axis];
-    std::vector<double> tab((size_t)n * DCOEF_STRIDE, 0.0);
+    std::vector<double> tab((size_t)n * DCOEF_SLOTS, 0.0);
     if (n < 2) return tab;
     const bool per = g.periodic[axis] && n >= 3;
     const std::vector<double> &x = g.coord[axis];
@@ -167,7 +167,7 @@ static std::vector<double> deriv_table(const Grid &g, int axis) {
             }
             xc[o + 2] = x[q] + sh;
         }
-        double *w = tab.data() + p * DCOEF_STRIDE;
+        double w[DCOEF_SLOTS] = {};
         if (ok[1] && ok[3]) lag3_weights(xc[2], xc[1], xc[2], xc[3], w);
         if (ok[3] && ok[4]) lag3_weights(xc[2], xc[2], xc[3], xc[4], w + 3);
         if (ok[0] && ok[1]) lag3_weights(xc[2], xc[0], xc[1], xc[2], w + 6);
@@ -179,6 +179,7 @@ static std::vector<double> deriv_table(const Grid &g, int axis) {
             w[12] = 1.0 / (xc[2] - xc[1]);
             w[11] = -w[12];
         }
+        for (int sl = 0; sl < DCOEF_SLOTS; sl++) tab[(size_t)sl * n + p] = w[sl];
     }
     return tab;
 }

@@ -31,10 +31,11 @@ struct alignas(32) Run {
 
 struct Workspace;
 
-// First-derivative stencil weights per axis index p (DCOEF_STRIDE doubles): [0,3) centred nodes
-// p−1, p, p+1; [3,6) forward p, p+1, p+2; [6,9) backward p−2, p−1, p; [9,11) two-node forward p, p+1;
-// [11,13) two-node backward p−1, p. Periodic axes (n ≥ 3) wrap index and coordinate.
-constexpr int DCOEF_STRIDE = 16;
+// First-derivative stencil weights of an axis of n points, DCOEF_SLOTS arrays of n doubles (slot s of
+// index p at s·n + p, so lanes on consecutive p read consecutive words): slots 0-2 centred nodes
+// p−1, p, p+1; 3-5 forward p, p+1, p+2; 6-8 backward p−2, p−1, p; 9-10 two-node forward p, p+1;
+// 11-12 two-node backward p−1, p. Periodic axes (n ≥ 3) wrap index and coordinate.
+constexpr int DCOEF_SLOTS = 13;
 
 // Ring engine plan (see cgef_ring.hip): mixed-radix factorization of the row length, the (dz, dy)
 // offsets of the footprint and the Fourier weights Ŵ[m][offset][band].
@@ -45,6 +46,7 @@ struct RingFft {
     int nstage = 0;
     int radix[RING_MAX_STAGES] = {};
     int twoff[RING_MAX_STAGES] = {};  // offset of stage st in the per-stage twiddle table
+    int twsize = 0;                    // entries of the per-stage twiddle table
     int nseq = 0;                      // complex sequences per block of the register-staged row FFT
                                        // (0: not supported for this length, the LDS ping-pong FFT serves)
 };
@@ -92,7 +94,7 @@ struct Grid {
     double *coord_d[3] = {nullptr, nullptr, nullptr};
     double *width_d[3] = {nullptr, nullptr, nullptr};
     double *trig_d = nullptr;     // sphere: [sinλ(Nx) cosλ(Nx) sinφ(Ny) cosφ(Ny)]
-    double *dcoef_d[3] = {nullptr, nullptr, nullptr};  // derivative stencil weights, DCOEF_STRIDE per index
+    double *dcoef_d[3] = {nullptr, nullptr, nullptr};  // derivative stencil weights (DCOEF_SLOTS × n)
     std::atomic<int> refs{1};
     std::mutex ws_mu;
     Workspace *internal_ws = nullptr;

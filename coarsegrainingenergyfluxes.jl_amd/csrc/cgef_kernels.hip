@@ -406,12 +406,12 @@ __device__ __forceinline__ Stencil stencil_axis(const GeomArgs &G, int axis, int
     };
     int64_t im1 = t, ip1 = t, im2 = t, ip2 = t;
     const bool okm1 = nb(-1, im1), okp1 = nb(1, ip1);
-    const double *w = G.dcoef[axis] + p * DCOEF_STRIDE;
-    if (okm1 && okp1) return Stencil{im1, t, ip1, w[0], w[1], w[2]};
-    if (okp1 && nb(2, ip2)) return Stencil{t, ip1, ip2, w[3], w[4], w[5]};
-    if (okm1 && nb(-2, im2)) return Stencil{im2, im1, t, w[6], w[7], w[8]};
-    if (okp1) return Stencil{t, ip1, t, w[9], w[10], 0.0};
-    if (okm1) return Stencil{im1, t, t, w[11], w[12], 0.0};
+    const double *w = G.dcoef[axis] + p;  // slot s at w[s·n]
+    if (okm1 && okp1) return Stencil{im1, t, ip1, w[0], w[n], w[2 * n]};
+    if (okp1 && nb(2, ip2)) return Stencil{t, ip1, ip2, w[3 * n], w[4 * n], w[5 * n]};
+    if (okm1 && nb(-2, im2)) return Stencil{im2, im1, t, w[6 * n], w[7 * n], w[8 * n]};
+    if (okp1) return Stencil{t, ip1, t, w[9 * n], w[10 * n], 0.0};
+    if (okm1) return Stencil{im1, t, t, w[11 * n], w[12 * n], 0.0};
     return S;
 }
 
@@ -426,10 +426,10 @@ struct RowChunk {
     int64_t i, j, k, r, t;
 };
 
-__device__ __forceinline__ RowChunk row_chunk(const GeomArgs &G, int64_t nchunk) {
+__device__ __forceinline__ RowChunk row_chunk(const GeomArgs &G, int64_t nchunk, int64_t item) {
     RowChunk c;
-    c.r = (int64_t)blockIdx.x / nchunk;
-    c.i = ((int64_t)blockIdx.x - c.r * nchunk) * 256 + threadIdx.x;
+    c.r = item / nchunk;
+    c.i = (item - c.r * nchunk) * 256 + threadIdx.x;
     c.j = c.r % G.n[1];
     c.k = c.r / G.n[1];
     c.t = c.r * G.n[0] + c.i;
@@ -437,6 +437,16 @@ __device__ __forceinline__ RowChunk row_chunk(const GeomArgs &G, int64_t nchunk)
 }
 
 int64_t row_chunks(const GeomArgs &G) { return (G.n[0] + 255) / 256; }
+
+// Block → work item for per-point kernels with stencils: the launch holds 8·⌈nblk/8⌉ blocks, which
+// the hardware deals round-robin over the 8 XCDs; each XCD takes a contiguous range of items, so a
+// row and its stencil neighbours are read through the same L2.
+__device__ __forceinline__ int64_t xcd_block(int64_t nblk) {
+    const int64_t chunk = (nblk + 7) / 8;
+    return (int64_t)(blockIdx.x % 8) * chunk + blockIdx.x / 8;
+}
+
+static unsigned xcd_grid(int64_t nblk) { return (unsigned)(8 * ((nblk + 7) / 8)); }
 
 // Strain S̄ (with the spherical curvature terms, Diagnostics.jl:652-669), contraction
 // (Diagnostics.jl:460-466, 683-698, 848-855), Π = −ρ₀ S̄:τ, and per-block partial energies
@@ -447,7 +457,9 @@ __global__ __launch_bounds__(256) void k_strain_pi(GeomArgs G, const double *__r
                                                    int64_t pi_stride, int64_t nchunk) {
     __shared__ double sh[4];
     const int64_t b = blockIdx.y;
-    const RowChunk c = row_chunk(G, nchunk);
+    const int64_t item = xcd_block(nblk);
+    if (item >= nblk) return;  // whole block: padding of the XCD-ordered launch
+    const RowChunk c = row_chunk(G, nchunk, item);
     const int64_t t = c.t, P = G.npts, Nx = G.n[0], Ny = G.n[1];
     double e = 0.0;
     if (c.i < Nx) {
@@ -500,7 +512,7 @@ __global__ __launch_bounds__(256) void k_strain_pi(GeomArgs G, const double *__r
     }
     if (partials) {
         const double s = block_sum<256>(e, sh);
-        if (threadIdx.x == 0) partials[b * nblk + blockIdx.x] = s;
+        if (threadIdx.x == 0) partials[b * nblk + item] = s;
     }
 }
 
@@ -510,7 +522,7 @@ void launch_strain_pi(const GeomArgs &G, int dtype, const double *ubar, const vo
                       int64_t nbatch, double *partials, int64_t npartials, hipStream_t s, int64_t pi_stride) {
     const int64_t nblk = strain_blocks(G);
     double *pp = (partials && nblk * nbatch <= npartials) ? partials : nullptr;
-    dim3 grid((unsigned)nblk, (unsigned)nbatch);
+    dim3 grid(xcd_grid(nblk), (unsigned)nbatch);
     const int64_t ps = pi_stride > 0 ? pi_stride : G.npts;
     const int64_t nch = row_chunks(G);
 #define CGEF_SP(TT, MM) hipLaunchKernelGGL((k_strain_pi<TT, MM>), grid, dim3(256), 0, s, G, ubar, (const TT *)tau, (TT *)Pi, rho0, pp, nblk, ps, nch)
@@ -545,7 +557,9 @@ __global__ __launch_bounds__(256) void k_energy_fields(GeomArgs G, const T *ub, 
                                                        int64_t nblk, int64_t nchunk) {
     __shared__ double sh[4];
     const int64_t b = blockIdx.y;
-    const RowChunk c = row_chunk(G, nchunk);
+    const int64_t item = xcd_block(nblk);
+    if (item >= nblk) return;
+    const RowChunk c = row_chunk(G, nchunk, item);
     const int64_t P = G.npts, t = c.t;
     double e = 0.0;
     if (c.i < G.n[0] && G.mask[t]) {
@@ -558,14 +572,14 @@ __global__ __launch_bounds__(256) void k_energy_fields(GeomArgs G, const T *ub, 
         e = 0.5 * v2 * cell_area(G, c.i, c.j, c.k);
     }
     const double s = block_sum<256>(e, sh);
-    if (threadIdx.x == 0) partials[b * nblk + blockIdx.x] = s;
+    if (threadIdx.x == 0) partials[b * nblk + item] = s;
 }
 
 void launch_energy_fields(const GeomArgs &G, int dtype, const void *ub, const void *vb, const void *wb, int64_t nbatch,
                           double *partials, int64_t npartials, hipStream_t s) {
     const int64_t nblk = strain_blocks(G);
     (void)npartials;
-    dim3 grid((unsigned)nblk, (unsigned)nbatch);
+    dim3 grid(xcd_grid(nblk), (unsigned)nbatch);
     if (dtype == CGEF_F32)
         hipLaunchKernelGGL(k_energy_fields<float>, grid, dim3(256), 0, s, G, (const float *)ub, (const float *)vb,
                            (const float *)wb, partials, nblk, row_chunks(G));
@@ -577,7 +591,10 @@ void launch_energy_fields(const GeomArgs &G, int dtype, const void *ub, const vo
 template <typename T>
 __global__ __launch_bounds__(256) void k_derivative(GeomArgs G, int axis, const T *in, T *out, int64_t nchunk) {
     const int64_t b = blockIdx.y;
-    const RowChunk c = row_chunk(G, nchunk);
+    const int64_t nblk = nchunk * G.n[1] * G.n[2];
+    const int64_t item = xcd_block(nblk);
+    if (item >= nblk) return;
+    const RowChunk c = row_chunk(G, nchunk, item);
     const int64_t P = G.npts, t = c.t, Nx = G.n[0];
     if (c.i >= Nx) return;
     double v = 0.0;
@@ -597,7 +614,7 @@ __global__ __launch_bounds__(256) void k_derivative(GeomArgs G, int axis, const 
 
 void launch_derivative(const GeomArgs &G, int axis, int dtype, const void *in, void *out, int64_t nbatch, hipStream_t s) {
     const int64_t nch = row_chunks(G);
-    dim3 grid((unsigned)(nch * G.n[1] * G.n[2]), (unsigned)nbatch);
+    dim3 grid(xcd_grid(nch * G.n[1] * G.n[2]), (unsigned)nbatch);
     if (dtype == CGEF_F32)
         hipLaunchKernelGGL(k_derivative<float>, grid, dim3(256), 0, s, G, axis, (const float *)in, (float *)out, nch);
     else

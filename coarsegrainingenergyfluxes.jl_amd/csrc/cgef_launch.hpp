@@ -23,7 +23,7 @@ struct GeomArgs {
     const double *coord[3];
     const double *width[3];
     const double *trig;    // spherical: sinλ[Nx], cosλ[Nx], sinφ[Ny], cosφ[Ny]
-    const double *dcoef[3];  // derivative stencil weights per axis index (DCOEF_STRIDE each)
+    const double *dcoef[3];  // derivative stencil weights, DCOEF_SLOTS arrays of n[axis] each
     double active_area;
 };
 

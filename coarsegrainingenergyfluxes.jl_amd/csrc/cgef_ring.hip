@@ -375,23 +375,19 @@ int ring_staged_nseq(const RingFft &F) {
 }
 
 // Banded contraction per Fourier mode m. Block: 256 target rows of one plane (j0..j0+255, level k)
-// × a chunk of CC columns, 256/RB threads, each accumulating RB consecutive target rows so that every
-// staged source value feeds RB rows (the offsets of a dz group form a contiguous dy range, so source
-// row s serves target row q at offset s − q). Source rows of each dz plane are staged in LDS (zero
-// outside a non-periodic axis) split into RB planes by s mod RB, which keeps the lanes of a read on
-// consecutive addresses; Ŵ is read along rows (coalesced on the sphere, broadcast on
-// translation-invariant grids).
+// × a chunk of CC columns, one target row per lane. The source rows of each dz plane are staged in
+// LDS (zero outside a non-periodic axis) and every staged value feeds CC columns' worth of FMAs per
+// weight; Ŵ is read along rows (coalesced on the sphere, broadcast on translation-invariant grids),
+// in batches whose next batch is in flight while the current one is used — the first batch of a
+// plane is issued before its staging, so the weight latency hides behind the staging loads.
 #ifndef CGEF_CONTRACT_CC
 #define CGEF_CONTRACT_CC 5
 #endif
-#ifndef CGEF_CONTRACT_RB
-#define CGEF_CONTRACT_RB 1
-#endif
 
-template <int CC, int RB>
-__global__ __launch_bounds__(256 / RB) void k_contract(RingArgs A, const double2 *__restrict__ XF, double2 *__restrict__ OUT,
-                                                       int64_t ncol) {
-    constexpr int NT = 256 / RB;
+template <int CC>
+__global__ __launch_bounds__(256) void k_contract(RingArgs A, const double2 *__restrict__ XF, double2 *__restrict__ OUT,
+                                                  int64_t ncol) {
+    constexpr int PB = 8;
     extern __shared__ double2 st[];
     const int64_t Ny = A.ny, Nz = A.nz;
     const int64_t nyb = (Ny + 255) / 256;
@@ -405,23 +401,14 @@ __global__ __launch_bounds__(256 / RB) void k_contract(RingArgs A, const double2
     const int64_t m = item / (ncc * nyb * Nz);
     const int64_t k = rbk / nyb, j0 = (rbk % nyb) * 256;
     const int tid = threadIdx.x;
+    const int64_t j = j0 + tid;
+    const bool valid = j < Ny;
+    const int band = valid ? A.band_of_row[j + Ny * k] : 0;
     const int span = 256 + A.dy_max - A.dy_min;
-    const int sp = (span + RB - 1) / RB;                          // LDS plane length
     const int nvc = (int)((ncol - c0) < CC ? (ncol - c0) : CC);  // valid columns in this chunk
-    int band[RB];
-    bool valid[RB], anyv = false;
+    double2 acc[CC];
 #pragma unroll
-    for (int q = 0; q < RB; q++) {
-        const int64_t j = j0 + tid * RB + q;
-        valid[q] = j < Ny;
-        band[q] = valid[q] ? A.band_of_row[j + Ny * k] : 0;
-        anyv = anyv || valid[q];
-    }
-    double2 acc[RB][CC];
-#pragma unroll
-    for (int q = 0; q < RB; q++)
-#pragma unroll
-        for (int c = 0; c < CC; c++) acc[q][c] = make_double2(0.0, 0.0);
+    for (int c = 0; c < CC; c++) acc[c] = make_double2(0.0, 0.0);
     const int64_t mrow = (int64_t)m * A.noff;
     for (int g = 0; g < A.ndz; g++) {
         const int dz = A.dz_val[g];
@@ -430,8 +417,14 @@ __global__ __launch_bounds__(256 / RB) void k_contract(RingArgs A, const double2
             if (!A.pz) continue;
             kk = ((kk % Nz) + Nz) % Nz;
         }
+        const int o0 = A.dz_off0[g], ng = A.dz_off0[g + 1] - o0;
+        const int e0 = A.off_dy[o0] - A.dy_min;  // staged index of offset 0 for lane 0
+        const double *wrow = A.what + (mrow + o0) * A.nbands + band;
+        double wn[PB];
+#pragma unroll
+        for (int u = 0; u < PB; u++) wn[u] = (valid && u < ng) ? wrow[(int64_t)u * A.nbands] : 0.0;
         __syncthreads();
-        for (int q = tid; q < span * CC; q += NT) {
+        for (int q = tid; q < span * CC; q += 256) {
             const int c = q / span, s = q - c * span;
             int64_t jj = j0 + A.dy_min + s;
             double2 v = make_double2(0.0, 0.0);
@@ -441,53 +434,38 @@ __global__ __launch_bounds__(256 / RB) void k_contract(RingArgs A, const double2
                 else in = false;
             }
             if (in && c < nvc) v = XF[(m * ncol + c0 + c) * A.nrows + jj + Ny * kk];
-            st[(c * RB + s % RB) * sp + s / RB] = v;
+            st[c * span + s] = v;
         }
         __syncthreads();
-        if (anyv) {
-            const int o0 = A.dz_off0[g], ng = A.dz_off0[g + 1] - o0;
-            const int e0 = A.off_dy[o0] - A.dy_min;  // staged index of (thread row 0, offset 0) − tid·RB
-            const double *wrow = A.what + (mrow + o0) * A.nbands;
-            // weights in batches of PB source rows, loaded before use so their latency overlaps
-            constexpr int PB = 8;
-            for (int p0 = 0; p0 < ng + RB - 1; p0 += PB) {
-                double w[PB][RB];
+        if (!valid) continue;
+        const double2 *src = st + tid + e0;
+        for (int p0 = 0; p0 < ng; p0 += PB) {
+            double w[PB];
 #pragma unroll
-                for (int pp = 0; pp < PB; pp++)
+            for (int u = 0; u < PB; u++) w[u] = wn[u];
+            if (p0 + PB < ng) {
 #pragma unroll
-                    for (int q = 0; q < RB; q++) {
-                        const int o = p0 + pp - q;
-                        w[pp][q] = (o >= 0 && o < ng) ? wrow[(int64_t)o * A.nbands + band[q]] : 0.0;
-                    }
+                for (int u = 0; u < PB; u++) wn[u] = (p0 + PB + u < ng) ? wrow[(int64_t)(p0 + PB + u) * A.nbands] : 0.0;
+            }
 #pragma unroll
-                for (int pp = 0; pp < PB; pp++) {
-                    if (p0 + pp >= ng + RB - 1) break;
-                    // staged row tid·RB + e is the source of target row q at offset p0 + pp − q
-                    const int e = e0 + p0 + pp;
-                    const double2 *src = st + (e % RB) * sp + tid + e / RB;
+            for (int u = 0; u < PB; u++) {
+                if (p0 + u >= ng) break;
 #pragma unroll
-                    for (int c = 0; c < CC; c++) {
-                        const double2 v = src[c * RB * sp];
-#pragma unroll
-                        for (int q = 0; q < RB; q++) {
-                            acc[q][c].x = fma(w[pp][q], v.x, acc[q][c].x);
-                            acc[q][c].y = fma(w[pp][q], v.y, acc[q][c].y);
-                        }
-                    }
+                for (int c = 0; c < CC; c++) {
+                    const double2 v = src[c * span + p0 + u];
+                    acc[c].x = fma(w[u], v.x, acc[c].x);
+                    acc[c].y = fma(w[u], v.y, acc[c].y);
                 }
             }
         }
     }
-    // fully unrolled with predicates: loops bounded by nvc would index acc dynamically and move the
+    if (!valid) return;
+    // fully unrolled with a predicate: a loop bounded by nvc would index acc dynamically and move the
     // accumulators to scratch memory
+    const int64_t r = j + Ny * k;
 #pragma unroll
-    for (int q = 0; q < RB; q++) {
-        if (!valid[q]) continue;
-        const int64_t r = j0 + tid * RB + q + Ny * k;
-#pragma unroll
-        for (int c = 0; c < CC; c++)
-            if (c < nvc) OUT[(m * ncol + c0 + c) * A.nrows + r] = acc[q][c];
-    }
+    for (int c = 0; c < CC; c++)
+        if (c < nvc) OUT[(m * ncol + c0 + c) * A.nrows + r] = acc[c];
 }
 
 // Ŵ for every (band, offset): the run's weights laid out circularly on a length-N row, FFT'd two
@@ -700,6 +678,9 @@ __global__ __launch_bounds__(256) void k_ring_normalize(GeomArgs G, int dtype, c
 // ------------------------------------------------------------------------------------ launches ----
 
 static size_t fft_shmem(const RingFft &F) { return (size_t)2 * F.n * sizeof(double2); }
+static size_t staged_shmem(const RingFft &F, int nseq) {
+    return (size_t)nseq * F.n * sizeof(double2);
+}
 
 static void set_shmem(const void *fn, size_t bytes) {
     if (bytes > 64 * 1024) CGEF_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
@@ -717,7 +698,7 @@ void ring_build_what(const RingPlan &R, const Run *runs_d, const int32_t *run_of
 void ring_forward(const RingPlan &R, const double *xin, double2 *XF, int64_t ncol, hipStream_t s) {
     const int ns = R.fft.nseq;
     if (ns > 0) {
-        const size_t sh = (size_t)ns * R.fft.n * sizeof(double2);
+        const size_t sh = staged_shmem(R.fft, ns);
         const int64_t items = (R.args.nrows + 2 * ns - 1) / (2 * ns) * ncol;
         const dim3 grid((unsigned)(8 * ((items + 7) / 8)));
         hipLaunchKernelGGL(k_fft_rows_staged<1>, grid, dim3(256), sh, s, R.fft, R.tw_d, R.tws_d, xin, XF, R.args.nrows, ncol);
@@ -733,7 +714,7 @@ void ring_forward(const RingPlan &R, const double *xin, double2 *XF, int64_t nco
 void ring_inverse(const RingPlan &R, const double2 *OUT, double *yout, int64_t ncol, hipStream_t s) {
     const int ns = R.fft.nseq;
     if (ns > 0) {
-        const size_t sh = (size_t)ns * R.fft.n * sizeof(double2);
+        const size_t sh = staged_shmem(R.fft, ns);
         const int64_t items = (R.args.nrows + 2 * ns - 1) / (2 * ns) * ncol;
         const dim3 grid((unsigned)(8 * ((items + 7) / 8)));
         hipLaunchKernelGGL(k_ifft_rows_staged<1>, grid, dim3(256), sh, s, R.fft, R.tw_d, R.tws_d, OUT, yout, R.args.nrows, ncol);
@@ -747,14 +728,13 @@ void ring_inverse(const RingPlan &R, const double2 *OUT, double *yout, int64_t n
 }
 
 void ring_contract(const RingPlan &R, const double2 *XF, double2 *OUT, int64_t ncol, hipStream_t s) {
-    constexpr int CC = CGEF_CONTRACT_CC, RB = CGEF_CONTRACT_RB;
+    constexpr int CC = CGEF_CONTRACT_CC;
     const int span = 256 + R.args.dy_max - R.args.dy_min;
-    const size_t sh = (size_t)CC * RB * ((span + RB - 1) / RB) * sizeof(double2);
-    set_shmem((const void *)k_contract<CC, RB>, sh);
+    const size_t sh = (size_t)CC * span * sizeof(double2);
+    set_shmem((const void *)k_contract<CC>, sh);
     const int64_t nyb = (R.args.ny + 255) / 256;
     const int64_t items = (ncol + CC - 1) / CC * nyb * R.args.nz * R.mh;
-    hipLaunchKernelGGL((k_contract<CC, RB>), dim3((unsigned)(8 * ((items + 7) / 8))), dim3(256 / RB), sh, s, R.args, XF,
-                       OUT, ncol);
+    hipLaunchKernelGGL(k_contract<CC>, dim3((unsigned)(8 * ((items + 7) / 8))), dim3(256), sh, s, R.args, XF, OUT, ncol);
 }
 
 void ring_prep(const GeomArgs &G, int dtype, const void *u, const void *v, const void *w, int nf, int withmask,

@@ -113,6 +113,7 @@ void build_ring(Plan &p, hipStream_t s) {
         Ns *= Rr;
     }
     if (tws.empty()) tws.push_back(make_double2(1.0, 0.0));
+    R.fft.twsize = (int)tws.size();
     R.tws_d = up(tws);
     R.fft.nseq = ring_staged_nseq(R.fft);
     R.off_dy_d = up(off_dy);

@@ -399,3 +399,48 @@ def test_coarse_grain_batch_equals_slices():
             assert torch.equal(r.Pi, batch.Pi[b0, b1])
             assert np.array_equal(r.cumulative_energy, batch.cumulative_energy[b0, b1])
             assert np.allclose(r.filtering_spectrum, batch.filtering_spectrum[b0, b1])
+
+
+def test_coarse_grain_profile_is_the_level_batch():
+    # Pipeline.jl:991-1010: per-level sweeps on a shared horizontal grid = coarse_grain_batch! over levels
+    og, g, _, _ = sphere_case(True)
+    fs = smooth_random_field((3,) + g.field_shape, 25, 2)
+    U, V = T(fs[0], torch.float32), T(fs[1], torch.float32)
+    scales = [9e5, 1.5e6]
+    ker = cgef.GaussianKernel(4.0)
+    prof = cgef.coarse_grain_profile(U, V, None, g, scales=scales, kernel=ker)
+    assert tuple(prof.Pi.shape) == (3, 2) + g.field_shape
+    for lev in range(3):
+        r = cgef.coarse_grain(U[lev].contiguous(), V[lev].contiguous(), None, g, scales=scales, kernel=ker)
+        assert torch.equal(r.Pi, prof.Pi[lev])
+        assert np.array_equal(r.cumulative_energy, prof.cumulative_energy[lev])
+
+
+def test_coarse_grain_slices_and_filter_slices_ragged_grids():
+    # Pipeline.jl:866-919 / Filtering.jl:3433-3470: independent slices, each with its own grid
+    og1, g1 = cart_case(True, True, nx=40, ny=24)
+    og2, g2 = cart_case(False, True, nx=28, ny=36)
+    ker = cgef.GaussianKernel(6.0)
+    scales = [4000.0, 7000.0]
+    us, vs, refs = [], [], []
+    for og, g, seed in ((og1, g1, 26), (og2, g2, 27)):
+        u, v = smooth_random_field(g.field_shape, seed, 2)
+        us.append(T(u))
+        vs.append(T(v))
+        # the default engine for a Gaussian on a uniform Cartesian grid is the reference's separable one
+        refs.append([O.compute_pi(og, u, v, kind=O.GAUSSIAN, alpha=6.0, ell=s, engine=O.ENGINE_SEPARABLE)[0]
+                     for s in scales])
+    results = [cgef.pipeline._allocate_result(g, len(scales), torch.float64, DEV) for g in (g1, g2)]
+    cgef.coarse_grain_slices_(results, us, vs, None, [g1, g2], scales=scales, kernel=ker)
+    for r, ref in zip(results, refs):
+        for q in range(len(scales)):
+            assert rel_l2(N(r.Pi[q]), ref[q]) <= 1e-10
+    plans = [cgef.plan_filter(g, ker, 5000.0) for g in (g1, g2)]
+    outs = [torch.empty_like(u) for u in us]
+    cgef.filter_slices_(outs, us, plans)
+    for o, u, p in zip(outs, us, plans):
+        single = torch.empty_like(u)
+        cgef.filter_apply_(single, u, p)
+        assert torch.equal(o, single)
+    with pytest.raises(cgef.DimensionMismatch):
+        cgef.filter_slices_(outs, us, plans[:1])

@@ -90,3 +90,14 @@ def test_compute_pi_validates_layout_on_host():
 class _FakePlan:
     def __init__(self, grid):
         self.grid, self.device = grid, 0
+
+
+def test_slice_entry_points_check_lengths_before_touching_a_device():
+    # Pipeline.jl:892-906, Filtering.jl:3452-3454: ragged collections are DimensionMismatch
+    g = cgef.StructuredGrid(cgef.CartesianGeometry(), np.arange(8.0), np.arange(6.0))
+    with pytest.raises(cgef.DimensionMismatch):
+        cgef.coarse_grain_slices_([None, None], [None], [None, None], None, [g, g], scales=[1.0],
+                                  kernel=cgef.GaussianKernel(6.0))
+    with pytest.raises(cgef.DimensionMismatch):
+        cgef.filter_slices_([None], [None, None], [None])
+    assert cgef.slice_pipeline_costs([g, g], 3) == [144, 144]
