@@ -37,7 +37,7 @@ class GPUBackend:
 
 
 ENGINES = {"auto": _lib.ENGINE_AUTO, "banded": _lib.ENGINE_BANDED, "separable": _lib.ENGINE_SEPARABLE,
-           "baseline": _lib.ENGINE_BASELINE, "ring": _lib.ENGINE_RING}
+           "baseline": _lib.ENGINE_BASELINE, "ring": _lib.ENGINE_RING, "prefixsum": _lib.ENGINE_PREFIX}
 ENGINE_NAMES = {v: k for k, v in ENGINES.items()}
 
 
@@ -89,8 +89,10 @@ def _validate_scale(scale):
 def plan_filter(grid: StructuredGrid, kernel: AbstractFilterKernel, scale: float, *, mask_strategy=ZeroFill(),
                 backend=None, method=RealSpace(), engine: str = "auto") -> FilterPlan:
     """Filtering.jl:2931-2954. `engine='auto'` takes the reference's default for the grid (the
-    separable engine for a Gaussian on a uniform 2-D Cartesian grid, the banded footprint
-    otherwise); 'banded' forces the disk/cap/ball direct sum; 'baseline' is the plain FP64 kernel."""
+    prefix-sum engine for a top-hat on a 2-D grid, the separable engine for a Gaussian on a uniform
+    2-D Cartesian grid, the banded footprint otherwise — evaluated by the ring engine where x is
+    periodic); 'banded' forces the disk/cap/ball direct sum, 'ring' its Fourier evaluation along x,
+    'prefixsum' the per-row prefix sums, 'baseline' the plain FP64 kernel."""
     _validate_scale(scale)
     if not isinstance(method, RealSpace):
         raise ArgumentError("only RealSpace() filtering is provided by this backend")

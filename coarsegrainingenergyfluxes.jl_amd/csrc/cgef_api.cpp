@@ -116,6 +116,7 @@ Plan::~Plan() {
     }
     dfree(sep_invden_d);
     free_ring(ring);
+    free_prefix(pre);
     if (grid && --grid->refs == 0) delete grid;
 }
 
@@ -398,6 +399,11 @@ static void fetch_energy(Workspace &w, int64_t nbatch, int64_t nblk, double acti
 }
 
 // Unfused Π (baseline and separable engines): FP64 planes, one filter launch per group of slices.
+// per-row prefix sums of nsl slices (prefix-sum engine)
+static size_t prefix_bytes(const Grid &g, int64_t nsl) {
+    return (size_t)nsl * g.n[1] * (g.n[0] + 1) * sizeof(double);
+}
+
 static void compute_pi_unfused(Plan &p, Workspace &w, void *Pi, const void *u, const void *v, const void *wv,
                                int64_t nbatch, int dtype, double rho0, double *E_out, hipStream_t s) {
     Grid &g = *p.grid;
@@ -412,6 +418,9 @@ static void compute_pi_unfused(Plan &p, Workspace &w, void *Pi, const void *u, c
         if (p.engine == CGEF_ENGINE_SEPARABLE) {
             double *rp = (double *)ws_fbuf(w, (size_t)P * nsl * sizeof(double));
             launch_separable(G, p, CGEF_F64, in, out, nsl, rp, s);
+        } else if (p.engine == CGEF_ENGINE_PREFIX) {
+            double *pb = (double *)ws_fbuf(w, prefix_bytes(g, nsl));
+            launch_prefix(G, p, CGEF_F64, in, out, nsl, pb, s);
         } else {
             launch_baseline_filter(G, p, in, out, nsl, s);
         }
@@ -578,16 +587,21 @@ cgef_status cgef_plan_create(cgef_grid *grid, const cgef_kernel_desc *kernel, do
             int eng = engine;
             bool sep = false;
             if (eng == CGEF_ENGINE_AUTO) {
-                // the reference's dispatch: a Gaussian on a uniform 2-D Cartesian grid takes the
-                // separable engine (Filtering.jl:1533-1540), everything else the banded footprint —
-                // evaluated by the ring engine where x is periodic and factorable
+                // the reference's dispatch: a top-hat on a 2-D grid takes the prefix-sum engine
+                // (Filtering.jl:1115-1141), a Gaussian on a uniform 2-D Cartesian grid the separable
+                // engine (Filtering.jl:1533-1540), everything else the banded footprint — evaluated
+                // by the ring engine where x is periodic and factorable
                 sep = g->ndim == 2 && g->geometry == CGEF_CARTESIAN && p->kind == CGEF_GAUSSIAN && g->uniform[0] &&
                       g->uniform[1];
+                if (prefix_supported(*g, p->kind)) eng = CGEF_ENGINE_PREFIX;
             }
             if (eng != CGEF_ENGINE_AUTO && eng != CGEF_ENGINE_BANDED && eng != CGEF_ENGINE_SEPARABLE &&
-                eng != CGEF_ENGINE_BASELINE && eng != CGEF_ENGINE_RING)
+                eng != CGEF_ENGINE_BASELINE && eng != CGEF_ENGINE_RING && eng != CGEF_ENGINE_PREFIX)
                 fail(CGEF_ERR_ARGUMENT, "unknown engine");
-            if (eng == CGEF_ENGINE_SEPARABLE || sep) {
+            if (eng == CGEF_ENGINE_PREFIX) {
+                p->engine = CGEF_ENGINE_PREFIX;
+                build_prefix(*p);
+            } else if (eng == CGEF_ENGINE_SEPARABLE || sep) {
                 p->engine = CGEF_ENGINE_SEPARABLE;
                 build_separable(*p);
             } else {
@@ -698,6 +712,9 @@ cgef_status cgef_filter_apply(cgef_plan *plan, void *out, const void *in, int64_
             Workspace &w = resolve_ws(g, nullptr, 1, dtype);
             double *rp = (double *)ws_fbuf(w, (size_t)g.npts * nsl * sizeof(double));
             launch_separable(G, *p, dtype, in, out, nsl, rp, s);
+        } else if (p->engine == CGEF_ENGINE_PREFIX) {
+            Workspace &w = resolve_ws(g, nullptr, 1, dtype);
+            launch_prefix(G, *p, dtype, in, out, nsl, (double *)ws_fbuf(w, prefix_bytes(g, nsl)), s);
         } else {
             if (dtype != CGEF_F64) fail(CGEF_ERR_ARGUMENT, "the baseline engine filters FP64 fields");
             launch_baseline_filter(G, *p, (const double *)in, (double *)out, nsl, s);
@@ -892,6 +909,9 @@ cgef_status cgef_cumulative_energy(cgef_plan *const *plans, int64_t nscales, cge
                 } else if (p->engine == CGEF_ENGINE_SEPARABLE) {
                     double *rp = ws_scratch64(W, P * nbatch);
                     launch_separable(G, *p, dtype, in, out, nbatch, rp, s);
+                } else if (p->engine == CGEF_ENGINE_PREFIX) {
+                    double *pb = ws_scratch64(W, (int64_t)(prefix_bytes(g, nbatch) / sizeof(double)));
+                    launch_prefix(G, *p, dtype, in, out, nbatch, pb, s);
                 } else {
                     if (dtype != CGEF_F64) fail(CGEF_ERR_ARGUMENT, "the baseline engine filters FP64 fields");
                     launch_baseline_filter(G, *p, (const double *)in, (double *)out, nbatch, s);

@@ -77,6 +77,31 @@ struct RingPlan {
     RingArgs args;
 };
 
+// Prefix-sum top-hat plan (2-D rectilinear grids): per target row j and row offset b the source row
+// jj[j·nb + b] (−1: no support), its measure factor wy, and how the axis-1 interval is found:
+// wl ≥ 0 a ±wl-cell window (uniform, non-periodic axis), −2 the whole row (support spans a period),
+// −1 by comparing the extended coordinates xe with x_i ± hw.
+struct PrefixArgs {
+    int64_t nx = 0, ny = 0, ne = 0;     // ne: extended axis length (3·nx when periodic, else nx)
+    int32_t nb = 0;
+    int32_t uniform_x = 0;              // constant step: the search starts from an exact guess
+    int32_t deform = 0;                 // Deformable on a masked grid: denominator from pd
+    double x0 = 0.0, dx = 0.0;          // xe[0] and the step (uniform_x)
+    const int32_t *jj = nullptr, *wl = nullptr;
+    const double *wy = nullptr, *hw = nullptr;
+    const double *wx = nullptr;         // axis-1 measure factor per index
+    const double *xe = nullptr;         // extended ascending coordinates
+    const double *pwx = nullptr;        // prefix of wx over the real axis (nx + 1)
+    const double *pd = nullptr;         // per row prefix of mask·wx (nx + 1 each), Deformable
+};
+
+struct PrefixPlan {
+    bool ok = false;
+    PrefixArgs args;
+    int32_t *jj_d = nullptr, *wl_d = nullptr;
+    double *wy_d = nullptr, *hw_d = nullptr, *wx_d = nullptr, *xe_d = nullptr, *pwx_d = nullptr, *pd_d = nullptr;
+};
+
 struct Grid {
     int device = 0;
     int ndim = 2, geometry = CGEF_CARTESIAN;
@@ -135,6 +160,8 @@ struct Plan {
     double *sep_invden_d = nullptr;
     // ring engine
     RingPlan ring;
+    // prefix-sum top-hat engine
+    PrefixPlan pre;
     // optional kernel timing: event pairs around each dominant-kernel launch
     bool profiling = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_events;
@@ -170,6 +197,10 @@ struct Workspace {
 
 // Footprint construction (host, FP64), cgef_footprint.cpp
 void build_banded(Plan &p, bool need_fp32);
+// Prefix-sum top-hat tables (host, FP64), cgef_prefix_host.cpp
+bool prefix_supported(const Grid &g, int kind);
+void build_prefix(Plan &p);
+void free_prefix(PrefixPlan &P);
 bool ring_grid_ok(const Grid &g);
 void build_separable(Plan &p);
 

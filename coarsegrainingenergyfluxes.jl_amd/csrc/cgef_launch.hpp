@@ -62,6 +62,9 @@ void launch_baseline_filter(const GeomArgs &G, const Plan &p, const double *in, 
                             hipStream_t s);
 void launch_separable(const GeomArgs &G, const Plan &p, int dtype, const void *in, void *out, int64_t nslices,
                       double *rowpass, hipStream_t s);
+// prefix-sum top-hat (cgef_prefix.hip): Pbuf holds nslices × Ny × (Nx + 1) doubles
+void launch_prefix(const GeomArgs &G, const Plan &p, int dtype, const void *in, void *out, int64_t nslices,
+                   double *Pbuf, hipStream_t s);
 void launch_unfused_prep(const GeomArgs &G, int dtype, const void *u, const void *v, const void *w, int64_t nbatch,
                          double *vel, hipStream_t s);
 void launch_products(const GeomArgs &G, const double *vel, double *prod, int64_t nbatch, hipStream_t s);

@@ -38,8 +38,10 @@ typedef enum {
     CGEF_ENGINE_BANDED = 1,    /* disk/cap/ball footprint direct sum (fused, LDS-tiled) */
     CGEF_ENGINE_SEPARABLE = 2, /* square-truncated two-pass Gaussian (Cartesian 2-D) */
     CGEF_ENGINE_BASELINE = 3,  /* plain FP64 direct sum, one thread per point, CSR in HBM */
-    CGEF_ENGINE_RING = 4       /* the same banded footprint evaluated in Fourier space along a
+    CGEF_ENGINE_RING = 4,      /* the same banded footprint evaluated in Fourier space along a
                                   periodic x axis (FP64 FFT + banded contraction over rows) */
+    CGEF_ENGINE_PREFIX = 5     /* top-hat on a 2-D rectilinear grid by per-row prefix sums
+                                  (Filtering.jl:751-1141), the reference's top-hat engine */
 } cgef_engine;
 
 /* Grid description. Reference: FlowGeometries.Grids.StructuredGrid (un-vendored), as consumed by

@@ -368,6 +368,150 @@ static double min_gap(const og_grid *g, int axis) {
     return m;
 }
 
+/* ----------------------------- prefix-sum top-hat (Filtering.jl:751-1097) --------------------- */
+
+/* Half-width along axis 1 of the top-hat support between a target on row coordinate y1 and a source
+ * row at y2 = y1 + dy, or < 0 when the rows are farther apart than rad (Filtering.jl:764-766:
+ * Cartesian |Δx| ≤ √(rad² − Δy²); sphere |Δλ| ≤ acos((cos(rad/R) − sinφ₁ sinφ₂)/(cosφ₁ cosφ₂))). This
+ * is FlowGeometries' metric_band, restated from those formulas (parity unpinned beyond them). */
+double og_metric_band(const og_grid *g, double y1, double y2, double rad) {
+    if (!g->spherical) {
+        double dy = y2 - y1;
+        double r2 = rad * rad - dy * dy;
+        return r2 < 0.0 ? -1.0 : sqrt(r2);
+    }
+    double c = cos(rad / g->R);
+    double cc = cos(y1) * cos(y2), ss = sin(y1) * sin(y2);
+    if (cc <= 0.0) return (c <= ss) ? acos(-1.0) : -1.0;   /* a pole row: all longitudes or none */
+    double a = (c - ss) / cc;
+    if (a > 1.0) return -1.0;
+    if (a < -1.0) a = -1.0;
+    return acos(a);
+}
+
+/* Filtering.jl:833-906 (_build_prefixsum_tophat) and 913-1097 (fill + apply): per-row prefix sums of
+ * mask·f·wx along axis 1 (tripled when periodic), then for each target and each row in the band an
+ * interval sum found by comparing coordinates (two monotone pointers), or the constant ±w window
+ * on a uniform non-periodic axis, or the whole row when the support spans a full period. */
+void og_apply_prefix(const og_grid *g, double ell, int strategy, int K, const double *const *in,
+                     double *const *out, int nthreads) {
+    const int64_t Nx = g->n[0], Ny = g->n[1];
+    const double *x = g->x[0], *y = g->x[1];
+    const double rad = ell / 2.0; /* Kernels.jl:354 */
+    const int px = g->periodic[0] && Nx > 1, py = g->periodic[1] && Ny > 1;
+    const double xper = px ? g->period[0] : 0.0, yper = py ? g->period[1] : 0.0;
+    double *wx = (double *)malloc(Nx * sizeof(double)), *wy = (double *)malloc(Ny * sizeof(double));
+    for (int64_t i = 0; i < Nx; i++) wx[i] = og_cell_width(g, 0, i);
+    for (int64_t j = 0; j < Ny; j++)
+        wy[j] = (g->spherical ? g->R * g->R * cos(y[j]) : 1.0) * og_cell_width(g, 1, j);
+    double min_dy = min_gap(g, 1);
+    double dyp = g->spherical ? g->R * min_dy : min_dy;
+    int64_t djl = (isfinite(dyp) && dyp > 0) ? (int64_t)ceil(rad / dyp) : 0;
+    if (djl > Ny - 1) djl = Ny - 1;
+    const int nrep = px ? 3 : 1;
+    const int64_t ne = nrep * Nx;
+    /* ascending order of axis 1 with the real index of each extended entry */
+    int64_t *ord = (int64_t *)malloc(Nx * sizeof(int64_t));
+    for (int64_t i = 0; i < Nx; i++) ord[i] = i;
+    for (int64_t a = 1; a < Nx; a++) { /* insertion sort by coordinate (stable) */
+        int64_t t = ord[a], b = a - 1;
+        while (b >= 0 && x[ord[b]] > x[t]) { ord[b + 1] = ord[b]; b--; }
+        ord[b + 1] = t;
+    }
+    double *xe = (double *)malloc(ne * sizeof(double));
+    int64_t *src = (int64_t *)malloc(ne * sizeof(int64_t));
+    for (int r = 0; r < nrep; r++)
+        for (int64_t t = 0; t < Nx; t++) {
+            xe[r * Nx + t] = x[ord[t]] + (nrep == 3 ? (r - 1) * xper : 0.0);
+            src[r * Nx + t] = ord[t];
+        }
+    double *Pwx = (double *)malloc((ne + 1) * sizeof(double));
+    Pwx[0] = 0.0;
+    for (int64_t k = 0; k < ne; k++) Pwx[k + 1] = Pwx[k] + wx[src[k]];
+    int masked = 0;
+    if (g->mask)
+        for (int64_t t = 0; t < Nx * Ny; t++) if (!g->mask[t]) { masked = 1; break; }
+    const int use_den = strategy == OG_DEFORMABLE && masked;
+    double *Pd = NULL;
+    if (use_den) {
+        Pd = (double *)malloc((ne + 1) * Ny * sizeof(double));
+        for (int64_t j = 0; j < Ny; j++) {
+            double acc = 0.0;
+            Pd[j * (ne + 1)] = 0.0;
+            for (int64_t k = 0; k < ne; k++) {
+                int64_t i = src[k];
+                acc += active_at(g, i + Nx * j) ? wx[i] : 0.0;
+                Pd[j * (ne + 1) + k + 1] = acc;
+            }
+        }
+    }
+    const int uniform_fast = !px && ne == Nx && g->uniform[0] && Nx > 1 && x[1] - x[0] > 0.0;
+    const double step = Nx > 1 ? x[1] - x[0] : 0.0;
+    double *Pn = (double *)malloc((ne + 1) * Ny * sizeof(double));
+    double *den = (double *)malloc(Nx * Ny * sizeof(double));
+    for (int q = 0; q < K; q++) {
+        const double *f = in[q];
+        double *o = out[q];
+#pragma omp parallel for schedule(dynamic) num_threads(nthreads > 0 ? nthreads : omp_get_max_threads())
+        for (int64_t j = 0; j < Ny; j++) {
+            double acc = 0.0;
+            Pn[j * (ne + 1)] = 0.0;
+            for (int64_t k = 0; k < ne; k++) {
+                int64_t i = src[k];
+                acc += active_at(g, i + Nx * j) ? f[i + Nx * j] * wx[i] : 0.0;
+                Pn[j * (ne + 1) + k + 1] = acc;
+            }
+        }
+#pragma omp parallel for schedule(dynamic) num_threads(nthreads > 0 ? nthreads : omp_get_max_threads())
+        for (int64_t j = 0; j < Ny; j++) {
+            double *orow = o + Nx * j, *drow = den + Nx * j;
+            for (int64_t i = 0; i < Nx; i++) orow[i] = drow[i] = 0.0;
+            const double yt = y[j];
+            int64_t nb = 2 * djl + 1 < Ny ? 2 * djl + 1 : Ny;
+            for (int64_t b = 0; b < nb; b++) {
+                int64_t jr = j - djl + b;
+                if (!py && (jr < 0 || jr >= Ny)) continue;
+                int64_t jj = wrap(jr, Ny);
+                double dy = y[jj] - yt;
+                if (py && yper > 0.0) dy -= yper * round(dy / yper);
+                double hw = og_metric_band(g, yt, yt + dy, rad);
+                if (hw < 0.0) continue;
+                const double wyj = wy[jj];
+                const double *P = Pn + jj * (ne + 1);
+                const double *D = use_den ? Pd + jj * (ne + 1) : Pwx;
+                if (px && ne > Nx && 2.0 * hw >= xper) {
+                    double na = P[Nx] - P[0], da = D[Nx] - D[0];
+                    for (int64_t i = 0; i < Nx; i++) { orow[i] += wyj * na; drow[i] += wyj * da; }
+                } else if (uniform_fast) {
+                    int64_t w = 0;
+                    while ((double)(w + 1) * step <= hw) w++;
+                    for (int64_t i = 0; i < Nx; i++) {
+                        int64_t lo = i - w < 0 ? 0 : i - w, hi = i + w > Nx - 1 ? Nx - 1 : i + w;
+                        orow[i] += wyj * (P[hi + 1] - P[lo]);
+                        drow[i] += wyj * (D[hi + 1] - D[lo]);
+                    }
+                } else {
+                    int64_t lo = 0, hi = -1;
+                    for (int64_t t = 0; t < Nx; t++) {
+                        int64_t i = src[t];
+                        double xlo = x[i] - hw, xhi = x[i] + hw;
+                        while (lo < ne && xe[lo] < xlo) lo++;
+                        while (hi < ne - 1 && xe[hi + 1] <= xhi) hi++;
+                        if (hi >= lo) {
+                            orow[i] += wyj * (P[hi + 1] - P[lo]);
+                            drow[i] += wyj * (D[hi + 1] - D[lo]);
+                        }
+                    }
+                }
+            }
+            for (int64_t i = 0; i < Nx; i++)
+                orow[i] = (active_at(g, i + Nx * j) && drow[i] > 1e-15) ? orow[i] / drow[i] : 0.0;
+        }
+    }
+    free(wx); free(wy); free(ord); free(xe); free(src); free(Pwx); free(Pd); free(Pn); free(den);
+}
+
+
 typedef struct {
     int32_t o0, o1, kk;
     double w;
@@ -582,6 +726,7 @@ static void filt(const og_grid *g, const og_filter_spec *s, const og_footprint *
                  const double *const *in, double *const *out) {
     if (s->engine == 1) og_apply_separable(g, s->alpha, s->ell, s->strategy, K, in, out);
     else if (s->engine == 2) og_apply_nd3(g, s->kind, s->alpha, s->ell, s->strategy, K, in, out, s->nthreads);
+    else if (s->engine == 3) og_apply_prefix(g, s->ell, s->strategy, K, in, out, s->nthreads);
     else og_apply_banded(g, fp, s->strategy, K, in, out, s->nthreads);
 }
 

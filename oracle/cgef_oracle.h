@@ -71,6 +71,11 @@ void og_apply_banded(const og_grid *g, const og_footprint *fp, int strategy, int
 void og_apply_separable(const og_grid *g, double alpha, double ell, int strategy, int K,
                         const double *const *in, double *const *out);
 
+/* Top-hat by per-row prefix sums (2-D rectilinear grids; the reference's top-hat engine). */
+double og_metric_band(const og_grid *g, double y1, double y2, double rad);
+void og_apply_prefix(const og_grid *g, double ell, int strategy, int K, const double *const *in,
+                     double *const *out, int nthreads);
+
 /* 3-D Cartesian ball footprint applied per point (uniform or stretched axes). */
 void og_apply_nd3(const og_grid *g, int kind, double alpha, double ell, int strategy, int K,
                   const double *const *in, double *const *out, int nthreads);
@@ -83,7 +88,8 @@ void og_vector_to_cartesian(double u, double v, double w, double lam, double phi
 void og_vector_from_cartesian(double X, double Y, double Z, double lam, double phi, double out[3]);
 void og_tensor_to_local(const double t[6], double lam, double phi, double out[6]);
 
-/* engine: 0 banded footprint, 1 separable (Cartesian Gaussian only), 2 3-D ball. */
+/* engine: 0 banded footprint, 1 separable (Cartesian Gaussian only), 2 3-D ball, 3 prefix-sum
+ * top-hat (2-D). */
 typedef struct {
     int32_t kind;
     double alpha;

@@ -19,7 +19,7 @@ _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 
 TOPHAT, GAUSSIAN = 0, 1
 ZEROFILL, DEFORMABLE = 0, 1
-ENGINE_BANDED, ENGINE_SEPARABLE, ENGINE_ND3 = 0, 1, 2
+ENGINE_BANDED, ENGINE_SEPARABLE, ENGINE_ND3, ENGINE_PREFIX = 0, 1, 2, 3
 
 
 def build(force: bool = False) -> str:
@@ -84,6 +84,9 @@ def lib():
         L.og_apply_separable.argtypes = [gp, C.c_double, C.c_double, C.c_int, C.c_int, C.POINTER(dp), C.POINTER(dp)]
         L.og_apply_nd3.argtypes = [gp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
                                    C.POINTER(dp), C.POINTER(dp), C.c_int]
+        L.og_apply_prefix.argtypes = [gp, C.c_double, C.c_int, C.c_int, C.POINTER(dp), C.POINTER(dp), C.c_int]
+        L.og_metric_band.restype = C.c_double
+        L.og_metric_band.argtypes = [gp, C.c_double, C.c_double, C.c_double]
         L.og_deriv.argtypes = [gp, C.c_int, dp, dp]
         L.og_vector_to_cartesian.argtypes = [C.c_double] * 5 + [dp]
         L.og_vector_from_cartesian.argtypes = [C.c_double] * 5 + [dp]
@@ -219,6 +222,19 @@ def apply_nd3(grid, kind, alpha, ell, strategy, fields, nthreads=0):
     return outs
 
 
+def apply_prefix(grid, ell, strategy, fields, nthreads=0):
+    """Filtering.jl:751-1097: the reference's top-hat engine on 2-D rectilinear grids."""
+    ins, pin = _arr(fields)
+    outs = [np.empty_like(f) for f in ins]
+    pout = (C.POINTER(C.c_double) * len(outs))(*[_dp(o) for o in outs])
+    lib().og_apply_prefix(grid.ref(), ell, strategy, len(ins), pin, pout, nthreads)
+    return outs
+
+
+def metric_band(grid, y1, y2, rad):
+    return lib().og_metric_band(grid.ref(), y1, y2, rad)
+
+
 def deriv(grid, axis, f):
     f = np.ascontiguousarray(f, dtype=np.float64)
     out = np.empty_like(f)
@@ -288,6 +304,8 @@ def filter_fields(grid, fields, *, kind, alpha, ell, strategy=ZEROFILL, engine=E
         return apply_separable(grid, alpha, ell, strategy, fields)
     if engine == ENGINE_ND3:
         return apply_nd3(grid, kind, alpha, ell, strategy, fields, nthreads)
+    if engine == ENGINE_PREFIX:
+        return apply_prefix(grid, ell, strategy, fields, nthreads)
     if fp is None:
         fp = Footprint(grid, kind, alpha, ell)
     return apply_banded(grid, fp, strategy, fields, nthreads)

@@ -444,3 +444,73 @@ def test_coarse_grain_slices_and_filter_slices_ragged_grids():
         assert torch.equal(o, single)
     with pytest.raises(cgef.DimensionMismatch):
         cgef.filter_slices_(outs, us, plans[:1])
+
+
+# ------------------------------------------------------------------- prefix-sum top-hat engine --
+# Filtering.jl:751-1141: the reference's engine for a top-hat on a 2-D grid, against the oracle's
+# restatement of the same algorithm (itself pinned to the disk footprint and brute force).
+
+def prefix_case(case, masked=True):
+    rng = np.random.Generator(np.random.PCG64(31))
+    if case == "sphere":
+        lam, phi = sphere_axes(96, 48)
+        mask = blob_mask((48, 96), 32) if masked else None
+        og = oracle_grid_sphere(lam, phi, mask=mask)
+        g = cgef.StructuredGrid(cgef.SphericalGeometry(6.371e6), lam, phi, mask=mask)
+        return og, g, 1.1e6
+    ny, nx = 30, 44
+    mask = blob_mask((ny, nx), 33) if masked else None
+    ys = cart_axes(ny, 1250.0)
+    if case.startswith("nonuniform"):
+        xs = np.cumsum(rng.uniform(600.0, 1400.0, nx))
+        per = case.endswith("periodic")
+        L = float(xs[-1] - xs[0] + 1000.0)
+        og = O.Grid([xs, ys], spherical=False, periodic=[per, per], period=[L, ny * 1250.0], mask=mask,
+                    uniform=[False, True])
+        g = cgef.StructuredGrid(cgef.CartesianGeometry(), xs, ys, mask=mask, periodic=(per, per),
+                                period=(L, ny * 1250.0))
+        return og, g, 9000.0
+    per = case in ("periodic", "full_row")
+    xs = cart_axes(nx, 1000.0, 500.0)
+    og = oracle_grid_cart(xs, ys, periodic=(per, per), mask=mask)
+    g = cgef.StructuredGrid(cgef.CartesianGeometry(), xs, ys, mask=mask, periodic=(per, per))
+    return og, g, (60000.0 if case == "full_row" else 9000.0)
+
+
+@pytest.mark.parametrize("case", ["periodic", "bounded", "nonuniform", "nonuniform_periodic", "sphere", "full_row"])
+@pytest.mark.parametrize("strategy", ["zerofill", "deformable"])
+def test_prefix_filter(case, strategy):
+    og, g, ell = prefix_case(case)
+    s_cg, s_o = STRATS[strategy]
+    plan = cgef.plan_filter(g, cgef.TopHatKernel(), ell, mask_strategy=s_cg)
+    assert plan.engine == "prefixsum"
+    f = smooth_random_field((2,) + g.field_shape, 34, 1)[0]
+    ref = O.apply_prefix(og, ell, s_o, [f[0], f[1]])
+    for dtype, tol in ((torch.float64, 1e-12), (torch.float32, 1e-6)):
+        inp = T(f, dtype)
+        out = torch.empty_like(inp)
+        cgef.filter_apply_(out, inp, plan)
+        got = N(out)
+        fin = N(inp)
+        ref = O.apply_prefix(og, ell, s_o, [fin[0], fin[1]])
+        for b in range(2):
+            assert np.max(np.abs(got[b] - ref[b])) <= tol * np.max(np.abs(ref[b]))
+
+
+@pytest.mark.parametrize("case", ["periodic", "sphere", "nonuniform"])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_prefix_pi_and_default_pipeline(case, dtype):
+    og, g, ell = prefix_case(case)
+    u, v = smooth_random_field(g.field_shape, 35, 2)
+    u, v = u + 0.3, v - 0.2
+    U, V = T(u, dtype), T(v, dtype)
+    Pi = torch.empty_like(U)
+    ws = cgef.PiWorkspace(g, (), dtype)
+    _, E = cgef.compute_pi_(Pi, U, V, None, g, cgef.TopHatKernel(), ell, workspace=ws, return_energy=True)
+    ref, ub, vb, _ = O.compute_pi(og, N(U), N(V), kind=O.TOPHAT, alpha=0.0, ell=ell, engine=O.ENGINE_PREFIX)
+    tol = 1e-10 if dtype == torch.float64 else 1e-5
+    assert rel_l2(N(Pi), ref) <= tol
+    assert abs(E - O.energy(og, ub, vb)) <= 1e-10 * abs(O.energy(og, ub, vb))
+    # the pipeline's default kernel is the top-hat (Pipeline.jl:344): coarse_grain takes this engine
+    res = cgef.coarse_grain(U, V, None, g, scales=[ell], spectrum=cgef.NoSpectrum())
+    assert rel_l2(N(res.Pi[0]), ref) <= tol

@@ -423,3 +423,61 @@ def test_energy_of_constant_field_and_spectral_density():
     kk = np.arange(1.0, 6.0)
     assert O.spectral_density(kk ** 2, kk)[2] == pytest.approx(2 * kk[2])
     assert np.all(O.spectral_density(np.array([1.0]), np.array([1.0])) == 0.0)
+
+
+# ------------------------------------------- prefix-sum top-hat (Filtering.jl:751-1097) --------
+
+@pytest.mark.parametrize("periodic", [True, False])
+@pytest.mark.parametrize("strategy", [O.ZEROFILL, O.DEFORMABLE])
+def test_prefix_tophat_equals_disk_footprint(periodic, strategy):
+    # the prefix-sum restatement (per-row prefix sums, two monotone pointers) against the independent
+    # disk-footprint restatement, which the brute-force tests above pin; radius below half the box,
+    # where the footprint's modular wrap and the prefix engine's minimum image agree
+    xs, ys = cart_axes(40, 1000.0, 500.0), cart_axes(30, 1250.0)
+    rng = np.random.Generator(np.random.PCG64(7))
+    mask = (rng.random((30, 40)) > 0.2).astype(np.uint8)
+    og = oracle_grid_cart(xs, ys, periodic=(periodic, periodic), mask=mask)
+    f = smooth_random_field((30, 40), 5)[0]
+    for ell in (3300.0, 7100.0, 12345.0, 30000.0):
+        a = O.apply_prefix(og, ell, strategy, [f])[0]
+        b = O.filter_fields(og, [f], kind=O.TOPHAT, alpha=0.0, ell=ell, strategy=strategy)[0]
+        assert np.max(np.abs(a - b)) <= 1e-13 * np.max(np.abs(b))
+
+
+def test_prefix_tophat_sphere_equals_haversine_cap():
+    # the cap |Δλ| ≤ acos((cos(rad/R) − sinφ₁sinφ₂)/(cosφ₁cosφ₂)) is the haversine disk d ≤ rad
+    lam, phi = sphere_axes(72, 36)
+    og = oracle_grid_sphere(lam, phi)
+    f = smooth_random_field((36, 72), 6)[0]
+    for ell in (5e5, 1.3e6, 3e6):
+        a = O.apply_prefix(og, ell, O.ZEROFILL, [f])[0]
+        b = O.filter_fields(og, [f], kind=O.TOPHAT, alpha=0.0, ell=ell)[0]
+        assert np.max(np.abs(a - b)) <= 1e-13 * np.max(np.abs(b))
+    # metric_band closed forms: the equator row at rad = one degree spans ±1°; beyond rad nothing
+    rad = R_EARTH * math.radians(1.0)
+    assert O.metric_band(og, 0.0, 0.0, rad) == pytest.approx(math.radians(1.0), rel=1e-12)
+    assert O.metric_band(og, 0.0, math.radians(1.5), rad) < 0.0
+
+
+def test_prefix_tophat_nonuniform_axis_brute_force():
+    # a stretched axis 1 (the two-pointer path): brute-force disk sum with the cell measures
+    rng = np.random.Generator(np.random.PCG64(8))
+    xs = np.cumsum(rng.uniform(600.0, 1400.0, 24))
+    ys = cart_axes(18, 1000.0)
+    mask = (rng.random((18, 24)) > 0.25).astype(np.uint8)
+    og = O.Grid([xs, ys], spherical=False, periodic=[False, False], period=[0.0, 0.0], mask=mask,
+                uniform=[False, True])
+    f = smooth_random_field((18, 24), 9)[0]
+    A = np.array([[O.area(og, i, j, 0) for i in range(24)] for j in range(18)])
+    for ell in (2500.0, 6100.0):
+        rad = ell / 2
+        got = O.apply_prefix(og, ell, O.ZEROFILL, [f])[0]
+        exp = np.zeros_like(f)
+        for j in range(18):
+            for i in range(24):
+                if not mask[j, i]:
+                    continue
+                d = np.sqrt((xs[None, :] - xs[i]) ** 2 + (ys[:, None] - ys[j]) ** 2)
+                sup = d <= rad
+                exp[j, i] = np.sum((A * f * mask)[sup]) / np.sum(A[sup])
+        assert np.max(np.abs(got - exp)) <= 1e-12 * np.max(np.abs(exp))
