@@ -128,6 +128,45 @@ def batched_leg(args, grid, mask, lam, phi):
     return out
 
 
+def tophat_leg(args, grid, lam, phi, mask, u32, v32):
+    """The pipeline's default kernel (the top-hat, Pipeline.jl:344) on the same 0.25° sphere and
+    fields at ℓ = 200 km: the library's default engine for it is the reference's prefix-sum engine
+    (per-row prefix sums); the ring engine evaluating the same cap is timed beside it."""
+    import numpy as np
+    import torch
+
+    import cgef
+
+    ker = cgef.TopHatKernel()
+    Pi = torch.empty_like(u32)
+    ws = cgef.PiWorkspace(grid, (), torch.float32)
+    out = {"workload": "spherical 0.25deg 1440x720 FP32, top-hat l=200km (cap radius 100 km), 30% land ZeroFill"}
+    for eng in ("auto", "ring"):
+        plan = cgef.plan_filter(grid, ker, SCALE, engine=eng)
+        steps = 20
+
+        def step():
+            cgef.compute_pi_(Pi, u32, v32, None, grid, ker, SCALE, workspace=ws, filter_plan=plan)
+
+        el, _ = timed_steps(step, steps, 3)
+        out[plan.engine] = {"ms_per_step": el * 1e3 / steps, "value": grid.npoints * steps / el,
+                            "unit": "point-scales/s"}
+        got = Pi.cpu().double().numpy()
+        log(f"top-hat sphere {plan.engine}: {el * 1e3 / steps:.3f} ms/step")
+        if not args.no_cpu:
+            from oracle import oracle as O
+            og = O.Grid([lam, phi], spherical=True, R=6.371e6, periodic=[True, False], period=[2 * np.pi, 0.0],
+                        mask=mask, uniform=[True, True])
+            oeng = O.ENGINE_PREFIX if plan.engine == "prefixsum" else O.ENGINE_BANDED
+            ncores = min(len(os.sched_getaffinity(0)), 16)
+            t0 = time.perf_counter()
+            ref = O.compute_pi(og, u32.cpu().double().numpy(), v32.cpu().double().numpy(), kind=O.TOPHAT,
+                               alpha=0.0, ell=SCALE, engine=oeng, nthreads=ncores)[0]
+            out[plan.engine]["cpu_s"] = time.perf_counter() - t0
+            out[plan.engine]["rel_l2_vs_fp64_oracle"] = float(np.sqrt(np.sum((got - ref) ** 2) / np.sum(ref ** 2)))
+    return out
+
+
 def cartesian_leg(args):
     """Plain method at small size: 256×256 doubly periodic Cartesian, L = 1000 km, FP64, no mask,
     Gaussian α = 6, ℓ = 8Δx, ZeroFill, Π with ρ₀ = 1025. The library's default engine here is the
@@ -341,6 +380,7 @@ def main():
     if not args.no_batched:
         extra["batched_spectrum"] = batched_leg(args, grid, mask, lam, phi)
     if not args.no_configs:
+        extra["tophat_sphere"] = tophat_leg(args, grid, lam, phi, mask, u32, v32)
         extra["cartesian_256"] = cartesian_leg(args)
         extra["volume_256x256x64"] = volume_leg(args)
 

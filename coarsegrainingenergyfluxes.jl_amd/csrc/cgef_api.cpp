@@ -758,10 +758,17 @@ cgef_status cgef_compute_pi(cgef_plan *plan, cgef_workspace *ws, void *Pi, const
             launch_strain_pi(G, dtype, W.ubar, W.tau, Pi, rho0, nbatch, W.partials, W.npartials, s);
             CGEF_HIP(hipGetLastError());
             if (E_out) fetch_energy(W, nbatch, strain_blocks(G), g.active_area, E_out, s);
+        } else if (p->engine == CGEF_ENGINE_PREFIX) {
+            const int nf = g.ndim == 2 && g.geometry == CGEF_CARTESIAN ? 5 : 9;
+            double *pb = (double *)ws_fbuf(W, prefix_bytes(g, nf * nbatch));
+            launch_prefix_pi(G, *p, dtype, u, v, w, nbatch, pb, W.ubar, W.tau, ubar, vbar, wbar, s);
+            launch_strain_pi(G, dtype, W.ubar, W.tau, Pi, rho0, nbatch, W.partials, W.npartials, s);
+            CGEF_HIP(hipGetLastError());
+            if (E_out) fetch_energy(W, nbatch, strain_blocks(G), g.active_area, E_out, s);
         } else {
             compute_pi_unfused(*p, W, Pi, u, v, w, nbatch, dtype, rho0, E_out, s);
         }
-        if (p->engine != CGEF_ENGINE_RING) {  // the ring epilogue writes them itself
+        if (p->engine != CGEF_ENGINE_RING && p->engine != CGEF_ENGINE_PREFIX) {  // those epilogues write them
             if (ubar) export_planes(g, W.ubar, 0, ubar, nbatch, dtype, s);
             if (vbar) export_planes(g, W.ubar, 1, vbar, nbatch, dtype, s);
             if (wbar) export_planes(g, W.ubar, 2, wbar, nbatch, dtype, s);

@@ -65,6 +65,11 @@ void launch_separable(const GeomArgs &G, const Plan &p, int dtype, const void *i
 // prefix-sum top-hat (cgef_prefix.hip): Pbuf holds nslices × Ny × (Nx + 1) doubles
 void launch_prefix(const GeomArgs &G, const Plan &p, int dtype, const void *in, void *out, int64_t nslices,
                    double *Pbuf, hipStream_t s);
+// Π planes scanned straight from (u, v[, w]) (Pbuf: nf·nbatch × Ny × (Nx + 1) doubles), then ū and τ
+// per point; ū/v̄/w̄ also leave in the caller's type when uo/vo/wo are given
+void launch_prefix_pi(const GeomArgs &G, const Plan &p, int dtype, const void *u, const void *v, const void *w,
+                      int64_t nbatch, double *Pbuf, double *ubar, void *tau, void *uo, void *vo, void *wo,
+                      hipStream_t s);
 void launch_unfused_prep(const GeomArgs &G, int dtype, const void *u, const void *v, const void *w, int64_t nbatch,
                          double *vel, hipStream_t s);
 void launch_products(const GeomArgs &G, const double *vel, double *prod, int64_t nbatch, hipStream_t s);

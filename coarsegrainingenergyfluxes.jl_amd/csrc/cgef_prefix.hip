@@ -11,7 +11,7 @@
 // N + 1 real-axis prefixes are stored.
 #include <hip/hip_runtime.h>
 
-#include "cgef_launch.hpp"
+#include "cgef_pi_point.hpp"
 
 namespace cgef {
 
@@ -137,6 +137,107 @@ __global__ __launch_bounds__(256) void k_prefix_apply(PrefixArgs A, GeomArgs G, 
         }
     }
     out[s * G.npts + t] = (T)((G.mask[t] && den > 1e-15) ? num / den : 0.0);
+}
+
+// Π path: the planes are formed from (u, v[, w]) inside the scan (one workgroup per (row, plane
+// column), columns fastest so the planes of a row re-read its velocities from L2), and the apply
+// finds each row's interval once for all NF planes, then runs the per-point epilogue.
+__global__ __launch_bounds__(256) void k_prefix_pi_rows(PrefixArgs A, GeomArgs G, int dtype, const void *u, const void *v,
+                                                        const void *w, int nf, int64_t ncol, double *__restrict__ P) {
+    __shared__ double wsum[4];
+    const int64_t row = (int64_t)blockIdx.x / ncol, col = (int64_t)blockIdx.x - row * ncol;
+    const int64_t b = col / nf;
+    const int f = (int)(col - b * nf);
+    const int64_t N = A.nx, j = row % A.ny;
+    const int64_t moff = row * N, off = b * G.npts;
+    double *Pr = P + (col * A.ny + row) * (N + 1);
+    if (threadIdx.x == 0) Pr[0] = 0.0;
+    double carry = 0.0;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int64_t base = 0; base < N; base += 256) {
+        const int64_t i = base + threadIdx.x;
+        double val = 0.0;
+        if (i < N && G.mask[moff + i]) {
+            double q[3];
+            frame_vel(G, dtype, u, v, w, off, moff + i, i, j, q);
+            val = pi_plane_value(G.mode, f, q) * A.wx[i];
+        }
+        val = wave_incl_scan(val);
+        if (lane == 63) wsum[wid] = val;
+        __syncthreads();
+        double pre = carry;
+        for (int q = 0; q < wid; q++) pre += wsum[q];
+        const double tile = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        if (i < N) Pr[i + 1] = pre + val;
+        carry += tile;
+        __syncthreads();
+    }
+}
+
+template <typename T, int NF>
+__global__ __launch_bounds__(256) void k_prefix_pi_apply(PrefixArgs A, GeomArgs G, const double *__restrict__ P,
+                                                         int64_t nchunk, double *__restrict__ ubar, T *__restrict__ tau,
+                                                         T *__restrict__ uo, T *__restrict__ vo, T *__restrict__ wo) {
+    const int64_t b = blockIdx.y;
+    const int64_t j = (int64_t)blockIdx.x / nchunk;
+    const int64_t i = ((int64_t)blockIdx.x - j * nchunk) * 256 + threadIdx.x;
+    const int64_t N = A.nx, NR = A.ny * (N + 1);
+    if (i >= N) return;
+    const int64_t t = j * N + i;
+    const double *Pb = P + b * NF * NR;
+    const double xi = G.coord[0][i];
+    double F[NF], den = 0.0;
+#pragma unroll
+    for (int f = 0; f < NF; f++) F[f] = 0.0;
+    for (int bb = 0; bb < A.nb; bb++) {
+        const int64_t e = j * A.nb + bb;
+        const int jj = A.jj[e];
+        if (jj < 0) continue;
+        const double wyj = A.wy[e];
+        const int wl = A.wl[e];
+        const double *Pr = Pb + (int64_t)jj * (N + 1);
+        const double *Dr = A.deform ? A.pd + (int64_t)jj * (N + 1) : A.pwx;
+        int64_t lo, hi1;  // interval [lo, hi1) of the extended axis
+        if (wl == -2) {
+            lo = 0;
+            hi1 = N;
+        } else if (wl >= 0) {
+            lo = i - wl < 0 ? 0 : i - wl;
+            hi1 = (i + wl > N - 1 ? N - 1 : i + wl) + 1;
+        } else {
+            const double hw = A.hw[e];
+            lo = first_ge(A, xi - hw);
+            hi1 = count_le(A, xi + hw);
+            if (hi1 <= lo) continue;
+        }
+        den += wyj * (pext(Dr, hi1, N) - pext(Dr, lo, N));
+#pragma unroll
+        for (int f = 0; f < NF; f++) F[f] += wyj * (pext(Pr + f * NR, hi1, N) - pext(Pr + f * NR, lo, N));
+    }
+    pi_point_epilogue<T>(G, t, i, j % G.n[1], F, den, b, ubar, tau, uo, vo, wo);
+}
+
+void launch_prefix_pi(const GeomArgs &G, const Plan &p, int dtype, const void *u, const void *v, const void *w,
+                      int64_t nbatch, double *Pbuf, double *ubar, void *tau, void *uo, void *vo, void *wo,
+                      hipStream_t s) {
+    const PrefixArgs &A = p.pre.args;
+    const int nf = G.mode == 0 ? 5 : 9;
+    const int64_t ncol = nf * nbatch;
+    const int64_t nchunk = (A.nx + 255) / 256;
+    hipLaunchKernelGGL(k_prefix_pi_rows, dim3((unsigned)(A.ny * ncol)), dim3(256), 0, s, A, G, dtype, u, v, w, nf, ncol,
+                       Pbuf);
+    const dim3 ga((unsigned)(nchunk * A.ny), (unsigned)nbatch);
+#define CGEF_PPA(TT, NN)                                                                                           \
+    hipLaunchKernelGGL((k_prefix_pi_apply<TT, NN>), ga, dim3(256), 0, s, A, G, Pbuf, nchunk, ubar, (TT *)tau, (TT *)uo, \
+                       (TT *)vo, (TT *)wo)
+    if (dtype == CGEF_F32) {
+        if (nf == 5) CGEF_PPA(float, 5);
+        else CGEF_PPA(float, 9);
+    } else {
+        if (nf == 5) CGEF_PPA(double, 5);
+        else CGEF_PPA(double, 9);
+    }
+#undef CGEF_PPA
 }
 
 void launch_prefix(const GeomArgs &G, const Plan &p, int dtype, const void *in, void *out, int64_t nslices,

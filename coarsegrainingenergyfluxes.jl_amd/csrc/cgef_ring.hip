@@ -16,6 +16,7 @@
 
 #include <cfloat>
 
+#include "cgef_pi_point.hpp"
 #include "cgef_ring.hpp"
 
 namespace cgef {
@@ -519,23 +520,6 @@ __global__ __launch_bounds__(256) void k_build_what(RingFft F, const double2 *__
 
 // ---- prep: shifted masked components (planetary frame on the sphere), their products, the mask --
 
-__device__ __forceinline__ void frame_vel(const GeomArgs &G, int dtype, const void *u, const void *v, const void *w,
-                                          int64_t off, int64_t t, int64_t i, int64_t j, double U[3]) {
-    const double uu = dtype == CGEF_F32 ? (double)((const float *)u)[off + t] : ((const double *)u)[off + t];
-    const double vv = dtype == CGEF_F32 ? (double)((const float *)v)[off + t] : ((const double *)v)[off + t];
-    if (G.mode == 1) {
-        const double sl = G.trig[i], cl = G.trig[G.n[0] + i];
-        const double sp = G.trig[2 * G.n[0] + j], cp = G.trig[2 * G.n[0] + G.n[1] + j];
-        U[0] = -sl * uu - sp * cl * vv;
-        U[1] = cl * uu - sp * sl * vv;
-        U[2] = cp * vv;
-    } else {
-        U[0] = uu;
-        U[1] = vv;
-        U[2] = G.mode == 2 ? (dtype == CGEF_F32 ? (double)((const float *)w)[off + t] : ((const double *)w)[off + t]) : 0.0;
-    }
-}
-
 // Field order per slice: q (NC), products (NP: xx xy [xz] yy [yz zz]), mask (if masked).
 __global__ __launch_bounds__(256) void k_ring_prep(GeomArgs G, int dtype, const void *u, const void *v, const void *w,
                                                    int nf, int withmask, double *xin) {
@@ -584,63 +568,13 @@ __global__ __launch_bounds__(256) void k_ring_epilogue(GeomArgs G, const double 
     const int64_t Nx = G.n[0], Ny = G.n[1];
     const int64_t i = t % Nx, r = t / Nx, j = r % Ny;
     const double *y = yout + b * nf * P;
-    const int NC = G.mode == 0 ? 2 : 3;
     // ZeroFill divides by the in-range weight sum of the row; Deformable by the filtered mask
     // (Filtering.jl:3180-3240)
     const double den = (strategy == CGEF_DEFORMABLE && withmask) ? y[(nf - 1) * P + t] : D_row[r];
-    const bool ok = G.mask[t] != 0 && den > 1e-15;
-    double U[3] = {0.0, 0.0, 0.0}, tc[6] = {0, 0, 0, 0, 0, 0};
-    if (ok) {
-        const double inv = 1.0 / den;
-        for (int a = 0; a < NC; a++) U[a] = y[a * P + t] * inv;
-        if (NC == 2) {
-            tc[0] = y[2 * P + t] * inv - U[0] * U[0];
-            tc[1] = y[3 * P + t] * inv - U[0] * U[1];
-            tc[3] = y[4 * P + t] * inv - U[1] * U[1];
-        } else {
-            const int ia[6] = {0, 0, 0, 1, 1, 2}, ib[6] = {0, 1, 2, 1, 2, 2};
-            for (int q6 = 0; q6 < 6; q6++) tc[q6] = y[(3 + q6) * P + t] * inv - U[ia[q6]] * U[ib[q6]];
-        }
-    }
-    double *ub = ubar + b * 3 * P;
-    T *tb = tau + b * 6 * P;
-    double o0, o1, o2;
-    if (G.mode == 1) {
-        const double sl = G.trig[i], cl = G.trig[Nx + i];
-        const double sp = G.trig[2 * Nx + j], cp = G.trig[2 * Nx + Ny + j];
-        const double ee[3] = {-sl, cl, 0.0}, en[3] = {-sp * cl, -sp * sl, cp}, er[3] = {cp * cl, cp * sl, sp};
-        o0 = ee[0] * U[0] + ee[1] * U[1];
-        o1 = en[0] * U[0] + en[1] * U[1] + en[2] * U[2];
-        o2 = er[0] * U[0] + er[1] * U[1] + er[2] * U[2];
-        const double T3[3][3] = {{tc[0], tc[1], tc[2]}, {tc[1], tc[3], tc[4]}, {tc[2], tc[4], tc[5]}};
-        double Te[3], Tn[3];
-        for (int rr = 0; rr < 3; rr++) {
-            Te[rr] = T3[rr][0] * ee[0] + T3[rr][1] * ee[1] + T3[rr][2] * ee[2];
-            Tn[rr] = T3[rr][0] * en[0] + T3[rr][1] * en[1] + T3[rr][2] * en[2];
-        }
-        tb[t] = (T)(ee[0] * Te[0] + ee[1] * Te[1] + ee[2] * Te[2]);
-        tb[P + t] = (T)(ee[0] * Tn[0] + ee[1] * Tn[1] + ee[2] * Tn[2]);
-        tb[2 * P + t] = (T)(en[0] * Tn[0] + en[1] * Tn[1] + en[2] * Tn[2]);
-    } else if (G.mode == 0) {
-        o0 = U[0];
-        o1 = U[1];
-        o2 = 0.0;
-        tb[t] = (T)tc[0];
-        tb[P + t] = (T)tc[1];
-        tb[2 * P + t] = (T)tc[3];
-    } else {
-        o0 = U[0];
-        o1 = U[1];
-        o2 = U[2];
-        for (int q6 = 0; q6 < 6; q6++) tb[q6 * P + t] = (T)tc[q6];
-    }
-    // the strain reads planes 0, 1 (and 2 in 3-D); the filtered velocities leave in T when asked for
-    ub[t] = o0;
-    ub[P + t] = o1;
-    if (G.mode == 2) ub[2 * P + t] = o2;
-    if (uo) uo[b * P + t] = (T)o0;
-    if (vo) vo[b * P + t] = (T)o1;
-    if (wo) wo[b * P + t] = (T)o2;
+    double F[9];
+    const int np = G.mode == 0 ? 5 : 9;
+    for (int f = 0; f < np; f++) F[f] = y[f * P + t];
+    pi_point_epilogue<T>(G, t, i, j, F, den, b, ubar, tau, uo, vo, wo);
 }
 
 // plain fields for filter_apply: x = f·active (FP64), plus the mask when a Deformable denominator is needed

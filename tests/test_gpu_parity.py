@@ -510,6 +510,8 @@ def test_prefix_pi_and_default_pipeline(case, dtype):
     ref, ub, vb, _ = O.compute_pi(og, N(U), N(V), kind=O.TOPHAT, alpha=0.0, ell=ell, engine=O.ENGINE_PREFIX)
     tol = 1e-10 if dtype == torch.float64 else 1e-5
     assert rel_l2(N(Pi), ref) <= tol
+    assert rel_l2(N(ws.u_filt), ub) <= (1e-12 if dtype == torch.float64 else 1e-7)
+    assert rel_l2(N(ws.v_filt), vb) <= (1e-12 if dtype == torch.float64 else 1e-7)
     assert abs(E - O.energy(og, ub, vb)) <= 1e-10 * abs(O.energy(og, ub, vb))
     # the pipeline's default kernel is the top-hat (Pipeline.jl:344): coarse_grain takes this engine
     res = cgef.coarse_grain(U, V, None, g, scales=[ell], spectrum=cgef.NoSpectrum())
