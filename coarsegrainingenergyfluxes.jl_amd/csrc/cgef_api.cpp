@@ -398,12 +398,13 @@ static void fetch_energy(Workspace &w, int64_t nbatch, int64_t nblk, double acti
     CGEF_HIP(hipStreamSynchronize(s));
 }
 
-// Unfused Π (baseline and separable engines): FP64 planes, one filter launch per group of slices.
 // per-row prefix sums of nsl slices (prefix-sum engine)
 static size_t prefix_bytes(const Grid &g, int64_t nsl) {
     return (size_t)nsl * g.n[1] * (g.n[0] + 1) * sizeof(double);
 }
 
+// Unfused Π for the baseline engine (the reference's plain per-field kernels restated): FP64 planes of
+// the frame velocity and its products, one filter launch per group of planes, then τ, ū and the strain.
 static void compute_pi_unfused(Plan &p, Workspace &w, void *Pi, const void *u, const void *v, const void *wv,
                                int64_t nbatch, int dtype, double rho0, double *E_out, hipStream_t s) {
     Grid &g = *p.grid;
@@ -414,17 +415,7 @@ static void compute_pi_unfused(Plan &p, Workspace &w, void *Pi, const void *u, c
     double *vel = base, *velf = vel + 3 * P * nbatch, *prod = velf + 3 * P * nbatch, *prodf = prod + 6 * P * nbatch;
     double *tau = prodf + 6 * P * nbatch, *ubar = tau + 6 * P * nbatch, *pi64 = ubar + 3 * P * nbatch;
     launch_unfused_prep(G, dtype, u, v, wv, nbatch, vel, s);
-    auto filt = [&](const double *in, double *out, int64_t nsl) {
-        if (p.engine == CGEF_ENGINE_SEPARABLE) {
-            double *rp = (double *)ws_fbuf(w, (size_t)P * nsl * sizeof(double));
-            launch_separable(G, p, CGEF_F64, in, out, nsl, rp, s);
-        } else if (p.engine == CGEF_ENGINE_PREFIX) {
-            double *pb = (double *)ws_fbuf(w, prefix_bytes(g, nsl));
-            launch_prefix(G, p, CGEF_F64, in, out, nsl, pb, s);
-        } else {
-            launch_baseline_filter(G, p, in, out, nsl, s);
-        }
-    };
+    auto filt = [&](const double *in, double *out, int64_t nsl) { launch_baseline_filter(G, p, in, out, nsl, s); };
     for (int64_t b = 0; b < nbatch; b++) filt(vel + b * 3 * P, velf + b * 3 * P, NC);
     launch_products(G, vel, prod, nbatch, s);
     // the Cartesian 2-D product set is uu, uv, vv: planes 0, 1, 3 of the six
@@ -758,6 +749,12 @@ cgef_status cgef_compute_pi(cgef_plan *plan, cgef_workspace *ws, void *Pi, const
             launch_strain_pi(G, dtype, W.ubar, W.tau, Pi, rho0, nbatch, W.partials, W.npartials, s);
             CGEF_HIP(hipGetLastError());
             if (E_out) fetch_energy(W, nbatch, strain_blocks(G), g.active_area, E_out, s);
+        } else if (p->engine == CGEF_ENGINE_SEPARABLE) {
+            double *rp = (double *)ws_fbuf(W, (size_t)5 * g.npts * nbatch * sizeof(double));
+            launch_separable_pi(G, *p, dtype, u, v, nbatch, rp, W.ubar, W.tau, ubar, vbar, wbar, s);
+            launch_strain_pi(G, dtype, W.ubar, W.tau, Pi, rho0, nbatch, W.partials, W.npartials, s);
+            CGEF_HIP(hipGetLastError());
+            if (E_out) fetch_energy(W, nbatch, strain_blocks(G), g.active_area, E_out, s);
         } else if (p->engine == CGEF_ENGINE_PREFIX) {
             const int nf = g.ndim == 2 && g.geometry == CGEF_CARTESIAN ? 5 : 9;
             double *pb = (double *)ws_fbuf(W, prefix_bytes(g, nf * nbatch));
@@ -768,7 +765,7 @@ cgef_status cgef_compute_pi(cgef_plan *plan, cgef_workspace *ws, void *Pi, const
         } else {
             compute_pi_unfused(*p, W, Pi, u, v, w, nbatch, dtype, rho0, E_out, s);
         }
-        if (p->engine != CGEF_ENGINE_RING && p->engine != CGEF_ENGINE_PREFIX) {  // those epilogues write them
+        if (p->engine == CGEF_ENGINE_BANDED || p->engine == CGEF_ENGINE_BASELINE) {  // the others' epilogues write them
             if (ubar) export_planes(g, W.ubar, 0, ubar, nbatch, dtype, s);
             if (vbar) export_planes(g, W.ubar, 1, vbar, nbatch, dtype, s);
             if (wbar) export_planes(g, W.ubar, 2, wbar, nbatch, dtype, s);

@@ -17,7 +17,7 @@
 #include <cfloat>
 #include <type_traits>
 
-#include "cgef_launch.hpp"
+#include "cgef_pi_point.hpp"
 
 namespace cgef {
 
@@ -861,6 +861,97 @@ void launch_separable(const GeomArgs &G, const Plan &p, int dtype, const void *i
         hipLaunchKernelGGL(k_sep_row<double>, grid, dim3(256), 0, s, G, p.sep_g_d[0], p.sep_lim[0], (const double *)in, rowpass);
         hipLaunchKernelGGL(k_sep_col<double>, grid, dim3(256), 0, s, G, p.sep_g_d[1], p.sep_lim[1], rowpass, p.sep_prof_d[0],
                            p.sep_prof_d[1], p.sep_invden_d, (double *)out);
+    }
+}
+
+// Separable Π path: the row pass forms the five Cartesian planes (u, v, uu, uv, vv of the masked
+// velocity) at each source cell from (u, v) directly, the column pass finishes all five at once and
+// hands the point to the shared epilogue (ZeroFill: profiles product; Deformable: 1 / filtered mask).
+template <typename T>
+__global__ __launch_bounds__(256) void k_sep_pi_row(GeomArgs G, const double *__restrict__ gx, int lx,
+                                                    const T *__restrict__ u, const T *__restrict__ v,
+                                                    double *__restrict__ rp) {
+    const int64_t b = blockIdx.y;
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= G.npts) return;
+    const int64_t Nx = G.n[0], P = G.npts;
+    const int64_t i = t % Nx, rowbase = t - i;
+    const T *ub = u + b * P, *vb = v + b * P;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, a4 = 0.0;
+    for (int d = -lx; d <= lx; d++) {
+        int64_t ii = i + d;
+        if (ii < 0 || ii >= Nx) {
+            if (!G.periodic[0]) continue;
+            ii = wrapi(ii, Nx);
+        }
+        const int64_t q = rowbase + ii;
+        if (!G.mask[q]) continue;
+        const double g = gx[d + lx], x = (double)ub[q], y = (double)vb[q];
+        a0 += g * x;
+        a1 += g * y;
+        a2 += g * (x * x);
+        a3 += g * (x * y);
+        a4 += g * (y * y);
+    }
+    double *o = rp + b * 5 * P;
+    o[t] = a0;
+    o[P + t] = a1;
+    o[2 * P + t] = a2;
+    o[3 * P + t] = a3;
+    o[4 * P + t] = a4;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_sep_pi_col(GeomArgs G, const double *__restrict__ gy, int ly,
+                                                    const double *__restrict__ rp, const double *__restrict__ px,
+                                                    const double *__restrict__ py, const double *__restrict__ invden,
+                                                    double *__restrict__ ubar, T *__restrict__ tau, T *__restrict__ uo,
+                                                    T *__restrict__ vo, T *__restrict__ wo) {
+    const int64_t b = blockIdx.y;
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= G.npts) return;
+    const int64_t Nx = G.n[0], Ny = G.n[1], P = G.npts;
+    const int64_t i = t % Nx, j = t / Nx;
+    const double *r = rp + b * 5 * P;
+    double F[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int d = -ly; d <= ly; d++) {
+        int64_t jj = j + d;
+        if (jj < 0 || jj >= Ny) {
+            if (!G.periodic[1]) continue;
+            jj = wrapi(jj, Ny);
+        }
+        const double g = gy[d + ly];
+        const int64_t q = i + Nx * jj;
+#pragma unroll
+        for (int f = 0; f < 5; f++) F[f] += g * r[f * P + q];
+    }
+    bool ok;
+    double inv;
+    if (invden) {
+        inv = invden[t];
+        ok = G.mask[t] != 0 && inv > 0.0;
+    } else {
+        const double den = px[i] * py[j];
+        ok = G.mask[t] != 0 && den > 1e-15;
+        inv = ok ? 1.0 / den : 0.0;
+    }
+    pi_point_epilogue_inv<T>(G, t, i, j, F, ok, inv, b, ubar, tau, uo, vo, wo);
+}
+
+void launch_separable_pi(const GeomArgs &G, const Plan &p, int dtype, const void *u, const void *v, int64_t nbatch,
+                         double *rp, double *ubar, void *tau, void *uo, void *vo, void *wo, hipStream_t s) {
+    dim3 grid((unsigned)((G.npts + 255) / 256), (unsigned)nbatch);
+    if (dtype == CGEF_F32) {
+        hipLaunchKernelGGL(k_sep_pi_row<float>, grid, dim3(256), 0, s, G, p.sep_g_d[0], p.sep_lim[0], (const float *)u,
+                           (const float *)v, rp);
+        hipLaunchKernelGGL(k_sep_pi_col<float>, grid, dim3(256), 0, s, G, p.sep_g_d[1], p.sep_lim[1], rp, p.sep_prof_d[0],
+                           p.sep_prof_d[1], p.sep_invden_d, ubar, (float *)tau, (float *)uo, (float *)vo, (float *)wo);
+    } else {
+        hipLaunchKernelGGL(k_sep_pi_row<double>, grid, dim3(256), 0, s, G, p.sep_g_d[0], p.sep_lim[0], (const double *)u,
+                           (const double *)v, rp);
+        hipLaunchKernelGGL(k_sep_pi_col<double>, grid, dim3(256), 0, s, G, p.sep_g_d[1], p.sep_lim[1], rp,
+                           p.sep_prof_d[0], p.sep_prof_d[1], p.sep_invden_d, ubar, (double *)tau, (double *)uo,
+                           (double *)vo, (double *)wo);
     }
 }
 

@@ -62,6 +62,10 @@ void launch_baseline_filter(const GeomArgs &G, const Plan &p, const double *in, 
                             hipStream_t s);
 void launch_separable(const GeomArgs &G, const Plan &p, int dtype, const void *in, void *out, int64_t nslices,
                       double *rowpass, hipStream_t s);
+// separable Π path (2-D Cartesian): rp holds 5 planes per slice; ū/v̄/w̄ also leave in the caller's
+// type when uo/vo/wo are given
+void launch_separable_pi(const GeomArgs &G, const Plan &p, int dtype, const void *u, const void *v, int64_t nbatch,
+                         double *rp, double *ubar, void *tau, void *uo, void *vo, void *wo, hipStream_t s);
 // prefix-sum top-hat (cgef_prefix.hip): Pbuf holds nslices × Ny × (Nx + 1) doubles
 void launch_prefix(const GeomArgs &G, const Plan &p, int dtype, const void *in, void *out, int64_t nslices,
                    double *Pbuf, hipStream_t s);

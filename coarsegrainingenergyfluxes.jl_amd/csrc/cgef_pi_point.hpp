@@ -39,19 +39,19 @@ __device__ __forceinline__ double pi_plane_value(int mode, int f, const double q
     return q[a] * q[b];
 }
 
-// ū = F[q]/den, τ = F[q_a q_b]/den − ū_a ū_b at active points with den > 1e-15 (0 elsewhere), rotated
-// to the local (east, north) frame on the sphere; writes ū (FP64 planes the strain reads), τ (3 or 6
-// planes of T) and, when asked for, ū/v̄/w̄ in T. F holds the plane numerators in pi_plane_value order.
+// ū = F[q]·inv, τ = F[q_a q_b]·inv − ū_a ū_b where `ok` (0 elsewhere), rotated to the local (east,
+// north) frame on the sphere; writes ū (FP64 planes the strain reads), τ (3 or 6 planes of T) and,
+// when asked for, ū/v̄/w̄ in T. F holds the plane numerators in pi_plane_value order; inv is the
+// reciprocal of the normalization.
 template <typename T>
-__device__ __forceinline__ void pi_point_epilogue(const GeomArgs &G, int64_t t, int64_t i, int64_t j, const double *F,
-                                                  double den, int64_t b, double *__restrict__ ubar, T *__restrict__ tau,
-                                                  T *__restrict__ uo, T *__restrict__ vo, T *__restrict__ wo) {
+__device__ __forceinline__ void pi_point_epilogue_inv(const GeomArgs &G, int64_t t, int64_t i, int64_t j,
+                                                      const double *F, bool ok, double inv, int64_t b,
+                                                      double *__restrict__ ubar, T *__restrict__ tau, T *__restrict__ uo,
+                                                      T *__restrict__ vo, T *__restrict__ wo) {
     const int64_t P = G.npts, Nx = G.n[0], Ny = G.n[1];
     const int NC = G.mode == 0 ? 2 : 3;
-    const bool ok = G.mask[t] != 0 && den > 1e-15;
     double U[3] = {0.0, 0.0, 0.0}, tc[6] = {0, 0, 0, 0, 0, 0};
     if (ok) {
-        const double inv = 1.0 / den;
         for (int a = 0; a < NC; a++) U[a] = F[a] * inv;
         if (NC == 2) {
             tc[0] = F[2] * inv - U[0] * U[0];
@@ -101,6 +101,16 @@ __device__ __forceinline__ void pi_point_epilogue(const GeomArgs &G, int64_t t, 
     if (uo) uo[b * P + t] = (T)o0;
     if (vo) vo[b * P + t] = (T)o1;
     if (wo) wo[b * P + t] = (T)o2;
+}
+
+// The same with the normalization as a denominator: active points with den > 1e-15 (Filtering.jl:
+// 1633) get F/den.
+template <typename T>
+__device__ __forceinline__ void pi_point_epilogue(const GeomArgs &G, int64_t t, int64_t i, int64_t j, const double *F,
+                                                  double den, int64_t b, double *__restrict__ ubar, T *__restrict__ tau,
+                                                  T *__restrict__ uo, T *__restrict__ vo, T *__restrict__ wo) {
+    const bool ok = G.mask[t] != 0 && den > 1e-15;
+    pi_point_epilogue_inv<T>(G, t, i, j, F, ok, ok ? 1.0 / den : 0.0, b, ubar, tau, uo, vo, wo);
 }
 
 }  // namespace cgef
