@@ -751,14 +751,18 @@ cgef_status cgef_compute_pi(cgef_plan *plan, cgef_workspace *ws, void *Pi, const
             if (E_out) fetch_energy(W, nbatch, strain_blocks(G), g.active_area, E_out, s);
         } else if (p->engine == CGEF_ENGINE_SEPARABLE) {
             double *rp = (double *)ws_fbuf(W, (size_t)5 * g.npts * nbatch * sizeof(double));
+            p->prof_begin(s);
             launch_separable_pi(G, *p, dtype, u, v, nbatch, rp, W.ubar, W.tau, ubar, vbar, wbar, s);
+            p->prof_end(s);
             launch_strain_pi(G, dtype, W.ubar, W.tau, Pi, rho0, nbatch, W.partials, W.npartials, s);
             CGEF_HIP(hipGetLastError());
             if (E_out) fetch_energy(W, nbatch, strain_blocks(G), g.active_area, E_out, s);
         } else if (p->engine == CGEF_ENGINE_PREFIX) {
             const int nf = g.ndim == 2 && g.geometry == CGEF_CARTESIAN ? 5 : 9;
             double *pb = (double *)ws_fbuf(W, prefix_bytes(g, nf * nbatch));
+            p->prof_begin(s);
             launch_prefix_pi(G, *p, dtype, u, v, w, nbatch, pb, W.ubar, W.tau, ubar, vbar, wbar, s);
+            p->prof_end(s);
             launch_strain_pi(G, dtype, W.ubar, W.tau, Pi, rho0, nbatch, W.partials, W.npartials, s);
             CGEF_HIP(hipGetLastError());
             if (E_out) fetch_energy(W, nbatch, strain_blocks(G), g.active_area, E_out, s);

@@ -865,34 +865,47 @@ void launch_separable(const GeomArgs &G, const Plan &p, int dtype, const void *i
 }
 
 // Separable Π path: the row pass forms the five Cartesian planes (u, v, uu, uv, vv of the masked
-// velocity) at each source cell from (u, v) directly, the column pass finishes all five at once and
-// hands the point to the shared epilogue (ZeroFill: profiles product; Deformable: 1 / filtered mask).
+// velocity) from (u, v) staged in LDS for a 256-cell row segment plus its halo; the column pass walks
+// down TJ output rows per lane with the input rows streamed once (each read row feeds every output
+// row within reach), finishes all five planes and hands each point to the shared epilogue (ZeroFill:
+// product of the axis profiles; Deformable: 1 / filtered mask). Taps are summed in the same order
+// as the plain two-pass filter.
 template <typename T>
 __global__ __launch_bounds__(256) void k_sep_pi_row(GeomArgs G, const double *__restrict__ gx, int lx,
                                                     const T *__restrict__ u, const T *__restrict__ v,
-                                                    double *__restrict__ rp) {
+                                                    double *__restrict__ rp, int64_t nchunk) {
+    extern __shared__ double srow[];
     const int64_t b = blockIdx.y;
-    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (t >= G.npts) return;
-    const int64_t Nx = G.n[0], P = G.npts;
-    const int64_t i = t % Nx, rowbase = t - i;
+    const int64_t j = (int64_t)blockIdx.x / nchunk;
+    const int64_t i0 = ((int64_t)blockIdx.x - j * nchunk) * 256;
+    const int64_t Nx = G.n[0], P = G.npts, rowbase = j * Nx;
+    const int L = 256 + 2 * lx;
+    double *su = srow, *sv = srow + L;
     const T *ub = u + b * P, *vb = v + b * P;
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, a4 = 0.0;
-    for (int d = -lx; d <= lx; d++) {
-        int64_t ii = i + d;
+    for (int sidx = threadIdx.x; sidx < L; sidx += 256) {
+        int64_t ii = i0 - lx + sidx;
+        bool in = true;
         if (ii < 0 || ii >= Nx) {
-            if (!G.periodic[0]) continue;
-            ii = wrapi(ii, Nx);
+            if (G.periodic[0]) ii = wrapi(ii, Nx);
+            else in = false;
         }
-        const int64_t q = rowbase + ii;
-        if (!G.mask[q]) continue;
-        const double g = gx[d + lx], x = (double)ub[q], y = (double)vb[q];
+        const bool act = in && G.mask[rowbase + ii] != 0;
+        su[sidx] = act ? (double)ub[rowbase + ii] : 0.0;
+        sv[sidx] = act ? (double)vb[rowbase + ii] : 0.0;
+    }
+    __syncthreads();
+    const int64_t i = i0 + threadIdx.x;
+    if (i >= Nx) return;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, a4 = 0.0;
+    for (int d = 0; d <= 2 * lx; d++) {
+        const double g = gx[d], x = su[threadIdx.x + d], y = sv[threadIdx.x + d];
         a0 += g * x;
         a1 += g * y;
         a2 += g * (x * x);
         a3 += g * (x * y);
         a4 += g * (y * y);
     }
+    const int64_t t = rowbase + i;
     double *o = rp + b * 5 * P;
     o[t] = a0;
     o[P + t] = a1;
@@ -901,57 +914,99 @@ __global__ __launch_bounds__(256) void k_sep_pi_row(GeomArgs G, const double *__
     o[4 * P + t] = a4;
 }
 
-template <typename T>
+template <typename T, int TJ>
 __global__ __launch_bounds__(256) void k_sep_pi_col(GeomArgs G, const double *__restrict__ gy, int ly,
                                                     const double *__restrict__ rp, const double *__restrict__ px,
                                                     const double *__restrict__ py, const double *__restrict__ invden,
                                                     double *__restrict__ ubar, T *__restrict__ tau, T *__restrict__ uo,
-                                                    T *__restrict__ vo, T *__restrict__ wo) {
+                                                    T *__restrict__ vo, T *__restrict__ wo, int64_t nchunk) {
     const int64_t b = blockIdx.y;
-    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (t >= G.npts) return;
+    const int64_t jt = (int64_t)blockIdx.x / nchunk;
+    const int64_t i = ((int64_t)blockIdx.x - jt * nchunk) * 256 + threadIdx.x;
     const int64_t Nx = G.n[0], Ny = G.n[1], P = G.npts;
-    const int64_t i = t % Nx, j = t / Nx;
+    if (i >= Nx) return;
+    const int64_t j0 = jt * TJ;
     const double *r = rp + b * 5 * P;
-    double F[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    for (int d = -ly; d <= ly; d++) {
-        int64_t jj = j + d;
-        if (jj < 0 || jj >= Ny) {
-            if (!G.periodic[1]) continue;
-            jj = wrapi(jj, Ny);
-        }
-        const double g = gy[d + ly];
-        const int64_t q = i + Nx * jj;
+    double F[TJ][5];
 #pragma unroll
-        for (int f = 0; f < 5; f++) F[f] += g * r[f * P + q];
+    for (int q = 0; q < TJ; q++)
+#pragma unroll
+        for (int f = 0; f < 5; f++) F[q][f] = 0.0;
+    for (int64_t jj = j0 - ly; jj < j0 + TJ + ly; jj++) {
+        int64_t jw = jj;
+        if (jw < 0 || jw >= Ny) {
+            if (!G.periodic[1]) continue;
+            jw = wrapi(jw, Ny);
+        }
+        const int64_t qi = i + Nx * jw;
+        double x[5];
+#pragma unroll
+        for (int f = 0; f < 5; f++) x[f] = r[f * P + qi];
+#pragma unroll
+        for (int q = 0; q < TJ; q++) {
+            const int64_t d = jj - (j0 + q);
+            if (d < -ly || d > ly) continue;
+            const double g = gy[d + ly];
+#pragma unroll
+            for (int f = 0; f < 5; f++) F[q][f] += g * x[f];
+        }
     }
-    bool ok;
-    double inv;
-    if (invden) {
-        inv = invden[t];
-        ok = G.mask[t] != 0 && inv > 0.0;
-    } else {
-        const double den = px[i] * py[j];
-        ok = G.mask[t] != 0 && den > 1e-15;
-        inv = ok ? 1.0 / den : 0.0;
+#pragma unroll
+    for (int q = 0; q < TJ; q++) {
+        const int64_t j = j0 + q;
+        if (j >= Ny) break;
+        const int64_t t = i + Nx * j;
+        bool ok;
+        double inv;
+        if (invden) {
+            inv = invden[t];
+            ok = G.mask[t] != 0 && inv > 0.0;
+        } else {
+            const double den = px[i] * py[j];
+            ok = G.mask[t] != 0 && den > 1e-15;
+            inv = ok ? 1.0 / den : 0.0;
+        }
+        pi_point_epilogue_inv<T>(G, t, i, j, F[q], ok, inv, b, ubar, tau, uo, vo, wo);
     }
-    pi_point_epilogue_inv<T>(G, t, i, j, F, ok, inv, b, ubar, tau, uo, vo, wo);
 }
 
 void launch_separable_pi(const GeomArgs &G, const Plan &p, int dtype, const void *u, const void *v, int64_t nbatch,
                          double *rp, double *ubar, void *tau, void *uo, void *vo, void *wo, hipStream_t s) {
-    dim3 grid((unsigned)((G.npts + 255) / 256), (unsigned)nbatch);
+    const int64_t nchunk = (G.n[0] + 255) / 256;
+    // 8 output rows per lane where that still leaves the chip ≥ 1024 workgroups, else one
+    const bool tall = nchunk * ((G.n[1] + 7) / 8) * nbatch >= 1024;
+    const int TJ = tall ? 8 : 1;
+    const int lx = p.sep_lim[0], ly = p.sep_lim[1];
+    const size_t sh = (size_t)2 * (256 + 2 * lx) * sizeof(double);
+    if (sh > 160 * 1024) fail(CGEF_ERR_UNSUPPORTED, "separable row half-width too large for the row pass");
+    if (sh > 64 * 1024) {
+        const void *fn = dtype == CGEF_F32 ? (const void *)k_sep_pi_row<float> : (const void *)k_sep_pi_row<double>;
+        CGEF_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+    }
+    const dim3 gr((unsigned)(nchunk * G.n[1]), (unsigned)nbatch);
+    const dim3 gc((unsigned)(nchunk * ((G.n[1] + TJ - 1) / TJ)), (unsigned)nbatch);
     if (dtype == CGEF_F32) {
-        hipLaunchKernelGGL(k_sep_pi_row<float>, grid, dim3(256), 0, s, G, p.sep_g_d[0], p.sep_lim[0], (const float *)u,
-                           (const float *)v, rp);
-        hipLaunchKernelGGL(k_sep_pi_col<float>, grid, dim3(256), 0, s, G, p.sep_g_d[1], p.sep_lim[1], rp, p.sep_prof_d[0],
-                           p.sep_prof_d[1], p.sep_invden_d, ubar, (float *)tau, (float *)uo, (float *)vo, (float *)wo);
+        hipLaunchKernelGGL(k_sep_pi_row<float>, gr, dim3(256), sh, s, G, p.sep_g_d[0], lx, (const float *)u,
+                           (const float *)v, rp, nchunk);
+        if (tall)
+            hipLaunchKernelGGL((k_sep_pi_col<float, 8>), gc, dim3(256), 0, s, G, p.sep_g_d[1], ly, rp, p.sep_prof_d[0],
+                               p.sep_prof_d[1], p.sep_invden_d, ubar, (float *)tau, (float *)uo, (float *)vo,
+                               (float *)wo, nchunk);
+        else
+            hipLaunchKernelGGL((k_sep_pi_col<float, 1>), gc, dim3(256), 0, s, G, p.sep_g_d[1], ly, rp, p.sep_prof_d[0],
+                               p.sep_prof_d[1], p.sep_invden_d, ubar, (float *)tau, (float *)uo, (float *)vo,
+                               (float *)wo, nchunk);
     } else {
-        hipLaunchKernelGGL(k_sep_pi_row<double>, grid, dim3(256), 0, s, G, p.sep_g_d[0], p.sep_lim[0], (const double *)u,
-                           (const double *)v, rp);
-        hipLaunchKernelGGL(k_sep_pi_col<double>, grid, dim3(256), 0, s, G, p.sep_g_d[1], p.sep_lim[1], rp,
-                           p.sep_prof_d[0], p.sep_prof_d[1], p.sep_invden_d, ubar, (double *)tau, (double *)uo,
-                           (double *)vo, (double *)wo);
+        hipLaunchKernelGGL(k_sep_pi_row<double>, gr, dim3(256), sh, s, G, p.sep_g_d[0], lx, (const double *)u,
+                           (const double *)v, rp, nchunk);
+        if (tall)
+            hipLaunchKernelGGL((k_sep_pi_col<double, 8>), gc, dim3(256), 0, s, G, p.sep_g_d[1], ly, rp,
+                               p.sep_prof_d[0], p.sep_prof_d[1], p.sep_invden_d, ubar, (double *)tau, (double *)uo,
+                               (double *)vo, (double *)wo, nchunk);
+        else
+            hipLaunchKernelGGL((k_sep_pi_col<double, 1>), gc, dim3(256), 0, s, G, p.sep_g_d[1], ly, rp,
+                               p.sep_prof_d[0], p.sep_prof_d[1], p.sep_invden_d, ubar, (double *)tau, (double *)uo,
+                               (double *)vo, (double *)wo, nchunk);
     }
 }
 

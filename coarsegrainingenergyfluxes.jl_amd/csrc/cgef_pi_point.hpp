@@ -52,14 +52,23 @@ __device__ __forceinline__ void pi_point_epilogue_inv(const GeomArgs &G, int64_t
     const int NC = G.mode == 0 ? 2 : 3;
     double U[3] = {0.0, 0.0, 0.0}, tc[6] = {0, 0, 0, 0, 0, 0};
     if (ok) {
-        for (int a = 0; a < NC; a++) U[a] = F[a] * inv;
+        // every index below is a compile-time constant, so F and the temporaries stay in registers
         if (NC == 2) {
+            U[0] = F[0] * inv;
+            U[1] = F[1] * inv;
             tc[0] = F[2] * inv - U[0] * U[0];
             tc[1] = F[3] * inv - U[0] * U[1];
             tc[3] = F[4] * inv - U[1] * U[1];
         } else {
-            const int ia[6] = {0, 0, 0, 1, 1, 2}, ib[6] = {0, 1, 2, 1, 2, 2};
-            for (int q6 = 0; q6 < 6; q6++) tc[q6] = F[3 + q6] * inv - U[ia[q6]] * U[ib[q6]];
+            U[0] = F[0] * inv;
+            U[1] = F[1] * inv;
+            U[2] = F[2] * inv;
+            tc[0] = F[3] * inv - U[0] * U[0];
+            tc[1] = F[4] * inv - U[0] * U[1];
+            tc[2] = F[5] * inv - U[0] * U[2];
+            tc[3] = F[6] * inv - U[1] * U[1];
+            tc[4] = F[7] * inv - U[1] * U[2];
+            tc[5] = F[8] * inv - U[2] * U[2];
         }
     }
     double *ub = ubar + b * 3 * P;
@@ -74,6 +83,7 @@ __device__ __forceinline__ void pi_point_epilogue_inv(const GeomArgs &G, int64_t
         o2 = er[0] * U[0] + er[1] * U[1] + er[2] * U[2];
         const double T3[3][3] = {{tc[0], tc[1], tc[2]}, {tc[1], tc[3], tc[4]}, {tc[2], tc[4], tc[5]}};
         double Te[3], Tn[3];
+#pragma unroll
         for (int rr = 0; rr < 3; rr++) {
             Te[rr] = T3[rr][0] * ee[0] + T3[rr][1] * ee[1] + T3[rr][2] * ee[2];
             Tn[rr] = T3[rr][0] * en[0] + T3[rr][1] * en[1] + T3[rr][2] * en[2];
@@ -92,6 +102,7 @@ __device__ __forceinline__ void pi_point_epilogue_inv(const GeomArgs &G, int64_t
         o0 = U[0];
         o1 = U[1];
         o2 = U[2];
+#pragma unroll
         for (int q6 = 0; q6 < 6; q6++) tb[q6 * P + t] = (T)tc[q6];
     }
     // the strain reads planes 0, 1 (and 2 in 3-D); the filtered velocities leave in T when asked for

@@ -149,7 +149,9 @@ cgef_status cgef_cumulative_energy(cgef_plan *const *plans, int64_t nscales, cge
                                    int32_t dtype, double *E_out, void *stream);
 
 /* Kernel timing (no counterpart in the reference; for measurement): while enabled, every launch of
- * the plan's dominant filter kernel is bracketed by HIP events recorded on the launch stream.
+ * the plan's dominant filter step is bracketed by HIP events recorded on the launch stream (banded:
+ * the fused filter kernel; ring: the contraction; separable and prefix-sum: their two filter
+ * kernels together).
  * cgef_profile_read synchronizes on those events and returns the summed kernel time (ms) and the
  * number of launches since enabling; enabling again resets the counters. */
 cgef_status cgef_profile_enable(cgef_plan *plan, int32_t on);
