@@ -16,7 +16,7 @@ F32, F64 = 0, 1
 CARTESIAN, SPHERICAL = 0, 1
 TOPHAT, GAUSSIAN = 0, 1
 ZEROFILL, DEFORMABLE = 0, 1
-ENGINE_AUTO, ENGINE_BANDED, ENGINE_SEPARABLE, ENGINE_BASELINE, ENGINE_RING, ENGINE_PREFIX = range(6)
+ENGINE_AUTO, ENGINE_BANDED, ENGINE_SEPARABLE, ENGINE_BASELINE, ENGINE_RING, ENGINE_PREFIX, ENGINE_SPECTRAL = range(7)
 
 EXPORTED = (
     "cgef_version", "cgef_last_error", "cgef_grid_create", "cgef_grid_destroy", "cgef_active_area",

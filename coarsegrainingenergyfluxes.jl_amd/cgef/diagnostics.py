@@ -89,13 +89,13 @@ def _validate_field_sizes(grid, Pi, u, v, w, dev):
 
 def compute_pi_(Pi, u, v, w, grid: StructuredGrid, kernel, scale, *, workspace: PiWorkspace | None = None,
                 filter_plan: FilterPlan | None = None, backend=None, mask_strategy=ZeroFill(), deriv_plan=None,
-                rho0: float = 1.0, return_energy: bool = False):
+                rho0: float = 1.0, return_energy: bool = False, method=None):
     """compute_Π! (Diagnostics.jl:389-416 2-D, 792-857 3-D): Π = −S̄:τ at scale ℓ (times `rho0`,
     default 1, which is the reference: it applies no density). Leading axes of the fields beyond the
     grid's are a batch. With return_energy=True also returns E(ℓ) per slice (energy_from_filtered)."""
     dev = Pi.device.index if Pi.is_cuda else -1
     plan = filter_plan if filter_plan is not None else plan_filter(
-        grid, kernel, scale, mask_strategy=mask_strategy, backend=backend)
+        grid, kernel, scale, mask_strategy=mask_strategy, backend=backend, method=method)
     if plan.grid is not grid:
         raise _lib.ArgumentError("filter_plan was built for a different grid")
     if dev != plan.device:
@@ -154,7 +154,7 @@ def energy_from_filtered(ws: PiWorkspace, grid: StructuredGrid, has_w: bool, tot
 
 
 def cumulative_energy_(spectrum, u, v, w, grid, kernel, scales, *, workspace=None, filter_plans=None, backend=None,
-                       mask_strategy=ZeroFill()):
+                       mask_strategy=ZeroFill(), method=None):
     """Diagnostics.jl:1144-1217 — filter the raw (u, v[, w]) per scale and reduce ½⟨|ū|²⟩."""
     dev = u.device.index if u.is_cuda else -1
     for name, t in (("u", u), ("v", v), ("w", w)):
@@ -167,7 +167,8 @@ def cumulative_energy_(spectrum, u, v, w, grid, kernel, scales, *, workspace=Non
     if len(scales) == 0:
         return spectrum
     plans = filter_plans if filter_plans is not None else [
-        plan_filter(grid, kernel, float(s), mask_strategy=mask_strategy, backend=backend) for s in scales]
+        plan_filter(grid, kernel, float(s), mask_strategy=mask_strategy, backend=backend, method=method)
+        for s in scales]
     arr = (C.c_void_p * len(plans))(*[p.h for p in plans])
     E = np.zeros(len(scales), dtype=np.float64)
     _lib.check(_lib.load().cgef_cumulative_energy(
@@ -178,10 +179,10 @@ def cumulative_energy_(spectrum, u, v, w, grid, kernel, scales, *, workspace=Non
     return spectrum
 
 
-def cumulative_energy(u, v, w, grid, kernel, scales, *, backend=None, mask_strategy=ZeroFill()):
+def cumulative_energy(u, v, w, grid, kernel, scales, *, backend=None, mask_strategy=ZeroFill(), method=None):
     """Diagnostics.jl:1238-1251."""
     return cumulative_energy_(np.zeros(len(scales)), u, v, w, grid, kernel, scales, backend=backend,
-                              mask_strategy=mask_strategy)
+                              mask_strategy=mask_strategy, method=method)
 
 
 def spectral_density_(g, Cv, k):
@@ -203,10 +204,11 @@ def spectral_density(Cv, k):
 
 
 def filtering_spectrum(u, v, w, grid, kernel, scales, *, L: float = 1.0, backend=None, mask_strategy=ZeroFill(),
-                       policy=StrictSpectrum()):
+                       policy=StrictSpectrum(), method=None):
     """Diagnostics.jl:1294-1309 — (k_ℓ = L/ℓ, Ẽ(k_ℓ))."""
     compute = gate_spectrum(kernel, policy)
-    cum = cumulative_energy(u, v, w, grid, kernel, scales, backend=backend, mask_strategy=mask_strategy)
+    cum = cumulative_energy(u, v, w, grid, kernel, scales, backend=backend, mask_strategy=mask_strategy,
+                            method=method)
     kl = float(L) / np.asarray(scales, dtype=np.float64)
     if not compute:
         return kl, np.full(len(kl), np.nan)

@@ -27,7 +27,19 @@ class Deformable:
 
 @dataclass(frozen=True)
 class RealSpace:
-    """Filtering.jl:62 — physical-space direct sum (the only method this backend provides)."""
+    """Filtering.jl:64-65 — physical-space convolution (the reference's default method)."""
+
+
+@dataclass(frozen=True)
+class Spectral:
+    """Filtering.jl:67-73 — the kernel applied as a multiply on the transformed field. Provided here for
+    a doubly periodic 2-D Cartesian grid (the reference's FFTW extension, FFTWExt.jl:121-208)."""
+
+
+@dataclass(frozen=True)
+class AutoMethod:
+    """Filtering.jl:75-85 — Spectral where a periodic transform is the exact filter (every axis
+    periodic and uniform), RealSpace otherwise."""
 
 
 @dataclass(frozen=True)
@@ -37,7 +49,8 @@ class GPUBackend:
 
 
 ENGINES = {"auto": _lib.ENGINE_AUTO, "banded": _lib.ENGINE_BANDED, "separable": _lib.ENGINE_SEPARABLE,
-           "baseline": _lib.ENGINE_BASELINE, "ring": _lib.ENGINE_RING, "prefixsum": _lib.ENGINE_PREFIX}
+           "baseline": _lib.ENGINE_BASELINE, "ring": _lib.ENGINE_RING, "prefixsum": _lib.ENGINE_PREFIX,
+           "spectral": _lib.ENGINE_SPECTRAL}
 ENGINE_NAMES = {v: k for k, v in ENGINES.items()}
 
 
@@ -86,18 +99,49 @@ def _validate_scale(scale):
         raise ArgumentError(f"filter scale must be finite and positive, got {scale}")
 
 
+def _spectral_exact(grid: StructuredGrid) -> bool:
+    """Filtering.jl:3000-3006: a periodic transform is the exact filter where every axis wraps and is
+    uniformly spaced (the transform of this backend is 2-D)."""
+    return (not grid.spherical and grid.ndim == 2 and all(grid.periodic) and all(grid.uniform))
+
+
+def _resolve_method(grid: StructuredGrid, method):
+    """Filtering.jl:3012-3019 (_resolve_method): AutoMethod becomes Spectral only where exact."""
+    if method is None:
+        return RealSpace()
+    if isinstance(method, AutoMethod):
+        return Spectral() if _spectral_exact(grid) else RealSpace()
+    if not isinstance(method, (RealSpace, Spectral)):
+        raise ArgumentError("method must be RealSpace(), Spectral() or AutoMethod()")
+    return method
+
+
 def plan_filter(grid: StructuredGrid, kernel: AbstractFilterKernel, scale: float, *, mask_strategy=ZeroFill(),
-                backend=None, method=RealSpace(), engine: str = "auto") -> FilterPlan:
+                backend=None, method=None, engine: str = "auto") -> FilterPlan:
     """Filtering.jl:2931-2954. `engine='auto'` takes the reference's default for the grid (the
     prefix-sum engine for a top-hat on a 2-D grid, the separable engine for a Gaussian on a uniform
     2-D Cartesian grid, the banded footprint otherwise — evaluated by the ring engine where x is
     periodic); 'banded' forces the disk/cap/ball direct sum, 'ring' its Fourier evaluation along x,
-    'prefixsum' the per-row prefix sums, 'baseline' the plain FP64 kernel."""
+    'prefixsum' the per-row prefix sums, 'baseline' the plain FP64 kernel. `method=Spectral()` (or
+    `engine='spectral'`, or AutoMethod() on a periodic uniform grid) filters by 2-D transform, as the
+    reference's FFTW extension does (FFTWExt.jl:121-208)."""
     _validate_scale(scale)
-    if not isinstance(method, RealSpace):
-        raise ArgumentError("only RealSpace() filtering is provided by this backend")
     if not isinstance(mask_strategy, (ZeroFill, Deformable)):
         raise ArgumentError("mask_strategy must be ZeroFill() or Deformable()")
+    m = _resolve_method(grid, method)
+    if engine == "spectral":
+        m = Spectral()
+    if isinstance(m, Spectral):
+        if engine not in ("auto", "spectral"):
+            raise ArgumentError(f"engine = {engine!r} is a real-space engine; method = Spectral() filters by transform")
+        if grid.spherical or grid.ndim != 2:
+            raise ArgumentError("this backend's Spectral() filtering is the 2-D Cartesian transform; "
+                                f"no transform is provided for a {grid.ndim}-D {'spherical' if grid.spherical else 'Cartesian'} grid")
+        if not all(grid.periodic):
+            # FFTWExt.jl:133-136
+            raise ArgumentError("Spectral FFT filtering requires a doubly-periodic Cartesian grid; build it with "
+                                "StructuredGrid(geom, x, y, mask=..., periodic=(True, True)).")
+        engine = "spectral"
     return FilterPlan(grid, kernel, scale, mask_strategy, _device_of(backend), engine)
 
 
@@ -163,7 +207,7 @@ def filter_apply_batch_(outs, fields, plan: FilterPlan):
 
 
 def filter_field_(out, field, grid, kernel, scale, *, mask_strategy=ZeroFill(), filter_plan=None, backend=None,
-                  method=RealSpace(), engine: str = "auto"):
+                  method=None, engine: str = "auto"):
     """Filtering.jl:266-280 filter_field!."""
     plan = filter_plan if filter_plan is not None else plan_filter(
         grid, kernel, scale, mask_strategy=mask_strategy, backend=backend, method=method, engine=engine)

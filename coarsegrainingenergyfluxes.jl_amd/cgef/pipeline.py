@@ -67,14 +67,16 @@ def _fill_spectrum(g, Cv, k, policy):
 
 def coarse_grain_(result: CoarseGrainResult, u, v, w, grid: StructuredGrid, *, scales, kernel=TopHatKernel(),
                   workspace=None, filter_plans=None, deriv_plan=None, backend=None, mask_strategy=ZeroFill(),
-                  L: float = 1.0, spectrum=StrictSpectrum(), rho0: float = 1.0):
-    """Pipeline.jl:412-468 — per scale: Π into result.Pi[s] and E(ℓ) from the same filtering."""
+                  L: float = 1.0, spectrum=StrictSpectrum(), rho0: float = 1.0, method=None):
+    """Pipeline.jl:412-468 — per scale: Π into result.Pi[s] and E(ℓ) from the same filtering.
+    `method=None` takes plan_filter's default for the grid (Pipeline.jl:366-371)."""
     ns = len(scales)
     if result.Pi.shape[0] != ns or tuple(result.Pi.shape[1:]) != grid.field_shape:
         raise _lib.DimensionMismatch(f"result.Π has shape {tuple(result.Pi.shape)}, expected {(ns,) + grid.field_shape}")
     gate_spectrum(kernel, spectrum)
     plans = filter_plans if filter_plans is not None else [
-        plan_filter(grid, kernel, float(s), mask_strategy=mask_strategy, backend=backend) for s in scales]
+        plan_filter(grid, kernel, float(s), mask_strategy=mask_strategy, backend=backend, method=method)
+        for s in scales]
     _run_coarse_grain(result.Pi, u, v, w, grid, plans, workspace, 1, rho0, result.cumulative_energy)
     result.scales[:] = [float(s) for s in scales]
     result.wavenumber[:] = float(L) / result.scales
@@ -83,11 +85,11 @@ def coarse_grain_(result: CoarseGrainResult, u, v, w, grid: StructuredGrid, *, s
 
 
 def coarse_grain(u, v, w, grid: StructuredGrid, *, scales, kernel=TopHatKernel(), backend=None,
-                 mask_strategy=ZeroFill(), L: float = 1.0, spectrum=StrictSpectrum(), rho0: float = 1.0):
+                 mask_strategy=ZeroFill(), L: float = 1.0, spectrum=StrictSpectrum(), rho0: float = 1.0, method=None):
     """Pipeline.jl:338-358."""
     res = _allocate_result(grid, len(scales), u.dtype, u.device)
     return coarse_grain_(res, u, v, w, grid, scales=scales, kernel=kernel, backend=backend,
-                         mask_strategy=mask_strategy, L=L, spectrum=spectrum, rho0=rho0)
+                         mask_strategy=mask_strategy, L=L, spectrum=spectrum, rho0=rho0, method=method)
 
 
 @dataclass
@@ -111,7 +113,8 @@ class CoarseGrainBatchResult:
 
 def coarse_grain_batch_(batch: CoarseGrainBatchResult, u, v, w, grid: StructuredGrid, *, scales,
                         kernel=TopHatKernel(), workspaces=None, filter_plans=None, deriv_plans=None, backend=None,
-                        mask_strategy=ZeroFill(), L: float = 1.0, spectrum=StrictSpectrum(), rho0: float = 1.0):
+                        mask_strategy=ZeroFill(), L: float = 1.0, spectrum=StrictSpectrum(), rho0: float = 1.0,
+                        method=None):
     """Pipeline.jl:688-724 with the GPU batch driver (519-563): one compute_Π! per scale over the
     whole batch (a single launch sequence covering every snapshot), E(ℓ) per slice."""
     dev = u.device.index if u.is_cuda else -1
@@ -127,7 +130,8 @@ def coarse_grain_batch_(batch: CoarseGrainBatchResult, u, v, w, grid: Structured
     if ws.nbatch < (int(np.prod(bs)) if bs else 1):
         raise _lib.DimensionMismatch("coarse_grain_batch! needs a batch-sized workspace")
     plans = filter_plans if filter_plans is not None else [
-        plan_filter(grid, kernel, float(s), mask_strategy=mask_strategy, backend=backend) for s in scales]
+        plan_filter(grid, kernel, float(s), mask_strategy=mask_strategy, backend=backend, method=method)
+        for s in scales]
     nb = int(np.prod(bs)) if bs else 1
     E = np.zeros(nb * ns)
     _run_coarse_grain(batch.Pi, u, v, w, grid, plans, ws, nb, rho0, E)
@@ -169,7 +173,7 @@ def slice_pipeline_costs(grids, nscales: int):
 
 def coarse_grain_slices_(results, us, vs, ws, grids, *, scales, kernel=TopHatKernel(), workspaces=None,
                          filter_plans=None, deriv_plans=None, backend=None, mask_strategy=ZeroFill(), L: float = 1.0,
-                         spectrum=StrictSpectrum(), rho0: float = 1.0):
+                         spectrum=StrictSpectrum(), rho0: float = 1.0, method=None):
     """Pipeline.jl:866-919 coarse_grain_slices!: a full coarse_grain! sweep per slice, each slice with
     its own grid; scratch is per slice. Slices run longest-first (slice_pipeline_costs)."""
     n = len(results)
@@ -188,5 +192,5 @@ def coarse_grain_slices_(results, us, vs, ws, grids, *, scales, kernel=TopHatKer
         coarse_grain_(results[t], us[t], vs[t], None if ws is None else ws[t], grids[t], scales=scales,
                       kernel=kernel, workspace=None if workspaces is None else workspaces[t],
                       filter_plans=None if filter_plans is None else filter_plans[t], backend=backend,
-                      mask_strategy=mask_strategy, L=L, spectrum=spectrum, rho0=rho0)
+                      mask_strategy=mask_strategy, L=L, spectrum=spectrum, rho0=rho0, method=method)
     return results

@@ -366,6 +366,24 @@ static void ring_filter_planes(Plan &p, Workspace &w, int64_t ncol, hipStream_t 
     ring_inverse(p.ring, w.rout, w.rxy, ncol, s);
 }
 
+// spectral engine: forward row transforms, per-mode column transform × Ĝ and back, row inverse
+static void spectral_filter_planes(Plan &p, Workspace &w, int64_t ncol, hipStream_t s) {
+    ring_forward(p.ring, w.rxy, w.rxf, ncol, s);
+    p.prof_begin(s);
+    ring_spectral_cols(p.ring, w.rxf, w.rout, ncol, s);
+    p.prof_end(s);
+    ring_inverse(p.ring, w.rout, w.rxy, ncol, s);
+}
+
+static void spectral_filter_apply(Plan &p, Workspace &w, int dtype, const void *in, void *out, int64_t nsl,
+                                  hipStream_t s) {
+    const GeomArgs G = geom_args(*p.grid);
+    ring_reserve(w, p, nsl);
+    ring_prep_fields(G, dtype, in, nsl, 0, w.rxy, s);
+    spectral_filter_planes(p, w, nsl, s);
+    ring_spectral_normalize(G, dtype, w.rxy, p.ring.invrenorm_d, nsl, out, s);
+}
+
 // Planes the ring engine filters per slice for Π: the frame velocity components and their products,
 // plus the mask when a Deformable plan on a masked grid needs its filtered value as the denominator.
 static int ring_withmask(const Grid &g, int strategy) { return g.has_mask && strategy == CGEF_DEFORMABLE ? 1 : 0; }
@@ -587,9 +605,13 @@ cgef_status cgef_plan_create(cgef_grid *grid, const cgef_kernel_desc *kernel, do
                 if (prefix_supported(*g, p->kind)) eng = CGEF_ENGINE_PREFIX;
             }
             if (eng != CGEF_ENGINE_AUTO && eng != CGEF_ENGINE_BANDED && eng != CGEF_ENGINE_SEPARABLE &&
-                eng != CGEF_ENGINE_BASELINE && eng != CGEF_ENGINE_RING && eng != CGEF_ENGINE_PREFIX)
+                eng != CGEF_ENGINE_BASELINE && eng != CGEF_ENGINE_RING && eng != CGEF_ENGINE_PREFIX &&
+                eng != CGEF_ENGINE_SPECTRAL)
                 fail(CGEF_ERR_ARGUMENT, "unknown engine");
-            if (eng == CGEF_ENGINE_PREFIX) {
+            if (eng == CGEF_ENGINE_SPECTRAL) {
+                p->engine = CGEF_ENGINE_SPECTRAL;
+                build_spectral(*p, geom_args(*g), nullptr);
+            } else if (eng == CGEF_ENGINE_PREFIX) {
                 p->engine = CGEF_ENGINE_PREFIX;
                 build_prefix(*p);
             } else if (eng == CGEF_ENGINE_SEPARABLE || sep) {
@@ -656,7 +678,7 @@ cgef_status cgef_plan_query(const cgef_plan *plan, cgef_plan_info *info) {
         info->radius = p->rad;
         info->ring_offsets = p->ring.ok ? p->ring.args.noff : 0;
         info->ring_modes = p->ring.ok ? p->ring.mh : 0;
-        info->ring_pi_fields = p->ring.ok ? ring_nfields(*p->grid, ring_withmask(*p->grid, p->strategy)) : 0;
+        info->ring_pi_fields = !p->ring.ok ? 0 : p->ring.spectral ? 5 : ring_nfields(*p->grid, ring_withmask(*p->grid, p->strategy));
     });
 }
 
@@ -706,6 +728,8 @@ cgef_status cgef_filter_apply(cgef_plan *plan, void *out, const void *in, int64_
         } else if (p->engine == CGEF_ENGINE_PREFIX) {
             Workspace &w = resolve_ws(g, nullptr, 1, dtype);
             launch_prefix(G, *p, dtype, in, out, nsl, (double *)ws_fbuf(w, prefix_bytes(g, nsl)), s);
+        } else if (p->engine == CGEF_ENGINE_SPECTRAL) {
+            spectral_filter_apply(*p, resolve_ws(g, nullptr, 1, dtype), dtype, in, out, nsl, s);
         } else {
             if (dtype != CGEF_F64) fail(CGEF_ERR_ARGUMENT, "the baseline engine filters FP64 fields");
             launch_baseline_filter(G, *p, (const double *)in, (double *)out, nsl, s);
@@ -766,6 +790,14 @@ cgef_status cgef_compute_pi(cgef_plan *plan, cgef_workspace *ws, void *Pi, const
             launch_strain_pi(G, dtype, W.ubar, W.tau, Pi, rho0, nbatch, W.partials, W.npartials, s);
             CGEF_HIP(hipGetLastError());
             if (E_out) fetch_energy(W, nbatch, strain_blocks(G), g.active_area, E_out, s);
+        } else if (p->engine == CGEF_ENGINE_SPECTRAL) {
+            ring_reserve(W, *p, 5 * nbatch);
+            ring_prep(G, dtype, u, v, w, 5, 0, W.rxy, nbatch, s);
+            spectral_filter_planes(*p, W, 5 * nbatch, s);
+            ring_spectral_epilogue(G, dtype, W.rxy, p->ring.invrenorm_d, W.ubar, W.tau, ubar, vbar, wbar, nbatch, s);
+            launch_strain_pi(G, dtype, W.ubar, W.tau, Pi, rho0, nbatch, W.partials, W.npartials, s);
+            CGEF_HIP(hipGetLastError());
+            if (E_out) fetch_energy(W, nbatch, strain_blocks(G), g.active_area, E_out, s);
         } else {
             compute_pi_unfused(*p, W, Pi, u, v, w, nbatch, dtype, rho0, E_out, s);
         }
@@ -787,11 +819,12 @@ cgef_status cgef_coarse_grain(cgef_plan *const *plans, int64_t nscales, cgef_wor
         check_dtype(dtype);
         if (nscales == 0) return;
         Grid &g = *reinterpret_cast<Plan *>(plans[0])->grid;
-        bool all_ring = true;
+        bool all_ring = true, all_spec = true;
         for (int64_t q = 0; q < nscales; q++) {
             Plan *p = reinterpret_cast<Plan *>(plans[q]);
             if (!p || p->grid != &g) fail(CGEF_ERR_ARGUMENT, "every plan must belong to the same grid");
             all_ring = all_ring && p->engine == CGEF_ENGINE_RING;
+            all_spec = all_spec && p->engine == CGEF_ENGINE_SPECTRAL;
         }
         if (g.ndim == 3 && !w) fail(CGEF_ERR_ARGUMENT, "3-D Π needs the vertical velocity w");
         if (g.ndim == 2 && w)
@@ -822,6 +855,26 @@ cgef_status cgef_coarse_grain(cgef_plan *const *plans, int64_t nscales, cgef_wor
                 ring_inverse(p.ring, W.rout, W.rxy, nf * nbatch, s);
                 ring_epilogue(G, dtype, W.rxy, p.D64_row_d, nf, wm, p.strategy, W.ubar, W.tau, nullptr, nullptr, nullptr,
                               nbatch, s);
+                launch_strain_pi(G, dtype, W.ubar, W.tau, (char *)Pi + (size_t)q * P * es, rho0, nbatch, W.partials,
+                                 W.npartials, s, nscales * P);
+                if (E_out) launch_energy_final(nbatch, nblk, W.partials, g.active_area, Eall + q * nbatch, s);
+            }
+            CGEF_HIP(hipGetLastError());
+        } else if (all_spec) {
+            // the spectrum of the masked inputs is taken once and reused by every scale
+            // (FFTWExt.jl:100-120 filter_analyze! / filter_synthesize!)
+            Plan &p0 = *reinterpret_cast<Plan *>(plans[0]);
+            ring_reserve(W, p0, 5 * nbatch);
+            ring_prep(G, dtype, u, v, w, 5, 0, W.rxy, nbatch, s);
+            ring_forward(p0.ring, W.rxy, W.rxf, 5 * nbatch, s);
+            for (int64_t q = 0; q < nscales; q++) {
+                Plan &p = *reinterpret_cast<Plan *>(plans[q]);
+                p.prof_begin(s);
+                ring_spectral_cols(p.ring, W.rxf, W.rout, 5 * nbatch, s);
+                p.prof_end(s);
+                ring_inverse(p.ring, W.rout, W.rxy, 5 * nbatch, s);
+                ring_spectral_epilogue(G, dtype, W.rxy, p.ring.invrenorm_d, W.ubar, W.tau, nullptr, nullptr, nullptr,
+                                       nbatch, s);
                 launch_strain_pi(G, dtype, W.ubar, W.tau, (char *)Pi + (size_t)q * P * es, rho0, nbatch, W.partials,
                                  W.npartials, s, nscales * P);
                 if (E_out) launch_energy_final(nbatch, nblk, W.partials, g.active_area, Eall + q * nbatch, s);
@@ -920,6 +973,8 @@ cgef_status cgef_cumulative_energy(cgef_plan *const *plans, int64_t nscales, cge
                 } else if (p->engine == CGEF_ENGINE_PREFIX) {
                     double *pb = ws_scratch64(W, (int64_t)(prefix_bytes(g, nbatch) / sizeof(double)));
                     launch_prefix(G, *p, dtype, in, out, nbatch, pb, s);
+                } else if (p->engine == CGEF_ENGINE_SPECTRAL) {
+                    spectral_filter_apply(*p, W, dtype, in, out, nbatch, s);
                 } else {
                     if (dtype != CGEF_F64) fail(CGEF_ERR_ARGUMENT, "the baseline engine filters FP64 fields");
                     launch_baseline_filter(G, *p, (const double *)in, (double *)out, nbatch, s);

@@ -67,6 +67,14 @@ struct RingArgs {
 
 struct RingPlan {
     bool ok = false;
+    // spectral engine (FFTWExt.jl:121-208): the same row transforms, then per Fourier mode along x a
+    // transform along y, the transfer function Ĝ(|k|), and back — instead of the banded contraction
+    bool spectral = false;
+    RingFft fy;
+    double2 *twy_d = nullptr, *twsy_d = nullptr;
+    int kind = CGEF_GAUSSIAN;
+    double alpha = 6.0, ell = 0.0, dkx = 0.0, dky = 0.0;
+    double *invrenorm_d = nullptr;     // Deformable on a masked grid: 1/filter(mask) where ≥ 0.01, else 0
     RingFft fft;
     int64_t mh = 0;                    // N/2 + 1 Fourier modes kept
     double2 *tw_d = nullptr;           // exp(−2πik/N), k < N

@@ -609,6 +609,89 @@ __global__ __launch_bounds__(256) void k_ring_normalize(GeomArgs G, int dtype, c
     else ((double *)out)[s * P + t] = v;
 }
 
+// ---- spectral engine (FFTWExt.jl:121-208): the row spectra XF[m][col][row] of a doubly periodic
+// 2-D grid, per (m, col) a contiguous column of Ny values, are transformed along y, multiplied by the
+// kernel's transfer function Ĝ(|k|) and transformed back, so the row inverse yields
+// irfft2(Ĝ · rfft2(mask·f)).
+
+// Kernels.jl spectral_transfer: Gaussian exp(−k²ℓ²/(4α)); top-hat 2·J₁(x)/x with x = kℓ/2.
+__device__ __forceinline__ double spectral_transfer(int kind, double alpha, double k, double ell) {
+    if (kind == CGEF_GAUSSIAN) return exp(-k * k * ell * ell / (4.0 * alpha));
+    const double x = 0.5 * k * ell;
+    return x == 0.0 ? 1.0 : 2.0 * j1(x) / x;
+}
+
+template <bool STAGED>
+__global__ __launch_bounds__(256) void k_spec_cols(RingFft F, const double2 *__restrict__ tw,
+                                                   const double2 *__restrict__ tws, int kind, double alpha, double ell,
+                                                   double dkx, double dky, const double2 *__restrict__ XF,
+                                                   double2 *__restrict__ OUT, int64_t ncol, int64_t mh) {
+    extern __shared__ double2 sm[];
+    const int n = F.n;
+    int64_t item;
+    if (!xcd_item(mh * ncol, item)) return;
+    const int64_t m = item / ncol;
+    const double2 *x = XF + item * n;
+    for (int e = threadIdx.x; e < n; e += 256) sm[e] = x[e];
+    double2 *z = sm;
+    if (STAGED) fft_staged<1>(sm, F, tw, tws, false);
+    else z = fft_lds(sm, sm + n, F, tw, false, 1);
+    // angular wavenumbers 2π·m/Lx (rfftfreq) and 2π·n/Ly with n folded to −Ny/2..(Ny−1)/2 (fftfreq)
+    const double kx = dkx * (double)m, sc = 1.0 / (double)n;
+    for (int e = threadIdx.x; e < n; e += 256) {
+        const int nn = e <= (n - 1) / 2 ? e : e - n;
+        const double ky = dky * (double)nn;
+        const double g = spectral_transfer(kind, alpha, sqrt(kx * kx + ky * ky), ell) * sc;
+        z[e] = make_double2(z[e].x * g, z[e].y * g);
+    }
+    double2 *r = z;
+    if (STAGED) fft_staged<1>(z, F, tw, tws, true);
+    else r = fft_lds(z, z == sm ? sm + n : sm, F, tw, true, 1);
+    double2 *o = OUT + item * n;
+    for (int e = threadIdx.x; e < n; e += 256) o[e] = r[e];
+}
+
+// FFTWExt.jl:162-176: the filtered mask inverted where |filter(mask)| ≥ 0.01, else 0
+__global__ __launch_bounds__(256) void k_spec_invert(int64_t P, const double *__restrict__ r, double *__restrict__ ir) {
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= P) return;
+    const double v = r[t];
+    ir[t] = fabs(v) >= 0.01 ? 1.0 / v : 0.0;
+}
+
+// filter_apply: the transform's output (× 1/filter(mask) for Deformable on a masked grid), with no
+// masking of the output (FFTWExt.jl:193-208)
+__global__ __launch_bounds__(256) void k_spec_normalize(GeomArgs G, int dtype, const double *__restrict__ yout,
+                                                        const double *__restrict__ invrenorm, void *out) {
+    const int64_t s = blockIdx.y;
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t P = G.npts;
+    if (t >= P) return;
+    const double v = invrenorm ? yout[s * P + t] * invrenorm[t] : yout[s * P + t];
+    if (dtype == CGEF_F32) ((float *)out)[s * P + t] = (float)v;
+    else ((double *)out)[s * P + t] = v;
+}
+
+// Π from the five filtered planes of a 2-D Cartesian slice: every point carries the transform's value
+// (the strain reads masked neighbours as the filter left them); Π itself is masked by the strain kernel.
+template <typename T>
+__global__ __launch_bounds__(256) void k_spec_epilogue(GeomArgs G, const double *__restrict__ yout,
+                                                       const double *__restrict__ invrenorm, double *__restrict__ ubar,
+                                                       T *__restrict__ tau, T *__restrict__ uo, T *__restrict__ vo,
+                                                       T *__restrict__ wo) {
+    const int64_t b = blockIdx.y;
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t P = G.npts;
+    if (t >= P) return;
+    const int64_t Nx = G.n[0];
+    const int64_t i = t % Nx, j = t / Nx;
+    const double *y = yout + b * 5 * P;
+    double F[9];
+#pragma unroll
+    for (int f = 0; f < 5; f++) F[f] = y[f * P + t];
+    pi_point_epilogue_inv<T>(G, t, i, j, F, true, invrenorm ? invrenorm[t] : 1.0, b, ubar, tau, uo, vo, wo);
+}
+
 // ------------------------------------------------------------------------------------ launches ----
 
 static size_t fft_shmem(const RingFft &F) { return (size_t)2 * F.n * sizeof(double2); }
@@ -697,6 +780,42 @@ void ring_normalize(const GeomArgs &G, int dtype, const double *yout, const doub
                     int strategy, void *out, hipStream_t s) {
     dim3 grid((unsigned)((G.npts + 255) / 256), (unsigned)nsl);
     hipLaunchKernelGGL(k_ring_normalize, grid, dim3(256), 0, s, G, dtype, yout, D_row, nsl, withmask, strategy, out);
+}
+
+void ring_spectral_cols(const RingPlan &R, const double2 *XF, double2 *OUT, int64_t ncol, hipStream_t s) {
+    const int64_t items = R.mh * ncol;
+    const dim3 grid((unsigned)(8 * ((items + 7) / 8)));
+    if (R.fy.nseq > 0) {
+        const size_t sh = staged_shmem(R.fy, 1);
+        hipLaunchKernelGGL(k_spec_cols<true>, grid, dim3(256), sh, s, R.fy, R.twy_d, R.twsy_d, R.kind, R.alpha, R.ell,
+                           R.dkx, R.dky, XF, OUT, ncol, R.mh);
+    } else {
+        const size_t sh = fft_shmem(R.fy);
+        set_shmem((const void *)k_spec_cols<false>, sh);
+        hipLaunchKernelGGL(k_spec_cols<false>, grid, dim3(256), sh, s, R.fy, R.twy_d, R.twsy_d, R.kind, R.alpha, R.ell,
+                           R.dkx, R.dky, XF, OUT, ncol, R.mh);
+    }
+}
+
+void ring_spectral_invert(const GeomArgs &G, const double *renorm, double *invrenorm, hipStream_t s) {
+    hipLaunchKernelGGL(k_spec_invert, dim3((unsigned)((G.npts + 255) / 256)), dim3(256), 0, s, G.npts, renorm, invrenorm);
+}
+
+void ring_spectral_normalize(const GeomArgs &G, int dtype, const double *yout, const double *invrenorm, int64_t nsl,
+                             void *out, hipStream_t s) {
+    dim3 grid((unsigned)((G.npts + 255) / 256), (unsigned)nsl);
+    hipLaunchKernelGGL(k_spec_normalize, grid, dim3(256), 0, s, G, dtype, yout, invrenorm, out);
+}
+
+void ring_spectral_epilogue(const GeomArgs &G, int dtype, const double *yout, const double *invrenorm, double *ubar,
+                            void *tau, void *uo, void *vo, void *wo, int64_t nbatch, hipStream_t s) {
+    dim3 grid((unsigned)((G.npts + 255) / 256), (unsigned)nbatch);
+    if (dtype == CGEF_F32)
+        hipLaunchKernelGGL(k_spec_epilogue<float>, grid, dim3(256), 0, s, G, yout, invrenorm, ubar, (float *)tau,
+                           (float *)uo, (float *)vo, (float *)wo);
+    else
+        hipLaunchKernelGGL(k_spec_epilogue<double>, grid, dim3(256), 0, s, G, yout, invrenorm, ubar, (double *)tau,
+                           (double *)uo, (double *)vo, (double *)wo);
 }
 
 }  // namespace cgef

@@ -16,6 +16,16 @@ bool ring_factor(int n, RingFft &F);
 // ping-pong FFT serves the plan).
 int ring_staged_nseq(const RingFft &F);
 bool ring_supported(const Plan &p);
+void fft_tables(RingFft &F, double2 *&tw_d, double2 *&tws_d);
+bool spectral_supported(const Grid &g, int kind);
+void build_spectral(Plan &p, const GeomArgs &G, hipStream_t s);
+// spectral engine launches: per (mode along x, plane column) transform along y, × Ĝ, transform back
+void ring_spectral_cols(const RingPlan &R, const double2 *XF, double2 *OUT, int64_t ncol, hipStream_t s);
+void ring_spectral_invert(const GeomArgs &G, const double *renorm, double *invrenorm, hipStream_t s);
+void ring_spectral_normalize(const GeomArgs &G, int dtype, const double *yout, const double *invrenorm, int64_t nsl,
+                             void *out, hipStream_t s);
+void ring_spectral_epilogue(const GeomArgs &G, int dtype, const double *yout, const double *invrenorm, double *ubar,
+                            void *tau, void *uo, void *vo, void *wo, int64_t nbatch, hipStream_t s);
 void build_ring(Plan &p, hipStream_t s);
 void free_ring(RingPlan &R);
 void ring_build_what(const RingPlan &R, const Run *runs_d, const int32_t *run_off_d, const double *w64_d, int64_t nruns,

@@ -50,6 +50,97 @@ bool ring_supported(const Plan &p) {
     return true;
 }
 
+// Twiddle tables of a factored length: exp(−2πik/N) and the per-stage table of the register-staged
+// FFT (stage of radix R after Ns points: exp(−2πi·r·k/(Ns·R)) at twoff + k·(R−1) + r−1).
+void fft_tables(RingFft &F, double2 *&tw_d, double2 *&tws_d) {
+    const int n = F.n;
+    std::vector<double2> tw(n);
+    for (int k = 0; k < n; k++) {
+        const double a = 2.0 * M_PI * (double)k / (double)n;
+        tw[k] = make_double2(std::cos(a), -std::sin(a));
+    }
+    tw_d = up(tw);
+    std::vector<double2> tws;
+    int Ns = 1;
+    for (int st = 0; st < F.nstage; st++) {
+        const int Rr = F.radix[st];
+        F.twoff[st] = (int)tws.size();
+        for (int k = 0; k < Ns; k++)
+            for (int r = 1; r < Rr; r++) {
+                const double a = 2.0 * M_PI * (double)r * (double)k / ((double)Ns * (double)Rr);
+                tws.push_back(make_double2(std::cos(a), -std::sin(a)));
+            }
+        Ns *= Rr;
+    }
+    if (tws.empty()) tws.push_back(make_double2(1.0, 0.0));
+    F.twsize = (int)tws.size();
+    tws_d = up(tws);
+    F.nseq = ring_staged_nseq(F);
+}
+
+// The spectral engine: row transforms of the ring engine along x, register-staged transforms along y.
+bool spectral_supported(const Grid &g, int kind) {
+    if (g.ndim != 2 || g.geometry != CGEF_CARTESIAN || (kind != CGEF_GAUSSIAN && kind != CGEF_TOPHAT)) return false;
+    // the transform needs only the periods (k = 2π·m/L), not the coordinates (FFTWExt.jl:138-142)
+    if (!g.periodic[0] || !g.periodic[1] || g.n[0] < 2 || g.n[1] < 2) return false;
+    RingFft fx, fy;
+    if (!ring_factor((int)g.n[0], fx) || !ring_factor((int)g.n[1], fy)) return false;
+    // each transform runs in one workgroup's LDS: register-staged, else the ping-pong pair
+    auto fits = [](const RingFft &F) {
+        return ring_staged_nseq(F) > 0 || (size_t)2 * F.n * sizeof(double2) <= 160 * 1024;
+    };
+    return fits(fx) && fits(fy);
+}
+
+void build_spectral(Plan &p, const GeomArgs &G, hipStream_t s) {
+    Grid &g = *p.grid;
+    if (!spectral_supported(g, p.kind))
+        fail(CGEF_ERR_UNSUPPORTED,
+             "the spectral engine needs a doubly periodic 2-D Cartesian grid whose sizes factor into "
+             "2,3,5,7,11,13, at most 5120 points each");
+    RingPlan &R = p.ring;
+    ring_factor((int)g.n[0], R.fft);
+    ring_factor((int)g.n[1], R.fy);
+    fft_tables(R.fft, R.tw_d, R.tws_d);
+    fft_tables(R.fy, R.twy_d, R.twsy_d);
+    R.mh = g.n[0] / 2 + 1;
+    RingArgs &A = R.args;
+    A.ny = g.n[1];
+    A.nz = 1;
+    A.nrows = g.n[1];
+    A.mh = R.mh;
+    A.py = 1;
+    R.spectral = true;
+    R.kind = p.kind;
+    R.alpha = p.alpha;
+    R.ell = p.scale;
+    // the transform's wavenumbers 2π·m/L (FFTWExt.jl:138-146)
+    R.dkx = 2.0 * M_PI / g.period[0];
+    R.dky = 2.0 * M_PI / g.period[1];
+    R.ok = true;
+    if (g.has_mask && p.strategy == CGEF_DEFORMABLE) {
+        // FFTWExt.jl:162-176: the mask's own filtered value, inverted where ≥ 0.01, built once
+        struct Tmp {
+            void *p = nullptr;
+            ~Tmp() {
+                if (p) (void)hipFree(p);
+            }
+        } xin, xf, xo;
+        const size_t nxf = (size_t)R.mh * g.n[1];
+        CGEF_HIP(hipMalloc(&xin.p, (size_t)g.npts * sizeof(double)));
+        CGEF_HIP(hipMalloc(&xf.p, nxf * sizeof(double2)));
+        CGEF_HIP(hipMalloc(&xo.p, nxf * sizeof(double2)));
+        CGEF_HIP(hipMalloc(&R.invrenorm_d, (size_t)g.npts * sizeof(double)));
+        ring_prep_fields(G, CGEF_F64, nullptr, 0, 1, (double *)xin.p, s);
+        ring_forward(R, (const double *)xin.p, (double2 *)xf.p, 1, s);
+        ring_spectral_cols(R, (const double2 *)xf.p, (double2 *)xo.p, 1, s);
+        ring_inverse(R, (const double2 *)xo.p, (double *)xin.p, 1, s);
+        ring_spectral_invert(G, (const double *)xin.p, R.invrenorm_d, s);
+        CGEF_HIP(hipGetLastError());
+        CGEF_HIP(hipStreamSynchronize(s));
+    }
+}
+
 // Requires p.runs / p.run_ptr / p.w64 from build_banded (before any run splitting) and the device
 // copies runs_d / w64_d / band_of_row_d.
 void build_ring(Plan &p, hipStream_t s) {
@@ -94,28 +185,7 @@ void build_ring(Plan &p, hipStream_t s) {
             run_off[2 * q] = b;
             run_off[2 * q + 1] = offs[{p.runs[q].dz, p.runs[q].dy}];
         }
-    std::vector<double2> tw(n);
-    for (int k = 0; k < n; k++) {
-        const double a = 2.0 * M_PI * (double)k / (double)n;
-        tw[k] = make_double2(std::cos(a), -std::sin(a));
-    }
-    R.tw_d = up(tw);
-    std::vector<double2> tws;
-    int Ns = 1;
-    for (int st = 0; st < R.fft.nstage; st++) {
-        const int Rr = R.fft.radix[st];
-        R.fft.twoff[st] = (int)tws.size();
-        for (int k = 0; k < Ns; k++)
-            for (int r = 1; r < Rr; r++) {
-                const double a = 2.0 * M_PI * (double)r * (double)k / ((double)Ns * (double)Rr);
-                tws.push_back(make_double2(std::cos(a), -std::sin(a)));
-            }
-        Ns *= Rr;
-    }
-    if (tws.empty()) tws.push_back(make_double2(1.0, 0.0));
-    R.fft.twsize = (int)tws.size();
-    R.tws_d = up(tws);
-    R.fft.nseq = ring_staged_nseq(R.fft);
+    fft_tables(R.fft, R.tw_d, R.tws_d);
     R.off_dy_d = up(off_dy);
     R.dz_val_d = up(dz_val);
     R.dz_off0_d = up(dz_off0);
@@ -146,6 +216,9 @@ void build_ring(Plan &p, hipStream_t s) {
 }
 
 void free_ring(RingPlan &R) {
+    if (R.twy_d) (void)hipFree(R.twy_d);
+    if (R.twsy_d) (void)hipFree(R.twsy_d);
+    if (R.invrenorm_d) (void)hipFree(R.invrenorm_d);
     if (R.tw_d) (void)hipFree(R.tw_d);
     if (R.tws_d) (void)hipFree(R.tws_d);
     if (R.what_d) (void)hipFree(R.what_d);

@@ -40,8 +40,11 @@ typedef enum {
     CGEF_ENGINE_BASELINE = 3,  /* plain FP64 direct sum, one thread per point, CSR in HBM */
     CGEF_ENGINE_RING = 4,      /* the same banded footprint evaluated in Fourier space along a
                                   periodic x axis (FP64 FFT + banded contraction over rows) */
-    CGEF_ENGINE_PREFIX = 5     /* top-hat on a 2-D rectilinear grid by per-row prefix sums
+    CGEF_ENGINE_PREFIX = 5,    /* top-hat on a 2-D rectilinear grid by per-row prefix sums
                                   (Filtering.jl:751-1141), the reference's top-hat engine */
+    CGEF_ENGINE_SPECTRAL = 6   /* method = Spectral() on a doubly periodic uniform 2-D Cartesian
+                                  grid: irfft2(Ĝ(|k|)·rfft2(mask·f)), FFTWExt.jl:121-208; never
+                                  chosen by AUTO (the reference's default method is RealSpace) */
 } cgef_engine;
 
 /* Grid description. Reference: FlowGeometries.Grids.StructuredGrid (un-vendored), as consumed by

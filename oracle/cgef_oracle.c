@@ -11,6 +11,7 @@
  * unpinned" items); the product kernels make the same choice, and the tests pin these functions
  * against closed forms, invariants and brute force instead.
  */
+#define _DEFAULT_SOURCE /* j1 (Bessel function of the first kind) from <math.h> */
 #include "cgef_oracle.h"
 
 #include <float.h>
@@ -351,6 +352,108 @@ void og_apply_separable(const og_grid *g, double alpha, double ell, int strategy
             if (!active_at(g, t)) out[m][t] = 0.0;
     }
     free(gx); free(gy); free(rp); free(mi); free(inv); free(px_prof); free(py_prof);
+}
+
+/* ------------------------- spectral filter (FFTW extension, FFTWExt.jl:121-208) ------------ */
+
+/* Kernels.jl:405-406 (Gaussian) and the SpecialFunctions extension's disk transform for the top-hat:
+ * Ĝ(k) = exp(−k²ℓ²/(4α)); 2·J₁(kR)/(kR) with R = ℓ/2; Ĝ(0) = 1. */
+double og_spectral_transfer(int kind, double alpha, double k, double ell) {
+    if (kind == OG_GAUSSIAN) return exp(-k * k * ell * ell / (4.0 * alpha));
+    double x = k * ell / 2.0;
+    return x == 0.0 ? 1.0 : 2.0 * j1(x) / x;
+}
+
+/* Periodic uniform 2-D Cartesian grid: out = irfft2(rfft2(mask·f) · Ĝ(|k|)), with |k| from the
+ * transform's wavenumbers 2π·m/L (FFTWExt.jl:138-146); Deformable multiplies by 1/filter(mask) where
+ * |filter(mask)| ≥ 0.01, else 0 (FFTWExt.jl:162-176). No masking of the output. Direct DFTs along
+ * each axis (an independent restatement of the transform, O(N·(Nx + Ny)) per field). */
+static void spectral_one(const og_grid *g, const double *G, const double *cx, const double *sx, const double *cy,
+                         const double *sy, const double *a, double *o) {
+    const int64_t Nx = g->n[0], Ny = g->n[1], mh = Nx / 2 + 1;
+    double *Xr = (double *)malloc(mh * Ny * sizeof(double)), *Xi = (double *)malloc(mh * Ny * sizeof(double));
+    double *Yr = (double *)malloc(Ny * sizeof(double)), *Yi = (double *)malloc(Ny * sizeof(double));
+    for (int64_t j = 0; j < Ny; j++)
+        for (int64_t m = 0; m < mh; m++) {
+            double re = 0.0, im = 0.0;
+            for (int64_t i = 0; i < Nx; i++) {
+                int64_t q = (m * i) % Nx;
+                re += a[i + Nx * j] * cx[q];
+                im -= a[i + Nx * j] * sx[q];
+            }
+            Xr[m + mh * j] = re;
+            Xi[m + mh * j] = im;
+        }
+    for (int64_t m = 0; m < mh; m++) {
+        for (int64_t n = 0; n < Ny; n++) {
+            double re = 0.0, im = 0.0;
+            for (int64_t j = 0; j < Ny; j++) {
+                int64_t q = (n * j) % Ny;
+                double xr = Xr[m + mh * j], xi = Xi[m + mh * j];
+                re += xr * cy[q] + xi * sy[q];
+                im += xi * cy[q] - xr * sy[q];
+            }
+            Yr[n] = re * G[m + mh * n];
+            Yi[n] = im * G[m + mh * n];
+        }
+        for (int64_t j = 0; j < Ny; j++) {
+            double re = 0.0, im = 0.0;
+            for (int64_t n = 0; n < Ny; n++) {
+                int64_t q = (n * j) % Ny;
+                re += Yr[n] * cy[q] - Yi[n] * sy[q];
+                im += Yi[n] * cy[q] + Yr[n] * sy[q];
+            }
+            Xr[m + mh * j] = re / (double)Ny;
+            Xi[m + mh * j] = im / (double)Ny;
+        }
+    }
+    for (int64_t j = 0; j < Ny; j++)
+        for (int64_t i = 0; i < Nx; i++) {
+            double acc = Xr[mh * j]; /* m = 0: real part only */
+            for (int64_t m = 1; m < mh; m++) {
+                int64_t q = (m * i) % Nx;
+                double term = Xr[m + mh * j] * cx[q] - Xi[m + mh * j] * sx[q];
+                acc += (Nx % 2 == 0 && m == Nx / 2) ? term : 2.0 * term;
+            }
+            o[i + Nx * j] = acc / (double)Nx;
+        }
+    free(Xr); free(Xi); free(Yr); free(Yi);
+}
+
+void og_apply_spectral(const og_grid *g, int kind, double alpha, double ell, int strategy, int K,
+                       const double *const *in, double *const *out) {
+    const int64_t Nx = g->n[0], Ny = g->n[1], mh = Nx / 2 + 1, N = Nx * Ny;
+    const double tau = 2.0 * acos(-1.0);
+    double *cx = (double *)malloc(Nx * sizeof(double)), *sx = (double *)malloc(Nx * sizeof(double));
+    double *cy = (double *)malloc(Ny * sizeof(double)), *sy = (double *)malloc(Ny * sizeof(double));
+    for (int64_t q = 0; q < Nx; q++) { cx[q] = cos(tau * q / Nx); sx[q] = sin(tau * q / Nx); }
+    for (int64_t q = 0; q < Ny; q++) { cy[q] = cos(tau * q / Ny); sy[q] = sin(tau * q / Ny); }
+    const double dkx = tau / g->period[0], dky = tau / g->period[1];
+    double *G = (double *)malloc(mh * Ny * sizeof(double));
+    for (int64_t n = 0; n < Ny; n++) {
+        int64_t nn = n <= (Ny - 1) / 2 ? n : n - Ny;
+        for (int64_t m = 0; m < mh; m++) {
+            double kx = dkx * (double)m, ky = dky * (double)nn;
+            G[m + mh * n] = og_spectral_transfer(kind, alpha, sqrt(kx * kx + ky * ky), ell);
+        }
+    }
+    int masked = 0;
+    if (g->mask)
+        for (int64_t t = 0; t < N; t++) if (!g->mask[t]) { masked = 1; break; }
+    double *a = (double *)malloc(N * sizeof(double)), *ir = NULL;
+    if (masked && strategy == OG_DEFORMABLE) {
+        ir = (double *)malloc(N * sizeof(double));
+        for (int64_t t = 0; t < N; t++) a[t] = active_at(g, t) ? 1.0 : 0.0;
+        spectral_one(g, G, cx, sx, cy, sy, a, ir);
+        for (int64_t t = 0; t < N; t++) ir[t] = fabs(ir[t]) >= 0.01 ? 1.0 / ir[t] : 0.0;
+    }
+    for (int q = 0; q < K; q++) {
+        for (int64_t t = 0; t < N; t++) a[t] = masked ? (active_at(g, t) ? in[q][t] : 0.0) : in[q][t];
+        spectral_one(g, G, cx, sx, cy, sy, a, out[q]);
+        if (ir)
+            for (int64_t t = 0; t < N; t++) out[q][t] *= ir[t];
+    }
+    free(cx); free(sx); free(cy); free(sy); free(G); free(a); free(ir);
 }
 
 /* ------------------------------ 3-D ball (Filtering.jl:2172-2203, 2268-2458) ---------------- */
@@ -727,6 +830,7 @@ static void filt(const og_grid *g, const og_filter_spec *s, const og_footprint *
     if (s->engine == 1) og_apply_separable(g, s->alpha, s->ell, s->strategy, K, in, out);
     else if (s->engine == 2) og_apply_nd3(g, s->kind, s->alpha, s->ell, s->strategy, K, in, out, s->nthreads);
     else if (s->engine == 3) og_apply_prefix(g, s->ell, s->strategy, K, in, out, s->nthreads);
+    else if (s->engine == 4) og_apply_spectral(g, s->kind, s->alpha, s->ell, s->strategy, K, in, out);
     else og_apply_banded(g, fp, s->strategy, K, in, out, s->nthreads);
 }
 

@@ -76,6 +76,11 @@ double og_metric_band(const og_grid *g, double y1, double y2, double rad);
 void og_apply_prefix(const og_grid *g, double ell, int strategy, int K, const double *const *in,
                      double *const *out, int nthreads);
 
+/* Spectral filter on a periodic uniform 2-D Cartesian grid (the reference's FFTW extension). */
+double og_spectral_transfer(int kind, double alpha, double k, double ell);
+void og_apply_spectral(const og_grid *g, int kind, double alpha, double ell, int strategy, int K,
+                       const double *const *in, double *const *out);
+
 /* 3-D Cartesian ball footprint applied per point (uniform or stretched axes). */
 void og_apply_nd3(const og_grid *g, int kind, double alpha, double ell, int strategy, int K,
                   const double *const *in, double *const *out, int nthreads);
@@ -89,7 +94,7 @@ void og_vector_from_cartesian(double X, double Y, double Z, double lam, double p
 void og_tensor_to_local(const double t[6], double lam, double phi, double out[6]);
 
 /* engine: 0 banded footprint, 1 separable (Cartesian Gaussian only), 2 3-D ball, 3 prefix-sum
- * top-hat (2-D). */
+ * top-hat (2-D), 4 spectral (periodic uniform 2-D Cartesian). */
 typedef struct {
     int32_t kind;
     double alpha;

@@ -19,7 +19,7 @@ _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 
 TOPHAT, GAUSSIAN = 0, 1
 ZEROFILL, DEFORMABLE = 0, 1
-ENGINE_BANDED, ENGINE_SEPARABLE, ENGINE_ND3, ENGINE_PREFIX = 0, 1, 2, 3
+ENGINE_BANDED, ENGINE_SEPARABLE, ENGINE_ND3, ENGINE_PREFIX, ENGINE_SPECTRAL = 0, 1, 2, 3, 4
 
 
 def build(force: bool = False) -> str:
@@ -85,6 +85,10 @@ def lib():
         L.og_apply_nd3.argtypes = [gp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
                                    C.POINTER(dp), C.POINTER(dp), C.c_int]
         L.og_apply_prefix.argtypes = [gp, C.c_double, C.c_int, C.c_int, C.POINTER(dp), C.POINTER(dp), C.c_int]
+        L.og_apply_spectral.argtypes = [gp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, C.POINTER(dp),
+                                        C.POINTER(dp)]
+        L.og_spectral_transfer.restype = C.c_double
+        L.og_spectral_transfer.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double]
         L.og_metric_band.restype = C.c_double
         L.og_metric_band.argtypes = [gp, C.c_double, C.c_double, C.c_double]
         L.og_deriv.argtypes = [gp, C.c_int, dp, dp]
@@ -231,6 +235,19 @@ def apply_prefix(grid, ell, strategy, fields, nthreads=0):
     return outs
 
 
+def apply_spectral(grid, kind, alpha, ell, strategy, fields):
+    """FFTWExt.jl:121-208: transform filter on a periodic uniform 2-D Cartesian grid."""
+    ins, pin = _arr(fields)
+    outs = [np.empty_like(f) for f in ins]
+    pout = (C.POINTER(C.c_double) * len(outs))(*[_dp(o) for o in outs])
+    lib().og_apply_spectral(grid.ref(), kind, alpha, ell, strategy, len(ins), pin, pout)
+    return outs
+
+
+def spectral_transfer(kind, alpha, k, ell):
+    return lib().og_spectral_transfer(kind, alpha, k, ell)
+
+
 def metric_band(grid, y1, y2, rad):
     return lib().og_metric_band(grid.ref(), y1, y2, rad)
 
@@ -306,6 +323,8 @@ def filter_fields(grid, fields, *, kind, alpha, ell, strategy=ZEROFILL, engine=E
         return apply_nd3(grid, kind, alpha, ell, strategy, fields, nthreads)
     if engine == ENGINE_PREFIX:
         return apply_prefix(grid, ell, strategy, fields, nthreads)
+    if engine == ENGINE_SPECTRAL:
+        return apply_spectral(grid, kind, alpha, ell, strategy, fields)
     if fp is None:
         fp = Footprint(grid, kind, alpha, ell)
     return apply_banded(grid, fp, strategy, fields, nthreads)

@@ -516,3 +516,100 @@ def test_prefix_pi_and_default_pipeline(case, dtype):
     # the pipeline's default kernel is the top-hat (Pipeline.jl:344): coarse_grain takes this engine
     res = cgef.coarse_grain(U, V, None, g, scales=[ell], spectrum=cgef.NoSpectrum())
     assert rel_l2(N(res.Pi[0]), ref) <= tol
+
+
+# ------------------------------------------------------------------ spectral (FFTWExt.jl:121-208) --
+
+def spectral_case(ny, nx, masked, seed=41):
+    xs, ys = cart_axes(nx, 1000.0, 500.0), cart_axes(ny, 1250.0)
+    mask = blob_mask((ny, nx), seed) if masked else None
+    og = oracle_grid_cart(xs, ys, periodic=(True, True), mask=mask)
+    g = cgef.StructuredGrid(cgef.CartesianGeometry(), xs, ys, mask=mask, periodic=(True, True))
+    return og, g
+
+
+KERNELS = {"gaussian": (cgef.GaussianKernel(6.0), O.GAUSSIAN, 6.0), "tophat": (cgef.TopHatKernel(), O.TOPHAT, 0.0)}
+
+
+# (26, 44): radix 13 along y takes the LDS ping-pong transform; (90, 120): several radix-2/3/4/5 stages
+@pytest.mark.parametrize("shape", [(24, 32), (21, 30), (26, 44), (90, 120)])
+@pytest.mark.parametrize("kernel", ["gaussian", "tophat"])
+@pytest.mark.parametrize("strategy", ["zerofill", "deformable"])
+def test_spectral_filter(shape, kernel, strategy):
+    og, g = spectral_case(*shape, masked=True)
+    ker, kind, alpha = KERNELS[kernel]
+    s_cg, s_o = STRATS[strategy]
+    ell = 6000.0
+    plan = cgef.plan_filter(g, ker, ell, mask_strategy=s_cg, method=cgef.Spectral())
+    assert plan.engine == "spectral"
+    f = smooth_random_field((2,) + g.field_shape, 42, 1)[0] + 0.5
+    # FP64 in: transform round-off only; FP32 in: the FP32 input and output roundings
+    for dtype, tol in ((torch.float64, 1e-12), (torch.float32, 1e-6)):
+        inp = T(f, dtype)
+        out = torch.empty_like(inp)
+        cgef.filter_apply_(out, inp, plan)
+        fin = N(inp)
+        ref = O.apply_spectral(og, kind, alpha, ell, s_o, [fin[0], fin[1]])
+        got = N(out)
+        for b in range(2):
+            assert np.max(np.abs(got[b] - ref[b])) <= tol * np.max(np.abs(ref[b]))
+
+
+def test_spectral_unmasked_constant_and_method_resolution():
+    og, g = spectral_case(24, 30, masked=False)
+    ker = cgef.GaussianKernel(6.0)
+    plan = cgef.plan_filter(g, ker, 5000.0, method=cgef.Spectral())
+    # Ĝ(0) = 1: a constant passes unchanged, with no renormalization on an unmasked grid
+    c = torch.full(g.field_shape, 2.5, dtype=torch.float64, device=DEV)
+    out = torch.empty_like(c)
+    cgef.filter_apply_(out, c, plan)
+    assert np.max(np.abs(N(out) - 2.5)) <= 1e-13
+    # AutoMethod takes the transform on a periodic uniform grid, RealSpace elsewhere (Filtering.jl:3012-3019)
+    assert cgef.plan_filter(g, ker, 5000.0, method=cgef.AutoMethod()).engine == "spectral"
+    assert cgef.plan_filter(g, ker, 5000.0).engine == "separable"
+    _, gb = cart_case(periodic=False)
+    assert cgef.plan_filter(gb, ker, 5000.0, method=cgef.AutoMethod()).engine != "spectral"
+    # a length with a prime factor above 13 has no device transform
+    _, g17 = spectral_case(24, 34, masked=False)
+    with pytest.raises(Exception, match="spectral engine"):
+        cgef.plan_filter(g17, ker, 5000.0, method=cgef.Spectral())
+
+
+@pytest.mark.parametrize("masked", [False, True])
+@pytest.mark.parametrize("strategy", ["zerofill", "deformable"])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_spectral_pi(masked, strategy, dtype):
+    og, g = spectral_case(40, 60, masked, seed=44)
+    s_cg, s_o = STRATS[strategy]
+    u, v = smooth_random_field(g.field_shape, 45, 2)
+    u, v = u + 0.3, v - 0.2
+    U, V = T(u, dtype), T(v, dtype)
+    ell = 7000.0
+    Pi = torch.empty_like(U)
+    ws = cgef.PiWorkspace(g, (), dtype)
+    _, E = cgef.compute_pi_(Pi, U, V, None, g, cgef.GaussianKernel(6.0), ell, workspace=ws, mask_strategy=s_cg,
+                            method=cgef.Spectral(), return_energy=True)
+    ref, ub, vb, _ = O.compute_pi(og, N(U), N(V), kind=O.GAUSSIAN, alpha=6.0, ell=ell, strategy=s_o,
+                                  engine=O.ENGINE_SPECTRAL)
+    tol = 1e-10 if dtype == torch.float64 else 1e-5
+    assert rel_l2(N(Pi), ref) <= tol
+    assert rel_l2(N(ws.u_filt), ub) <= (1e-12 if dtype == torch.float64 else 1e-7)
+    assert rel_l2(N(ws.v_filt), vb) <= (1e-12 if dtype == torch.float64 else 1e-7)
+    assert abs(E - O.energy(og, ub, vb)) <= 1e-10 * abs(O.energy(og, ub, vb))
+    # several scales share one forward transform of the inputs
+    scales = [4000.0, 9000.0]
+    res = cgef.coarse_grain(U, V, None, g, scales=scales, kernel=cgef.TopHatKernel(), mask_strategy=s_cg,
+                            method=cgef.Spectral(), spectrum=cgef.NoSpectrum())
+    for q, s in enumerate(scales):
+        rq, ubq, vbq, _ = O.compute_pi(og, N(U), N(V), kind=O.TOPHAT, alpha=0.0, ell=s, strategy=s_o,
+                                       engine=O.ENGINE_SPECTRAL)
+        assert rel_l2(N(res.Pi[q]), rq) <= tol
+        Eq = O.energy(og, ubq, vbq)
+        assert abs(res.cumulative_energy[q] - Eq) <= 1e-10 * abs(Eq)
+    # filtering_spectrum's cumulative energy from the filtered raw velocities
+    cum = cgef.cumulative_energy(U, V, None, g, cgef.TopHatKernel(), scales, mask_strategy=s_cg,
+                                 method=cgef.Spectral())
+    for q, s in enumerate(scales):
+        fu, fv = O.apply_spectral(og, O.TOPHAT, 0.0, s, s_o, [N(U), N(V)])
+        Eq = O.energy(og, fu, fv)
+        assert abs(cum[q] - Eq) <= (1e-12 if dtype == torch.float64 else 1e-6) * abs(Eq)

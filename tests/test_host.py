@@ -65,6 +65,33 @@ def test_plan_filter_rejects_bad_scale_before_touching_a_device():
             cgef.plan_filter(grid, cgef.GaussianKernel(), bad)
 
 
+def test_spectral_method_checks_the_grid_before_touching_a_device():
+    x = np.arange(8.0)
+    bounded = cgef.StructuredGrid(cgef.CartesianGeometry(), x, x)
+    periodic = cgef.StructuredGrid(cgef.CartesianGeometry(), x, x, periodic=(True, True))
+    lam = (np.arange(16) + 0.5) * (2 * np.pi / 16)
+    phi = -np.pi / 2 + (np.arange(8) + 0.5) * (np.pi / 8)
+    sphere = cgef.StructuredGrid(cgef.SphericalGeometry(6.371e6), lam, phi)
+    k = cgef.GaussianKernel()
+    # FFTWExt.jl:133-136: a transform filter needs both axes periodic
+    with pytest.raises(cgef.ArgumentError, match="doubly-periodic"):
+        cgef.plan_filter(bounded, k, 2.0, method=cgef.Spectral())
+    with pytest.raises(cgef.ArgumentError, match="no transform"):
+        cgef.plan_filter(sphere, k, 2.0, method=cgef.Spectral())
+    with pytest.raises(cgef.ArgumentError, match="real-space engine"):
+        cgef.plan_filter(periodic, k, 2.0, method=cgef.Spectral(), engine="ring")
+    with pytest.raises(cgef.ArgumentError, match="method must be"):
+        cgef.plan_filter(periodic, k, 2.0, method="spectral")
+    # AutoMethod is Spectral only where the transform is exact (Filtering.jl:3000-3019)
+    from cgef.filtering import _resolve_method
+    assert isinstance(_resolve_method(periodic, cgef.AutoMethod()), cgef.Spectral)
+    assert isinstance(_resolve_method(bounded, cgef.AutoMethod()), cgef.RealSpace)
+    assert isinstance(_resolve_method(sphere, cgef.AutoMethod()), cgef.RealSpace)
+    xn = np.cumsum(np.linspace(1.0, 2.0, 8))
+    stretched = cgef.StructuredGrid(cgef.CartesianGeometry(), xn, x, periodic=(True, True), period=(20.0, 8.0))
+    assert isinstance(_resolve_method(stretched, cgef.AutoMethod()), cgef.RealSpace)
+
+
 def test_grid_description():
     lam = (np.arange(16) + 0.5) * (2 * np.pi / 16)
     phi = -np.pi / 2 + (np.arange(8) + 0.5) * (np.pi / 8)

@@ -481,3 +481,55 @@ def test_prefix_tophat_nonuniform_axis_brute_force():
                 sup = d <= rad
                 exp[j, i] = np.sum((A * f * mask)[sup]) / np.sum(A[sup])
         assert np.max(np.abs(got - exp)) <= 1e-12 * np.max(np.abs(exp))
+
+
+# ------------------------------------------------ spectral filter (FFTWExt.jl:121-208) --------
+
+def _np_spectral(f, mask, Lx, Ly, transfer, deformable):
+    """numpy's real FFT (an independent transform) of the same definition: irfft2(rfft2(mask·f)·Ĝ)."""
+    Ny, Nx = f.shape
+    kx = 2 * np.pi * np.fft.rfftfreq(Nx, Lx / Nx)
+    ky = 2 * np.pi * np.fft.fftfreq(Ny, Ly / Ny)
+    G = transfer(np.sqrt(kx[None, :] ** 2 + ky[:, None] ** 2))
+    filt = lambda a: np.fft.irfft2(np.fft.rfft2(a) * G, s=(Ny, Nx))  # noqa: E731
+    m = np.ones_like(f) if mask is None else mask.astype(np.float64)
+    out = filt(m * f)
+    if deformable and mask is not None:
+        r = filt(m)
+        out = out * np.where(np.abs(r) >= 0.01, 1.0 / np.where(r == 0, 1, r), 0.0)
+    return out
+
+
+@pytest.mark.parametrize("kind", [O.GAUSSIAN, O.TOPHAT])
+@pytest.mark.parametrize("strategy", [O.ZEROFILL, O.DEFORMABLE])
+@pytest.mark.parametrize("shape", [(24, 32), (21, 30)])
+def test_spectral_filter_matches_numpy_fft(kind, strategy, shape):
+    import scipy.special as sp
+    Ny, Nx = shape
+    xs, ys = cart_axes(Nx, 1000.0), cart_axes(Ny, 1500.0)
+    rng = np.random.Generator(np.random.PCG64(40))
+    mask = (rng.random(shape) > 0.2).astype(np.uint8)
+    og = oracle_grid_cart(xs, ys, periodic=(True, True), mask=mask)
+    f = smooth_random_field(shape, 41)[0] + 0.3
+    ell, alpha = 5000.0, 6.0
+    if kind == O.GAUSSIAN:
+        tr = lambda k: np.exp(-k ** 2 * ell ** 2 / (4 * alpha))  # noqa: E731
+    else:
+        tr = lambda k: np.where(k == 0, 1.0, 2 * sp.j1(k * ell / 2) / np.where(k == 0, 1, k * ell / 2))  # noqa: E731
+    got = O.apply_spectral(og, kind, alpha, ell, strategy, [f])[0]
+    exp = _np_spectral(f, mask, Nx * 1000.0, Ny * 1500.0, tr, strategy == O.DEFORMABLE)
+    assert np.max(np.abs(got - exp)) <= 1e-12 * np.max(np.abs(exp))
+
+
+def test_spectral_filter_single_mode_and_constant():
+    # a Fourier mode is an eigenfunction: multiplied by Ĝ(|k|) exactly; Ĝ(0) = 1 keeps a constant
+    Nx, Ny, L = 32, 24, 32000.0
+    xs, ys = cart_axes(Nx, L / Nx), cart_axes(Ny, L / Nx)
+    og = oracle_grid_cart(xs, ys, periodic=(True, True))
+    X, Y = np.meshgrid(xs, ys)
+    k1, k2 = 2 * np.pi * 3 / L, 2 * np.pi * 2 / (Ny * L / Nx)
+    f = np.cos(k1 * X + k2 * Y)
+    ell, alpha = 4000.0, 6.0
+    got = O.apply_spectral(og, O.GAUSSIAN, alpha, ell, O.ZEROFILL, [f, np.full_like(f, 2.5)])
+    assert np.max(np.abs(got[0] - math.exp(-(k1 ** 2 + k2 ** 2) * ell ** 2 / (4 * alpha)) * f)) <= 1e-13
+    assert np.max(np.abs(got[1] - 2.5)) <= 1e-13
