@@ -262,6 +262,106 @@ def large_cartesian_leg(args):
     return out
 
 
+def spectral_leg(args):
+    """method = Spectral() on doubly periodic Cartesian grids (the reference's FFTW extension):
+    (a) Π on the 256² FP64 workload (Gaussian α = 6, ℓ = 8Δx, ρ₀ = 1025) with parity against the
+    oracle's transform filter; (b) one FP64 top-hat filter at half-width 64 cells on 256², transform
+    vs prefix sums (the reference's own comparison, Filtering.jl:3008-3011); (c) a 24-scale sweep
+    (ℓ = 2Δx … 64Δx) sharing one analysis, vs the separable engine; (d) Π on 4096² FP32, with the
+    column-transform kernel timed by HIP events."""
+    import numpy as np
+    import torch
+
+    import cgef
+    from cgef import synthetic as S
+
+    dev = "cuda:0"
+    out = {}
+    N, L = 256, 1.0e6
+    dx = L / N
+    u, v, xs = S.cartesian_turbulence(N=N, L=L, seed=0)
+    grid = cgef.StructuredGrid(cgef.CartesianGeometry(), xs, xs, periodic=(True, True))
+    U, V = torch.tensor(u, device=dev), torch.tensor(v, device=dev)
+    Pi = torch.empty_like(U)
+    ws = cgef.PiWorkspace(grid, (), torch.float64)
+    ker, ell = cgef.GaussianKernel(6.0), 8 * dx
+    plan = cgef.plan_filter(grid, ker, ell, method=cgef.Spectral())
+    steps = 20
+
+    def step():
+        cgef.compute_pi_(Pi, U, V, None, grid, ker, ell, workspace=ws, filter_plan=plan, rho0=1025.0)
+
+    el, _ = timed_steps(step, steps, 3)
+    r = {"workload": "cartesian 256x256 doubly periodic FP64, Gaussian alpha=6, l=8dx, rho0=1025, Spectral()",
+         "engine": plan.engine, "ms_per_step": el * 1e3 / steps, "value": grid.npoints * steps / el,
+         "unit": "point-scales/s"}
+    if not args.no_cpu:
+        from oracle import oracle as O
+        og = O.Grid([xs, xs], periodic=[True, True], period=[L, L], uniform=[True, True])
+        ref = 1025.0 * O.compute_pi(og, u, v, kind=O.GAUSSIAN, alpha=6.0, ell=ell, engine=O.ENGINE_SPECTRAL)[0]
+        got = Pi.cpu().numpy()
+        r["rel_l2_vs_fp64_oracle"] = float(np.sqrt(np.sum((got - ref) ** 2) / np.sum(ref ** 2)))
+    out["pi_256"] = r
+    log(f"spectral 256^2 Π: {r['ms_per_step']:.3f} ms/step")
+
+    th = cgef.TopHatKernel()
+    f = U.clone()
+    g = torch.empty_like(f)
+    r = {"workload": "one FP64 field, 256x256 doubly periodic, top-hat half-width 64 cells"}
+    for eng, meth in (("spectral", cgef.Spectral()), ("prefixsum", None)):
+        tp = cgef.plan_filter(grid, th, 128 * dx, method=meth)
+        el, _ = timed_steps(lambda: cgef.filter_apply_(g, f, tp), 50, 5)
+        r[f"{tp.engine}_ms"] = el * 1e3 / 50
+    out["tophat_halfwidth64_256"] = r
+    log(f"top-hat half-width 64: spectral {r['spectral_ms']:.3f} ms, prefix {r['prefixsum_ms']:.3f} ms")
+
+    scales = list(np.geomspace(2 * dx, 64 * dx, 24))
+    r = {"workload": "256x256 FP64 Gaussian alpha=6, 24 scales 2dx..64dx, coarse_grain"}
+    for name, meth in (("spectral", cgef.Spectral()), ("separable", None)):
+        plans = [cgef.plan_filter(grid, ker, s, method=meth) for s in scales]
+        res = cgef.pipeline._allocate_result(grid, len(scales), torch.float64, dev)
+
+        def sweep():
+            cgef.coarse_grain_(res, U, V, None, grid, scales=scales, kernel=ker, filter_plans=plans,
+                               spectrum=cgef.NoSpectrum())
+
+        el, _ = timed_steps(sweep, 5, 1)
+        r[f"{name}_ms_per_sweep"] = el * 1e3 / 5
+        r[f"{name}_value"] = grid.npoints * len(scales) * 5 / el
+    r["unit"] = "point-scales/s"
+    out["sweep_24_scales_256"] = r
+    log(f"24-scale sweep: spectral {r['spectral_ms_per_sweep']:.3f} ms, separable {r['separable_ms_per_sweep']:.3f} ms")
+    del U, V, Pi, ws, plan, f, g
+
+    N, L = 4096, 1.6e7
+    dx = L / N
+    u, v, xs = S.cartesian_turbulence(N=N, L=L, seed=0, kmax=1300)
+    grid = cgef.StructuredGrid(cgef.CartesianGeometry(), xs, xs, periodic=(True, True))
+    U = torch.tensor(u, dtype=torch.float32, device=dev)
+    V = torch.tensor(v, dtype=torch.float32, device=dev)
+    del u, v
+    Pi = torch.empty_like(U)
+    ws = cgef.PiWorkspace(grid, (), torch.float32)
+    plan = cgef.plan_filter(grid, ker, 8 * dx, method=cgef.Spectral())
+    steps = 10
+
+    def step4k():
+        cgef.compute_pi_(Pi, U, V, None, grid, ker, 8 * dx, workspace=ws, filter_plan=plan)
+
+    el, kms = timed_steps(step4k, steps, 2, plan)
+    # the column kernel reads and writes the five planes' half spectra once: 2 · 5 · (N/2+1) · N · 16 B
+    alg = 2 * 5 * (N // 2 + 1) * N * 16
+    out["pi_4096_fp32"] = {"workload": "cartesian 4096x4096 doubly periodic FP32, Gaussian alpha=6, l=8dx, Spectral()",
+                           "engine": plan.engine, "ms_per_step": el * 1e3 / steps,
+                           "value": grid.npoints * steps / el, "unit": "point-scales/s", "kernel": "k_spec_cols",
+                           "kernel_ms": kms, "kernel_algorithmic_bytes": alg,
+                           "kernel_hbm_gbs": alg / (kms * 1e-3) / 1e9}
+    log(f"spectral 4096^2 Π: {el * 1e3 / steps:.3f} ms/step, column kernel {kms:.3f} ms")
+    del U, V, Pi, ws, plan
+    torch.cuda.empty_cache()
+    return out
+
+
 def volume_leg(args):
     """True 3-D: 256×256×64, Δx = Δy = 1, Δz_k = Δz₁·1.03^(k−1) with Δz = 4 at mid-depth (so the
     top-hat ball of radius ℓ/2 = 8 spans ±2 levels there), x/y periodic, z bounded, FP32, top-hat
@@ -426,6 +526,7 @@ def main():
         extra["cartesian_256"] = cartesian_leg(args)
         extra["cartesian_4096_separable"] = large_cartesian_leg(args)
         extra["volume_256x256x64"] = volume_leg(args)
+        extra["spectral"] = spectral_leg(args)
 
     cpu = None
     if not args.no_cpu:

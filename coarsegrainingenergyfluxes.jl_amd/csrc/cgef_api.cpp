@@ -867,12 +867,13 @@ cgef_status cgef_coarse_grain(cgef_plan *const *plans, int64_t nscales, cgef_wor
             ring_reserve(W, p0, 5 * nbatch);
             ring_prep(G, dtype, u, v, w, 5, 0, W.rxy, nbatch, s);
             ring_forward(p0.ring, W.rxy, W.rxf, 5 * nbatch, s);
+            ring_spectral_analyze_cols(p0.ring, W.rxf, W.rout, 5 * nbatch, s);
             for (int64_t q = 0; q < nscales; q++) {
                 Plan &p = *reinterpret_cast<Plan *>(plans[q]);
                 p.prof_begin(s);
-                ring_spectral_cols(p.ring, W.rxf, W.rout, 5 * nbatch, s);
+                ring_spectral_synth_cols(p.ring, W.rout, W.rxf, 5 * nbatch, s);
                 p.prof_end(s);
-                ring_inverse(p.ring, W.rout, W.rxy, 5 * nbatch, s);
+                ring_inverse(p.ring, W.rxf, W.rxy, 5 * nbatch, s);
                 ring_spectral_epilogue(G, dtype, W.rxy, p.ring.invrenorm_d, W.ubar, W.tau, nullptr, nullptr, nullptr,
                                        nbatch, s);
                 launch_strain_pi(G, dtype, W.ubar, W.tau, (char *)Pi + (size_t)q * P * es, rho0, nbatch, W.partials,
@@ -960,6 +961,29 @@ cgef_status cgef_cumulative_energy(cgef_plan *const *plans, int64_t nscales, cge
         void *uf = fb, *vf = fb + (size_t)P * nbatch * es, *wf = w ? fb + 2 * (size_t)P * nbatch * es : nullptr;
         const GeomArgs G = geom_args(g);
         std::vector<double> E(nbatch);
+        bool all_spec = true;
+        for (int64_t q = 0; q < nscales; q++) all_spec = all_spec && reinterpret_cast<Plan *>(plans[q])->engine == CGEF_ENGINE_SPECTRAL;
+        if (all_spec) {
+            // analyze the masked velocities once, synthesize per scale (Diagnostics.jl:270-301)
+            Plan &p0 = *reinterpret_cast<Plan *>(plans[0]);
+            const int64_t nsl = (w ? 3 : 2) * nbatch;
+            ring_reserve(W, p0, nsl);
+            ring_prep_fields(G, dtype, u, nbatch, 0, W.rxy, s);
+            ring_prep_fields(G, dtype, v, nbatch, 0, W.rxy + nbatch * P, s);
+            if (w) ring_prep_fields(G, dtype, w, nbatch, 0, W.rxy + 2 * nbatch * P, s);
+            ring_forward(p0.ring, W.rxy, W.rxf, nsl, s);
+            ring_spectral_analyze_cols(p0.ring, W.rxf, W.rout, nsl, s);
+            for (int64_t q = 0; q < nscales; q++) {
+                Plan &p = *reinterpret_cast<Plan *>(plans[q]);
+                ring_spectral_synth_cols(p.ring, W.rout, W.rxf, nsl, s);
+                ring_inverse(p.ring, W.rxf, W.rxy, nsl, s);
+                ring_spectral_normalize(G, dtype, W.rxy, p.ring.invrenorm_d, nsl, fb, s);
+                launch_energy_fields(G, dtype, uf, vf, wf, nbatch, W.partials, W.npartials, s);
+                fetch_energy(W, nbatch, strain_blocks(G), g.active_area, E.data(), s);
+                for (int64_t b = 0; b < nbatch; b++) E_out[q + nscales * b] = E[b];
+            }
+            return;
+        }
         for (int64_t q = 0; q < nscales; q++) {
             Plan *p = reinterpret_cast<Plan *>(plans[q]);
             auto filt = [&](const void *in, void *out) {

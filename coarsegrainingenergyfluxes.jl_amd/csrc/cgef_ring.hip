@@ -621,7 +621,10 @@ __device__ __forceinline__ double spectral_transfer(int kind, double alpha, doub
     return x == 0.0 ? 1.0 : 2.0 * j1(x) / x;
 }
 
-template <bool STAGED>
+// MODE 0: transform along y, × Ĝ/Ny, transform back (one scale); MODE 1: the forward transform only
+// (the analysis a sweep over scales shares, FFTWExt.jl:100-110); MODE 2: × Ĝ/Ny and back (one
+// scale's synthesis from the shared analysis, FFTWExt.jl:112-120).
+template <bool STAGED, int MODE>
 __global__ __launch_bounds__(256) void k_spec_cols(RingFft F, const double2 *__restrict__ tw,
                                                    const double2 *__restrict__ tws, int kind, double alpha, double ell,
                                                    double dkx, double dky, const double2 *__restrict__ XF,
@@ -634,19 +637,24 @@ __global__ __launch_bounds__(256) void k_spec_cols(RingFft F, const double2 *__r
     const double2 *x = XF + item * n;
     for (int e = threadIdx.x; e < n; e += 256) sm[e] = x[e];
     double2 *z = sm;
-    if (STAGED) fft_staged<1>(sm, F, tw, tws, false);
-    else z = fft_lds(sm, sm + n, F, tw, false, 1);
-    // angular wavenumbers 2π·m/Lx (rfftfreq) and 2π·n/Ly with n folded to −Ny/2..(Ny−1)/2 (fftfreq)
-    const double kx = dkx * (double)m, sc = 1.0 / (double)n;
-    for (int e = threadIdx.x; e < n; e += 256) {
-        const int nn = e <= (n - 1) / 2 ? e : e - n;
-        const double ky = dky * (double)nn;
-        const double g = spectral_transfer(kind, alpha, sqrt(kx * kx + ky * ky), ell) * sc;
-        z[e] = make_double2(z[e].x * g, z[e].y * g);
+    if (MODE != 2) {
+        if (STAGED) fft_staged<1>(sm, F, tw, tws, false);
+        else z = fft_lds(sm, sm + n, F, tw, false, 1);
     }
     double2 *r = z;
-    if (STAGED) fft_staged<1>(z, F, tw, tws, true);
-    else r = fft_lds(z, z == sm ? sm + n : sm, F, tw, true, 1);
+    if (MODE != 1) {
+        if (MODE == 2) __syncthreads();
+        // angular wavenumbers 2π·m/Lx (rfftfreq) and 2π·n/Ly with n folded to −Ny/2..(Ny−1)/2 (fftfreq)
+        const double kx = dkx * (double)m, sc = 1.0 / (double)n;
+        for (int e = threadIdx.x; e < n; e += 256) {
+            const int nn = e <= (n - 1) / 2 ? e : e - n;
+            const double ky = dky * (double)nn;
+            const double g = spectral_transfer(kind, alpha, sqrt(kx * kx + ky * ky), ell) * sc;
+            z[e] = make_double2(z[e].x * g, z[e].y * g);
+        }
+        if (STAGED) fft_staged<1>(z, F, tw, tws, true);
+        else r = fft_lds(z, z == sm ? sm + n : sm, F, tw, true, 1);
+    }
     double2 *o = OUT + item * n;
     for (int e = threadIdx.x; e < n; e += 256) o[e] = r[e];
 }
@@ -782,19 +790,30 @@ void ring_normalize(const GeomArgs &G, int dtype, const double *yout, const doub
     hipLaunchKernelGGL(k_ring_normalize, grid, dim3(256), 0, s, G, dtype, yout, D_row, nsl, withmask, strategy, out);
 }
 
-void ring_spectral_cols(const RingPlan &R, const double2 *XF, double2 *OUT, int64_t ncol, hipStream_t s) {
+template <int MODE>
+static void spec_cols(const RingPlan &R, const double2 *XF, double2 *OUT, int64_t ncol, hipStream_t s) {
     const int64_t items = R.mh * ncol;
     const dim3 grid((unsigned)(8 * ((items + 7) / 8)));
     if (R.fy.nseq > 0) {
         const size_t sh = staged_shmem(R.fy, 1);
-        hipLaunchKernelGGL(k_spec_cols<true>, grid, dim3(256), sh, s, R.fy, R.twy_d, R.twsy_d, R.kind, R.alpha, R.ell,
-                           R.dkx, R.dky, XF, OUT, ncol, R.mh);
+        hipLaunchKernelGGL((k_spec_cols<true, MODE>), grid, dim3(256), sh, s, R.fy, R.twy_d, R.twsy_d, R.kind, R.alpha,
+                           R.ell, R.dkx, R.dky, XF, OUT, ncol, R.mh);
     } else {
         const size_t sh = fft_shmem(R.fy);
-        set_shmem((const void *)k_spec_cols<false>, sh);
-        hipLaunchKernelGGL(k_spec_cols<false>, grid, dim3(256), sh, s, R.fy, R.twy_d, R.twsy_d, R.kind, R.alpha, R.ell,
-                           R.dkx, R.dky, XF, OUT, ncol, R.mh);
+        set_shmem((const void *)k_spec_cols<false, MODE>, sh);
+        hipLaunchKernelGGL((k_spec_cols<false, MODE>), grid, dim3(256), sh, s, R.fy, R.twy_d, R.twsy_d, R.kind, R.alpha,
+                           R.ell, R.dkx, R.dky, XF, OUT, ncol, R.mh);
     }
+}
+
+void ring_spectral_cols(const RingPlan &R, const double2 *XF, double2 *OUT, int64_t ncol, hipStream_t s) {
+    spec_cols<0>(R, XF, OUT, ncol, s);
+}
+void ring_spectral_analyze_cols(const RingPlan &R, const double2 *XF, double2 *OUT, int64_t ncol, hipStream_t s) {
+    spec_cols<1>(R, XF, OUT, ncol, s);
+}
+void ring_spectral_synth_cols(const RingPlan &R, const double2 *XF, double2 *OUT, int64_t ncol, hipStream_t s) {
+    spec_cols<2>(R, XF, OUT, ncol, s);
 }
 
 void ring_spectral_invert(const GeomArgs &G, const double *renorm, double *invrenorm, hipStream_t s) {

@@ -21,6 +21,9 @@ bool spectral_supported(const Grid &g, int kind);
 void build_spectral(Plan &p, const GeomArgs &G, hipStream_t s);
 // spectral engine launches: per (mode along x, plane column) transform along y, × Ĝ, transform back
 void ring_spectral_cols(const RingPlan &R, const double2 *XF, double2 *OUT, int64_t ncol, hipStream_t s);
+// the same split for sweeps over scales: the forward y transform once, then × Ĝ and back per scale
+void ring_spectral_analyze_cols(const RingPlan &R, const double2 *XF, double2 *OUT, int64_t ncol, hipStream_t s);
+void ring_spectral_synth_cols(const RingPlan &R, const double2 *XF, double2 *OUT, int64_t ncol, hipStream_t s);
 void ring_spectral_invert(const GeomArgs &G, const double *renorm, double *invrenorm, hipStream_t s);
 void ring_spectral_normalize(const GeomArgs &G, int dtype, const double *yout, const double *invrenorm, int64_t nsl,
                              void *out, hipStream_t s);
