@@ -45,9 +45,14 @@ struct RingFft {
     int n = 0;
     int nstage = 0;
     int radix[RING_MAX_STAGES] = {};
-    int twoff[RING_MAX_STAGES] = {};  // offset of stage st in the per-stage twiddle table
+    // register-staged factorization (radices up to 16, composite ones evaluated in registers):
+    // snst stages of radix srad[st]; the staged FFT's LDS index shift lpsh (4: padded, 31: not)
+    int snst = 0;
+    int srad[RING_MAX_STAGES] = {};
+    int lpsh = 31;
+    int twoff[RING_MAX_STAGES] = {};  // offset of staged stage st in the per-stage twiddle table
     int twsize = 0;                    // entries of the per-stage twiddle table
-    int nseq = 0;                      // complex sequences per block of the register-staged row FFT
+    int nseq = 0;                      // 1: the register-staged FFT serves this length (one sequence per block)
                                        // (0: not supported for this length, the LDS ping-pong FFT serves)
 };
 

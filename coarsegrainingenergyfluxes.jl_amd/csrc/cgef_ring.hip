@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
+#include <utility>
 
 #include "cgef_pi_point.hpp"
 #include "cgef_ring.hpp"
@@ -34,14 +35,61 @@ __device__ __forceinline__ double2 twid(const double2 *tw, int k, bool inv) {
     return inv ? cconj(t) : t;
 }
 
-// In-LDS DFT of length R on v (R ≤ RING_MAX_RADIX), roots from the N-point table.
+// exp(−2πi·e/R) at compile time: the angle is reduced to |x| ≤ π/4 about the nearest quarter turn
+// and cos, sin of x are summed from their Taylor series (terms below 1e-20 at |x| ≤ π/4).
+struct Root {
+    double c, s;
+};
+constexpr double kPi = 3.14159265358979323846;
+constexpr Root taylor_cs(double x) {
+    double c = 1.0, s = x, tc = 1.0, ts = x;
+    const double x2 = x * x;
+    for (int k = 1; k < 12; k++) {
+        tc *= -x2 / ((2.0 * k - 1.0) * (2.0 * k));
+        ts *= -x2 / ((2.0 * k) * (2.0 * k + 1.0));
+        c += tc;
+        s += ts;
+    }
+    return Root{c, s};
+}
+constexpr Root root_of_unity(int e, int R) {
+    e %= R;
+    const int q = (8 * e + R) / (2 * R);  // nearest quarter turn: round(4e/R)
+    const double x = 2.0 * kPi * (double)(4 * e - q * R) / (4.0 * R);
+    const Root r = taylor_cs(x);
+    // angle = q·π/2 + x; exp(−i·angle) = (cos, −sin)
+    double c = r.c, sn = r.s;
+    switch (q & 3) {
+        case 1: c = -r.s; sn = r.c; break;
+        case 2: c = -r.c; sn = -r.s; break;
+        case 3: c = r.s; sn = -r.c; break;
+        default: break;
+    }
+    return Root{c, -sn};
+}
+
+// v · exp(∓2πi·e/R) with the root folded to constants (∓: forward / inverse)
+template <int E, int R>
+__device__ __forceinline__ double2 mul_root(double2 v, bool inv) {
+    constexpr Root w = root_of_unity(E, R);
+    const double ws = inv ? -w.s : w.s;
+    return make_double2(v.x * w.c - v.y * ws, v.x * ws + v.y * w.c);
+}
+
+template <int A, int B>
+__device__ __forceinline__ void dft_comp(double2 *v, bool inv);
+template <int R>
+__device__ __forceinline__ void dft_direct(double2 *v, bool inv);
+
+// In-register DFT of length R on v, roots from the N-point table (R divides N): closed forms for
+// 2, 3, 4, 5, two-level products of those for the composite radices, the direct sum otherwise.
 template <int R>
 __device__ __forceinline__ void dft_small(double2 *v, const double2 *tw, int n, bool inv) {
-    if (R == 2) {
+    if constexpr (R == 2) {
         const double2 a = v[0], b = v[1];
         v[0] = cadd(a, b);
         v[1] = csub(a, b);
-    } else if (R == 4) {
+    } else if constexpr (R == 4) {
         const double2 a = cadd(v[0], v[2]), b = csub(v[0], v[2]);
         const double2 c = cadd(v[1], v[3]), d = csub(v[1], v[3]);
         // −i·d forward, +i·d inverse
@@ -50,7 +98,7 @@ __device__ __forceinline__ void dft_small(double2 *v, const double2 *tw, int n, 
         v[2] = csub(a, c);
         v[1] = cadd(b, dj);
         v[3] = csub(b, dj);
-    } else if (R == 3) {
+    } else if constexpr (R == 3) {
         // y1,2 = v0 − (v1+v2)/2 ∓ i·(√3/2)(v1 − v2) (forward; signs swap for the inverse)
         const double s3 = 0.86602540378443865;  // sin(2π/3)
         const double2 a = cadd(v[1], v[2]), b = csub(v[1], v[2]);
@@ -60,7 +108,7 @@ __device__ __forceinline__ void dft_small(double2 *v, const double2 *tw, int n, 
         v[0] = cadd(v[0], a);
         v[1] = cadd(m, dj);
         v[2] = csub(m, dj);
-    } else if (R == 5) {
+    } else if constexpr (R == 5) {
         // pairs a_k = v_k + v_{5−k}, b_k = v_k − v_{5−k}; cosines mix the a's, sines the b's
         const double c1 = 0.30901699437494742, c2 = -0.80901699437494742;  // cos(2π/5), cos(4π/5)
         const double s1 = 0.95105651629515357, s2 = 0.58778525229247313;   // sin(2π/5), sin(4π/5)
@@ -77,18 +125,78 @@ __device__ __forceinline__ void dft_small(double2 *v, const double2 *tw, int n, 
         v[4] = csub(m1, j1);
         v[2] = cadd(m2, j2);
         v[3] = csub(m2, j2);
+    } else if constexpr (R == 6) {
+        dft_comp<3, 2>(v, inv);
+    } else if constexpr (R == 8) {
+        dft_comp<4, 2>(v, inv);
+    } else if constexpr (R == 9) {
+        dft_comp<3, 3>(v, inv);
+    } else if constexpr (R == 10) {
+        dft_comp<5, 2>(v, inv);
+    } else if constexpr (R == 12) {
+        dft_comp<4, 3>(v, inv);
+    } else if constexpr (R == 15) {
+        dft_comp<5, 3>(v, inv);
+    } else if constexpr (R == 16) {
+        dft_comp<4, 4>(v, inv);
     } else {
-        double2 o[R];
-        const int step = n / R;
+        dft_direct<R>(v, inv);
+    }
+}
+
+// The direct sum for the prime radices 7, 11, 13 (roots as compile-time constants).
+template <int R, int A, int... B>
+__device__ __forceinline__ double2 direct_sum(const double2 *v, bool inv, std::integer_sequence<int, B...>) {
+    double2 s = v[0];
+    ((s = cadd(s, mul_root<(A * (B + 1)) % R, R>(v[B + 1], inv))), ...);
+    return s;
+}
+
+template <int R, int... A>
+__device__ __forceinline__ void dft_direct_rows(double2 *v, bool inv, std::integer_sequence<int, A...>) {
+    const double2 o[R] = {direct_sum<R, A>(v, inv, std::make_integer_sequence<int, R - 1>{})...};
 #pragma unroll
-        for (int a = 0; a < R; a++) {
-            double2 s = v[0];
+    for (int a = 0; a < R; a++) v[a] = o[a];
+}
+
+template <int R>
+__device__ __forceinline__ void dft_direct(double2 *v, bool inv) {
+    dft_direct_rows<R>(v, inv, std::make_integer_sequence<int, R>{});
+}
+
+// R = A·B: with q = q2 + B·q1 and k = k1 + A·k2,
+// X[k] = Σ_q2 ω_B^(q2·k2) · ω_R^(q2·k1) · DFT_A(v[q2 + B·q1])[k1].
+template <int A, int R, int Q2, int... K>
+__device__ __forceinline__ void twiddle_col(double2 *y, const double2 *t, bool inv, std::integer_sequence<int, K...>) {
+    y[Q2 * A] = t[0];
+    ((y[Q2 * A + K + 1] = mul_root<Q2 * (K + 1), R>(t[K + 1], inv)), ...);
+}
+
+template <int A, int B, int Q2 = 0>
+__device__ __forceinline__ void comp_cols(double2 *v, double2 *y, bool inv) {
+    if constexpr (Q2 < B) {
+        double2 t[A];
 #pragma unroll
-            for (int b = 1; b < R; b++) s = cadd(s, cmul(v[b], twid(tw, ((a * b) % R) * step, inv)));
-            o[a] = s;
-        }
+        for (int q1 = 0; q1 < A; q1++) t[q1] = v[Q2 + B * q1];
+        dft_small<A>(t, nullptr, 0, inv);
+        twiddle_col<A, A * B, Q2>(y, t, inv, std::make_integer_sequence<int, A - 1>{});
+        comp_cols<A, B, Q2 + 1>(v, y, inv);
+    }
+}
+
+template <int A, int B>
+__device__ __forceinline__ void dft_comp(double2 *v, bool inv) {
+    constexpr int R = A * B;
+    double2 y[R];  // y[q2·A + k1]
+    comp_cols<A, B>(v, y, inv);
 #pragma unroll
-        for (int a = 0; a < R; a++) v[a] = o[a];
+    for (int k1 = 0; k1 < A; k1++) {
+        double2 t[B];
+#pragma unroll
+        for (int q2 = 0; q2 < B; q2++) t[q2] = y[q2 * A + k1];
+        dft_small<B>(t, nullptr, 0, inv);
+#pragma unroll
+        for (int k2 = 0; k2 < B; k2++) v[k1 + A * k2] = t[k2];
     }
 }
 
@@ -179,6 +287,11 @@ __global__ __launch_bounds__(256) void k_fft_rows(RingFft F, const double2 *__re
     }
 }
 
+// Offset of mode m of row `row` of column `col` in a spectra set [m][col][row].
+__device__ __forceinline__ int64_t spec_at(int64_t m, int64_t col, int64_t row, int64_t nrows, int64_t ncol) {
+    return (m * ncol + col) * nrows + row;
+}
+
 // Inverse: OUT[m][col][row] (Hermitian halves) → yout[col][row][i], normalized by 1/N.
 __global__ __launch_bounds__(256) void k_ifft_rows(RingFft F, const double2 *__restrict__ tw, const double2 *__restrict__ OUT,
                                                    double *__restrict__ yout, int64_t nrows, int64_t ncol) {
@@ -194,9 +307,8 @@ __global__ __launch_bounds__(256) void k_ifft_rows(RingFft F, const double2 *__r
     for (int m = threadIdx.x; m < n; m += blockDim.x) {
         const bool lo = m < mh;
         const int mm = lo ? m : n - m;
-        const double2 *o = OUT + ((int64_t)mm * ncol + col) * nrows;
-        double2 X = o[r0];
-        double2 Y = has1 ? o[r1] : make_double2(0.0, 0.0);
+        double2 X = OUT[spec_at(mm, col, r0, nrows, ncol)];
+        double2 Y = has1 ? OUT[spec_at(mm, col, r1, nrows, ncol)] : make_double2(0.0, 0.0);
         if (!lo) {
             X = cconj(X);
             Y = cconj(Y);
@@ -212,77 +324,78 @@ __global__ __launch_bounds__(256) void k_ifft_rows(RingFft F, const double2 *__r
     }
 }
 
-// ---- register-staged row FFT: one LDS buffer per sequence, each stage reads its butterflies into
-// registers, waits, then writes them back in Stockham order (half the LDS of a ping-pong, so twice
-// the transforms in flight per CU), with per-stage twiddles read contiguously.
+// ---- register-staged FFT: one sequence per 256-thread block in LDS; each stage reads its butterflies
+// into registers, waits, then writes them back in Stockham order. The stages use radices up to 16
+// (composite radices evaluated in registers), so a 1440-point transform takes 3 passes through LDS
+// instead of 6. The first stage's writes step by R elements across lanes: an odd first radix keeps
+// them on distinct banks; an even one (power-of-two lengths) gets a padded layout, element e at
+// e + e/16 (F.lpsh = 4; 31 leaves the layout unpadded).
 
-// Butterflies per thread and stage held in registers: at most 12 complex values for the in-place
-// radices 2, 3, 4, and 16 with the output copy of the generic radices 5, 7 (11 and 13 are not staged).
-__host__ __device__ constexpr int staged_mb_max(int R) {
-    return R == 2 ? 6 : R == 3 ? 4 : R == 4 ? 3 : R == 5 ? 2 : R == 7 ? 1 : 0;
-}
+__device__ __forceinline__ int lp(int e, int sh) { return e + (e >> sh); }
 
-template <int R, int MB, int NSEQ>
-__device__ __forceinline__ void stage_staged(double2 *sm, int n, int Ns, const double2 *__restrict__ tws,
+template <int R, int MB>
+__device__ __forceinline__ void stage_staged(double2 *sm, int n, int Ns, int sh, const double2 *__restrict__ tws,
                                              const double2 *__restrict__ tw, bool inv) {
+    // n and the layout shift pass through an empty asm so that this radix's addresses are formed here:
+    // hoisted out of the stage loop, every radix of the switch would keep its own set of registers.
+    n = __builtin_amdgcn_readfirstlane(n);
+    sh = __builtin_amdgcn_readfirstlane(sh);
+    asm volatile("" : "+s"(n), "+s"(sh));
     const int nb = n / R;
-    const int tot = nb * NSEQ;
     double2 v[MB][R];
     int dst[MB];
 #pragma unroll
     for (int q = 0; q < MB; q++) {
-        const int p = threadIdx.x + q * 256;
-        if (p < tot) {
-            const int sq = NSEQ == 1 ? 0 : (p >= nb ? 1 : 0);
-            const int j = p - sq * nb;
+        const int j = (int)threadIdx.x + q * 256;
+        if (j < nb) {
             const int k = j % Ns;
-            const double2 *x = sm + sq * n;
             const double2 *t = tws + k * (R - 1);
-            v[q][0] = x[j];
+            v[q][0] = sm[lp(j, sh)];
             if (Ns == 1) {  // first stage: every twiddle is 1
 #pragma unroll
-                for (int r = 1; r < R; r++) v[q][r] = x[j + r * nb];
+                for (int r = 1; r < R; r++) v[q][r] = sm[lp(j + r * nb, sh)];
             } else {
 #pragma unroll
                 for (int r = 1; r < R; r++) {
                     const double2 w = t[r - 1];
-                    v[q][r] = cmul(x[j + r * nb], inv ? cconj(w) : w);
+                    v[q][r] = cmul(sm[lp(j + r * nb, sh)], inv ? cconj(w) : w);
                 }
             }
             dft_small<R>(v[q], tw, n, inv);
-            dst[q] = sq * n + (j - k) * R + k;
+            dst[q] = (j - k) * R + k;
         }
     }
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < MB; q++) {
-        const int p = threadIdx.x + q * 256;
-        if (p < tot) {
+        const int j = (int)threadIdx.x + q * 256;
+        if (j < nb) {
 #pragma unroll
-            for (int r = 0; r < R; r++) sm[dst[q] + r * Ns] = v[q][r];
+            for (int r = 0; r < R; r++) sm[lp(dst[q] + r * Ns, sh)] = v[q][r];
         }
     }
     __syncthreads();
 }
 
-// NSEQ complex sequences of length F.n in sm (transformed in place); the host has checked that every
-// stage fits staged_mb_max.
-template <int NSEQ>
+// One sequence of length F.n in sm (padded layout, transformed in place) with the staged factorization
+// F.srad (every stage's butterflies fit the per-thread budget the host checked).
 __device__ void fft_staged(double2 *sm, const RingFft &F, const double2 *tw, const double2 *tws, bool inv) {
     __syncthreads();
     int Ns = 1;
-    for (int st = 0; st < F.nstage; st++) {
-        const int R = F.radix[st];
-        const int mb = (F.n / R * NSEQ + 255) / 256;
+    for (int st = 0; st < F.snst; st++) {
+        const int R = F.srad[st];
+        const int mb = (F.n / R + 255) / 256;
         const double2 *t = tws + F.twoff[st];
         switch (R * 16 + mb) {
 #define CGEF_ST(RR, MM) \
-    case RR * 16 + MM: stage_staged<RR, MM, NSEQ>(sm, F.n, Ns, t, tw, inv); break;
+    case RR * 16 + MM: stage_staged<RR, MM>(sm, F.n, Ns, F.lpsh, t, tw, inv); break;
             CGEF_ST(2, 1) CGEF_ST(2, 2) CGEF_ST(2, 3) CGEF_ST(2, 4) CGEF_ST(2, 5) CGEF_ST(2, 6)
             CGEF_ST(3, 1) CGEF_ST(3, 2) CGEF_ST(3, 3) CGEF_ST(3, 4)
             CGEF_ST(4, 1) CGEF_ST(4, 2) CGEF_ST(4, 3)
             CGEF_ST(5, 1) CGEF_ST(5, 2)
-            CGEF_ST(7, 1)
+            CGEF_ST(6, 1) CGEF_ST(6, 2)
+            CGEF_ST(8, 1) CGEF_ST(8, 2)
+            CGEF_ST(9, 1) CGEF_ST(10, 1) CGEF_ST(12, 1) CGEF_ST(15, 1) CGEF_ST(16, 1)
 #undef CGEF_ST
             default: break;
         }
@@ -290,89 +403,86 @@ __device__ void fft_staged(double2 *sm, const RingFft &F, const double2 *tw, con
     }
 }
 
-// Forward, 2·NSEQ real rows per block: rows r0.. of column `col` → XF[m][col][row], m ≤ N/2; the
-// spectra of the block's rows land contiguously for each m.
-template <int NSEQ>
+// Forward, two real rows per block: rows r0, r0+1 of column `col` → XF[m][col][row], m ≤ N/2.
+// LPT·256 ≥ N: every load of the rows is issued before the first is waited for.
+template <int LPT>
 __global__ __launch_bounds__(256) void k_fft_rows_staged(RingFft F, const double2 *__restrict__ tw,
                                                          const double2 *__restrict__ tws, const double *__restrict__ xin,
                                                          double2 *__restrict__ XF, int64_t nrows, int64_t ncol) {
     extern __shared__ double2 sm[];
-    constexpr int RPB = 2 * NSEQ;
     const int n = F.n;
-    const int64_t ngrp = (nrows + RPB - 1) / RPB;
+    const int64_t ngrp = (nrows + 1) / 2;
     int64_t item;
     if (!xcd_item(ngrp * ncol, item)) return;
-    const int64_t col = item / ngrp, r0 = (item % ngrp) * RPB;
+    const int64_t col = item / ngrp, r0 = (item % ngrp) * 2;
     const double *x = xin + (col * nrows + r0) * n;
-    for (int e = threadIdx.x; e < n * NSEQ; e += 256) {
-        const int sq = NSEQ == 1 ? 0 : (e >= n ? 1 : 0);
-        const int i = e - sq * n;
-        const bool va = r0 + 2 * sq < nrows, vb = r0 + 2 * sq + 1 < nrows;
-        sm[e] = make_double2(va ? x[(2 * sq) * n + i] : 0.0, vb ? x[(2 * sq + 1) * n + i] : 0.0);
+    const bool vb = r0 + 1 < nrows;
+    double a_[LPT], b_[LPT];
+#pragma unroll
+    for (int q = 0; q < LPT; q++) {
+        const int i = (int)threadIdx.x + q * 256;
+        a_[q] = i < n ? x[i] : 0.0;
+        b_[q] = i < n && vb ? x[n + i] : 0.0;
     }
-    fft_staged<NSEQ>(sm, F, tw, tws, false);
+#pragma unroll
+    for (int q = 0; q < LPT; q++) {
+        const int i = (int)threadIdx.x + q * 256;
+        if (i < n) sm[lp(i, F.lpsh)] = make_double2(a_[q], b_[q]);
+    }
+    fft_staged(sm, F, tw, tws, false);
     const int mh = n / 2 + 1;
-    for (int e = threadIdx.x; e < mh * NSEQ; e += 256) {
-        const int m = e / NSEQ, sq = e - m * NSEQ;
-        const double2 a = sm[sq * n + m], b = cconj(sm[sq * n + (m == 0 ? 0 : n - m)]);
+    for (int m = threadIdx.x; m < mh; m += 256) {
+        const double2 a = sm[lp(m, F.lpsh)], b = cconj(sm[lp(m == 0 ? 0 : n - m, F.lpsh)]);
         const double2 X = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y + b.y));
         const double2 d = csub(a, b);
         const double2 Y = make_double2(0.5 * d.y, -0.5 * d.x);  // (a − b)/(2i)
-        const int64_t ra = r0 + 2 * sq;
-        double2 *o = XF + ((int64_t)m * ncol + col) * nrows + ra;
-        if (ra < nrows) o[0] = X;
-        if (ra + 1 < nrows) o[1] = Y;
+        double2 *o = XF + ((int64_t)m * ncol + col) * nrows + r0;
+        o[0] = X;
+        if (vb) o[1] = Y;
     }
 }
 
-// Inverse, 2·NSEQ rows per block: OUT[m][col][row] (Hermitian halves) → yout[col][row][i] / N.
-template <int NSEQ>
+// Inverse, two rows per block: OUT[m][col][row] (Hermitian halves) → yout[col][row][i] / N.
+template <int LPT>
 __global__ __launch_bounds__(256) void k_ifft_rows_staged(RingFft F, const double2 *__restrict__ tw,
                                                           const double2 *__restrict__ tws, const double2 *__restrict__ OUT,
                                                           double *__restrict__ yout, int64_t nrows, int64_t ncol) {
     extern __shared__ double2 sm[];
-    constexpr int RPB = 2 * NSEQ;
     const int n = F.n;
-    const int64_t ngrp = (nrows + RPB - 1) / RPB;
+    const int64_t ngrp = (nrows + 1) / 2;
     int64_t item;
     if (!xcd_item(ngrp * ncol, item)) return;
-    const int64_t col = item / ngrp, r0 = (item % ngrp) * RPB;
+    const int64_t col = item / ngrp, r0 = (item % ngrp) * 2;
+    const bool vb = r0 + 1 < nrows;
     const int mh = n / 2 + 1;
-    for (int e = threadIdx.x; e < n * NSEQ; e += 256) {
-        const int m = e / NSEQ, sq = e - m * NSEQ;
-        const bool lo = m < mh;
-        const int mm = lo ? m : n - m;
-        const int64_t ra = r0 + 2 * sq;
-        const double2 *o = OUT + ((int64_t)mm * ncol + col) * nrows + ra;
-        double2 X = ra < nrows ? o[0] : make_double2(0.0, 0.0);
-        double2 Y = ra + 1 < nrows ? o[1] : make_double2(0.0, 0.0);
-        if (!lo) {
-            X = cconj(X);
-            Y = cconj(Y);
-        }
-        sm[sq * n + m] = make_double2(X.x - Y.y, X.y + Y.x);  // X + iY
+    double2 X_[LPT], Y_[LPT];  // all gathers in flight together
+#pragma unroll
+    for (int q = 0; q < LPT; q++) {
+        const int m = (int)threadIdx.x + q * 256;
+        const int mm = m < mh ? m : n - m;
+        X_[q] = m < n ? OUT[spec_at(mm, col, r0, nrows, ncol)] : make_double2(0.0, 0.0);
+        Y_[q] = m < n && vb ? OUT[spec_at(mm, col, r0 + 1, nrows, ncol)] : make_double2(0.0, 0.0);
     }
-    fft_staged<NSEQ>(sm, F, tw, tws, true);
+#pragma unroll
+    for (int q = 0; q < LPT; q++) {
+        const int m = (int)threadIdx.x + q * 256;
+        if (m < n) {
+            double2 X = X_[q], Y = Y_[q];
+            if (m >= mh) {
+                X = cconj(X);
+                Y = cconj(Y);
+            }
+            sm[lp(m, F.lpsh)] = make_double2(X.x - Y.y, X.y + Y.x);  // X + iY
+        }
+    }
+    fft_staged(sm, F, tw, tws, true);
     const double sc = 1.0 / n;
     double *y = yout + (col * nrows + r0) * n;
-    for (int e = threadIdx.x; e < n * NSEQ; e += 256) {
-        const int sq = NSEQ == 1 ? 0 : (e >= n ? 1 : 0);
-        const int i = e - sq * n;
-        const double2 z = sm[e];
-        if (r0 + 2 * sq < nrows) y[(2 * sq) * n + i] = z.x * sc;
-        if (r0 + 2 * sq + 1 < nrows) y[(2 * sq + 1) * n + i] = z.y * sc;
+    for (int i = threadIdx.x; i < n; i += 256) {
+        const double2 z = sm[lp(i, F.lpsh)];
+        y[i] = z.x * sc;
+        if (vb) y[n + i] = z.y * sc;
     }
-}
-
-// One sequence (two real rows) per 256-thread block: 2-row blocks keep the kernels near 100 VGPRs,
-// so four blocks share a CU; the LDS of a sequence must also leave room for them.
-int ring_staged_nseq(const RingFft &F) {
-    if ((size_t)F.n * sizeof(double2) > 32 * 1024 || F.nstage == 0) return 0;
-    for (int st = 0; st < F.nstage; st++) {
-        const int R = F.radix[st];
-        if (staged_mb_max(R) == 0 || (F.n / R + 255) / 256 > staged_mb_max(R)) return 0;
-    }
-    return 1;
 }
 
 // Banded contraction per Fourier mode m. Block: 256 target rows of one plane (j0..j0+255, level k)
@@ -624,21 +734,39 @@ __device__ __forceinline__ double spectral_transfer(int kind, double alpha, doub
 // MODE 0: transform along y, × Ĝ/Ny, transform back (one scale); MODE 1: the forward transform only
 // (the analysis a sweep over scales shares, FFTWExt.jl:100-110); MODE 2: × Ĝ/Ny and back (one
 // scale's synthesis from the shared analysis, FFTWExt.jl:112-120).
-template <bool STAGED, int MODE>
+// LPT > 0: the register-staged transform (padded LDS, LPT·256 ≥ Ny); LPT = 0: the LDS ping-pong.
+template <int LPT, int MODE>
 __global__ __launch_bounds__(256) void k_spec_cols(RingFft F, const double2 *__restrict__ tw,
                                                    const double2 *__restrict__ tws, int kind, double alpha, double ell,
                                                    double dkx, double dky, const double2 *__restrict__ XF,
                                                    double2 *__restrict__ OUT, int64_t ncol, int64_t mh) {
     extern __shared__ double2 sm[];
+    constexpr bool STAGED = LPT > 0;
     const int n = F.n;
     int64_t item;
     if (!xcd_item(mh * ncol, item)) return;
     const int64_t m = item / ncol;
     const double2 *x = XF + item * n;
-    for (int e = threadIdx.x; e < n; e += 256) sm[e] = x[e];
+    const int sh = STAGED ? F.lpsh : 31;
+    auto at = [sh](int e) { return lp(e, sh); };
+    if constexpr (STAGED) {  // all loads in flight together
+        double2 v_[LPT];
+#pragma unroll
+        for (int q = 0; q < LPT; q++) {
+            const int e = (int)threadIdx.x + q * 256;
+            v_[q] = e < n ? x[e] : make_double2(0.0, 0.0);
+        }
+#pragma unroll
+        for (int q = 0; q < LPT; q++) {
+            const int e = (int)threadIdx.x + q * 256;
+            if (e < n) sm[lp(e, sh)] = v_[q];
+        }
+    } else {
+        for (int e = threadIdx.x; e < n; e += 256) sm[e] = x[e];
+    }
     double2 *z = sm;
     if (MODE != 2) {
-        if (STAGED) fft_staged<1>(sm, F, tw, tws, false);
+        if (STAGED) fft_staged(sm, F, tw, tws, false);
         else z = fft_lds(sm, sm + n, F, tw, false, 1);
     }
     double2 *r = z;
@@ -650,13 +778,15 @@ __global__ __launch_bounds__(256) void k_spec_cols(RingFft F, const double2 *__r
             const int nn = e <= (n - 1) / 2 ? e : e - n;
             const double ky = dky * (double)nn;
             const double g = spectral_transfer(kind, alpha, sqrt(kx * kx + ky * ky), ell) * sc;
-            z[e] = make_double2(z[e].x * g, z[e].y * g);
+            const double2 c = z[at(e)];
+            z[at(e)] = make_double2(c.x * g, c.y * g);
         }
-        if (STAGED) fft_staged<1>(z, F, tw, tws, true);
+        if (STAGED) fft_staged(z, F, tw, tws, true);
         else r = fft_lds(z, z == sm ? sm + n : sm, F, tw, true, 1);
     }
+    if (MODE == 1) __syncthreads();
     double2 *o = OUT + item * n;
-    for (int e = threadIdx.x; e < n; e += 256) o[e] = r[e];
+    for (int e = threadIdx.x; e < n; e += 256) o[e] = r[at(e)];
 }
 
 // FFTWExt.jl:162-176: the filtered mask inverted where |filter(mask)| ≥ 0.01, else 0
@@ -703,9 +833,7 @@ __global__ __launch_bounds__(256) void k_spec_epilogue(GeomArgs G, const double 
 // ------------------------------------------------------------------------------------ launches ----
 
 static size_t fft_shmem(const RingFft &F) { return (size_t)2 * F.n * sizeof(double2); }
-static size_t staged_shmem(const RingFft &F, int nseq) {
-    return (size_t)nseq * F.n * sizeof(double2);
-}
+static size_t staged_shmem(const RingFft &F) { return (size_t)(F.n + F.n / 16 + 1) * sizeof(double2); }
 
 static void set_shmem(const void *fn, size_t bytes) {
     if (bytes > 64 * 1024) CGEF_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
@@ -721,35 +849,43 @@ void ring_build_what(const RingPlan &R, const Run *runs_d, const int32_t *run_of
 }
 
 void ring_forward(const RingPlan &R, const double *xin, double2 *XF, int64_t ncol, hipStream_t s) {
-    const int ns = R.fft.nseq;
-    if (ns > 0) {
-        const size_t sh = staged_shmem(R.fft, ns);
-        const int64_t items = (R.args.nrows + 2 * ns - 1) / (2 * ns) * ncol;
-        const dim3 grid((unsigned)(8 * ((items + 7) / 8)));
-        hipLaunchKernelGGL(k_fft_rows_staged<1>, grid, dim3(256), sh, s, R.fft, R.tw_d, R.tws_d, xin, XF, R.args.nrows, ncol);
+    const int64_t items = (R.args.nrows + 1) / 2 * ncol;
+    const dim3 grid((unsigned)(8 * ((items + 7) / 8)));
+    if (R.fft.nseq > 0) {
+        const size_t sh = staged_shmem(R.fft);
+        if (R.fft.n <= 2048) {
+            hipLaunchKernelGGL(k_fft_rows_staged<8>, grid, dim3(256), sh, s, R.fft, R.tw_d, R.tws_d, xin, XF,
+                               R.args.nrows, ncol);
+        } else {
+            set_shmem((const void *)k_fft_rows_staged<16>, sh);
+            hipLaunchKernelGGL(k_fft_rows_staged<16>, grid, dim3(256), sh, s, R.fft, R.tw_d, R.tws_d, xin, XF,
+                               R.args.nrows, ncol);
+        }
         return;
     }
     const size_t sh = fft_shmem(R.fft);
     set_shmem((const void *)k_fft_rows, sh);
-    const int64_t items = (R.args.nrows + 1) / 2 * ncol;
-    hipLaunchKernelGGL(k_fft_rows, dim3((unsigned)(8 * ((items + 7) / 8))), dim3(256), sh, s, R.fft, R.tw_d, xin, XF,
-                       R.args.nrows, ncol);
+    hipLaunchKernelGGL(k_fft_rows, grid, dim3(256), sh, s, R.fft, R.tw_d, xin, XF, R.args.nrows, ncol);
 }
 
 void ring_inverse(const RingPlan &R, const double2 *OUT, double *yout, int64_t ncol, hipStream_t s) {
-    const int ns = R.fft.nseq;
-    if (ns > 0) {
-        const size_t sh = staged_shmem(R.fft, ns);
-        const int64_t items = (R.args.nrows + 2 * ns - 1) / (2 * ns) * ncol;
-        const dim3 grid((unsigned)(8 * ((items + 7) / 8)));
-        hipLaunchKernelGGL(k_ifft_rows_staged<1>, grid, dim3(256), sh, s, R.fft, R.tw_d, R.tws_d, OUT, yout, R.args.nrows, ncol);
+    const int64_t items = (R.args.nrows + 1) / 2 * ncol;
+    const dim3 grid((unsigned)(8 * ((items + 7) / 8)));
+    if (R.fft.nseq > 0) {
+        const size_t sh = staged_shmem(R.fft);
+        if (R.fft.n <= 2048) {
+            hipLaunchKernelGGL(k_ifft_rows_staged<8>, grid, dim3(256), sh, s, R.fft, R.tw_d, R.tws_d, OUT, yout,
+                               R.args.nrows, ncol);
+        } else {
+            set_shmem((const void *)k_ifft_rows_staged<16>, sh);
+            hipLaunchKernelGGL(k_ifft_rows_staged<16>, grid, dim3(256), sh, s, R.fft, R.tw_d, R.tws_d, OUT, yout,
+                               R.args.nrows, ncol);
+        }
         return;
     }
     const size_t sh = fft_shmem(R.fft);
     set_shmem((const void *)k_ifft_rows, sh);
-    const int64_t items = (R.args.nrows + 1) / 2 * ncol;
-    hipLaunchKernelGGL(k_ifft_rows, dim3((unsigned)(8 * ((items + 7) / 8))), dim3(256), sh, s, R.fft, R.tw_d, OUT, yout,
-                       R.args.nrows, ncol);
+    hipLaunchKernelGGL(k_ifft_rows, grid, dim3(256), sh, s, R.fft, R.tw_d, OUT, yout, R.args.nrows, ncol);
 }
 
 void ring_contract(const RingPlan &R, const double2 *XF, double2 *OUT, int64_t ncol, hipStream_t s) {
@@ -795,13 +931,19 @@ static void spec_cols(const RingPlan &R, const double2 *XF, double2 *OUT, int64_
     const int64_t items = R.mh * ncol;
     const dim3 grid((unsigned)(8 * ((items + 7) / 8)));
     if (R.fy.nseq > 0) {
-        const size_t sh = staged_shmem(R.fy, 1);
-        hipLaunchKernelGGL((k_spec_cols<true, MODE>), grid, dim3(256), sh, s, R.fy, R.twy_d, R.twsy_d, R.kind, R.alpha,
-                           R.ell, R.dkx, R.dky, XF, OUT, ncol, R.mh);
+        const size_t sh = staged_shmem(R.fy);
+        if (R.fy.n <= 2048) {
+            hipLaunchKernelGGL((k_spec_cols<8, MODE>), grid, dim3(256), sh, s, R.fy, R.twy_d, R.twsy_d, R.kind, R.alpha,
+                               R.ell, R.dkx, R.dky, XF, OUT, ncol, R.mh);
+        } else {
+            set_shmem((const void *)k_spec_cols<16, MODE>, sh);
+            hipLaunchKernelGGL((k_spec_cols<16, MODE>), grid, dim3(256), sh, s, R.fy, R.twy_d, R.twsy_d, R.kind, R.alpha,
+                               R.ell, R.dkx, R.dky, XF, OUT, ncol, R.mh);
+        }
     } else {
         const size_t sh = fft_shmem(R.fy);
-        set_shmem((const void *)k_spec_cols<false, MODE>, sh);
-        hipLaunchKernelGGL((k_spec_cols<false, MODE>), grid, dim3(256), sh, s, R.fy, R.twy_d, R.twsy_d, R.kind, R.alpha,
+        set_shmem((const void *)k_spec_cols<0, MODE>, sh);
+        hipLaunchKernelGGL((k_spec_cols<0, MODE>), grid, dim3(256), sh, s, R.fy, R.twy_d, R.twsy_d, R.kind, R.alpha,
                            R.ell, R.dkx, R.dky, XF, OUT, ncol, R.mh);
     }
 }

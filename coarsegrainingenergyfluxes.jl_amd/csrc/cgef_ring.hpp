@@ -11,10 +11,10 @@
 namespace cgef {
 
 bool ring_factor(int n, RingFft &F);
-// Complex sequences per block of the register-staged row FFT (1), or 0 when a stage's radix or
-// butterfly count exceeds its register budget or the row exceeds 32 KiB of LDS (then the LDS
-// ping-pong FFT serves the plan).
-int ring_staged_nseq(const RingFft &F);
+// The register-staged factorization of F.n (F.snst, F.srad, F.lpsh): the fewest stages whose
+// butterflies fit the per-thread register budget of a 256-thread block, odd radices first. False when
+// there is none or the length exceeds 4096 (then the LDS ping-pong FFT serves).
+bool ring_staged_factor(RingFft &F);
 bool ring_supported(const Plan &p);
 void fft_tables(RingFft &F, double2 *&tw_d, double2 *&tws_d);
 bool spectral_supported(const Grid &g, int kind);

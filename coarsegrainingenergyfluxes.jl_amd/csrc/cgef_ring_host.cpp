@@ -50,6 +50,52 @@ bool ring_supported(const Plan &p) {
     return true;
 }
 
+// Butterflies per thread a staged stage of radix R may hold (the radix cases compiled into the staged
+// kernels, about 100 VGPRs together); 0 for radices the staged FFT does not evaluate (the primes 7,
+// 11, 13 take the LDS ping-pong FFT).
+static int staged_mb_max(int R) {
+    switch (R) {
+        case 2: return 6;
+        case 3: return 4;
+        case 4: return 3;
+        case 5: case 6: case 8: return 2;
+        case 9: case 10: case 12: case 15: case 16: return 1;
+        default: return 0;
+    }
+}
+
+static bool staged_dfs(int n, int m, int depth, int *rad, int k) {
+    if (m == 1) return true;
+    if (depth == 0) return false;
+    static const int pref[] = {16, 12, 10, 9, 15, 8, 6, 5, 4, 3, 2};
+    for (int R : pref) {
+        if (m % R != 0 || (n / R + 255) / 256 > staged_mb_max(R)) continue;
+        rad[k] = R;
+        if (staged_dfs(n, m / R, depth - 1, rad, k + 1)) return true;
+    }
+    return false;
+}
+
+bool ring_staged_factor(RingFft &F) {
+    const int n = F.n;
+    F.snst = 0;
+    F.lpsh = 31;
+    if (n < 2 || n > 4096 || (size_t)(n + n / 16 + 1) * sizeof(double2) > 80 * 1024) return false;
+    int rad[RING_MAX_STAGES];
+    for (int d = 1; d <= RING_MAX_STAGES; d++) {
+        if (!staged_dfs(n, n, d, rad, 0)) continue;
+        int cnt = 0, m = n;
+        while (m > 1) m /= rad[cnt++];
+        // odd radices first: the first stage's stride-R writes then fall on distinct banks
+        std::stable_sort(rad, rad + cnt, [](int a, int b) { return (a & 1) > (b & 1); });
+        F.snst = cnt;
+        for (int st = 0; st < cnt; st++) F.srad[st] = rad[st];
+        F.lpsh = (rad[0] & 1) ? 31 : 4;
+        return true;
+    }
+    return false;
+}
+
 // Twiddle tables of a factored length: exp(−2πik/N) and the per-stage table of the register-staged
 // FFT (stage of radix R after Ns points: exp(−2πi·r·k/(Ns·R)) at twoff + k·(R−1) + r−1).
 void fft_tables(RingFft &F, double2 *&tw_d, double2 *&tws_d) {
@@ -60,10 +106,11 @@ void fft_tables(RingFft &F, double2 *&tw_d, double2 *&tws_d) {
         tw[k] = make_double2(std::cos(a), -std::sin(a));
     }
     tw_d = up(tw);
+    F.nseq = ring_staged_factor(F) ? 1 : 0;
     std::vector<double2> tws;
     int Ns = 1;
-    for (int st = 0; st < F.nstage; st++) {
-        const int Rr = F.radix[st];
+    for (int st = 0; st < F.snst; st++) {
+        const int Rr = F.srad[st];
         F.twoff[st] = (int)tws.size();
         for (int k = 0; k < Ns; k++)
             for (int r = 1; r < Rr; r++) {
@@ -75,7 +122,6 @@ void fft_tables(RingFft &F, double2 *&tw_d, double2 *&tws_d) {
     if (tws.empty()) tws.push_back(make_double2(1.0, 0.0));
     F.twsize = (int)tws.size();
     tws_d = up(tws);
-    F.nseq = ring_staged_nseq(F);
 }
 
 // The spectral engine: row transforms of the ring engine along x, register-staged transforms along y.
@@ -87,7 +133,8 @@ bool spectral_supported(const Grid &g, int kind) {
     if (!ring_factor((int)g.n[0], fx) || !ring_factor((int)g.n[1], fy)) return false;
     // each transform runs in one workgroup's LDS: register-staged, else the ping-pong pair
     auto fits = [](const RingFft &F) {
-        return ring_staged_nseq(F) > 0 || (size_t)2 * F.n * sizeof(double2) <= 160 * 1024;
+        RingFft t = F;
+        return ring_staged_factor(t) || (size_t)2 * F.n * sizeof(double2) <= 160 * 1024;
     };
     return fits(fx) && fits(fy);
 }

@@ -306,10 +306,12 @@ def test_ring_filter_sphere(strategy, dtype, tol):
 
 
 @pytest.mark.parametrize("nx,ny,ell", [(45, 30, 9000.0), (56, 24, 8000.0), (32, 32, 40000.0), (44, 20, 7000.0),
-                                       (26, 18, 5000.0)])
+                                       (26, 18, 5000.0), (256, 6, 9000.0), (360, 6, 9000.0), (1440, 4, 30000.0),
+                                       (2048, 4, 9000.0), (3000, 4, 9000.0), (4096, 3, 12000.0)])
 def test_ring_filter_cartesian_odd_sizes_and_multiwrap(nx, ny, ell):
     # odd length (radices 3·3·5), a radix-7 length, a radius larger than the box (runs that wrap the
-    # ring more than once), and radix-11 / radix-13 lengths (served by the LDS ping-pong FFT)
+    # ring more than once), radix-11 / radix-13 lengths (served by the LDS ping-pong FFT), and the
+    # register-staged factorizations 16·16, 3·12·10, 9·16·10, 8·16·16, 3·5·…, 16·16·16 (padded LDS)
     xs, ys = cart_axes(nx, 1000.0), cart_axes(ny, 1000.0)
     mask = blob_mask((ny, nx), 41)
     og = oracle_grid_cart(xs, ys, periodic=(True, True), mask=mask)
@@ -531,8 +533,9 @@ def spectral_case(ny, nx, masked, seed=41):
 KERNELS = {"gaussian": (cgef.GaussianKernel(6.0), O.GAUSSIAN, 6.0), "tophat": (cgef.TopHatKernel(), O.TOPHAT, 0.0)}
 
 
-# (26, 44): radix 13 along y takes the LDS ping-pong transform; (90, 120): several radix-2/3/4/5 stages
-@pytest.mark.parametrize("shape", [(24, 32), (21, 30), (26, 44), (90, 120)])
+# (26, 44): radix 13 along y takes the LDS ping-pong transform; (90, 120): radices 9·10 and 15·8;
+# (256, 192): 16·16 and 12·16 in the padded layout; (4096, 6): a 4096-point column transform
+@pytest.mark.parametrize("shape", [(24, 32), (21, 30), (26, 44), (90, 120), (256, 192), (4096, 6)])
 @pytest.mark.parametrize("kernel", ["gaussian", "tophat"])
 @pytest.mark.parametrize("strategy", ["zerofill", "deformable"])
 def test_spectral_filter(shape, kernel, strategy):
