@@ -79,7 +79,7 @@ __device__ __forceinline__ double2 mul_root(double2 v, bool inv) {
 template <int A, int B>
 __device__ __forceinline__ void dft_comp(double2 *v, bool inv);
 template <int R>
-__device__ __forceinline__ void dft_direct(double2 *v, bool inv);
+__device__ __forceinline__ void dft_direct(double2 *v, const double2 *tw, int n, bool inv);
 
 // In-register DFT of length R on v, roots from the N-point table (R divides N): closed forms for
 // 2, 3, 4, 5, two-level products of those for the composite radices, the direct sum otherwise.
@@ -140,28 +140,25 @@ __device__ __forceinline__ void dft_small(double2 *v, const double2 *tw, int n, 
     } else if constexpr (R == 16) {
         dft_comp<4, 4>(v, inv);
     } else {
-        dft_direct<R>(v, inv);
+        dft_direct<R>(v, tw, n, inv);
     }
 }
 
-// The direct sum for the prime radices 7, 11, 13 (roots as compile-time constants).
-template <int R, int A, int... B>
-__device__ __forceinline__ double2 direct_sum(const double2 *v, bool inv, std::integer_sequence<int, B...>) {
-    double2 s = v[0];
-    ((s = cadd(s, mul_root<(A * (B + 1)) % R, R>(v[B + 1], inv))), ...);
-    return s;
-}
-
-template <int R, int... A>
-__device__ __forceinline__ void dft_direct_rows(double2 *v, bool inv, std::integer_sequence<int, A...>) {
-    const double2 o[R] = {direct_sum<R, A>(v, inv, std::make_integer_sequence<int, R - 1>{})...};
+// The direct sum for the prime radices 7, 11, 13 (roots from the N-point table; these radices serve
+// only the LDS ping-pong FFT).
+template <int R>
+__device__ __forceinline__ void dft_direct(double2 *v, const double2 *tw, int n, bool inv) {
+    double2 o[R];
+    const int step = n / R;
+#pragma unroll
+    for (int a = 0; a < R; a++) {
+        double2 s = v[0];
+#pragma unroll
+        for (int b = 1; b < R; b++) s = cadd(s, cmul(v[b], twid(tw, ((a * b) % R) * step, inv)));
+        o[a] = s;
+    }
 #pragma unroll
     for (int a = 0; a < R; a++) v[a] = o[a];
-}
-
-template <int R>
-__device__ __forceinline__ void dft_direct(double2 *v, bool inv) {
-    dft_direct_rows<R>(v, inv, std::make_integer_sequence<int, R>{});
 }
 
 // R = A·B: with q = q2 + B·q1 and k = k1 + A·k2,
@@ -260,6 +257,28 @@ __device__ __forceinline__ bool xcd_item(int64_t total, int64_t &item) {
     return item < total;
 }
 
+// Offset of mode m of row `row` of column `col` in a spectra set [m][col][row].
+__device__ __forceinline__ int64_t spec_at(int64_t m, int64_t col, int64_t row, int64_t nrows, int64_t ncol) {
+    return (m * ncol + col) * nrows + row;
+}
+
+// The forward half-spectra of a row pair from the complex transform z of x_r0 + i·x_r1 (element e of
+// z at z[at(e)]): output e of the pair's 2·mh is mode e/2 of row r0 + e%2, so that consecutive lanes
+// write the pair's adjacent values.
+template <typename At>
+__device__ __forceinline__ void write_pair_spectra(const double2 *z, At at, int n, double2 *XF, int64_t col, int64_t r0,
+                                                   int64_t nrows, int64_t ncol) {
+    const int mh = n / 2 + 1;
+    for (int e = threadIdx.x; e < 2 * mh; e += blockDim.x) {
+        const int rr = e & 1, m = e >> 1;
+        if (r0 + rr >= nrows) continue;
+        const double2 a = z[at(m)], b = cconj(z[at(m == 0 ? 0 : n - m)]);
+        // X = (a + b)/2 for row r0, Y = (a − b)/(2i) for row r0 + 1
+        XF[spec_at(m, col, r0 + rr, nrows, ncol)] = rr == 0 ? make_double2(0.5 * (a.x + b.x), 0.5 * (a.y + b.y))
+                                                            : make_double2(0.5 * (a.y - b.y), -0.5 * (a.x - b.x));
+    }
+}
+
 // Forward: real rows (r0, r0+1) of column `col` (xin[col][row][i]) → XF[m][col][row], m ≤ N/2.
 __global__ __launch_bounds__(256) void k_fft_rows(RingFft F, const double2 *__restrict__ tw, const double *__restrict__ xin,
                                                   double2 *__restrict__ XF, int64_t nrows, int64_t ncol) {
@@ -275,21 +294,7 @@ __global__ __launch_bounds__(256) void k_fft_rows(RingFft F, const double2 *__re
     const double *x1 = has1 ? x0 + n : nullptr;
     for (int i = threadIdx.x; i < n; i += blockDim.x) sm[i] = make_double2(x0[i], has1 ? x1[i] : 0.0);
     const double2 *Z = fft_lds(sm, sm + n, F, tw, false, 1);
-    const int mh = n / 2 + 1;
-    for (int m = threadIdx.x; m < mh; m += blockDim.x) {
-        const double2 a = Z[m], b = cconj(Z[(n - m) % n]);
-        const double2 X = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y + b.y));
-        const double2 d = csub(a, b);
-        const double2 Y = make_double2(0.5 * d.y, -0.5 * d.x);  // (a − b)/(2i)
-        double2 *o = XF + ((int64_t)m * ncol + col) * nrows;
-        o[r0] = X;
-        if (has1) o[r1] = Y;
-    }
-}
-
-// Offset of mode m of row `row` of column `col` in a spectra set [m][col][row].
-__device__ __forceinline__ int64_t spec_at(int64_t m, int64_t col, int64_t row, int64_t nrows, int64_t ncol) {
-    return (m * ncol + col) * nrows + row;
+    write_pair_spectra(Z, [](int e) { return e; }, n, XF, col, r0, nrows, ncol);
 }
 
 // Inverse: OUT[m][col][row] (Hermitian halves) → yout[col][row][i], normalized by 1/N.
@@ -430,16 +435,8 @@ __global__ __launch_bounds__(256) void k_fft_rows_staged(RingFft F, const double
         if (i < n) sm[lp(i, F.lpsh)] = make_double2(a_[q], b_[q]);
     }
     fft_staged(sm, F, tw, tws, false);
-    const int mh = n / 2 + 1;
-    for (int m = threadIdx.x; m < mh; m += 256) {
-        const double2 a = sm[lp(m, F.lpsh)], b = cconj(sm[lp(m == 0 ? 0 : n - m, F.lpsh)]);
-        const double2 X = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y + b.y));
-        const double2 d = csub(a, b);
-        const double2 Y = make_double2(0.5 * d.y, -0.5 * d.x);  // (a − b)/(2i)
-        double2 *o = XF + ((int64_t)m * ncol + col) * nrows + r0;
-        o[0] = X;
-        if (vb) o[1] = Y;
-    }
+    const int sh = F.lpsh;
+    write_pair_spectra(sm, [sh](int e) { return lp(e, sh); }, n, XF, col, r0, nrows, ncol);
 }
 
 // Inverse, two rows per block: OUT[m][col][row] (Hermitian halves) → yout[col][row][i] / N.
