@@ -439,7 +439,9 @@ __global__ __launch_bounds__(256) void k_fft_rows_staged(RingFft F, const double
     write_pair_spectra(sm, [sh](int e) { return lp(e, sh); }, n, XF, col, r0, nrows, ncol);
 }
 
-// Inverse, two rows per block: OUT[m][col][row] (Hermitian halves) → yout[col][row][i] / N.
+// Inverse, two rows per block: OUT[m][col][row] (Hermitian halves) → yout[col][row][i] / N. Each
+// stored value is read once, lane per (mode, row) so that a pair's two values are adjacent; the lane
+// pair exchanges them and writes X + iY at m and its conjugate mirror at N − m. LPT·256 ≥ 2·(N/2+1).
 template <int LPT>
 __global__ __launch_bounds__(256) void k_ifft_rows_staged(RingFft F, const double2 *__restrict__ tw,
                                                           const double2 *__restrict__ tws, const double2 *__restrict__ OUT,
@@ -452,31 +454,32 @@ __global__ __launch_bounds__(256) void k_ifft_rows_staged(RingFft F, const doubl
     const int64_t col = item / ngrp, r0 = (item % ngrp) * 2;
     const bool vb = r0 + 1 < nrows;
     const int mh = n / 2 + 1;
-    double2 X_[LPT], Y_[LPT];  // all gathers in flight together
+    const int sh = F.lpsh;
+    double2 v_[LPT];  // all gathers in flight together
 #pragma unroll
     for (int q = 0; q < LPT; q++) {
-        const int m = (int)threadIdx.x + q * 256;
-        const int mm = m < mh ? m : n - m;
-        X_[q] = m < n ? OUT[spec_at(mm, col, r0, nrows, ncol)] : make_double2(0.0, 0.0);
-        Y_[q] = m < n && vb ? OUT[spec_at(mm, col, r0 + 1, nrows, ncol)] : make_double2(0.0, 0.0);
+        const int e = (int)threadIdx.x + q * 256;
+        const int rr = e & 1, mm = e >> 1;
+        v_[q] = mm < mh && (rr == 0 || vb) ? OUT[spec_at(mm, col, r0 + rr, nrows, ncol)] : make_double2(0.0, 0.0);
     }
 #pragma unroll
     for (int q = 0; q < LPT; q++) {
-        const int m = (int)threadIdx.x + q * 256;
-        if (m < n) {
-            double2 X = X_[q], Y = Y_[q];
-            if (m >= mh) {
-                X = cconj(X);
-                Y = cconj(Y);
-            }
-            sm[lp(m, F.lpsh)] = make_double2(X.x - Y.y, X.y + Y.x);  // X + iY
+        const int e = (int)threadIdx.x + q * 256;
+        const int rr = e & 1, mm = e >> 1;
+        // lanes 2i, 2i+1 hold X (row r0) and Y (row r0+1) of mode mm
+        const double2 o = make_double2(__shfl_xor(v_[q].x, 1), __shfl_xor(v_[q].y, 1));
+        const double2 X = rr == 0 ? v_[q] : o, Y = rr == 0 ? o : v_[q];
+        if (mm < mh) {
+            if (rr == 0) sm[lp(mm, sh)] = make_double2(X.x - Y.y, X.y + Y.x);  // X + iY
+            else if (mm > 0 && n - mm >= mh)
+                sm[lp(n - mm, sh)] = make_double2(X.x + Y.y, Y.x - X.y);  // conj(X) + i·conj(Y)
         }
     }
     fft_staged(sm, F, tw, tws, true);
     const double sc = 1.0 / n;
     double *y = yout + (col * nrows + r0) * n;
     for (int i = threadIdx.x; i < n; i += 256) {
-        const double2 z = sm[lp(i, F.lpsh)];
+        const double2 z = sm[lp(i, sh)];
         y[i] = z.x * sc;
         if (vb) y[n + i] = z.y * sc;
     }
@@ -871,11 +874,11 @@ void ring_inverse(const RingPlan &R, const double2 *OUT, double *yout, int64_t n
     if (R.fft.nseq > 0) {
         const size_t sh = staged_shmem(R.fft);
         if (R.fft.n <= 2048) {
-            hipLaunchKernelGGL(k_ifft_rows_staged<8>, grid, dim3(256), sh, s, R.fft, R.tw_d, R.tws_d, OUT, yout,
+            hipLaunchKernelGGL(k_ifft_rows_staged<9>, grid, dim3(256), sh, s, R.fft, R.tw_d, R.tws_d, OUT, yout,
                                R.args.nrows, ncol);
         } else {
-            set_shmem((const void *)k_ifft_rows_staged<16>, sh);
-            hipLaunchKernelGGL(k_ifft_rows_staged<16>, grid, dim3(256), sh, s, R.fft, R.tw_d, R.tws_d, OUT, yout,
+            set_shmem((const void *)k_ifft_rows_staged<17>, sh);
+            hipLaunchKernelGGL(k_ifft_rows_staged<17>, grid, dim3(256), sh, s, R.fft, R.tw_d, R.tws_d, OUT, yout,
                                R.args.nrows, ncol);
         }
         return;
