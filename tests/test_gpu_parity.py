@@ -286,19 +286,23 @@ def test_error_paths():
 # The ring engine evaluates the same banded footprint in Fourier space along the periodic x axis
 # (FP64 throughout): parity with the oracle's direct sum to FFT rounding.
 
+# ℓ = 6000 km spans ±35 latitude rows (71 offsets): the register-blocked wide-footprint contraction
+@pytest.mark.parametrize("ell", [1.6e6, 6.0e6])
 @pytest.mark.parametrize("strategy", ["zerofill", "deformable"])
 @pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-11), (torch.float32, 1e-6)])
-def test_ring_filter_sphere(strategy, dtype, tol):
+def test_ring_filter_sphere(ell, strategy, dtype, tol):
     og, g, _, _ = sphere_case(True)
     f = smooth_random_field((3,) + g.field_shape, 18, 1)[0]
     s_cg, s_o = STRATS[strategy]
-    plan = cgef.plan_filter(g, cgef.GaussianKernel(4.0), 1.6e6, mask_strategy=s_cg)
+    plan = cgef.plan_filter(g, cgef.GaussianKernel(4.0), ell, mask_strategy=s_cg)
     assert plan.engine == "ring"
+    if ell > 5e6:
+        assert plan.info.ring_offsets >= 64
     inp = T(f, dtype)
     out = torch.empty_like(inp)
     cgef.filter_apply_(out, inp, plan)
     fin = N(inp)
-    fp = O.Footprint(og, O.GAUSSIAN, 4.0, 1.6e6)
+    fp = O.Footprint(og, O.GAUSSIAN, 4.0, ell)
     got = N(out)
     for b in range(3):
         ref = O.apply_banded(og, fp, s_o, [fin[b]])[0]
@@ -323,6 +327,23 @@ def test_ring_filter_cartesian_odd_sizes_and_multiwrap(nx, ny, ell):
         cgef.filter_apply_(out, T(f), plan)
         ref = O.filter_fields(og, [f], kind=kind, alpha=alpha, ell=ell)[0]
         assert np.max(np.abs(N(out) - ref)) <= 1e-11 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("strategy", ["zerofill", "deformable"])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_ring_pi_wide_footprint(strategy, dtype):
+    # 71 offsets per mode on the sphere: the register-blocked contraction, bands differing per row
+    og, g, _, _ = sphere_case(True)
+    s_cg, s_o = STRATS[strategy]
+    u, v = smooth_random_field(g.field_shape, 46, 2)
+    U, V = T(u, dtype), T(v, dtype)
+    ker = cgef.GaussianKernel(4.0)
+    plan = cgef.plan_filter(g, ker, 6.0e6, mask_strategy=s_cg)
+    assert plan.engine == "ring" and plan.info.ring_offsets >= 64
+    Pi = torch.empty_like(U)
+    cgef.compute_pi_(Pi, U, V, None, g, ker, 6.0e6, filter_plan=plan)
+    ref = O.compute_pi(og, N(U), N(V), kind=O.GAUSSIAN, alpha=4.0, ell=6.0e6, strategy=s_o)[0]
+    assert rel_l2(N(Pi), ref) <= (1e-10 if dtype == torch.float64 else 1e-5)
 
 
 @pytest.mark.parametrize("geom", ["cartesian", "sphere", "volume"])
