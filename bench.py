@@ -420,8 +420,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extra", action="store_true", help="skip the secondary GPU engines")
     ap.add_argument("--no-batched", action="store_true", help="skip the 8-snapshot × 24-scale batched leg")
+    ap.add_argument("--only", default=None,
+                    help="comma-separated extra legs to run after the headline step (batched, tophat, cartesian, "
+                         "cartesian4096, volume, spectral); overrides --no-batched / --no-configs")
     ap.add_argument("--no-configs", action="store_true",
-                    help="skip the 256² Cartesian and 256×256×64 volume legs")
+                    help="skip the top-hat sphere, Cartesian 256² and 4096², volume and spectral legs")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 512 MiB → 512 MiB FP32 elementwise copy first, so that hardware counter runs "
                          "can scale FETCH_SIZE / WRITE_SIZE by a kernel of known traffic")
@@ -519,14 +522,24 @@ def main():
                                               "speedup_of_measured_engine": value / (npts / tb)}
         del bplan, P64, u64, v64
 
-    if not args.no_batched:
-        extra["batched_spectrum"] = batched_leg(args, grid, mask, lam, phi)
-    if not args.no_configs:
-        extra["tophat_sphere"] = tophat_leg(args, grid, lam, phi, mask, u32, v32)
-        extra["cartesian_256"] = cartesian_leg(args)
-        extra["cartesian_4096_separable"] = large_cartesian_leg(args)
-        extra["volume_256x256x64"] = volume_leg(args)
-        extra["spectral"] = spectral_leg(args)
+    legs = {"batched": ("batched_spectrum", lambda: batched_leg(args, grid, mask, lam, phi)),
+            "tophat": ("tophat_sphere", lambda: tophat_leg(args, grid, lam, phi, mask, u32, v32)),
+            "cartesian": ("cartesian_256", lambda: cartesian_leg(args)),
+            "cartesian4096": ("cartesian_4096_separable", lambda: large_cartesian_leg(args)),
+            "volume": ("volume_256x256x64", lambda: volume_leg(args)),
+            "spectral": ("spectral", lambda: spectral_leg(args))}
+    if args.only is not None:
+        chosen = [x for x in args.only.split(",") if x]
+        unknown = [x for x in chosen if x not in legs]
+        if unknown:
+            print(f"bench.py: unknown leg(s) {unknown}; choose from {sorted(legs)}", file=sys.stderr)
+            sys.exit(2)
+    else:
+        chosen = ([] if args.no_batched else ["batched"]) + \
+                 ([] if args.no_configs else ["tophat", "cartesian", "cartesian4096", "volume", "spectral"])
+    for name in chosen:
+        key, run = legs[name]
+        extra[key] = run()
 
     cpu = None
     if not args.no_cpu:

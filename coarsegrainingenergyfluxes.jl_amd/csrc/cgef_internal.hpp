@@ -50,6 +50,8 @@ struct RingFft {
     int snst = 0;
     int srad[RING_MAX_STAGES] = {};
     int lpsh = 31;
+    int nsq = 1;                       // staged row transforms: sequences (row pairs) per block
+    int sstr = 0;                      // staged LDS stride between sequences (padded length)
     int twoff[RING_MAX_STAGES] = {};  // offset of staged stage st in the per-stage twiddle table
     int twsize = 0;                    // entries of the per-stage twiddle table
     int nseq = 0;                      // 1: the register-staged FFT serves this length (one sequence per block)

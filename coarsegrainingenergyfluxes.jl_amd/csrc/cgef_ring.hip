@@ -262,20 +262,20 @@ __device__ __forceinline__ int64_t spec_at(int64_t m, int64_t col, int64_t row, 
     return (m * ncol + col) * nrows + row;
 }
 
-// The forward half-spectra of a row pair from the complex transform z of x_r0 + i·x_r1 (element e of
-// z at z[at(e)]): output e of the pair's 2·mh is mode e/2 of row r0 + e%2, so that consecutive lanes
-// write the pair's adjacent values.
+// The forward half-spectra of row pairs from the complex transforms z_sq of x_(r0+2sq) + i·x_(r0+2sq+1)
+// (element e of z_sq at z[at(sq, e)]): output e is mode e/(2·nsq) of row r0 + e%(2·nsq), so that
+// consecutive lanes write the block's consecutive rows of one mode.
 template <typename At>
-__device__ __forceinline__ void write_pair_spectra(const double2 *z, At at, int n, double2 *XF, int64_t col, int64_t r0,
-                                                   int64_t nrows, int64_t ncol) {
-    const int mh = n / 2 + 1;
-    for (int e = threadIdx.x; e < 2 * mh; e += blockDim.x) {
-        const int rr = e & 1, m = e >> 1;
-        if (r0 + rr >= nrows) continue;
-        const double2 a = z[at(m)], b = cconj(z[at(m == 0 ? 0 : n - m)]);
-        // X = (a + b)/2 for row r0, Y = (a − b)/(2i) for row r0 + 1
-        XF[spec_at(m, col, r0 + rr, nrows, ncol)] = rr == 0 ? make_double2(0.5 * (a.x + b.x), 0.5 * (a.y + b.y))
-                                                            : make_double2(0.5 * (a.y - b.y), -0.5 * (a.x - b.x));
+__device__ __forceinline__ void write_pair_spectra(const double2 *z, At at, int n, int nsq, double2 *XF, int64_t col,
+                                                   int64_t r0, int64_t nrows, int64_t ncol) {
+    const int mh = n / 2 + 1, rpb = 2 * nsq;
+    for (int e = threadIdx.x; e < rpb * mh; e += blockDim.x) {
+        const int m = e / rpb, ro = e - m * rpb, sq = ro >> 1;
+        if (r0 + ro >= nrows) continue;
+        const double2 a = z[at(sq, m)], b = cconj(z[at(sq, m == 0 ? 0 : n - m)]);
+        // X = (a + b)/2 for row r0 + 2sq, Y = (a − b)/(2i) for row r0 + 2sq + 1
+        XF[spec_at(m, col, r0 + ro, nrows, ncol)] = (ro & 1) == 0 ? make_double2(0.5 * (a.x + b.x), 0.5 * (a.y + b.y))
+                                                                  : make_double2(0.5 * (a.y - b.y), -0.5 * (a.x - b.x));
     }
 }
 
@@ -294,7 +294,7 @@ __global__ __launch_bounds__(256) void k_fft_rows(RingFft F, const double2 *__re
     const double *x1 = has1 ? x0 + n : nullptr;
     for (int i = threadIdx.x; i < n; i += blockDim.x) sm[i] = make_double2(x0[i], has1 ? x1[i] : 0.0);
     const double2 *Z = fft_lds(sm, sm + n, F, tw, false, 1);
-    write_pair_spectra(Z, [](int e) { return e; }, n, XF, col, r0, nrows, ncol);
+    write_pair_spectra(Z, [](int, int e) { return e; }, n, 1, XF, col, r0, nrows, ncol);
 }
 
 // Inverse: OUT[m][col][row] (Hermitian halves) → yout[col][row][i], normalized by 1/N.
@@ -338,62 +338,69 @@ __global__ __launch_bounds__(256) void k_ifft_rows(RingFft F, const double2 *__r
 
 __device__ __forceinline__ int lp(int e, int sh) { return e + (e >> sh); }
 
+// F.nsq sequences of F.n points at sm + sq·F.sstr are transformed together by the block's 256 threads
+// (short rows: each stage then has a wave's worth of butterflies or more).
 template <int R, int MB>
-__device__ __forceinline__ void stage_staged(double2 *sm, int n, int Ns, int sh, const double2 *__restrict__ tws,
-                                             const double2 *__restrict__ tw, bool inv) {
-    // n and the layout shift pass through an empty asm so that this radix's addresses are formed here:
+__device__ __forceinline__ void stage_staged(double2 *sm, int n, int Ns, int sh, int nsq, int sstr,
+                                             const double2 *__restrict__ tws, const double2 *__restrict__ tw, bool inv) {
+    // the stage parameters pass through an empty asm so that this radix's addresses are formed here:
     // hoisted out of the stage loop, every radix of the switch would keep its own set of registers.
     n = __builtin_amdgcn_readfirstlane(n);
     sh = __builtin_amdgcn_readfirstlane(sh);
-    asm volatile("" : "+s"(n), "+s"(sh));
-    const int nb = n / R;
+    nsq = __builtin_amdgcn_readfirstlane(nsq);
+    asm volatile("" : "+s"(n), "+s"(sh), "+s"(nsq));
+    const int nb = n / R, tot = nb * nsq;
     double2 v[MB][R];
-    int dst[MB];
+    int dst[MB], base[MB];
 #pragma unroll
     for (int q = 0; q < MB; q++) {
-        const int j = (int)threadIdx.x + q * 256;
-        if (j < nb) {
+        const int p = (int)threadIdx.x + q * 256;
+        if (p < tot) {
+            const int sq = nsq == 1 ? 0 : p / nb;
+            const int j = p - sq * nb;
             const int k = j % Ns;
             const double2 *t = tws + k * (R - 1);
-            v[q][0] = sm[lp(j, sh)];
+            const double2 *x = sm + sq * sstr;
+            v[q][0] = x[lp(j, sh)];
             if (Ns == 1) {  // first stage: every twiddle is 1
 #pragma unroll
-                for (int r = 1; r < R; r++) v[q][r] = sm[lp(j + r * nb, sh)];
+                for (int r = 1; r < R; r++) v[q][r] = x[lp(j + r * nb, sh)];
             } else {
 #pragma unroll
                 for (int r = 1; r < R; r++) {
                     const double2 w = t[r - 1];
-                    v[q][r] = cmul(sm[lp(j + r * nb, sh)], inv ? cconj(w) : w);
+                    v[q][r] = cmul(x[lp(j + r * nb, sh)], inv ? cconj(w) : w);
                 }
             }
             dft_small<R>(v[q], tw, n, inv);
             dst[q] = (j - k) * R + k;
+            base[q] = sq * sstr;
         }
     }
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < MB; q++) {
-        const int j = (int)threadIdx.x + q * 256;
-        if (j < nb) {
+        const int p = (int)threadIdx.x + q * 256;
+        if (p < tot) {
 #pragma unroll
-            for (int r = 0; r < R; r++) sm[lp(dst[q] + r * Ns, sh)] = v[q][r];
+            for (int r = 0; r < R; r++) sm[base[q] + lp(dst[q] + r * Ns, sh)] = v[q][r];
         }
     }
     __syncthreads();
 }
 
-// One sequence of length F.n in sm (padded layout, transformed in place) with the staged factorization
-// F.srad (every stage's butterflies fit the per-thread budget the host checked).
+// F.nsq sequences of length F.n at sm + sq·F.sstr (padded layout, transformed in place) with the
+// staged factorization F.srad (every stage's butterflies fit the per-thread budget the host checked).
 __device__ void fft_staged(double2 *sm, const RingFft &F, const double2 *tw, const double2 *tws, bool inv) {
     __syncthreads();
     int Ns = 1;
     for (int st = 0; st < F.snst; st++) {
         const int R = F.srad[st];
-        const int mb = (F.n / R + 255) / 256;
+        const int mb = (F.n / R * F.nsq + 255) / 256;
         const double2 *t = tws + F.twoff[st];
         switch (R * 16 + mb) {
 #define CGEF_ST(RR, MM) \
-    case RR * 16 + MM: stage_staged<RR, MM>(sm, F.n, Ns, F.lpsh, t, tw, inv); break;
+    case RR * 16 + MM: stage_staged<RR, MM>(sm, F.n, Ns, F.lpsh, F.nsq, F.sstr, t, tw, inv); break;
             CGEF_ST(2, 1) CGEF_ST(2, 2) CGEF_ST(2, 3) CGEF_ST(2, 4) CGEF_ST(2, 5) CGEF_ST(2, 6)
             CGEF_ST(3, 1) CGEF_ST(3, 2) CGEF_ST(3, 3) CGEF_ST(3, 4)
             CGEF_ST(4, 1) CGEF_ST(4, 2) CGEF_ST(4, 3)
@@ -408,80 +415,85 @@ __device__ void fft_staged(double2 *sm, const RingFft &F, const double2 *tw, con
     }
 }
 
-// Forward, two real rows per block: rows r0, r0+1 of column `col` → XF[m][col][row], m ≤ N/2.
-// LPT·256 ≥ N: every load of the rows is issued before the first is waited for.
+// Forward, 2·F.nsq real rows per block: rows r0.. of column `col` → XF[m][col][row], m ≤ N/2.
+// LPT·256 ≥ nsq·N: every load of the rows is issued before the first is waited for.
 template <int LPT>
 __global__ __launch_bounds__(256) void k_fft_rows_staged(RingFft F, const double2 *__restrict__ tw,
                                                          const double2 *__restrict__ tws, const double *__restrict__ xin,
                                                          double2 *__restrict__ XF, int64_t nrows, int64_t ncol) {
     extern __shared__ double2 sm[];
-    const int n = F.n;
-    const int64_t ngrp = (nrows + 1) / 2;
+    const int n = F.n, nsq = F.nsq, sh = F.lpsh, sstr = F.sstr;
+    const int64_t rpb = 2 * nsq;
+    const int64_t ngrp = (nrows + rpb - 1) / rpb;
     int64_t item;
     if (!xcd_item(ngrp * ncol, item)) return;
-    const int64_t col = item / ngrp, r0 = (item % ngrp) * 2;
+    const int64_t col = item / ngrp, r0 = (item % ngrp) * rpb;
     const double *x = xin + (col * nrows + r0) * n;
-    const bool vb = r0 + 1 < nrows;
     double a_[LPT], b_[LPT];
 #pragma unroll
     for (int q = 0; q < LPT; q++) {
-        const int i = (int)threadIdx.x + q * 256;
-        a_[q] = i < n ? x[i] : 0.0;
-        b_[q] = i < n && vb ? x[n + i] : 0.0;
+        const int e = (int)threadIdx.x + q * 256;
+        const int sq = e / n, i = e - sq * n;
+        const bool in = e < nsq * n;
+        a_[q] = in && r0 + 2 * sq < nrows ? x[(int64_t)(2 * sq) * n + i] : 0.0;
+        b_[q] = in && r0 + 2 * sq + 1 < nrows ? x[(int64_t)(2 * sq + 1) * n + i] : 0.0;
     }
 #pragma unroll
     for (int q = 0; q < LPT; q++) {
-        const int i = (int)threadIdx.x + q * 256;
-        if (i < n) sm[lp(i, F.lpsh)] = make_double2(a_[q], b_[q]);
+        const int e = (int)threadIdx.x + q * 256;
+        const int sq = e / n, i = e - sq * n;
+        if (e < nsq * n) sm[sq * sstr + lp(i, sh)] = make_double2(a_[q], b_[q]);
     }
     fft_staged(sm, F, tw, tws, false);
-    const int sh = F.lpsh;
-    write_pair_spectra(sm, [sh](int e) { return lp(e, sh); }, n, XF, col, r0, nrows, ncol);
+    write_pair_spectra(sm, [sh, sstr](int sq, int e) { return sq * sstr + lp(e, sh); }, n, nsq, XF, col, r0, nrows, ncol);
 }
 
-// Inverse, two rows per block: OUT[m][col][row] (Hermitian halves) → yout[col][row][i] / N. Each
-// stored value is read once, lane per (mode, row) so that a pair's two values are adjacent; the lane
-// pair exchanges them and writes X + iY at m and its conjugate mirror at N − m. LPT·256 ≥ 2·(N/2+1).
+// Inverse, 2·F.nsq rows per block: OUT[m][col][row] (Hermitian halves) → yout[col][row][i] / N. Each
+// stored value is read once, lane per (mode, row) so that the block's rows of one mode are adjacent;
+// each lane pair exchanges its X, Y and writes X + iY at m and the conjugate mirror at N − m.
+// LPT·256 ≥ 2·nsq·(N/2 + 1).
 template <int LPT>
 __global__ __launch_bounds__(256) void k_ifft_rows_staged(RingFft F, const double2 *__restrict__ tw,
                                                           const double2 *__restrict__ tws, const double2 *__restrict__ OUT,
                                                           double *__restrict__ yout, int64_t nrows, int64_t ncol) {
     extern __shared__ double2 sm[];
-    const int n = F.n;
-    const int64_t ngrp = (nrows + 1) / 2;
+    const int n = F.n, nsq = F.nsq, sh = F.lpsh, sstr = F.sstr;
+    const int rpb = 2 * nsq;
+    const int64_t ngrp = (nrows + rpb - 1) / rpb;
     int64_t item;
     if (!xcd_item(ngrp * ncol, item)) return;
-    const int64_t col = item / ngrp, r0 = (item % ngrp) * 2;
-    const bool vb = r0 + 1 < nrows;
+    const int64_t col = item / ngrp, r0 = (item % ngrp) * rpb;
     const int mh = n / 2 + 1;
-    const int sh = F.lpsh;
     double2 v_[LPT];  // all gathers in flight together
 #pragma unroll
     for (int q = 0; q < LPT; q++) {
         const int e = (int)threadIdx.x + q * 256;
-        const int rr = e & 1, mm = e >> 1;
-        v_[q] = mm < mh && (rr == 0 || vb) ? OUT[spec_at(mm, col, r0 + rr, nrows, ncol)] : make_double2(0.0, 0.0);
+        const int m = e / rpb, ro = e - m * rpb;
+        v_[q] = m < mh && r0 + ro < nrows ? OUT[spec_at(m, col, r0 + ro, nrows, ncol)] : make_double2(0.0, 0.0);
     }
 #pragma unroll
     for (int q = 0; q < LPT; q++) {
         const int e = (int)threadIdx.x + q * 256;
-        const int rr = e & 1, mm = e >> 1;
-        // lanes 2i, 2i+1 hold X (row r0) and Y (row r0+1) of mode mm
+        const int m = e / rpb, ro = e - m * rpb;
+        // lanes with ro = 2sq, 2sq+1 hold X (row r0+2sq) and Y (row r0+2sq+1) of mode m
         const double2 o = make_double2(__shfl_xor(v_[q].x, 1), __shfl_xor(v_[q].y, 1));
-        const double2 X = rr == 0 ? v_[q] : o, Y = rr == 0 ? o : v_[q];
-        if (mm < mh) {
-            if (rr == 0) sm[lp(mm, sh)] = make_double2(X.x - Y.y, X.y + Y.x);  // X + iY
-            else if (mm > 0 && n - mm >= mh)
-                sm[lp(n - mm, sh)] = make_double2(X.x + Y.y, Y.x - X.y);  // conj(X) + i·conj(Y)
+        const bool even = (ro & 1) == 0;
+        const double2 X = even ? v_[q] : o, Y = even ? o : v_[q];
+        double2 *z = sm + (ro >> 1) * sstr;
+        if (m < mh) {
+            if (even) z[lp(m, sh)] = make_double2(X.x - Y.y, X.y + Y.x);  // X + iY
+            else if (m > 0 && n - m >= mh)
+                z[lp(n - m, sh)] = make_double2(X.x + Y.y, Y.x - X.y);  // conj(X) + i·conj(Y)
         }
     }
     fft_staged(sm, F, tw, tws, true);
     const double sc = 1.0 / n;
     double *y = yout + (col * nrows + r0) * n;
-    for (int i = threadIdx.x; i < n; i += 256) {
-        const double2 z = sm[lp(i, sh)];
-        y[i] = z.x * sc;
-        if (vb) y[n + i] = z.y * sc;
+    for (int e = threadIdx.x; e < nsq * n; e += 256) {
+        const int sq = e / n, i = e - sq * n;
+        const double2 z = sm[sq * sstr + lp(i, sh)];
+        if (r0 + 2 * sq < nrows) y[(int64_t)(2 * sq) * n + i] = z.x * sc;
+        if (r0 + 2 * sq + 1 < nrows) y[(int64_t)(2 * sq + 1) * n + i] = z.y * sc;
     }
 }
 
@@ -833,7 +845,7 @@ __global__ __launch_bounds__(256) void k_spec_epilogue(GeomArgs G, const double 
 // ------------------------------------------------------------------------------------ launches ----
 
 static size_t fft_shmem(const RingFft &F) { return (size_t)2 * F.n * sizeof(double2); }
-static size_t staged_shmem(const RingFft &F) { return (size_t)(F.n + F.n / 16 + 1) * sizeof(double2); }
+static size_t staged_shmem(const RingFft &F) { return (size_t)F.nsq * F.sstr * sizeof(double2); }
 
 static void set_shmem(const void *fn, size_t bytes) {
     if (bytes > 64 * 1024) CGEF_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
@@ -848,41 +860,55 @@ void ring_build_what(const RingPlan &R, const Run *runs_d, const int32_t *run_of
                        run_off_d, w64_d, nruns, R.what_d);
 }
 
+// The staged row transform of a launch: the plan's row pairs per block, halved while the launch would
+// have fewer than 4 blocks per CU (small grids keep their parallelism).
+static RingFft launch_rows_fft(const RingPlan &R, int64_t ncol) {
+    RingFft F = R.fft;
+    while (F.nsq > 1 && (R.args.nrows + 2 * F.nsq - 1) / (2 * F.nsq) * ncol < 4 * 256) F.nsq /= 2;
+    return F;
+}
+
 void ring_forward(const RingPlan &R, const double *xin, double2 *XF, int64_t ncol, hipStream_t s) {
-    const int64_t items = (R.args.nrows + 1) / 2 * ncol;
-    const dim3 grid((unsigned)(8 * ((items + 7) / 8)));
     if (R.fft.nseq > 0) {
-        const size_t sh = staged_shmem(R.fft);
-        if (R.fft.n <= 2048) {
-            hipLaunchKernelGGL(k_fft_rows_staged<8>, grid, dim3(256), sh, s, R.fft, R.tw_d, R.tws_d, xin, XF,
-                               R.args.nrows, ncol);
+        const RingFft F = launch_rows_fft(R, ncol);
+        const int64_t items = (R.args.nrows + 2 * F.nsq - 1) / (2 * F.nsq) * ncol;
+        const dim3 grid((unsigned)(8 * ((items + 7) / 8)));
+        const size_t sh = staged_shmem(F);
+        if (F.n <= 2048) {
+            hipLaunchKernelGGL(k_fft_rows_staged<8>, grid, dim3(256), sh, s, F, R.tw_d, R.tws_d, xin, XF, R.args.nrows,
+                               ncol);
         } else {
             set_shmem((const void *)k_fft_rows_staged<16>, sh);
-            hipLaunchKernelGGL(k_fft_rows_staged<16>, grid, dim3(256), sh, s, R.fft, R.tw_d, R.tws_d, xin, XF,
-                               R.args.nrows, ncol);
+            hipLaunchKernelGGL(k_fft_rows_staged<16>, grid, dim3(256), sh, s, F, R.tw_d, R.tws_d, xin, XF, R.args.nrows,
+                               ncol);
         }
         return;
     }
+    const int64_t items = (R.args.nrows + 1) / 2 * ncol;
+    const dim3 grid((unsigned)(8 * ((items + 7) / 8)));
     const size_t sh = fft_shmem(R.fft);
     set_shmem((const void *)k_fft_rows, sh);
     hipLaunchKernelGGL(k_fft_rows, grid, dim3(256), sh, s, R.fft, R.tw_d, xin, XF, R.args.nrows, ncol);
 }
 
 void ring_inverse(const RingPlan &R, const double2 *OUT, double *yout, int64_t ncol, hipStream_t s) {
-    const int64_t items = (R.args.nrows + 1) / 2 * ncol;
-    const dim3 grid((unsigned)(8 * ((items + 7) / 8)));
     if (R.fft.nseq > 0) {
-        const size_t sh = staged_shmem(R.fft);
-        if (R.fft.n <= 2048) {
-            hipLaunchKernelGGL(k_ifft_rows_staged<9>, grid, dim3(256), sh, s, R.fft, R.tw_d, R.tws_d, OUT, yout,
+        const RingFft F = launch_rows_fft(R, ncol);
+        const int64_t items = (R.args.nrows + 2 * F.nsq - 1) / (2 * F.nsq) * ncol;
+        const dim3 grid((unsigned)(8 * ((items + 7) / 8)));
+        const size_t sh = staged_shmem(F);
+        if (F.n <= 2048) {
+            hipLaunchKernelGGL(k_ifft_rows_staged<9>, grid, dim3(256), sh, s, F, R.tw_d, R.tws_d, OUT, yout,
                                R.args.nrows, ncol);
         } else {
             set_shmem((const void *)k_ifft_rows_staged<17>, sh);
-            hipLaunchKernelGGL(k_ifft_rows_staged<17>, grid, dim3(256), sh, s, R.fft, R.tw_d, R.tws_d, OUT, yout,
+            hipLaunchKernelGGL(k_ifft_rows_staged<17>, grid, dim3(256), sh, s, F, R.tw_d, R.tws_d, OUT, yout,
                                R.args.nrows, ncol);
         }
         return;
     }
+    const int64_t items = (R.args.nrows + 1) / 2 * ncol;
+    const dim3 grid((unsigned)(8 * ((items + 7) / 8)));
     const size_t sh = fft_shmem(R.fft);
     set_shmem((const void *)k_ifft_rows, sh);
     hipLaunchKernelGGL(k_ifft_rows, grid, dim3(256), sh, s, R.fft, R.tw_d, OUT, yout, R.args.nrows, ncol);

@@ -91,9 +91,27 @@ bool ring_staged_factor(RingFft &F) {
         F.snst = cnt;
         for (int st = 0; st < cnt; st++) F.srad[st] = rad[st];
         F.lpsh = (rad[0] & 1) ? 31 : 4;
+        F.sstr = n + n / 16 + 1;
+        F.nsq = 1;
         return true;
     }
     return false;
+}
+
+// Row pairs per block of the staged row transforms: doubled while the block's LDS stays within 32 KiB
+// and every stage's butterflies (over all the block's sequences) fit the per-thread budget, up to 8
+// pairs — short rows then keep all 256 threads busy (a 256-point row has 64 radix-4 butterflies).
+static void ring_rows_nsq(RingFft &F) {
+    F.nsq = 1;
+    if (F.nseq == 0) return;
+    for (int q = 2; q <= 8; q *= 2) {
+        if ((size_t)q * F.sstr * sizeof(double2) > 32 * 1024) break;
+        bool ok = true;
+        for (int st = 0; st < F.snst; st++)
+            ok = ok && (F.n / F.srad[st] * q + 255) / 256 <= staged_mb_max(F.srad[st]);
+        if (!ok) break;
+        F.nsq = q;
+    }
 }
 
 // Twiddle tables of a factored length: exp(−2πik/N) and the per-stage table of the register-staged
@@ -149,6 +167,7 @@ void build_spectral(Plan &p, const GeomArgs &G, hipStream_t s) {
     ring_factor((int)g.n[0], R.fft);
     ring_factor((int)g.n[1], R.fy);
     fft_tables(R.fft, R.tw_d, R.tws_d);
+    ring_rows_nsq(R.fft);
     fft_tables(R.fy, R.twy_d, R.twsy_d);
     R.mh = g.n[0] / 2 + 1;
     RingArgs &A = R.args;
@@ -233,6 +252,7 @@ void build_ring(Plan &p, hipStream_t s) {
             run_off[2 * q + 1] = offs[{p.runs[q].dz, p.runs[q].dy}];
         }
     fft_tables(R.fft, R.tw_d, R.tws_d);
+    ring_rows_nsq(R.fft);
     R.off_dy_d = up(off_dy);
     R.dz_val_d = up(dz_val);
     R.dz_off0_d = up(dz_off0);
