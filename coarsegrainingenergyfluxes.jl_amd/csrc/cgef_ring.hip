@@ -9,9 +9,11 @@
 // at O(N log N + N·rows·band) instead of O(N·taps). Everything here is FP64, so the one-pass form
 // τ = F[qq] − q̄q̄ keeps ~13 significant digits even where τ is 10⁻³ of the resolved energy.
 //
-// Kernels: k_fft_rows (two real rows per complex FFT, mixed-radix Stockham in LDS), k_contract
-// (source rows staged in LDS per column chunk, Ŵ read coalesced along rows), k_ifft_rows, and the
-// prep/epilogue kernels that turn velocities into the filtered fields and τ, ū.
+// Kernels: k_fft_rows_staged / k_ifft_rows_staged (two real rows per complex FFT, register-staged
+// mixed-radix Stockham with radices up to 16, several row pairs per block for short rows; the LDS
+// ping-pong k_fft_rows / k_ifft_rows for other lengths), k_contract (source rows staged in LDS per
+// column chunk, Ŵ read coalesced along rows), the prep/epilogue kernels that turn velocities into the
+// filtered fields and τ, ū, and the spectral engine's column transforms (k_spec_cols).
 #include <hip/hip_runtime.h>
 
 #include <cfloat>

@@ -329,6 +329,34 @@ def test_ring_filter_cartesian_odd_sizes_and_multiwrap(nx, ny, ell):
         assert np.max(np.abs(N(out) - ref)) <= 1e-11 * np.max(np.abs(ref))
 
 
+@pytest.mark.parametrize("symmetric", [True, False])
+@pytest.mark.parametrize("strategy", ["zerofill", "deformable"])
+def test_ring_latitude_grids(symmetric, strategy):
+    # a latitude grid symmetric about the equator and one from 80°S to 85°N (bands of no symmetry)
+    lam, phi = sphere_axes(96, 48)
+    if not symmetric:
+        phi = np.radians(np.linspace(-80.0, 85.0, 48))
+    mask = blob_mask((48, 96), 47, 0.3)
+    og = oracle_grid_sphere(lam, phi, mask=mask)
+    g = cgef.StructuredGrid(cgef.SphericalGeometry(6.371e6), lam, phi, mask=mask)
+    s_cg, s_o = STRATS[strategy]
+    plan = cgef.plan_filter(g, cgef.GaussianKernel(4.0), 1.6e6, mask_strategy=s_cg, engine="ring")
+    assert plan.engine == "ring"
+    f = smooth_random_field((2,) + g.field_shape, 48, 1)[0]
+    out = torch.empty(f.shape, dtype=torch.float64, device=DEV)
+    cgef.filter_apply_(out, T(f), plan)
+    fp = O.Footprint(og, O.GAUSSIAN, 4.0, 1.6e6)
+    got = N(out)
+    for b in range(2):
+        ref = O.apply_banded(og, fp, s_o, [f[b]])[0]
+        assert np.max(np.abs(got[b] - ref)) <= 1e-11 * np.max(np.abs(ref))
+    u, v = smooth_random_field(g.field_shape, 49, 2)
+    Pi = torch.empty(g.field_shape, dtype=torch.float64, device=DEV)
+    cgef.compute_pi_(Pi, T(u), T(v), None, g, cgef.GaussianKernel(4.0), 1.6e6, filter_plan=plan)
+    ref = O.compute_pi(og, u, v, kind=O.GAUSSIAN, alpha=4.0, ell=1.6e6, strategy=s_o, fp=fp)[0]
+    assert rel_l2(N(Pi), ref) <= 1e-10
+
+
 @pytest.mark.parametrize("strategy", ["zerofill", "deformable"])
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 def test_ring_pi_wide_footprint(strategy, dtype):
