@@ -67,6 +67,7 @@ double Grid::cell_area(int64_t i, int64_t j, int64_t k) const {
 Grid::~Grid() {
     delete internal_ws;
     dfree(mask_d);
+    dfree(scode_d);
     for (int d = 0; d < 3; d++) {
         dfree(coord_d[d]);
         dfree(width_d[d]);
@@ -143,7 +144,7 @@ static void lag3_weights(double x, double a, double b, double c, double *w) {
 
 // Stencil weights of the 3-node first derivative along one axis (Derivatives.jl:26-31 → the
 // FlowGeometries derivative with nodes = 3): which of the five stencils applies at a point depends
-// on the mask and is decided on the device; the weights depend only on the axis index.
+// on the mask (stencil_codes); the weights depend only on the axis index.
 static std::vector<double> deriv_table(const Grid &g, int axis) {
     const int64_t n = g.n[axis];
     std::vector<double> tab((size_t)n * DCOEF_SLOTS, 0.0);
@@ -185,6 +186,47 @@ static std::vector<double> deriv_table(const Grid &g, int axis) {
     return tab;
 }
 
+// Which first-derivative stencil applies at each point, per axis (Derivatives.jl:26-31 → FlowGeometries
+// derivative!, nodes = 3, ReduceInRun): centred 3-node inside a run of active cells, the window shifted
+// inward at a domain or mask edge, two nodes for a run of two, none for an isolated cell. The choice
+// depends only on the mask and the axis periodicity, so it is made once per grid: STENCIL_* kinds in
+// bits 3·axis .. 3·axis+2 of the point's code, STENCIL_ACTIVE set on active points (inactive: 0).
+static std::vector<uint16_t> stencil_codes(const Grid &g, const std::vector<uint8_t> &m) {
+    std::vector<uint16_t> code((size_t)g.npts, 0);
+    const int64_t stride[3] = {1, g.n[0], g.n[0] * g.n[1]};
+    for (int64_t k = 0; k < g.n[2]; k++)
+        for (int64_t j = 0; j < g.n[1]; j++)
+            for (int64_t i = 0; i < g.n[0]; i++) {
+                const int64_t t = i + g.n[0] * (j + g.n[1] * k);
+                if (!m[t]) continue;
+                const int64_t pos[3] = {i, j, k};
+                uint16_t c = STENCIL_ACTIVE;
+                for (int d = 0; d < g.ndim; d++) {
+                    const int64_t n = g.n[d], p = pos[d];
+                    if (n < 2) continue;
+                    const bool per = g.periodic[d] && n >= 3;
+                    auto ok = [&](int o) {  // neighbour p+o in range (or wrapped) and active
+                        int64_t q = p + o;
+                        if (q < 0 || q >= n) {
+                            if (!per) return false;
+                            q = q < 0 ? q + n : q - n;
+                        }
+                        return m[t + (q - p) * stride[d]] != 0;
+                    };
+                    const bool m1 = ok(-1), p1 = ok(1);
+                    int kind = STENCIL_NONE;
+                    if (m1 && p1) kind = STENCIL_CENTRED;
+                    else if (p1 && ok(2)) kind = STENCIL_FORWARD3;
+                    else if (m1 && ok(-2)) kind = STENCIL_BACKWARD3;
+                    else if (p1) kind = STENCIL_FORWARD2;
+                    else if (m1) kind = STENCIL_BACKWARD2;
+                    c |= (uint16_t)(kind << (3 * d));
+                }
+                code[t] = c;
+            }
+    return code;
+}
+
 GeomArgs geom_args(const Grid &g) {
     GeomArgs a{};
     for (int d = 0; d < 3; d++) {
@@ -199,6 +241,7 @@ GeomArgs geom_args(const Grid &g) {
     a.R = g.R;
     a.mode = g.geometry == CGEF_SPHERICAL ? 1 : (g.ndim == 3 ? 2 : 0);
     a.mask = g.mask_d;
+    a.scode = g.scode_d;
     a.trig = g.trig_d;
     a.active_area = g.active_area;
     return a;
@@ -527,6 +570,7 @@ cgef_status cgef_grid_create(const cgef_grid_desc *desc, int32_t device, cgef_gr
                         if (m[i + g->n[0] * (j + g->n[1] * k)]) tot += g->cell_area(i, j, k);
             g->active_area = tot;
             g->mask_d = upload(m);
+            g->scode_d = upload(stencil_codes(*g, m));
             for (int d = 0; d < 3; d++) {
                 g->coord_d[d] = upload(g->coord[d]);
                 g->width_d[d] = upload(g->width[d]);

@@ -37,6 +37,19 @@ struct Workspace;
 // 11-12 two-node backward p−1, p. Periodic axes (n ≥ 3) wrap index and coordinate.
 constexpr int DCOEF_SLOTS = 13;
 
+// First-derivative stencil kinds of a point along one axis (3 bits each in the point's stencil code):
+// none (isolated cell), centred (slots 0-2), forward and backward 3-node (3-5, 6-8), forward and
+// backward 2-node (9-10, 11-12). STENCIL_ACTIVE marks an active point.
+enum : int {
+    STENCIL_NONE = 0,
+    STENCIL_CENTRED = 1,
+    STENCIL_FORWARD3 = 2,
+    STENCIL_BACKWARD3 = 3,
+    STENCIL_FORWARD2 = 4,
+    STENCIL_BACKWARD2 = 5,
+};
+constexpr uint16_t STENCIL_ACTIVE = 0x8000;
+
 // Ring engine plan (see cgef_ring.hip): mixed-radix factorization of the row length, the (dz, dy)
 // offsets of the footprint and the Fourier weights Ŵ[m][offset][band].
 constexpr int RING_MAX_STAGES = 24;
@@ -131,6 +144,7 @@ struct Grid {
     int64_t npts = 0, nrows = 0;  // npts = Nx*Ny*Nz, nrows = Ny*Nz
     // device
     uint8_t *mask_d = nullptr;    // always allocated (all ones when unmasked)
+    uint16_t *scode_d = nullptr;  // per-point derivative stencil code (STENCIL_* per axis, STENCIL_ACTIVE)
     double *coord_d[3] = {nullptr, nullptr, nullptr};
     double *width_d[3] = {nullptr, nullptr, nullptr};
     double *trig_d = nullptr;     // sphere: [sinλ(Nx) cosλ(Nx) sinφ(Ny) cosφ(Ny)]

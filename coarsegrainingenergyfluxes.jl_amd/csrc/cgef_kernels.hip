@@ -380,38 +380,38 @@ void launch_pi_filter(const GeomArgs &G, const BandArgs &B, int dtype, bool mask
 
 // ------------------------------------------------------------ derivatives, strain, Π, energy ----
 
-// First-derivative stencil along `axis` at linear index t (axis index p, index stride `stride`):
-// centred 3-node stencil inside a run of active cells, the window shifted inward at a domain or mask
-// edge, two nodes for a run of two, none for an isolated cell (Derivatives.jl:26-31 → FlowGeometries
-// derivative!, nodes = 3, ReduceInRun). Weights come from the grid's per-index table; only the choice
-// of stencil depends on the mask. The derivative is c0·f[q0] + c1·f[q1] + c2·f[q2].
+// First-derivative stencil of the given kind (STENCIL_*, chosen per point from the mask once per
+// grid) along `axis` at linear index t (axis index p, index stride `stride`): the derivative is
+// c0·f[q0] + c1·f[q1] + c2·f[q2], weights from the grid's per-index table (Derivatives.jl:26-31 →
+// FlowGeometries derivative!, nodes = 3, ReduceInRun). A kind only leaves the axis range on a
+// periodic axis, where the index wraps; STENCIL_NONE gives a zero derivative.
 struct Stencil {
     int64_t q0, q1, q2;
     double c0, c1, c2;
 };
 
-__device__ __forceinline__ Stencil stencil_axis(const GeomArgs &G, int axis, int64_t t, int64_t p, int64_t stride) {
+__device__ __forceinline__ Stencil stencil_of(const GeomArgs &G, int axis, int kind, int64_t t, int64_t p,
+                                              int64_t stride) {
     Stencil S{t, t, t, 0.0, 0.0, 0.0};
+    if (kind == STENCIL_NONE) return S;
     const int64_t n = G.n[axis];
-    if (n < 2) return S;
-    const bool per = G.periodic[axis] && n >= 3;
-    auto nb = [&](int o, int64_t &ix) {  // neighbour p+o: in range (or wrapped) and active
+    const int o0 = kind == STENCIL_BACKWARD3 ? -2 : ((kind == STENCIL_CENTRED || kind == STENCIL_BACKWARD2) ? -1 : 0);
+    const int sb = kind == STENCIL_CENTRED ? 0 : kind == STENCIL_FORWARD3 ? 3 : kind == STENCIL_BACKWARD3 ? 6
+                 : kind == STENCIL_FORWARD2 ? 9 : 11;
+    auto at = [&](int o) {
         int64_t q = p + o;
-        if (q < 0 || q >= n) {
-            if (!per) return false;
-            q = q < 0 ? q + n : q - n;
-        }
-        ix = t + (q - p) * stride;
-        return G.mask[ix] != 0;
+        q = q < 0 ? q + n : (q >= n ? q - n : q);
+        return t + (q - p) * stride;
     };
-    int64_t im1 = t, ip1 = t, im2 = t, ip2 = t;
-    const bool okm1 = nb(-1, im1), okp1 = nb(1, ip1);
     const double *w = G.dcoef[axis] + p;  // slot s at w[s·n]
-    if (okm1 && okp1) return Stencil{im1, t, ip1, w[0], w[n], w[2 * n]};
-    if (okp1 && nb(2, ip2)) return Stencil{t, ip1, ip2, w[3 * n], w[4 * n], w[5 * n]};
-    if (okm1 && nb(-2, im2)) return Stencil{im2, im1, t, w[6 * n], w[7 * n], w[8 * n]};
-    if (okp1) return Stencil{t, ip1, t, w[9 * n], w[10 * n], 0.0};
-    if (okm1) return Stencil{im1, t, t, w[11 * n], w[12 * n], 0.0};
+    S.q0 = at(o0);
+    S.q1 = at(o0 + 1);
+    S.c0 = w[sb * n];
+    S.c1 = w[(sb + 1) * n];
+    if (kind <= STENCIL_BACKWARD3) {
+        S.q2 = at(o0 + 2);
+        S.c2 = w[(sb + 2) * n];
+    }
     return S;
 }
 
@@ -449,8 +449,62 @@ __device__ __forceinline__ int64_t xcd_block(int64_t nblk) {
 static unsigned xcd_grid(int64_t nblk) { return (unsigned)(8 * ((nblk + 7) / 8)); }
 
 // Strain S̄ (with the spherical curvature terms, Diagnostics.jl:652-669), contraction
-// (Diagnostics.jl:460-466, 683-698, 848-855), Π = −ρ₀ S̄:τ, and per-block partial energies
-// ½ |ū|² A (Diagnostics.jl:1125-1134; the radial component is excluded on the sphere).
+// (Diagnostics.jl:460-466, 683-698, 848-855) and −S̄:τ at one point t = (i, j, k) of slice b; the point's
+// kinetic energy density ½ |ū|² A (Diagnostics.jl:1125-1134; the radial component is excluded on the
+// sphere) goes to e. Inactive points give 0 for both. The loads at t (ū, τ) do not depend on the
+// point's code and go out with it; only the neighbours' indices and weights follow from the code.
+template <typename T, int MODE>
+__device__ __forceinline__ double strain_point(const GeomArgs &G, const double *__restrict__ U, const T *__restrict__ tb,
+                                               int64_t t, int64_t i, int64_t j, int64_t k, double &e) {
+    const int64_t P = G.npts, Nx = G.n[0], Ny = G.n[1];
+    const double *V = U + P, *W = U + 2 * P;
+    const unsigned code = G.scode[t];
+    const double ut = U[t], vt = V[t];
+    auto fu = [&](int64_t q) { return U[q]; };
+    auto fv = [&](int64_t q) { return V[q]; };
+    const Stencil sx = stencil_of(G, 0, code & 7, t, i, 1);
+    const Stencil sy = stencil_of(G, 1, (code >> 3) & 7, t, j, Nx);
+    double cc, v2;
+    if (MODE == 2) {
+        const double wt = W[t];
+        auto fw = [&](int64_t q) { return W[q]; };
+        const Stencil sz = stencil_of(G, 2, (code >> 6) & 7, t, k, Nx * Ny);
+        const double Sxx = apply_stencil(sx, fu), Syy = apply_stencil(sy, fv), Szz = apply_stencil(sz, fw);
+        const double Sxy = 0.5 * (apply_stencil(sy, fu) + apply_stencil(sx, fv));
+        const double Sxz = 0.5 * (apply_stencil(sz, fu) + apply_stencil(sx, fw));
+        const double Syz = 0.5 * (apply_stencil(sz, fv) + apply_stencil(sy, fw));
+        cc = Sxx * (double)tb[t] + 2.0 * Sxy * (double)tb[P + t] + Syy * (double)tb[3 * P + t] +
+             2.0 * Sxz * (double)tb[2 * P + t] + 2.0 * Syz * (double)tb[4 * P + t] + Szz * (double)tb[5 * P + t];
+        v2 = ut * ut + vt * vt + wt * wt;
+    } else {
+        double dudx = apply_stencil(sx, fu), dudy = apply_stencil(sy, fu);
+        double dvdx = apply_stencil(sx, fv), dvdy = apply_stencil(sy, fv);
+        double Sxx, Sxy, Syy;
+        if (MODE == 1) {
+            const double sp = G.trig[2 * Nx + j], cp = G.trig[2 * Nx + Ny + j];
+            const double hx = (fabs(cp) < sqrt(DBL_EPSILON)) ? 0.0 : 1.0 / (G.R * cp);
+            dudx *= hx;
+            dvdx *= hx;
+            dudy /= G.R;
+            dvdy /= G.R;
+            const double tf = fabs(cp) > 1e-12 ? sp / (G.R * cp) : 0.0;
+            Sxx = dudx - vt * tf;
+            Syy = dvdy;
+            Sxy = 0.5 * (dudy + dvdx + ut * tf);
+        } else {
+            Sxx = dudx;
+            Syy = dvdy;
+            Sxy = 0.5 * (dudy + dvdx);
+        }
+        cc = Sxx * (double)tb[t] + 2.0 * Sxy * (double)tb[P + t] + Syy * (double)tb[2 * P + t];
+        v2 = ut * ut + vt * vt;
+    }
+    const bool act = (code & STENCIL_ACTIVE) != 0;
+    e = act ? 0.5 * v2 * cell_area(G, i, j, k) : 0.0;
+    return act ? -cc : 0.0;
+}
+
+// Π = −ρ₀ S̄:τ, one point per lane, and per-block partial energies.
 template <typename T, int MODE>
 __global__ __launch_bounds__(256) void k_strain_pi(GeomArgs G, const double *__restrict__ ubar, const T *__restrict__ tau,
                                                    T *__restrict__ Pi, double rho0, double *partials, int64_t nblk,
@@ -460,55 +514,10 @@ __global__ __launch_bounds__(256) void k_strain_pi(GeomArgs G, const double *__r
     const int64_t item = xcd_block(nblk);
     if (item >= nblk) return;  // whole block: padding of the XCD-ordered launch
     const RowChunk c = row_chunk(G, nchunk, item);
-    const int64_t t = c.t, P = G.npts, Nx = G.n[0], Ny = G.n[1];
     double e = 0.0;
-    if (c.i < Nx) {
-        const double *U = ubar + b * 3 * P, *V = U + P, *W = U + 2 * P;
-        const T *tb = tau + b * 6 * P;
-        double out = 0.0;
-        if (G.mask[t]) {
-            auto fu = [&](int64_t q) { return U[q]; };
-            auto fv = [&](int64_t q) { return V[q]; };
-            const Stencil sx = stencil_axis(G, 0, t, c.i, 1);
-            const Stencil sy = stencil_axis(G, 1, t, c.j, Nx);
-            if (MODE == 2) {
-                auto fw = [&](int64_t q) { return W[q]; };
-                const Stencil sz = stencil_axis(G, 2, t, c.k, Nx * Ny);
-                const double Sxx = apply_stencil(sx, fu), Syy = apply_stencil(sy, fv), Szz = apply_stencil(sz, fw);
-                const double Sxy = 0.5 * (apply_stencil(sy, fu) + apply_stencil(sx, fv));
-                const double Sxz = 0.5 * (apply_stencil(sz, fu) + apply_stencil(sx, fw));
-                const double Syz = 0.5 * (apply_stencil(sz, fv) + apply_stencil(sy, fw));
-                const double cc = Sxx * (double)tb[t] + 2.0 * Sxy * (double)tb[P + t] + Syy * (double)tb[3 * P + t] +
-                                  2.0 * Sxz * (double)tb[2 * P + t] + 2.0 * Syz * (double)tb[4 * P + t] +
-                                  Szz * (double)tb[5 * P + t];
-                out = -cc;
-                e = 0.5 * (U[t] * U[t] + V[t] * V[t] + W[t] * W[t]) * cell_area(G, c.i, c.j, c.k);
-            } else {
-                double dudx = apply_stencil(sx, fu), dudy = apply_stencil(sy, fu);
-                double dvdx = apply_stencil(sx, fv), dvdy = apply_stencil(sy, fv);
-                double Sxx, Sxy, Syy;
-                if (MODE == 1) {
-                    const double sp = G.trig[2 * Nx + c.j], cp = G.trig[2 * Nx + Ny + c.j];
-                    const double hx = (fabs(cp) < sqrt(DBL_EPSILON)) ? 0.0 : 1.0 / (G.R * cp);
-                    dudx *= hx;
-                    dvdx *= hx;
-                    dudy /= G.R;
-                    dvdy /= G.R;
-                    const double tf = fabs(cp) > 1e-12 ? sp / (G.R * cp) : 0.0;
-                    Sxx = dudx - V[t] * tf;
-                    Syy = dvdy;
-                    Sxy = 0.5 * (dudy + dvdx + U[t] * tf);
-                } else {
-                    Sxx = dudx;
-                    Syy = dvdy;
-                    Sxy = 0.5 * (dudy + dvdx);
-                }
-                const double cc = Sxx * (double)tb[t] + 2.0 * Sxy * (double)tb[P + t] + Syy * (double)tb[2 * P + t];
-                out = -cc;
-                e = 0.5 * (U[t] * U[t] + V[t] * V[t]) * cell_area(G, c.i, c.j, c.k);
-            }
-        }
-        Pi[b * pi_stride + t] = (T)(rho0 * out);
+    if (c.i < G.n[0]) {
+        const double out = strain_point<T, MODE>(G, ubar + b * 3 * G.npts, tau + b * 6 * G.npts, c.t, c.i, c.j, c.k, e);
+        Pi[b * pi_stride + c.t] = (T)(rho0 * out);
     }
     if (partials) {
         const double s = block_sum<256>(e, sh);
@@ -598,11 +607,13 @@ __global__ __launch_bounds__(256) void k_derivative(GeomArgs G, int axis, const 
     const int64_t P = G.npts, t = c.t, Nx = G.n[0];
     if (c.i >= Nx) return;
     double v = 0.0;
-    if (G.mask[t]) {
+    const unsigned code = G.scode[t];
+    if (code & STENCIL_ACTIVE) {
         const T *f = in + b * P;
         const int64_t p = axis == 0 ? c.i : (axis == 1 ? c.j : c.k);
         const int64_t stride = axis == 0 ? 1 : (axis == 1 ? Nx : Nx * G.n[1]);
-        v = apply_stencil(stencil_axis(G, axis, t, p, stride), [&](int64_t q) { return (double)f[q]; });
+        v = apply_stencil(stencil_of(G, axis, (code >> (3 * axis)) & 7, t, p, stride),
+                          [&](int64_t q) { return (double)f[q]; });
         if (G.mode == 1) {
             const double cp = G.trig[2 * Nx + G.n[1] + c.j];
             if (axis == 0) v = (fabs(cp) < sqrt(DBL_EPSILON)) ? 0.0 : v / (G.R * cp);

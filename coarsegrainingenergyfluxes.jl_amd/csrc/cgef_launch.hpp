@@ -20,6 +20,7 @@ struct GeomArgs {
     double R;
     int32_t mode;          // 0 Cartesian 2-D, 1 spherical, 2 Cartesian 3-D
     const uint8_t *mask;   // one byte per cell, never null
+    const uint16_t *scode; // derivative stencil code per cell (STENCIL_* kind per axis, STENCIL_ACTIVE)
     const double *coord[3];
     const double *width[3];
     const double *trig;    // spherical: sinλ[Nx], cosλ[Nx], sinφ[Ny], cosφ[Ny]

@@ -645,36 +645,38 @@ __global__ __launch_bounds__(256) void k_build_what(RingFft F, const double2 *__
 // ---- prep: shifted masked components (planetary frame on the sphere), their products, the mask --
 
 // Field order per slice: q (NC), products (NP: xx xy [xz] yy [yz zz]), mask (if masked).
+// VP consecutive points per lane (2 when the plane length is even: 16-byte stores of both points).
+template <int VP>
 __global__ __launch_bounds__(256) void k_ring_prep(GeomArgs G, int dtype, const void *u, const void *v, const void *w,
                                                    int nf, int withmask, double *xin) {
     const int64_t b = blockIdx.y;
-    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t t0 = ((int64_t)blockIdx.x * 256 + threadIdx.x) * VP;
     const int64_t P = G.npts;
-    if (t >= P) return;
+    if (t0 >= P) return;  // with VP = 2, P is even, so t0 + 1 < P as well
     const int64_t Nx = G.n[0], Ny = G.n[1];
-    const int64_t i = t % Nx, j = (t / Nx) % Ny;
-    const bool act = G.mask[t] != 0;
-    double q[3] = {0.0, 0.0, 0.0};
-    if (act) frame_vel(G, dtype, u, v, w, b * P, t, i, j, q);
-    double *o = xin + b * nf * P;
-    if (G.mode == 0) {
-        o[t] = q[0];
-        o[P + t] = q[1];
-        o[2 * P + t] = q[0] * q[0];
-        o[3 * P + t] = q[0] * q[1];
-        o[4 * P + t] = q[1] * q[1];
-        if (withmask) o[5 * P + t] = act ? 1.0 : 0.0;
-    } else {
-        o[t] = q[0];
-        o[P + t] = q[1];
-        o[2 * P + t] = q[2];
-        o[3 * P + t] = q[0] * q[0];
-        o[4 * P + t] = q[0] * q[1];
-        o[5 * P + t] = q[0] * q[2];
-        o[6 * P + t] = q[1] * q[1];
-        o[7 * P + t] = q[1] * q[2];
-        o[8 * P + t] = q[2] * q[2];
-        if (withmask) o[9 * P + t] = act ? 1.0 : 0.0;
+    double q[VP][3];
+    double act[VP];
+#pragma unroll
+    for (int p = 0; p < VP; p++) {
+        const int64_t t = t0 + p;
+        const int64_t i = t % Nx, j = (t / Nx) % Ny;
+        act[p] = G.mask[t] != 0 ? 1.0 : 0.0;
+        q[p][0] = q[p][1] = q[p][2] = 0.0;
+        if (act[p] != 0.0) frame_vel(G, dtype, u, v, w, b * P, t, i, j, q[p]);
+    }
+    double *o = xin + b * nf * P + t0;
+    const int npl = G.mode == 0 ? 5 : 9;
+#pragma unroll
+    for (int f = 0; f < 9; f++) {
+        if (f >= npl) break;
+        if (VP == 2)
+            *(double2 *)(o + f * P) = make_double2(pi_plane_value(G.mode, f, q[0]), pi_plane_value(G.mode, f, q[VP - 1]));
+        else
+            o[f * P] = pi_plane_value(G.mode, f, q[0]);
+    }
+    if (withmask) {
+        if (VP == 2) *(double2 *)(o + npl * P) = make_double2(act[0], act[VP - 1]);
+        else o[npl * P] = act[0];
     }
 }
 
@@ -928,8 +930,13 @@ void ring_contract(const RingPlan &R, const double2 *XF, double2 *OUT, int64_t n
 
 void ring_prep(const GeomArgs &G, int dtype, const void *u, const void *v, const void *w, int nf, int withmask,
                double *xin, int64_t nbatch, hipStream_t s) {
-    dim3 grid((unsigned)((G.npts + 255) / 256), (unsigned)nbatch);
-    hipLaunchKernelGGL(k_ring_prep, grid, dim3(256), 0, s, G, dtype, u, v, w, nf, withmask, xin);
+    if (G.npts % 2 == 0) {
+        dim3 grid((unsigned)((G.npts / 2 + 255) / 256), (unsigned)nbatch);
+        hipLaunchKernelGGL(k_ring_prep<2>, grid, dim3(256), 0, s, G, dtype, u, v, w, nf, withmask, xin);
+    } else {
+        dim3 grid((unsigned)((G.npts + 255) / 256), (unsigned)nbatch);
+        hipLaunchKernelGGL(k_ring_prep<1>, grid, dim3(256), 0, s, G, dtype, u, v, w, nf, withmask, xin);
+    }
 }
 
 void ring_epilogue(const GeomArgs &G, int dtype, const double *yout, const double *D_row, int nf, int withmask,
