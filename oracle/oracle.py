@@ -202,7 +202,15 @@ def _arr(fields):
     return fs, (C.POINTER(C.c_double) * len(fs))(*[_dp(f) for f in fs])
 
 
+def _planar_only(grid, what):
+    """The banded, separable, prefix-sum and transform restatements walk one (x, y) plane; a 3-D grid
+    goes through apply_nd3 (Diagnostics.jl:792-857's 3-D path)."""
+    if grid.ndim != 2:
+        raise ValueError(f"the oracle's {what} engine is 2-D only; use ENGINE_ND3 on a 3-D grid")
+
+
 def apply_banded(grid, fp, strategy, fields, nthreads=0):
+    _planar_only(grid, "banded")
     ins, pin = _arr(fields)
     outs = [np.empty_like(f) for f in ins]
     pout = (C.POINTER(C.c_double) * len(outs))(*[_dp(o) for o in outs])
@@ -211,6 +219,7 @@ def apply_banded(grid, fp, strategy, fields, nthreads=0):
 
 
 def apply_separable(grid, alpha, ell, strategy, fields):
+    _planar_only(grid, "separable")
     ins, pin = _arr(fields)
     outs = [np.empty_like(f) for f in ins]
     pout = (C.POINTER(C.c_double) * len(outs))(*[_dp(o) for o in outs])
@@ -228,6 +237,7 @@ def apply_nd3(grid, kind, alpha, ell, strategy, fields, nthreads=0):
 
 def apply_prefix(grid, ell, strategy, fields, nthreads=0):
     """Filtering.jl:751-1097: the reference's top-hat engine on 2-D rectilinear grids."""
+    _planar_only(grid, "prefix-sum")
     ins, pin = _arr(fields)
     outs = [np.empty_like(f) for f in ins]
     pout = (C.POINTER(C.c_double) * len(outs))(*[_dp(o) for o in outs])
@@ -237,6 +247,7 @@ def apply_prefix(grid, ell, strategy, fields, nthreads=0):
 
 def apply_spectral(grid, kind, alpha, ell, strategy, fields):
     """FFTWExt.jl:121-208: transform filter on a periodic uniform 2-D Cartesian grid."""
+    _planar_only(grid, "transform")
     ins, pin = _arr(fields)
     outs = [np.empty_like(f) for f in ins]
     pout = (C.POINTER(C.c_double) * len(outs))(*[_dp(o) for o in outs])
@@ -284,6 +295,9 @@ def compute_pi(grid, u, v, w=None, *, kind=GAUSSIAN, alpha=6.0, ell, strategy=ZE
     u = np.ascontiguousarray(u, dtype=np.float64)
     v = np.ascontiguousarray(v, dtype=np.float64)
     w = None if w is None else np.ascontiguousarray(w, dtype=np.float64)
+    if engine != ENGINE_ND3:
+        _planar_only(grid, {ENGINE_BANDED: "banded", ENGINE_SEPARABLE: "separable", ENGINE_PREFIX: "prefix-sum",
+                            ENGINE_SPECTRAL: "transform"}.get(engine, str(engine)))
     if engine == ENGINE_BANDED and fp is None:
         fp = Footprint(grid, kind, alpha, ell)
     spec = _Spec(kind, alpha, ell, strategy, engine, nthreads)

@@ -126,6 +126,73 @@ def test_derivatives_sphere_and_cartesian():
         assert np.max(np.abs(N(out) - ref)) <= 1e-12 * np.max(np.abs(ref))
 
 
+def _run_lengths(mask, axis):
+    """Lengths of the runs of active cells along `axis` (domain ends cut runs)."""
+    m = np.moveaxis(np.asarray(mask, bool), axis, -1).reshape(-1, np.shape(mask)[axis])
+    out = []
+    for row in m:
+        n = 0
+        for a in row:
+            if a:
+                n += 1
+            elif n:
+                out.append(n)
+                n = 0
+        if n:
+            out.append(n)
+    return np.array(out)
+
+
+@pytest.mark.parametrize("periodic", [True, False])
+def test_derivatives_and_pi_speckled_mask(periodic):
+    # a speckled mask (35 % of cells masked, independently) gives every derivative stencil along both
+    # axes: centred, 3-node windows shifted at run ends, 2-node runs and isolated cells (ReduceInRun),
+    # and on periodic axes runs that cross the seam
+    rng = np.random.default_rng(5)
+    nx, ny = 36, 29
+    mask = rng.random((ny, nx)) > 0.35
+    for ax in (0, 1):
+        rl = _run_lengths(mask, 1 - ax)
+        assert (rl == 1).any() and (rl == 2).any() and (rl >= 4).any()
+    xs, ys = cart_axes(nx, 1000.0), cart_axes(ny, 1250.0)
+    og = oracle_grid_cart(xs, ys, periodic=(periodic, periodic), mask=mask)
+    g = cgef.StructuredGrid(cgef.CartesianGeometry(), xs, ys, mask=mask, periodic=(periodic, periodic))
+    f = smooth_random_field(g.field_shape, 9, 1)[0]
+    for ax, fn in ((0, cgef.ddx_), (1, cgef.ddy_)):
+        out = torch.empty(g.field_shape, dtype=torch.float64, device=DEV)
+        fn(out, T(f), g)
+        ref = O.deriv(og, ax, f)
+        assert np.max(np.abs(N(out) - ref)) <= 1e-12 * np.max(np.abs(ref))
+    u, v = smooth_random_field(g.field_shape, 10, 2)
+    ref = O.compute_pi(og, u, v, kind=O.GAUSSIAN, alpha=6.0, ell=4000.0)[0]
+    for engine in ("banded", "ring") if periodic else ("banded",):
+        plan = cgef.plan_filter(g, cgef.GaussianKernel(6.0), 4000.0, engine=engine)
+        Pi = torch.empty(g.field_shape, dtype=torch.float64, device=DEV)
+        cgef.compute_pi_(Pi, T(u), T(v), None, g, cgef.GaussianKernel(6.0), 4000.0, filter_plan=plan)
+        assert rel_l2(N(Pi), ref) <= 1e-10, engine
+
+
+def test_derivative_z_speckled_mask():
+    rng = np.random.default_rng(6)
+    xs, ys, zs = cart_axes(12, 1.0), cart_axes(10, 1.0), np.concatenate([[0.0], np.cumsum(0.8 * 1.15 ** np.arange(13))])
+    shape = (len(zs), len(ys), len(xs))
+    mask = rng.random(shape) > 0.35
+    rl = _run_lengths(mask, 0)
+    assert (rl == 1).any() and (rl == 2).any() and (rl >= 4).any()
+    og = oracle_grid_cart(xs, ys, zs, periodic=(True, True, False), mask=mask)
+    g = cgef.StructuredGrid(cgef.CartesianGeometry(), xs, ys, zs, mask=mask, periodic=(True, True, False))
+    f = smooth_random_field(shape, 11, 1)[0]
+    out = torch.empty(shape, dtype=torch.float64, device=DEV)
+    cgef.ddz_(out, T(f), g)
+    ref = O.deriv(og, 2, f)
+    assert np.max(np.abs(N(out) - ref)) <= 1e-12 * np.max(np.abs(ref))
+    u, v, w = smooth_random_field(shape, 12, 3)
+    Pi = torch.empty(shape, dtype=torch.float64, device=DEV)
+    cgef.compute_pi_(Pi, T(u), T(v), T(w), g, cgef.TopHatKernel(), 2.5)
+    ref = O.compute_pi(og, u, v, w, kind=O.TOPHAT, alpha=0.0, ell=2.5, engine=O.ENGINE_ND3)[0]
+    assert rel_l2(N(Pi), ref) <= 1e-10
+
+
 # ------------------------------------------------------------------------------------ Π, 2-D --
 
 @pytest.mark.parametrize("engine", ["banded", "baseline", "separable"])

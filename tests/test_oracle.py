@@ -354,6 +354,21 @@ def test_sphere_longitude_shift_equivariance():
 
 # ------------------------------------------------------------------------ 3-D (test_nd.jl) --
 
+def test_planar_engines_refuse_a_3d_grid():
+    # the banded / separable / prefix-sum / transform restatements walk one (x, y) plane only
+    x = cart_axes(8, 1.0)
+    g3 = oracle_grid_cart(x, x, x[:4], periodic=(True, True, False))
+    f = np.zeros((4, 8, 8))
+    for call in (lambda: O.compute_pi(g3, f, f, f, kind=O.TOPHAT, alpha=0.0, ell=3.0),
+                 lambda: O.filter_fields(g3, [f], kind=O.TOPHAT, alpha=0.0, ell=3.0),
+                 lambda: O.apply_separable(g3, 6.0, 3.0, O.ZEROFILL, [f]),
+                 lambda: O.apply_prefix(g3, 3.0, O.ZEROFILL, [f]),
+                 lambda: O.apply_spectral(g3, O.TOPHAT, 0.0, 3.0, O.ZEROFILL, [f])):
+        with pytest.raises(ValueError):
+            call()
+
+
+
 def test_3d_z_invariant_reduces_to_2d():
     x = cart_axes(21, 1.0)
     z = cart_axes(4, 100.0)
