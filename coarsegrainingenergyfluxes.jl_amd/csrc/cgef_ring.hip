@@ -503,16 +503,17 @@ __global__ __launch_bounds__(256) void k_ifft_rows_staged(RingFft F, const doubl
 // × a chunk of CC columns, one target row per lane. The source rows of each dz plane are staged in
 // LDS (zero outside a non-periodic axis) and every staged value feeds CC columns' worth of FMAs per
 // weight; Ŵ is read along rows (coalesced on the sphere, broadcast on translation-invariant grids),
-// in batches whose next batch is in flight while the current one is used — the first batch of a
-// plane is issued before its staging, so the weight latency hides behind the staging loads.
+// in batches of PB whose next batch is in flight while the current one is used — the first batch of
+// a plane is issued before its staging, so the weight latency hides behind the staging loads. PB and
+// the occupancy bound MINW (waves per SIMD the register allocation must allow) trade prefetch depth
+// against resident blocks; ring_contract picks them per launch.
 #ifndef CGEF_CONTRACT_CC
 #define CGEF_CONTRACT_CC 5
 #endif
 
-template <int CC>
-__global__ __launch_bounds__(256) void k_contract(RingArgs A, const double2 *__restrict__ XF, double2 *__restrict__ OUT,
-                                                  int64_t ncol) {
-    constexpr int PB = 8;
+template <int CC, int PB, int MINW>
+__global__ __launch_bounds__(256, MINW) void k_contract(RingArgs A, const double2 *__restrict__ XF, double2 *__restrict__ OUT,
+                                                        int64_t ncol) {
     extern __shared__ double2 st[];
     const int64_t Ny = A.ny, Nz = A.nz;
     const int64_t nyb = (Ny + 255) / 256;
@@ -918,14 +919,25 @@ void ring_inverse(const RingPlan &R, const double2 *OUT, double *yout, int64_t n
     hipLaunchKernelGGL(k_ifft_rows, grid, dim3(256), sh, s, R.fft, R.tw_d, OUT, yout, R.args.nrows, ncol);
 }
 
+// Weight batch depth and occupancy bound of the contraction, from the blocks per CU its LDS slab
+// allows (160 KiB per CU), measured on MI355X: with at most 5 resident blocks the deepest prefetch
+// (PB 8, 94 VGPRs); with room for 7 and many blocks to schedule (many columns, or several dz groups
+// of few offsets each) PB 4 at 72 VGPRs; otherwise PB 6 held to 80 VGPRs for 6 waves per SIMD.
 void ring_contract(const RingPlan &R, const double2 *XF, double2 *OUT, int64_t ncol, hipStream_t s) {
     constexpr int CC = CGEF_CONTRACT_CC;
     const int span = 256 + R.args.dy_max - R.args.dy_min;
     const size_t sh = (size_t)CC * span * sizeof(double2);
-    set_shmem((const void *)k_contract<CC>, sh);
     const int64_t nyb = (R.args.ny + 255) / 256;
     const int64_t items = (ncol + CC - 1) / CC * nyb * R.args.nz * R.mh;
-    hipLaunchKernelGGL(k_contract<CC>, dim3((unsigned)(8 * ((items + 7) / 8))), dim3(256), sh, s, R.args, XF, OUT, ncol);
+    const dim3 grid((unsigned)(8 * ((items + 7) / 8)));
+    const size_t lds_blocks = (size_t)160 * 1024 / sh;
+    auto launch = [&](auto kern) {
+        set_shmem((const void *)kern, sh);
+        hipLaunchKernelGGL(kern, grid, dim3(256), sh, s, R.args, XF, OUT, ncol);
+    };
+    if (lds_blocks <= 5) launch(k_contract<CC, 8, 1>);
+    else if (lds_blocks >= 7 && (R.args.ndz > 1 || ncol > 16)) launch(k_contract<CC, 4, 1>);
+    else launch(k_contract<CC, 6, 6>);
 }
 
 void ring_prep(const GeomArgs &G, int dtype, const void *u, const void *v, const void *w, int nf, int withmask,
