@@ -53,6 +53,9 @@ constexpr uint16_t STENCIL_ACTIVE = 0x8000;
 // Ring engine plan (see cgef_ring.hip): mixed-radix factorization of the row length, the (dz, dy)
 // offsets of the footprint and the Fourier weights Ŵ[m][offset][band].
 constexpr int RING_MAX_STAGES = 24;
+// Row lengths whose staged factorization the row kernels also carry at compile time (RingFft::fixed):
+// 1440 (0.25° in longitude) as 12·12·10.
+constexpr int RING_FIXED_1440 = 1;
 
 struct RingFft {
     int n = 0;
@@ -69,6 +72,7 @@ struct RingFft {
     int twsize = 0;                    // entries of the per-stage twiddle table
     int nseq = 0;                      // 1: the register-staged FFT serves this length (one sequence per block)
                                        // (0: not supported for this length, the LDS ping-pong FFT serves)
+    int fixed = 0;                     // RING_FIXED_*: srad is that compile-time factorization (0: none)
 };
 
 struct RingArgs {

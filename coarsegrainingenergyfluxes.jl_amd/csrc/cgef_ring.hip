@@ -417,9 +417,27 @@ __device__ void fft_staged(double2 *sm, const RingFft &F, const double2 *tw, con
     }
 }
 
+// The same with the factorization fixed at compile time (RingFft::fixed, one sequence per block, one
+// butterfly per lane per stage): only these stages' registers are allocated, where the switch above
+// holds those of its largest radix (96 VGPRs against 67 for 12·12·10).
+template <int FIX>
+__device__ __forceinline__ void fft_fixed(double2 *sm, const RingFft &F, const double2 *tw, const double2 *tws, bool inv) {
+    static_assert(FIX == RING_FIXED_1440, "one fixed factorization");
+    __syncthreads();
+    stage_staged<12, 1>(sm, F.n, 1, F.lpsh, 1, F.sstr, tws + F.twoff[0], tw, inv);
+    stage_staged<12, 1>(sm, F.n, 12, F.lpsh, 1, F.sstr, tws + F.twoff[1], tw, inv);
+    stage_staged<10, 1>(sm, F.n, 144, F.lpsh, 1, F.sstr, tws + F.twoff[2], tw, inv);
+}
+
+template <int FIX>
+__device__ __forceinline__ void fft_rows_any(double2 *sm, const RingFft &F, const double2 *tw, const double2 *tws, bool inv) {
+    if constexpr (FIX == 0) fft_staged(sm, F, tw, tws, inv);
+    else fft_fixed<FIX>(sm, F, tw, tws, inv);
+}
+
 // Forward, 2·F.nsq real rows per block: rows r0.. of column `col` → XF[m][col][row], m ≤ N/2.
 // LPT·256 ≥ nsq·N: every load of the rows is issued before the first is waited for.
-template <int LPT>
+template <int LPT, int FIX>
 __global__ __launch_bounds__(256) void k_fft_rows_staged(RingFft F, const double2 *__restrict__ tw,
                                                          const double2 *__restrict__ tws, const double *__restrict__ xin,
                                                          double2 *__restrict__ XF, int64_t nrows, int64_t ncol) {
@@ -446,7 +464,7 @@ __global__ __launch_bounds__(256) void k_fft_rows_staged(RingFft F, const double
         const int sq = e / n, i = e - sq * n;
         if (e < nsq * n) sm[sq * sstr + lp(i, sh)] = make_double2(a_[q], b_[q]);
     }
-    fft_staged(sm, F, tw, tws, false);
+    fft_rows_any<FIX>(sm, F, tw, tws, false);
     write_pair_spectra(sm, [sh, sstr](int sq, int e) { return sq * sstr + lp(e, sh); }, n, nsq, XF, col, r0, nrows, ncol);
 }
 
@@ -454,7 +472,7 @@ __global__ __launch_bounds__(256) void k_fft_rows_staged(RingFft F, const double
 // stored value is read once, lane per (mode, row) so that the block's rows of one mode are adjacent;
 // each lane pair exchanges its X, Y and writes X + iY at m and the conjugate mirror at N − m.
 // LPT·256 ≥ 2·nsq·(N/2 + 1).
-template <int LPT>
+template <int LPT, int FIX>
 __global__ __launch_bounds__(256) void k_ifft_rows_staged(RingFft F, const double2 *__restrict__ tw,
                                                           const double2 *__restrict__ tws, const double2 *__restrict__ OUT,
                                                           double *__restrict__ yout, int64_t nrows, int64_t ncol) {
@@ -488,7 +506,7 @@ __global__ __launch_bounds__(256) void k_ifft_rows_staged(RingFft F, const doubl
                 z[lp(n - m, sh)] = make_double2(X.x + Y.y, Y.x - X.y);  // conj(X) + i·conj(Y)
         }
     }
-    fft_staged(sm, F, tw, tws, true);
+    fft_rows_any<FIX>(sm, F, tw, tws, true);
     const double sc = 1.0 / n;
     double *y = yout + (col * nrows + r0) * n;
     for (int e = threadIdx.x; e < nsq * n; e += 256) {
@@ -879,12 +897,15 @@ void ring_forward(const RingPlan &R, const double *xin, double2 *XF, int64_t nco
         const int64_t items = (R.args.nrows + 2 * F.nsq - 1) / (2 * F.nsq) * ncol;
         const dim3 grid((unsigned)(8 * ((items + 7) / 8)));
         const size_t sh = staged_shmem(F);
-        if (F.n <= 2048) {
-            hipLaunchKernelGGL(k_fft_rows_staged<8>, grid, dim3(256), sh, s, F, R.tw_d, R.tws_d, xin, XF, R.args.nrows,
+        if (F.fixed == RING_FIXED_1440 && F.nsq == 1) {
+            hipLaunchKernelGGL((k_fft_rows_staged<8, RING_FIXED_1440>), grid, dim3(256), sh, s, F, R.tw_d, R.tws_d, xin, XF,
+                               R.args.nrows, ncol);
+        } else if (F.n <= 2048) {
+            hipLaunchKernelGGL((k_fft_rows_staged<8, 0>), grid, dim3(256), sh, s, F, R.tw_d, R.tws_d, xin, XF, R.args.nrows,
                                ncol);
         } else {
-            set_shmem((const void *)k_fft_rows_staged<16>, sh);
-            hipLaunchKernelGGL(k_fft_rows_staged<16>, grid, dim3(256), sh, s, F, R.tw_d, R.tws_d, xin, XF, R.args.nrows,
+            set_shmem((const void *)k_fft_rows_staged<16, 0>, sh);
+            hipLaunchKernelGGL((k_fft_rows_staged<16, 0>), grid, dim3(256), sh, s, F, R.tw_d, R.tws_d, xin, XF, R.args.nrows,
                                ncol);
         }
         return;
@@ -902,12 +923,15 @@ void ring_inverse(const RingPlan &R, const double2 *OUT, double *yout, int64_t n
         const int64_t items = (R.args.nrows + 2 * F.nsq - 1) / (2 * F.nsq) * ncol;
         const dim3 grid((unsigned)(8 * ((items + 7) / 8)));
         const size_t sh = staged_shmem(F);
-        if (F.n <= 2048) {
-            hipLaunchKernelGGL(k_ifft_rows_staged<9>, grid, dim3(256), sh, s, F, R.tw_d, R.tws_d, OUT, yout,
+        if (F.fixed == RING_FIXED_1440 && F.nsq == 1) {
+            hipLaunchKernelGGL((k_ifft_rows_staged<9, RING_FIXED_1440>), grid, dim3(256), sh, s, F, R.tw_d, R.tws_d, OUT,
+                               yout, R.args.nrows, ncol);
+        } else if (F.n <= 2048) {
+            hipLaunchKernelGGL((k_ifft_rows_staged<9, 0>), grid, dim3(256), sh, s, F, R.tw_d, R.tws_d, OUT, yout,
                                R.args.nrows, ncol);
         } else {
-            set_shmem((const void *)k_ifft_rows_staged<17>, sh);
-            hipLaunchKernelGGL(k_ifft_rows_staged<17>, grid, dim3(256), sh, s, F, R.tw_d, R.tws_d, OUT, yout,
+            set_shmem((const void *)k_ifft_rows_staged<17, 0>, sh);
+            hipLaunchKernelGGL((k_ifft_rows_staged<17, 0>), grid, dim3(256), sh, s, F, R.tw_d, R.tws_d, OUT, yout,
                                R.args.nrows, ncol);
         }
         return;

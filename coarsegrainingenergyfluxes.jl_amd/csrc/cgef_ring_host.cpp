@@ -81,6 +81,17 @@ bool ring_staged_factor(RingFft &F) {
     F.snst = 0;
     F.lpsh = 31;
     if (n < 2 || n > 4096 || (size_t)(n + n / 16 + 1) * sizeof(double2) > 80 * 1024) return false;
+    F.fixed = 0;
+    if (n == 1440) {  // 12·12·10: the row kernels carry it at compile time (fewer registers than 9·16·10)
+        const int r3[3] = {12, 12, 10};
+        F.snst = 3;
+        for (int st = 0; st < 3; st++) F.srad[st] = r3[st];
+        F.lpsh = 4;
+        F.sstr = n + n / 16 + 1;
+        F.nsq = 1;
+        F.fixed = RING_FIXED_1440;
+        return true;
+    }
     int rad[RING_MAX_STAGES];
     for (int d = 1; d <= RING_MAX_STAGES; d++) {
         if (!staged_dfs(n, n, d, rad, 0)) continue;
