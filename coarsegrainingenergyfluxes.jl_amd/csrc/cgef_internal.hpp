@@ -54,8 +54,9 @@ constexpr uint16_t STENCIL_ACTIVE = 0x8000;
 // offsets of the footprint and the Fourier weights Ŵ[m][offset][band].
 constexpr int RING_MAX_STAGES = 24;
 // Row lengths whose staged factorization the row kernels also carry at compile time (RingFft::fixed):
-// 1440 (0.25° in longitude) as 12·12·10.
+// 1440 (0.25° in longitude) as 12·12·10, 256 as 4·4·4·4.
 constexpr int RING_FIXED_1440 = 1;
+constexpr int RING_FIXED_256 = 2;
 
 struct RingFft {
     int n = 0;

@@ -417,22 +417,40 @@ __device__ void fft_staged(double2 *sm, const RingFft &F, const double2 *tw, con
     }
 }
 
-// The same with the factorization fixed at compile time (RingFft::fixed, one sequence per block, one
-// butterfly per lane per stage): only these stages' registers are allocated, where the switch above
-// holds those of its largest radix (96 VGPRs against 67 for 12·12·10).
+// The same with the factorization fixed at compile time (RingFft::fixed; every stage one butterfly per
+// lane over the block's sequences): only these stages' registers are allocated, where the switch
+// above holds those of its largest radix (96 VGPRs against 67 for 12·12·10 and 30 for 4·4·4·4).
+template <int... Rs>
+struct Radices {};
 template <int FIX>
-__device__ __forceinline__ void fft_fixed(double2 *sm, const RingFft &F, const double2 *tw, const double2 *tws, bool inv) {
-    static_assert(FIX == RING_FIXED_1440, "one fixed factorization");
+struct FixedRadices;
+template <>
+struct FixedRadices<RING_FIXED_1440> {
+    using type = Radices<12, 12, 10>;
+};
+template <>
+struct FixedRadices<RING_FIXED_256> {
+    using type = Radices<4, 4, 4, 4>;
+};
+
+template <int Ns, int R, int... Rest>
+__device__ __forceinline__ void fixed_stages(double2 *sm, const RingFft &F, const double2 *tw, const double2 *tws,
+                                             bool inv, int st) {
+    stage_staged<R, 1>(sm, F.n, Ns, F.lpsh, F.nsq, F.sstr, tws + F.twoff[st], tw, inv);
+    if constexpr (sizeof...(Rest) > 0) fixed_stages<Ns * R, Rest...>(sm, F, tw, tws, inv, st + 1);
+}
+
+template <int... Rs>
+__device__ __forceinline__ void fft_fixed(Radices<Rs...>, double2 *sm, const RingFft &F, const double2 *tw,
+                                          const double2 *tws, bool inv) {
     __syncthreads();
-    stage_staged<12, 1>(sm, F.n, 1, F.lpsh, 1, F.sstr, tws + F.twoff[0], tw, inv);
-    stage_staged<12, 1>(sm, F.n, 12, F.lpsh, 1, F.sstr, tws + F.twoff[1], tw, inv);
-    stage_staged<10, 1>(sm, F.n, 144, F.lpsh, 1, F.sstr, tws + F.twoff[2], tw, inv);
+    fixed_stages<1, Rs...>(sm, F, tw, tws, inv, 0);
 }
 
 template <int FIX>
 __device__ __forceinline__ void fft_rows_any(double2 *sm, const RingFft &F, const double2 *tw, const double2 *tws, bool inv) {
     if constexpr (FIX == 0) fft_staged(sm, F, tw, tws, inv);
-    else fft_fixed<FIX>(sm, F, tw, tws, inv);
+    else fft_fixed(typename FixedRadices<FIX>::type{}, sm, F, tw, tws, inv);
 }
 
 // Forward, 2·F.nsq real rows per block: rows r0.. of column `col` → XF[m][col][row], m ≤ N/2.
@@ -885,6 +903,14 @@ void ring_build_what(const RingPlan &R, const Run *runs_d, const int32_t *run_of
 
 // The staged row transform of a launch: the plan's row pairs per block, halved while the launch would
 // have fewer than 4 blocks per CU (small grids keep their parallelism).
+// The compile-time factorization serves a launch when every stage is one butterfly per lane.
+static bool fixed_fits(const RingFft &F) {
+    if (F.fixed == 0) return false;
+    for (int st = 0; st < F.snst; st++)
+        if (F.n / F.srad[st] * F.nsq > 256) return false;
+    return true;
+}
+
 static RingFft launch_rows_fft(const RingPlan &R, int64_t ncol) {
     RingFft F = R.fft;
     while (F.nsq > 1 && (R.args.nrows + 2 * F.nsq - 1) / (2 * F.nsq) * ncol < 4 * 256) F.nsq /= 2;
@@ -897,8 +923,11 @@ void ring_forward(const RingPlan &R, const double *xin, double2 *XF, int64_t nco
         const int64_t items = (R.args.nrows + 2 * F.nsq - 1) / (2 * F.nsq) * ncol;
         const dim3 grid((unsigned)(8 * ((items + 7) / 8)));
         const size_t sh = staged_shmem(F);
-        if (F.fixed == RING_FIXED_1440 && F.nsq == 1) {
+        if (fixed_fits(F) && F.fixed == RING_FIXED_1440) {
             hipLaunchKernelGGL((k_fft_rows_staged<8, RING_FIXED_1440>), grid, dim3(256), sh, s, F, R.tw_d, R.tws_d, xin, XF,
+                               R.args.nrows, ncol);
+        } else if (fixed_fits(F) && F.fixed == RING_FIXED_256) {  // ≤ 4 row pairs: 1024 values per block
+            hipLaunchKernelGGL((k_fft_rows_staged<4, RING_FIXED_256>), grid, dim3(256), sh, s, F, R.tw_d, R.tws_d, xin, XF,
                                R.args.nrows, ncol);
         } else if (F.n <= 2048) {
             hipLaunchKernelGGL((k_fft_rows_staged<8, 0>), grid, dim3(256), sh, s, F, R.tw_d, R.tws_d, xin, XF, R.args.nrows,
@@ -923,8 +952,11 @@ void ring_inverse(const RingPlan &R, const double2 *OUT, double *yout, int64_t n
         const int64_t items = (R.args.nrows + 2 * F.nsq - 1) / (2 * F.nsq) * ncol;
         const dim3 grid((unsigned)(8 * ((items + 7) / 8)));
         const size_t sh = staged_shmem(F);
-        if (F.fixed == RING_FIXED_1440 && F.nsq == 1) {
+        if (fixed_fits(F) && F.fixed == RING_FIXED_1440) {
             hipLaunchKernelGGL((k_ifft_rows_staged<9, RING_FIXED_1440>), grid, dim3(256), sh, s, F, R.tw_d, R.tws_d, OUT,
+                               yout, R.args.nrows, ncol);
+        } else if (fixed_fits(F) && F.fixed == RING_FIXED_256) {  // ≤ 4 row pairs: 8 × 129 stored modes
+            hipLaunchKernelGGL((k_ifft_rows_staged<5, RING_FIXED_256>), grid, dim3(256), sh, s, F, R.tw_d, R.tws_d, OUT,
                                yout, R.args.nrows, ncol);
         } else if (F.n <= 2048) {
             hipLaunchKernelGGL((k_ifft_rows_staged<9, 0>), grid, dim3(256), sh, s, F, R.tw_d, R.tws_d, OUT, yout,

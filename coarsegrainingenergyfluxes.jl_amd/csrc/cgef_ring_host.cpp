@@ -82,14 +82,16 @@ bool ring_staged_factor(RingFft &F) {
     F.lpsh = 31;
     if (n < 2 || n > 4096 || (size_t)(n + n / 16 + 1) * sizeof(double2) > 80 * 1024) return false;
     F.fixed = 0;
-    if (n == 1440) {  // 12·12·10: the row kernels carry it at compile time (fewer registers than 9·16·10)
-        const int r3[3] = {12, 12, 10};
-        F.snst = 3;
-        for (int st = 0; st < 3; st++) F.srad[st] = r3[st];
-        F.lpsh = 4;
+    // lengths the row kernels also carry at compile time (fewer registers than the radix switch):
+    // 1440 as 12·12·10 rather than 9·16·10, 256 as 4·4·4·4 (the factorization found below anyway)
+    if (n == 1440 || n == 256) {
+        const int r1440[3] = {12, 12, 10}, r256[4] = {4, 4, 4, 4};
+        F.snst = n == 1440 ? 3 : 4;
+        for (int st = 0; st < F.snst; st++) F.srad[st] = n == 1440 ? r1440[st] : r256[st];
+        F.lpsh = 4;  // even first radix: padded LDS
         F.sstr = n + n / 16 + 1;
         F.nsq = 1;
-        F.fixed = RING_FIXED_1440;
+        F.fixed = n == 1440 ? RING_FIXED_1440 : RING_FIXED_256;
         return true;
     }
     int rad[RING_MAX_STAGES];
