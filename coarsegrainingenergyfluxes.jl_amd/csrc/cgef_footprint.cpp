@@ -24,14 +24,6 @@ static double kernel_weight(int kind, double alpha, double d, double ell) {
     return std::exp(-alpha * (r * r));
 }
 
-static double haversine(double R, double phi1, double dlam, double phi2) {
-    double s1 = std::sin((phi2 - phi1) / 2.0);
-    double s2 = std::sin(dlam / 2.0);
-    double a = s1 * s1 + std::cos(phi1) * std::cos(phi2) * (s2 * s2);
-    if (a > 1.0) a = 1.0;
-    return 2.0 * R * std::asin(std::sqrt(a));
-}
-
 static int64_t iceil(double v) { return (int64_t)std::ceil(v); }
 
 namespace {
@@ -47,14 +39,37 @@ struct Builder {
         band_w.assign(nb, {});
     }
 
-    // Appends the run of columns in [-lim_lo, lim_hi] whose distance passes d <= rad. dist(di) must
-    // grow with |di|; the run is found by scanning outward from di = 0 on both sides.
+    // Appends the run of columns in [-lim_lo, lim_hi] whose distance passes d <= rad: the reference's
+    // outward scan from di = 0, which stops at the first column that fails (Filtering.jl:665-671,
+    // 705-720). dist(di) grows with |di|, so a binary search on each side finds where the scan would
+    // stop; the weight pass then evaluates each column's distance once, from di = 0 outward, and still
+    // cuts the run at the first failure it meets.
     template <class Dist, class Wt>
     void add_run(int band, int dy, int dz, int64_t lim_lo, int64_t lim_hi, Dist dist, Wt weight) {
-        if (dist(0) > p.rad) return;
-        int64_t hi = 0, lo = 0;
-        while (hi + 1 <= lim_hi && dist(hi + 1) <= p.rad) hi++;
-        while (-(lo - 1) <= lim_lo && dist(lo - 1) <= p.rad) lo--;
+        const double d0 = dist(0);
+        if (d0 > p.rad) return;
+        auto last_in = [&](int64_t sgn, int64_t lim) {  // largest x ≤ lim with dist(sgn·x) ≤ rad
+            int64_t a = 0, b = lim;
+            while (a < b) {
+                const int64_t mid = a + (b - a + 1) / 2;
+                if (dist(sgn * mid) <= p.rad) a = mid;
+                else b = mid - 1;
+            }
+            return a;
+        };
+        thread_local std::vector<double> dpos, dneg;  // distances at di = +x and di = −x
+        auto side = [&](int64_t sgn, int64_t lim, std::vector<double> &dv) {
+            const int64_t cand = last_in(sgn, lim);
+            dv.assign(1, d0);
+            for (int64_t x = 1; x <= cand; x++) {
+                const double d = dist(sgn * x);
+                if (d > p.rad) break;
+                dv.push_back(d);
+            }
+            return (int64_t)dv.size() - 1;
+        };
+        const int64_t hi = side(1, lim_hi, dpos);
+        const int64_t lo = -side(-1, lim_lo, dneg);
         Run r{};
         r.dy = dy;
         r.dz = dz;
@@ -62,7 +77,8 @@ struct Builder {
         r.hi = (int32_t)hi;
         std::vector<double> &w = band_w[band];
         r.woff = (int32_t)w.size();
-        for (int64_t di = lo; di <= hi; di++) w.push_back(weight(dist(di)));
+        for (int64_t di = lo; di < 0; di++) w.push_back(weight(dneg[-di]));
+        for (int64_t di = 0; di <= hi; di++) w.push_back(weight(dpos[di]));
         band_runs[band].push_back(r);
     }
 };
@@ -116,7 +132,16 @@ void build_banded(Plan &p, bool need_fp32) {
                 if (jj < 0 || jj >= Ny) continue;
                 const double phi2 = g.coord[1][jj];
                 const double A = area[jj];
-                auto dist = [&](int64_t di) { return haversine(R, phi, (double)di * dlam, phi2); };
+                // haversine(R, phi, di·dλ, phi2) with its per-run factors taken out of the column loop
+                // (the same operations in the same order, so the same values)
+                const double s1 = std::sin((phi2 - phi) / 2.0);
+                const double s1s1 = s1 * s1, c12 = std::cos(phi) * std::cos(phi2);
+                auto dist = [&](int64_t di) {
+                    const double s2 = std::sin((double)di * dlam / 2.0);
+                    double a = s1s1 + c12 * (s2 * s2);
+                    if (a > 1.0) a = 1.0;
+                    return 2.0 * R * std::asin(std::sqrt(a));
+                };
                 auto wt = [&](double d) { return kernel_weight(p.kind, p.alpha, d, p.scale) * A; };
                 B.add_run((int)j, (int)ddj, 0, lim_lo, lim_hi, dist, wt);
             }
@@ -190,21 +215,23 @@ void build_banded(Plan &p, bool need_fp32) {
     size_t total = 0;
     for (int b = 0; b < p.nbands; b++) total += B.band_w[b].size();
     if (total >= (size_t)INT32_MAX) fail(CGEF_ERR_UNSUPPORTED, "footprint table exceeds 2^31 entries");
-    p.w64.clear();
-    p.w64.reserve(total);
+    std::vector<size_t> wbase(p.nbands + 1, 0);
     p.run_ptr.assign(p.nbands + 1, 0);
-    p.runs.clear();
     for (int b = 0; b < p.nbands; b++) {
-        p.run_ptr[b] = (int32_t)p.runs.size();
-        const int32_t base = (int32_t)p.w64.size();
-        for (Run r : B.band_runs[b]) {
-            r.woff += base;
-            p.runs.push_back(r);
-        }
-        p.w64.insert(p.w64.end(), B.band_w[b].begin(), B.band_w[b].end());
-        std::vector<double>().swap(B.band_w[b]);
+        wbase[b + 1] = wbase[b] + B.band_w[b].size();
+        p.run_ptr[b + 1] = p.run_ptr[b] + (int32_t)B.band_runs[b].size();
     }
-    p.run_ptr[p.nbands] = (int32_t)p.runs.size();
+    p.w64.resize(total);
+    p.runs.resize(p.run_ptr[p.nbands]);
+    parallel_bands(p.nbands, [&](int b) {
+        Run *dst = p.runs.data() + p.run_ptr[b];
+        for (Run r : B.band_runs[b]) {
+            r.woff += (int32_t)wbase[b];
+            *dst++ = r;
+        }
+        std::copy(B.band_w[b].begin(), B.band_w[b].end(), p.w64.begin() + (ptrdiff_t)wbase[b]);
+        std::vector<double>().swap(B.band_w[b]);
+    });
     p.nentries = (int64_t)p.w64.size();
 
     // FP32 weights scaled per band to O(1) magnitude; the FP32 path normalizes with sums of the
@@ -237,7 +264,9 @@ void build_banded(Plan &p, bool need_fp32) {
     p.D64_row.assign(nrows, 0.0);
     p.D32_row.assign(nrows, 0.0);
     std::vector<int64_t> cost(nrows, 0);
-    for (int64_t r = 0; r < nrows; r++) {
+    const int64_t rchunk = 64;  // rows per task: every row's sums keep their order
+    parallel_bands((int)((nrows + rchunk - 1) / rchunk), [&](int rb) {
+    for (int64_t r = rb * rchunk; r < std::min<int64_t>(nrows, (rb + 1) * rchunk); r++) {
         const int64_t j = r % Ny, k = r / Ny;
         int band = (g.geometry == CGEF_SPHERICAL) ? (int)j : (p.nbands == 1 ? 0 : (int)k);
         p.band_of_row[r] = band;
@@ -257,6 +286,7 @@ void build_banded(Plan &p, bool need_fp32) {
         p.D32_row[r] = d32;
         cost[r] = taps;
     }
+    });
     // heaviest rows first: the hardware dispatches workgroups in order, so this is an LPT schedule
     p.row_order.resize(nrows);
     std::iota(p.row_order.begin(), p.row_order.end(), 0);

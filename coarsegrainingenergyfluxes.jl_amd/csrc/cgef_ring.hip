@@ -705,11 +705,12 @@ __global__ __launch_bounds__(256) void k_ring_prep(GeomArgs G, int dtype, const 
     const int npl = G.mode == 0 ? 5 : 9;
 #pragma unroll
     for (int f = 0; f < 9; f++) {
-        if (f >= npl) break;
-        if (VP == 2)
-            *(double2 *)(o + f * P) = make_double2(pi_plane_value(G.mode, f, q[0]), pi_plane_value(G.mode, f, q[VP - 1]));
-        else
-            o[f * P] = pi_plane_value(G.mode, f, q[0]);
+        if (f < npl) {
+            if (VP == 2)
+                *(double2 *)(o + f * P) = make_double2(pi_plane_value(G.mode, f, q[0]), pi_plane_value(G.mode, f, q[VP - 1]));
+            else
+                o[f * P] = pi_plane_value(G.mode, f, q[0]);
+        }
     }
     if (withmask) {
         if (VP == 2) *(double2 *)(o + npl * P) = make_double2(act[0], act[VP - 1]);
