@@ -20,6 +20,8 @@ from .kernels import (GaussianKernel, TopHatKernel, check_spectrum_kernel, kerne
 from .pipeline import (CoarseGrainBatchResult, CoarseGrainResult, coarse_grain, coarse_grain_, coarse_grain_batch_,
                        coarse_grain_profile, coarse_grain_profile_, coarse_grain_slices_, slice_pipeline_costs)
 
+from . import closure  # noqa: E402  (learned τ closure: device training and inference, exact-Π fallback)
+
 ΠWorkspace = PiWorkspace
 
 __all__ = [n for n in dir() if not n.startswith("_")]

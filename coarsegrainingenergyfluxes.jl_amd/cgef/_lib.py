@@ -24,6 +24,10 @@ EXPORTED = (
     "cgef_workspace_destroy", "cgef_filter_apply", "cgef_compute_pi", "cgef_coarse_grain", "cgef_derivative",
     "cgef_energy",
     "cgef_cumulative_energy", "cgef_profile_enable", "cgef_profile_read", "cgef_spectral_density",
+    # learned closure (no reference counterpart)
+    "cgef_conv5", "cgef_conv5_wgrad_scratch", "cgef_conv5_wgrad", "cgef_conv5_pack_weights", "cgef_adam_update",
+    "cgef_mse_grad", "cgef_closure_crop", "cgef_closure_products", "cgef_closure_targets", "cgef_closure_pi",
+    "cgef_closure_gradient_model",
 )
 
 

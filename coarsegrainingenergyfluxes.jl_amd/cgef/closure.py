@@ -1,0 +1,473 @@
+"""Learned closure for the subfilter stress τ_ℓ (no counterpart in the reference, which computes τ
+only by the exact filter, Diagnostics.jl:496-527).
+
+A 4-layer 5×5 convolutional network maps the filtered-velocity gradients (∂xū, ∂yū, ∂xv̄, ∂yv̄)
+to τ = (τxx, τxy, τyy). It is trained on the device by this library's own kernels (libcgef_hip.so:
+cgef_conv5 forward and data gradient, cgef_conv5_wgrad, cgef_adam_update, cgef_mse_grad) on pairs
+that the exact filter itself produces (closure_dataset), with BF16 weights and activations and FP32
+accumulation. The exact Π stays its judge: LearnedClosure measures the network's Π against the exact
+Π and falls back to the exact pipeline when the relative L2 error exceeds a threshold.
+
+Tensors are torch device tensors; torch provides memory only. Every compute step is a HIP kernel.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import time
+
+import numpy as np
+import torch
+
+from . import _lib
+from .diagnostics import compute_pi_
+from .derivatives import ddx_, ddy_
+from .filtering import _stream, filter_apply_, plan_filter
+from .grids import StructuredGrid
+from .kernels import GaussianKernel
+
+BF16 = 2
+HALO = 2          # each 5×5 valid layer trims 2 cells per side
+
+
+def _p(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _ocb(c):
+    return 4 if c <= 4 else (16 if c <= 16 else 64)
+
+
+def _round_up(a, b):
+    return (a + b - 1) // b * b
+
+
+_sig_done = False
+
+
+def _L():
+    global _sig_done
+    L = _lib.load()
+    if not _sig_done:
+        vp, i32, i64, dbl = C.c_void_p, C.c_int32, C.c_int64, C.c_double
+        sigs = {
+            "cgef_conv5": [vp, i32, i32, i64, i64, i64, i64, vp, i64, vp, i64, i32, vp, i32, vp, vp],
+            "cgef_conv5_wgrad_scratch": [i64, i64, i64, i64, i64, C.POINTER(i64)],
+            "cgef_conv5_wgrad": [vp, vp, i32, i32, i64, i64, i64, i64, i64, vp, vp, vp, i64, vp],
+            "cgef_conv5_pack_weights": [vp, i64, i64, vp, i64, vp, i64, vp],
+            "cgef_adam_update": [vp, vp, vp, vp, i64, dbl, dbl, dbl, dbl, i64, i64, i64, vp, i64, vp, i64, vp],
+            "cgef_mse_grad": [vp, vp, i64, vp, vp, vp],
+            "cgef_closure_crop": [vp, i32, i64, i64, i64, i64, vp, i64, i64, i64, vp, vp, i32, vp],
+            "cgef_closure_products": [vp, vp, i32, i64, i64, vp, vp],
+            "cgef_closure_targets": [vp, vp, i64, i64, vp, vp, vp],
+            "cgef_closure_pi": [vp, vp, vp, i64, i64, vp, vp],
+            "cgef_closure_gradient_model": [vp, dbl, i64, i64, vp, vp],
+        }
+        for name, args in sigs.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = C.c_int
+        _sig_done = True
+    return L
+
+
+def _check_dev(t, name, dtype=None):
+    if not t.is_cuda:
+        raise _lib.ArgumentError(f"{name} must be a device tensor")
+    if not t.is_contiguous():
+        raise _lib.ArgumentError(f"{name} must be contiguous")
+    if dtype is not None and t.dtype != dtype:
+        raise _lib.ArgumentError(f"{name} must be {dtype}")
+
+
+# ------------------------------------------------------------------------------------------------
+# Layer-level entry points (thin wrappers over the C ABI)
+# ------------------------------------------------------------------------------------------------
+
+def conv5(out, x, wf, ocs, bias, *, in_gelu=False, pad=0, zgrad=None):
+    """One 5×5 layer into `out` (see cgef_conv5). x BF16 or FP32 (n, cin, h, w); wf BF16 forward
+    layout [cin][25][ocs] (or the backward-data layout for pad=4)."""
+    n, cin, h, w = x.shape
+    cout = out.shape[1]
+    ho, wo = h + 2 * pad - 4, w + 2 * pad - 4
+    if tuple(out.shape) != (n, cout, ho, wo):
+        raise _lib.DimensionMismatch(f"out is {tuple(out.shape)}, expected {(n, cout, ho, wo)}")
+    if wf.numel() != cin * 25 * ocs:
+        raise _lib.DimensionMismatch("weight layout does not match cin / ocs")
+    if zgrad is not None and tuple(zgrad.shape) != tuple(out.shape):
+        raise _lib.DimensionMismatch("zgrad must be shaped as out")
+    for t, nm in ((out, "out"), (x, "x"), (wf, "weights")):
+        _check_dev(t, nm)
+    in_dt = BF16 if x.dtype == torch.bfloat16 else _lib.F32
+    out_dt = BF16 if out.dtype == torch.bfloat16 else _lib.F32
+    _lib.check(_L().cgef_conv5(_p(x), in_dt, int(in_gelu), n, cin, h, w, _p(wf), ocs, _p(bias), cout, pad, _p(out),
+                               out_dt, _p(zgrad), _stream(x.device.index)))
+    return out
+
+
+def conv5_wgrad(dw, db, dz, a, scratch, *, in_gelu=False):
+    n, cout, ho, wo = dz.shape
+    cin = a.shape[1]
+    if tuple(a.shape) != (n, cin, ho + 4, wo + 4):
+        raise _lib.DimensionMismatch("layer input must be (n, cin, hout+4, wout+4)")
+    in_dt = BF16 if a.dtype == torch.bfloat16 else _lib.F32
+    _lib.check(_L().cgef_conv5_wgrad(_p(dz), _p(a), in_dt, int(in_gelu), n, cin, cout, ho, wo, _p(dw), _p(db),
+                                     _p(scratch), scratch.numel(), _stream(dz.device.index)))
+    return dw, db
+
+
+def wgrad_scratch_floats(n, cin, cout, ho, wo):
+    out = C.c_int64()
+    _lib.check(_L().cgef_conv5_wgrad_scratch(n, cin, cout, ho, wo, C.byref(out)))
+    return int(out.value)
+
+
+# ------------------------------------------------------------------------------------------------
+# The network
+# ------------------------------------------------------------------------------------------------
+
+class ClosureNet:
+    """conv5 → GELU → conv5 → GELU → conv5 → GELU → conv5 with `channels` = (4, 64, 64, 64, 3):
+    FP32 master weights and Adam moments, BF16 weight layouts and activations, FP32 accumulation.
+    The input must carry HALO·L cells of context on each side of the predicted window."""
+
+    def __init__(self, channels=(4, 64, 64, 64, 3), *, seed=0, device=None, lr=1e-3, betas=(0.9, 0.999),
+                 eps=1e-8):
+        self.device = torch.cuda.current_device() if device is None else device
+        self.channels = tuple(int(c) for c in channels)
+        self.L = len(self.channels) - 1
+        self.lr, self.betas, self.eps = float(lr), (float(betas[0]), float(betas[1])), float(eps)
+        self.t = 0
+        rng = np.random.Generator(np.random.PCG64(seed))
+        dev = f"cuda:{self.device}"
+        self.W, self.b, self.mW, self.vW, self.mb, self.vb = [], [], [], [], [], []
+        self.wf, self.wb, self.ocs_f, self.ocs_b = [], [], [], []
+        for l in range(self.L):
+            ci, co = self.channels[l], self.channels[l + 1]
+            gain = math.sqrt(2.0) if l < self.L - 1 else 1.0          # He init before GELU
+            w = rng.standard_normal((co, ci, 5, 5)) * (gain / math.sqrt(ci * 25))
+            self.W.append(torch.tensor(w, dtype=torch.float32, device=dev))
+            self.b.append(torch.zeros(co, dtype=torch.float32, device=dev))
+            for lst in (self.mW, self.vW):
+                lst.append(torch.zeros_like(self.W[-1]))
+            for lst in (self.mb, self.vb):
+                lst.append(torch.zeros_like(self.b[-1]))
+            of, ob = _round_up(co, _ocb(co)), _round_up(ci, _ocb(ci))
+            self.ocs_f.append(of)
+            self.ocs_b.append(ob)
+            self.wf.append(torch.zeros(ci * 25 * of, dtype=torch.bfloat16, device=dev))
+            self.wb.append(torch.zeros(co * 25 * ob, dtype=torch.bfloat16, device=dev))
+        self.pack()
+        self._shape = None
+        self.gW = [torch.zeros_like(w) for w in self.W]
+        self.gb = [torch.zeros_like(b) for b in self.b]
+        self.loss_sum = torch.zeros(1, dtype=torch.float32, device=dev)
+
+    @property
+    def halo(self):
+        return HALO * self.L
+
+    def pack(self):
+        """Write the BF16 layouts from the FP32 master weights."""
+        for l in range(self.L):
+            co, ci = self.channels[l + 1], self.channels[l]
+            _lib.check(_L().cgef_conv5_pack_weights(_p(self.W[l]), co, ci, _p(self.wf[l]), self.ocs_f[l],
+                                                    _p(self.wb[l]), self.ocs_b[l], _stream(self.device)))
+
+    def load_state(self, weights, biases):
+        for l in range(self.L):
+            self.W[l].copy_(torch.as_tensor(np.asarray(weights[l]), dtype=torch.float32))
+            self.b[l].copy_(torch.as_tensor(np.asarray(biases[l]), dtype=torch.float32))
+        self.pack()
+
+    def state(self):
+        return ([w.detach().cpu().double().numpy() for w in self.W], [b.detach().cpu().double().numpy() for b in self.b])
+
+    def _buffers(self, n, h, w):
+        if self._shape == (n, h, w):
+            return
+        dev = f"cuda:{self.device}"
+        self.Z, self.dZ = [], []
+        for l in range(self.L):
+            co = self.channels[l + 1]
+            ho, wo = h - 4 * (l + 1), w - 4 * (l + 1)
+            if ho < 1 or wo < 1:
+                raise _lib.DimensionMismatch(f"input {h}x{w} is too small for {self.L} valid 5x5 layers")
+            dt = torch.float32 if l == self.L - 1 else torch.bfloat16
+            self.Z.append(torch.empty((n, co, ho, wo), dtype=dt, device=dev))
+            self.dZ.append(torch.empty((n, co, ho, wo), dtype=torch.float32, device=dev))
+        need = 0
+        for l in range(self.L):
+            ho, wo = h - 4 * (l + 1), w - 4 * (l + 1)
+            need = max(need, wgrad_scratch_floats(n, self.channels[l], self.channels[l + 1], ho, wo))
+        self.scratch = torch.empty(need, dtype=torch.float32, device=dev)
+        self._shape = (n, h, w)
+
+    def forward(self, x):
+        """x BF16 (n, c0, h, w) → FP32 (n, cL, h − 4L, w − 4L) (a buffer reused by the next call)."""
+        _check_dev(x, "x", torch.bfloat16)
+        n, c0, h, w = x.shape
+        if c0 != self.channels[0]:
+            raise _lib.DimensionMismatch(f"x has {c0} channels, the network takes {self.channels[0]}")
+        self._buffers(n, h, w)
+        self.x = x
+        a, gel = x, False
+        for l in range(self.L):
+            conv5(self.Z[l], a, self.wf[l], self.ocs_f[l], self.b[l], in_gelu=gel, pad=0)
+            a, gel = self.Z[l], True
+        return self.Z[-1]
+
+    def backward(self, target, *, accumulate_loss=True):
+        """Gradients of mean((Y − target)²) after forward(); fills gW, gb."""
+        y = self.Z[-1]
+        if tuple(target.shape) != tuple(y.shape):
+            raise _lib.DimensionMismatch(f"target is {tuple(target.shape)}, output is {tuple(y.shape)}")
+        _check_dev(target, "target", torch.float32)
+        _lib.check(_L().cgef_mse_grad(_p(y), _p(target), y.numel(), _p(self.dZ[-1]),
+                                      _p(self.loss_sum) if accumulate_loss else None, _stream(self.device)))
+        self.backward_from(self.dZ[-1])
+
+    def backward_from(self, dy):
+        """Backpropagate dL/dY (FP32, shaped as the output) through the layers."""
+        if dy.data_ptr() != self.dZ[-1].data_ptr():
+            self.dZ[-1].copy_(dy)
+        for l in range(self.L - 1, -1, -1):
+            a, gel = (self.x, False) if l == 0 else (self.Z[l - 1], True)
+            conv5_wgrad(self.gW[l], self.gb[l], self.dZ[l], a, self.scratch, in_gelu=gel)
+            if l > 0:
+                conv5(self.dZ[l - 1], self.dZ[l], self.wb[l], self.ocs_b[l], None, pad=4, zgrad=self.Z[l - 1])
+
+    def step(self):
+        self.t += 1
+        b1, b2 = self.betas
+        for l in range(self.L):
+            co, ci = self.channels[l + 1], self.channels[l]
+            _lib.check(_L().cgef_adam_update(_p(self.W[l]), _p(self.mW[l]), _p(self.vW[l]), _p(self.gW[l]),
+                                             self.W[l].numel(), self.lr, b1, b2, self.eps, self.t, co, ci,
+                                             _p(self.wf[l]), self.ocs_f[l], _p(self.wb[l]), self.ocs_b[l],
+                                             _stream(self.device)))
+            _lib.check(_L().cgef_adam_update(_p(self.b[l]), _p(self.mb[l]), _p(self.vb[l]), _p(self.gb[l]),
+                                             co, self.lr, b1, b2, self.eps, self.t, co, 1, None, 0, None, 0,
+                                             _stream(self.device)))
+
+    def train_step(self, x, target):
+        self.forward(x)
+        self.backward(target)
+        self.step()
+
+    def flops_per_sample(self, h, w):
+        """Multiply-adds × 2 of one forward over an (h, w) input (backward: twice this)."""
+        f = 0
+        for l in range(self.L):
+            ho, wo = h - 4 * (l + 1), w - 4 * (l + 1)
+            f += 2 * ho * wo * self.channels[l] * self.channels[l + 1] * 25
+        return f
+
+
+# ------------------------------------------------------------------------------------------------
+# Data: exact (features, τ) pairs from the filter
+# ------------------------------------------------------------------------------------------------
+
+def crop_(dst, src, origins, chan_scale=None):
+    """Periodic crops of src (S, c, H, W) FP32 at origins (n, 3) int32 (img, y0, x0) into dst
+    (n, c, h, w) FP32 or BF16, each channel times chan_scale[c]."""
+    S, c, H, W = src.shape
+    n, c2, h, w = dst.shape
+    if c2 != c or tuple(origins.shape) != (n, 3):
+        raise _lib.DimensionMismatch("crop: channel count or origins shape mismatch")
+    _check_dev(src, "src", torch.float32)
+    _check_dev(origins, "origins", torch.int32)
+    dt = BF16 if dst.dtype == torch.bfloat16 else _lib.F32
+    _lib.check(_L().cgef_closure_crop(_p(src), _lib.F32, S, c, H, W, _p(origins), n, h, w, _p(chan_scale), _p(dst), dt,
+                                      _stream(src.device.index)))
+    return dst
+
+
+def closure_features(u, v, grid: StructuredGrid, plan, *, tau=True):
+    """Exact pairs for one batch of velocity slices u, v (nb, Ny, Nx) (FP32 or FP64): the filter
+    (`plan`) of u, v and, with tau=True, of u², uv, v²; the gradients of ū, v̄ by the reference's
+    stencils (ddx!/ddy!). Returns feat (nb, 4, Ny, Nx) FP32 and tau (nb, 3, Ny, Nx) FP32 or None."""
+    if u.dim() == 2:
+        u, v = u[None], v[None]
+    nb = u.shape[0]
+    npts = u.shape[-1] * u.shape[-2]
+    dev = u.device
+    if u.dtype != v.dtype or u.shape != v.shape:
+        raise _lib.ArgumentError("u and v must share shape and dtype")
+    prod = torch.empty((nb, 5, *u.shape[-2:]), dtype=torch.float64, device=dev)
+    _lib.check(_L().cgef_closure_products(_p(u.contiguous()), _p(v.contiguous()),
+                                          _lib.F32 if u.dtype == torch.float32 else _lib.F64, npts, nb, _p(prod),
+                                          _stream(dev.index)))
+    if not tau:
+        prod = prod[:, :2].contiguous()          # ū, v̄ only
+    filt = torch.empty_like(prod)
+    filter_apply_(filt, prod, plan)
+    grad = torch.empty((nb, 4, *u.shape[-2:]), dtype=torch.float64, device=dev)
+    for s in range(nb):
+        ddx_(grad[s, 0], filt[s, 0], grid)
+        ddy_(grad[s, 1], filt[s, 0], grid)
+        ddx_(grad[s, 2], filt[s, 1], grid)
+        ddy_(grad[s, 3], filt[s, 1], grid)
+    feat = torch.empty((nb, 4, *u.shape[-2:]), dtype=torch.float32, device=dev)
+    tau_out = torch.empty((nb, 3, *u.shape[-2:]), dtype=torch.float32, device=dev) if tau else None
+    _lib.check(_L().cgef_closure_targets(_p(filt) if tau else None, _p(grad), npts, nb, _p(tau_out), _p(feat),
+                                         _stream(dev.index)))
+    return feat, tau_out
+
+
+def closure_dataset(seeds, *, N=1024, L=1.0e6, alpha=6.0, ell_cells=8.0, device=None, generator=None):
+    """(feat, tau, grid, plan) for the doubly periodic N² Cartesian box: one slice per seed of the
+    random-phase k⁻³ streamfunction generator (cgef.synthetic.cartesian_turbulence), Gaussian α,
+    ℓ = ell_cells·Δx, filtered by this library's default engine for that grid."""
+    from . import synthetic as S
+    from .grids import CartesianGeometry
+    dev = torch.cuda.current_device() if device is None else device
+    gen = generator or S.cartesian_turbulence
+    _, _, xs = gen(N=N, L=L, seed=int(seeds[0]))
+    grid = StructuredGrid(CartesianGeometry(), xs, xs, periodic=(True, True))
+    dx = xs[1] - xs[0]
+    plan = plan_filter(grid, GaussianKernel(alpha), ell_cells * dx)
+    feat = torch.empty((len(seeds), 4, N, N), dtype=torch.float32, device=f"cuda:{dev}")
+    tau = torch.empty((len(seeds), 3, N, N), dtype=torch.float32, device=f"cuda:{dev}")
+    for i, s in enumerate(seeds):
+        u, v, _ = gen(N=N, L=L, seed=int(s))
+        ut = torch.tensor(u, dtype=torch.float64, device=f"cuda:{dev}")
+        vt = torch.tensor(v, dtype=torch.float64, device=f"cuda:{dev}")
+        f, t = closure_features(ut, vt, grid, plan)
+        feat[i].copy_(f[0])
+        tau[i].copy_(t[0])
+    return feat, tau, grid, plan
+
+
+def rms_per_channel(x):
+    """√mean(x²) per channel of (S, c, H, W) (a data statistic for normalization)."""
+    return torch.sqrt(torch.mean(x.double() ** 2, dim=(0, 2, 3))).float()
+
+
+# ------------------------------------------------------------------------------------------------
+# Training and evaluation
+# ------------------------------------------------------------------------------------------------
+
+def train_closure(feat, tau, *, steps=2000, batch=16, crop=64, lr=1e-3, seed=0, net=None, channels=(4, 64, 64, 64, 3),
+                  log_every=0, log=None):
+    """Adam on mean-squared error of the RMS-normalized τ over random periodic crops (batch × crop²
+    predicted cells, each with its 8-cell context). Returns (net, feat_scale, tau_scale, history),
+    feat_scale = 1/RMS of each input channel and tau_scale = RMS of each τ channel."""
+    dev = feat.device
+    S, c0, N, _ = feat.shape
+    fs = (1.0 / rms_per_channel(feat)).to(dev)
+    ts = rms_per_channel(tau).to(dev)
+    inv_ts = (1.0 / ts).to(dev)
+    if net is None:
+        net = ClosureNet(channels, seed=seed, device=dev.index, lr=lr)
+    halo = net.halo
+    rng = np.random.Generator(np.random.PCG64(seed + 1))
+    img = rng.integers(0, S, size=(steps, batch))
+    y0 = rng.integers(0, N, size=(steps, batch))
+    x0 = rng.integers(0, N, size=(steps, batch))
+    org_in = torch.tensor(np.stack([img, y0 - halo, x0 - halo], -1).astype(np.int32), device=dev)
+    org_t = torch.tensor(np.stack([img, y0, x0], -1).astype(np.int32), device=dev)
+    X = torch.empty((batch, c0, crop + 2 * halo, crop + 2 * halo), dtype=torch.bfloat16, device=dev)
+    T = torch.empty((batch, tau.shape[1], crop, crop), dtype=torch.float32, device=dev)
+    hist = []
+    net.loss_sum.zero_()
+    t0 = time.perf_counter()
+    for s in range(steps):
+        crop_(X, feat, org_in[s], fs)
+        crop_(T, tau, org_t[s], inv_ts)
+        net.train_step(X, T)
+        if log_every and ((s + 1) % log_every == 0 or s == steps - 1):
+            tot = float(net.loss_sum.item())
+            cnt = log_every if (s + 1) % log_every == 0 else (s + 1) % log_every
+            hist.append((s + 1, tot / (cnt * T.numel()), time.perf_counter() - t0))
+            net.loss_sum.zero_()
+            if log:
+                log(f"closure step {s + 1}: mse {hist[-1][1]:.4e}  ({hist[-1][2]:.1f} s)")
+    return net, fs, ts, hist
+
+
+def predict_tau(net, feat, feat_scale, *, chunk=2):
+    """The network's τ over whole periodic fields feat (S, 4, N, M), in the normalized units it was
+    trained in (pi_from_tau applies tau_scale); `chunk` slices per forward."""
+    S, c0, N, M = feat.shape
+    h = net.halo
+    dev = feat.device
+    out = torch.empty((S, net.channels[-1], N, M), dtype=torch.float32, device=dev)
+    for s0 in range(0, S, chunk):
+        n = min(chunk, S - s0)
+        org = torch.tensor([[s0 + i, -h, -h] for i in range(n)], dtype=torch.int32, device=dev)
+        X = torch.empty((n, c0, N + 2 * h, M + 2 * h), dtype=torch.bfloat16, device=dev)
+        crop_(X, feat, org, feat_scale)
+        out[s0:s0 + n].copy_(net.forward(X))
+    return out
+
+
+def pi_from_tau(feat, tau, tau_scale=None):
+    """Π = −S̄:τ per slice (FP32) from gradients (S, 4, …) and τ (S, 3, …) · tau_scale."""
+    S = feat.shape[0]
+    npts = feat.shape[-1] * feat.shape[-2]
+    Pi = torch.empty((S, *feat.shape[-2:]), dtype=torch.float32, device=feat.device)
+    _lib.check(_L().cgef_closure_pi(_p(feat), _p(tau), _p(tau_scale), npts, S, _p(Pi), _stream(feat.device.index)))
+    return Pi
+
+
+def gradient_model_tau(feat, sigma2):
+    S = feat.shape[0]
+    npts = feat.shape[-1] * feat.shape[-2]
+    tau = torch.empty((S, 3, *feat.shape[-2:]), dtype=torch.float32, device=feat.device)
+    _lib.check(_L().cgef_closure_gradient_model(_p(feat), float(sigma2), npts, S, _p(tau), _stream(feat.device.index)))
+    return tau
+
+
+def pi_errors(pred, exact):
+    """(correlation, relative L2) of a predicted Π against the exact Π, over all slices (FP64 host)."""
+    p = pred.double().cpu().numpy().ravel()
+    e = exact.double().cpu().numpy().ravel()
+    corr = float(np.corrcoef(p, e)[0, 1])
+    rel = float(np.linalg.norm(p - e) / np.linalg.norm(e))
+    return corr, rel
+
+
+class LearnedClosure:
+    """Π from the network's τ, judged against the exact Π: the first call and every
+    `check_every`-th call also compute the exact Π (compute_Π! through this library's filter) and
+    the relative L2 error; above `threshold` (default 20 %) that call returns the exact Π and the
+    closure stays on the exact path until `revalidate()`."""
+
+    def __init__(self, net, feat_scale, tau_scale, grid, plan, *, threshold=0.2, check_every=16):
+        self.net, self.fs, self.ts = net, feat_scale, tau_scale
+        self.grid, self.plan = grid, plan
+        self.threshold, self.check_every = float(threshold), int(check_every)
+        self.calls = 0
+        self.disabled = False
+        self.last_error = None
+        self.fallbacks = 0
+
+    def revalidate(self):
+        self.disabled = False
+
+    def compute_pi_(self, Pi, u, v):
+        """Π (Ny, Nx) FP32 for one slice of u, v. Returns (Pi, used_exact)."""
+        self.calls += 1
+        if self.disabled:
+            self._exact(Pi, u, v)
+            self.fallbacks += 1
+            return Pi, True
+        feat, _ = closure_features(u, v, self.grid, self.plan, tau=False)
+        tau_n = predict_tau(self.net, feat, self.fs)
+        pred = pi_from_tau(feat, tau_n, self.ts)[0]
+        if self.check_every and (self.calls - 1) % self.check_every == 0:
+            exact = torch.empty_like(pred)
+            self._exact(exact, u, v)
+            _, rel = pi_errors(pred, exact)
+            self.last_error = rel
+            if rel > self.threshold:
+                self.disabled = True
+                self.fallbacks += 1
+                Pi.copy_(exact)
+                return Pi, True
+        Pi.copy_(pred)
+        return Pi, False
+
+    def _exact(self, Pi, u, v):
+        compute_pi_(Pi, u.float(), v.float(), None, self.grid, self.plan.kernel, self.plan.scale, filter_plan=self.plan)

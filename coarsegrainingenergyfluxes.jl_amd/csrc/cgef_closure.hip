@@ -1,0 +1,635 @@
+// Learned closure for τ_ℓ: a 4-layer 5×5 convolutional network, trained and run on the device with
+// hand-written kernels (no reference counterpart: the reference computes τ only by the exact filter).
+//
+// Storage: activations and weights BF16, accumulation FP32, optimizer state and master weights FP32.
+// Tensors are (n, channels, rows, cols) with cols fastest. A layer stores its pre-activation Z in
+// BF16; the next layer applies GELU while staging Z into LDS, so the activation is never written.
+//
+//   forward        Z_l = conv5_valid(GELU(Z_{l-1}), W_l) + b_l          k_conv5, pad 0
+//   backward data  dZ_{l-1} = conv5_full(dZ_l, flip(W_l)ᵀ) · GELU'(Z_{l-1})  k_conv5, pad 4
+//   backward W     dW_l[o][i][k] = Σ_{n,p} dZ_l[n][o][p] · A_{l-1}[n][i][p+k]  k_conv5_wgrad
+//
+// Weights reach the convolution in two BF16 layouts written from the FP32 master copy:
+// forward [i][k][o] and backward-data [o][24−k][i] (output channel fastest in both, padded to a
+// multiple of the output-channel block), so each tap's weights for a block of output channels are
+// one contiguous wave-uniform read.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <string>
+
+#include "cgef_internal.hpp"
+
+namespace cgef {
+namespace {
+
+constexpr int CT = 16;       // output tile edge (16×16 outputs per 256-thread block)
+constexpr int CS = CT + 4;   // staged input tile edge (5×5 stencil halo)
+constexpr int ICC = 8;       // input channels staged per LDS pass in k_conv5
+constexpr int WG = 16;       // output / input channels per k_conv5_wgrad block (16 × 16 lanes)
+constexpr int ASTR = CS * (CS + 1) + 1;  // LDS channel stride of the wgrad input tile (odd: 16 channels on distinct banks)
+constexpr int DSTR = CT * CT + 1;        // LDS channel stride of the wgrad dZ tile
+
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(uint32_t(h) << 16); }
+
+// Round to nearest even (finite inputs).
+__device__ __forceinline__ uint16_t f2bf(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return uint16_t(u >> 16);
+}
+
+// GELU(x) = x·Φ(x) with the exact normal CDF, and its derivative Φ(x) + x·φ(x).
+__device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.7071067811865476f)); }
+__device__ __forceinline__ float gelu_grad(float x) {
+    const float cdf = 0.5f * (1.0f + erff(x * 0.7071067811865476f));
+    return cdf + x * 0.3989422804014327f * expf(-0.5f * x * x);
+}
+
+template <int T>
+__device__ __forceinline__ float load_as_float(const void *p, int64_t i) {
+    if constexpr (T == CGEF_BF16) return bf2f(static_cast<const uint16_t *>(p)[i]);
+    else if constexpr (T == CGEF_F32) return static_cast<const float *>(p)[i];
+    else return float(static_cast<const double *>(p)[i]);
+}
+
+// One 16×16 output tile × OCB output channels per block; lane = output pixel, OCB FP32 accumulators.
+// Input channels pass through LDS ICC at a time (zero outside the input: pad 4 gives the full
+// convolution used by the data gradient). OUT: 0 BF16 Z + bias, 1 FP32 + bias, 2 FP32 × GELU'(zg).
+template <int OCB, int IN_T, int IN_GELU, int OUT>
+__global__ __launch_bounds__(256) void k_conv5(const void *__restrict__ in, const uint16_t *__restrict__ w,
+                                               const float *__restrict__ bias, const uint16_t *__restrict__ zg,
+                                               void *__restrict__ out, int64_t cin, int64_t hin, int64_t win,
+                                               int64_t cout, int64_t ocs, int64_t hout, int64_t wout, int pad,
+                                               int tiles_x) {
+    constexpr int KUNROLL = OCB >= 64 ? 1 : 25;
+    __shared__ float tile[ICC][CS][CS + 1];
+    const int t = threadIdx.x, tx = t & (CT - 1), ty = t >> 4;
+    const int64_t ox0 = int64_t(blockIdx.x % tiles_x) * CT, oy0 = int64_t(blockIdx.x / tiles_x) * CT;
+    const int64_t ob = blockIdx.y, img = blockIdx.z;
+    const int64_t iy0 = oy0 - pad, ix0 = ox0 - pad;
+    float acc[OCB];
+#pragma unroll
+    for (int o = 0; o < OCB; ++o) acc[o] = 0.0f;
+
+    for (int64_t c0 = 0; c0 < cin; c0 += ICC) {
+        const int nc = int(cin - c0 < ICC ? cin - c0 : ICC);
+        __syncthreads();
+        for (int e = t; e < ICC * CS * CS; e += 256) {
+            const int c = e / (CS * CS), r = e - c * (CS * CS);
+            const int yy = r / CS, xx = r - yy * CS;
+            const int64_t iy = iy0 + yy, ix = ix0 + xx;
+            float val = 0.0f;
+            if (c < nc && iy >= 0 && iy < hin && ix >= 0 && ix < win) {
+                val = load_as_float<IN_T>(in, ((img * cin + c0 + c) * hin + iy) * win + ix);
+                if constexpr (IN_GELU) val = gelu(val);
+            }
+            tile[c][yy][xx] = val;
+        }
+        __syncthreads();
+        for (int c = 0; c < nc; ++c) {
+            const uint32_t *wc = reinterpret_cast<const uint32_t *>(w + ((c0 + c) * 25) * ocs + ob * OCB);
+            // a tap's OCB weights are wave-uniform scalar loads; with 64 of them per tap the taps are
+            // not unrolled, so the scalar registers hold one tap's weights at a time
+#pragma unroll KUNROLL
+            for (int k = 0; k < 25; ++k) {
+                const float x = tile[c][ty + k / 5][tx + k % 5];
+                const uint32_t *wk = wc + k * (ocs >> 1);
+#pragma unroll
+                for (int o = 0; o < OCB; o += 2) {
+                    const uint32_t pr = wk[o >> 1];
+                    acc[o] = fmaf(x, __uint_as_float(pr << 16), acc[o]);
+                    acc[o + 1] = fmaf(x, __uint_as_float(pr & 0xFFFF0000u), acc[o + 1]);
+                }
+            }
+        }
+    }
+
+    const int64_t oy = oy0 + ty, ox = ox0 + tx;
+    if (oy >= hout || ox >= wout) return;
+#pragma unroll
+    for (int o = 0; o < OCB; ++o) {
+        const int64_t co = ob * OCB + o;
+        if (co >= cout) break;
+        const int64_t idx = ((img * cout + co) * hout + oy) * wout + ox;
+        if constexpr (OUT == 0) {
+            static_cast<uint16_t *>(out)[idx] = f2bf(acc[o] + bias[co]);
+        } else if constexpr (OUT == 1) {
+            static_cast<float *>(out)[idx] = acc[o] + bias[co];
+        } else {
+            static_cast<float *>(out)[idx] = acc[o] * gelu_grad(bf2f(zg[idx]));
+        }
+    }
+}
+
+// Weight and bias gradients of a valid 5×5 convolution. Block = 16 output × 16 input channels (one
+// (o, i) pair per lane, its 25 taps in registers) over a range of 16×16 output tiles of one image;
+// each tile stages dZ (16 channels × 256 pixels) and the input (16 channels × 20×20) in LDS, and a
+// lane walks each output row with a 5×5 register window of the input, 6 LDS reads per 25 FMAs.
+// The block's sums go to its own slot of `part` (deterministic; k_wgrad_reduce adds the slots).
+template <int IN_T, int IN_GELU>
+__global__ __launch_bounds__(256) void k_conv5_wgrad(const float *__restrict__ dz, const void *__restrict__ in,
+                                                     float *__restrict__ part, int64_t cin, int64_t cout,
+                                                     int64_t hout, int64_t wout, int tiles_x, int ntiles,
+                                                     int tiles_per_block, int nig) {
+    __shared__ float ds[WG * DSTR];
+    __shared__ float as[WG * ASTR];
+    const int t = threadIdx.x, ol = t >> 4, il = t & 15;
+    const int64_t og = blockIdx.x / nig, ig = blockIdx.x % nig;
+    const int64_t img = blockIdx.y;
+    const int split = blockIdx.z;
+    const int64_t hin = hout + 4, win = wout + 4;
+    float acc[25];
+#pragma unroll
+    for (int k = 0; k < 25; ++k) acc[k] = 0.0f;
+    float dsum = 0.0f;
+
+    const int tile_lo = split * tiles_per_block;
+    const int tile_hi = min(ntiles, tile_lo + tiles_per_block);
+    for (int tl = tile_lo; tl < tile_hi; ++tl) {
+        const int64_t ox0 = int64_t(tl % tiles_x) * CT, oy0 = int64_t(tl / tiles_x) * CT;
+        __syncthreads();
+        for (int e = t; e < WG * CT * CT; e += 256) {
+            const int c = e >> 8, p = e & 255;
+            const int64_t co = og * WG + c, oy = oy0 + (p >> 4), ox = ox0 + (p & 15);
+            float val = 0.0f;
+            if (co < cout && oy < hout && ox < wout) val = dz[((img * cout + co) * hout + oy) * wout + ox];
+            ds[c * DSTR + p] = val;
+        }
+        for (int e = t; e < WG * CS * CS; e += 256) {
+            const int c = e / (CS * CS), r = e - c * (CS * CS);
+            const int yy = r / CS, xx = r - yy * CS;
+            const int64_t ci = ig * WG + c, iy = oy0 + yy, ix = ox0 + xx;
+            float val = 0.0f;
+            if (ci < cin && iy < hin && ix < win) {
+                val = load_as_float<IN_T>(in, ((img * cin + ci) * hin + iy) * win + ix);
+                if constexpr (IN_GELU) val = gelu(val);
+            }
+            as[c * ASTR + yy * (CS + 1) + xx] = val;
+        }
+        __syncthreads();
+        const float *dsl = ds + ol * DSTR;
+        const float *asl = as + il * ASTR;
+        for (int py = 0; py < CT; ++py) {
+            float wnd[5][5];
+#pragma unroll
+            for (int ky = 0; ky < 5; ++ky)
+#pragma unroll
+                for (int kx = 0; kx < 4; ++kx) wnd[ky][kx] = asl[(py + ky) * (CS + 1) + kx];
+#pragma unroll
+            for (int px = 0; px < CT; ++px) {
+                const float d = dsl[py * CT + px];
+                dsum += d;
+#pragma unroll
+                for (int ky = 0; ky < 5; ++ky) wnd[ky][4] = asl[(py + ky) * (CS + 1) + px + 4];
+#pragma unroll
+                for (int ky = 0; ky < 5; ++ky)
+#pragma unroll
+                    for (int kx = 0; kx < 5; ++kx) acc[ky * 5 + kx] = fmaf(d, wnd[ky][kx], acc[ky * 5 + kx]);
+#pragma unroll
+                for (int ky = 0; ky < 5; ++ky)
+#pragma unroll
+                    for (int kx = 0; kx < 4; ++kx) wnd[ky][kx] = wnd[ky][kx + 1];
+            }
+        }
+    }
+
+    // slot layout: [slot][cout·cin·25 weights | cout biases]
+    const int64_t nw = cout * cin * 25;
+    float *slot = part + (img * gridDim.z + split) * (nw + cout);
+    const int64_t co = og * WG + ol, ci = ig * WG + il;
+    if (co < cout && ci < cin) {
+#pragma unroll
+        for (int k = 0; k < 25; ++k) slot[(co * cin + ci) * 25 + k] = acc[k];
+    }
+    if (co < cout && il == 0 && ig == 0) slot[nw + co] = dsum;
+}
+
+__global__ void k_wgrad_reduce(const float *__restrict__ part, int64_t nslots, int64_t nw, int64_t cout,
+                               float *__restrict__ dw, float *__restrict__ db) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t len = nw + cout;
+    if (i >= len) return;
+    float s = 0.0f;
+    for (int64_t p = 0; p < nslots; ++p) s += part[p * len + i];
+    if (i < nw) dw[i] = s;
+    else db[i - nw] = s;
+}
+
+// BF16 copies of a master weight tensor W[o][i][k] (k = 5·ky + kx): forward [i][k][o] (stride ocs_f
+// per (i, k)), backward-data [o][24 − k][i] (stride ocs_b per (o, k)).
+__device__ __forceinline__ void write_layouts(float p, int64_t idx, int64_t cin, uint16_t *wf, int64_t ocs_f,
+                                              uint16_t *wb, int64_t ocs_b) {
+    const int64_t k = idx % 25, oi = idx / 25, i = oi % cin, o = oi / cin;
+    const uint16_t h = f2bf(p);
+    if (wf) wf[(i * 25 + k) * ocs_f + o] = h;
+    if (wb) wb[(o * 25 + (24 - k)) * ocs_b + i] = h;
+}
+
+__global__ void k_pack_weights(const float *__restrict__ p, int64_t count, int64_t cin, uint16_t *wf, int64_t ocs_f,
+                               uint16_t *wb, int64_t ocs_b) {
+    const int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx < count) write_layouts(p[idx], idx, cin, wf, ocs_f, wb, ocs_b);
+}
+
+// Adam (Kingma & Ba 2015, Algorithm 1) with bias correction; bc1 = 1 − β₁ᵗ, bc2 = 1 − β₂ᵗ.
+__global__ void k_adam(float *__restrict__ p, float *__restrict__ m, float *__restrict__ v,
+                       const float *__restrict__ g, int64_t count, float lr, float b1, float b2, float eps,
+                       float bc1, float bc2, int64_t cin, uint16_t *wf, int64_t ocs_f, uint16_t *wb,
+                       int64_t ocs_b) {
+    const int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= count) return;
+    const float gi = g[idx];
+    const float mi = b1 * m[idx] + (1.0f - b1) * gi;
+    const float vi = b2 * v[idx] + (1.0f - b2) * gi * gi;
+    m[idx] = mi;
+    v[idx] = vi;
+    const float pi = p[idx] - lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+    p[idx] = pi;
+    if (wf || wb) write_layouts(pi, idx, cin, wf, ocs_f, wb, ocs_b);
+}
+
+// Mean-squared error: dpred = 2(pred − target)/count; Σ(pred − target)² added to *loss_sum.
+__global__ __launch_bounds__(256) void k_mse(const float *__restrict__ pred, const float *__restrict__ target,
+                                             int64_t count, float inv_count, float *__restrict__ dpred,
+                                             float *__restrict__ loss_sum) {
+    __shared__ float red[4];
+    const int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    float e = 0.0f;
+    if (idx < count) {
+        e = pred[idx] - target[idx];
+        dpred[idx] = 2.0f * e * inv_count;
+    }
+    float s = e * e;
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0 && loss_sum) atomicAdd(loss_sum, red[0] + red[1] + red[2] + red[3]);
+}
+
+// Periodic crop: dst[i][c][y][x] = scale[c] · src[img_i][c][(y0_i + y) mod H][(x0_i + x) mod W].
+// An origin whose image index is outside [0, nsrc) yields zeros.
+template <int ST, int DT>
+__global__ void k_crop(const void *__restrict__ src, int64_t nsrc, int64_t c, int64_t H, int64_t W,
+                       const int32_t *__restrict__ org, int64_t h, int64_t w, const float *__restrict__ scale,
+                       void *__restrict__ dst, int64_t total) {
+    const int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= total) return;
+    const int64_t x = idx % w, y = (idx / w) % h, ch = (idx / (w * h)) % c, i = idx / (w * h * c);
+    const int64_t img = org[3 * i];
+    int64_t sy = (int64_t(org[3 * i + 1]) + y) % H, sx = (int64_t(org[3 * i + 2]) + x) % W;
+    if (sy < 0) sy += H;
+    if (sx < 0) sx += W;
+    float val = 0.0f;
+    if (img >= 0 && img < nsrc) val = load_as_float<ST>(src, ((img * c + ch) * H + sy) * W + sx);
+    if (scale) val *= scale[ch];
+    if constexpr (DT == CGEF_BF16) static_cast<uint16_t *>(dst)[idx] = f2bf(val);
+    else static_cast<float *>(dst)[idx] = val;
+}
+
+// The five planes the exact τ needs, per slice: u, v, u², u·v, v² (FP64).
+template <typename T>
+__global__ void k_products(const T *__restrict__ u, const T *__restrict__ v, int64_t npts, int64_t total,
+                           double *__restrict__ out) {
+    const int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= total) return;
+    const int64_t s = idx / npts, p = idx - s * npts;
+    const double a = double(u[idx]), b = double(v[idx]);
+    double *o = out + s * 5 * npts + p;
+    o[0] = a;
+    o[npts] = b;
+    o[2 * npts] = a * a;
+    o[3 * npts] = a * b;
+    o[4 * npts] = b * b;
+}
+
+// τ = F[q_a q_b] − q̄_a q̄_b (FP64, stored FP32; skipped when tau is null) and the four filtered-velocity
+// gradients as FP32.
+__global__ void k_targets(const double *__restrict__ filt, const double *__restrict__ grad, int64_t npts,
+                          int64_t total, float *__restrict__ tau, float *__restrict__ feat) {
+    const int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= total) return;
+    const int64_t s = idx / npts, p = idx - s * npts;
+    if (tau) {
+        const double *f = filt + s * 5 * npts + p;
+        const double ub = f[0], vb = f[npts];
+        float *tt = tau + s * 3 * npts + p;
+        tt[0] = float(f[2 * npts] - ub * ub);
+        tt[npts] = float(f[3 * npts] - ub * vb);
+        tt[2 * npts] = float(f[4 * npts] - vb * vb);
+    }
+    const double *g = grad + s * 4 * npts + p;
+    float *fo = feat + s * 4 * npts + p;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) fo[c * npts] = float(g[c * npts]);
+}
+
+// Π = −(S̄xx τxx + 2 S̄xy τxy + S̄yy τyy) from the gradients (∂xū, ∂yū, ∂xv̄, ∂yv̄) and τ·tau_scale.
+__global__ void k_closure_pi(const float *__restrict__ feat, const float *__restrict__ tau,
+                             const float *__restrict__ tau_scale, int64_t npts, int64_t total,
+                             float *__restrict__ Pi) {
+    const int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= total) return;
+    const int64_t s = idx / npts, p = idx - s * npts;
+    const float *g = feat + s * 4 * npts + p;
+    const float *tt = tau + s * 3 * npts + p;
+    const float s0 = tau_scale ? tau_scale[0] : 1.0f, s1 = tau_scale ? tau_scale[1] : 1.0f,
+                s2 = tau_scale ? tau_scale[2] : 1.0f;
+    const float ux = g[0], uy = g[npts], vx = g[2 * npts], vy = g[3 * npts];
+    Pi[idx] = -(ux * (tt[0] * s0) + (uy + vx) * (tt[npts] * s1) + vy * (tt[2 * npts] * s2));
+}
+
+// Gradient (nonlinear) model τ_ab ≈ σ² Σ_k ∂_k ū_a ∂_k ū_b, the closure without learning.
+__global__ void k_gradient_model(const float *__restrict__ feat, float sigma2, int64_t npts, int64_t total,
+                                 float *__restrict__ tau) {
+    const int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= total) return;
+    const int64_t s = idx / npts, p = idx - s * npts;
+    const float *g = feat + s * 4 * npts + p;
+    const float ux = g[0], uy = g[npts], vx = g[2 * npts], vy = g[3 * npts];
+    float *tt = tau + s * 3 * npts + p;
+    tt[0] = sigma2 * (ux * ux + uy * uy);
+    tt[npts] = sigma2 * (ux * vx + uy * vy);
+    tt[2 * npts] = sigma2 * (vx * vx + vy * vy);
+}
+
+template <class F>
+cgef_status guarded(F &&f) {
+    try {
+        f();
+        return CGEF_OK;
+    } catch (const Error &e) {
+        set_error(e.msg);
+        return e.status;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return CGEF_ERR_ARGUMENT;
+    }
+}
+
+void need(bool ok, cgef_status s, const char *msg) {
+    if (!ok) fail(s, msg);
+}
+
+unsigned blocks_for(int64_t n, int b = 256) { return unsigned((n + b - 1) / b); }
+
+void check_launch() { CGEF_HIP(hipGetLastError()); }
+
+template <int OCB, int IN_T, int IN_GELU, int OUT>
+void launch_conv(dim3 g, hipStream_t st, const void *in, const uint16_t *w, const float *bias, const uint16_t *zg,
+                 void *out, int64_t cin, int64_t hin, int64_t win, int64_t cout, int64_t ocs, int64_t hout,
+                 int64_t wout, int pad, int tiles_x) {
+    hipLaunchKernelGGL((k_conv5<OCB, IN_T, IN_GELU, OUT>), g, dim3(256), 0, st, in, w, bias, zg, out, cin, hin, win,
+                       cout, ocs, hout, wout, pad, tiles_x);
+}
+
+template <int OCB, int IN_T, int IN_GELU>
+void conv_out(int out_kind, dim3 g, hipStream_t st, const void *in, const uint16_t *w, const float *bias,
+              const uint16_t *zg, void *out, int64_t cin, int64_t hin, int64_t win, int64_t cout, int64_t ocs,
+              int64_t hout, int64_t wout, int pad, int tiles_x) {
+    if (out_kind == 0)
+        launch_conv<OCB, IN_T, IN_GELU, 0>(g, st, in, w, bias, zg, out, cin, hin, win, cout, ocs, hout, wout, pad,
+                                           tiles_x);
+    else if (out_kind == 1)
+        launch_conv<OCB, IN_T, IN_GELU, 1>(g, st, in, w, bias, zg, out, cin, hin, win, cout, ocs, hout, wout, pad,
+                                           tiles_x);
+    else
+        launch_conv<OCB, IN_T, IN_GELU, 2>(g, st, in, w, bias, zg, out, cin, hin, win, cout, ocs, hout, wout, pad,
+                                           tiles_x);
+}
+
+template <int OCB>
+void conv_in(int in_t, int in_gelu, int out_kind, dim3 g, hipStream_t st, const void *in, const uint16_t *w,
+             const float *bias, const uint16_t *zg, void *out, int64_t cin, int64_t hin, int64_t win, int64_t cout,
+             int64_t ocs, int64_t hout, int64_t wout, int pad, int tiles_x) {
+    if (in_t == CGEF_BF16 && in_gelu)
+        conv_out<OCB, CGEF_BF16, 1>(out_kind, g, st, in, w, bias, zg, out, cin, hin, win, cout, ocs, hout, wout, pad,
+                                    tiles_x);
+    else if (in_t == CGEF_BF16)
+        conv_out<OCB, CGEF_BF16, 0>(out_kind, g, st, in, w, bias, zg, out, cin, hin, win, cout, ocs, hout, wout, pad,
+                                    tiles_x);
+    else
+        conv_out<OCB, CGEF_F32, 0>(out_kind, g, st, in, w, bias, zg, out, cin, hin, win, cout, ocs, hout, wout, pad,
+                                   tiles_x);
+}
+
+// Split of the output tiles over blocks for the weight gradient: about 2048 blocks in flight.
+void wgrad_split(int64_t n, int64_t cin, int64_t cout, int64_t hout, int64_t wout, int &tiles_x, int &ntiles,
+                 int &tpb, int &nsplit, int &nig, int &nog) {
+    tiles_x = int((wout + CT - 1) / CT);
+    ntiles = tiles_x * int((hout + CT - 1) / CT);
+    nig = int((cin + WG - 1) / WG);
+    nog = int((cout + WG - 1) / WG);
+    const int64_t base = n * nig * nog;
+    int64_t want = std::max<int64_t>(1, 2048 / std::max<int64_t>(base, 1));
+    want = std::min<int64_t>(want, ntiles);
+    tpb = int((ntiles + want - 1) / want);
+    nsplit = (ntiles + tpb - 1) / tpb;
+}
+
+}  // namespace
+}  // namespace cgef
+
+using namespace cgef;
+
+extern "C" {
+
+cgef_status cgef_conv5(const void *in, int32_t in_dtype, int32_t in_gelu, int64_t n, int64_t cin, int64_t hin,
+                       int64_t win, const uint16_t *w, int64_t ocs, const float *bias, int64_t cout, int32_t pad,
+                       void *out, int32_t out_dtype, const uint16_t *zgrad, void *stream) {
+    return guarded([&] {
+        need(in && w && out, CGEF_ERR_ARGUMENT, "cgef_conv5: null pointer");
+        need(in_dtype == CGEF_BF16 || (in_dtype == CGEF_F32 && !in_gelu), CGEF_ERR_ARGUMENT,
+             "cgef_conv5: input must be BF16 (optionally GELU-applied) or FP32");
+        need(pad == 0 || pad == 4, CGEF_ERR_ARGUMENT, "cgef_conv5: pad must be 0 (valid) or 4 (full)");
+        need(n >= 1 && n <= 65535 && cin >= 1 && cout >= 1, CGEF_ERR_DIMENSION, "cgef_conv5: bad n / channels");
+        const int64_t hout = hin + 2 * pad - 4, wout = win + 2 * pad - 4;
+        need(hin >= 1 && win >= 1 && hout >= 1 && wout >= 1, CGEF_ERR_DIMENSION, "cgef_conv5: input smaller than 5x5");
+        int out_kind;
+        if (zgrad) {
+            need(out_dtype == CGEF_F32, CGEF_ERR_ARGUMENT, "cgef_conv5: GELU'-scaled output is FP32");
+            out_kind = 2;
+        } else {
+            need(bias != nullptr, CGEF_ERR_ARGUMENT, "cgef_conv5: bias required without zgrad");
+            need(out_dtype == CGEF_BF16 || out_dtype == CGEF_F32, CGEF_ERR_ARGUMENT, "cgef_conv5: output BF16 or FP32");
+            out_kind = out_dtype == CGEF_BF16 ? 0 : 1;
+        }
+        const int ocb = cout <= 4 ? 4 : (cout <= 16 ? 16 : 64);
+        need(ocs % ocb == 0 && ocs >= cout, CGEF_ERR_DIMENSION,
+             "cgef_conv5: weight layout stride must cover cout in whole output-channel blocks (4, 16 or 64)");
+        const int tiles_x = int((wout + CT - 1) / CT);
+        const int64_t ntiles = int64_t(tiles_x) * ((hout + CT - 1) / CT);
+        need(ntiles <= 0x7fffffff, CGEF_ERR_DIMENSION, "cgef_conv5: too many tiles");
+        dim3 g(unsigned(ntiles), unsigned((cout + ocb - 1) / ocb), unsigned(n));
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        if (ocb == 4)
+            conv_in<4>(in_dtype, in_gelu, out_kind, g, st, in, w, bias, zgrad, out, cin, hin, win, cout, ocs, hout, wout,
+                       pad, tiles_x);
+        else if (ocb == 16)
+            conv_in<16>(in_dtype, in_gelu, out_kind, g, st, in, w, bias, zgrad, out, cin, hin, win, cout, ocs, hout,
+                        wout, pad, tiles_x);
+        else
+            conv_in<64>(in_dtype, in_gelu, out_kind, g, st, in, w, bias, zgrad, out, cin, hin, win, cout, ocs, hout,
+                        wout, pad, tiles_x);
+        check_launch();
+    });
+}
+
+cgef_status cgef_conv5_wgrad_scratch(int64_t n, int64_t cin, int64_t cout, int64_t hout, int64_t wout,
+                                     int64_t *floats) {
+    return guarded([&] {
+        need(floats != nullptr, CGEF_ERR_ARGUMENT, "null output");
+        need(n >= 1 && cin >= 1 && cout >= 1 && hout >= 1 && wout >= 1, CGEF_ERR_DIMENSION, "bad sizes");
+        int tiles_x, ntiles, tpb, nsplit, nig, nog;
+        wgrad_split(n, cin, cout, hout, wout, tiles_x, ntiles, tpb, nsplit, nig, nog);
+        *floats = n * nsplit * (cout * cin * 25 + cout);
+    });
+}
+
+cgef_status cgef_conv5_wgrad(const float *dz, const void *in, int32_t in_dtype, int32_t in_gelu, int64_t n,
+                             int64_t cin, int64_t cout, int64_t hout, int64_t wout, float *dw, float *db,
+                             float *scratch, int64_t scratch_floats, void *stream) {
+    return guarded([&] {
+        need(dz && in && dw && db && scratch, CGEF_ERR_ARGUMENT, "cgef_conv5_wgrad: null pointer");
+        need(in_dtype == CGEF_BF16 || (in_dtype == CGEF_F32 && !in_gelu), CGEF_ERR_ARGUMENT,
+             "cgef_conv5_wgrad: input must be BF16 (optionally GELU-applied) or FP32");
+        need(n >= 1 && n <= 65535 && cin >= 1 && cout >= 1 && hout >= 1 && wout >= 1, CGEF_ERR_DIMENSION,
+             "cgef_conv5_wgrad: bad sizes");
+        int tiles_x, ntiles, tpb, nsplit, nig, nog;
+        wgrad_split(n, cin, cout, hout, wout, tiles_x, ntiles, tpb, nsplit, nig, nog);
+        const int64_t nw = cout * cin * 25, nslots = n * nsplit;
+        need(scratch_floats >= nslots * (nw + cout), CGEF_ERR_DIMENSION,
+             "cgef_conv5_wgrad: scratch smaller than cgef_conv5_wgrad_scratch");
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        dim3 g(unsigned(nog * nig), unsigned(n), unsigned(nsplit));
+        if (in_dtype == CGEF_BF16 && in_gelu)
+            hipLaunchKernelGGL((k_conv5_wgrad<CGEF_BF16, 1>), g, dim3(256), 0, st, dz, in, scratch, cin, cout, hout,
+                               wout, tiles_x, ntiles, tpb, nig);
+        else if (in_dtype == CGEF_BF16)
+            hipLaunchKernelGGL((k_conv5_wgrad<CGEF_BF16, 0>), g, dim3(256), 0, st, dz, in, scratch, cin, cout, hout,
+                               wout, tiles_x, ntiles, tpb, nig);
+        else
+            hipLaunchKernelGGL((k_conv5_wgrad<CGEF_F32, 0>), g, dim3(256), 0, st, dz, in, scratch, cin, cout, hout,
+                               wout, tiles_x, ntiles, tpb, nig);
+        check_launch();
+        hipLaunchKernelGGL(k_wgrad_reduce, dim3(blocks_for(nw + cout)), dim3(256), 0, st, scratch, nslots, nw, cout, dw,
+                           db);
+        check_launch();
+    });
+}
+
+cgef_status cgef_conv5_pack_weights(const float *w, int64_t cout, int64_t cin, uint16_t *wf, int64_t ocs_f,
+                                    uint16_t *wb, int64_t ocs_b, void *stream) {
+    return guarded([&] {
+        need(w && (wf || wb), CGEF_ERR_ARGUMENT, "cgef_conv5_pack_weights: null pointer");
+        need(cout >= 1 && cin >= 1 && (!wf || ocs_f >= cout) && (!wb || ocs_b >= cin), CGEF_ERR_DIMENSION,
+             "cgef_conv5_pack_weights: layout strides must cover the channels");
+        const int64_t count = cout * cin * 25;
+        hipLaunchKernelGGL(k_pack_weights, dim3(blocks_for(count)), dim3(256), 0, static_cast<hipStream_t>(stream), w,
+                           count, cin, wf, ocs_f, wb, ocs_b);
+        check_launch();
+    });
+}
+
+cgef_status cgef_adam_update(float *param, float *m, float *v, const float *grad, int64_t count, double lr,
+                             double beta1, double beta2, double eps, int64_t step, int64_t cout, int64_t cin,
+                             uint16_t *wf, int64_t ocs_f, uint16_t *wb, int64_t ocs_b, void *stream) {
+    return guarded([&] {
+        need(param && m && v && grad, CGEF_ERR_ARGUMENT, "cgef_adam_update: null pointer");
+        need(step >= 1 && count >= 1, CGEF_ERR_ARGUMENT, "cgef_adam_update: step counts from 1");
+        if (wf || wb)
+            need(count == cout * cin * 25 && (!wf || ocs_f >= cout) && (!wb || ocs_b >= cin), CGEF_ERR_DIMENSION,
+                 "cgef_adam_update: BF16 layouts need count = cout·cin·25 and strides covering the channels");
+        const double bc1 = 1.0 - std::pow(beta1, double(step)), bc2 = 1.0 - std::pow(beta2, double(step));
+        hipLaunchKernelGGL(k_adam, dim3(blocks_for(count)), dim3(256), 0, static_cast<hipStream_t>(stream), param, m, v,
+                           grad, count, float(lr), float(beta1), float(beta2), float(eps), float(bc1), float(bc2),
+                           cin > 0 ? cin : 1, wf, ocs_f, wb, ocs_b);
+        check_launch();
+    });
+}
+
+cgef_status cgef_mse_grad(const float *pred, const float *target, int64_t count, float *dpred, float *loss_sum,
+                          void *stream) {
+    return guarded([&] {
+        need(pred && target && dpred && count >= 1, CGEF_ERR_ARGUMENT, "cgef_mse_grad: bad arguments");
+        hipLaunchKernelGGL(k_mse, dim3(blocks_for(count)), dim3(256), 0, static_cast<hipStream_t>(stream), pred, target,
+                           count, float(1.0 / double(count)), dpred, loss_sum);
+        check_launch();
+    });
+}
+
+cgef_status cgef_closure_crop(const void *src, int32_t src_dtype, int64_t nsrc, int64_t c, int64_t H, int64_t W,
+                              const int32_t *origins, int64_t n, int64_t h, int64_t w, const float *chan_scale,
+                              void *dst, int32_t dst_dtype, void *stream) {
+    return guarded([&] {
+        need(src && origins && dst, CGEF_ERR_ARGUMENT, "cgef_closure_crop: null pointer");
+        need(src_dtype == CGEF_F32 && (dst_dtype == CGEF_F32 || dst_dtype == CGEF_BF16), CGEF_ERR_ARGUMENT,
+             "cgef_closure_crop: FP32 source, FP32 or BF16 destination");
+        need(nsrc >= 1 && c >= 1 && H >= 1 && W >= 1 && n >= 1 && h >= 1 && w >= 1, CGEF_ERR_DIMENSION,
+             "cgef_closure_crop: bad sizes");
+        const int64_t total = n * c * h * w;
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        if (dst_dtype == CGEF_BF16)
+            hipLaunchKernelGGL((k_crop<CGEF_F32, CGEF_BF16>), dim3(blocks_for(total)), dim3(256), 0, st, src, nsrc, c, H, W,
+                               origins, h, w, chan_scale, dst, total);
+        else
+            hipLaunchKernelGGL((k_crop<CGEF_F32, CGEF_F32>), dim3(blocks_for(total)), dim3(256), 0, st, src, nsrc, c, H, W,
+                               origins, h, w, chan_scale, dst, total);
+        check_launch();
+    });
+}
+
+cgef_status cgef_closure_products(const void *u, const void *v, int32_t dtype, int64_t npts, int64_t nb, double *out5,
+                                  void *stream) {
+    return guarded([&] {
+        need(u && v && out5 && npts >= 1 && nb >= 1, CGEF_ERR_ARGUMENT, "cgef_closure_products: bad arguments");
+        need(dtype == CGEF_F32 || dtype == CGEF_F64, CGEF_ERR_ARGUMENT, "cgef_closure_products: FP32 or FP64 fields");
+        const int64_t total = npts * nb;
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        if (dtype == CGEF_F32)
+            hipLaunchKernelGGL(k_products<float>, dim3(blocks_for(total)), dim3(256), 0, st,
+                               static_cast<const float *>(u), static_cast<const float *>(v), npts, total, out5);
+        else
+            hipLaunchKernelGGL(k_products<double>, dim3(blocks_for(total)), dim3(256), 0, st,
+                               static_cast<const double *>(u), static_cast<const double *>(v), npts, total, out5);
+        check_launch();
+    });
+}
+
+cgef_status cgef_closure_targets(const double *filt5, const double *grad4, int64_t npts, int64_t nb, float *tau3,
+                                 float *feat4, void *stream) {
+    return guarded([&] {
+        need(grad4 && feat4 && (!tau3 || filt5) && npts >= 1 && nb >= 1, CGEF_ERR_ARGUMENT,
+             "cgef_closure_targets: bad arguments");
+        const int64_t total = npts * nb;
+        hipLaunchKernelGGL(k_targets, dim3(blocks_for(total)), dim3(256), 0, static_cast<hipStream_t>(stream), filt5,
+                           grad4, npts, total, tau3, feat4);
+        check_launch();
+    });
+}
+
+cgef_status cgef_closure_pi(const float *feat4, const float *tau3, const float *tau_scale, int64_t npts, int64_t nb,
+                            float *Pi, void *stream) {
+    return guarded([&] {
+        need(feat4 && tau3 && Pi && npts >= 1 && nb >= 1, CGEF_ERR_ARGUMENT, "cgef_closure_pi: bad arguments");
+        const int64_t total = npts * nb;
+        hipLaunchKernelGGL(k_closure_pi, dim3(blocks_for(total)), dim3(256), 0, static_cast<hipStream_t>(stream), feat4,
+                           tau3, tau_scale, npts, total, Pi);
+        check_launch();
+    });
+}
+
+cgef_status cgef_closure_gradient_model(const float *feat4, double sigma2, int64_t npts, int64_t nb, float *tau3,
+                                        void *stream) {
+    return guarded([&] {
+        need(feat4 && tau3 && npts >= 1 && nb >= 1, CGEF_ERR_ARGUMENT, "cgef_closure_gradient_model: bad arguments");
+        const int64_t total = npts * nb;
+        hipLaunchKernelGGL(k_gradient_model, dim3(blocks_for(total)), dim3(256), 0, static_cast<hipStream_t>(stream),
+                           feat4, float(sigma2), npts, total, tau3);
+        check_launch();
+    });
+}
+
+}  // extern "C"

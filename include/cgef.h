@@ -29,7 +29,11 @@ typedef enum {
     CGEF_ERR_UNSUPPORTED = 5  /* valid request this backend has no kernel for */
 } cgef_status;
 
-typedef enum { CGEF_F32 = 0, CGEF_F64 = 1 } cgef_dtype;
+typedef enum {
+    CGEF_F32 = 0,
+    CGEF_F64 = 1,
+    CGEF_BF16 = 2  /* learned-closure tensors only (stored as uint16 bit patterns) */
+} cgef_dtype;
 typedef enum { CGEF_CARTESIAN = 0, CGEF_SPHERICAL = 1 } cgef_geometry;
 typedef enum { CGEF_TOPHAT = 0, CGEF_GAUSSIAN = 1 } cgef_kernel_kind;
 typedef enum { CGEF_ZEROFILL = 0, CGEF_DEFORMABLE = 1 } cgef_mask_strategy;
@@ -162,6 +166,79 @@ cgef_status cgef_profile_read(cgef_plan *plan, double *total_ms, int64_t *nlaunc
 
 /* Diagnostics.jl:1319-1333 spectral_density!: host arrays of length n. */
 cgef_status cgef_spectral_density(int64_t n, const double *C, const double *k, double *g);
+
+/* ---------------------------------------------------------------------------------------------
+ * Learned closure for τ_ℓ. No counterpart in the reference, which obtains τ only from the exact
+ * filter: these entry points train and run a 4-layer 5×5 convolutional network (BF16 weights and
+ * activations, FP32 accumulation) whose targets are the exact τ this library's filter produces,
+ * and the exact Π stays its judge and fallback. Tensors are device (n, channels, rows, cols), cols
+ * fastest; BF16 tensors are uint16 bit patterns. All are stream-ordered and do not synchronize.
+ * --------------------------------------------------------------------------------------------- */
+
+/* One 5×5 convolution layer. pad 0: valid, out = (n, cout, hin−4, win−4); pad 4: full, out =
+ * (n, cout, hin+4, win+4) with zeros outside the input (the data gradient uses it with the
+ * backward-data weight layout). Input BF16 (in_gelu: GELU applied while loading, i.e. the input
+ * is the previous layer's pre-activation) or FP32. w is BF16 [cin][25][ocs], k = 5·ky + kx, ocs a
+ * multiple of the output-channel block (4 for cout ≤ 4, 16 for cout ≤ 16, else 64). Without zgrad:
+ * out = conv + bias as BF16 or FP32. With zgrad (BF16, shaped as out): out = conv · GELU'(zgrad),
+ * FP32, bias unused. */
+cgef_status cgef_conv5(const void *in, int32_t in_dtype, int32_t in_gelu, int64_t n, int64_t cin, int64_t hin,
+                       int64_t win, const uint16_t *w, int64_t ocs, const float *bias, int64_t cout, int32_t pad,
+                       void *out, int32_t out_dtype, const uint16_t *zgrad, void *stream);
+
+/* Weight and bias gradients of a valid 5×5 layer: dw[o][i][k] = Σ dz[n][o][p]·A[n][i][p+k],
+ * db[o] = Σ dz[n][o][p], with A the layer input (n, cin, hout+4, wout+4) as in cgef_conv5 and dz
+ * FP32 (n, cout, hout, wout). Deterministic: per-block partial sums go to `scratch` (at least the
+ * floats cgef_conv5_wgrad_scratch returns) and are added in a fixed order. Overwrites dw, db. */
+cgef_status cgef_conv5_wgrad_scratch(int64_t n, int64_t cin, int64_t cout, int64_t hout, int64_t wout,
+                                     int64_t *floats);
+cgef_status cgef_conv5_wgrad(const float *dz, const void *in, int32_t in_dtype, int32_t in_gelu, int64_t n,
+                             int64_t cin, int64_t cout, int64_t hout, int64_t wout, float *dw, float *db,
+                             float *scratch, int64_t scratch_floats, void *stream);
+
+/* BF16 layouts of an FP32 master weight tensor W[cout][cin][25]: forward wf[cin][25][ocs_f] and
+ * backward-data wb[cout][25][ocs_b] with wb[o][24−k][i] = W[o][i][k] (either may be NULL). Entries
+ * of the padding channels are left as they are (zero them once at allocation). */
+cgef_status cgef_conv5_pack_weights(const float *w, int64_t cout, int64_t cin, uint16_t *wf, int64_t ocs_f,
+                                    uint16_t *wb, int64_t ocs_b, void *stream);
+
+/* One Adam step (bias-corrected; step counts from 1) on an FP32 parameter tensor with FP32 moments;
+ * if wf / wb are given the updated weights are also written as BF16 layouts as in
+ * cgef_conv5_pack_weights (then count = cout·cin·25). */
+cgef_status cgef_adam_update(float *param, float *m, float *v, const float *grad, int64_t count, double lr,
+                             double beta1, double beta2, double eps, int64_t step, int64_t cout, int64_t cin,
+                             uint16_t *wf, int64_t ocs_f, uint16_t *wb, int64_t ocs_b, void *stream);
+
+/* Mean-squared-error gradient dpred = 2(pred − target)/count (FP32); Σ(pred − target)² is added to
+ * the device scalar *loss_sum when it is not NULL. */
+cgef_status cgef_mse_grad(const float *pred, const float *target, int64_t count, float *dpred, float *loss_sum,
+                          void *stream);
+
+/* Periodic crops: dst[i][c][y][x] = chan_scale[c]·src[img][c][(y0+y) mod H][(x0+x) mod W] with
+ * origins[i] = (img, y0, x0) (device int32, n × 3); src FP32 (nsrc, c, H, W); dst FP32 or BF16
+ * (n, c, h, w); chan_scale may be NULL. An origin whose image index is outside [0, nsrc) gives zeros. */
+cgef_status cgef_closure_crop(const void *src, int32_t src_dtype, int64_t nsrc, int64_t c, int64_t H, int64_t W,
+                              const int32_t *origins, int64_t n, int64_t h, int64_t w, const float *chan_scale,
+                              void *dst, int32_t dst_dtype, void *stream);
+
+/* The planes the exact τ is filtered from: out5 = (nb, 5, npts) FP64 holding u, v, u², u·v, v². */
+cgef_status cgef_closure_products(const void *u, const void *v, int32_t dtype, int64_t npts, int64_t nb, double *out5,
+                                  void *stream);
+
+/* From the filtered planes (nb, 5, npts) and the gradients (nb, 4, npts) of ū, v̄ (∂x ū, ∂y ū,
+ * ∂x v̄, ∂y v̄), all FP64: tau3 = (τxx, τxy, τyy) = F[q_a q_b] − q̄_a q̄_b and feat4 = the gradients,
+ * both FP32. With tau3 NULL only feat4 is written and filt5 may be NULL. */
+cgef_status cgef_closure_targets(const double *filt5, const double *grad4, int64_t npts, int64_t nb, float *tau3,
+                                 float *feat4, void *stream);
+
+/* Π = −(S̄xx τxx + 2 S̄xy τxy + S̄yy τyy) (Diagnostics.jl:460-466 contraction) from feat4 and
+ * tau3 · tau_scale[c] (tau_scale may be NULL), FP32 (nb, npts). */
+cgef_status cgef_closure_pi(const float *feat4, const float *tau3, const float *tau_scale, int64_t npts, int64_t nb,
+                            float *Pi, void *stream);
+
+/* The closure without learning, the gradient model τ_ab = σ² Σ_k ∂_k ū_a ∂_k ū_b. */
+cgef_status cgef_closure_gradient_model(const float *feat4, double sigma2, int64_t npts, int64_t nb, float *tau3,
+                                        void *stream);
 
 #ifdef __cplusplus
 }

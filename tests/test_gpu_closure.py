@@ -1,0 +1,188 @@
+"""GPU parity of the learned closure's kernels (libcgef_hip.so through cgef.closure) against the FP64
+oracle (oracle/closure.py, pinned in test_closure_oracle.py), and its training / fallback behaviour.
+
+Tolerances: the oracle rounds what the device stores in BF16 (input, weights, hidden
+pre-activations), so the device differs from it by FP32 accumulation order, plus the rare BF16
+rounding that lands on the other side of a tie-point: forward rel-L2 ≤ 5e-3, gradients ≤ 1e-2 (an
+index or sign mistake gives O(1)). Exact-τ pairs and Π: FP32 storage of FP64 results, 1e-6."""
+import numpy as np
+import pytest
+import torch
+
+import cgef
+from cgef import closure as CL
+from cgef import synthetic as SY
+from helpers import cart_axes, oracle_grid_cart, rel_l2
+from oracle import closure as OC
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+
+
+def N(t):
+    torch.cuda.synchronize()
+    return t.detach().float().cpu().numpy().astype(np.float64)
+
+
+def _net_and_input(channels, n, h, w, seed):
+    net = CL.ClosureNet(channels, seed=seed, device=0)
+    # non-zero biases so the bias path is exercised
+    rng = np.random.Generator(np.random.PCG64(seed + 100))
+    Ws, bs = net.state()
+    bs = [0.1 * rng.standard_normal(b.shape) for b in bs]
+    net.load_state(Ws, bs)
+    x = rng.standard_normal((n, channels[0], h, w))
+    xb = torch.tensor(x, dtype=torch.float32, device=DEV).to(torch.bfloat16)
+    params = [(Ws[l], bs[l]) for l in range(len(Ws))]
+    return net, xb, x, params
+
+
+@pytest.mark.parametrize("channels,n,h,w", [((4, 64, 64, 64, 3), 2, 40, 37), ((4, 8, 20, 3), 3, 23, 50)])
+def test_conv_forward_matches_oracle(channels, n, h, w):
+    print(f"closure forward {channels} n={n} {h}x{w}", flush=True)
+    net, xb, x, params = _net_and_input(channels, n, h, w, 1)
+    y = net.forward(xb)
+    yo, zs, _ = OC.forward(params, x, bf16=True)
+    assert rel_l2(N(y), yo) <= 5e-3
+    for l, z in enumerate(zs):
+        assert rel_l2(N(net.Z[l]), z) <= 5e-3, l
+
+
+@pytest.mark.parametrize("channels,n,h,w", [((4, 64, 64, 64, 3), 2, 40, 37), ((4, 8, 20, 3), 3, 23, 50)])
+def test_conv_backward_matches_oracle(channels, n, h, w):
+    print(f"closure backward {channels} n={n} {h}x{w}", flush=True)
+    net, xb, x, params = _net_and_input(channels, n, h, w, 2)
+    L = len(channels) - 1
+    rng = np.random.Generator(np.random.PCG64(9))
+    target = rng.standard_normal((n, channels[-1], h - 4 * L, w - 4 * L))
+    net.loss_sum.zero_()
+    net.forward(xb)
+    net.backward(torch.tensor(target, dtype=torch.float32, device=DEV))
+    loss, grads = OC.backward(params, x, target, bf16=True)
+    assert abs(float(net.loss_sum.item()) / target.size - loss) <= 5e-3 * loss
+    for l in range(L):
+        assert rel_l2(N(net.gW[l]), grads[l][0]) <= 1e-2, l
+        assert rel_l2(N(net.gb[l]), grads[l][1]) <= 1e-2, l
+
+
+def test_adam_and_bf16_layouts():
+    print("closure adam", flush=True)
+    channels = (4, 8, 20, 3)
+    net, xb, x, params = _net_and_input(channels, 1, 20, 20, 3)
+    rng = np.random.Generator(np.random.PCG64(4))
+    Ws, bs = net.state()
+    ms = [(np.zeros_like(w), np.zeros_like(w), np.zeros_like(b), np.zeros_like(b)) for w, b in zip(Ws, bs)]
+    for step in range(1, 4):
+        gs = [(rng.standard_normal(w.shape), rng.standard_normal(b.shape)) for w, b in zip(Ws, bs)]
+        for l, (gw, gb) in enumerate(gs):
+            net.gW[l].copy_(torch.tensor(gw, dtype=torch.float32))
+            net.gb[l].copy_(torch.tensor(gb, dtype=torch.float32))
+            gw32, gb32 = gw.astype(np.float32).astype(np.float64), gb.astype(np.float32).astype(np.float64)
+            mw, vw, mb, vb = ms[l]
+            Ws[l], mw, vw = OC.adam(Ws[l], mw, vw, gw32, step)
+            bs[l], mb, vb = OC.adam(bs[l], mb, vb, gb32, step)
+            ms[l] = (mw, vw, mb, vb)
+        net.step()
+    got_W, got_b = net.state()
+    for l in range(len(Ws)):
+        assert np.max(np.abs(got_W[l] - Ws[l])) <= 1e-6 * (1 + np.max(np.abs(Ws[l])))
+        assert np.max(np.abs(got_b[l] - bs[l])) <= 1e-6
+        ci, co = channels[l], channels[l + 1]
+        wb16 = torch.tensor(got_W[l], dtype=torch.float32).to(torch.bfloat16).float().numpy().reshape(co, ci, 25)
+        wf = N(net.wf[l]).reshape(ci, 25, net.ocs_f[l])
+        wbk = N(net.wb[l]).reshape(co, 25, net.ocs_b[l])
+        np.testing.assert_array_equal(wf[:, :, :co], wb16.transpose(1, 2, 0))
+        np.testing.assert_array_equal(wbk[:, :, :ci], wb16[:, :, ::-1].transpose(0, 2, 1))
+        assert not wf[:, :, co:].any() and not wbk[:, :, ci:].any()
+
+
+def test_crop_is_a_periodic_gather():
+    print("closure crop", flush=True)
+    rng = np.random.Generator(np.random.PCG64(5))
+    src = rng.standard_normal((3, 2, 17, 23)).astype(np.float32)
+    org = np.array([[2, -5, 20], [0, 16, -30], [1, 3, 4]], dtype=np.int32)
+    scale = np.array([2.0, -0.5], dtype=np.float32)
+    S = torch.tensor(src, device=DEV)
+    O_ = torch.tensor(org, device=DEV)
+    sc = torch.tensor(scale, device=DEV)
+    for dt in (torch.float32, torch.bfloat16):
+        dst = torch.empty((3, 2, 9, 31), dtype=dt, device=DEV)
+        CL.crop_(dst, S, O_, sc)
+        ref = np.empty((3, 2, 9, 31), dtype=np.float32)
+        for i, (img, y0, x0) in enumerate(org):
+            ys = (y0 + np.arange(9)) % 17
+            xs = (x0 + np.arange(31)) % 23
+            ref[i] = src[img][:, ys][:, :, xs] * scale[:, None, None]
+        if dt == torch.bfloat16:
+            ref = torch.tensor(ref).to(torch.bfloat16).float().numpy()
+        np.testing.assert_array_equal(N(dst), ref.astype(np.float64))
+
+
+def _periodic_case(n=64, ell_cells=4.0, seed=0):
+    u, v, xs = SY.cartesian_turbulence(N=n, L=1.0e6, seed=seed, kmax=n // 3)
+    grid = cgef.StructuredGrid(cgef.CartesianGeometry(), xs, xs, periodic=(True, True))
+    dx = xs[1] - xs[0]
+    plan = cgef.plan_filter(grid, cgef.GaussianKernel(6.0), ell_cells * dx)
+    og = oracle_grid_cart(xs, xs, periodic=(True, True))
+    return u, v, xs, grid, plan, og, ell_cells * dx
+
+
+def test_exact_pairs_match_oracle_filter():
+    print("closure exact pairs", flush=True)
+    u, v, xs, grid, plan, og, ell = _periodic_case()
+    assert plan.engine == "separable"
+    feat, tau = CL.closure_features(torch.tensor(u, device=DEV), torch.tensor(v, device=DEV), grid, plan)
+    f = O.filter_fields(og, [u, v, u * u, u * v, v * v], kind=O.GAUSSIAN, alpha=6.0, ell=ell,
+                        engine=O.ENGINE_SEPARABLE)
+    ub, vb = f[0], f[1]
+    tau_ref = np.stack([f[2] - ub * ub, f[3] - ub * vb, f[4] - vb * vb])
+    grad_ref = np.stack([O.deriv(og, 0, ub), O.deriv(og, 1, ub), O.deriv(og, 0, vb), O.deriv(og, 1, vb)])
+    assert rel_l2(N(tau[0]), tau_ref) <= 1e-6
+    assert rel_l2(N(feat[0]), grad_ref) <= 1e-6
+    feat2, none = CL.closure_features(torch.tensor(u, device=DEV), torch.tensor(v, device=DEV), grid, plan, tau=False)
+    assert none is None
+    np.testing.assert_array_equal(N(feat2), N(feat))
+    # Π from the exact pairs is compute_Π!'s Π
+    Pi = torch.empty((64, 64), dtype=torch.float64, device=DEV)
+    cgef.compute_pi_(Pi, torch.tensor(u, device=DEV), torch.tensor(v, device=DEV), None, grid, plan.kernel, plan.scale,
+                     filter_plan=plan)
+    assert rel_l2(N(CL.pi_from_tau(feat, tau)[0]), N(Pi)) <= 1e-5
+    # gradient model and Π kernels against the oracle's closed forms
+    fn = N(feat[0])
+    gm = N(CL.gradient_model_tau(feat, ell * ell / 12.0)[0])
+    assert rel_l2(gm, OC.gradient_model(fn, ell * ell / 12.0)) <= 1e-6
+    sc = torch.tensor([2.0, 3.0, 0.5], device=DEV)
+    assert rel_l2(N(CL.pi_from_tau(feat, tau, sc)[0]),
+                  OC.closure_pi(fn, N(tau[0]) * np.array([2.0, 3.0, 0.5])[:, None, None])) <= 1e-6
+
+
+def test_training_learns_tau_and_fallback_uses_exact_pi():
+    """A short run at small size: the loss falls, the held-out Π correlates with the exact Π, and
+    LearnedClosure returns the exact Π when the measured error is above its threshold."""
+    print("closure training", flush=True)
+    torch.manual_seed(0)
+    seeds = list(range(10))
+    feat, tau, grid, plan = CL.closure_dataset(seeds, N=128, ell_cells=8.0, device=0)
+    net, fs, ts, hist = CL.train_closure(feat[:8], tau[:8], steps=300, batch=8, crop=32, log_every=50, seed=0)
+    assert hist[-1][1] < 0.5 * hist[0][1], hist
+    tau_n = CL.predict_tau(net, feat[8:], fs)
+    pred = CL.pi_from_tau(feat[8:], tau_n, ts)
+    exact = CL.pi_from_tau(feat[8:], tau[8:].contiguous())
+    corr, rel = CL.pi_errors(pred, exact)
+    print(f"held-out Π: corr {corr:.3f} rel-L2 {rel:.3f}", flush=True)
+    assert corr > 0.7
+
+    u, v, _ = SY.cartesian_turbulence(N=128, L=1.0e6, seed=20)
+    ut, vt = torch.tensor(u, device=DEV), torch.tensor(v, device=DEV)
+    exact_pi = torch.empty((128, 128), dtype=torch.float32, device=DEV)
+    cgef.compute_pi_(exact_pi, ut.float(), vt.float(), None, grid, plan.kernel, plan.scale, filter_plan=plan)
+    strict = CL.LearnedClosure(net, fs, ts, grid, plan, threshold=1e-9)
+    out = torch.empty_like(exact_pi)
+    _, used_exact = strict.compute_pi_(out, ut, vt)
+    assert used_exact and strict.disabled
+    np.testing.assert_array_equal(N(out), N(exact_pi))
+    lax = CL.LearnedClosure(net, fs, ts, grid, plan, threshold=10.0)
+    _, used_exact = lax.compute_pi_(out, ut, vt)
+    assert not used_exact and lax.last_error is not None
+    assert rel_l2(N(out), N(exact_pi)) == pytest.approx(lax.last_error, rel=1e-3)
