@@ -364,15 +364,16 @@ def spectral_leg(args):
 
 def closure_leg(args):
     """Learned τ closure on the doubly periodic k⁻³ box (1024², Gaussian α = 6, ℓ = 8Δx): exact (∇ū, τ)
-    pairs from the filter for 64 training and 16 held-out seeds, the network trained on the device for
-    2000 Adam steps at batch 16 (64² crops with their 8-cell context), then Π of the held-out fields
-    from the network's τ against the exact Π, next to the gradient model (the closure without
-    learning) and the exact pipeline's cost."""
+    pairs from the filter for 64 training and 16 held-out seeds; the network learns what the gradient
+    model σ²∇ū∇ūᵀ misses of τ, trained on the device for 2000 Adam steps at batch 16 (64² crops with
+    their 8-cell context); then Π of the held-out fields against the exact Π, next to the gradient
+    model alone, the exact pipeline's cost, and the exact-Π judge with its 20 % fallback threshold."""
     import numpy as np
     import torch
 
     import cgef
     from cgef import closure as CL
+    from cgef import synthetic as S
 
     N, ntrain, ntest = args.closure_n, 64, 16
     steps, batch, crop = args.closure_steps, 16, 64
@@ -381,23 +382,24 @@ def closure_leg(args):
     feat, tau, grid, plan = CL.closure_dataset(list(range(1000, 1000 + ntrain + ntest)), N=N, ell_cells=8.0, device=0)
     torch.cuda.synchronize()
     t_data = time.perf_counter() - t0
+    s2 = CL.gradient_model_sigma2(plan.kernel, plan.scale)
     ftr, ttr = feat[:ntrain], tau[:ntrain]
     fte, tte = feat[ntrain:].contiguous(), tau[ntrain:].contiguous()
     log(f"closure: data in {t_data:.1f} s; training {steps} steps, batch {batch}, {crop}^2 crops")
     t0 = time.perf_counter()
-    net, fs, ts, hist = CL.train_closure(ftr, ttr, steps=steps, batch=batch, crop=crop, log_every=max(1, steps // 8),
-                                         seed=0, log=log)
+    model = CL.fit_closure(ftr, ttr, s2, steps=steps, batch=batch, crop=crop, log_every=max(1, steps // 8), seed=0,
+                           log=log)
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
+    net, hist = model.net, model.history
     h = crop + 2 * net.halo
     fwd_flops = batch * net.flops_per_sample(h, h)
-    # forward and one full training step of a batch, timed with events on the launch stream
+    # forward and one training step's kernels on one batch, timed with events on the launch stream
     X = torch.empty((batch, 4, h, h), dtype=torch.bfloat16, device="cuda:0")
-    T = torch.empty((batch, 3, crop, crop), dtype=torch.float32, device="cuda:0")
+    T = torch.zeros((batch, 3, crop, crop), dtype=torch.float32, device="cuda:0")
     org = torch.zeros((batch, 3), dtype=torch.int32, device="cuda:0")
-    CL.crop_(X, ftr, org, fs)
-    CL.crop_(T, ttr, org, 1.0 / ts)
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    CL.crop_(X, ftr, org, model.feat_scale)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
     net.forward(X)
     torch.cuda.synchronize()
     reps = 20
@@ -414,17 +416,13 @@ def closure_leg(args):
     fb_ms = ev[1].elapsed_time(ev[2]) / reps
     # held-out evaluation
     t0 = time.perf_counter()
-    tau_n = CL.predict_tau(net, fte, fs, chunk=2)
-    pred = CL.pi_from_tau(fte, tau_n, ts)
+    pred = model.predict_pi(fte)
     torch.cuda.synchronize()
     t_inf = (time.perf_counter() - t0) / ntest
     exact = CL.pi_from_tau(fte, tte)
     corr, rel = CL.pi_errors(pred, exact)
-    ell = 8.0 * (1.0e6 / N)
-    gm = CL.pi_from_tau(fte, CL.gradient_model_tau(fte, ell * ell / 12.0))
-    gcorr, grel = CL.pi_errors(gm, exact)
-    # the exact pipeline's cost per field (compute_Π! on FP32 fields through the same plan)
-    from cgef import synthetic as S
+    gcorr, grel = CL.pi_errors(CL.pi_from_tau(fte, CL.gradient_model_tau(fte, s2)), exact)
+    # the exact pipeline's cost per field (compute_Π! on FP32 fields through the same plan), and the judge
     u, v, _ = S.cartesian_turbulence(N=N, L=1.0e6, seed=1000 + ntrain)
     ut = torch.tensor(u, dtype=torch.float32, device="cuda:0")
     vt = torch.tensor(v, dtype=torch.float32, device="cuda:0")
@@ -434,17 +432,22 @@ def closure_leg(args):
         cgef.compute_pi_(Pi, ut, vt, None, grid, plan.kernel, plan.scale, filter_plan=plan)
 
     t_exact = timed_steps(exact_step, 10, 2)[0] / 10
+    judge = CL.LearnedClosure(model, grid, plan, threshold=0.2)
+    out = torch.empty_like(ut)
+    _, used_exact = judge.compute_pi_(out, ut, vt)
+    log(f"closure: held-out Π corr {corr:.4f} rel-L2 {rel:.4f} (gradient model {gcorr:.4f} / {grel:.4f}); "
+        f"judge error {judge.last_error:.4f}, exact used: {used_exact}")
     return {
         "workload": f"doubly periodic {N}x{N} k^-3 box, Gaussian alpha=6, l=8dx; net 4-64-64-64-3, 5x5, GELU, "
-                    f"BF16 weights/activations, FP32 accumulation; Adam lr 1e-3, {steps} steps, batch {batch} "
-                    f"{crop}^2 crops; {ntrain} training / {ntest} held-out seeds",
+                    f"BF16 weights/activations, FP32 accumulation, learns tau minus the gradient model; Adam lr 1e-3, "
+                    f"{steps} steps, batch {batch} {crop}^2 crops; {ntrain} training / {ntest} held-out seeds",
         "data_s": t_data, "train_s": t_train, "ms_per_train_step": 1e3 * t_train / steps,
         "final_train_mse": hist[-1][1] if hist else None,
         "forward_ms_per_batch": fwd_ms, "forward_tflops": fwd_flops / (fwd_ms * 1e-3) / 1e12,
         "train_step_kernels_ms": fb_ms, "train_step_tflops": 3 * fwd_flops / (fb_ms * 1e-3) / 1e12,
         "heldout_pi_corr": corr, "heldout_pi_rel_l2": rel,
         "gradient_model_pi_corr": gcorr, "gradient_model_pi_rel_l2": grel,
-        "fallback_to_exact": rel > 0.2,
+        "judge_rel_l2": judge.last_error, "judge_threshold": 0.2, "judge_used_exact": used_exact,
         "inference_ms_per_field": 1e3 * t_inf, "exact_pi_ms_per_field": 1e3 * t_exact, "exact_engine": plan.engine,
     }
 

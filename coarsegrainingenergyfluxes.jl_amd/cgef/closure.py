@@ -2,7 +2,9 @@
 only by the exact filter, Diagnostics.jl:496-527).
 
 A 4-layer 5×5 convolutional network maps the filtered-velocity gradients (∂xū, ∂yū, ∂xv̄, ∂yv̄)
-to τ = (τxx, τxy, τyy). It is trained on the device by this library's own kernels (libcgef_hip.so:
+to what the gradient model τ ≈ σ² G Gᵀ (G = ∇ū, σ² the footprint's second moment) misses of
+τ = (τxx, τxy, τyy): in 2-D incompressible flow the gradient model's Π vanishes identically
+(tr S̄ = 0 makes −S̄:G Gᵀ = 0), so Π is carried by that residual. It is trained on the device by this library's own kernels (libcgef_hip.so:
 cgef_conv5 forward and data gradient, cgef_conv5_wgrad, cgef_adam_update, cgef_mse_grad) on pairs
 that the exact filter itself produces (closure_dataset), with BF16 weights and activations and FP32
 accumulation. The exact Π stays its judge: LearnedClosure measures the network's Π against the exact
@@ -57,11 +59,12 @@ def _L():
             "cgef_conv5_pack_weights": [vp, i64, i64, vp, i64, vp, i64, vp],
             "cgef_adam_update": [vp, vp, vp, vp, i64, dbl, dbl, dbl, dbl, i64, i64, i64, vp, i64, vp, i64, vp],
             "cgef_mse_grad": [vp, vp, i64, vp, vp, vp],
+            "cgef_closure_loss_grad": [vp, vp, vp, vp, dbl, i64, i64, vp, vp, vp],
             "cgef_closure_crop": [vp, i32, i64, i64, i64, i64, vp, i64, i64, i64, vp, vp, i32, vp],
             "cgef_closure_products": [vp, vp, i32, i64, i64, vp, vp],
             "cgef_closure_targets": [vp, vp, i64, i64, vp, vp, vp],
-            "cgef_closure_pi": [vp, vp, vp, i64, i64, vp, vp],
-            "cgef_closure_gradient_model": [vp, dbl, i64, i64, vp, vp],
+            "cgef_closure_pi": [vp, vp, vp, dbl, i64, i64, vp, vp],
+            "cgef_closure_gradient_model": [vp, dbl, vp, i64, i64, vp, vp],
         }
         for name, args in sigs.items():
             f = getattr(L, name)
@@ -161,7 +164,7 @@ class ClosureNet:
         self._shape = None
         self.gW = [torch.zeros_like(w) for w in self.W]
         self.gb = [torch.zeros_like(b) for b in self.b]
-        self.loss_sum = torch.zeros(1, dtype=torch.float32, device=dev)
+        self.loss_sum = torch.zeros(2, dtype=torch.float32, device=dev)    # (Σ τ part, Σ Π part)
 
     @property
     def halo(self):
@@ -217,14 +220,24 @@ class ClosureNet:
             a, gel = self.Z[l], True
         return self.Z[-1]
 
-    def backward(self, target, *, accumulate_loss=True):
-        """Gradients of mean((Y − target)²) after forward(); fills gW, gb."""
+    def backward(self, target, *, accumulate_loss=True, feat=None, coef=None, lambda_pi=0.0):
+        """Gradients of mean((Y − target)²) after forward(); fills gW, gb. With `feat` (the normalized
+        gradients at the output pixels, (n, 4, h, w)) and `coef` (cgef_closure_loss_grad) the loss
+        adds lambda_pi · mean((Π(Y) − Π(target))²)."""
         y = self.Z[-1]
         if tuple(target.shape) != tuple(y.shape):
             raise _lib.DimensionMismatch(f"target is {tuple(target.shape)}, output is {tuple(y.shape)}")
         _check_dev(target, "target", torch.float32)
-        _lib.check(_L().cgef_mse_grad(_p(y), _p(target), y.numel(), _p(self.dZ[-1]),
-                                      _p(self.loss_sum) if accumulate_loss else None, _stream(self.device)))
+        ls = _p(self.loss_sum) if accumulate_loss else None
+        if feat is not None:
+            n, c, h, w = y.shape
+            if c != 3 or tuple(feat.shape) != (n, 4, h, w) or coef is None or coef.numel() != 7:
+                raise _lib.DimensionMismatch("the Π loss needs 3 output channels, feat (n, 4, h, w) and 7 coefficients")
+            _check_dev(feat, "feat", torch.float32)
+            _lib.check(_L().cgef_closure_loss_grad(_p(y), _p(target), _p(feat), _p(coef), float(lambda_pi), n, h * w,
+                                                   _p(self.dZ[-1]), ls, _stream(self.device)))
+        else:
+            _lib.check(_L().cgef_mse_grad(_p(y), _p(target), y.numel(), _p(self.dZ[-1]), ls, _stream(self.device)))
         self.backward_from(self.dZ[-1])
 
     def backward_from(self, dy):
@@ -250,9 +263,9 @@ class ClosureNet:
                                              co, self.lr, b1, b2, self.eps, self.t, co, 1, None, 0, None, 0,
                                              _stream(self.device)))
 
-    def train_step(self, x, target):
+    def train_step(self, x, target, **loss):
         self.forward(x)
-        self.backward(target)
+        self.backward(target, **loss)
         self.step()
 
     def flops_per_sample(self, h, w):
@@ -349,15 +362,23 @@ def rms_per_channel(x):
 # ------------------------------------------------------------------------------------------------
 
 def train_closure(feat, tau, *, steps=2000, batch=16, crop=64, lr=1e-3, seed=0, net=None, channels=(4, 64, 64, 64, 3),
-                  log_every=0, log=None):
-    """Adam on mean-squared error of the RMS-normalized τ over random periodic crops (batch × crop²
-    predicted cells, each with its 8-cell context). Returns (net, feat_scale, tau_scale, history),
-    feat_scale = 1/RMS of each input channel and tau_scale = RMS of each τ channel."""
+                  lambda_pi=0.0, log_every=0, log=None):
+    """Adam on the mean-squared error of the RMS-normalized τ over random periodic crops (batch ×
+    crop² predicted cells, each with its 8-cell context), plus lambda_pi × the mean-squared error of
+    the Π those τ give with the exact gradients, in units of the training set's RMS Π. Returns
+    (net, feat_scale, tau_scale, history): feat_scale = 1/RMS of each input channel, tau_scale =
+    RMS of each τ channel, history = (step, τ mse, Π mse, seconds)."""
     dev = feat.device
     S, c0, N, _ = feat.shape
     fs = (1.0 / rms_per_channel(feat)).to(dev)
     ts = rms_per_channel(tau).to(dev)
     inv_ts = (1.0 / ts).to(dev)
+    coef = None
+    if lambda_pi > 0:
+        pi = pi_from_tau(feat, tau)
+        ps = 1.0 / float(torch.sqrt(torch.mean(pi.double() ** 2)))
+        coef = torch.cat([1.0 / fs, ts * ps]).float().contiguous()
+        del pi
     if net is None:
         net = ClosureNet(channels, seed=seed, device=dev.index, lr=lr)
     halo = net.halo
@@ -369,20 +390,28 @@ def train_closure(feat, tau, *, steps=2000, batch=16, crop=64, lr=1e-3, seed=0, 
     org_t = torch.tensor(np.stack([img, y0, x0], -1).astype(np.int32), device=dev)
     X = torch.empty((batch, c0, crop + 2 * halo, crop + 2 * halo), dtype=torch.bfloat16, device=dev)
     T = torch.empty((batch, tau.shape[1], crop, crop), dtype=torch.float32, device=dev)
+    G = torch.empty((batch, c0, crop, crop), dtype=torch.float32, device=dev) if coef is not None else None
     hist = []
     net.loss_sum.zero_()
     t0 = time.perf_counter()
+    since = 0
     for s in range(steps):
         crop_(X, feat, org_in[s], fs)
         crop_(T, tau, org_t[s], inv_ts)
-        net.train_step(X, T)
+        if G is not None:
+            crop_(G, feat, org_t[s], fs)
+            net.train_step(X, T, feat=G, coef=coef, lambda_pi=lambda_pi)
+        else:
+            net.train_step(X, T)
+        since += 1
         if log_every and ((s + 1) % log_every == 0 or s == steps - 1):
-            tot = float(net.loss_sum.item())
-            cnt = log_every if (s + 1) % log_every == 0 else (s + 1) % log_every
-            hist.append((s + 1, tot / (cnt * T.numel()), time.perf_counter() - t0))
+            tot = net.loss_sum.tolist()
+            hist.append((s + 1, tot[0] / (since * T.numel()), tot[1] / (since * T.numel() / 3),
+                         time.perf_counter() - t0))
             net.loss_sum.zero_()
+            since = 0
             if log:
-                log(f"closure step {s + 1}: mse {hist[-1][1]:.4e}  ({hist[-1][2]:.1f} s)")
+                log(f"closure step {s + 1}: τ mse {hist[-1][1]:.4e}, Π mse {hist[-1][2]:.4e} ({hist[-1][3]:.1f} s)")
     return net, fs, ts, hist
 
 
@@ -402,21 +431,36 @@ def predict_tau(net, feat, feat_scale, *, chunk=2):
     return out
 
 
-def pi_from_tau(feat, tau, tau_scale=None):
-    """Π = −S̄:τ per slice (FP32) from gradients (S, 4, …) and τ (S, 3, …) · tau_scale."""
+def pi_from_tau(feat, tau, tau_scale=None, sigma2=0.0):
+    """Π = −S̄:τ per slice (FP32) from gradients (S, 4, …) and τ = tau·tau_scale + sigma2·G Gᵀ."""
     S = feat.shape[0]
     npts = feat.shape[-1] * feat.shape[-2]
+    if tau.shape[0] != S or tau.shape[1] != 3 or not tau.is_contiguous() or not feat.is_contiguous():
+        raise _lib.DimensionMismatch("pi_from_tau: feat (S, 4, …) and tau (S, 3, …), both contiguous")
     Pi = torch.empty((S, *feat.shape[-2:]), dtype=torch.float32, device=feat.device)
-    _lib.check(_L().cgef_closure_pi(_p(feat), _p(tau), _p(tau_scale), npts, S, _p(Pi), _stream(feat.device.index)))
+    _lib.check(_L().cgef_closure_pi(_p(feat), _p(tau), _p(tau_scale), float(sigma2), npts, S, _p(Pi),
+                                    _stream(feat.device.index)))
     return Pi
 
 
-def gradient_model_tau(feat, sigma2):
+def gradient_model_tau(feat, sigma2, base=None):
+    """base + σ² G Gᵀ (the gradient model when base is None) per slice, FP32 (S, 3, …)."""
     S = feat.shape[0]
     npts = feat.shape[-1] * feat.shape[-2]
+    if base is not None and (tuple(base.shape) != (S, 3, *feat.shape[-2:]) or not base.is_contiguous()):
+        raise _lib.DimensionMismatch("base must be a contiguous (S, 3, …) tensor")
     tau = torch.empty((S, 3, *feat.shape[-2:]), dtype=torch.float32, device=feat.device)
-    _lib.check(_L().cgef_closure_gradient_model(_p(feat), float(sigma2), npts, S, _p(tau), _stream(feat.device.index)))
+    _lib.check(_L().cgef_closure_gradient_model(_p(feat), float(sigma2), _p(base), npts, S, _p(tau),
+                                                _stream(feat.device.index)))
     return tau
+
+
+def gradient_model_sigma2(kernel, scale):
+    """Per-axis second moment σ² of the kernel's footprint, the gradient model's coefficient:
+    Gaussian exp(−α d²/ℓ²) → ℓ²/(2α); top-hat disk of diameter ℓ → ℓ²/16."""
+    if isinstance(kernel, GaussianKernel):
+        return scale * scale / (2.0 * kernel.alpha)
+    return scale * scale / 16.0
 
 
 def pi_errors(pred, exact):
@@ -428,14 +472,36 @@ def pi_errors(pred, exact):
     return corr, rel
 
 
-class LearnedClosure:
-    """Π from the network's τ, judged against the exact Π: the first call and every
-    `check_every`-th call also compute the exact Π (compute_Π! through this library's filter) and
-    the relative L2 error; above `threshold` (default 20 %) that call returns the exact Π and the
-    closure stays on the exact path until `revalidate()`."""
+class ClosureModel:
+    """A trained closure: τ ≈ σ² G Gᵀ (gradient model) + tau_scale · net(feat_scale · ∇ū)."""
 
-    def __init__(self, net, feat_scale, tau_scale, grid, plan, *, threshold=0.2, check_every=16):
-        self.net, self.fs, self.ts = net, feat_scale, tau_scale
+    def __init__(self, net, feat_scale, tau_scale, sigma2, history=()):
+        self.net, self.feat_scale, self.tau_scale, self.sigma2 = net, feat_scale, tau_scale, float(sigma2)
+        self.history = list(history)
+
+    def predict_pi(self, feat, *, chunk=2):
+        """Π of whole periodic fields feat (S, 4, N, M) from the network's τ."""
+        r = predict_tau(self.net, feat, self.feat_scale, chunk=chunk)
+        return pi_from_tau(feat, r, self.tau_scale, self.sigma2)
+
+
+def fit_closure(feat, tau, sigma2, **train):
+    """Train the network on the residual τ − σ² G Gᵀ (what the gradient model misses) of the exact pairs
+    (feat, tau); keyword arguments go to train_closure. Returns a ClosureModel."""
+    resid = gradient_model_tau(feat, -float(sigma2), base=tau.contiguous())
+    net, fs, ts, hist = train_closure(feat, resid, **train)
+    del resid
+    return ClosureModel(net, fs, ts, sigma2, hist)
+
+
+class LearnedClosure:
+    """Π from a ClosureModel, judged against the exact Π: the first call and every `check_every`-th
+    call also compute the exact Π (compute_Π! through this library's filter) and the relative L2
+    error; above `threshold` (default 20 %) that call returns the exact Π and the closure stays on
+    the exact path until `revalidate()`."""
+
+    def __init__(self, model, grid, plan, *, threshold=0.2, check_every=16):
+        self.model = model
         self.grid, self.plan = grid, plan
         self.threshold, self.check_every = float(threshold), int(check_every)
         self.calls = 0
@@ -454,8 +520,7 @@ class LearnedClosure:
             self.fallbacks += 1
             return Pi, True
         feat, _ = closure_features(u, v, self.grid, self.plan, tau=False)
-        tau_n = predict_tau(self.net, feat, self.fs)
-        pred = pi_from_tau(feat, tau_n, self.ts)[0]
+        pred = self.model.predict_pi(feat)[0]
         if self.check_every and (self.calls - 1) % self.check_every == 0:
             exact = torch.empty_like(pred)
             self._exact(exact, u, v)

@@ -269,6 +269,52 @@ __global__ __launch_bounds__(256) void k_mse(const float *__restrict__ pred, con
     if (threadIdx.x == 0 && loss_sum) atomicAdd(loss_sum, red[0] + red[1] + red[2] + red[3]);
 }
 
+// Loss with a Π term: L = mean_τ (pred − target)² + λ mean_p (Π(pred) − Π(target))², where at each
+// pixel Π(τ) = −(a0 τxx + a1 τxy + a2 τyy) with a0 = g0·c0, a1 = (g1 + g2)·c1, a2 = g3·c2 from the
+// normalized gradients g, so that Π is in normalized units:
+// coef = (1/s_g0, 1/s_g1, 1/s_g2, 1/s_g3, c0, c1, c2): g_phys = g·coef[k], c = τ scale × Π scale.
+// One lane per pixel (all three τ channels); dpred overwritten; loss[0] += Σ τ part, loss[1] += Σ Π part.
+__global__ __launch_bounds__(256) void k_tau_pi_loss(const float *__restrict__ pred, const float *__restrict__ target,
+                                                     const float *__restrict__ feat, const float *__restrict__ coef,
+                                                     float lam, int64_t npix, int64_t total, float inv_tau,
+                                                     float inv_pix, float *__restrict__ dpred,
+                                                     float *__restrict__ loss) {
+    __shared__ float red[2][4];
+    const int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    float lt = 0.0f, lp = 0.0f;
+    if (idx < total) {
+        const int64_t img = idx / npix, p = idx - img * npix;
+        const float *pr = pred + img * 3 * npix + p;
+        const float *tg = target + img * 3 * npix + p;
+        const float *g = feat + img * 4 * npix + p;
+        const float a0 = g[0] * coef[0] * coef[4];
+        const float a1 = (g[npix] * coef[1] + g[2 * npix] * coef[2]) * coef[5];
+        const float a2 = g[3 * npix] * coef[3] * coef[6];
+        const float e0 = pr[0] - tg[0], e1 = pr[npix] - tg[npix], e2 = pr[2 * npix] - tg[2 * npix];
+        const float dpi = -(a0 * e0 + a1 * e1 + a2 * e2);
+        lt = e0 * e0 + e1 * e1 + e2 * e2;
+        lp = dpi * dpi;
+        const float kp = -2.0f * lam * dpi * inv_pix, kt = 2.0f * inv_tau;
+        float *dp = dpred + img * 3 * npix + p;
+        dp[0] = kt * e0 + kp * a0;
+        dp[npix] = kt * e1 + kp * a1;
+        dp[2 * npix] = kt * e2 + kp * a2;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        lt += __shfl_down(lt, off, 64);
+        lp += __shfl_down(lp, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = lt;
+        red[1][threadIdx.x >> 6] = lp;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && loss) {
+        atomicAdd(loss, red[0][0] + red[0][1] + red[0][2] + red[0][3]);
+        atomicAdd(loss + 1, red[1][0] + red[1][1] + red[1][2] + red[1][3]);
+    }
+}
+
 // Periodic crop: dst[i][c][y][x] = scale[c] · src[img_i][c][(y0_i + y) mod H][(x0_i + x) mod W].
 // An origin whose image index is outside [0, nsrc) yields zeros.
 template <int ST, int DT>
@@ -326,9 +372,10 @@ __global__ void k_targets(const double *__restrict__ filt, const double *__restr
     for (int c = 0; c < 4; ++c) fo[c * npts] = float(g[c * npts]);
 }
 
-// Π = −(S̄xx τxx + 2 S̄xy τxy + S̄yy τyy) from the gradients (∂xū, ∂yū, ∂xv̄, ∂yv̄) and τ·tau_scale.
+// Π = −(S̄xx τxx + 2 S̄xy τxy + S̄yy τyy) from the gradients (∂xū, ∂yū, ∂xv̄, ∂yv̄) and the stress
+// τ = tau·tau_scale + σ² G Gᵀ (G the velocity-gradient matrix; σ² = 0: τ as given).
 __global__ void k_closure_pi(const float *__restrict__ feat, const float *__restrict__ tau,
-                             const float *__restrict__ tau_scale, int64_t npts, int64_t total,
+                             const float *__restrict__ tau_scale, float sigma2, int64_t npts, int64_t total,
                              float *__restrict__ Pi) {
     const int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (idx >= total) return;
@@ -338,21 +385,26 @@ __global__ void k_closure_pi(const float *__restrict__ feat, const float *__rest
     const float s0 = tau_scale ? tau_scale[0] : 1.0f, s1 = tau_scale ? tau_scale[1] : 1.0f,
                 s2 = tau_scale ? tau_scale[2] : 1.0f;
     const float ux = g[0], uy = g[npts], vx = g[2 * npts], vy = g[3 * npts];
-    Pi[idx] = -(ux * (tt[0] * s0) + (uy + vx) * (tt[npts] * s1) + vy * (tt[2 * npts] * s2));
+    const float txx = tt[0] * s0 + sigma2 * (ux * ux + uy * uy);
+    const float txy = tt[npts] * s1 + sigma2 * (ux * vx + uy * vy);
+    const float tyy = tt[2 * npts] * s2 + sigma2 * (vx * vx + vy * vy);
+    Pi[idx] = -(ux * txx + (uy + vx) * txy + vy * tyy);
 }
 
-// Gradient (nonlinear) model τ_ab ≈ σ² Σ_k ∂_k ū_a ∂_k ū_b, the closure without learning.
-__global__ void k_gradient_model(const float *__restrict__ feat, float sigma2, int64_t npts, int64_t total,
-                                 float *__restrict__ tau) {
+// Gradient (nonlinear) model τ_ab = σ² Σ_k ∂_k ū_a ∂_k ū_b, the closure without learning, added to
+// `base` when given (base = τ and σ² < 0 gives the residual the network learns).
+__global__ void k_gradient_model(const float *__restrict__ feat, float sigma2, const float *__restrict__ base,
+                                 int64_t npts, int64_t total, float *__restrict__ tau) {
     const int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (idx >= total) return;
     const int64_t s = idx / npts, p = idx - s * npts;
     const float *g = feat + s * 4 * npts + p;
     const float ux = g[0], uy = g[npts], vx = g[2 * npts], vy = g[3 * npts];
     float *tt = tau + s * 3 * npts + p;
-    tt[0] = sigma2 * (ux * ux + uy * uy);
-    tt[npts] = sigma2 * (ux * vx + uy * vy);
-    tt[2 * npts] = sigma2 * (vx * vx + vy * vy);
+    const float *bb = base ? base + s * 3 * npts + p : nullptr;
+    tt[0] = (bb ? bb[0] : 0.0f) + sigma2 * (ux * ux + uy * uy);
+    tt[npts] = (bb ? bb[npts] : 0.0f) + sigma2 * (ux * vx + uy * vy);
+    tt[2 * npts] = (bb ? bb[2 * npts] : 0.0f) + sigma2 * (vx * vx + vy * vy);
 }
 
 template <class F>
@@ -560,6 +612,19 @@ cgef_status cgef_mse_grad(const float *pred, const float *target, int64_t count,
     });
 }
 
+cgef_status cgef_closure_loss_grad(const float *pred, const float *target, const float *feat, const float *coef,
+                                   double lambda_pi, int64_t n, int64_t npix, float *dpred, float *loss, void *stream) {
+    return guarded([&] {
+        need(pred && target && feat && coef && dpred && n >= 1 && npix >= 1, CGEF_ERR_ARGUMENT,
+             "cgef_closure_loss_grad: bad arguments");
+        const int64_t total = n * npix;
+        hipLaunchKernelGGL(k_tau_pi_loss, dim3(blocks_for(total)), dim3(256), 0, static_cast<hipStream_t>(stream), pred,
+                           target, feat, coef, float(lambda_pi), npix, total, float(1.0 / double(3 * total)),
+                           float(1.0 / double(total)), dpred, loss);
+        check_launch();
+    });
+}
+
 cgef_status cgef_closure_crop(const void *src, int32_t src_dtype, int64_t nsrc, int64_t c, int64_t H, int64_t W,
                               const int32_t *origins, int64_t n, int64_t h, int64_t w, const float *chan_scale,
                               void *dst, int32_t dst_dtype, void *stream) {
@@ -610,24 +675,24 @@ cgef_status cgef_closure_targets(const double *filt5, const double *grad4, int64
     });
 }
 
-cgef_status cgef_closure_pi(const float *feat4, const float *tau3, const float *tau_scale, int64_t npts, int64_t nb,
-                            float *Pi, void *stream) {
+cgef_status cgef_closure_pi(const float *feat4, const float *tau3, const float *tau_scale, double sigma2, int64_t npts,
+                            int64_t nb, float *Pi, void *stream) {
     return guarded([&] {
         need(feat4 && tau3 && Pi && npts >= 1 && nb >= 1, CGEF_ERR_ARGUMENT, "cgef_closure_pi: bad arguments");
         const int64_t total = npts * nb;
         hipLaunchKernelGGL(k_closure_pi, dim3(blocks_for(total)), dim3(256), 0, static_cast<hipStream_t>(stream), feat4,
-                           tau3, tau_scale, npts, total, Pi);
+                           tau3, tau_scale, float(sigma2), npts, total, Pi);
         check_launch();
     });
 }
 
-cgef_status cgef_closure_gradient_model(const float *feat4, double sigma2, int64_t npts, int64_t nb, float *tau3,
-                                        void *stream) {
+cgef_status cgef_closure_gradient_model(const float *feat4, double sigma2, const float *base3, int64_t npts,
+                                        int64_t nb, float *tau3, void *stream) {
     return guarded([&] {
         need(feat4 && tau3 && npts >= 1 && nb >= 1, CGEF_ERR_ARGUMENT, "cgef_closure_gradient_model: bad arguments");
         const int64_t total = npts * nb;
         hipLaunchKernelGGL(k_gradient_model, dim3(blocks_for(total)), dim3(256), 0, static_cast<hipStream_t>(stream),
-                           feat4, float(sigma2), npts, total, tau3);
+                           feat4, float(sigma2), base3, npts, total, tau3);
         check_launch();
     });
 }

@@ -214,6 +214,14 @@ cgef_status cgef_adam_update(float *param, float *m, float *v, const float *grad
 cgef_status cgef_mse_grad(const float *pred, const float *target, int64_t count, float *dpred, float *loss_sum,
                           void *stream);
 
+/* Closure loss with a Π term: L = mean over the n·3·npix τ values of (pred − target)² + λ·mean over
+ * the n·npix pixels of (Π(pred) − Π(target))², Π(τ) = −(a0 τxx + a1 τxy + a2 τyy) with a0 = g0·coef[0]
+ * ·coef[4], a1 = (g1·coef[1] + g2·coef[2])·coef[5], a2 = g3·coef[3]·coef[6] from feat = g (n, 4, npix)
+ * at the same pixels (coef: 7 device floats). pred, target, dpred (n, 3, npix) FP32; dpred = ∂L/∂pred.
+ * loss (2 device floats, may be NULL) += (Σ τ part, Σ Π part) before the means. */
+cgef_status cgef_closure_loss_grad(const float *pred, const float *target, const float *feat, const float *coef,
+                                   double lambda_pi, int64_t n, int64_t npix, float *dpred, float *loss, void *stream);
+
 /* Periodic crops: dst[i][c][y][x] = chan_scale[c]·src[img][c][(y0+y) mod H][(x0+x) mod W] with
  * origins[i] = (img, y0, x0) (device int32, n × 3); src FP32 (nsrc, c, H, W); dst FP32 or BF16
  * (n, c, h, w); chan_scale may be NULL. An origin whose image index is outside [0, nsrc) gives zeros. */
@@ -231,14 +239,16 @@ cgef_status cgef_closure_products(const void *u, const void *v, int32_t dtype, i
 cgef_status cgef_closure_targets(const double *filt5, const double *grad4, int64_t npts, int64_t nb, float *tau3,
                                  float *feat4, void *stream);
 
-/* Π = −(S̄xx τxx + 2 S̄xy τxy + S̄yy τyy) (Diagnostics.jl:460-466 contraction) from feat4 and
- * tau3 · tau_scale[c] (tau_scale may be NULL), FP32 (nb, npts). */
-cgef_status cgef_closure_pi(const float *feat4, const float *tau3, const float *tau_scale, int64_t npts, int64_t nb,
-                            float *Pi, void *stream);
+/* Π = −(S̄xx τxx + 2 S̄xy τxy + S̄yy τyy) (Diagnostics.jl:460-466 contraction) from feat4, with
+ * τ = tau3 · tau_scale[c] + sigma2 · G Gᵀ (tau_scale may be NULL; sigma2 = 0 uses tau3 alone; G the
+ * gradient matrix in feat4: the learned residual plus the gradient model), FP32 (nb, npts). */
+cgef_status cgef_closure_pi(const float *feat4, const float *tau3, const float *tau_scale, double sigma2, int64_t npts,
+                            int64_t nb, float *Pi, void *stream);
 
-/* The closure without learning, the gradient model τ_ab = σ² Σ_k ∂_k ū_a ∂_k ū_b. */
-cgef_status cgef_closure_gradient_model(const float *feat4, double sigma2, int64_t npts, int64_t nb, float *tau3,
-                                        void *stream);
+/* The closure without learning, the gradient model τ_ab = σ² Σ_k ∂_k ū_a ∂_k ū_b, plus base3 when
+ * it is not NULL (base3 = exact τ with σ² negated gives the residual the network learns). */
+cgef_status cgef_closure_gradient_model(const float *feat4, double sigma2, const float *base3, int64_t npts,
+                                        int64_t nb, float *tau3, void *stream);
 
 #ifdef __cplusplus
 }

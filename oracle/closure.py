@@ -14,7 +14,7 @@ and GELU(z) = z·Φ(z) (exact normal CDF). ``bf16=True`` rounds what the device 
 input, the weights, and every hidden pre-activation Z_l) to nearest even, so the device differs from
 it only by FP32 accumulation order. Gradients are those of the loss mean((Y − T)²).
 
-Pinned in tests/test_oracle.py against torch's CPU FP64 autograd (torch.nn.functional.conv2d and
+Pinned in tests/test_closure_oracle.py against torch's CPU FP64 autograd (torch.nn.functional.conv2d and
 gelu, an independent library) and against central finite differences of the FP64 forward.
 """
 from __future__ import annotations
@@ -123,3 +123,16 @@ def closure_pi(grad4, tau3):
     """Π = −(S̄xx τxx + 2 S̄xy τxy + S̄yy τyy) with S̄xy = ½(∂yū + ∂xv̄) (Diagnostics.jl:460-466)."""
     ux, uy, vx, vy = (grad4[..., c, :, :] for c in range(4))
     return -(ux * tau3[..., 0, :, :] + (uy + vx) * tau3[..., 1, :, :] + vy * tau3[..., 2, :, :])
+
+
+def tau_pi_loss(pred, target, g, coef, lam):
+    """Loss mean((pred − target)²) + lam·mean((Π(pred) − Π(target))²) and its gradient in pred, with
+    Π(τ) = −(a0 τxx + a1 τxy + a2 τyy), a0 = g0·c[0]·c[4], a1 = (g1·c[1] + g2·c[2])·c[5],
+    a2 = g3·c[3]·c[6]; pred, target (n, 3, h, w), g (n, 4, h, w)."""
+    a = np.stack([g[:, 0] * coef[0] * coef[4], (g[:, 1] * coef[1] + g[:, 2] * coef[2]) * coef[5],
+                  g[:, 3] * coef[3] * coef[6]], axis=1)
+    e = pred - target
+    dpi = -np.sum(a * e, axis=1)
+    loss = float(np.mean(e ** 2) + lam * np.mean(dpi ** 2))
+    grad = 2.0 * e / e.size - 2.0 * lam * (dpi / dpi.size)[:, None] * a
+    return loss, grad

@@ -126,3 +126,23 @@ def test_bf16_forward_stays_close_to_fp64(bf16):
     y, _, _ = OC.forward(params, x, bf16=bf16)
     rel = np.linalg.norm(y - y64) / np.linalg.norm(y64)
     assert rel <= (2e-2 if bf16 else 0.0)
+
+
+def test_tau_pi_loss_gradient_matches_central_differences():
+    rng = np.random.Generator(np.random.PCG64(8))
+    pred, target = rng.standard_normal((2, 3, 4, 5)), rng.standard_normal((2, 3, 4, 5))
+    g = rng.standard_normal((2, 4, 4, 5))
+    coef = rng.uniform(0.5, 2.0, 7)
+    loss, grad = OC.tau_pi_loss(pred, target, g, coef, 3.0)
+    # Π(τ) by its definition, independently: −S̄:τ with S̄ from the physical gradients, times the scales
+    ux, uy, vx, vy = (g[:, k] * coef[k] for k in range(4))
+    pi = lambda t: -(ux * t[:, 0] * coef[4] + (uy + vx) * t[:, 1] * coef[5] + vy * t[:, 2] * coef[6])
+    ref = np.mean((pred - target) ** 2) + 3.0 * np.mean((pi(pred) - pi(target)) ** 2)
+    assert abs(loss - ref) <= 1e-12 * ref
+    h = 1e-6
+    for idx in [(0, 0, 1, 2), (1, 1, 3, 4), (0, 2, 0, 0)]:
+        p1, p2 = pred.copy(), pred.copy()
+        p1[idx] += h
+        p2[idx] -= h
+        fd = (OC.tau_pi_loss(p1, target, g, coef, 3.0)[0] - OC.tau_pi_loss(p2, target, g, coef, 3.0)[0]) / (2 * h)
+        assert abs(fd - grad[idx]) <= 1e-7 * max(1.0, abs(fd))

@@ -60,7 +60,7 @@ def test_conv_backward_matches_oracle(channels, n, h, w):
     net.forward(xb)
     net.backward(torch.tensor(target, dtype=torch.float32, device=DEV))
     loss, grads = OC.backward(params, x, target, bf16=True)
-    assert abs(float(net.loss_sum.item()) / target.size - loss) <= 5e-3 * loss
+    assert abs(float(net.loss_sum[0].item()) / target.size - loss) <= 5e-3 * loss
     for l in range(L):
         assert rel_l2(N(net.gW[l]), grads[l][0]) <= 1e-2, l
         assert rel_l2(N(net.gb[l]), grads[l][1]) <= 1e-2, l
@@ -153,36 +153,69 @@ def test_exact_pairs_match_oracle_filter():
     gm = N(CL.gradient_model_tau(feat, ell * ell / 12.0)[0])
     assert rel_l2(gm, OC.gradient_model(fn, ell * ell / 12.0)) <= 1e-6
     sc = torch.tensor([2.0, 3.0, 0.5], device=DEV)
-    assert rel_l2(N(CL.pi_from_tau(feat, tau, sc)[0]),
-                  OC.closure_pi(fn, N(tau[0]) * np.array([2.0, 3.0, 0.5])[:, None, None])) <= 1e-6
+    scaled = N(tau[0]) * np.array([2.0, 3.0, 0.5])[:, None, None]
+    assert rel_l2(N(CL.pi_from_tau(feat, tau, sc)[0]), OC.closure_pi(fn, scaled)) <= 1e-6
+    # the residual the network learns, and Π of residual + gradient model
+    s2 = ell * ell / 12.0
+    res = N(CL.gradient_model_tau(feat, -s2, base=tau)[0])
+    assert rel_l2(res, N(tau[0]) - OC.gradient_model(fn, s2)) <= 1e-5
+    assert rel_l2(N(CL.pi_from_tau(feat, tau, sc, s2)[0]),
+                  OC.closure_pi(fn, scaled + OC.gradient_model(fn, s2))) <= 1e-5
 
 
 def test_training_learns_tau_and_fallback_uses_exact_pi():
-    """A short run at small size: the loss falls, the held-out Π correlates with the exact Π, and
-    LearnedClosure returns the exact Π when the measured error is above its threshold."""
+    """A short run at small size: the loss falls, the held-out Π of the learned closure (gradient
+    model + learned residual) is closer to the exact Π than the gradient model's, and LearnedClosure
+    returns the exact Π when the measured error is above its threshold."""
     print("closure training", flush=True)
-    torch.manual_seed(0)
     seeds = list(range(10))
     feat, tau, grid, plan = CL.closure_dataset(seeds, N=128, ell_cells=8.0, device=0)
-    net, fs, ts, hist = CL.train_closure(feat[:8], tau[:8], steps=300, batch=8, crop=32, log_every=50, seed=0)
+    s2 = CL.gradient_model_sigma2(plan.kernel, plan.scale)
+    assert s2 == pytest.approx(plan.scale ** 2 / 12.0)
+    model = CL.fit_closure(feat[:8], tau[:8], s2, steps=300, batch=8, crop=32, log_every=50, seed=0)
+    hist = model.history
     assert hist[-1][1] < 0.5 * hist[0][1], hist
-    tau_n = CL.predict_tau(net, feat[8:], fs)
-    pred = CL.pi_from_tau(feat[8:], tau_n, ts)
-    exact = CL.pi_from_tau(feat[8:], tau[8:].contiguous())
-    corr, rel = CL.pi_errors(pred, exact)
-    print(f"held-out Π: corr {corr:.3f} rel-L2 {rel:.3f}", flush=True)
-    assert corr > 0.7
+    fte, tte = feat[8:].contiguous(), tau[8:].contiguous()
+    exact = CL.pi_from_tau(fte, tte)
+    corr, rel = CL.pi_errors(model.predict_pi(fte), exact)
+    gcorr, grel = CL.pi_errors(CL.pi_from_tau(fte, CL.gradient_model_tau(fte, s2)), exact)
+    print(f"held-out Π: learned corr {corr:.3f} rel-L2 {rel:.3f}; gradient model corr {gcorr:.3f} rel-L2 {grel:.3f}",
+          flush=True)
+    assert corr > 0.9 and rel < 0.5 and rel < grel
 
     u, v, _ = SY.cartesian_turbulence(N=128, L=1.0e6, seed=20)
     ut, vt = torch.tensor(u, device=DEV), torch.tensor(v, device=DEV)
     exact_pi = torch.empty((128, 128), dtype=torch.float32, device=DEV)
     cgef.compute_pi_(exact_pi, ut.float(), vt.float(), None, grid, plan.kernel, plan.scale, filter_plan=plan)
-    strict = CL.LearnedClosure(net, fs, ts, grid, plan, threshold=1e-9)
+    strict = CL.LearnedClosure(model, grid, plan, threshold=1e-9)
     out = torch.empty_like(exact_pi)
     _, used_exact = strict.compute_pi_(out, ut, vt)
     assert used_exact and strict.disabled
     np.testing.assert_array_equal(N(out), N(exact_pi))
-    lax = CL.LearnedClosure(net, fs, ts, grid, plan, threshold=10.0)
+    _, used_exact = strict.compute_pi_(out, ut, vt)
+    assert used_exact and strict.fallbacks == 2
+    lax = CL.LearnedClosure(model, grid, plan, threshold=10.0)
     _, used_exact = lax.compute_pi_(out, ut, vt)
     assert not used_exact and lax.last_error is not None
     assert rel_l2(N(out), N(exact_pi)) == pytest.approx(lax.last_error, rel=1e-3)
+
+
+def test_tau_pi_loss_gradient_matches_oracle():
+    print("closure Π loss", flush=True)
+    rng = np.random.Generator(np.random.PCG64(12))
+    pred, target = rng.standard_normal((2, 3, 9, 13)), rng.standard_normal((2, 3, 9, 13))
+    g = rng.standard_normal((2, 4, 9, 13))
+    coef = rng.uniform(0.5, 2.0, 7)
+    dev = lambda a: torch.tensor(a, dtype=torch.float32, device=DEV)
+    pred32, target32, g32, coef32 = (a.astype(np.float32).astype(np.float64) for a in (pred, target, g, coef))
+    net = CL.ClosureNet((4, 4, 3), seed=0, device=0)   # any network: only its loss kernel is used here
+    dpred = torch.empty((2, 3, 9, 13), dtype=torch.float32, device=DEV)
+    loss = torch.zeros(2, dtype=torch.float32, device=DEV)
+    tp, tt, tg, tc = dev(pred), dev(target), dev(g), dev(coef)    # held for the call's duration
+    cgef._lib.check(CL._L().cgef_closure_loss_grad(CL._p(tp), CL._p(tt), CL._p(tg), CL._p(tc), 2.5, 2, 9 * 13,
+                                                    CL._p(dpred), CL._p(loss), CL._stream(0)))
+    ref_loss, ref_grad = OC.tau_pi_loss(pred32, target32, g32, coef32, 2.5)
+    assert rel_l2(N(dpred), ref_grad) <= 1e-6
+    ls = N(loss)
+    assert abs(ls[0] / pred.size + 2.5 * ls[1] / (pred.size / 3) - ref_loss) <= 1e-5 * ref_loss
+    del net
