@@ -54,6 +54,8 @@ def _L():
         vp, i32, i64, dbl = C.c_void_p, C.c_int32, C.c_int64, C.c_double
         sigs = {
             "cgef_conv5": [vp, i32, i32, i64, i64, i64, i64, vp, i64, vp, i64, i32, vp, i32, vp, vp],
+            "cgef_conv5_mfma": [vp, i32, i32, i64, i64, i64, vp, vp, i32, vp, i32, vp, vp],
+            "cgef_conv5_pack_mfma": [vp, vp, vp, vp],
             "cgef_conv5_wgrad_scratch": [i64, i64, i64, i64, i64, C.POINTER(i64)],
             "cgef_conv5_wgrad": [vp, vp, i32, i32, i64, i64, i64, i64, i64, vp, vp, vp, i64, vp],
             "cgef_conv5_pack_weights": [vp, i64, i64, vp, i64, vp, i64, vp],
@@ -108,6 +110,27 @@ def conv5(out, x, wf, ocs, bias, *, in_gelu=False, pad=0, zgrad=None):
     return out
 
 
+def conv5_mfma(out, x, wm, bias, *, in_gelu=False, pad=0, zgrad=None):
+    """A 64 → 64-channel 5×5 layer on the matrix cores (cgef_conv5_mfma); wm is the forward or the
+    backward-data matrix-core layout (cgef_conv5_pack_mfma)."""
+    n, cin, h, w = x.shape
+    ho, wo = h + 2 * pad - 4, w + 2 * pad - 4
+    if cin != 64 or tuple(out.shape) != (n, 64, ho, wo):
+        raise _lib.DimensionMismatch(f"matrix-core layer needs 64 channels in and out; got x {tuple(x.shape)}, "
+                                     f"out {tuple(out.shape)}")
+    if wm.numel() != 64 * 64 * 25:
+        raise _lib.DimensionMismatch("matrix-core weight layout has 64·64·25 entries")
+    if zgrad is not None and tuple(zgrad.shape) != tuple(out.shape):
+        raise _lib.DimensionMismatch("zgrad must be shaped as out")
+    for t, nm in ((out, "out"), (x, "x"), (wm, "weights")):
+        _check_dev(t, nm)
+    in_dt = BF16 if x.dtype == torch.bfloat16 else _lib.F32
+    out_dt = BF16 if out.dtype == torch.bfloat16 else _lib.F32
+    _lib.check(_L().cgef_conv5_mfma(_p(x), in_dt, int(in_gelu), n, h, w, _p(wm), _p(bias), pad, _p(out), out_dt,
+                                    _p(zgrad), _stream(x.device.index)))
+    return out
+
+
 def conv5_wgrad(dw, db, dz, a, scratch, *, in_gelu=False):
     n, cout, ho, wo = dz.shape
     cin = a.shape[1]
@@ -132,11 +155,13 @@ def wgrad_scratch_floats(n, cin, cout, ho, wo):
 class ClosureNet:
     """conv5 → GELU → conv5 → GELU → conv5 → GELU → conv5 with `channels` = (4, 64, 64, 64, 3):
     FP32 master weights and Adam moments, BF16 weight layouts and activations, FP32 accumulation.
-    The input must carry HALO·L cells of context on each side of the predicted window."""
+    The input must carry HALO·L cells of context on each side of the predicted window. With
+    mfma=True the 64 → 64-channel layers (forward and data gradient) run on the matrix cores."""
 
     def __init__(self, channels=(4, 64, 64, 64, 3), *, seed=0, device=None, lr=1e-3, betas=(0.9, 0.999),
-                 eps=1e-8):
+                 eps=1e-8, mfma=True):
         self.device = torch.cuda.current_device() if device is None else device
+        self.mfma = bool(mfma)
         self.channels = tuple(int(c) for c in channels)
         self.L = len(self.channels) - 1
         self.lr, self.betas, self.eps = float(lr), (float(betas[0]), float(betas[1])), float(eps)
@@ -144,7 +169,7 @@ class ClosureNet:
         rng = np.random.Generator(np.random.PCG64(seed))
         dev = f"cuda:{self.device}"
         self.W, self.b, self.mW, self.vW, self.mb, self.vb = [], [], [], [], [], []
-        self.wf, self.wb, self.ocs_f, self.ocs_b = [], [], [], []
+        self.wf, self.wb, self.ocs_f, self.ocs_b, self.wmf, self.wmb = [], [], [], [], [], []
         for l in range(self.L):
             ci, co = self.channels[l], self.channels[l + 1]
             gain = math.sqrt(2.0) if l < self.L - 1 else 1.0          # He init before GELU
@@ -160,6 +185,9 @@ class ClosureNet:
             self.ocs_b.append(ob)
             self.wf.append(torch.zeros(ci * 25 * of, dtype=torch.bfloat16, device=dev))
             self.wb.append(torch.zeros(co * 25 * ob, dtype=torch.bfloat16, device=dev))
+            mm = self.mfma and ci == 64 and co == 64
+            self.wmf.append(torch.zeros(64 * 64 * 25, dtype=torch.bfloat16, device=dev) if mm else None)
+            self.wmb.append(torch.zeros(64 * 64 * 25, dtype=torch.bfloat16, device=dev) if mm else None)
         self.pack()
         self._shape = None
         self.gW = [torch.zeros_like(w) for w in self.W]
@@ -176,6 +204,11 @@ class ClosureNet:
             co, ci = self.channels[l + 1], self.channels[l]
             _lib.check(_L().cgef_conv5_pack_weights(_p(self.W[l]), co, ci, _p(self.wf[l]), self.ocs_f[l],
                                                     _p(self.wb[l]), self.ocs_b[l], _stream(self.device)))
+            self._pack_mfma(l)
+
+    def _pack_mfma(self, l):
+        if self.wmf[l] is not None:
+            _lib.check(_L().cgef_conv5_pack_mfma(_p(self.W[l]), _p(self.wmf[l]), _p(self.wmb[l]), _stream(self.device)))
 
     def load_state(self, weights, biases):
         for l in range(self.L):
@@ -216,7 +249,10 @@ class ClosureNet:
         self.x = x
         a, gel = x, False
         for l in range(self.L):
-            conv5(self.Z[l], a, self.wf[l], self.ocs_f[l], self.b[l], in_gelu=gel, pad=0)
+            if self.wmf[l] is not None:
+                conv5_mfma(self.Z[l], a, self.wmf[l], self.b[l], in_gelu=gel, pad=0)
+            else:
+                conv5(self.Z[l], a, self.wf[l], self.ocs_f[l], self.b[l], in_gelu=gel, pad=0)
             a, gel = self.Z[l], True
         return self.Z[-1]
 
@@ -247,7 +283,9 @@ class ClosureNet:
         for l in range(self.L - 1, -1, -1):
             a, gel = (self.x, False) if l == 0 else (self.Z[l - 1], True)
             conv5_wgrad(self.gW[l], self.gb[l], self.dZ[l], a, self.scratch, in_gelu=gel)
-            if l > 0:
+            if l > 0 and self.wmb[l] is not None:
+                conv5_mfma(self.dZ[l - 1], self.dZ[l], self.wmb[l], None, pad=4, zgrad=self.Z[l - 1])
+            elif l > 0:
                 conv5(self.dZ[l - 1], self.dZ[l], self.wb[l], self.ocs_b[l], None, pad=4, zgrad=self.Z[l - 1])
 
     def step(self):
@@ -259,6 +297,7 @@ class ClosureNet:
                                              self.W[l].numel(), self.lr, b1, b2, self.eps, self.t, co, ci,
                                              _p(self.wf[l]), self.ocs_f[l], _p(self.wb[l]), self.ocs_b[l],
                                              _stream(self.device)))
+            self._pack_mfma(l)
             _lib.check(_L().cgef_adam_update(_p(self.b[l]), _p(self.mb[l]), _p(self.vb[l]), _p(self.gb[l]),
                                              co, self.lr, b1, b2, self.eps, self.t, co, 1, None, 0, None, 0,
                                              _stream(self.device)))

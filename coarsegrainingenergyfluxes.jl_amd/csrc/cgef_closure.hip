@@ -124,6 +124,102 @@ __global__ __launch_bounds__(256) void k_conv5(const void *__restrict__ in, cons
     }
 }
 
+// The 64 → 64-channel layers on the matrix cores: an implicit GEMM, output pixels × output channels
+// over K = 25 taps × 64 input channels, with v_mfma_f32_16x16x32_bf16. The block's 20×20 input tile
+// is staged in LDS channel-fastest ([pixel][channel], pixel stride MPS), so the 8 consecutive K values
+// a lane supplies (8 channels of one tap at one pixel) are one 16-byte LDS read; weights are laid out
+// [tap][out][in] for the same reason. Wave w computes output rows 4w..4w+3 of the 16×16 tile (four
+// 16-pixel M tiles) × all 64 output channels (four N tiles): 16 accumulators of 4 floats per lane.
+// Lane layout of the 16×16 MFMA: A row = lane % 16, B column = lane % 16, both supplying the K slice
+// of lane group lane / 16 (the same slice for A and B, so the product does not depend on which K
+// values a group holds); result rows 4·(lane / 16) + i, column lane % 16.
+constexpr int MCH = 64;
+constexpr int MPS = MCH + 8;
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+template <int IN_T, int IN_GELU, int OUT>
+__global__ __launch_bounds__(256) void k_conv5_mfma(const void *__restrict__ in, const uint16_t *__restrict__ wm,
+                                                    const float *__restrict__ bias, const uint16_t *__restrict__ zg,
+                                                    void *__restrict__ out, int64_t hin, int64_t win, int64_t hout,
+                                                    int64_t wout, int pad, int tiles_x) {
+    __shared__ __attribute__((aligned(16))) uint16_t xs[CS * CS * MPS];
+    const int t = threadIdx.x, wave = t >> 6, lane = t & 63, g = lane >> 4, r16 = lane & 15;
+    const int64_t ox0 = int64_t(blockIdx.x % tiles_x) * CT, oy0 = int64_t(blockIdx.x / tiles_x) * CT;
+    const int64_t img = blockIdx.z;
+    const int64_t iy0 = oy0 - pad, ix0 = ox0 - pad;
+    for (int e = t; e < MCH * CS * CS; e += 256) {
+        const int c = e / (CS * CS), r = e - c * (CS * CS);
+        const int yy = r / CS, xx = r - yy * CS;
+        const int64_t iy = iy0 + yy, ix = ix0 + xx;
+        float val = 0.0f;
+        if (iy >= 0 && iy < hin && ix >= 0 && ix < win) {
+            val = load_as_float<IN_T>(in, ((img * MCH + c) * hin + iy) * win + ix);
+            if constexpr (IN_GELU) val = gelu(val);
+        }
+        xs[r * MPS + c] = f2bf(val);
+    }
+    __syncthreads();
+
+    f32x4_t acc[4][4];
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) acc[mt][nt] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+    const uint16_t *xw = xs + (wave * 4 * CS + r16) * MPS + g * 8;
+    const uint16_t *ww = wm + r16 * MCH + g * 8;
+    for (int tap = 0; tap < 25; ++tap) {
+        const int ky = tap / 5, kx = tap - 5 * ky;
+#pragma unroll
+        for (int half = 0; half < 2; ++half) {
+            bf16x8_t a[4], b[4];
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt)
+                a[mt] = *reinterpret_cast<const bf16x8_t *>(xw + ((mt + ky) * CS + kx) * MPS + half * 32);
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt)
+                b[nt] = *reinterpret_cast<const bf16x8_t *>(ww + (tap * MCH + nt * 16) * MCH + half * 32);
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+                for (int nt = 0; nt < 4; ++nt)
+                    acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mt], b[nt], acc[mt][nt], 0, 0, 0);
+        }
+    }
+
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+        const int64_t py = oy0 + wave * 4 + mt;
+        if (py >= hout) continue;
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) {
+            const int64_t co = nt * 16 + r16;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int64_t px = ox0 + 4 * g + i;
+                if (px >= wout) continue;
+                const int64_t idx = ((img * MCH + co) * hout + py) * wout + px;
+                const float v = acc[mt][nt][i];
+                if constexpr (OUT == 0) static_cast<uint16_t *>(out)[idx] = f2bf(v + bias[co]);
+                else if constexpr (OUT == 1) static_cast<float *>(out)[idx] = v + bias[co];
+                else static_cast<float *>(out)[idx] = v * gelu_grad(bf2f(zg[idx]));
+            }
+        }
+    }
+}
+
+// BF16 matrix-core layouts of a 64 × 64 × 25 master weight tensor: forward wmf[k][o][i] and
+// backward-data wmb[24 − k][i][o].
+__global__ void k_pack_mfma(const float *__restrict__ p, uint16_t *wmf, uint16_t *wmb) {
+    const int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= MCH * MCH * 25) return;
+    const int64_t k = idx % 25, oi = idx / 25, i = oi % MCH, o = oi / MCH;
+    const uint16_t h = f2bf(p[idx]);
+    if (wmf) wmf[(k * MCH + o) * MCH + i] = h;
+    if (wmb) wmb[((24 - k) * MCH + i) * MCH + o] = h;
+}
+
 // Weight and bias gradients of a valid 5×5 convolution. Block = 16 output × 16 input channels (one
 // (o, i) pair per lane, its 25 taps in registers) over a range of 16×16 output tiles of one image;
 // each tile stages dZ (16 channels × 256 pixels) and the input (16 channels × 20×20) in LDS, and a
@@ -525,6 +621,63 @@ cgef_status cgef_conv5(const void *in, int32_t in_dtype, int32_t in_gelu, int64_
         else
             conv_in<64>(in_dtype, in_gelu, out_kind, g, st, in, w, bias, zgrad, out, cin, hin, win, cout, ocs, hout,
                         wout, pad, tiles_x);
+        check_launch();
+    });
+}
+
+cgef_status cgef_conv5_mfma(const void *in, int32_t in_dtype, int32_t in_gelu, int64_t n, int64_t hin, int64_t win,
+                            const uint16_t *wm, const float *bias, int32_t pad, void *out, int32_t out_dtype,
+                            const uint16_t *zgrad, void *stream) {
+    return guarded([&] {
+        need(in && wm && out, CGEF_ERR_ARGUMENT, "cgef_conv5_mfma: null pointer");
+        need(in_dtype == CGEF_BF16 || (in_dtype == CGEF_F32 && !in_gelu), CGEF_ERR_ARGUMENT,
+             "cgef_conv5_mfma: input must be BF16 (optionally GELU-applied) or FP32");
+        need(pad == 0 || pad == 4, CGEF_ERR_ARGUMENT, "cgef_conv5_mfma: pad must be 0 (valid) or 4 (full)");
+        need(n >= 1 && n <= 65535, CGEF_ERR_DIMENSION, "cgef_conv5_mfma: bad n");
+        const int64_t hout = hin + 2 * pad - 4, wout = win + 2 * pad - 4;
+        need(hin >= 1 && win >= 1 && hout >= 1 && wout >= 1, CGEF_ERR_DIMENSION,
+             "cgef_conv5_mfma: input smaller than 5x5");
+        int out_kind;
+        if (zgrad) {
+            need(out_dtype == CGEF_F32, CGEF_ERR_ARGUMENT, "cgef_conv5_mfma: GELU'-scaled output is FP32");
+            out_kind = 2;
+        } else {
+            need(bias != nullptr, CGEF_ERR_ARGUMENT, "cgef_conv5_mfma: bias required without zgrad");
+            need(out_dtype == CGEF_BF16 || out_dtype == CGEF_F32, CGEF_ERR_ARGUMENT,
+                 "cgef_conv5_mfma: output BF16 or FP32");
+            out_kind = out_dtype == CGEF_BF16 ? 0 : 1;
+        }
+        const int tiles_x = int((wout + CT - 1) / CT);
+        const int64_t ntiles = int64_t(tiles_x) * ((hout + CT - 1) / CT);
+        need(ntiles <= 0x7fffffff, CGEF_ERR_DIMENSION, "cgef_conv5_mfma: too many tiles");
+        dim3 g(unsigned(ntiles), 1, unsigned(n));
+        hipStream_t st = static_cast<hipStream_t>(stream);
+#define CGEF_MFMA_LAUNCH(IT, IG, OK)                                                                                  \
+    hipLaunchKernelGGL((k_conv5_mfma<IT, IG, OK>), g, dim3(256), 0, st, in, wm, bias, zgrad, out, hin, win, hout, wout, \
+                       pad, tiles_x)
+        if (in_dtype == CGEF_BF16 && in_gelu) {
+            if (out_kind == 0) CGEF_MFMA_LAUNCH(CGEF_BF16, 1, 0);
+            else if (out_kind == 1) CGEF_MFMA_LAUNCH(CGEF_BF16, 1, 1);
+            else CGEF_MFMA_LAUNCH(CGEF_BF16, 1, 2);
+        } else if (in_dtype == CGEF_BF16) {
+            if (out_kind == 0) CGEF_MFMA_LAUNCH(CGEF_BF16, 0, 0);
+            else if (out_kind == 1) CGEF_MFMA_LAUNCH(CGEF_BF16, 0, 1);
+            else CGEF_MFMA_LAUNCH(CGEF_BF16, 0, 2);
+        } else {
+            if (out_kind == 0) CGEF_MFMA_LAUNCH(CGEF_F32, 0, 0);
+            else if (out_kind == 1) CGEF_MFMA_LAUNCH(CGEF_F32, 0, 1);
+            else CGEF_MFMA_LAUNCH(CGEF_F32, 0, 2);
+        }
+#undef CGEF_MFMA_LAUNCH
+        check_launch();
+    });
+}
+
+cgef_status cgef_conv5_pack_mfma(const float *w, uint16_t *wmf, uint16_t *wmb, void *stream) {
+    return guarded([&] {
+        need(w && (wmf || wmb), CGEF_ERR_ARGUMENT, "cgef_conv5_pack_mfma: null pointer");
+        hipLaunchKernelGGL(k_pack_mfma, dim3(blocks_for(MCH * MCH * 25)), dim3(256), 0,
+                           static_cast<hipStream_t>(stream), w, wmf, wmb);
         check_launch();
     });
 }

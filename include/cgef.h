@@ -186,6 +186,18 @@ cgef_status cgef_conv5(const void *in, int32_t in_dtype, int32_t in_gelu, int64_
                        int64_t win, const uint16_t *w, int64_t ocs, const float *bias, int64_t cout, int32_t pad,
                        void *out, int32_t out_dtype, const uint16_t *zgrad, void *stream);
 
+/* The same layer for 64 input and 64 output channels on the matrix cores (BF16 MFMA, FP32
+ * accumulation; an FP32 input is rounded to BF16 as it is staged). wm is the layout written by
+ * cgef_conv5_pack_mfma: forward [25][64 out][64 in], or backward-data [25][64][64] for pad 4.
+ * Other arguments as in cgef_conv5. */
+cgef_status cgef_conv5_mfma(const void *in, int32_t in_dtype, int32_t in_gelu, int64_t n, int64_t hin, int64_t win,
+                            const uint16_t *wm, const float *bias, int32_t pad, void *out, int32_t out_dtype,
+                            const uint16_t *zgrad, void *stream);
+
+/* BF16 matrix-core layouts of a 64 × 64 master weight tensor W[o][i][25]: forward wmf[k][o][i] and
+ * backward-data wmb[24 − k][i][o] (either may be NULL), 64·64·25 entries each. */
+cgef_status cgef_conv5_pack_mfma(const float *w, uint16_t *wmf, uint16_t *wmb, void *stream);
+
 /* Weight and bias gradients of a valid 5×5 layer: dw[o][i][k] = Σ dz[n][o][p]·A[n][i][p+k],
  * db[o] = Σ dz[n][o][p], with A the layer input (n, cin, hout+4, wout+4) as in cgef_conv5 and dz
  * FP32 (n, cout, hout, wout). Deterministic: per-block partial sums go to `scratch` (at least the

@@ -25,8 +25,8 @@ def N(t):
     return t.detach().float().cpu().numpy().astype(np.float64)
 
 
-def _net_and_input(channels, n, h, w, seed):
-    net = CL.ClosureNet(channels, seed=seed, device=0)
+def _net_and_input(channels, n, h, w, seed, mfma=True):
+    net = CL.ClosureNet(channels, seed=seed, device=0, mfma=mfma)
     # non-zero biases so the bias path is exercised
     rng = np.random.Generator(np.random.PCG64(seed + 100))
     Ws, bs = net.state()
@@ -38,10 +38,13 @@ def _net_and_input(channels, n, h, w, seed):
     return net, xb, x, params
 
 
-@pytest.mark.parametrize("channels,n,h,w", [((4, 64, 64, 64, 3), 2, 40, 37), ((4, 8, 20, 3), 3, 23, 50)])
-def test_conv_forward_matches_oracle(channels, n, h, w):
-    print(f"closure forward {channels} n={n} {h}x{w}", flush=True)
-    net, xb, x, params = _net_and_input(channels, n, h, w, 1)
+CASES = [((4, 64, 64, 64, 3), 2, 40, 37, True), ((4, 64, 64, 64, 3), 2, 40, 37, False), ((4, 8, 20, 3), 3, 23, 50, True)]
+
+
+@pytest.mark.parametrize("channels,n,h,w,mfma", CASES)
+def test_conv_forward_matches_oracle(channels, n, h, w, mfma):
+    print(f"closure forward {channels} n={n} {h}x{w} mfma={mfma}", flush=True)
+    net, xb, x, params = _net_and_input(channels, n, h, w, 1, mfma)
     y = net.forward(xb)
     yo, zs, _ = OC.forward(params, x, bf16=True)
     assert rel_l2(N(y), yo) <= 5e-3
@@ -49,10 +52,10 @@ def test_conv_forward_matches_oracle(channels, n, h, w):
         assert rel_l2(N(net.Z[l]), z) <= 5e-3, l
 
 
-@pytest.mark.parametrize("channels,n,h,w", [((4, 64, 64, 64, 3), 2, 40, 37), ((4, 8, 20, 3), 3, 23, 50)])
-def test_conv_backward_matches_oracle(channels, n, h, w):
-    print(f"closure backward {channels} n={n} {h}x{w}", flush=True)
-    net, xb, x, params = _net_and_input(channels, n, h, w, 2)
+@pytest.mark.parametrize("channels,n,h,w,mfma", CASES)
+def test_conv_backward_matches_oracle(channels, n, h, w, mfma):
+    print(f"closure backward {channels} n={n} {h}x{w} mfma={mfma}", flush=True)
+    net, xb, x, params = _net_and_input(channels, n, h, w, 2, mfma)
     L = len(channels) - 1
     rng = np.random.Generator(np.random.PCG64(9))
     target = rng.standard_normal((n, channels[-1], h - 4 * L, w - 4 * L))
