@@ -128,7 +128,8 @@ __global__ __launch_bounds__(256) void k_conv5(const void *__restrict__ in, cons
 // over K = 25 taps × 64 input channels, with v_mfma_f32_16x16x32_bf16. The block's 20×20 input tile
 // is staged in LDS channel-fastest ([pixel][channel], pixel stride MPS), so the 8 consecutive K values
 // a lane supplies (8 channels of one tap at one pixel) are one 16-byte LDS read; weights are laid out
-// [tap][out][in] for the same reason. Wave w computes output rows 4w..4w+3 of the 16×16 tile (four
+// [tap][out][in] for the same reason, and each tap's 64 × 64 block passes through a double-buffered
+// LDS slab (the next tap's block is loaded into registers while the current one is multiplied). Wave w computes output rows 4w..4w+3 of the 16×16 tile (four
 // 16-pixel M tiles) × all 64 output channels (four N tiles): 16 accumulators of 4 floats per lane.
 // Lane layout of the 16×16 MFMA: A row = lane % 16, B column = lane % 16, both supplying the K slice
 // of lane group lane / 16 (the same slice for A and B, so the product does not depend on which K
@@ -145,10 +146,18 @@ __global__ __launch_bounds__(256) void k_conv5_mfma(const void *__restrict__ in,
                                                     void *__restrict__ out, int64_t hin, int64_t win, int64_t hout,
                                                     int64_t wout, int pad, int tiles_x) {
     __shared__ __attribute__((aligned(16))) uint16_t xs[CS * CS * MPS];
+    __shared__ __attribute__((aligned(16))) uint16_t wsm[2][MCH * MPS];
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63, g = lane >> 4, r16 = lane & 15;
     const int64_t ox0 = int64_t(blockIdx.x % tiles_x) * CT, oy0 = int64_t(blockIdx.x / tiles_x) * CT;
     const int64_t img = blockIdx.z;
     const int64_t iy0 = oy0 - pad, ix0 = ox0 - pad;
+    // each lane moves 16 weights (32 bytes) of a tap's 64 × 64 block: row t / 4, columns 16·(t % 4)…
+    const uint4 *wsrc = reinterpret_cast<const uint4 *>(wm) + 2 * t;
+    uint16_t *wdst0 = &wsm[0][(t >> 2) * MPS + (t & 3) * 16];
+    uint16_t *wdst1 = &wsm[1][(t >> 2) * MPS + (t & 3) * 16];
+    uint4 wr0 = wsrc[0], wr1 = wsrc[1];
+    reinterpret_cast<uint4 *>(wdst0)[0] = wr0;
+    reinterpret_cast<uint4 *>(wdst0)[1] = wr1;
     for (int e = t; e < MCH * CS * CS; e += 256) {
         const int c = e / (CS * CS), r = e - c * (CS * CS);
         const int yy = r / CS, xx = r - yy * CS;
@@ -168,9 +177,14 @@ __global__ __launch_bounds__(256) void k_conv5_mfma(const void *__restrict__ in,
 #pragma unroll
         for (int nt = 0; nt < 4; ++nt) acc[mt][nt] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
     const uint16_t *xw = xs + (wave * 4 * CS + r16) * MPS + g * 8;
-    const uint16_t *ww = wm + r16 * MCH + g * 8;
     for (int tap = 0; tap < 25; ++tap) {
         const int ky = tap / 5, kx = tap - 5 * ky;
+        const bool more = tap + 1 < 25;
+        if (more) {   // next tap's weights in flight during this tap's products
+            wr0 = wsrc[(tap + 1) * (MCH * MCH / 8)];
+            wr1 = wsrc[(tap + 1) * (MCH * MCH / 8) + 1];
+        }
+        const uint16_t *wt = &wsm[tap & 1][r16 * MPS + g * 8];
 #pragma unroll
         for (int half = 0; half < 2; ++half) {
             bf16x8_t a[4], b[4];
@@ -179,13 +193,19 @@ __global__ __launch_bounds__(256) void k_conv5_mfma(const void *__restrict__ in,
                 a[mt] = *reinterpret_cast<const bf16x8_t *>(xw + ((mt + ky) * CS + kx) * MPS + half * 32);
 #pragma unroll
             for (int nt = 0; nt < 4; ++nt)
-                b[nt] = *reinterpret_cast<const bf16x8_t *>(ww + (tap * MCH + nt * 16) * MCH + half * 32);
+                b[nt] = *reinterpret_cast<const bf16x8_t *>(wt + nt * 16 * MPS + half * 32);
 #pragma unroll
             for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
                 for (int nt = 0; nt < 4; ++nt)
                     acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mt], b[nt], acc[mt][nt], 0, 0, 0);
         }
+        if (more) {
+            uint16_t *wd = (tap & 1) ? wdst0 : wdst1;
+            reinterpret_cast<uint4 *>(wd)[0] = wr0;
+            reinterpret_cast<uint4 *>(wd)[1] = wr1;
+        }
+        __syncthreads();
     }
 
 #pragma unroll
