@@ -53,11 +53,11 @@ def _L():
     if not _sig_done:
         vp, i32, i64, dbl = C.c_void_p, C.c_int32, C.c_int64, C.c_double
         sigs = {
-            "cgef_conv5": [vp, i32, i32, i64, i64, i64, i64, vp, i64, vp, i64, i32, vp, i32, vp, vp],
-            "cgef_conv5_mfma": [vp, i32, i32, i64, i64, i64, vp, vp, i32, vp, i32, vp, vp],
+            "cgef_conv5": [vp, i32, i32, i64, i64, i64, i64, vp, i64, vp, i64, i32, vp, i32, vp, vp, vp],
+            "cgef_conv5_mfma": [vp, i32, i32, i64, i64, i64, vp, vp, i32, vp, i32, vp, vp, vp],
             "cgef_conv5_pack_mfma": [vp, vp, vp, vp],
-            "cgef_conv5_wgrad_scratch": [i64, i64, i64, i64, i64, C.POINTER(i64)],
-            "cgef_conv5_wgrad": [vp, vp, i32, i32, i64, i64, i64, i64, i64, vp, vp, vp, i64, vp],
+            "cgef_conv5_wgrad_scratch": [i64, i64, i64, i64, i64, i32, C.POINTER(i64)],
+            "cgef_conv5_wgrad": [vp, vp, i32, i32, i64, i64, i64, i64, i64, i32, vp, vp, vp, i64, vp],
             "cgef_conv5_pack_weights": [vp, i64, i64, vp, i64, vp, i64, vp],
             "cgef_adam_update": [vp, vp, vp, vp, i64, dbl, dbl, dbl, dbl, i64, i64, i64, vp, i64, vp, i64, vp],
             "cgef_mse_grad": [vp, vp, i64, vp, vp, vp],
@@ -89,7 +89,7 @@ def _check_dev(t, name, dtype=None):
 # Layer-level entry points (thin wrappers over the C ABI)
 # ------------------------------------------------------------------------------------------------
 
-def conv5(out, x, wf, ocs, bias, *, in_gelu=False, pad=0, zgrad=None):
+def conv5(out, x, wf, ocs, bias, *, in_gelu=False, pad=0, zgrad=None, act=None):
     """One 5×5 layer into `out` (see cgef_conv5). x BF16 or FP32 (n, cin, h, w); wf BF16 forward
     layout [cin][25][ocs] (or the backward-data layout for pad=4)."""
     n, cin, h, w = x.shape
@@ -105,12 +105,14 @@ def conv5(out, x, wf, ocs, bias, *, in_gelu=False, pad=0, zgrad=None):
         _check_dev(t, nm)
     in_dt = BF16 if x.dtype == torch.bfloat16 else _lib.F32
     out_dt = BF16 if out.dtype == torch.bfloat16 else _lib.F32
+    if act is not None and (tuple(act.shape) != tuple(out.shape) or act.dtype != torch.bfloat16):
+        raise _lib.DimensionMismatch("act must be a BF16 tensor shaped as out")
     _lib.check(_L().cgef_conv5(_p(x), in_dt, int(in_gelu), n, cin, h, w, _p(wf), ocs, _p(bias), cout, pad, _p(out),
-                               out_dt, _p(zgrad), _stream(x.device.index)))
+                               out_dt, _p(zgrad), _p(act), _stream(x.device.index)))
     return out
 
 
-def conv5_mfma(out, x, wm, bias, *, in_gelu=False, pad=0, zgrad=None):
+def conv5_mfma(out, x, wm, bias, *, in_gelu=False, pad=0, zgrad=None, act=None):
     """A 64 → 64-channel 5×5 layer on the matrix cores (cgef_conv5_mfma); wm is the forward or the
     backward-data matrix-core layout (cgef_conv5_pack_mfma)."""
     n, cin, h, w = x.shape
@@ -126,25 +128,27 @@ def conv5_mfma(out, x, wm, bias, *, in_gelu=False, pad=0, zgrad=None):
         _check_dev(t, nm)
     in_dt = BF16 if x.dtype == torch.bfloat16 else _lib.F32
     out_dt = BF16 if out.dtype == torch.bfloat16 else _lib.F32
+    if act is not None and (tuple(act.shape) != tuple(out.shape) or act.dtype != torch.bfloat16):
+        raise _lib.DimensionMismatch("act must be a BF16 tensor shaped as out")
     _lib.check(_L().cgef_conv5_mfma(_p(x), in_dt, int(in_gelu), n, h, w, _p(wm), _p(bias), pad, _p(out), out_dt,
-                                    _p(zgrad), _stream(x.device.index)))
+                                    _p(zgrad), _p(act), _stream(x.device.index)))
     return out
 
 
-def conv5_wgrad(dw, db, dz, a, scratch, *, in_gelu=False):
+def conv5_wgrad(dw, db, dz, a, scratch, *, in_gelu=False, mfma=False):
     n, cout, ho, wo = dz.shape
     cin = a.shape[1]
     if tuple(a.shape) != (n, cin, ho + 4, wo + 4):
         raise _lib.DimensionMismatch("layer input must be (n, cin, hout+4, wout+4)")
     in_dt = BF16 if a.dtype == torch.bfloat16 else _lib.F32
-    _lib.check(_L().cgef_conv5_wgrad(_p(dz), _p(a), in_dt, int(in_gelu), n, cin, cout, ho, wo, _p(dw), _p(db),
-                                     _p(scratch), scratch.numel(), _stream(dz.device.index)))
+    _lib.check(_L().cgef_conv5_wgrad(_p(dz), _p(a), in_dt, int(in_gelu), n, cin, cout, ho, wo, int(mfma), _p(dw),
+                                     _p(db), _p(scratch), scratch.numel(), _stream(dz.device.index)))
     return dw, db
 
 
-def wgrad_scratch_floats(n, cin, cout, ho, wo):
+def wgrad_scratch_floats(n, cin, cout, ho, wo, mfma=False):
     out = C.c_int64()
-    _lib.check(_L().cgef_conv5_wgrad_scratch(n, cin, cout, ho, wo, C.byref(out)))
+    _lib.check(_L().cgef_conv5_wgrad_scratch(n, cin, cout, ho, wo, int(mfma), C.byref(out)))
     return int(out.value)
 
 
@@ -223,19 +227,21 @@ class ClosureNet:
         if self._shape == (n, h, w):
             return
         dev = f"cuda:{self.device}"
-        self.Z, self.dZ = [], []
+        self.Z, self.A, self.dZ = [], [], []
         for l in range(self.L):
             co = self.channels[l + 1]
             ho, wo = h - 4 * (l + 1), w - 4 * (l + 1)
             if ho < 1 or wo < 1:
                 raise _lib.DimensionMismatch(f"input {h}x{w} is too small for {self.L} valid 5x5 layers")
-            dt = torch.float32 if l == self.L - 1 else torch.bfloat16
-            self.Z.append(torch.empty((n, co, ho, wo), dtype=dt, device=dev))
+            last = l == self.L - 1
+            self.Z.append(torch.empty((n, co, ho, wo), dtype=torch.float32 if last else torch.bfloat16, device=dev))
+            self.A.append(None if last else torch.empty((n, co, ho, wo), dtype=torch.bfloat16, device=dev))
             self.dZ.append(torch.empty((n, co, ho, wo), dtype=torch.float32, device=dev))
         need = 0
         for l in range(self.L):
             ho, wo = h - 4 * (l + 1), w - 4 * (l + 1)
-            need = max(need, wgrad_scratch_floats(n, self.channels[l], self.channels[l + 1], ho, wo))
+            need = max(need, wgrad_scratch_floats(n, self.channels[l], self.channels[l + 1], ho, wo,
+                                                  self.wmf[l] is not None))
         self.scratch = torch.empty(need, dtype=torch.float32, device=dev)
         self._shape = (n, h, w)
 
@@ -247,13 +253,13 @@ class ClosureNet:
             raise _lib.DimensionMismatch(f"x has {c0} channels, the network takes {self.channels[0]}")
         self._buffers(n, h, w)
         self.x = x
-        a, gel = x, False
+        a = x
         for l in range(self.L):
             if self.wmf[l] is not None:
-                conv5_mfma(self.Z[l], a, self.wmf[l], self.b[l], in_gelu=gel, pad=0)
+                conv5_mfma(self.Z[l], a, self.wmf[l], self.b[l], pad=0, act=self.A[l])
             else:
-                conv5(self.Z[l], a, self.wf[l], self.ocs_f[l], self.b[l], in_gelu=gel, pad=0)
-            a, gel = self.Z[l], True
+                conv5(self.Z[l], a, self.wf[l], self.ocs_f[l], self.b[l], pad=0, act=self.A[l])
+            a = self.A[l]
         return self.Z[-1]
 
     def backward(self, target, *, accumulate_loss=True, feat=None, coef=None, lambda_pi=0.0):
@@ -281,8 +287,8 @@ class ClosureNet:
         if dy.data_ptr() != self.dZ[-1].data_ptr():
             self.dZ[-1].copy_(dy)
         for l in range(self.L - 1, -1, -1):
-            a, gel = (self.x, False) if l == 0 else (self.Z[l - 1], True)
-            conv5_wgrad(self.gW[l], self.gb[l], self.dZ[l], a, self.scratch, in_gelu=gel)
+            a = self.x if l == 0 else self.A[l - 1]
+            conv5_wgrad(self.gW[l], self.gb[l], self.dZ[l], a, self.scratch, mfma=self.wmf[l] is not None)
             if l > 0 and self.wmb[l] is not None:
                 conv5_mfma(self.dZ[l - 1], self.dZ[l], self.wmb[l], None, pad=4, zgrad=self.Z[l - 1])
             elif l > 0:

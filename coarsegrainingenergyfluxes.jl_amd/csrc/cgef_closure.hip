@@ -2,10 +2,11 @@
 // hand-written kernels (no reference counterpart: the reference computes τ only by the exact filter).
 //
 // Storage: activations and weights BF16, accumulation FP32, optimizer state and master weights FP32.
-// Tensors are (n, channels, rows, cols) with cols fastest. A layer stores its pre-activation Z in
-// BF16; the next layer applies GELU while staging Z into LDS, so the activation is never written.
+// Tensors are (n, channels, rows, cols) with cols fastest. A hidden layer's epilogue stores its
+// pre-activation Z and its activation A = GELU(Z), both BF16: GELU is evaluated once per element,
+// the next layer and the weight gradient read A, and the data gradient reads Z for GELU'.
 //
-//   forward        Z_l = conv5_valid(GELU(Z_{l-1}), W_l) + b_l          k_conv5, pad 0
+//   forward        Z_l = conv5_valid(A_{l-1}, W_l) + b_l, A_l = GELU(Z_l)   k_conv5(_mfma), pad 0
 //   backward data  dZ_{l-1} = conv5_full(dZ_l, flip(W_l)ᵀ) · GELU'(Z_{l-1})  k_conv5, pad 4
 //   backward W     dW_l[o][i][k] = Σ_{n,p} dZ_l[n][o][p] · A_{l-1}[n][i][p+k]  k_conv5_wgrad
 //
@@ -41,6 +42,10 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
     return uint16_t(u >> 16);
 }
 
+__device__ __forceinline__ uint32_t pack_bf2(float a, float b) {
+    return uint32_t(f2bf(a)) | (uint32_t(f2bf(b)) << 16);
+}
+
 // GELU(x) = x·Φ(x) with the exact normal CDF, and its derivative Φ(x) + x·φ(x).
 __device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.7071067811865476f)); }
 __device__ __forceinline__ float gelu_grad(float x) {
@@ -61,7 +66,8 @@ __device__ __forceinline__ float load_as_float(const void *p, int64_t i) {
 template <int OCB, int IN_T, int IN_GELU, int OUT>
 __global__ __launch_bounds__(256) void k_conv5(const void *__restrict__ in, const uint16_t *__restrict__ w,
                                                const float *__restrict__ bias, const uint16_t *__restrict__ zg,
-                                               void *__restrict__ out, int64_t cin, int64_t hin, int64_t win,
+                                               void *__restrict__ out, uint16_t *__restrict__ act, int64_t cin,
+                                               int64_t hin, int64_t win,
                                                int64_t cout, int64_t ocs, int64_t hout, int64_t wout, int pad,
                                                int tiles_x) {
     constexpr int KUNROLL = OCB >= 64 ? 1 : 25;
@@ -77,16 +83,22 @@ __global__ __launch_bounds__(256) void k_conv5(const void *__restrict__ in, cons
     for (int64_t c0 = 0; c0 < cin; c0 += ICC) {
         const int nc = int(cin - c0 < ICC ? cin - c0 : ICC);
         __syncthreads();
-        for (int e = t; e < ICC * CS * CS; e += 256) {
-            const int c = e / (CS * CS), r = e - c * (CS * CS);
-            const int yy = r / CS, xx = r - yy * CS;
-            const int64_t iy = iy0 + yy, ix = ix0 + xx;
-            float val = 0.0f;
-            if (c < nc && iy >= 0 && iy < hin && ix >= 0 && ix < win) {
-                val = load_as_float<IN_T>(in, ((img * cin + c0 + c) * hin + iy) * win + ix);
-                if constexpr (IN_GELU) val = gelu(val);
+        {
+            const int64_t plane = hin * win;
+            for (int r = t; r < CS * CS; r += 256) {
+                const int yy = r / CS, xx = r - yy * CS;
+                const int64_t iy = iy0 + yy, ix = ix0 + xx;
+                const bool ok = iy >= 0 && iy < hin && ix >= 0 && ix < win;
+                const int64_t off = ok ? (img * cin + c0) * plane + iy * win + ix : 0;
+                float v[ICC];
+#pragma unroll
+                for (int c = 0; c < ICC; ++c) {
+                    v[c] = (ok && c < nc) ? load_as_float<IN_T>(in, off + c * plane) : 0.0f;
+                    if constexpr (IN_GELU) v[c] = gelu(v[c]);
+                }
+#pragma unroll
+                for (int c = 0; c < ICC; ++c) tile[c][yy][xx] = v[c];
             }
-            tile[c][yy][xx] = val;
         }
         __syncthreads();
         for (int c = 0; c < nc; ++c) {
@@ -115,7 +127,9 @@ __global__ __launch_bounds__(256) void k_conv5(const void *__restrict__ in, cons
         if (co >= cout) break;
         const int64_t idx = ((img * cout + co) * hout + oy) * wout + ox;
         if constexpr (OUT == 0) {
-            static_cast<uint16_t *>(out)[idx] = f2bf(acc[o] + bias[co]);
+            const uint16_t z = f2bf(acc[o] + bias[co]);
+            static_cast<uint16_t *>(out)[idx] = z;
+            if (act) act[idx] = f2bf(gelu(bf2f(z)));
         } else if constexpr (OUT == 1) {
             static_cast<float *>(out)[idx] = acc[o] + bias[co];
         } else {
@@ -143,7 +157,8 @@ typedef float f32x4_t __attribute__((ext_vector_type(4)));
 template <int IN_T, int IN_GELU, int OUT>
 __global__ __launch_bounds__(256) void k_conv5_mfma(const void *__restrict__ in, const uint16_t *__restrict__ wm,
                                                     const float *__restrict__ bias, const uint16_t *__restrict__ zg,
-                                                    void *__restrict__ out, int64_t hin, int64_t win, int64_t hout,
+                                                    void *__restrict__ out, uint16_t *__restrict__ act, int64_t hin,
+                                                    int64_t win, int64_t hout,
                                                     int64_t wout, int pad, int tiles_x) {
     __shared__ __attribute__((aligned(16))) uint16_t xs[CS * CS * MPS];
     __shared__ __attribute__((aligned(16))) uint16_t wsm[2][MCH * MPS];
@@ -158,16 +173,27 @@ __global__ __launch_bounds__(256) void k_conv5_mfma(const void *__restrict__ in,
     uint4 wr0 = wsrc[0], wr1 = wsrc[1];
     reinterpret_cast<uint4 *>(wdst0)[0] = wr0;
     reinterpret_cast<uint4 *>(wdst0)[1] = wr1;
-    for (int e = t; e < MCH * CS * CS; e += 256) {
-        const int c = e / (CS * CS), r = e - c * (CS * CS);
-        const int yy = r / CS, xx = r - yy * CS;
-        const int64_t iy = iy0 + yy, ix = ix0 + xx;
-        float val = 0.0f;
-        if (iy >= 0 && iy < hin && ix >= 0 && ix < win) {
-            val = load_as_float<IN_T>(in, ((img * MCH + c) * hin + iy) * win + ix);
-            if constexpr (IN_GELU) val = gelu(val);
+    // staging: a lane owns pixels r = t, t + 256 of the 20×20 tile and walks the 64 channel planes,
+    // 8 channels per 16-byte LDS store (all address arithmetic hoisted out of the channel loop)
+    {
+        const int64_t plane = hin * win;
+        for (int r = t; r < CS * CS; r += 256) {
+            const int yy = r / CS, xx = r - yy * CS;
+            const int64_t iy = iy0 + yy, ix = ix0 + xx;
+            const bool ok = iy >= 0 && iy < hin && ix >= 0 && ix < win;
+            const int64_t off = ok ? img * MCH * plane + iy * win + ix : 0;
+            uint4 *dst = reinterpret_cast<uint4 *>(xs + r * MPS);
+#pragma unroll
+            for (int cg = 0; cg < MCH / 8; ++cg) {
+                float v[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    v[j] = ok ? load_as_float<IN_T>(in, off + (cg * 8 + j) * plane) : 0.0f;
+                    if constexpr (IN_GELU) v[j] = gelu(v[j]);
+                }
+                dst[cg] = uint4{pack_bf2(v[0], v[1]), pack_bf2(v[2], v[3]), pack_bf2(v[4], v[5]), pack_bf2(v[6], v[7])};
+            }
         }
-        xs[r * MPS + c] = f2bf(val);
     }
     __syncthreads();
 
@@ -221,8 +247,11 @@ __global__ __launch_bounds__(256) void k_conv5_mfma(const void *__restrict__ in,
                 if (px >= wout) continue;
                 const int64_t idx = ((img * MCH + co) * hout + py) * wout + px;
                 const float v = acc[mt][nt][i];
-                if constexpr (OUT == 0) static_cast<uint16_t *>(out)[idx] = f2bf(v + bias[co]);
-                else if constexpr (OUT == 1) static_cast<float *>(out)[idx] = v + bias[co];
+                if constexpr (OUT == 0) {
+                    const uint16_t z = f2bf(v + bias[co]);
+                    static_cast<uint16_t *>(out)[idx] = z;
+                    if (act) act[idx] = f2bf(gelu(bf2f(z)));
+                } else if constexpr (OUT == 1) static_cast<float *>(out)[idx] = v + bias[co];
                 else static_cast<float *>(out)[idx] = v * gelu_grad(bf2f(zg[idx]));
             }
         }
@@ -267,23 +296,35 @@ __global__ __launch_bounds__(256) void k_conv5_wgrad(const float *__restrict__ d
     for (int tl = tile_lo; tl < tile_hi; ++tl) {
         const int64_t ox0 = int64_t(tl % tiles_x) * CT, oy0 = int64_t(tl / tiles_x) * CT;
         __syncthreads();
-        for (int e = t; e < WG * CT * CT; e += 256) {
-            const int c = e >> 8, p = e & 255;
-            const int64_t co = og * WG + c, oy = oy0 + (p >> 4), ox = ox0 + (p & 15);
-            float val = 0.0f;
-            if (co < cout && oy < hout && ox < wout) val = dz[((img * cout + co) * hout + oy) * wout + ox];
-            ds[c * DSTR + p] = val;
-        }
-        for (int e = t; e < WG * CS * CS; e += 256) {
-            const int c = e / (CS * CS), r = e - c * (CS * CS);
-            const int yy = r / CS, xx = r - yy * CS;
-            const int64_t ci = ig * WG + c, iy = oy0 + yy, ix = ox0 + xx;
-            float val = 0.0f;
-            if (ci < cin && iy < hin && ix < win) {
-                val = load_as_float<IN_T>(in, ((img * cin + ci) * hin + iy) * win + ix);
-                if constexpr (IN_GELU) val = gelu(val);
+        {
+            // dZ: lane t owns output pixel t of the tile, 16 channel planes
+            const int64_t oplane = hout * wout;
+            {
+                const int64_t oy = oy0 + (t >> 4), ox = ox0 + (t & 15);
+                const bool ok = oy < hout && ox < wout;
+                const float *src = dz + (img * cout + og * WG) * oplane + (ok ? oy * wout + ox : 0);
+                float v[WG];
+#pragma unroll
+                for (int c = 0; c < WG; ++c) v[c] = (ok && og * WG + c < cout) ? src[c * oplane] : 0.0f;
+#pragma unroll
+                for (int c = 0; c < WG; ++c) ds[c * DSTR + t] = v[c];
             }
-            as[c * ASTR + yy * (CS + 1) + xx] = val;
+            // input: lane owns pixels r = t, t + 256 of the 20×20 tile, 16 channel planes
+            const int64_t iplane = hin * win;
+            for (int r = t; r < CS * CS; r += 256) {
+                const int yy = r / CS, xx = r - yy * CS;
+                const int64_t iy = oy0 + yy, ix = ox0 + xx;
+                const bool ok = iy < hin && ix < win;
+                const int64_t off = ok ? (img * cin + ig * WG) * iplane + iy * win + ix : 0;
+                float v[WG];
+#pragma unroll
+                for (int c = 0; c < WG; ++c) {
+                    v[c] = (ok && ig * WG + c < cin) ? load_as_float<IN_T>(in, off + c * iplane) : 0.0f;
+                    if constexpr (IN_GELU) v[c] = gelu(v[c]);
+                }
+#pragma unroll
+                for (int c = 0; c < WG; ++c) as[c * ASTR + yy * (CS + 1) + xx] = v[c];
+            }
         }
         __syncthreads();
         const float *dsl = ds + ol * DSTR;
@@ -323,15 +364,144 @@ __global__ __launch_bounds__(256) void k_conv5_wgrad(const float *__restrict__ d
     if (co < cout && il == 0 && ig == 0) slot[nw + co] = dsum;
 }
 
-__global__ void k_wgrad_reduce(const float *__restrict__ part, int64_t nslots, int64_t nw, int64_t cout,
-                               float *__restrict__ dw, float *__restrict__ db) {
-    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+// Weight and bias gradients of a 64 → 64-channel layer on the matrix cores. Per tap (ky, kx) the
+// gradient is a 64 × 64 GEMM over pixels: dW[o][i] = Σ_p dZ[o][p] · A[i][p + (ky, kx)]. A block takes
+// one kernel row ky (5 taps) and a range of 16 × 8-pixel output tiles of one image; wave w owns input
+// channels 16w…16w+15 for all 64 output channels (4 M tiles) and the 5 taps: 20 accumulator tiles.
+// K runs over the tile's pixels, 32 per MFMA: lane group g supplies 8 consecutive pixels of row
+// g / 2, columns 8·(g % 2)…+7. dZ (staged BF16, [o][pixel]) gives aligned 16-byte fragments; the
+// input ([i][row][col], staged with the GELU applied) is read at the tap's shift, 2 bytes at a time.
+// Bias gradients come from ky = 0 blocks (wave 0 sums its dZ fragments). Sums go to the block's slot
+// of `part` as in k_conv5_wgrad.
+constexpr int WTY = 8;                 // output rows per wgrad tile (tile = WTY × 16 pixels)
+constexpr int DZS = WTY * CT + 8;      // LDS stride of a dZ channel (BF16)
+constexpr int AXS = CT + 4 + 4;        // LDS row pitch of the staged input (BF16; 20 used)
+static_assert((MCH * WTY * 2) % 256 == 0 && (MCH * WTY * 5) % 256 == 0, "wgrad staging trip counts");
+
+template <int IN_T, int IN_GELU>
+__global__ __launch_bounds__(256) void k_conv5_wgrad_mfma(const float *__restrict__ dz, const void *__restrict__ in,
+                                                          float *__restrict__ part, int64_t hout, int64_t wout,
+                                                          int tiles_x, int ntiles, int tiles_per_block) {
+    __shared__ __attribute__((aligned(16))) uint16_t dzs[MCH * DZS];
+    __shared__ __attribute__((aligned(16))) uint16_t axs[MCH * WTY * AXS];
+    const int t = threadIdx.x, wave = t >> 6, lane = t & 63, g = lane >> 4, r16 = lane & 15;
+    const int ky = blockIdx.x;
+    const int64_t img = blockIdx.y;
+    const int split = blockIdx.z;
+    const int64_t hin = hout + 4, win = wout + 4;
+    f32x4_t acc[5][4];
+#pragma unroll
+    for (int kx = 0; kx < 5; ++kx)
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) acc[kx][mt] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+    float bsum[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    const int tile_lo = split * tiles_per_block;
+    const int tile_hi = min(ntiles, tile_lo + tiles_per_block);
+    for (int tl = tile_lo; tl < tile_hi; ++tl) {
+        const int64_t ox0 = int64_t(tl % tiles_x) * CT, oy0 = int64_t(tl / tiles_x) * WTY;
+        __syncthreads();
+        {
+            // dZ: a lane owns (channel, row, 8-column half) segments and writes each as one 16-byte store
+            const int64_t oplane = hout * wout;
+#pragma unroll
+            for (int it = 0; it < MCH * WTY * 2 / 256; ++it) {
+                const int sgi = t + 256 * it;
+                const int c = sgi / (WTY * 2), rr = (sgi / 2) % WTY, hf = sgi & 1;
+                const int64_t oy = oy0 + rr, oxs = ox0 + 8 * hf;
+                const float *src = dz + (img * MCH + c) * oplane + oy * wout + oxs;
+                float v[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) v[j] = (oy < hout && oxs + j < wout) ? src[j] : 0.0f;
+                *reinterpret_cast<uint4 *>(&dzs[c * DZS + rr * CT + 8 * hf]) =
+                    uint4{pack_bf2(v[0], v[1]), pack_bf2(v[2], v[3]), pack_bf2(v[4], v[5]), pack_bf2(v[6], v[7])};
+            }
+            // input rows oy0 + ky …: (channel, row, 4-column quarter) segments, 8-byte stores
+            const int64_t iplane = hin * win;
+#pragma unroll 5
+            for (int it = 0; it < MCH * WTY * 5 / 256; ++it) {
+                const int sgi = t + 256 * it;
+                const int c = sgi / (WTY * 5), rem = sgi - c * (WTY * 5), yy = rem / 5, q = rem - yy * 5;
+                const int64_t iy = oy0 + yy + ky, ixs = ox0 + 4 * q;
+                const int64_t base = (img * MCH + c) * iplane + iy * win + ixs;
+                float v[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    v[j] = (iy < hin && ixs + j < win) ? load_as_float<IN_T>(in, base + j) : 0.0f;
+                    if constexpr (IN_GELU) v[j] = gelu(v[j]);
+                }
+                *reinterpret_cast<uint2 *>(&axs[(c * WTY + yy) * AXS + 4 * q]) =
+                    uint2{pack_bf2(v[0], v[1]), pack_bf2(v[2], v[3])};
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < WTY / 2; ++ks) {
+            const int row = 2 * ks + (g >> 1), col = 8 * (g & 1);
+            bf16x8_t a[4];
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt)
+                a[mt] = *reinterpret_cast<const bf16x8_t *>(&dzs[(mt * 16 + r16) * DZS + row * CT + col]);
+            if (wave == 0 && ky == 0) {
+#pragma unroll
+                for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) bsum[mt] += float(a[mt][j]);
+            }
+            const uint16_t *arow = &axs[((wave * 16 + r16) * WTY + row) * AXS + col];
+#pragma unroll
+            for (int kx = 0; kx < 5; ++kx) {
+                bf16x8_t b;
+#pragma unroll
+                for (int j = 0; j < 8; ++j) b[j] = __builtin_bit_cast(__bf16, arow[kx + j]);
+#pragma unroll
+                for (int mt = 0; mt < 4; ++mt)
+                    acc[kx][mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mt], b, acc[kx][mt], 0, 0, 0);
+            }
+        }
+    }
+
+    // D[m = 4g + i][n = r16]: output channel 16·mt + 4g + i, input channel 16·wave + r16
+    const int64_t nw = int64_t(MCH) * MCH * 25;
+    float *slot = part + (img * gridDim.z + split) * (nw + MCH);
+    const int ci = wave * 16 + r16;
+#pragma unroll
+    for (int kx = 0; kx < 5; ++kx)
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int co = mt * 16 + 4 * g + i;
+                slot[(int64_t(co) * MCH + ci) * 25 + ky * 5 + kx] = acc[kx][mt][i];
+            }
+    if (wave == 0 && ky == 0) {
+        // lanes r16 of the four groups hold partial sums of output channel 16·mt + r16
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) {
+            float v = bsum[mt];
+            v += __shfl_xor(v, 16, 64);
+            v += __shfl_xor(v, 32, 64);
+            if (g == 0) slot[nw + mt * 16 + r16] = v;
+        }
+    }
+}
+
+// Adds the per-block slots: 64 outputs × 4 slot phases per block, the phases combined in LDS.
+__global__ __launch_bounds__(256) void k_wgrad_reduce(const float *__restrict__ part, int64_t nslots, int64_t nw,
+                                                      int64_t cout, float *__restrict__ dw, float *__restrict__ db) {
+    __shared__ float red[4][64];
+    const int64_t i = int64_t(blockIdx.x) * 64 + (threadIdx.x & 63);
+    const int ph = threadIdx.x >> 6;
     const int64_t len = nw + cout;
-    if (i >= len) return;
     float s = 0.0f;
-    for (int64_t p = 0; p < nslots; ++p) s += part[p * len + i];
-    if (i < nw) dw[i] = s;
-    else db[i - nw] = s;
+    if (i < len)
+        for (int64_t p = ph; p < nslots; p += 4) s += part[p * len + i];
+    red[ph][threadIdx.x & 63] = s;
+    __syncthreads();
+    if (ph == 0 && i < len) {
+        s = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+        if (i < nw) dw[i] = s;
+        else db[i - nw] = s;
+    }
 }
 
 // BF16 copies of a master weight tensor W[o][i][k] (k = 5·ky + kx): forward [i][k][o] (stride ocs_f
@@ -547,51 +717,52 @@ void check_launch() { CGEF_HIP(hipGetLastError()); }
 
 template <int OCB, int IN_T, int IN_GELU, int OUT>
 void launch_conv(dim3 g, hipStream_t st, const void *in, const uint16_t *w, const float *bias, const uint16_t *zg,
-                 void *out, int64_t cin, int64_t hin, int64_t win, int64_t cout, int64_t ocs, int64_t hout,
-                 int64_t wout, int pad, int tiles_x) {
-    hipLaunchKernelGGL((k_conv5<OCB, IN_T, IN_GELU, OUT>), g, dim3(256), 0, st, in, w, bias, zg, out, cin, hin, win,
-                       cout, ocs, hout, wout, pad, tiles_x);
+                 void *out, uint16_t *act, int64_t cin, int64_t hin, int64_t win, int64_t cout, int64_t ocs,
+                 int64_t hout, int64_t wout, int pad, int tiles_x) {
+    hipLaunchKernelGGL((k_conv5<OCB, IN_T, IN_GELU, OUT>), g, dim3(256), 0, st, in, w, bias, zg, out, act, cin, hin,
+                       win, cout, ocs, hout, wout, pad, tiles_x);
 }
 
 template <int OCB, int IN_T, int IN_GELU>
 void conv_out(int out_kind, dim3 g, hipStream_t st, const void *in, const uint16_t *w, const float *bias,
-              const uint16_t *zg, void *out, int64_t cin, int64_t hin, int64_t win, int64_t cout, int64_t ocs,
+              const uint16_t *zg, void *out, uint16_t *act, int64_t cin, int64_t hin, int64_t win, int64_t cout, int64_t ocs,
               int64_t hout, int64_t wout, int pad, int tiles_x) {
     if (out_kind == 0)
-        launch_conv<OCB, IN_T, IN_GELU, 0>(g, st, in, w, bias, zg, out, cin, hin, win, cout, ocs, hout, wout, pad,
+        launch_conv<OCB, IN_T, IN_GELU, 0>(g, st, in, w, bias, zg, out, act, cin, hin, win, cout, ocs, hout, wout, pad,
                                            tiles_x);
     else if (out_kind == 1)
-        launch_conv<OCB, IN_T, IN_GELU, 1>(g, st, in, w, bias, zg, out, cin, hin, win, cout, ocs, hout, wout, pad,
+        launch_conv<OCB, IN_T, IN_GELU, 1>(g, st, in, w, bias, zg, out, act, cin, hin, win, cout, ocs, hout, wout, pad,
                                            tiles_x);
     else
-        launch_conv<OCB, IN_T, IN_GELU, 2>(g, st, in, w, bias, zg, out, cin, hin, win, cout, ocs, hout, wout, pad,
+        launch_conv<OCB, IN_T, IN_GELU, 2>(g, st, in, w, bias, zg, out, act, cin, hin, win, cout, ocs, hout, wout, pad,
                                            tiles_x);
 }
 
 template <int OCB>
 void conv_in(int in_t, int in_gelu, int out_kind, dim3 g, hipStream_t st, const void *in, const uint16_t *w,
-             const float *bias, const uint16_t *zg, void *out, int64_t cin, int64_t hin, int64_t win, int64_t cout,
+             const float *bias, const uint16_t *zg, void *out, uint16_t *act, int64_t cin, int64_t hin, int64_t win, int64_t cout,
              int64_t ocs, int64_t hout, int64_t wout, int pad, int tiles_x) {
     if (in_t == CGEF_BF16 && in_gelu)
-        conv_out<OCB, CGEF_BF16, 1>(out_kind, g, st, in, w, bias, zg, out, cin, hin, win, cout, ocs, hout, wout, pad,
+        conv_out<OCB, CGEF_BF16, 1>(out_kind, g, st, in, w, bias, zg, out, act, cin, hin, win, cout, ocs, hout, wout, pad,
                                     tiles_x);
     else if (in_t == CGEF_BF16)
-        conv_out<OCB, CGEF_BF16, 0>(out_kind, g, st, in, w, bias, zg, out, cin, hin, win, cout, ocs, hout, wout, pad,
+        conv_out<OCB, CGEF_BF16, 0>(out_kind, g, st, in, w, bias, zg, out, act, cin, hin, win, cout, ocs, hout, wout, pad,
                                     tiles_x);
     else
-        conv_out<OCB, CGEF_F32, 0>(out_kind, g, st, in, w, bias, zg, out, cin, hin, win, cout, ocs, hout, wout, pad,
+        conv_out<OCB, CGEF_F32, 0>(out_kind, g, st, in, w, bias, zg, out, act, cin, hin, win, cout, ocs, hout, wout, pad,
                                    tiles_x);
 }
 
-// Split of the output tiles over blocks for the weight gradient: about 2048 blocks in flight.
-void wgrad_split(int64_t n, int64_t cin, int64_t cout, int64_t hout, int64_t wout, int &tiles_x, int &ntiles,
-                 int &tpb, int &nsplit, int &nig, int &nog) {
+// Split of the output tiles over blocks for the weight gradient: about 2048 blocks in flight (VALU:
+// 16×16 tiles, channel groups of 16; matrix cores: 8×16 tiles, one block per kernel row, ~1024).
+void wgrad_split(bool mfma, int64_t n, int64_t cin, int64_t cout, int64_t hout, int64_t wout, int &tiles_x,
+                 int &ntiles, int &tpb, int &nsplit, int &nig, int &nog) {
     tiles_x = int((wout + CT - 1) / CT);
-    ntiles = tiles_x * int((hout + CT - 1) / CT);
+    ntiles = tiles_x * int((hout + (mfma ? WTY : CT) - 1) / (mfma ? WTY : CT));
     nig = int((cin + WG - 1) / WG);
     nog = int((cout + WG - 1) / WG);
-    const int64_t base = n * nig * nog;
-    int64_t want = std::max<int64_t>(1, 2048 / std::max<int64_t>(base, 1));
+    const int64_t base = mfma ? 5 * n : n * nig * nog;
+    int64_t want = std::max<int64_t>(1, (mfma ? 1024 : 2048) / std::max<int64_t>(base, 1));
     want = std::min<int64_t>(want, ntiles);
     tpb = int((ntiles + want - 1) / want);
     nsplit = (ntiles + tpb - 1) / tpb;
@@ -606,7 +777,7 @@ extern "C" {
 
 cgef_status cgef_conv5(const void *in, int32_t in_dtype, int32_t in_gelu, int64_t n, int64_t cin, int64_t hin,
                        int64_t win, const uint16_t *w, int64_t ocs, const float *bias, int64_t cout, int32_t pad,
-                       void *out, int32_t out_dtype, const uint16_t *zgrad, void *stream) {
+                       void *out, int32_t out_dtype, const uint16_t *zgrad, uint16_t *act, void *stream) {
     return guarded([&] {
         need(in && w && out, CGEF_ERR_ARGUMENT, "cgef_conv5: null pointer");
         need(in_dtype == CGEF_BF16 || (in_dtype == CGEF_F32 && !in_gelu), CGEF_ERR_ARGUMENT,
@@ -624,6 +795,7 @@ cgef_status cgef_conv5(const void *in, int32_t in_dtype, int32_t in_gelu, int64_
             need(out_dtype == CGEF_BF16 || out_dtype == CGEF_F32, CGEF_ERR_ARGUMENT, "cgef_conv5: output BF16 or FP32");
             out_kind = out_dtype == CGEF_BF16 ? 0 : 1;
         }
+        need(!act || out_kind == 0, CGEF_ERR_ARGUMENT, "cgef_conv5: the activation is written with a BF16 output");
         const int ocb = cout <= 4 ? 4 : (cout <= 16 ? 16 : 64);
         need(ocs % ocb == 0 && ocs >= cout, CGEF_ERR_DIMENSION,
              "cgef_conv5: weight layout stride must cover cout in whole output-channel blocks (4, 16 or 64)");
@@ -633,13 +805,13 @@ cgef_status cgef_conv5(const void *in, int32_t in_dtype, int32_t in_gelu, int64_
         dim3 g(unsigned(ntiles), unsigned((cout + ocb - 1) / ocb), unsigned(n));
         hipStream_t st = static_cast<hipStream_t>(stream);
         if (ocb == 4)
-            conv_in<4>(in_dtype, in_gelu, out_kind, g, st, in, w, bias, zgrad, out, cin, hin, win, cout, ocs, hout, wout,
+            conv_in<4>(in_dtype, in_gelu, out_kind, g, st, in, w, bias, zgrad, out, act, cin, hin, win, cout, ocs, hout, wout,
                        pad, tiles_x);
         else if (ocb == 16)
-            conv_in<16>(in_dtype, in_gelu, out_kind, g, st, in, w, bias, zgrad, out, cin, hin, win, cout, ocs, hout,
+            conv_in<16>(in_dtype, in_gelu, out_kind, g, st, in, w, bias, zgrad, out, act, cin, hin, win, cout, ocs, hout,
                         wout, pad, tiles_x);
         else
-            conv_in<64>(in_dtype, in_gelu, out_kind, g, st, in, w, bias, zgrad, out, cin, hin, win, cout, ocs, hout,
+            conv_in<64>(in_dtype, in_gelu, out_kind, g, st, in, w, bias, zgrad, out, act, cin, hin, win, cout, ocs, hout,
                         wout, pad, tiles_x);
         check_launch();
     });
@@ -647,7 +819,7 @@ cgef_status cgef_conv5(const void *in, int32_t in_dtype, int32_t in_gelu, int64_
 
 cgef_status cgef_conv5_mfma(const void *in, int32_t in_dtype, int32_t in_gelu, int64_t n, int64_t hin, int64_t win,
                             const uint16_t *wm, const float *bias, int32_t pad, void *out, int32_t out_dtype,
-                            const uint16_t *zgrad, void *stream) {
+                            const uint16_t *zgrad, uint16_t *act, void *stream) {
     return guarded([&] {
         need(in && wm && out, CGEF_ERR_ARGUMENT, "cgef_conv5_mfma: null pointer");
         need(in_dtype == CGEF_BF16 || (in_dtype == CGEF_F32 && !in_gelu), CGEF_ERR_ARGUMENT,
@@ -667,14 +839,15 @@ cgef_status cgef_conv5_mfma(const void *in, int32_t in_dtype, int32_t in_gelu, i
                  "cgef_conv5_mfma: output BF16 or FP32");
             out_kind = out_dtype == CGEF_BF16 ? 0 : 1;
         }
+        need(!act || out_kind == 0, CGEF_ERR_ARGUMENT, "cgef_conv5_mfma: the activation is written with a BF16 output");
         const int tiles_x = int((wout + CT - 1) / CT);
         const int64_t ntiles = int64_t(tiles_x) * ((hout + CT - 1) / CT);
         need(ntiles <= 0x7fffffff, CGEF_ERR_DIMENSION, "cgef_conv5_mfma: too many tiles");
         dim3 g(unsigned(ntiles), 1, unsigned(n));
         hipStream_t st = static_cast<hipStream_t>(stream);
 #define CGEF_MFMA_LAUNCH(IT, IG, OK)                                                                                  \
-    hipLaunchKernelGGL((k_conv5_mfma<IT, IG, OK>), g, dim3(256), 0, st, in, wm, bias, zgrad, out, hin, win, hout, wout, \
-                       pad, tiles_x)
+    hipLaunchKernelGGL((k_conv5_mfma<IT, IG, OK>), g, dim3(256), 0, st, in, wm, bias, zgrad, out, act, hin, win, hout, \
+                       wout, pad, tiles_x)
         if (in_dtype == CGEF_BF16 && in_gelu) {
             if (out_kind == 0) CGEF_MFMA_LAUNCH(CGEF_BF16, 1, 0);
             else if (out_kind == 1) CGEF_MFMA_LAUNCH(CGEF_BF16, 1, 1);
@@ -702,19 +875,20 @@ cgef_status cgef_conv5_pack_mfma(const float *w, uint16_t *wmf, uint16_t *wmb, v
     });
 }
 
-cgef_status cgef_conv5_wgrad_scratch(int64_t n, int64_t cin, int64_t cout, int64_t hout, int64_t wout,
+cgef_status cgef_conv5_wgrad_scratch(int64_t n, int64_t cin, int64_t cout, int64_t hout, int64_t wout, int32_t mfma,
                                      int64_t *floats) {
     return guarded([&] {
         need(floats != nullptr, CGEF_ERR_ARGUMENT, "null output");
         need(n >= 1 && cin >= 1 && cout >= 1 && hout >= 1 && wout >= 1, CGEF_ERR_DIMENSION, "bad sizes");
+        need(!mfma || (cin == MCH && cout == MCH), CGEF_ERR_DIMENSION, "the matrix-core path takes 64 -> 64 layers");
         int tiles_x, ntiles, tpb, nsplit, nig, nog;
-        wgrad_split(n, cin, cout, hout, wout, tiles_x, ntiles, tpb, nsplit, nig, nog);
+        wgrad_split(mfma != 0, n, cin, cout, hout, wout, tiles_x, ntiles, tpb, nsplit, nig, nog);
         *floats = n * nsplit * (cout * cin * 25 + cout);
     });
 }
 
 cgef_status cgef_conv5_wgrad(const float *dz, const void *in, int32_t in_dtype, int32_t in_gelu, int64_t n,
-                             int64_t cin, int64_t cout, int64_t hout, int64_t wout, float *dw, float *db,
+                             int64_t cin, int64_t cout, int64_t hout, int64_t wout, int32_t mfma, float *dw, float *db,
                              float *scratch, int64_t scratch_floats, void *stream) {
     return guarded([&] {
         need(dz && in && dw && db && scratch, CGEF_ERR_ARGUMENT, "cgef_conv5_wgrad: null pointer");
@@ -722,25 +896,40 @@ cgef_status cgef_conv5_wgrad(const float *dz, const void *in, int32_t in_dtype, 
              "cgef_conv5_wgrad: input must be BF16 (optionally GELU-applied) or FP32");
         need(n >= 1 && n <= 65535 && cin >= 1 && cout >= 1 && hout >= 1 && wout >= 1, CGEF_ERR_DIMENSION,
              "cgef_conv5_wgrad: bad sizes");
+        need(!mfma || (cin == MCH && cout == MCH), CGEF_ERR_DIMENSION,
+             "cgef_conv5_wgrad: the matrix-core path takes 64 -> 64 layers");
         int tiles_x, ntiles, tpb, nsplit, nig, nog;
-        wgrad_split(n, cin, cout, hout, wout, tiles_x, ntiles, tpb, nsplit, nig, nog);
+        wgrad_split(mfma != 0, n, cin, cout, hout, wout, tiles_x, ntiles, tpb, nsplit, nig, nog);
         const int64_t nw = cout * cin * 25, nslots = n * nsplit;
         need(scratch_floats >= nslots * (nw + cout), CGEF_ERR_DIMENSION,
              "cgef_conv5_wgrad: scratch smaller than cgef_conv5_wgrad_scratch");
         hipStream_t st = static_cast<hipStream_t>(stream);
-        dim3 g(unsigned(nog * nig), unsigned(n), unsigned(nsplit));
-        if (in_dtype == CGEF_BF16 && in_gelu)
-            hipLaunchKernelGGL((k_conv5_wgrad<CGEF_BF16, 1>), g, dim3(256), 0, st, dz, in, scratch, cin, cout, hout,
-                               wout, tiles_x, ntiles, tpb, nig);
-        else if (in_dtype == CGEF_BF16)
-            hipLaunchKernelGGL((k_conv5_wgrad<CGEF_BF16, 0>), g, dim3(256), 0, st, dz, in, scratch, cin, cout, hout,
-                               wout, tiles_x, ntiles, tpb, nig);
-        else
-            hipLaunchKernelGGL((k_conv5_wgrad<CGEF_F32, 0>), g, dim3(256), 0, st, dz, in, scratch, cin, cout, hout,
-                               wout, tiles_x, ntiles, tpb, nig);
+        if (mfma) {
+            dim3 g(5, unsigned(n), unsigned(nsplit));
+            if (in_dtype == CGEF_BF16 && in_gelu)
+                hipLaunchKernelGGL((k_conv5_wgrad_mfma<CGEF_BF16, 1>), g, dim3(256), 0, st, dz, in, scratch, hout, wout,
+                                   tiles_x, ntiles, tpb);
+            else if (in_dtype == CGEF_BF16)
+                hipLaunchKernelGGL((k_conv5_wgrad_mfma<CGEF_BF16, 0>), g, dim3(256), 0, st, dz, in, scratch, hout, wout,
+                                   tiles_x, ntiles, tpb);
+            else
+                hipLaunchKernelGGL((k_conv5_wgrad_mfma<CGEF_F32, 0>), g, dim3(256), 0, st, dz, in, scratch, hout, wout,
+                                   tiles_x, ntiles, tpb);
+        } else {
+            dim3 g(unsigned(nog * nig), unsigned(n), unsigned(nsplit));
+            if (in_dtype == CGEF_BF16 && in_gelu)
+                hipLaunchKernelGGL((k_conv5_wgrad<CGEF_BF16, 1>), g, dim3(256), 0, st, dz, in, scratch, cin, cout, hout,
+                                   wout, tiles_x, ntiles, tpb, nig);
+            else if (in_dtype == CGEF_BF16)
+                hipLaunchKernelGGL((k_conv5_wgrad<CGEF_BF16, 0>), g, dim3(256), 0, st, dz, in, scratch, cin, cout, hout,
+                                   wout, tiles_x, ntiles, tpb, nig);
+            else
+                hipLaunchKernelGGL((k_conv5_wgrad<CGEF_F32, 0>), g, dim3(256), 0, st, dz, in, scratch, cin, cout, hout,
+                                   wout, tiles_x, ntiles, tpb, nig);
+        }
         check_launch();
-        hipLaunchKernelGGL(k_wgrad_reduce, dim3(blocks_for(nw + cout)), dim3(256), 0, st, scratch, nslots, nw, cout, dw,
-                           db);
+        hipLaunchKernelGGL(k_wgrad_reduce, dim3(unsigned((nw + cout + 63) / 64)), dim3(256), 0, st, scratch, nslots, nw,
+                           cout, dw, db);
         check_launch();
     });
 }

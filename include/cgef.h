@@ -180,11 +180,12 @@ cgef_status cgef_spectral_density(int64_t n, const double *C, const double *k, d
  * backward-data weight layout). Input BF16 (in_gelu: GELU applied while loading, i.e. the input
  * is the previous layer's pre-activation) or FP32. w is BF16 [cin][25][ocs], k = 5·ky + kx, ocs a
  * multiple of the output-channel block (4 for cout ≤ 4, 16 for cout ≤ 16, else 64). Without zgrad:
- * out = conv + bias as BF16 or FP32. With zgrad (BF16, shaped as out): out = conv · GELU'(zgrad),
- * FP32, bias unused. */
+ * out = conv + bias as BF16 or FP32; with a BF16 output, act (may be NULL, shaped as out) receives
+ * GELU(out) in BF16, the next layer's input. With zgrad (BF16, shaped as out): out = conv ·
+ * GELU'(zgrad), FP32, bias unused. */
 cgef_status cgef_conv5(const void *in, int32_t in_dtype, int32_t in_gelu, int64_t n, int64_t cin, int64_t hin,
                        int64_t win, const uint16_t *w, int64_t ocs, const float *bias, int64_t cout, int32_t pad,
-                       void *out, int32_t out_dtype, const uint16_t *zgrad, void *stream);
+                       void *out, int32_t out_dtype, const uint16_t *zgrad, uint16_t *act, void *stream);
 
 /* The same layer for 64 input and 64 output channels on the matrix cores (BF16 MFMA, FP32
  * accumulation; an FP32 input is rounded to BF16 as it is staged). wm is the layout written by
@@ -192,7 +193,7 @@ cgef_status cgef_conv5(const void *in, int32_t in_dtype, int32_t in_gelu, int64_
  * Other arguments as in cgef_conv5. */
 cgef_status cgef_conv5_mfma(const void *in, int32_t in_dtype, int32_t in_gelu, int64_t n, int64_t hin, int64_t win,
                             const uint16_t *wm, const float *bias, int32_t pad, void *out, int32_t out_dtype,
-                            const uint16_t *zgrad, void *stream);
+                            const uint16_t *zgrad, uint16_t *act, void *stream);
 
 /* BF16 matrix-core layouts of a 64 × 64 master weight tensor W[o][i][25]: forward wmf[k][o][i] and
  * backward-data wmb[24 − k][i][o] (either may be NULL), 64·64·25 entries each. */
@@ -200,12 +201,14 @@ cgef_status cgef_conv5_pack_mfma(const float *w, uint16_t *wmf, uint16_t *wmb, v
 
 /* Weight and bias gradients of a valid 5×5 layer: dw[o][i][k] = Σ dz[n][o][p]·A[n][i][p+k],
  * db[o] = Σ dz[n][o][p], with A the layer input (n, cin, hout+4, wout+4) as in cgef_conv5 and dz
- * FP32 (n, cout, hout, wout). Deterministic: per-block partial sums go to `scratch` (at least the
- * floats cgef_conv5_wgrad_scratch returns) and are added in a fixed order. Overwrites dw, db. */
-cgef_status cgef_conv5_wgrad_scratch(int64_t n, int64_t cin, int64_t cout, int64_t hout, int64_t wout,
+ * FP32 (n, cout, hout, wout). mfma = 1 (64 → 64 layers only) runs the GEMMs on the matrix cores with
+ * dz and A rounded to BF16; mfma = 0 uses FP32 VALU products of the stored values. Deterministic:
+ * per-block partial sums go to `scratch` (at least the floats cgef_conv5_wgrad_scratch returns for
+ * the same arguments) and are added in a fixed order. Overwrites dw, db. */
+cgef_status cgef_conv5_wgrad_scratch(int64_t n, int64_t cin, int64_t cout, int64_t hout, int64_t wout, int32_t mfma,
                                      int64_t *floats);
 cgef_status cgef_conv5_wgrad(const float *dz, const void *in, int32_t in_dtype, int32_t in_gelu, int64_t n,
-                             int64_t cin, int64_t cout, int64_t hout, int64_t wout, float *dw, float *db,
+                             int64_t cin, int64_t cout, int64_t hout, int64_t wout, int32_t mfma, float *dw, float *db,
                              float *scratch, int64_t scratch_floats, void *stream);
 
 /* BF16 layouts of an FP32 master weight tensor W[cout][cin][25]: forward wf[cin][25][ocs_f] and

@@ -11,8 +11,9 @@ device kernels compute, in FP64:
 
 with valid 5×5 cross-correlation conv5(A, W)[n, o, y, x] = Σ_{i,ky,kx} W[o, i, ky, kx]·A[n, i, y+ky, x+kx]
 and GELU(z) = z·Φ(z) (exact normal CDF). ``bf16=True`` rounds what the device stores in BF16 (the
-input, the weights, and every hidden pre-activation Z_l) to nearest even, so the device differs from
-it only by FP32 accumulation order. Gradients are those of the loss mean((Y − T)²).
+input, the weights, every hidden pre-activation Z_l and activation GELU(Z_l)) to nearest even, so the
+device differs from it by FP32 accumulation order (and, on the matrix-core path, by the BF16
+rounding of the data gradient it multiplies). Gradients are those of the loss mean((Y − T)²).
 
 Pinned in tests/test_closure_oracle.py against torch's CPU FP64 autograd (torch.nn.functional.conv2d and
 gelu, an independent library) and against central finite differences of the FP64 forward.
@@ -84,7 +85,7 @@ def forward(params, x, bf16=True):
             return z, zs, ins
         z = r(z)
         zs.append(z)
-        a = gelu(z)
+        a = r(gelu(z))
     raise ValueError("empty network")
 
 

@@ -222,3 +222,26 @@ def test_tau_pi_loss_gradient_matches_oracle():
     ls = N(loss)
     assert abs(ls[0] / pred.size + 2.5 * ls[1] / (pred.size / 3) - ref_loss) <= 1e-5 * ref_loss
     del net
+
+
+@pytest.mark.parametrize("mfma", [True, False])
+def test_gelu_on_load_matches_stored_activation(mfma):
+    """The kernels' GELU-while-staging input mode (in_gelu) against the stored BF16 activation path:
+    they differ only by the BF16 rounding of GELU(Z)."""
+    print(f"closure in_gelu mfma={mfma}", flush=True)
+    net, xb, x, params = _net_and_input((4, 64, 64, 64, 3), 1, 36, 34, 5, mfma)
+    net.forward(xb)
+    z2 = torch.empty_like(net.Z[1])
+    if mfma:
+        CL.conv5_mfma(z2, net.Z[0], net.wmf[1], net.b[1], in_gelu=True)
+    else:
+        CL.conv5(z2, net.Z[0], net.wf[1], net.ocs_f[1], net.b[1], in_gelu=True)
+    assert rel_l2(N(z2), N(net.Z[1])) <= 5e-3
+    gw, gb = torch.empty_like(net.gW[1]), torch.empty_like(net.gb[1])
+    dz = torch.tensor(np.random.Generator(np.random.PCG64(3)).standard_normal(tuple(net.Z[1].shape)),
+                      dtype=torch.float32, device=DEV)
+    CL.conv5_wgrad(gw, gb, dz, net.Z[0], net.scratch, in_gelu=True, mfma=mfma)
+    gw2, gb2 = torch.empty_like(gw), torch.empty_like(gb)
+    CL.conv5_wgrad(gw2, gb2, dz, net.A[0], net.scratch, mfma=mfma)
+    assert rel_l2(N(gw), N(gw2)) <= 5e-3
+    np.testing.assert_allclose(N(gb), N(gb2), rtol=1e-6, atol=1e-6)
