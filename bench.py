@@ -516,7 +516,7 @@ def main():
     ap.add_argument("--closure-steps", type=int, default=2000, help="Adam steps of the learned-closure leg")
     ap.add_argument("--closure-n", type=int, default=1024, help="box size of the learned-closure leg")
     ap.add_argument("--no-configs", action="store_true",
-                    help="skip the top-hat sphere, Cartesian 256² and 4096², volume and spectral legs")
+                    help="skip the top-hat sphere, Cartesian 256² and 4096², volume, spectral and learned-closure legs")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 512 MiB → 512 MiB FP32 elementwise copy first, so that hardware counter runs "
                          "can scale FETCH_SIZE / WRITE_SIZE by a kernel of known traffic")
@@ -629,7 +629,7 @@ def main():
             sys.exit(2)
     else:
         chosen = ([] if args.no_batched else ["batched"]) + \
-                 ([] if args.no_configs else ["tophat", "cartesian", "cartesian4096", "volume", "spectral"])
+                 ([] if args.no_configs else ["tophat", "cartesian", "cartesian4096", "volume", "spectral", "closure"])
     for name in chosen:
         key, run = legs[name]
         extra[key] = run()
