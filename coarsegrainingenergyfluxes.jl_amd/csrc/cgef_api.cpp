@@ -1,6 +1,7 @@
 // C ABI of libcgef_hip.so (declared in include/cgef.h).
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <exception>
 #include <new>
@@ -133,6 +134,8 @@ Workspace::~Workspace() {
     dfree(rxy);
     dfree(rxf);
     dfree(rout);
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (cap_stream) (void)hipStreamDestroy(cap_stream);
 }
 
 // Lagrange weights of the derivative at x of the quadratic through (a, b, c).
@@ -881,50 +884,102 @@ cgef_status cgef_coarse_grain(cgef_plan *const *plans, int64_t nscales, cgef_wor
         const size_t es = esize(dtype);
         double *Eall = (double *)ws_fbuf(W, (size_t)nscales * nbatch * sizeof(double));
         const int64_t nblk = strain_blocks(G);
-        if (all_ring) {
-            // one forward transform for every scale (Diagnostics.jl:283-301's analyze-once idea, here
-            // for the real-space footprint): only the contraction and the inverse depend on ℓ
+        if (all_ring || all_spec) {
             Plan &p0 = *reinterpret_cast<Plan *>(plans[0]);
             int wm = 0;
-            for (int64_t q = 0; q < nscales; q++) wm |= ring_withmask(g, reinterpret_cast<Plan *>(plans[q])->strategy);
-            const int nf = ring_nfields(g, wm);
+            if (all_ring)
+                for (int64_t q = 0; q < nscales; q++) wm |= ring_withmask(g, reinterpret_cast<Plan *>(plans[q])->strategy);
+            const int nf = all_ring ? ring_nfields(g, wm) : 5;
             ring_reserve(W, p0, nf * nbatch);
-            ring_prep(G, dtype, u, v, w, nf, wm, W.rxy, nbatch, s);
-            ring_forward(p0.ring, W.rxy, W.rxf, nf * nbatch, s);
-            for (int64_t q = 0; q < nscales; q++) {
-                Plan &p = *reinterpret_cast<Plan *>(plans[q]);
-                p.prof_begin(s);
-                ring_contract(p.ring, W.rxf, W.rout, nf * nbatch, s);
-                p.prof_end(s);
-                ring_inverse(p.ring, W.rout, W.rxy, nf * nbatch, s);
-                ring_epilogue(G, dtype, W.rxy, p.D64_row_d, nf, wm, p.strategy, W.ubar, W.tau, nullptr, nullptr, nullptr,
-                              nbatch, s);
-                launch_strain_pi(G, dtype, W.ubar, W.tau, (char *)Pi + (size_t)q * P * es, rho0, nbatch, W.partials,
-                                 W.npartials, s, nscales * P);
-                if (E_out) launch_energy_final(nbatch, nblk, W.partials, g.active_area, Eall + q * nbatch, s);
+            auto sweep = [&](hipStream_t st) {
+                if (all_ring) {
+                    // one forward transform for every scale (Diagnostics.jl:283-301's analyze-once idea,
+                    // here for the real-space footprint): only the contraction and the inverse depend on ℓ
+                    ring_prep(G, dtype, u, v, w, nf, wm, W.rxy, nbatch, st);
+                    ring_forward(p0.ring, W.rxy, W.rxf, nf * nbatch, st);
+                    for (int64_t q = 0; q < nscales; q++) {
+                        Plan &p = *reinterpret_cast<Plan *>(plans[q]);
+                        p.prof_begin(st);
+                        ring_contract(p.ring, W.rxf, W.rout, nf * nbatch, st);
+                        p.prof_end(st);
+                        ring_inverse(p.ring, W.rout, W.rxy, nf * nbatch, st);
+                        ring_epilogue(G, dtype, W.rxy, p.D64_row_d, nf, wm, p.strategy, W.ubar, W.tau, nullptr, nullptr,
+                                      nullptr, nbatch, st);
+                        launch_strain_pi(G, dtype, W.ubar, W.tau, (char *)Pi + (size_t)q * P * es, rho0, nbatch,
+                                         W.partials, W.npartials, st, nscales * P);
+                        if (E_out) launch_energy_final(nbatch, nblk, W.partials, g.active_area, Eall + q * nbatch, st);
+                    }
+                } else {
+                    // the spectrum of the masked inputs is taken once and reused by every scale
+                    // (FFTWExt.jl:100-120 filter_analyze! / filter_synthesize!)
+                    ring_prep(G, dtype, u, v, w, 5, 0, W.rxy, nbatch, st);
+                    ring_forward(p0.ring, W.rxy, W.rxf, 5 * nbatch, st);
+                    ring_spectral_analyze_cols(p0.ring, W.rxf, W.rout, 5 * nbatch, st);
+                    for (int64_t q = 0; q < nscales; q++) {
+                        Plan &p = *reinterpret_cast<Plan *>(plans[q]);
+                        p.prof_begin(st);
+                        ring_spectral_synth_cols(p.ring, W.rout, W.rxf, 5 * nbatch, st);
+                        p.prof_end(st);
+                        ring_inverse(p.ring, W.rxf, W.rxy, 5 * nbatch, st);
+                        ring_spectral_epilogue(G, dtype, W.rxy, p.ring.invrenorm_d, W.ubar, W.tau, nullptr, nullptr,
+                                               nullptr, nbatch, st);
+                        launch_strain_pi(G, dtype, W.ubar, W.tau, (char *)Pi + (size_t)q * P * es, rho0, nbatch,
+                                         W.partials, W.npartials, st, nscales * P);
+                        if (E_out) launch_energy_final(nbatch, nblk, W.partials, g.active_area, Eall + q * nbatch, st);
+                    }
+                }
+                CGEF_HIP(hipGetLastError());
+            };
+            // The sweep as one HIP graph: the first sweep with a given set of arguments runs eagerly
+            // (and settles the contraction's measured variants), the second is captured on the
+            // workspace's own stream and launched, later ones replay it. A graph holds no memset
+            // node. CGEF_GRAPH=0 turns this off; plans being profiled always run eagerly. Calls that
+            // share a workspace must not overlap (as for every entry point using it).
+            bool profiling = false;
+            for (int64_t q = 0; q < nscales; q++) profiling = profiling || reinterpret_cast<Plan *>(plans[q])->profiling;
+            const char *ge = std::getenv("CGEF_GRAPH");
+            const bool use_graph = !(ge && ge[0] == '0') && !profiling && !W.graph_broken;
+            std::vector<uint64_t> key;
+            if (use_graph) {
+                auto u64 = [](const void *ptr) { return (uint64_t)(uintptr_t)ptr; };
+                uint64_t rb;
+                std::memcpy(&rb, &rho0, sizeof rb);
+                key = {(uint64_t)all_ring, (uint64_t)nscales, u64(Pi), u64(u), u64(v), u64(w), (uint64_t)nbatch,
+                       (uint64_t)dtype, rb, (uint64_t)(E_out != nullptr), u64(W.rxy), u64(W.rxf), u64(W.rout),
+                       u64(W.ubar), u64(W.tau), u64(W.partials), u64(Eall)};
+                for (int64_t q = 0; q < nscales; q++) key.push_back(reinterpret_cast<Plan *>(plans[q])->uid);
             }
-            CGEF_HIP(hipGetLastError());
-        } else if (all_spec) {
-            // the spectrum of the masked inputs is taken once and reused by every scale
-            // (FFTWExt.jl:100-120 filter_analyze! / filter_synthesize!)
-            Plan &p0 = *reinterpret_cast<Plan *>(plans[0]);
-            ring_reserve(W, p0, 5 * nbatch);
-            ring_prep(G, dtype, u, v, w, 5, 0, W.rxy, nbatch, s);
-            ring_forward(p0.ring, W.rxy, W.rxf, 5 * nbatch, s);
-            ring_spectral_analyze_cols(p0.ring, W.rxf, W.rout, 5 * nbatch, s);
-            for (int64_t q = 0; q < nscales; q++) {
-                Plan &p = *reinterpret_cast<Plan *>(plans[q]);
-                p.prof_begin(s);
-                ring_spectral_synth_cols(p.ring, W.rout, W.rxf, 5 * nbatch, s);
-                p.prof_end(s);
-                ring_inverse(p.ring, W.rxf, W.rxy, 5 * nbatch, s);
-                ring_spectral_epilogue(G, dtype, W.rxy, p.ring.invrenorm_d, W.ubar, W.tau, nullptr, nullptr, nullptr,
-                                       nbatch, s);
-                launch_strain_pi(G, dtype, W.ubar, W.tau, (char *)Pi + (size_t)q * P * es, rho0, nbatch, W.partials,
-                                 W.npartials, s, nscales * P);
-                if (E_out) launch_energy_final(nbatch, nblk, W.partials, g.active_area, Eall + q * nbatch, s);
+            if (use_graph && W.graph_exec && key == W.graph_key) {
+                CGEF_HIP(hipGraphLaunch(W.graph_exec, s));
+            } else if (use_graph && key == W.seen_key) {
+                if (!W.cap_stream) CGEF_HIP(hipStreamCreateWithFlags(&W.cap_stream, hipStreamNonBlocking));
+                hipGraph_t graph = nullptr;
+                hipGraphExec_t exec = nullptr;
+                bool ok = hipStreamBeginCapture(W.cap_stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
+                if (ok) {
+                    try {
+                        sweep(W.cap_stream);
+                    } catch (const Error &) {
+                        ok = false;
+                    }
+                    ok = hipStreamEndCapture(W.cap_stream, &graph) == hipSuccess && ok && graph != nullptr;
+                }
+                if (ok) ok = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess;
+                if (graph) (void)hipGraphDestroy(graph);
+                if (ok) {
+                    if (W.graph_exec) (void)hipGraphExecDestroy(W.graph_exec);
+                    W.graph_exec = exec;
+                    W.graph_key = key;
+                    CGEF_HIP(hipGraphLaunch(W.graph_exec, s));
+                } else {
+                    (void)hipGetLastError();
+                    W.graph_broken = true;   // capture not possible here: stay eager for this workspace
+                    sweep(s);
+                }
+            } else {
+                sweep(s);
+                if (use_graph) W.seen_key = key;
             }
-            CGEF_HIP(hipGetLastError());
         } else {
             // other engines: one compute_Π! per scale, written into the scale's strided slots
             for (int64_t q = 0; q < nscales; q++) {

@@ -4,6 +4,7 @@
 
 #include <atomic>
 #include <cstdint>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -90,8 +91,15 @@ struct RingArgs {
     const int32_t *off_dy = nullptr;   // per offset
 };
 
+// Measured choice of the banded-contraction kernel variant per column count (see ring_contract).
+struct ContractTune {
+    std::mutex mu;
+    std::vector<std::pair<int64_t, int>> choice;   // (columns, variant)
+};
+
 struct RingPlan {
     bool ok = false;
+    std::shared_ptr<ContractTune> tune = std::make_shared<ContractTune>();
     // spectral engine (FFTWExt.jl:121-208): the same row transforms, then per Fourier mode along x a
     // transform along y, the transfer function Ĝ(|k|), and back — instead of the banded contraction
     bool spectral = false;
@@ -161,7 +169,14 @@ struct Grid {
     ~Grid();
 };
 
+// Identity of a plan for the lifetime of the process (addresses can be reused after a destroy).
+inline uint64_t next_plan_uid() {
+    static std::atomic<uint64_t> n{0};
+    return ++n;
+}
+
 struct Plan {
+    uint64_t uid = next_plan_uid();
     Grid *grid = nullptr;
     int kind = CGEF_GAUSSIAN;
     double alpha = 6.0, scale = 0.0, rad = 0.0;
@@ -226,6 +241,12 @@ struct Workspace {
     double *rxy = nullptr;
     double2 *rxf = nullptr, *rout = nullptr;
     size_t nrxy = 0, nrxf = 0;
+    // multi-scale sweeps (cgef_coarse_grain, ring and spectral plans) replayed as one HIP graph: the
+    // arguments of the last eager sweep (seen_key) and of the captured one (graph_key)
+    std::vector<uint64_t> seen_key, graph_key;
+    hipGraphExec_t graph_exec = nullptr;
+    hipStream_t cap_stream = nullptr;
+    bool graph_broken = false;
     ~Workspace();
 };
 

@@ -16,6 +16,8 @@
 // filtered fields and τ, ū, and the spectral engine's column transforms (k_spec_cols).
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+
 #include <cfloat>
 #include <utility>
 
@@ -980,6 +982,19 @@ void ring_inverse(const RingPlan &R, const double2 *OUT, double *yout, int64_t n
 // allows (160 KiB per CU), measured on MI355X: with at most 5 resident blocks the deepest prefetch
 // (PB 8, 94 VGPRs); with room for 7 and many blocks to schedule (many columns, or several dz groups
 // of few offsets each) PB 4 at 72 VGPRs; otherwise PB 6 held to 80 VGPRs for 6 waves per SIMD.
+// The contraction comes in three variants (weight batch PB, waves per SIMD held): 0 = <8, 1>,
+// 1 = <4, 1>, 2 = <6, 6>. Which is fastest depends on the footprint's row span (LDS slab, hence
+// blocks per CU), the number of vertical offset groups and the column count. The first call of a
+// plan for a column count times all three on that call's operands (each overwrites OUT with the same
+// sums; the chosen one runs last) and keeps the fastest; later calls, and calls under stream capture,
+// use the kept choice or, before any measurement, the occupancy rule below. CGEF_TUNE=0 turns the
+// measurement off; CGEF_CONTRACT_VARIANT=0|1|2 forces a variant (testing).
+static int contract_rule(const RingPlan &R, int64_t ncol, size_t lds_blocks) {
+    if (lds_blocks <= 5) return 0;
+    if (lds_blocks >= 7 && (R.args.ndz > 1 || ncol > 16)) return 1;
+    return 2;
+}
+
 void ring_contract(const RingPlan &R, const double2 *XF, double2 *OUT, int64_t ncol, hipStream_t s) {
     constexpr int CC = CGEF_CONTRACT_CC;
     const int span = 256 + R.args.dy_max - R.args.dy_min;
@@ -988,13 +1003,60 @@ void ring_contract(const RingPlan &R, const double2 *XF, double2 *OUT, int64_t n
     const int64_t items = (ncol + CC - 1) / CC * nyb * R.args.nz * R.mh;
     const dim3 grid((unsigned)(8 * ((items + 7) / 8)));
     const size_t lds_blocks = (size_t)160 * 1024 / sh;
-    auto launch = [&](auto kern) {
-        set_shmem((const void *)kern, sh);
-        hipLaunchKernelGGL(kern, grid, dim3(256), sh, s, R.args, XF, OUT, ncol);
+    auto launch = [&](int variant) {
+        auto go = [&](auto kern) {
+            set_shmem((const void *)kern, sh);
+            hipLaunchKernelGGL(kern, grid, dim3(256), sh, s, R.args, XF, OUT, ncol);
+        };
+        if (variant == 0) go(k_contract<CC, 8, 1>);
+        else if (variant == 1) go(k_contract<CC, 4, 1>);
+        else go(k_contract<CC, 6, 6>);
     };
-    if (lds_blocks <= 5) launch(k_contract<CC, 8, 1>);
-    else if (lds_blocks >= 7 && (R.args.ndz > 1 || ncol > 16)) launch(k_contract<CC, 4, 1>);
-    else launch(k_contract<CC, 6, 6>);
+    if (const char *f = std::getenv("CGEF_CONTRACT_VARIANT")) {
+        const int v = std::atoi(f);
+        if (v >= 0 && v <= 2) {
+            launch(v);
+            return;
+        }
+    }
+    int variant = -1;
+    {
+        std::lock_guard<std::mutex> lk(R.tune->mu);
+        for (const auto &c : R.tune->choice)
+            if (c.first == ncol) variant = c.second;
+    }
+    if (variant >= 0) {
+        launch(variant);
+        return;
+    }
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    CGEF_HIP(hipStreamIsCapturing(s, &cap));
+    const char *tune = std::getenv("CGEF_TUNE");
+    if (cap != hipStreamCaptureStatusNone || (tune && tune[0] == '0')) {
+        launch(contract_rule(R, ncol, lds_blocks));
+        return;
+    }
+    hipEvent_t ev[2];
+    for (auto &e : ev) CGEF_HIP(hipEventCreate(&e));
+    float best_ms = 0.0f;
+    int best = contract_rule(R, ncol, lds_blocks);
+    for (int v = 0; v < 3; ++v) {
+        launch(v);   // first launch of the variant (code load, LDS limit)
+        CGEF_HIP(hipEventRecord(ev[0], s));
+        launch(v);
+        CGEF_HIP(hipEventRecord(ev[1], s));
+        CGEF_HIP(hipEventSynchronize(ev[1]));
+        float ms = 0.0f;
+        CGEF_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+        if (v == 0 || ms < best_ms) {
+            best_ms = ms;
+            best = v;
+        }
+    }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    launch(best);
+    std::lock_guard<std::mutex> lk(R.tune->mu);
+    R.tune->choice.emplace_back(ncol, best);
 }
 
 void ring_prep(const GeomArgs &G, int dtype, const void *u, const void *v, const void *w, int nf, int withmask,
