@@ -588,17 +588,22 @@ __global__ __launch_bounds__(256, MINW) void k_contract(RingArgs A, const double
 #pragma unroll
         for (int u = 0; u < PB; u++) wn[u] = (valid && u < ng) ? wrow[(int64_t)u * A.nbands] : 0.0;
         __syncthreads();
-        for (int q = tid; q < span * CC; q += 256) {
-            const int c = q / span, s = q - c * span;
+        // staging: a lane takes source-row offsets s = tid, tid + 256, … and loads the CC columns of
+        // each together (CC independent loads in flight, no division by the span)
+        for (int s = tid; s < span; s += 256) {
             int64_t jj = j0 + A.dy_min + s;
-            double2 v = make_double2(0.0, 0.0);
             bool in = true;
             if (jj < 0 || jj >= Ny) {
                 if (A.py) jj = ((jj % Ny) + Ny) % Ny;
                 else in = false;
             }
-            if (in && c < nvc) v = XF[(m * ncol + c0 + c) * A.nrows + jj + Ny * kk];
-            st[c * span + s] = v;
+            const double2 *xs = XF + (m * ncol + c0) * A.nrows + jj + Ny * kk;
+            double2 v[CC];
+#pragma unroll
+            for (int c = 0; c < CC; c++)
+                v[c] = (in && c < nvc) ? xs[(int64_t)c * A.nrows] : make_double2(0.0, 0.0);
+#pragma unroll
+            for (int c = 0; c < CC; c++) st[c * span + s] = v[c];
         }
         __syncthreads();
         if (!valid) continue;
