@@ -128,3 +128,44 @@ def test_slice_entry_points_check_lengths_before_touching_a_device():
     with pytest.raises(cgef.DimensionMismatch):
         cgef.filter_slices_([None], [None, None], [None])
     assert cgef.slice_pipeline_costs([g, g], 3) == [144, 144]
+
+
+def test_closure_entry_points_validate_on_the_host():
+    """The learned closure's C-ABI entry points reject bad arguments before any device call."""
+    import ctypes as C
+
+    from cgef import closure as CL
+    L = CL._L()
+    vp = C.c_void_p(1)   # never dereferenced: every call below fails its host checks first
+    with pytest.raises(_lib.ArgumentError):
+        _lib.check(L.cgef_conv5(None, 2, 0, 1, 4, 20, 20, vp, 64, vp, 64, 0, vp, 2, None, None, None))
+    with pytest.raises(_lib.ArgumentError):     # pad must be 0 or 4
+        _lib.check(L.cgef_conv5(vp, 2, 0, 1, 4, 20, 20, vp, 64, vp, 64, 2, vp, 2, None, None, None))
+    with pytest.raises(_lib.DimensionMismatch):  # smaller than the stencil
+        _lib.check(L.cgef_conv5(vp, 2, 0, 1, 4, 3, 20, vp, 64, vp, 64, 0, vp, 2, None, None, None))
+    with pytest.raises(_lib.DimensionMismatch):  # layout stride not a whole number of channel blocks
+        _lib.check(L.cgef_conv5(vp, 2, 0, 1, 4, 20, 20, vp, 60, vp, 60, 0, vp, 2, None, None, None))
+    with pytest.raises(_lib.ArgumentError):     # FP32 input cannot take GELU on load
+        _lib.check(L.cgef_conv5_mfma(vp, 0, 1, 1, 20, 20, vp, vp, 0, vp, 2, None, None, None))
+    with pytest.raises(_lib.ArgumentError):     # the activation comes with a BF16 output only
+        _lib.check(L.cgef_conv5_mfma(vp, 2, 0, 1, 20, 20, vp, vp, 0, vp, 0, None, vp, None))
+    with pytest.raises(_lib.ArgumentError):     # Adam steps count from 1
+        _lib.check(L.cgef_adam_update(vp, vp, vp, vp, 10, 1e-3, 0.9, 0.999, 1e-8, 0, 1, 1, None, 0, None, 0, None))
+    with pytest.raises(_lib.DimensionMismatch):  # the matrix-core weight gradient takes 64 -> 64 layers
+        CL.wgrad_scratch_floats(16, 4, 64, 76, 76, mfma=True)
+
+
+def test_closure_wgrad_scratch_sizes():
+    from cgef import closure as CL
+    # VALU: 16x16 tiles, channel groups of 16, about 2048 blocks; matrix cores: 8x16 tiles, ~1024 blocks
+    n, ci, co, ho, wo = 16, 64, 64, 72, 72
+    tiles = 5 * 5
+    want = max(1, 2048 // (n * 4 * 4))
+    tpb = -(-tiles // min(want, tiles))
+    assert CL.wgrad_scratch_floats(n, ci, co, ho, wo) == n * (-(-tiles // tpb)) * (co * ci * 25 + co)
+    tiles_m = 5 * 9
+    want_m = max(1, 1024 // (5 * n))
+    tpb_m = -(-tiles_m // min(want_m, tiles_m))
+    assert CL.wgrad_scratch_floats(n, ci, co, ho, wo, mfma=True) == n * (-(-tiles_m // tpb_m)) * (co * ci * 25 + co)
+    assert CL.gradient_model_sigma2(cgef.GaussianKernel(6.0), 8.0) == pytest.approx(64.0 / 12.0)
+    assert CL.gradient_model_sigma2(cgef.TopHatKernel(), 8.0) == pytest.approx(4.0)
