@@ -687,11 +687,11 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if plan.engine == "banded" else "f32 in/out, f64 inside",
+        "dtype": "f32" if plan.engine == "banded" else "f64",
         "data": "synthetic",
         "config": {"workload": "spherical 0.25deg 1440x720 FP32, Gaussian alpha=4 haversine, l=200km, 30% land "
                                "ZeroFill, 1 scale, 1 snapshot", "engine": plan.engine,
-                   "points": npts, "scales": 1, "snapshots": 1, "parallelism": "1 GPU"},
+                   "points": npts, "scales": 1, "snapshots": 1, "io_dtype": "f32", "parallelism": "1 GPU"},
         "roofline": roof,
         "cpu_baseline": cpu,
     }
