@@ -370,7 +370,7 @@ def closure_leg(args):
     pairs from the filter for 64 training and 16 held-out seeds; the network learns what the gradient
     model σ²∇ū∇ūᵀ misses of τ, trained on the device for 2000 Adam steps at batch 16 (64² crops with
     their 8-cell context); then Π of the held-out fields against the exact Π, next to the gradient
-    model alone, the exact pipeline's cost, and the exact-Π judge with its 20 % fallback threshold."""
+    model alone, the exact pipeline's cost, and the exact-Π check with its 20 % fallback threshold."""
     import numpy as np
     import torch
 
@@ -425,7 +425,7 @@ def closure_leg(args):
     exact = CL.pi_from_tau(fte, tte)
     corr, rel = CL.pi_errors(pred, exact)
     gcorr, grel = CL.pi_errors(CL.pi_from_tau(fte, CL.gradient_model_tau(fte, s2)), exact)
-    # the exact pipeline's cost per field (compute_Π! on FP32 fields through the same plan), and the judge
+    # the exact pipeline's cost per field (compute_Π! on FP32 fields through the same plan), and the check
     u, v, _ = S.cartesian_turbulence(N=N, L=1.0e6, seed=1000 + ntrain)
     ut = torch.tensor(u, dtype=torch.float32, device="cuda:0")
     vt = torch.tensor(v, dtype=torch.float32, device="cuda:0")
@@ -435,11 +435,11 @@ def closure_leg(args):
         cgef.compute_pi_(Pi, ut, vt, None, grid, plan.kernel, plan.scale, filter_plan=plan)
 
     t_exact = timed_steps(exact_step, 10, 2)[0] / 10
-    judge = CL.LearnedClosure(model, grid, plan, threshold=0.2)
+    checked = CL.LearnedClosure(model, grid, plan, threshold=0.2)
     out = torch.empty_like(ut)
-    _, used_exact = judge.compute_pi_(out, ut, vt)
+    _, used_exact = checked.compute_pi_(out, ut, vt)
     log(f"closure: held-out Π corr {corr:.4f} rel-L2 {rel:.4f} (gradient model {gcorr:.4f} / {grel:.4f}); "
-        f"judge error {judge.last_error:.4f}, exact used: {used_exact}")
+        f"checked error {checked.last_error:.4f}, exact used: {used_exact}")
     return {
         "workload": f"doubly periodic {N}x{N} k^-3 box, Gaussian alpha=6, l=8dx; net 4-64-64-64-3, 5x5, GELU, "
                     f"BF16 weights/activations, FP32 accumulation, learns tau minus the gradient model; Adam lr 1e-3, "
@@ -450,7 +450,7 @@ def closure_leg(args):
         "train_step_kernels_ms": fb_ms, "train_step_tflops": 3 * fwd_flops / (fb_ms * 1e-3) / 1e12,
         "heldout_pi_corr": corr, "heldout_pi_rel_l2": rel,
         "gradient_model_pi_corr": gcorr, "gradient_model_pi_rel_l2": grel,
-        "judge_rel_l2": judge.last_error, "judge_threshold": 0.2, "judge_used_exact": used_exact,
+        "check_rel_l2": checked.last_error, "check_threshold": 0.2, "check_used_exact": used_exact,
         "inference_ms_per_field": 1e3 * t_inf, "exact_pi_ms_per_field": 1e3 * t_exact, "exact_engine": plan.engine,
     }
 

@@ -7,8 +7,9 @@ to what the gradient model τ ≈ σ² G Gᵀ (G = ∇ū, σ² the footprint's s
 (tr S̄ = 0 makes −S̄:G Gᵀ = 0), so Π is carried by that residual. It is trained on the device by this library's own kernels (libcgef_hip.so:
 cgef_conv5 forward and data gradient, cgef_conv5_wgrad, cgef_adam_update, cgef_mse_grad) on pairs
 that the exact filter itself produces (closure_dataset), with BF16 weights and activations and FP32
-accumulation. The exact Π stays its judge: LearnedClosure measures the network's Π against the exact
-Π and falls back to the exact pipeline when the relative L2 error exceeds a threshold.
+accumulation. The exact Π remains the check and the fallback: LearnedClosure measures the network's
+Π against the exact Π and falls back to the exact pipeline when the relative L2 error exceeds a
+threshold.
 
 Tensors are torch device tensors; torch provides memory only. Every compute step is a HIP kernel.
 """
@@ -540,7 +541,7 @@ def fit_closure(feat, tau, sigma2, **train):
 
 
 class LearnedClosure:
-    """Π from a ClosureModel, judged against the exact Π: the first call and every `check_every`-th
+    """Π from a ClosureModel, checked against the exact Π: the first call and every `check_every`-th
     call also compute the exact Π (compute_Π! through this library's filter) and the relative L2
     error; above `threshold` (default 20 %) that call returns the exact Π and the closure stays on
     the exact path until `revalidate()`."""

@@ -171,7 +171,7 @@ cgef_status cgef_spectral_density(int64_t n, const double *C, const double *k, d
  * Learned closure for τ_ℓ. No counterpart in the reference, which obtains τ only from the exact
  * filter: these entry points train and run a 4-layer 5×5 convolutional network (BF16 weights and
  * activations, FP32 accumulation) whose targets are the exact τ this library's filter produces,
- * and the exact Π stays its judge and fallback. Tensors are device (n, channels, rows, cols), cols
+ * and the exact Π remains the check and the fallback. Tensors are device (n, channels, rows, cols), cols
  * fastest; BF16 tensors are uint16 bit patterns. All are stream-ordered and do not synchronize.
  * --------------------------------------------------------------------------------------------- */
 
