@@ -78,6 +78,7 @@ class PlanInfo(C.Structure):
         ("ring_offsets", C.c_int64),
         ("ring_modes", C.c_int64),
         ("ring_pi_fields", C.c_int64),
+        ("ring_contract_kernel", C.c_int64),
     ]
 
 

@@ -48,6 +48,12 @@ def _run_coarse_grain(Pi, u, v, w, grid, plans, workspace, nbatch, rho0, E_out):
         raise _lib.ArgumentError("every filter plan must belong to this grid and device")
     if Pi.dtype != u.dtype or v.dtype != u.dtype or (w is not None and w.dtype != u.dtype):
         raise _lib.ArgumentError("Π and the velocity components must share a dtype")
+    # the library writes nbatch × len(plans) × npoints values into Π: check before any launch
+    if Pi.numel() != nbatch * len(plans) * grid.npoints:
+        raise _lib.DimensionMismatch(f"Π holds {Pi.numel()} values; {nbatch} slice(s) × {len(plans)} scale(s) × "
+                                     f"{grid.npoints} points need {nbatch * len(plans) * grid.npoints}")
+    if u.numel() != nbatch * grid.npoints:
+        raise _lib.DimensionMismatch(f"u holds {u.numel()} values, expected {nbatch} slice(s) × {grid.npoints} points")
     arr = (C.c_void_p * len(plans))(*[p.h for p in plans])
     Eb = np.zeros(len(plans) * nbatch, dtype=np.float64)
     _lib.check(_lib.load().cgef_coarse_grain(
@@ -56,6 +62,12 @@ def _run_coarse_grain(Pi, u, v, w, grid, plans, workspace, nbatch, rho0, E_out):
         nbatch, _dtype_code(u), C.c_double(rho0), Eb.ctypes.data_as(C.POINTER(C.c_double)), _stream(dev)))
     E_out[...] = Eb.reshape(np.shape(E_out))
     return Pi
+
+
+def _check_plans(filter_plans, scales):
+    """Pipeline.jl:443-444 builds one plan per scale; a caller-supplied list must match the scales."""
+    if filter_plans is not None and len(filter_plans) != len(scales):
+        raise _lib.DimensionMismatch(f"{len(filter_plans)} filter plans for {len(scales)} scales")
 
 
 def _fill_spectrum(g, Cv, k, policy):
@@ -73,6 +85,9 @@ def coarse_grain_(result: CoarseGrainResult, u, v, w, grid: StructuredGrid, *, s
     ns = len(scales)
     if result.Pi.shape[0] != ns or tuple(result.Pi.shape[1:]) != grid.field_shape:
         raise _lib.DimensionMismatch(f"result.Π has shape {tuple(result.Pi.shape)}, expected {(ns,) + grid.field_shape}")
+    if len(result.cumulative_energy) != ns:
+        raise _lib.DimensionMismatch(f"result holds {len(result.cumulative_energy)} energies for {ns} scales")
+    _check_plans(filter_plans, scales)
     gate_spectrum(kernel, spectrum)
     plans = filter_plans if filter_plans is not None else [
         plan_filter(grid, kernel, float(s), mask_strategy=mask_strategy, backend=backend, method=method)
@@ -117,15 +132,22 @@ def coarse_grain_batch_(batch: CoarseGrainBatchResult, u, v, w, grid: Structured
                         method=None):
     """Pipeline.jl:688-724 with the GPU batch driver (519-563): one compute_Π! per scale over the
     whole batch (a single launch sequence covering every snapshot), E(ℓ) per slice."""
-    dev = u.device.index if u.is_cuda else -1
     bs = batch.batch_size
+    ns = len(scales)
+    # sizes first, before anything reaches the device (the library writes batch × scales × points)
+    if batch_dims(batch.Pi, grid) != bs + (ns,) or tuple(batch.Pi.shape[len(bs) + 1:]) != grid.field_shape:
+        raise _lib.DimensionMismatch(f"batch.Π has shape {tuple(batch.Pi.shape)}, expected {bs + (ns,) + grid.field_shape}")
+    if np.shape(batch.cumulative_energy) != bs + (ns,):
+        raise _lib.DimensionMismatch(f"batch holds energies of shape {np.shape(batch.cumulative_energy)}, "
+                                     f"expected {bs + (ns,)}")
+    _check_plans(filter_plans, scales)
+    dev = u.device.index if u.is_cuda else -1
     for name, t in (("u", u), ("v", v), ("w", w)):
         if t is not None:
             _check_field(t, name, grid, dev)
             if batch_dims(t, grid) != bs:
                 raise _lib.DimensionMismatch(f"{name}'s batch dimensions {batch_dims(t, grid)} do not match batch size {bs}")
     gate_spectrum(kernel, spectrum)
-    ns = len(scales)
     ws = workspaces[0] if workspaces else PiWorkspace(grid, bs, u.dtype, dev)
     if ws.nbatch < (int(np.prod(bs)) if bs else 1):
         raise _lib.DimensionMismatch("coarse_grain_batch! needs a batch-sized workspace")

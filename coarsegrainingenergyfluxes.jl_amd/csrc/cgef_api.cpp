@@ -134,6 +134,7 @@ Workspace::~Workspace() {
     dfree(rxy);
     dfree(rxf);
     dfree(rout);
+    dfree(pistage);
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (cap_stream) (void)hipStreamDestroy(cap_stream);
 }
@@ -665,11 +666,16 @@ cgef_status cgef_plan_create(cgef_grid *grid, const cgef_kernel_desc *kernel, do
                 p->engine = CGEF_ENGINE_SEPARABLE;
                 build_separable(*p);
             } else {
-                if (eng == CGEF_ENGINE_AUTO) eng = ring_grid_ok(*g) ? CGEF_ENGINE_RING : CGEF_ENGINE_BANDED;
+                const bool auto_eng = eng == CGEF_ENGINE_AUTO;
+                if (auto_eng) eng = ring_grid_ok(*g) ? CGEF_ENGINE_RING : CGEF_ENGINE_BANDED;
                 build_banded(*p, eng != CGEF_ENGINE_RING);
-                if (eng == CGEF_ENGINE_RING && !ring_supported(*p))
-                    fail(CGEF_ERR_UNSUPPORTED,
-                         "the ring engine needs a periodic, uniform x axis whose length factors into 2,3,5,7,11,13");
+                std::string why;
+                if (eng == CGEF_ENGINE_RING && !ring_supported(*p, &why)) {
+                    if (!auto_eng) fail(CGEF_ERR_UNSUPPORTED, why);
+                    // AUTO: the same footprint by the fused direct sum (its FP32 tables are needed now)
+                    eng = CGEF_ENGINE_BANDED;
+                    build_banded(*p, true);
+                }
                 p->engine = eng;
                 p->band_of_row_d = upload(p->band_of_row);
                 p->D64_row_d = upload(p->D64_row);
@@ -726,6 +732,16 @@ cgef_status cgef_plan_query(const cgef_plan *plan, cgef_plan_info *info) {
         info->ring_offsets = p->ring.ok ? p->ring.args.noff : 0;
         info->ring_modes = p->ring.ok ? p->ring.mh : 0;
         info->ring_pi_fields = !p->ring.ok ? 0 : p->ring.spectral ? 5 : ring_nfields(*p->grid, ring_withmask(*p->grid, p->strategy));
+        info->ring_contract_kernel = -1;
+        if (p->ring.ok && !p->ring.spectral) {
+            std::lock_guard<std::mutex> lk(p->ring.tune->mu);
+            int64_t widest = -1;
+            for (const auto &c : p->ring.tune->choice)
+                if (c.first > widest) {
+                    widest = c.first;
+                    info->ring_contract_kernel = c.second;
+                }
+        }
     });
 }
 
@@ -930,15 +946,18 @@ cgef_status cgef_coarse_grain(cgef_plan *const *plans, int64_t nscales, cgef_wor
                 }
                 CGEF_HIP(hipGetLastError());
             };
-            // The sweep as one HIP graph: the first sweep with a given set of arguments runs eagerly
-            // (and settles the contraction's measured variants), the second is captured on the
-            // workspace's own stream and launched, later ones replay it. A graph holds no memset
-            // node. CGEF_GRAPH=0 turns this off; plans being profiled always run eagerly. Calls that
-            // share a workspace must not overlap (as for every entry point using it).
+            // Optionally the sweep as one HIP graph (CGEF_GRAPH=1): the first sweep with a given set
+            // of arguments runs eagerly (and settles the contraction's measured variants), the second
+            // is captured on the workspace's own stream and launched, later ones replay it. A graph
+            // holds no memset node. Off by default: on MI355X the replayed 8-snapshot × 24-scale
+            // sweep on the 0.25° sphere measured 45.4 ms against 33.2 ms launched eagerly (its
+            // 145 launches run 0.1–2.6 ms each, so launch overhead is not what limits it). Plans
+            // being profiled always run eagerly. Calls that share a workspace must not overlap (as
+            // for every entry point using it).
             bool profiling = false;
             for (int64_t q = 0; q < nscales; q++) profiling = profiling || reinterpret_cast<Plan *>(plans[q])->profiling;
             const char *ge = std::getenv("CGEF_GRAPH");
-            const bool use_graph = !(ge && ge[0] == '0') && !profiling && !W.graph_broken;
+            const bool use_graph = (ge && ge[0] == '1') && !profiling && !W.graph_broken;
             std::vector<uint64_t> key;
             if (use_graph) {
                 auto u64 = [](const void *ptr) { return (uint64_t)(uintptr_t)ptr; };
@@ -981,17 +1000,27 @@ cgef_status cgef_coarse_grain(cgef_plan *const *plans, int64_t nscales, cgef_wor
                 if (use_graph) W.seen_key = key;
             }
         } else {
-            // other engines: one compute_Π! per scale, written into the scale's strided slots
+            // other engines: one compute_Π! per scale, written into the scale's strided slots — directly
+            // for one slice, else through the workspace's Π staging buffer (allocated on first use and
+            // kept, as every workspace buffer is)
+            void *pis = nullptr;
+            if (nbatch > 1) {
+                if (W.npistage < (size_t)nbatch * P * es) {
+                    dfree(W.pistage);
+                    CGEF_HIP(hipMalloc(&W.pistage, (size_t)nbatch * P * es));
+                    W.npistage = (size_t)nbatch * P * es;
+                }
+                pis = W.pistage;
+            }
+            std::vector<double> Eq(nbatch);
             for (int64_t q = 0; q < nscales; q++) {
-                std::vector<double> Eq(nbatch);
-                void *pis = nullptr;
-                CGEF_HIP(hipMallocAsync(&pis, (size_t)nbatch * P * es, s));
-                cgef_status st = cgef_compute_pi(plans[q], ws, pis, u, v, w, nbatch, dtype, rho0,
-                                                 E_out ? Eq.data() : nullptr, nullptr, nullptr, nullptr, stream);
+                void *dst = nbatch > 1 ? pis : (void *)((char *)Pi + (size_t)q * P * es);
+                cgef_status st = cgef_compute_pi(plans[q], reinterpret_cast<cgef_workspace *>(&W), dst, u, v, w, nbatch,
+                                                 dtype, rho0, E_out ? Eq.data() : nullptr, nullptr, nullptr, nullptr, stream);
                 if (st != CGEF_OK) fail(st, g_last_error);
-                CGEF_HIP(hipMemcpy2DAsync((char *)Pi + (size_t)q * P * es, (size_t)nscales * P * es, pis, (size_t)P * es,
-                                          (size_t)P * es, (size_t)nbatch, hipMemcpyDeviceToDevice, s));
-                CGEF_HIP(hipFreeAsync(pis, s));
+                if (nbatch > 1)
+                    CGEF_HIP(hipMemcpy2DAsync((char *)Pi + (size_t)q * P * es, (size_t)nscales * P * es, pis,
+                                              (size_t)P * es, (size_t)P * es, (size_t)nbatch, hipMemcpyDeviceToDevice, s));
                 if (E_out)
                     for (int64_t b = 0; b < nbatch; b++) E_out[q + nscales * b] = Eq[b];
             }

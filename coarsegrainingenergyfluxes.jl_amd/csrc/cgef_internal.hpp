@@ -77,6 +77,15 @@ struct RingFft {
     int fixed = 0;                     // RING_FIXED_*: srad is that compile-time factorization (0: none)
 };
 
+// Columns (planes) per block of the ring engine's banded contraction, whose LDS slab holds
+// RING_CONTRACT_CC × (256 + the footprint's dy range) complex FP64 values.
+constexpr int RING_CONTRACT_CC = 5;
+// Largest Ŵ table (mh × offsets × bands FP64) a ring plan may hold; wider footprints go to the
+// banded direct sum under engine = AUTO.
+constexpr double RING_WHAT_BUDGET_BYTES = 32.0 * 1024 * 1024 * 1024;
+// LDS per CU on gfx950, which is also the most one workgroup may take.
+constexpr size_t LDS_BYTES_PER_CU = 160 * 1024;
+
 struct RingArgs {
     int64_t ny = 1, nz = 1, nrows = 1;
     int64_t mh = 0;                    // Fourier modes kept (N/2 + 1)
@@ -241,6 +250,9 @@ struct Workspace {
     double *rxy = nullptr;
     double2 *rxf = nullptr, *rout = nullptr;
     size_t nrxy = 0, nrxf = 0;
+    // Π of one scale for nbatch slices before it is scattered into a multi-scale result
+    void *pistage = nullptr;
+    size_t npistage = 0;
     // multi-scale sweeps (cgef_coarse_grain, ring and spectral plans) replayed as one HIP graph: the
     // arguments of the last eager sweep (seen_key) and of the captured one (graph_key)
     std::vector<uint64_t> seen_key, graph_key;

@@ -16,6 +16,7 @@
 // filtered fields and τ, ū, and the spectral engine's column transforms (k_spec_cols).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdlib>
 
 #include <cfloat>
@@ -545,9 +546,6 @@ __global__ __launch_bounds__(256) void k_ifft_rows_staged(RingFft F, const doubl
 // a plane is issued before its staging, so the weight latency hides behind the staging loads. PB and
 // the occupancy bound MINW (waves per SIMD the register allocation must allow) trade prefetch depth
 // against resident blocks; ring_contract picks them per launch.
-#ifndef CGEF_CONTRACT_CC
-#define CGEF_CONTRACT_CC 5
-#endif
 
 template <int CC, int PB, int MINW>
 __global__ __launch_bounds__(256, MINW) void k_contract(RingArgs A, const double2 *__restrict__ XF, double2 *__restrict__ OUT,
@@ -991,7 +989,7 @@ void ring_inverse(const RingPlan &R, const double2 *OUT, double *yout, int64_t n
 // 1 = <4, 1>, 2 = <6, 6>. Which is fastest depends on the footprint's row span (LDS slab, hence
 // blocks per CU), the number of vertical offset groups and the column count. The first call of a
 // plan for a column count times all three on that call's operands (each overwrites OUT with the same
-// sums; the chosen one runs last) and keeps the fastest; later calls, and calls under stream capture,
+// sums; the chosen one runs last), each by the median of three launches, and keeps the fastest; later calls, and calls under stream capture,
 // use the kept choice or, before any measurement, the occupancy rule below. CGEF_TUNE=0 turns the
 // measurement off; CGEF_CONTRACT_VARIANT=0|1|2 forces a variant (testing).
 static int contract_rule(const RingPlan &R, int64_t ncol, size_t lds_blocks) {
@@ -1001,13 +999,13 @@ static int contract_rule(const RingPlan &R, int64_t ncol, size_t lds_blocks) {
 }
 
 void ring_contract(const RingPlan &R, const double2 *XF, double2 *OUT, int64_t ncol, hipStream_t s) {
-    constexpr int CC = CGEF_CONTRACT_CC;
+    constexpr int CC = RING_CONTRACT_CC;
     const int span = 256 + R.args.dy_max - R.args.dy_min;
     const size_t sh = (size_t)CC * span * sizeof(double2);
     const int64_t nyb = (R.args.ny + 255) / 256;
     const int64_t items = (ncol + CC - 1) / CC * nyb * R.args.nz * R.mh;
     const dim3 grid((unsigned)(8 * ((items + 7) / 8)));
-    const size_t lds_blocks = (size_t)160 * 1024 / sh;
+    const size_t lds_blocks = LDS_BYTES_PER_CU / sh;
     auto launch = [&](int variant) {
         auto go = [&](auto kern) {
             set_shmem((const void *)kern, sh);
@@ -1041,18 +1039,24 @@ void ring_contract(const RingPlan &R, const double2 *XF, double2 *OUT, int64_t n
         launch(contract_rule(R, ncol, lds_blocks));
         return;
     }
-    hipEvent_t ev[2];
+    // each variant: one untimed launch (code load, LDS limit), then the median of three timed ones
+    constexpr int kReps = 3;
+    hipEvent_t ev[kReps + 1];
     for (auto &e : ev) CGEF_HIP(hipEventCreate(&e));
     float best_ms = 0.0f;
     int best = contract_rule(R, ncol, lds_blocks);
     for (int v = 0; v < 3; ++v) {
-        launch(v);   // first launch of the variant (code load, LDS limit)
-        CGEF_HIP(hipEventRecord(ev[0], s));
         launch(v);
-        CGEF_HIP(hipEventRecord(ev[1], s));
-        CGEF_HIP(hipEventSynchronize(ev[1]));
-        float ms = 0.0f;
-        CGEF_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+        CGEF_HIP(hipEventRecord(ev[0], s));
+        for (int r = 0; r < kReps; ++r) {
+            launch(v);
+            CGEF_HIP(hipEventRecord(ev[r + 1], s));
+        }
+        CGEF_HIP(hipEventSynchronize(ev[kReps]));
+        float t[kReps];
+        for (int r = 0; r < kReps; ++r) CGEF_HIP(hipEventElapsedTime(&t[r], ev[r], ev[r + 1]));
+        std::sort(t, t + kReps);
+        const float ms = t[kReps / 2];
         if (v == 0 || ms < best_ms) {
             best_ms = ms;
             best = v;

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <string>
 #include <vector>
 
 #include "cgef_launch.hpp"
@@ -15,7 +16,7 @@ bool ring_factor(int n, RingFft &F);
 // butterflies fit the per-thread register budget of a 256-thread block, odd radices first. False when
 // there is none or the length exceeds 4096 (then the LDS ping-pong FFT serves).
 bool ring_staged_factor(RingFft &F);
-bool ring_supported(const Plan &p);
+bool ring_supported(const Plan &p, std::string *why = nullptr);
 void fft_tables(RingFft &F, double2 *&tw_d, double2 *&tws_d);
 bool spectral_supported(const Grid &g, int kind);
 void build_spectral(Plan &p, const GeomArgs &G, hipStream_t s);

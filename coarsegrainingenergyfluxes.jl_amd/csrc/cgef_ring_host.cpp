@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cmath>
 #include <map>
+#include <string>
 
 #include "cgef_ring.hpp"
 
@@ -41,12 +42,38 @@ bool ring_grid_ok(const Grid &g) {
     return ring_factor((int)g.n[0], F);
 }
 
-// ... and a plan: its runs must be symmetric modulo the ring (so Ŵ is real).
-bool ring_supported(const Plan &p) {
-    if (!ring_grid_ok(*p.grid)) return false;
+// ... and a plan: its runs must be symmetric modulo the ring (so Ŵ is real), the contraction's LDS
+// slab (RING_CONTRACT_CC columns × (256 + dy range) complex values) must fit a workgroup, and Ŵ
+// (mh × offsets × bands FP64, each dz group's dy range filled) must stay within the plan budget.
+bool ring_supported(const Plan &p, std::string *why) {
+    auto no = [&](const std::string &m) {
+        if (why) *why = m;
+        return false;
+    };
+    if (!ring_grid_ok(*p.grid))
+        return no("the ring engine needs a periodic, uniform x axis whose length factors into 2,3,5,7,11,13");
     const Grid &g = *p.grid;
-    for (const Run &r : p.runs)
-        if (!(r.lo == -r.hi || (int64_t)(r.hi - r.lo + 1) == g.n[0])) return false;
+    std::map<int, std::pair<int, int>> dy_range;
+    int dymin = 0, dymax = 0;
+    for (const Run &r : p.runs) {
+        if (!(r.lo == -r.hi || (int64_t)(r.hi - r.lo + 1) == g.n[0]))
+            return no("the footprint's longitude runs are not symmetric about the target column");
+        auto it = dy_range.find(r.dz);
+        if (it == dy_range.end()) dy_range[r.dz] = {r.dy, r.dy};
+        else it->second = {std::min(it->second.first, r.dy), std::max(it->second.second, r.dy)};
+        dymin = std::min(dymin, r.dy);
+        dymax = std::max(dymax, r.dy);
+    }
+    int64_t noff = 0;
+    for (const auto &kv : dy_range) noff += kv.second.second - kv.second.first + 1;
+    const size_t slab = (size_t)RING_CONTRACT_CC * (size_t)(256 + dymax - dymin) * sizeof(double2);
+    if (slab > LDS_BYTES_PER_CU)
+        return no("the footprint spans " + std::to_string(dymax - dymin + 1) +
+                  " source rows: the ring contraction's LDS slab would exceed 160 KiB");
+    const double what = (double)(g.n[0] / 2 + 1) * (double)noff * (double)p.nbands * sizeof(double);
+    if (what > RING_WHAT_BUDGET_BYTES)
+        return no("the ring engine's Fourier weights would take " + std::to_string(what / 1073741824.0) +
+                  " GiB, above the 32 GiB per-plan budget");
     return true;
 }
 

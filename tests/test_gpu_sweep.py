@@ -1,5 +1,5 @@
-"""The multi-scale sweep's launch machinery on the GPU: the HIP-graph replay of cgef_coarse_grain and
-the measured choice of the contraction kernel variant. Every replay is compared bit for bit with the
+"""The multi-scale sweep's launch machinery on the GPU: the HIP-graph replay of cgef_coarse_grain (opt-in,
+CGEF_GRAPH=1) and the measured choice of the contraction kernel variant. Every replay is compared bit for bit with the
 eager sweep (same kernels, same order), with the program's own eager work and synchronizations
 between replays, and after the inputs change in place; every forced contraction variant gives the
 same bits."""
@@ -49,7 +49,7 @@ def test_graph_replays_equal_the_eager_sweep(spectral, monkeypatch):
     ws = cgef.PiWorkspace(g, (2,), torch.float32)
     monkeypatch.setenv("CGEF_GRAPH", "0")
     ref_pi, ref_e = _sweep(batch, U, V, g, scales, ker, plans, ws)
-    monkeypatch.delenv("CGEF_GRAPH")
+    monkeypatch.setenv("CGEF_GRAPH", "1")
     other = torch.empty((2,) + g.field_shape, dtype=torch.float32, device=DEV)
     for call in range(5):   # eager (arguments seen), captured + launched, then replays
         batch.Pi.zero_()

@@ -130,6 +130,31 @@ def test_slice_entry_points_check_lengths_before_touching_a_device():
     assert cgef.slice_pipeline_costs([g, g], 3) == [144, 144]
 
 
+def test_coarse_grain_checks_plan_and_result_sizes_before_touching_a_device():
+    # Pipeline.jl:443-444 (one plan per scale) and the result shape (Pipeline.jl:1286-1298): a list of
+    # plans or a result sized for other scales is a DimensionMismatch before any library call
+    import torch
+    g = cgef.StructuredGrid(cgef.CartesianGeometry(), np.arange(8.0), np.arange(6.0))
+    u = torch.zeros(g.field_shape, dtype=torch.float64)
+    res = cgef.pipeline._allocate_result(g, 2, torch.float64, "cpu")
+    fake = [_FakePlan(g)] * 3
+    with pytest.raises(cgef.DimensionMismatch, match="3 filter plans for 2 scales"):
+        cgef.coarse_grain_(res, u, u, None, g, scales=[1.0, 2.0], kernel=cgef.GaussianKernel(6.0), filter_plans=fake)
+    with pytest.raises(cgef.DimensionMismatch, match="result.Π has shape"):
+        cgef.coarse_grain_(res, u, u, None, g, scales=[1.0, 2.0, 3.0], kernel=cgef.GaussianKernel(6.0))
+    batch = cgef.CoarseGrainBatchResult.allocate(g, 2, (4,), torch.float64, "cpu")
+    ub = torch.zeros((4,) + g.field_shape, dtype=torch.float64)
+    with pytest.raises(cgef.DimensionMismatch, match="batch.Π has shape"):
+        cgef.coarse_grain_batch_(batch, ub, ub, None, g, scales=[1.0, 2.0, 3.0], kernel=cgef.GaussianKernel(6.0))
+    with pytest.raises(cgef.DimensionMismatch, match="3 filter plans for 2 scales"):
+        cgef.coarse_grain_batch_(batch, ub, ub, None, g, scales=[1.0, 2.0], kernel=cgef.GaussianKernel(6.0),
+                                 filter_plans=fake)
+    other = cgef.CoarseGrainBatchResult.allocate(g, 2, (3,), torch.float64, "cpu")
+    other.batch_size = (4,)   # Π allocated for 3 slices, the call claims 4
+    with pytest.raises(cgef.DimensionMismatch, match="batch.Π has shape"):
+        cgef.coarse_grain_batch_(other, ub, ub, None, g, scales=[1.0, 2.0], kernel=cgef.GaussianKernel(6.0))
+
+
 def test_closure_entry_points_validate_on_the_host():
     """The learned closure's C-ABI entry points reject bad arguments before any device call."""
     import ctypes as C
