@@ -24,6 +24,13 @@
 
 #define GAUSSIAN_TRUNCATION_TOL 1e-10 /* Kernels.jl:307 */
 
+/* Planted mistakes, for showing that the pinning tests catch them (0 = none, the restatement):
+ * OG_MUTATE_FORWARD_WIDTH — forward-difference widths on non-uniform axes; OG_MUTATE_RUNEND_2NODE —
+ * two-node stencils at domain and mask-run ends instead of the shifted three-node window. */
+static int g_mutation = 0;
+void og_set_mutation(int m) { g_mutation = m; }
+
+
 /* ---------------------------------------------------------------- kernels (Kernels.jl) ----- */
 
 /* Kernels.jl:318-324 — unnormalized weights. */
@@ -57,6 +64,7 @@ double og_cell_width(const og_grid *g, int axis, int64_t i) {
     const double *x = g->x[axis];
     if (n < 2) return 1.0;
     if (g->uniform[axis]) return fabs(x[1] - x[0]);
+    if (g_mutation == OG_MUTATE_FORWARD_WIDTH) return i < n - 1 ? fabs(x[i + 1] - x[i]) : fabs(x[n - 1] - x[n - 2]);
     if (g->periodic[axis]) {
         double lo = (i == 0) ? x[n - 1] - g->period[axis] : x[i - 1];
         double hi = (i == n - 1) ? x[0] + g->period[axis] : x[i + 1];
@@ -109,6 +117,12 @@ static void vpush(vec_t *v, int32_t di, int32_t dj, double w) {
 }
 
 og_footprint *og_footprint_build(const og_grid *g, int kind, double alpha, double ell) {
+    return og_footprint_build_bands(g, kind, alpha, ell, NULL);
+}
+
+/* bandsel (spherical grids; NULL = all): bands it leaves unmarked are built empty, so a check on a
+ * few latitude rows builds only their bands. A marked band's entries are those of the full build. */
+og_footprint *og_footprint_build_bands(const og_grid *g, int kind, double alpha, double ell, const uint8_t *bandsel) {
     og_footprint *fp = (og_footprint *)calloc(1, sizeof(og_footprint));
     vec_t v = {0};
     double rad = og_kernel_radius(kind, alpha, ell);
@@ -143,6 +157,10 @@ og_footprint *og_footprint_build(const og_grid *g, int kind, double alpha, doubl
          * yields the same entries in the same order, so the exact support plus one is used. */
         for (int64_t j = 0; j < Ny; j++) {
             int64_t kmax = -1;
+            if (bandsel && !bandsel[j]) {
+                dil[j] = 0;
+                continue;
+            }
             for (int64_t ddj = -dj_lim; ddj <= dj_lim; ddj++) {
                 int64_t jj = j + ddj;
                 if (jj < 0 || jj >= Ny) continue;
@@ -161,6 +179,10 @@ og_footprint *og_footprint_build(const og_grid *g, int kind, double alpha, doubl
         for (int64_t j = 0; j < Ny; j++) {
             double phi = g->x[1][j];
             int64_t di_lim = dil[j];
+            if (bandsel && !bandsel[j]) {
+                fp->ptr[j + 1] = v.n;
+                continue;
+            }
             for (int64_t ddj = -dj_lim; ddj <= dj_lim; ddj++) {
                 int64_t jj = j + ddj;
                 if (jj < 0 || jj >= Ny) continue;
@@ -205,8 +227,8 @@ void og_footprint_copy(const og_footprint *fp, int32_t *di, int32_t *dj, double 
  * neighbour walk per point shared by K fields, k ascending; ZeroFill counts inactive neighbours
  * in the denominator only, Deformable drops them; ws/wn if wn > 1e-15, inactive targets 0.
  * Rows in parallel with a dynamic schedule, as OhMyThreadsExt.jl:20-43, 97-123. */
-void og_apply_banded(const og_grid *g, const og_footprint *fp, int strategy, int K,
-                     const double *const *in, double *const *out, int nthreads) {
+static void apply_banded_rows(const og_grid *g, const og_footprint *fp, int strategy, int K,
+                              const double *const *in, double *const *out, const uint8_t *rowsel, int nthreads) {
     int64_t Nx = g->n[0], Ny = g->n[1];
     int px = g->periodic[0], py = g->periodic[1];
 #ifdef _OPENMP
@@ -214,6 +236,7 @@ void og_apply_banded(const og_grid *g, const og_footprint *fp, int strategy, int
 #pragma omp parallel for schedule(dynamic) num_threads(nthreads)
 #endif
     for (int64_t j = 0; j < Ny; j++) {
+        if (rowsel && !rowsel[j]) continue;
         double ws[16];
         int64_t b = fp->nbands == 1 ? 0 : j;
         int64_t lo = fp->ptr[b], hi = fp->ptr[b + 1];
@@ -251,6 +274,11 @@ void og_apply_banded(const og_grid *g, const og_footprint *fp, int strategy, int
             for (int m = 0; m < K; m++) out[m][t] = wn > 1e-15 ? ws[m] / wn : 0.0;
         }
     }
+}
+
+void og_apply_banded(const og_grid *g, const og_footprint *fp, int strategy, int K,
+                     const double *const *in, double *const *out, int nthreads) {
+    apply_banded_rows(g, fp, strategy, K, in, out, NULL, nthreads);
 }
 
 /* ------------------------------------- separable Gaussian (Filtering.jl:1143-1561) ---------- */
@@ -764,10 +792,10 @@ void og_deriv(const og_grid *g, int axis, const double *f, double *df) {
         if (ok[1] && ok[3]) {
             lag3(xc[2], xc[1], xc[2], xc[3], w3);
             v = w3[0] * f[base + ix[1] * stride] + w3[1] * f[t] + w3[2] * f[base + ix[3] * stride];
-        } else if (ok[3] && ok[4]) {
+        } else if (ok[3] && ok[4] && g_mutation != OG_MUTATE_RUNEND_2NODE) {
             lag3(xc[2], xc[2], xc[3], xc[4], w3);
             v = w3[0] * f[t] + w3[1] * f[base + ix[3] * stride] + w3[2] * f[base + ix[4] * stride];
-        } else if (ok[1] && ok[0]) {
+        } else if (ok[1] && ok[0] && g_mutation != OG_MUTATE_RUNEND_2NODE) {
             lag3(xc[2], xc[0], xc[1], xc[2], w3);
             v = w3[0] * f[base + ix[0] * stride] + w3[1] * f[base + ix[1] * stride] + w3[2] * f[t];
         } else if (ok[3]) {
@@ -825,21 +853,57 @@ void og_tensor_to_local(const double t[6], double lam, double phi, double out[6]
 
 /* ------------------------------------------------------------------------ Π (Diagnostics.jl) - */
 
+/* rowsel (2-D banded engine only; NULL = every row): filter only the rows it marks. */
 static void filt(const og_grid *g, const og_filter_spec *s, const og_footprint *fp, int K,
-                 const double *const *in, double *const *out) {
+                 const double *const *in, double *const *out, const uint8_t *rowsel) {
     if (s->engine == 1) og_apply_separable(g, s->alpha, s->ell, s->strategy, K, in, out);
     else if (s->engine == 2) og_apply_nd3(g, s->kind, s->alpha, s->ell, s->strategy, K, in, out, s->nthreads);
     else if (s->engine == 3) og_apply_prefix(g, s->ell, s->strategy, K, in, out, s->nthreads);
     else if (s->engine == 4) og_apply_spectral(g, s->kind, s->alpha, s->ell, s->strategy, K, in, out);
-    else og_apply_banded(g, fp, s->strategy, K, in, out, s->nthreads);
+    else apply_banded_rows(g, fp, s->strategy, K, in, out, rowsel, s->nthreads);
 }
+
+static void compute_pi_impl(const og_grid *g, const og_filter_spec *spec, const og_footprint *fp,
+                            const double *u, const double *v, const double *w, const uint8_t *rowsel,
+                            double *Pi, double *ubar, double *vbar, double *wbar);
 
 void og_compute_pi(const og_grid *g, const og_filter_spec *spec, const og_footprint *fp,
                    const double *u, const double *v, const double *w,
                    double *Pi, double *ubar, double *vbar, double *wbar) {
+    compute_pi_impl(g, spec, fp, u, v, w, NULL, Pi, ubar, vbar, wbar);
+}
+
+/* The same Π at the rows `rows` marks only (2-D grids, banded footprint): the filter runs on those
+ * rows and the two on either side (all the derivative stencils reach), the rest is elementwise or
+ * O(N). Π, ū, v̄ are NaN on unmarked rows. The values on marked rows are those og_compute_pi gives,
+ * bit for bit: every row is filtered by the same per-row code. */
+void og_compute_pi_rows(const og_grid *g, const og_filter_spec *spec, const og_footprint *fp,
+                        const double *u, const double *v, const uint8_t *rows,
+                        double *Pi, double *ubar, double *vbar) {
+    int64_t Nx = g->n[0], Ny = g->n[1];
+    uint8_t *need = (uint8_t *)calloc(Ny, 1);
+    for (int64_t j = 0; j < Ny; j++)
+        if (rows[j])
+            for (int64_t d = -2; d <= 2; d++)
+                if (j + d >= 0 && j + d < Ny) need[j + d] = 1;
+    compute_pi_impl(g, spec, fp, u, v, NULL, need, Pi, ubar, vbar, NULL);
+    for (int64_t j = 0; j < Ny; j++)
+        if (!rows[j])
+            for (int64_t i = 0; i < Nx; i++) {
+                Pi[i + Nx * j] = NAN;
+                if (ubar) ubar[i + Nx * j] = NAN;
+                if (vbar) vbar[i + Nx * j] = NAN;
+            }
+    free(need);
+}
+
+static void compute_pi_impl(const og_grid *g, const og_filter_spec *spec, const og_footprint *fp,
+                            const double *u, const double *v, const double *w, const uint8_t *rowsel,
+                            double *Pi, double *ubar, double *vbar, double *wbar) {
     int64_t N = g->n[0] * g->n[1] * (g->ndim == 3 ? g->n[2] : 1);
     int64_t Nx = g->n[0], Ny = g->n[1];
-    double *buf = (double *)malloc(24 * N * sizeof(double));
+    /* zeroed, so rows a row selection leaves unfiltered hold finite values */
+    double *buf = (double *)calloc(24 * N, sizeof(double));
     double *B[24];
     for (int q = 0; q < 24; q++) B[q] = buf + q * N;
     if (g->ndim == 3) {
@@ -847,14 +911,14 @@ void og_compute_pi(const og_grid *g, const og_filter_spec *spec, const og_footpr
         double *uf = B[0], *vf = B[1], *wf = B[2];
         const double *vin[3] = {u, v, w};
         double *vout[3] = {uf, vf, wf};
-        filt(g, spec, fp, 3, vin, vout);
+        filt(g, spec, fp, 3, vin, vout, rowsel);
         for (int64_t t = 0; t < N; t++) {
             B[3][t] = u[t] * u[t]; B[4][t] = u[t] * v[t]; B[5][t] = u[t] * w[t];
             B[6][t] = v[t] * v[t]; B[7][t] = v[t] * w[t]; B[8][t] = w[t] * w[t];
         }
         const double *pin[6] = {B[3], B[4], B[5], B[6], B[7], B[8]};
         double *pout[6] = {B[9], B[10], B[11], B[12], B[13], B[14]}; /* uu uv uw vv vw ww */
-        filt(g, spec, fp, 6, pin, pout);
+        filt(g, spec, fp, 6, pin, pout, rowsel);
         double *Sxx = B[15], *Syy = B[16], *Szz = B[17], *Sxy = B[18], *Sxz = B[19], *Syz = B[20], *sc = B[21];
         og_deriv(g, 0, uf, Sxx);
         og_deriv(g, 1, vf, Syy);
@@ -880,13 +944,13 @@ void og_compute_pi(const og_grid *g, const og_filter_spec *spec, const og_footpr
         double *uf = B[0], *vf = B[1];
         const double *vin[2] = {u, v};
         double *vout[2] = {uf, vf};
-        filt(g, spec, fp, 2, vin, vout);
+        filt(g, spec, fp, 2, vin, vout, rowsel);
         for (int64_t t = 0; t < N; t++) {
             B[2][t] = u[t] * u[t]; B[3][t] = u[t] * v[t]; B[4][t] = v[t] * v[t];
         }
         const double *pin[3] = {B[2], B[3], B[4]};
         double *pout[3] = {B[5], B[6], B[7]};
-        filt(g, spec, fp, 3, pin, pout);
+        filt(g, spec, fp, 3, pin, pout, rowsel);
         double *Sxx = B[8], *Sxy = B[9], *Syy = B[10], *sc = B[11];
         og_deriv(g, 0, uf, Sxx);
         og_deriv(g, 1, uf, Sxy);
@@ -917,14 +981,14 @@ void og_compute_pi(const og_grid *g, const og_filter_spec *spec, const og_footpr
         double *uxf = B[3], *uyf = B[4], *uzf = B[5];
         const double *vin[3] = {ux, uy, uz};
         double *vout[3] = {uxf, uyf, uzf};
-        filt(g, spec, fp, 3, vin, vout);
+        filt(g, spec, fp, 3, vin, vout, rowsel);
         for (int64_t t = 0; t < N; t++) {
             B[6][t] = ux[t] * ux[t]; B[7][t] = ux[t] * uy[t]; B[8][t] = ux[t] * uz[t];
             B[9][t] = uy[t] * uy[t]; B[10][t] = uy[t] * uz[t]; B[11][t] = uz[t] * uz[t];
         }
         const double *pin[6] = {B[6], B[7], B[8], B[9], B[10], B[11]};
         double *pout[6] = {B[12], B[13], B[14], B[15], B[16], B[17]}; /* uu uv uw vv vw ww */
-        filt(g, spec, fp, 6, pin, pout);
+        filt(g, spec, fp, 6, pin, pout, rowsel);
         double *uf = B[6], *vf = B[7], *tee = B[8], *ten = B[9], *tnn = B[10];
         for (int64_t j = 0; j < Ny; j++)
             for (int64_t i = 0; i < Nx; i++) {

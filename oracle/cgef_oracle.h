@@ -9,11 +9,16 @@
  * (y / latitude), then k (z), then trailing batch. 0-based indices here.
  *
  * Pinning: tests/test_oracle.py checks these functions against the reference's own known answers,
- * closed forms, invariants and brute force. What only the un-vendored FlowGeometries.jl could fix
- * is "parity unpinned", with the choice made here (the HIP kernels make the same one):
- *   - og_deriv at domain and mask edges: pinned only to exactness for linear fields; here a 3-node
- *     one-sided stencil when two same-side neighbours are active, 2 nodes when one is, 0 if none;
- *   - og_cell_width / og_area on non-uniform axes: midpoint widths, half cells at non-periodic ends;
+ * closed forms, invariants, the identities its test suite asserts, and brute force. Two choices the
+ * un-vendored FlowGeometries.jl makes are pinned by what the reference states and the mathematics
+ * fixes, each test also failing on the plausible wrong choice planted with og_set_mutation:
+ *   - og_deriv at domain and mask-run ends: the 3-node window shifted inward (CHANGELOG.md:257-259),
+ *     so a quadratic is differentiated exactly in every run of three or more cells; 2 nodes for a run
+ *     of two, 0 for an isolated cell;
+ *   - og_cell_width on non-uniform axes: midpoint widths with half cells at bounded ends — the widths
+ *     tile the axis, equal the step inside a collected uniform axis (test_grids.jl:203-206), and make
+ *     the measure-weighted filter converge at 2nd order on a smoothly stretched axis.
+ * Still "parity unpinned", with the choice made here (the HIP kernels make the same one):
  *   - periodicity: explicit flags (no auto-detection);
  *   - og_haversine rounding at a spherical top-hat rim;
  *   - inputs are promoted to FP64 (no mixed-precision promotion rules).
@@ -41,6 +46,10 @@ typedef struct {
 enum { OG_TOPHAT = 0, OG_GAUSSIAN = 1 };
 enum { OG_ZEROFILL = 0, OG_DEFORMABLE = 1 };
 
+/* Test-only: plant a known mistake so a pinning test can show it fails on it (0 restores). */
+enum { OG_MUTATE_NONE = 0, OG_MUTATE_FORWARD_WIDTH = 1, OG_MUTATE_RUNEND_2NODE = 2 };
+void og_set_mutation(int m);
+
 double og_kernel_weight(int kind, double alpha, double d, double ell);
 double og_kernel_radius(int kind, double alpha, double ell);
 double og_haversine(double R, double lam1, double phi1, double lam2, double phi2);
@@ -58,6 +67,8 @@ typedef struct {
 } og_footprint;
 
 og_footprint *og_footprint_build(const og_grid *g, int kind, double alpha, double ell);
+/* Spherical grids: only the latitude bands bandsel marks (the others empty); NULL = all. */
+og_footprint *og_footprint_build_bands(const og_grid *g, int kind, double alpha, double ell, const uint8_t *bandsel);
 void og_footprint_free(og_footprint *fp);
 int64_t og_footprint_nentries(const og_footprint *fp);
 int64_t og_footprint_nbands(const og_footprint *fp);
@@ -109,6 +120,12 @@ typedef struct {
 void og_compute_pi(const og_grid *g, const og_filter_spec *spec, const og_footprint *fp,
                    const double *u, const double *v, const double *w,
                    double *Pi, double *ubar, double *vbar, double *wbar);
+
+/* Π at the rows rows[j] != 0 marks only (2-D grids, banded footprint; NaN elsewhere), identical on
+ * those rows to og_compute_pi: for checking wide footprints at full size on a few rows. */
+void og_compute_pi_rows(const og_grid *g, const og_filter_spec *spec, const og_footprint *fp,
+                        const double *u, const double *v, const uint8_t *rows,
+                        double *Pi, double *ubar, double *vbar);
 
 double og_active_area(const og_grid *g);
 double og_energy(const og_grid *g, const double *ub, const double *vb, const double *wb);

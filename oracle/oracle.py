@@ -64,6 +64,9 @@ def lib():
         L = C.CDLL(_LIB_PATH)
         dp = C.POINTER(C.c_double)
         gp = C.POINTER(_Grid)
+        L.og_set_mutation.argtypes = [C.c_int]
+        L.og_cell_width.restype = C.c_double
+        L.og_cell_width.argtypes = [gp, C.c_int, C.c_int64]
         L.og_kernel_weight.restype = C.c_double
         L.og_kernel_weight.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double]
         L.og_kernel_radius.restype = C.c_double
@@ -74,6 +77,8 @@ def lib():
         L.og_area.argtypes = [gp, C.c_int64, C.c_int64, C.c_int64]
         L.og_footprint_build.restype = C.c_void_p
         L.og_footprint_build.argtypes = [gp, C.c_int, C.c_double, C.c_double]
+        L.og_footprint_build_bands.restype = C.c_void_p
+        L.og_footprint_build_bands.argtypes = [gp, C.c_int, C.c_double, C.c_double, C.c_void_p]
         L.og_footprint_free.argtypes = [C.c_void_p]
         L.og_footprint_nentries.restype = C.c_int64
         L.og_footprint_nentries.argtypes = [C.c_void_p]
@@ -96,6 +101,7 @@ def lib():
         L.og_vector_from_cartesian.argtypes = [C.c_double] * 5 + [dp]
         L.og_tensor_to_local.argtypes = [dp, C.c_double, C.c_double, dp]
         L.og_compute_pi.argtypes = [gp, C.POINTER(_Spec), C.c_void_p, dp, dp, dp, dp, dp, dp, dp]
+        L.og_compute_pi_rows.argtypes = [gp, C.POINTER(_Spec), C.c_void_p, dp, dp, C.c_void_p, dp, dp, dp]
         L.og_active_area.restype = C.c_double
         L.og_active_area.argtypes = [gp]
         L.og_energy.restype = C.c_double
@@ -176,12 +182,44 @@ def area(grid, i, j, k=0):
     return lib().og_area(grid.ref(), i, j, k)
 
 
+def cell_width(grid, axis, i):
+    return lib().og_cell_width(grid.ref(), axis, i)
+
+
+MUTATE_FORWARD_WIDTH, MUTATE_RUNEND_2NODE = 1, 2
+
+
+class mutation:
+    """Test-only context: the oracle with a planted mistake (forward-difference widths, or two-node
+    stencils at run ends), to show that a pinning test fails on it."""
+
+    def __init__(self, m):
+        self.m = m
+
+    def __enter__(self):
+        lib().og_set_mutation(self.m)
+
+    def __exit__(self, *exc):
+        lib().og_set_mutation(0)
+
+
 class Footprint:
-    def __init__(self, grid, kind, alpha, ell):
+    """The banded footprint (Filtering.jl:636-732). bands: on a spherical grid, build only these
+    latitude bands (the others empty); copy=False leaves the entries in the C table only."""
+
+    def __init__(self, grid, kind, alpha, ell, bands=None, copy=True):
         L = lib()
-        self.h = L.og_footprint_build(grid.ref(), kind, alpha, ell)
+        if bands is None:
+            self.h = L.og_footprint_build(grid.ref(), kind, alpha, ell)
+        else:
+            sel = np.zeros(grid.shape[1], np.uint8)
+            sel[np.asarray(list(bands), dtype=np.int64)] = 1
+            self.h = L.og_footprint_build_bands(grid.ref(), kind, alpha, ell, sel.ctypes.data)
         n = L.og_footprint_nentries(self.h)
         nb = L.og_footprint_nbands(self.h)
+        self.nbands, self.nentries = nb, n
+        if not copy:
+            return
         self.di = np.zeros(n, np.int32)
         self.dj = np.zeros(n, np.int32)
         self.w = np.zeros(n, np.float64)
@@ -308,6 +346,24 @@ def compute_pi(grid, u, v, w=None, *, kind=GAUSSIAN, alpha=6.0, ell, strategy=ZE
     lib().og_compute_pi(grid.ref(), C.byref(spec), fp.h if fp is not None else None,
                         _dp(u), _dp(v), _dp(w), _dp(Pi), _dp(ub), _dp(vb), _dp(wb))
     return Pi, ub, vb, wb
+
+
+def compute_pi_rows(grid, u, v, rows, *, kind=GAUSSIAN, alpha=6.0, ell, strategy=ZEROFILL, fp=None, nthreads=0):
+    """Π (and ū, v̄) of the 2-D banded footprint at the listed rows j only (NaN elsewhere); on those rows
+    identical to compute_pi. Costs the filter on the rows and their two neighbours either side."""
+    _planar_only(grid, "banded")
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    sel = np.zeros(grid.shape[1], np.uint8)
+    sel[np.asarray(list(rows), dtype=np.int64)] = 1
+    if fp is None:
+        need = sorted({min(max(j + d, 0), grid.shape[1] - 1) for j in rows for d in range(-2, 3)})
+        fp = Footprint(grid, kind, alpha, ell, bands=need if grid.spherical else None, copy=False)
+    spec = _Spec(kind, alpha, ell, strategy, ENGINE_BANDED, nthreads)
+    Pi, ub, vb = np.empty_like(u), np.empty_like(u), np.empty_like(u)
+    lib().og_compute_pi_rows(grid.ref(), C.byref(spec), fp.h, _dp(u), _dp(v), sel.ctypes.data, _dp(Pi), _dp(ub),
+                             _dp(vb))
+    return Pi, ub, vb
 
 
 def active_area(grid):

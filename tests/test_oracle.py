@@ -1,5 +1,8 @@
 """Pins the FP64 CPU oracle (the checker) to the reference: its own known-answer numbers, closed forms,
-invariants and brute force on tiny inputs. CPU only."""
+invariants, identities its test suite asserts, and brute force on tiny inputs. CPU only. Where a pin
+guards a choice the reference leaves to an un-vendored dependency (cell widths on non-uniform axes,
+the derivative stencil at run ends), the test also plants the plausible wrong choice in the oracle
+(oracle.mutation) and shows that the pin fails on it."""
 import math
 
 import numpy as np
@@ -548,3 +551,275 @@ def test_spectral_filter_single_mode_and_constant():
     got = O.apply_spectral(og, O.GAUSSIAN, alpha, ell, O.ZEROFILL, [f, np.full_like(f, 2.5)])
     assert np.max(np.abs(got[0] - math.exp(-(k1 ** 2 + k2 ** 2) * ell ** 2 / (4 * alpha)) * f)) <= 1e-13
     assert np.max(np.abs(got[1] - 2.5)) <= 1e-13
+
+
+# ------------------------------------- row-subset Π (for wide footprints at full size) -----------
+
+def test_pi_rows_equals_full_pi_on_its_rows():
+    lam, phi = sphere_axes(144, 72)
+    f = smooth_random_field((72, 144), 3, 3)
+    g = oracle_grid_sphere(lam, phi, mask=f[2] > np.quantile(f[2], 0.3))
+    rows = [0, 1, 10, 35, 70, 71]
+    for ell in (8e5, 3e6):
+        P, ub, vb, _ = O.compute_pi(g, f[0], f[1], kind=O.GAUSSIAN, alpha=4.0, ell=ell)
+        Pr, ubr, vbr = O.compute_pi_rows(g, f[0], f[1], rows, kind=O.GAUSSIAN, alpha=4.0, ell=ell)
+        assert np.array_equal(P[rows], Pr[rows])
+        assert np.array_equal(ub[rows], ubr[rows]) and np.array_equal(vb[rows], vbr[rows])
+        others = np.setdiff1d(np.arange(72), rows)
+        assert np.isnan(Pr[others]).all()
+    # the band-restricted footprint holds exactly the full footprint's entries on its bands
+    full = O.Footprint(g, O.GAUSSIAN, 4.0, 3e6)
+    part = O.Footprint(g, O.GAUSSIAN, 4.0, 3e6, bands=[3, 40])
+    for b in (3, 40):
+        a0, a1 = full.ptr[b], full.ptr[b + 1]
+        b0, b1 = part.ptr[b], part.ptr[b + 1]
+        assert np.array_equal(full.w[a0:a1], part.w[b0:b1]) and np.array_equal(full.di[a0:a1], part.di[b0:b1])
+    assert part.ptr[4] == part.ptr[40]
+
+
+# ----------------------------- cell widths on non-uniform axes (FlowGeometries measure) ---------
+
+def _stretched(N, L=20.0):
+    t = np.linspace(0.0, 1.0, N)
+    return L * (t + 0.15 * np.sin(2 * np.pi * t) / (2 * np.pi))   # smooth, monotone, non-uniform
+
+
+def test_cell_widths_tile_the_axis():
+    # A measure tiles the domain: midpoint widths with half cells at bounded ends sum to the extent,
+    # and to the period on a periodic axis. On a collected (non-Range) uniform axis the interior widths
+    # are the step (test_grids.jl:203-206 checks exactly the interior, 2:N-1, of such an axis).
+    z = np.concatenate([[0.0], np.cumsum(0.5 * 1.07 ** np.arange(19))])
+    one = np.array([0.0])
+    g = O.Grid([one, one, z], periodic=[False, False, False], uniform=[True, True, False])
+    w = np.array([O.cell_width(g, 2, k) for k in range(len(z))])
+    assert np.sum(w) == pytest.approx(z[-1] - z[0], rel=1e-14)
+    assert np.all(w > 0)
+    gp = O.Grid([one, one, z], periodic=[False, False, True], period=[0.0, 0.0, z[-1] + 3.0],
+                uniform=[True, True, False])
+    wp = np.array([O.cell_width(gp, 2, k) for k in range(len(z))])
+    assert np.sum(wp) == pytest.approx(z[-1] + 3.0, rel=1e-14)
+    x1 = np.arange(0.0, 10001.0, 1000.0)
+    gc = O.Grid([x1, one], periodic=[False, False], uniform=[False, True])
+    wc = np.array([O.cell_width(gc, 0, i) for i in range(len(x1))])
+    assert np.allclose(wc[1:-1], 1000.0, rtol=1e-14)
+    with O.mutation(O.MUTATE_FORWARD_WIDTH):      # planted: forward differences over-count the end
+        wf = np.array([O.cell_width(g, 2, k) for k in range(len(z))])
+    assert abs(np.sum(wf) - (z[-1] - z[0])) > 0.1
+
+
+def _filtered_quadratic_error(N, ell=2.0):
+    # Gaussian (α = 6) filter along a smoothly stretched bounded axis of a 1×1×N grid: away from the
+    # ends F[z²] = z₀² + σ², σ² = ℓ²/(2α), up to the truncation at 1e-10. The measure-weighted sum is a
+    # quadrature of the kernel integral: midpoint widths (the trapezoid rule) converge at 2nd order,
+    # forward-difference widths (a one-sided Riemann sum) at 1st.
+    z = _stretched(N)
+    one = np.array([0.0])
+    g = O.Grid([one, one, z], periodic=[False, False, False], uniform=[True, True, False])
+    out = O.apply_nd3(g, O.GAUSSIAN, 6.0, ell, O.ZEROFILL, [(z ** 2)[:, None, None]])[0][:, 0, 0]
+    sel = (z > 6.0) & (z < 14.0)
+    return np.max(np.abs(out[sel] - (z[sel] ** 2 + ell ** 2 / 12.0)))
+
+
+def test_filter_on_a_stretched_axis_converges_at_second_order():
+    Ns = (81, 161, 321, 641)
+    errs = [_filtered_quadratic_error(N) for N in Ns]
+    rates = [math.log2(errs[i] / errs[i + 1]) for i in range(3)]
+    assert all(1.9 < r < 2.1 for r in rates), rates
+    with O.mutation(O.MUTATE_FORWARD_WIDTH):
+        bad = [_filtered_quadratic_error(N) for N in Ns]
+    bad_rates = [math.log2(bad[i] / bad[i + 1]) for i in range(3)]
+    assert all(r < 1.2 for r in bad_rates), bad_rates
+
+
+# ------------------------- derivative stencil at mask-run ends (CHANGELOG.md:257-259) -------------
+
+def test_derivative_quadratic_exact_at_mask_run_ends():
+    # "Bounded-domain end points are now 2nd order ... apply_stencil! keeps three nodes and shifts the
+    # window inward" (CHANGELOG.md:257-259): with the ReduceInRun policy a mask edge is a run end, so in
+    # every run of three or more active cells the derivative of a quadratic is exact, ends included —
+    # on a uniform and on a stretched axis. A two-node end stencil is not.
+    row = np.array([1, 1, 1, 0, 1, 1, 1, 1, 0, 0, 1, 1, 1, 1, 1, 1, 0, 1, 1, 1], bool)
+    n = len(row)
+    for x in (cart_axes(n, 0.7, 2.0), np.cumsum(0.4 * 1.09 ** np.arange(n))):
+        for axis in (0, 1):
+            mask = np.broadcast_to(row, (3, n)).copy() if axis == 0 else np.broadcast_to(row[:, None], (n, 3)).copy()
+            xs, ys = (x, cart_axes(3, 1.0)) if axis == 0 else (cart_axes(3, 1.0), x)
+            g = O.Grid([xs, ys], periodic=[False, False], mask=mask,
+                       uniform=[axis != 0 or np.allclose(np.diff(x), x[1] - x[0]), axis != 1 or np.allclose(np.diff(x), x[1] - x[0])])
+            X = np.broadcast_to(xs, (len(ys), len(xs))) if axis == 0 else np.broadcast_to(ys[:, None], (len(ys), len(xs)))
+            f = 2.0 * X ** 2 - 3.0 * X + 1.0
+            f = np.where(mask, f, 1e6)                       # garbage under the mask must not leak
+            want = np.where(mask, 4.0 * X - 3.0, 0.0)
+            d = O.deriv(g, axis, f)
+            assert np.allclose(d, want, rtol=1e-11, atol=1e-9)
+            with O.mutation(O.MUTATE_RUNEND_2NODE):
+                dbad = O.deriv(g, axis, f)
+            assert np.max(np.abs(dbad - want)) > 0.1
+
+
+def test_taylor_green_derivatives():
+    # test_diagnostics.jl:60-101: u = sin x cos y, v = −cos x sin y on 0:0.1:2π, rtol 0.01 at (10, 10)
+    xs = np.arange(0.0, 2 * np.pi, 0.1)
+    g = O.Grid([xs, xs], periodic=[False, False], uniform=[True, True])
+    X, Y = np.meshgrid(xs, xs)
+    u, v = np.sin(X) * np.cos(Y), -np.cos(X) * np.sin(Y)
+    j = i = 9
+    x, y = xs[i], xs[j]
+    assert O.deriv(g, 0, u)[j, i] == pytest.approx(np.cos(x) * np.cos(y), rel=0.01)
+    assert O.deriv(g, 1, u)[j, i] == pytest.approx(-np.sin(x) * np.sin(y), rel=0.01)
+    assert O.deriv(g, 0, v)[j, i] == pytest.approx(np.sin(x) * np.sin(y), rel=0.01)
+    assert O.deriv(g, 1, v)[j, i] == pytest.approx(-np.cos(x) * np.cos(y), rel=0.01)
+
+
+# ------------------------------------ identities the reference's test suite asserts --------------
+
+def _bump(N, ic, jc, rad):
+    """test_filtering.jl:841-848 (0-based centre): (1 − r²)⁴ inside radius `rad` cells."""
+    j, i = np.mgrid[0:N, 0:N].astype(np.float64)
+    r2 = ((i - ic) ** 2 + (j - jc) ** 2) / rad ** 2
+    return np.where(r2 < 1, (1 - r2) ** 4, 0.0)
+
+
+def test_zerofill_commutes_with_ddx_deformable_does_not():
+    # test_filtering.jl:850-916
+    N, dx = 96, 1000.0
+    xs = cart_axes(N, dx)
+    icoast = 60                                    # first land column (0-based)
+    mask = np.ones((N, N), bool)
+    mask[:, icoast:] = False
+    g = oracle_grid_cart(xs, xs, periodic=(False, False), mask=mask)
+    ell = 8000.0                                   # top-hat, radius 4 cells
+    f = _bump(N, 48, 47, 8)
+    df = O.deriv(g, 0, f)
+    assert np.all(f[:, icoast:] == 0) and np.all(df[:, icoast:] == 0)
+    fp = O.Footprint(g, O.TOPHAT, 0.0, ell)
+    rel, edge, drift = {}, {}, {}
+    for st in (O.ZEROFILL, O.DEFORMABLE):
+        lhs = O.apply_banded(g, fp, st, [df])[0]
+        ff = O.apply_banded(g, fp, st, [f])[0]
+        rhs = O.deriv(g, 0, ff)
+        ii = slice(1, icoast - 2)
+        s = np.max(np.abs(rhs[:, ii]))
+        rel[st] = np.max(np.abs(lhs[:, ii] - rhs[:, ii])) / s
+        edge[st] = np.max(np.abs(lhs[:, icoast - 1] - rhs[:, icoast - 1])) / s
+        drift[st] = (np.sum(ff) - np.sum(f)) / np.sum(f)
+    assert rel[O.ZEROFILL] < 1e-13
+    assert rel[O.DEFORMABLE] > 1e-3 and rel[O.DEFORMABLE] > 1e9 * rel[O.ZEROFILL]
+    assert edge[O.ZEROFILL] > 1e-3
+    assert abs(drift[O.ZEROFILL]) < 1e-13 and abs(drift[O.DEFORMABLE]) > 1e-6
+    # unmasked: both strategies coincide and commute, top-hat and Gaussian alike
+    gf = oracle_grid_cart(xs, xs, periodic=(False, False))
+    fu = _bump(N, 47, 47, 10)
+    dfu = O.deriv(gf, 0, fu)
+    for kind, alpha in ((O.TOPHAT, 0.0), (O.GAUSSIAN, 6.0)):
+        lhs = O.filter_fields(gf, [dfu], kind=kind, alpha=alpha, ell=ell)[0]
+        rhs = O.deriv(gf, 0, O.filter_fields(gf, [fu], kind=kind, alpha=alpha, ell=ell)[0])
+        ii = slice(1, N - 1)
+        assert np.max(np.abs(lhs[:, ii] - rhs[:, ii])) < 1e-13 * np.max(np.abs(rhs[:, ii]))
+
+
+_PHI_A, _PHI_B = 0.7548776662466927, 0.5698402909980532
+
+
+def _spectral_field(N, dx, alpha, symbol="exact"):
+    """test_validation.jl:22-40: ψ̂ = |k|^−(α+3)/2 with irrational-rotation phases (1-based i, j; i
+    along x), round-tripped through a real field; û = −i κ(k_y) ψ̂, v̂ = i κ(k_x) ψ̂ with κ(k) = k
+    (spectral) or sin(kΔx)/Δx (divergence-free for the centred 3-point stencil)."""
+    n = np.arange(N)
+    k = 2 * np.pi / (N * dx) * np.where(n <= N // 2, n, n - N)
+    I, J = np.meshgrid(n + 1, n + 1)                 # [j, i], 1-based indices
+    KX, KY = k[I - 1], k[J - 1]
+    km = np.sqrt(KX ** 2 + KY ** 2)
+    phase = 2 * np.pi * np.mod(_PHI_A * I + _PHI_B * J, 1.0)
+    psi = np.where(km > 0, np.where(km > 0, km, 1.0) ** (-(alpha + 3) / 2), 0.0) * np.exp(1j * phase)
+    psi = np.fft.fft2(np.real(np.fft.ifft2(psi)))
+    sym = (lambda q: q) if symbol == "exact" else (lambda q: np.sin(q * dx) / dx)
+    return np.real(np.fft.ifft2(-1j * sym(KY) * psi)), np.real(np.fft.ifft2(1j * sym(KX) * psi))
+
+
+def _logfit(x, y):
+    lx, ly = np.log10(x), np.log10(y)
+    return float(np.sum((lx - lx.mean()) * (ly - ly.mean())) / np.sum((lx - lx.mean()) ** 2))
+
+
+def _fourier_slope(u, v, dx, nlo, nhi):
+    N = u.shape[0]
+    n = np.arange(N)
+    k = 2 * np.pi / (N * dx) * np.where(n <= N // 2, n, n - N)
+    dk = 2 * np.pi / (N * dx)
+    uh, vh = np.fft.fft2(u) / N ** 2, np.fft.fft2(v) / N ** 2
+    sh = np.rint(np.sqrt(k[None, :] ** 2 + k[:, None] ** 2) / dk).astype(int)
+    E = np.zeros(N // 2 + 1)
+    ok = (sh >= 1) & (sh <= N // 2)
+    np.add.at(E, sh[ok], 0.5 * (np.abs(uh[ok]) ** 2 + np.abs(vh[ok]) ** 2))
+    ns = [q for q in range(nlo, nhi + 1) if E[q] > 0]
+    return _logfit(np.array(ns, float), E[ns])
+
+
+def test_filter_commutes_with_divergence():
+    # test_validation.jl:146-182, for the reference's default engines of each kernel (separable
+    # Gaussian, prefix-sum top-hat) and the banded footprint
+    dx = 1000.0
+    for N in (64, 96):
+        xs = cart_axes(N, dx)
+        g = oracle_grid_cart(xs, xs, periodic=(True, True))
+        u, v = _spectral_field(N, dx, 3.0, symbol="stencil")
+        du, dv = O.deriv(g, 0, u), O.deriv(g, 1, v)
+        divu = du + dv
+        assert np.max(np.abs(divu)) < 1e-12 * max(np.max(np.abs(du)), np.max(np.abs(dv)))
+        for kind, alpha, engines in ((O.GAUSSIAN, 6.0, (O.ENGINE_SEPARABLE, O.ENGINE_BANDED)),
+                                     (O.TOPHAT, 0.0, (O.ENGINE_PREFIX, O.ENGINE_BANDED))):
+            for ell in (8 * dx, 16 * dx):
+                for eng in engines:
+                    ub, vb, fdiv = O.filter_fields(g, [u, v, divu], kind=kind, alpha=alpha, ell=ell, engine=eng)
+                    dub, dvb = O.deriv(g, 0, ub), O.deriv(g, 1, vb)
+                    s2 = max(np.max(np.abs(dub)), np.max(np.abs(dvb)))
+                    assert np.max(np.abs(dub + dvb)) < 1e-11 * s2
+                    assert np.max(np.abs(fdiv - (dub + dvb))) < 1e-11 * s2
+
+
+def test_filtering_spectrum_slope_saturates_at_p_plus_2():
+    # test_validation.jl:68-111: with p = 1 vanishing moments both kernels recover k^−α only for
+    # α < 3 and saturate near k^−3 above (cumulative_energy + spectral_density, kℓ = 1/ℓ)
+    N, dx = 256, 1000.0
+    xs = cart_axes(N, dx)
+    g = oracle_grid_cart(xs, xs, periodic=(True, True))
+    scales = 10 ** np.linspace(np.log10(5 * dx), np.log10(64 * dx), 17)
+    kl = 1.0 / scales
+    w = (scales >= 8 * dx) & (scales <= 40 * dx)
+    got = {}
+    for alpha in (5 / 3, 4.0, 7.0):
+        u, v = _spectral_field(N, dx, alpha)
+        assert _fourier_slope(u, v, dx, 6, 32) == pytest.approx(-alpha, abs=0.05)
+        for name, kind, a, eng in (("Gaussian", O.GAUSSIAN, 6.0, O.ENGINE_SEPARABLE),
+                                   ("TopHat", O.TOPHAT, 0.0, O.ENGINE_PREFIX)):
+            cum = O.cumulative_energy(g, u, v, scales, kind=kind, alpha=a, engine=eng)
+            got[(name, alpha)] = _logfit(kl[w], O.spectral_density(cum, kl)[w])
+    for name in ("Gaussian", "TopHat"):
+        assert got[(name, 5 / 3)] == pytest.approx(-5 / 3, abs=0.20)
+        assert -3.15 < got[(name, 7.0)] < -2.75
+        assert -3.10 < got[(name, 4.0)] < -2.60
+        assert got[(name, 7.0)] < got[(name, 4.0)] < got[(name, 5 / 3)]
+    for alpha in (5 / 3, 4.0, 7.0):
+        assert got[("Gaussian", alpha)] == pytest.approx(got[("TopHat", alpha)], abs=0.05)
+
+
+def test_sfs_stress_symmetric_and_energy_budget():
+    # test_diagnostics.jl:184-243, 283-344: τ_xy = τ_yx, tr τ ≥ 0 (Jensen), and the resolved plus
+    # subfilter energies add up to the filtered total energy
+    x = np.arange(0.0, 30001.0, 1000.0)
+    g = oracle_grid_cart(x, x, periodic=(False, False))
+    rng = np.random.Generator(np.random.PCG64(11))
+    u, v = rng.random((31, 31)), rng.random((31, 31))
+    ell = 5000.0
+    ub, vb, uu, uv, vu, vv, tot = O.filter_fields(g, [u, v, u * u, u * v, v * u, v * v, u * u + v * v],
+                                                  kind=O.TOPHAT, alpha=0.0, ell=ell)
+    inner = (slice(9, 22), slice(9, 22))
+    txx, txy, tyy = uu - ub * ub, uv - ub * vb, vv - vb * vb
+    assert np.allclose(txy[inner], (vu - vb * ub)[inner], rtol=1e-10)
+    assert np.all((txx + tyy)[inner] >= -1e-10)
+    rho0 = 1025.0
+    sfs = 0.5 * rho0 * (txx + tyy)
+    res = 0.5 * rho0 * (ub ** 2 + vb ** 2)
+    assert np.all(sfs[inner] >= -1e-12)
+    assert np.allclose((sfs + res)[inner], (0.5 * rho0 * tot)[inner], rtol=1e-10)
