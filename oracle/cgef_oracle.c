@@ -57,8 +57,9 @@ double og_haversine(double R, double lam1, double phi1, double lam2, double phi2
 }
 
 /* Cell width along one axis. A uniform (Range) axis has the constant step everywhere; a
- * non-uniform axis uses midpoint widths, half cells at non-periodic ends (FlowGeometries'
- * measure is un-vendored: this is the stated choice). */
+ * non-uniform axis uses midpoint widths, half cells at non-periodic ends (FlowGeometries' measure
+ * is un-vendored; the choice is pinned by tiling, by the step inside a collected uniform axis,
+ * test_grids.jl:203-206, and by 2nd-order convergence of the weighted filter). */
 double og_cell_width(const og_grid *g, int axis, int64_t i) {
     int64_t n = g->n[axis];
     const double *x = g->x[axis];
