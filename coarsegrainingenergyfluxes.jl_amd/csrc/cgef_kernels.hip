@@ -8,9 +8,9 @@
 // costs no VGPR traffic and each staged cell is reused by up to 256 targets.
 //
 // Precision (FP32 path): pass 1 accumulates Σ w·q in FP64 (v_fma_f64 runs at the FP32 VALU rate on
-// CDNA4), so ū carries no accumulation error beyond its FP32 inputs. Pass 2 accumulates the CENTRED
-// products Σ w·(q_a − q̄_a)(q_b − q̄_b) in FP32: positive-definite sums with no cancellation, and an
-// error in the centre enters τ only at second order. τ = F[(q−q̄)(q−q̄)] + (1−m)[…] with m = F[1]
+// CDNA4), so ū carries no accumulation error beyond its FP32 inputs. Pass 2 forms the CENTRED
+// products w·(q_a − q̄_a)(q_b − q̄_b) in FP32 (no cancellation: an error in the centre enters τ only
+// at second order) and sums them in two levels, FP32 over chunks of 32 taps, FP64 across chunks. τ = F[(q−q̄)(q−q̄)] + (1−m)[…] with m = F[1]
 // (ZeroFill near a mask) is algebraically F[q_a q_b] − q̄_a q̄_b, i.e. the reference's τ.
 #include <hip/hip_runtime.h>
 
@@ -255,8 +255,13 @@ __global__ __launch_bounds__(TI) void k_pi_filter(GeomArgs G, BandArgs B, const 
     const double qb0 = ok ? s0 / den : 0.0, qb1 = ok ? s1 / den : 0.0, qb2 = ok ? s2 / den : 0.0;
     const T qc0 = (T)qb0, qc1 = (T)qb1, qc2 = (T)qb2;
 
-    // ---- pass 2: centred products Σ w (q_a − q̄_a)(q_b − q̄_b) over active in-range cells
-    T c00 = 0, c01 = 0, c02 = 0, c11 = 0, c12 = 0, c22 = 0;
+    // ---- pass 2: centred products Σ w (q_a − q̄_a)(q_b − q̄_b) over active in-range cells, summed
+    // in two levels: T partials over chunks of at most P2_CHUNK taps, added into FP64 totals. A
+    // single FP32 running sum over a footprint of N taps carries a relative error of about
+    // 2e-8·√N (N up to 25,000 at the poles of the 0.25° sphere at ℓ = 200 km), which the
+    // contraction −S̄:τ amplifies where τ is nearly isotropic; the chunked sum keeps it near 1e-8.
+    constexpr int P2_CHUNK = 32;
+    double c00 = 0, c01 = 0, c02 = 0, c11 = 0, c12 = 0, c22 = 0;
     for (int ru = rb; ru < re; ++ru) {
         const Run R = B.runs[ru];
         int64_t jj = j + R.dy, kk = k + R.dz;
@@ -276,23 +281,35 @@ __global__ __launch_bounds__(TI) void k_pi_filter(GeomArgs G, BandArgs B, const 
             const T *wr = wt + R.woff;
             const T4 *sp = seg + tid;
             const int n = R.hi - R.lo + 1;
+            for (int e0 = 0; e0 < n; e0 += P2_CHUNK) {
+                const int e1 = n - e0 < P2_CHUNK ? n : e0 + P2_CHUNK;
+                T p00 = 0, p01 = 0, p02 = 0, p11 = 0, p12 = 0, p22 = 0;
 #pragma unroll 4
-            for (int e = 0; e < n; ++e) {
-                const T4 c = sp[e];
-                T tw = wr[e];
-                if (EDGE_X) tw = (c.w < (T)0.5) ? tw : (T)0;
-                else if (MASKED) tw = tw * ((T)1 - c.w);
-                const T d0 = c.x - qc0, d1 = c.y - qc1;
-                const T t0 = tw * d0, t1 = tw * d1;
-                c00 = fma(t0, d0, c00);
-                c01 = fma(t0, d1, c01);
-                c11 = fma(t1, d1, c11);
+                for (int e = e0; e < e1; ++e) {
+                    const T4 c = sp[e];
+                    T tw = wr[e];
+                    if (EDGE_X) tw = (c.w < (T)0.5) ? tw : (T)0;
+                    else if (MASKED) tw = tw * ((T)1 - c.w);
+                    const T d0 = c.x - qc0, d1 = c.y - qc1;
+                    const T t0 = tw * d0, t1 = tw * d1;
+                    p00 = fma(t0, d0, p00);
+                    p01 = fma(t0, d1, p01);
+                    p11 = fma(t1, d1, p11);
+                    if (NC == 3) {
+                        const T d2 = c.z - qc2;
+                        const T t2 = tw * d2;
+                        p02 = fma(t0, d2, p02);
+                        p12 = fma(t1, d2, p12);
+                        p22 = fma(t2, d2, p22);
+                    }
+                }
+                c00 += (double)p00;
+                c01 += (double)p01;
+                c11 += (double)p11;
                 if (NC == 3) {
-                    const T d2 = c.z - qc2;
-                    const T t2 = tw * d2;
-                    c02 = fma(t0, d2, c02);
-                    c12 = fma(t1, d2, c12);
-                    c22 = fma(t2, d2, c22);
+                    c02 += (double)p02;
+                    c12 += (double)p12;
+                    c22 += (double)p22;
                 }
             }
         }
@@ -306,7 +323,7 @@ __global__ __launch_bounds__(TI) void k_pi_filter(GeomArgs G, BandArgs B, const 
         const double om = (B.strategy == CGEF_ZEROFILL) ? mo / D : 0.0;  // 1 − m, m = F[1]
         const double m = 1.0 - om;
         const double qb[3] = {qb0, qb1, qb2};
-        const double cv[6] = {(double)c00, (double)c01, (double)c02, (double)c11, (double)c12, (double)c22};
+        const double cv[6] = {c00, c01, c02, c11, c12, c22};
         const int ia[6] = {0, 0, 0, 1, 1, 2}, ib[6] = {0, 1, 2, 1, 2, 2};
         for (int a = 0; a < 3; a++) U[a] = qb[a] + cm[a] * ((B.strategy == CGEF_ZEROFILL) ? m : 1.0);
         for (int q6 = 0; q6 < 6; q6++) {

@@ -441,6 +441,27 @@ def test_ring_pi_wide_footprint(strategy, dtype):
     assert rel_l2(N(Pi), ref) <= (1e-10 if dtype == torch.float64 else 1e-5)
 
 
+def test_ring_falls_back_when_the_contraction_slab_exceeds_lds():
+    # A footprint spanning more source rows than the contraction can stage in 160 KiB of LDS (5 columns
+    # × (256 + dy range) × 16 B): engine = auto takes the banded direct sum of the same footprint,
+    # engine = ring is refused at plan time rather than failing at launch.
+    print("ring LDS-slab fallback", flush=True)
+    lam, phi = sphere_axes(4, 2400)
+    g = cgef.StructuredGrid(cgef.SphericalGeometry(6.371e6), lam, phi)
+    ker, ell = cgef.GaussianKernel(4.0), 3.2e6        # radius 7,680 km: about ±921 rows of 0.075°
+    plan = cgef.plan_filter(g, ker, ell)
+    assert plan.engine == "banded"
+    with pytest.raises(cgef.ArgumentError, match="LDS"):
+        cgef.plan_filter(g, ker, ell, engine="ring")
+    assert cgef.plan_filter(g, ker, 2.0e5).engine == "ring"
+    f = smooth_random_field(g.field_shape, 47, 1)[0]
+    out = torch.empty((2400, 4), dtype=torch.float64, device=DEV)
+    cgef.filter_apply_(out, T(f), plan)
+    og = oracle_grid_sphere(lam, phi)
+    ref = O.filter_fields(og, [f], kind=O.GAUSSIAN, alpha=4.0, ell=ell)[0]
+    assert np.max(np.abs(N(out) - ref)) <= 1e-11 * np.max(np.abs(ref))
+
+
 @pytest.mark.parametrize("geom", ["cartesian", "sphere", "volume"])
 @pytest.mark.parametrize("strategy", ["zerofill", "deformable"])
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
