@@ -81,6 +81,14 @@ class FilterPlan:
         self.info = info
         self.engine = ENGINE_NAMES[info.engine]
 
+    def query(self):
+        """The plan's current description (cgef_plan_query): refreshes `info`, whose contraction kernel
+        choice is made by measurement on the plan's first ring call."""
+        info = _lib.PlanInfo()
+        _lib.check(_lib.load().cgef_plan_query(self.h, C.byref(info)))
+        self.info = info
+        return info
+
     @property
     def mean_taps(self):
         return self.info.mean_taps
