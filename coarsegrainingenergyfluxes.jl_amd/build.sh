@@ -16,10 +16,11 @@ COMMON="-O3 -fPIC -std=c++17 -I$HERE/../include"
 "$HIPCC" $ARCH $COMMON $DEVFLAGS -c "$SRC/cgef_ring.hip" -o "$OBJ/cgef_ring.o"
 "$HIPCC" $ARCH $COMMON $DEVFLAGS -c "$SRC/cgef_prefix.hip" -o "$OBJ/cgef_prefix.o"
 "$HIPCC" $ARCH $COMMON $DEVFLAGS -c "$SRC/cgef_closure.hip" -o "$OBJ/cgef_closure.o"
+"$HIPCC" $ARCH $COMMON $DEVFLAGS -c "$SRC/cgef_noalloc.hip" -o "$OBJ/cgef_noalloc.o"
 "$HIPCC" $ARCH $COMMON -ffp-contract=off -c "$SRC/cgef_prefix_host.cpp" -o "$OBJ/cgef_prefix_host.o"
 "$HIPCC" $ARCH $COMMON -ffp-contract=off -c "$SRC/cgef_ring_host.cpp" -o "$OBJ/cgef_ring_host.o"
 "$HIPCC" $ARCH $COMMON -ffp-contract=off -c "$SRC/cgef_footprint.cpp" -o "$OBJ/cgef_footprint.o"
 "$HIPCC" $ARCH $COMMON -ffp-contract=off -c "$SRC/cgef_api.cpp" -o "$OBJ/cgef_api.o"
-"$HIPCC" $ARCH -shared -fPIC -o "$OUT.tmp" "$OBJ/cgef_kernels.o" "$OBJ/cgef_ring.o" "$OBJ/cgef_prefix.o" "$OBJ/cgef_closure.o" "$OBJ/cgef_prefix_host.o" "$OBJ/cgef_ring_host.o" "$OBJ/cgef_footprint.o" "$OBJ/cgef_api.o"
+"$HIPCC" $ARCH -shared -fPIC -o "$OUT.tmp" "$OBJ/cgef_kernels.o" "$OBJ/cgef_ring.o" "$OBJ/cgef_prefix.o" "$OBJ/cgef_closure.o" "$OBJ/cgef_noalloc.o" "$OBJ/cgef_prefix_host.o" "$OBJ/cgef_ring_host.o" "$OBJ/cgef_footprint.o" "$OBJ/cgef_api.o"
 mv -f "$OUT.tmp" "$OUT"
 echo "built $OUT"

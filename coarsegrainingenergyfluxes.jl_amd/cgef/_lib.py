@@ -22,7 +22,7 @@ EXPORTED = (
     "cgef_version", "cgef_last_error", "cgef_grid_create", "cgef_grid_destroy", "cgef_active_area",
     "cgef_plan_create", "cgef_plan_destroy", "cgef_plan_query", "cgef_workspace_create",
     "cgef_workspace_destroy", "cgef_filter_apply", "cgef_compute_pi", "cgef_coarse_grain", "cgef_derivative",
-    "cgef_energy",
+    "cgef_energy", "cgef_energy_noalloc",
     "cgef_cumulative_energy", "cgef_profile_enable", "cgef_profile_read", "cgef_spectral_density",
     # learned closure (no reference counterpart)
     "cgef_conv5", "cgef_conv5_mfma", "cgef_conv5_pack_mfma", "cgef_conv5_wgrad_scratch", "cgef_conv5_wgrad", "cgef_conv5_pack_weights", "cgef_adam_update",
@@ -115,6 +115,7 @@ def load():
         "cgef_coarse_grain": [C.POINTER(vp), i64, vp, vp, vp, vp, vp, i64, i32, dbl, dp, vp],
         "cgef_derivative": [vp, i32, vp, vp, i64, i32, vp],
         "cgef_energy": [vp, vp, vp, vp, i64, i32, dp, vp],
+        "cgef_energy_noalloc": [vp, vp, vp, vp, i64, i32, dp, vp],
         "cgef_cumulative_energy": [C.POINTER(vp), i64, vp, vp, vp, vp, i64, i32, dp, vp],
         "cgef_profile_enable": [vp, i32],
         "cgef_profile_read": [vp, dp, C.POINTER(i64)],

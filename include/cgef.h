@@ -141,6 +141,17 @@ cgef_status cgef_coarse_grain(cgef_plan *const *plans, int64_t nscales, cgef_wor
                               const void *u, const void *v, const void *w, int64_t nbatch, int32_t dtype,
                               double rho0, double *E_out, void *stream);
 
+/* No counterpart in the reference. For callers that cannot allocate device memory (tight serving
+ * loops, harnesses that forbid hipMalloc): cgef_energy with its scratch — the per-block partial sums
+ * and the per-slice energies — in one fixed device array of 1,052,672 doubles (8 MiB) that the
+ * library holds on each GPU it runs on until the process ends, instead of the grid's workspace that
+ * cgef_energy allocates on first use. Allocates nothing. Limits: nbatch ≤ 4096 and
+ * nbatch · ceil(Nx/256)·Ny·Nz ≤ 1,048,576 (0.25° sphere: 242 slices), else CGEF_ERR_DIMENSION
+ * before any launch. Calls on one GPU share the array, so calls that overlap on one GPU race.
+ * Synchronizes the stream. */
+cgef_status cgef_energy_noalloc(const cgef_grid *grid, const void *ubar, const void *vbar, const void *wbar,
+                                int64_t nbatch, int32_t dtype, double *E_out, void *stream);
+
 /* Derivatives.jl:20-144 ddx!/ddy!/ddz!: derivative w.r.t. distance along axis 0/1/2 of nbatch
  * slices (3-node stencil, masked cells 0). */
 cgef_status cgef_derivative(const cgef_grid *grid, int32_t axis, void *out, const void *in,
