@@ -164,7 +164,7 @@ def sweep_headline(args, grid, mask, lam, phi, cpu):
     value = npts * ns * NSNAP * args.steps / el
     log(f"sweep: {ms:.2f} ms per {NSNAP}x{ns} call (eager) -> {value:.3e} point-scales/s; "
         f"2000 km contraction {kern_ms:.3f} ms")
-    names = {0: "k_contract<5,8,1>", 1: "k_contract<5,4,1>", 2: "k_contract<5,6,6>", -1: None}
+    names = {0: "k_contract<5,8,1>", 1: "k_contract<5,4,1>", 2: "k_contract<5,6,6>", 3: "k_contract_mfma", -1: None}
     variants = {f"{s / 1e3:.0f}km": names[int(p.query().ring_contract_kernel)] for s, p in zip(scales, plans)}
     # the same sweep replayed as one HIP graph (opt-in), timed in this process
     ms_graph = None

@@ -413,6 +413,20 @@ static void ring_filter_planes(Plan &p, Workspace &w, int64_t ncol, hipStream_t 
     ring_inverse(p.ring, w.rout, w.rxy, ncol, s);
 }
 
+// The contracted spectra of the Π planes (w.rout) to ū and τ: the fused inverse transform + epilogue
+// where the row length allows it (CGEF_FUSE=0 keeps the two kernels), else the inverse into w.rxy and
+// the separate epilogue.
+static void ring_pi_inverse_epilogue(Plan &p, Workspace &w, const GeomArgs &G, int dtype, int nf, int wm, void *uo,
+                                     void *vo, void *wo, int64_t nbatch, hipStream_t s) {
+    const char *fe = std::getenv("CGEF_FUSE");
+    const bool fuse = !(fe && fe[0] == '0');
+    if (fuse && ring_inverse_epilogue(p.ring, G, dtype, w.rout, p.D64_row_d, nf, wm, p.strategy, w.ubar, w.tau, uo, vo,
+                                      wo, nbatch, s))
+        return;
+    ring_inverse(p.ring, w.rout, w.rxy, nf * nbatch, s);
+    ring_epilogue(G, dtype, w.rxy, p.D64_row_d, nf, wm, p.strategy, w.ubar, w.tau, uo, vo, wo, nbatch, s);
+}
+
 // spectral engine: forward row transforms, per-mode column transform × Ĝ and back, row inverse
 static void spectral_filter_planes(Plan &p, Workspace &w, int64_t ncol, hipStream_t s) {
     ring_forward(p.ring, w.rxy, w.rxf, ncol, s);
@@ -831,8 +845,11 @@ cgef_status cgef_compute_pi(cgef_plan *plan, cgef_workspace *ws, void *Pi, const
             const int wm = ring_withmask(g, p->strategy), nf = ring_nfields(g, wm);
             ring_reserve(W, *p, nf * nbatch);
             ring_prep(G, dtype, u, v, w, nf, wm, W.rxy, nbatch, s);
-            ring_filter_planes(*p, W, nf * nbatch, s);
-            ring_epilogue(G, dtype, W.rxy, p->D64_row_d, nf, wm, p->strategy, W.ubar, W.tau, ubar, vbar, wbar, nbatch, s);
+            ring_forward(p->ring, W.rxy, W.rxf, nf * nbatch, s);
+            p->prof_begin(s);
+            ring_contract(p->ring, W.rxf, W.rout, nf * nbatch, s);
+            p->prof_end(s);
+            ring_pi_inverse_epilogue(*p, W, G, dtype, nf, wm, ubar, vbar, wbar, nbatch, s);
             launch_strain_pi(G, dtype, W.ubar, W.tau, Pi, rho0, nbatch, W.partials, W.npartials, s);
             CGEF_HIP(hipGetLastError());
             if (E_out) fetch_energy(W, nbatch, strain_blocks(G), g.active_area, E_out, s);
@@ -918,9 +935,7 @@ cgef_status cgef_coarse_grain(cgef_plan *const *plans, int64_t nscales, cgef_wor
                         p.prof_begin(st);
                         ring_contract(p.ring, W.rxf, W.rout, nf * nbatch, st);
                         p.prof_end(st);
-                        ring_inverse(p.ring, W.rout, W.rxy, nf * nbatch, st);
-                        ring_epilogue(G, dtype, W.rxy, p.D64_row_d, nf, wm, p.strategy, W.ubar, W.tau, nullptr, nullptr,
-                                      nullptr, nbatch, st);
+                        ring_pi_inverse_epilogue(p, W, G, dtype, nf, wm, nullptr, nullptr, nullptr, nbatch, st);
                         launch_strain_pi(G, dtype, W.ubar, W.tau, (char *)Pi + (size_t)q * P * es, rho0, nbatch,
                                          W.partials, W.npartials, st, nscales * P);
                         if (E_out) launch_energy_final(nbatch, nblk, W.partials, g.active_area, Eall + q * nbatch, st);

@@ -83,6 +83,8 @@ constexpr int RING_CONTRACT_CC = 5;
 // Largest Ŵ table (mh × offsets × bands FP64) a ring plan may hold; wider footprints go to the
 // banded direct sum under engine = AUTO.
 constexpr double RING_WHAT_BUDGET_BYTES = 32.0 * 1024 * 1024 * 1024;
+// Offsets from which a 2-D ring plan also carries the matrix-core contraction's weight table.
+constexpr int RING_MFMA_MIN_OFFSETS = 48;
 // LDS per CU on gfx950, which is also the most one workgroup may take.
 constexpr size_t LDS_BYTES_PER_CU = 160 * 1024;
 
@@ -124,6 +126,12 @@ struct RingPlan {
                                        // twoff + k·(R−1) + r−1, 0 ≤ k < Ns, 1 ≤ r < R
     double *what_d = nullptr;
     int32_t *dz_val_d = nullptr, *dz_off0_d = nullptr, *off_dy_d = nullptr;
+    // Ŵ in the A-operand order of the matrix-core contraction (k_contract_mfma), built for 2-D plans
+    // with at least RING_MFMA_MIN_OFFSETS offsets: per (m, 16-row tile, 4-row step) the 64 lane values
+    // A[i = lane%16][k = lane/16] = Ŵ[m][4·step + k − i][band(16·tile + i)] (0 outside the footprint).
+    // A grid with one band (translation invariant along y) stores one tile for all.
+    double *wt_d = nullptr;
+    int32_t wt_steps = 0, wt_tiles = 0;
     RingArgs args;
 };
 

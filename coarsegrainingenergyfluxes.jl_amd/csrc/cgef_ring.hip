@@ -20,6 +20,7 @@
 #include <cstdlib>
 
 #include <cfloat>
+#include <type_traits>
 #include <utility>
 
 #include "cgef_pi_point.hpp"
@@ -635,6 +636,133 @@ __global__ __launch_bounds__(256, MINW) void k_contract(RingArgs A, const double
         if (c < nvc) OUT[(m * ncol + c0 + c) * A.nrows + r] = acc[c];
 }
 
+// ---- the contraction on the matrix cores (2-D grids, wide footprints) ------------------------------
+//
+// Per Fourier mode m the contraction is a banded real matrix times the spectra of the plane columns:
+//     OUT[c][r] = Σ_s A[r][s] · X[c][s],   A[r][s] = Ŵ[m][s − r − dy_min][band(r)] inside the band.
+// With the real and imaginary parts of the columns as separate real columns it maps onto
+// v_mfma_f64_16x16x4f64: D(16 target rows × 16 real columns) += A(16 × 4 source rows) · B(4 × 16).
+// A workgroup (4 waves) takes 64 consecutive target rows of one mode (one 16-row tile per wave) and
+// NT·16 real columns; the source rows its tiles reach are staged through LDS KC rows at a time
+// ([row][real column], shared by the four waves), each wave running the 4-row steps of its tile that
+// fall in the staged rows: one A fragment (one coalesced 8-byte load per lane from the lane-ordered
+// weight table) serves NT MFMAs, each with a B fragment read from LDS. Operand layouts (probed on
+// gfx950): lane l holds A[l%16][l/16] and B[l/16][l%16]; the accumulator's value v is D[l/16 + 4v][l%16].
+// Each staged spectrum value feeds up to 16 products in the matrix unit, where the VALU contraction
+// reads it from LDS once per FMA pair; the tile's band edges add (15 + offsets)/offsets of zero work.
+typedef double mfma_f64x4 __attribute__((ext_vector_type(4)));
+constexpr int MFMA_KC = 32;                   // source rows staged per pass (8 four-row steps)
+
+template <int NT>
+__global__ __launch_bounds__(256) void k_contract_mfma(RingArgs A, const double *__restrict__ wt, int nsteps, int wtiles,
+                                                       const double2 *__restrict__ XF, double2 *__restrict__ OUT,
+                                                       int64_t ncol) {
+    constexpr int LDW = NT * 16 + 2;          // row stride in doubles (≡ 4 banks mod 32: 16-byte stores spread)
+    extern __shared__ double lds[];
+    const int64_t Ny = A.ny;
+    const int64_t nrb = (Ny + 63) / 64;
+    const int64_t ncc = (2 * ncol + NT * 16 - 1) / (NT * 16);
+    // items (m, row block, column chunk), column chunk fastest, contiguous ranges per XCD: the row
+    // blocks of one mode re-read overlapping source rows through the same L2
+    int64_t item;
+    if (!xcd_item((int64_t)A.mh * nrb * ncc, item)) return;
+    const int64_t cc = item % ncc;
+    const int64_t rb = (item / ncc) % nrb;
+    const int64_t m = item / (ncc * nrb);
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int64_t R0 = rb * 64;
+    const int ntile = (int)((Ny - R0) < 64 ? (Ny - R0 + 15) / 16 : 4);   // tiles in this block
+    const int64_t c0 = cc * (NT * 8);                                    // first complex column
+    const int nvc = (int)((ncol - c0) < NT * 8 ? (ncol - c0) : NT * 8);
+    const bool active = wave < ntile;
+    const int64_t tile = R0 / 16 + wave;
+    const double *wp = wt + ((m * wtiles + (wtiles == 1 ? 0 : tile)) * nsteps) * 64 + lane;
+    const int krows = 16 * (ntile - 1) + 4 * nsteps;                     // block source rows used
+    const int64_t S0 = R0 + A.dy_min;                                    // block source row 0
+    // on a non-periodic axis the source rows outside 0..Ny−1 contribute nothing: skip the 4-row steps
+    // (and staged passes) that lie wholly outside (the rows beyond a pole at wide footprints)
+    const int brlo = A.py ? 0 : (int)(S0 < 0 ? (-S0 < krows ? -S0 : krows) : 0);
+    const int brhi = A.py ? krows : (int)(Ny - S0 < krows ? (Ny - S0 > 0 ? Ny - S0 : 0) : krows);
+    const int brlo4 = brlo & ~3;
+    mfma_f64x4 acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++) acc[t] = mfma_f64x4{0.0, 0.0, 0.0, 0.0};
+    const double2 *xs = XF + (m * ncol + c0) * A.nrows;
+    for (int base = brlo / MFMA_KC * MFMA_KC; base < brhi; base += MFMA_KC) {
+        // this wave's steps within the staged rows [base, base + KC) of the block's source rows; their
+        // weights are requested before the staging so the loads overlap it
+        int lo = base > 16 * wave ? base : 16 * wave;
+        lo = lo > brlo4 ? lo : brlo4;
+        int hi = base + MFMA_KC;
+        hi = hi < 16 * wave + 4 * nsteps ? hi : 16 * wave + 4 * nsteps;
+        hi = hi < brhi ? hi : brhi;
+        double a[MFMA_KC / 4];
+#pragma unroll
+        for (int q = 0; q < MFMA_KC / 4; q++) {
+            const int br = lo + 4 * q;
+            a[q] = (active && br < hi) ? wp[(int64_t)((br - 16 * wave) >> 2) * 64] : 0.0;
+        }
+        __syncthreads();
+        // stage rows S0 + base .. + KC − 1 of the chunk's columns: lane → (column, row), 32 consecutive
+        // rows of a column per half wave (512 contiguous bytes); zero outside a non-periodic axis and
+        // beyond the valid columns
+        for (int idx = tid; idx < MFMA_KC * NT * 8; idx += 256) {
+            const int row = idx % MFMA_KC, col = idx / MFMA_KC;
+            int64_t s = S0 + base + row;
+            bool in = col < nvc;
+            if (s < 0 || s >= Ny) {
+                if (A.py) s = ((s % Ny) + Ny) % Ny;
+                else in = false;
+            }
+            const double2 v = in ? xs[(int64_t)col * A.nrows + s] : make_double2(0.0, 0.0);
+            *reinterpret_cast<double2 *>(lds + row * LDW + 2 * col) = v;
+        }
+        __syncthreads();
+        if (!active) continue;
+#pragma unroll
+        for (int q = 0; q < MFMA_KC / 4; q++) {
+            const int br = lo + 4 * q;
+            if (br >= hi) break;
+            const double *brow = lds + (br - base + (lane >> 4)) * LDW + (lane & 15);
+#pragma unroll
+            for (int t = 0; t < NT; t++) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[q], brow[t * 16], acc[t], 0, 0, 0);
+        }
+    }
+    if (!active) return;
+    // lane holds D[lane/16 + 4v][lane%16] of each column tile: target row, real column (re/im of a
+    // complex column)
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+        const int j = t * 16 + (lane & 15);
+        const int c = j >> 1;
+        if (c >= nvc) continue;
+        double *o = reinterpret_cast<double *>(OUT + (m * ncol + c0 + c) * A.nrows) + (j & 1);
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int64_t r = tile * 16 + (lane >> 4) + 4 * v;
+            if (r < Ny) o[2 * r] = acc[t][v];
+        }
+    }
+}
+
+// The A-operand table of k_contract_mfma from Ŵ: entry (m, tile, step, lane) = Ŵ[m][o][band(r)] with
+// r = 16·tile + lane%16 and o = 4·step + lane/16 − lane%16, zero where o is outside 0..noff−1 or r ≥ Ny.
+__global__ __launch_bounds__(256) void k_build_wtiles(RingArgs A, int nsteps, int wtiles, double *__restrict__ wt) {
+    const int64_t total = (int64_t)A.mh * wtiles * nsteps * 64;
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (int64_t)gridDim.x * 256) {
+        const int lane = (int)(e % 64);
+        const int64_t step = (e / 64) % nsteps;
+        const int64_t tile = (e / (64 * nsteps)) % wtiles;
+        const int64_t m = e / (64 * nsteps * wtiles);
+        const int i = lane & 15, k = lane >> 4;
+        const int64_t r = tile * 16 + i;
+        const int64_t o = 4 * step + k - i;
+        double v = 0.0;
+        if (o >= 0 && o < A.noff && r < A.ny) v = A.what[(m * A.noff + o) * A.nbands + A.band_of_row[r]];
+        wt[e] = v;
+    }
+}
+
 // Ŵ for every (band, offset): the run's weights laid out circularly on a length-N row, FFT'd two
 // rows at a time; the real part is kept (the weights are symmetric, so the spectrum is real).
 __global__ __launch_bounds__(256) void k_build_what(RingFft F, const double2 *__restrict__ tw, RingArgs A,
@@ -744,6 +872,150 @@ __global__ __launch_bounds__(256) void k_ring_epilogue(GeomArgs G, const double 
     const int np = G.mode == 0 ? 5 : 9;
     for (int f = 0; f < np; f++) F[f] = y[f * P + t];
     pi_point_epilogue<T>(G, t, i, j, F, den, b, ubar, tau, uo, vo, wo);
+}
+
+// The inverse row transforms fused with the epilogue (lengths with a compiled-in factorization): a
+// block takes F.nsq consecutive rows of one snapshot and inverse-transforms that snapshot's planes of
+// those rows two at a time (z = X_a + i·X_b gives both real rows), folding each transformed plane into
+// per-point state as soon as it leaves LDS — the numerators of ū and, on the sphere, of the three
+// local-frame components of τ (e_xᵀ F[q qᵀ] e_y is linear in the product planes) — so the filtered
+// planes never return to HBM. The epilogue then forms ū = F[q]/den and τ = F[q_a q_b]/den − ū_a ū_b
+// (Diagnostics.jl:496-527, 596-650) and writes what k_ring_epilogue writes. Plane order per slice:
+// the NC components, the products (xx xy [xz] yy [yz zz]), then the mask when WM (Deformable).
+template <typename T, int FIX, int PPT, int MODE, int WM>
+__global__ __launch_bounds__(256) void k_ifft_epilogue(RingFft F, const double2 *__restrict__ tw,
+                                                       const double2 *__restrict__ tws, const double2 *__restrict__ OUT,
+                                                       GeomArgs G, const double *__restrict__ D_row, int64_t nbatch,
+                                                       double *__restrict__ ubar, T *__restrict__ tau, T *__restrict__ uo,
+                                                       T *__restrict__ vo, T *__restrict__ wo) {
+    constexpr int NC = MODE == 0 ? 2 : 3;
+    constexpr int NP = MODE == 0 ? 3 : 6;
+    constexpr int NF = NC + NP + WM;
+    constexpr int NS = MODE == 1 ? 6 : NC + NP;     // state per point (sphere: ū numerators, 3 local τ sums)
+    extern __shared__ double2 sm[];
+    const int n = F.n, nsq = F.nsq, sh = F.lpsh, sstr = F.sstr;
+    const int64_t nrows = G.n[1] * G.n[2], P = G.npts, Nx = G.n[0], Ny = G.n[1];
+    const int64_t ngrp = (nrows + nsq - 1) / nsq;
+    int64_t item;
+    if (!xcd_item(ngrp * nbatch, item)) return;
+    const int64_t b = item / ngrp, r0 = (item % ngrp) * nsq;
+    const int mh = n / 2 + 1;
+    const int64_t ncol = nbatch * NF;
+    const double sc = 1.0 / n;
+    double st[PPT][NS], dn[PPT];
+#pragma unroll
+    for (int p = 0; p < PPT; p++) {
+        dn[p] = 0.0;
+#pragma unroll
+        for (int q = 0; q < NS; q++) st[p][q] = 0.0;
+    }
+#pragma unroll
+    for (int pr = 0; pr < (NF + 1) / 2; pr++) {
+        const int fa = 2 * pr, fb = 2 * pr + 1;
+        // compose z = X_a + i·X_b per sequence (row) and its conjugate mirror; lane per (mode, row) so a
+        // block's rows of one mode are adjacent in the [m][col][row] spectra
+        for (int e = threadIdx.x; e < nsq * mh; e += 256) {
+            const int m = e / nsq, sq = e - m * nsq;
+            const int64_t row = r0 + sq;
+            double2 X = make_double2(0.0, 0.0), Y = X;
+            if (row < nrows) {
+                X = OUT[spec_at(m, b * NF + fa, row, nrows, ncol)];
+                if (fb < NF) Y = OUT[spec_at(m, b * NF + fb, row, nrows, ncol)];
+            }
+            double2 *z = sm + sq * sstr;
+            z[lp(m, sh)] = make_double2(X.x - Y.y, X.y + Y.x);
+            if (m > 0 && n - m >= mh) z[lp(n - m, sh)] = make_double2(X.x + Y.y, Y.x - X.y);
+        }
+        fft_rows_any<FIX>(sm, F, tw, tws, true);
+#pragma unroll
+        for (int p = 0; p < PPT; p++) {
+            const int e = (int)threadIdx.x + p * 256;
+            if (e >= nsq * n) continue;
+            const int sq = e / n, i = e - sq * n;
+            const double2 z = sm[sq * sstr + lp(i, sh)];
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const int f = h == 0 ? fa : fb;
+                if (f >= NF) continue;
+                const double x = (h == 0 ? z.x : z.y) * sc;
+                if (f < NC) {
+                    st[p][f] = x;
+                } else if (f < NC + NP) {
+                    if constexpr (MODE == 1) {
+                        // e_eastᵀ, e_northᵀ of the planetary product plane (a, b), off-diagonal planes twice
+                        const int64_t j = (r0 + sq) % Ny;
+                        const double sl = G.trig[i], cl = G.trig[Nx + i];
+                        const double sp = G.trig[2 * Nx + j], cp = G.trig[2 * Nx + Ny + j];
+                        const double ee[3] = {-sl, cl, 0.0}, en[3] = {-sp * cl, -sp * sl, cp};
+                        const int k6 = f - NC;
+                        const int a = k6 < 3 ? 0 : (k6 < 5 ? 1 : 2);
+                        const int c = k6 < 3 ? k6 : (k6 < 5 ? k6 - 2 : 2);
+                        const double wee = a == c ? ee[a] * ee[a] : 2.0 * ee[a] * ee[c];
+                        const double wen = a == c ? ee[a] * en[a] : ee[a] * en[c] + ee[c] * en[a];
+                        const double wnn = a == c ? en[a] * en[a] : 2.0 * en[a] * en[c];
+                        st[p][3] += wee * x;
+                        st[p][4] += wen * x;
+                        st[p][5] += wnn * x;
+                    } else {
+                        st[p][f] = x;
+                    }
+                } else {
+                    dn[p] = x;   // the filtered mask: Deformable's denominator
+                }
+            }
+        }
+        __syncthreads();   // every lane has read the transform before the next pair is composed
+    }
+#pragma unroll
+    for (int p = 0; p < PPT; p++) {
+        const int e = (int)threadIdx.x + p * 256;
+        if (e >= nsq * n) continue;
+        const int sq = e / n, i = e - sq * n;
+        const int64_t row = r0 + sq;
+        if (row >= nrows) continue;
+        const int64_t t = row * Nx + i, j = row % Ny;
+        const double den = WM ? dn[p] : D_row[row];
+        const bool ok = G.mask[t] != 0 && den > 1e-15;
+        const double inv = ok ? 1.0 / den : 0.0;
+        double *ub = ubar + b * 3 * P;
+        T *tb = tau + b * 6 * P;
+        double o0, o1, o2;
+        if constexpr (MODE == 1) {
+            const double sl = G.trig[i], cl = G.trig[Nx + i];
+            const double sp = G.trig[2 * Nx + j], cp = G.trig[2 * Nx + Ny + j];
+            const double U0 = st[p][0] * inv, U1 = st[p][1] * inv, U2 = st[p][2] * inv;
+            o0 = -sl * U0 + cl * U1;
+            o1 = -sp * cl * U0 - sp * sl * U1 + cp * U2;
+            o2 = cp * cl * U0 + cp * sl * U1 + sp * U2;
+            tb[t] = (T)(st[p][3] * inv - o0 * o0);
+            tb[P + t] = (T)(st[p][4] * inv - o0 * o1);
+            tb[2 * P + t] = (T)(st[p][5] * inv - o1 * o1);
+        } else if constexpr (MODE == 0) {
+            o0 = st[p][0] * inv;
+            o1 = st[p][1] * inv;
+            o2 = 0.0;
+            tb[t] = (T)(st[p][2] * inv - o0 * o0);
+            tb[P + t] = (T)(st[p][3] * inv - o0 * o1);
+            tb[2 * P + t] = (T)(st[p][4] * inv - o1 * o1);
+        } else {
+            o0 = st[p][0] * inv;
+            o1 = st[p][1] * inv;
+            o2 = st[p][2] * inv;
+            const double U[3] = {o0, o1, o2};
+#pragma unroll
+            for (int q6 = 0; q6 < 6; q6++) {
+                const int a = q6 < 3 ? 0 : (q6 < 5 ? 1 : 2);
+                const int c = q6 < 3 ? q6 : (q6 < 5 ? q6 - 2 : 2);
+                tb[q6 * P + t] = (T)(st[p][3 + q6] * inv - U[a] * U[c]);
+            }
+        }
+        ub[t] = o0;
+        ub[P + t] = o1;
+        if (MODE == 2) ub[2 * P + t] = o2;
+        if (uo) uo[b * P + t] = (T)o0;
+        if (vo) vo[b * P + t] = (T)o1;
+        if (wo) wo[b * P + t] = (T)o2;
+    }
 }
 
 // plain fields for filter_apply: x = f·active (FP64), plus the mask when a Deformable denominator is needed
@@ -998,6 +1270,46 @@ static int contract_rule(const RingPlan &R, int64_t ncol, size_t lds_blocks) {
     return 2;
 }
 
+// Column tiles (16 real columns each) per workgroup of the matrix-core contraction for a launch of
+// ncol complex columns: the instance that wastes the fewest padded columns, the wider on a tie.
+static int mfma_nt(int64_t ncol) {
+    const int64_t need = (2 * ncol + 15) / 16;
+    const int cand[5] = {9, 5, 4, 3, 2};
+    int best = 9;
+    int64_t best_cols = INT64_MAX;
+    for (int nt : cand) {
+        const int64_t cols = (need + nt - 1) / nt * nt;
+        if (cols < best_cols) {
+            best_cols = cols;
+            best = nt;
+        }
+    }
+    return best;
+}
+
+static void launch_contract_mfma(const RingPlan &R, const double2 *XF, double2 *OUT, int64_t ncol, hipStream_t s) {
+    const int nt = mfma_nt(ncol);
+    const int64_t nrb = (R.args.ny + 63) / 64;
+    const int64_t ncc = (2 * ncol + nt * 16 - 1) / (nt * 16);
+    const int64_t items = R.mh * nrb * ncc;
+    const dim3 grid((unsigned)(8 * ((items + 7) / 8)));
+    const size_t sh = (size_t)MFMA_KC * (nt * 16 + 2) * sizeof(double);
+    auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, grid, dim3(256), sh, s, R.args, R.wt_d, R.wt_steps, R.wt_tiles, XF, OUT, ncol);
+    };
+    if (nt == 9) go(k_contract_mfma<9>);
+    else if (nt == 5) go(k_contract_mfma<5>);
+    else if (nt == 4) go(k_contract_mfma<4>);
+    else if (nt == 3) go(k_contract_mfma<3>);
+    else go(k_contract_mfma<2>);
+}
+
+void ring_build_wtiles(RingPlan &R, hipStream_t s) {
+    const int64_t total = R.mh * R.wt_tiles * R.wt_steps * 64;
+    const int64_t blocks = std::min<int64_t>((total + 255) / 256, 8 * 1024);
+    hipLaunchKernelGGL(k_build_wtiles, dim3((unsigned)blocks), dim3(256), 0, s, R.args, R.wt_steps, R.wt_tiles, R.wt_d);
+}
+
 void ring_contract(const RingPlan &R, const double2 *XF, double2 *OUT, int64_t ncol, hipStream_t s) {
     constexpr int CC = RING_CONTRACT_CC;
     const int span = 256 + R.args.dy_max - R.args.dy_min;
@@ -1006,18 +1318,21 @@ void ring_contract(const RingPlan &R, const double2 *XF, double2 *OUT, int64_t n
     const int64_t items = (ncol + CC - 1) / CC * nyb * R.args.nz * R.mh;
     const dim3 grid((unsigned)(8 * ((items + 7) / 8)));
     const size_t lds_blocks = LDS_BYTES_PER_CU / sh;
+    const bool has_mfma = R.wt_d != nullptr;
+    const int nvar = has_mfma ? 4 : 3;
     auto launch = [&](int variant) {
         auto go = [&](auto kern) {
             set_shmem((const void *)kern, sh);
             hipLaunchKernelGGL(kern, grid, dim3(256), sh, s, R.args, XF, OUT, ncol);
         };
-        if (variant == 0) go(k_contract<CC, 8, 1>);
+        if (variant == 3) launch_contract_mfma(R, XF, OUT, ncol, s);
+        else if (variant == 0) go(k_contract<CC, 8, 1>);
         else if (variant == 1) go(k_contract<CC, 4, 1>);
         else go(k_contract<CC, 6, 6>);
     };
     if (const char *f = std::getenv("CGEF_CONTRACT_VARIANT")) {
         const int v = std::atoi(f);
-        if (v >= 0 && v <= 2) {
+        if (v >= 0 && v < nvar) {
             launch(v);
             return;
         }
@@ -1045,7 +1360,7 @@ void ring_contract(const RingPlan &R, const double2 *XF, double2 *OUT, int64_t n
     for (auto &e : ev) CGEF_HIP(hipEventCreate(&e));
     float best_ms = 0.0f;
     int best = contract_rule(R, ncol, lds_blocks);
-    for (int v = 0; v < 3; ++v) {
+    for (int v = 0; v < nvar; ++v) {
         launch(v);
         CGEF_HIP(hipEventRecord(ev[0], s));
         for (int r = 0; r < kReps; ++r) {
@@ -1077,6 +1392,52 @@ void ring_prep(const GeomArgs &G, int dtype, const void *u, const void *v, const
         dim3 grid((unsigned)((G.npts + 255) / 256), (unsigned)nbatch);
         hipLaunchKernelGGL(k_ring_prep<1>, grid, dim3(256), 0, s, G, dtype, u, v, w, nf, withmask, xin);
     }
+}
+
+// The fused inverse + epilogue where the row length has a compiled-in factorization that one block's
+// sequences fit (1440 points: one row per block, 6 points per lane; 256 points: 4 rows, 4 per lane);
+// false when the two separate kernels must serve.
+bool ring_inverse_epilogue(const RingPlan &R, const GeomArgs &G, int dtype, const double2 *OUT, const double *D_row,
+                           int nf, int withmask, int strategy, double *ubar, void *tau, void *uo, void *vo, void *wo,
+                           int64_t nbatch, hipStream_t s) {
+    RingFft F = R.fft;
+    if (R.fft.nseq == 0 || F.fixed == 0) return false;
+    if (withmask && strategy != CGEF_DEFORMABLE) return false;
+    const int mode = G.mode;
+    if (nf != (mode == 0 ? 5 : 9) + withmask) return false;
+    const int64_t nrows = G.n[1] * G.n[2];
+    if (F.fixed == RING_FIXED_1440) F.nsq = 1;
+    else F.nsq = 4;
+    if (!fixed_fits(F)) return false;
+    const int64_t items = (nrows + F.nsq - 1) / F.nsq * nbatch;
+    const dim3 grid((unsigned)(8 * ((items + 7) / 8)));
+    const size_t sh = staged_shmem(F);
+    bool launched = false;
+    auto go = [&](auto kern, auto *tp) {
+        using TT = std::remove_pointer_t<decltype(tp)>;
+        hipLaunchKernelGGL(kern, grid, dim3(256), sh, s, F, R.tw_d, R.tws_d, OUT, G, D_row, nbatch, ubar, (TT *)tau,
+                           (TT *)uo, (TT *)vo, (TT *)wo);
+        launched = true;
+    };
+#define CGEF_IFE(TT, FIXV, PPTV, MODEV)                                                              \
+    if (withmask) go(k_ifft_epilogue<TT, FIXV, PPTV, MODEV, 1>, (TT *)nullptr);                     \
+    else go(k_ifft_epilogue<TT, FIXV, PPTV, MODEV, 0>, (TT *)nullptr);
+    if (F.fixed == RING_FIXED_1440) {
+        if (mode != 1) return false;   // 1440-point rows: the longitude of a 0.25° sphere
+        if (dtype == CGEF_F32) { CGEF_IFE(float, RING_FIXED_1440, 6, 1) } else { CGEF_IFE(double, RING_FIXED_1440, 6, 1) }
+    } else {
+        if (dtype == CGEF_F32) {
+            if (mode == 0) { CGEF_IFE(float, RING_FIXED_256, 4, 0) }
+            else if (mode == 1) { CGEF_IFE(float, RING_FIXED_256, 4, 1) }
+            else { CGEF_IFE(float, RING_FIXED_256, 4, 2) }
+        } else {
+            if (mode == 0) { CGEF_IFE(double, RING_FIXED_256, 4, 0) }
+            else if (mode == 1) { CGEF_IFE(double, RING_FIXED_256, 4, 1) }
+            else { CGEF_IFE(double, RING_FIXED_256, 4, 2) }
+        }
+    }
+#undef CGEF_IFE
+    return launched;
 }
 
 void ring_epilogue(const GeomArgs &G, int dtype, const double *yout, const double *D_row, int nf, int withmask,

@@ -36,11 +36,18 @@ void ring_build_what(const RingPlan &R, const Run *runs_d, const int32_t *run_of
                      hipStream_t s);
 void ring_forward(const RingPlan &R, const double *xin, double2 *XF, int64_t ncol, hipStream_t s);
 void ring_contract(const RingPlan &R, const double2 *XF, double2 *OUT, int64_t ncol, hipStream_t s);
+// fills R.wt_d (allocated by the caller: mh × wt_tiles × wt_steps × 64 doubles) from Ŵ
+void ring_build_wtiles(RingPlan &R, hipStream_t s);
 void ring_inverse(const RingPlan &R, const double2 *OUT, double *yout, int64_t ncol, hipStream_t s);
 // Π inputs: per slice nf planes (frame velocity components, their products, then the mask when
 // `withmask`); the epilogue turns the filtered planes into ū (FP64) and τ, optionally exporting ū in T.
 void ring_prep(const GeomArgs &G, int dtype, const void *u, const void *v, const void *w, int nf, int withmask,
                double *xin, int64_t nbatch, hipStream_t s);
+// inverse row transforms of the contracted spectra fused with the epilogue (false: not served for this
+// length or plane set; use ring_inverse + ring_epilogue)
+bool ring_inverse_epilogue(const RingPlan &R, const GeomArgs &G, int dtype, const double2 *OUT, const double *D_row,
+                           int nf, int withmask, int strategy, double *ubar, void *tau, void *uo, void *vo, void *wo,
+                           int64_t nbatch, hipStream_t s);
 void ring_epilogue(const GeomArgs &G, int dtype, const double *yout, const double *D_row, int nf, int withmask,
                    int strategy, double *ubar, void *tau, void *uo, void *vo, void *wo, int64_t nbatch, hipStream_t s);
 void ring_prep_fields(const GeomArgs &G, int dtype, const void *in, int64_t nsl, int withmask, double *xin, hipStream_t s);

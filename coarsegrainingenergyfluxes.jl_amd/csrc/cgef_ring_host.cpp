@@ -316,6 +316,13 @@ void build_ring(Plan &p, hipStream_t s) {
     CGEF_HIP(hipMalloc(&R.what_d, (size_t)R.mh * A.noff * A.nbands * sizeof(double)));
     A.what = R.what_d;
     ring_build_what(R, p.runs_d, run_off_d, p.w64_d, (int64_t)p.runs.size(), s);
+    if (A.ndz == 1 && A.noff >= RING_MFMA_MIN_OFFSETS) {
+        // the matrix-core contraction's lane-ordered copy of Ŵ (a wide 2-D footprint)
+        R.wt_steps = (15 + A.noff + 3) / 4;
+        R.wt_tiles = A.nbands == 1 ? 1 : (int32_t)((A.ny + 15) / 16);
+        CGEF_HIP(hipMalloc(&R.wt_d, (size_t)R.mh * R.wt_tiles * R.wt_steps * 64 * sizeof(double)));
+        ring_build_wtiles(R, s);
+    }
     CGEF_HIP(hipGetLastError());
     CGEF_HIP(hipStreamSynchronize(s));
     (void)hipFree(run_off_d);
@@ -329,6 +336,7 @@ void free_ring(RingPlan &R) {
     if (R.tw_d) (void)hipFree(R.tw_d);
     if (R.tws_d) (void)hipFree(R.tws_d);
     if (R.what_d) (void)hipFree(R.what_d);
+    if (R.wt_d) (void)hipFree(R.wt_d);
     if (R.dz_val_d) (void)hipFree(R.dz_val_d);
     if (R.dz_off0_d) (void)hipFree(R.dz_off0_d);
     if (R.off_dy_d) (void)hipFree(R.off_dy_d);

@@ -87,8 +87,9 @@ typedef struct {
     int64_t ring_offsets;     /* ring engine: distinct (dz, dy) source-row offsets; 0 otherwise */
     int64_t ring_modes;       /* ring engine: Fourier modes kept along x (N/2 + 1); 0 otherwise */
     int64_t ring_pi_fields;   /* ring engine: planes filtered per snapshot by compute_pi; 0 otherwise */
-    int64_t ring_contract_kernel; /* ring engine: the contraction kernel variant kept by measurement for
-                                     the widest call so far (0-2), −1 before any measured call */
+    int64_t ring_contract_kernel; /* ring engine: the contraction kernel kept by measurement for the
+                                     widest call so far (0-2 VALU weight-batch variants, 3 matrix
+                                     cores), −1 before any measured call */
 } cgef_plan_info;
 
 const char *cgef_version(void);

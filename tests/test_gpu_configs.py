@@ -7,7 +7,7 @@ the reference gates the same way in test/test_backends.jl:694-793).
 * 0.25° sphere (1440 × 720), FP32 in and out, 30 % land ZeroFill, Gaussian α = 4, ℓ = 200 km:
   rel-L2(Π) ≤ 1e-5 overall, on the coast band and in the open-ocean interior; E(ℓ) and ū, v̄.
 * 8 snapshots × 24 scales (50–2000 km) on the same grid, one batched sweep per contraction kernel
-  variant: Π at the poles, ±60° and the equator for the two widest-footprint scales (≈1000 and
+  (three VALU variants and the matrix-core one, forced, and the measured choice): Π at the poles, ±60° and the equator for the two widest-footprint scales (≈1000 and
   2000 km: up to 347 row offsets, source slabs crossing the pole) and the full field at 50 km,
   rel-L2 ≤ 1e-5; the oracle evaluates only the checked rows (and their stencil neighbours).
 * 256 × 256 × 64 volume, stretched z, FP32, top-hat ℓ = 16Δx: rel-L2(Π) ≤ 1e-5.
@@ -152,7 +152,7 @@ def sweep_cfg(sphere_cfg):
     return U, V, g, ker, plans, refs
 
 
-@pytest.mark.parametrize("variant", ["0", "1", "2", None])
+@pytest.mark.parametrize("variant", ["0", "1", "2", "3", None])
 def test_batched_sweep_widest_scales(sweep_cfg, variant, monkeypatch):
     U, V, g, ker, plans, refs = sweep_cfg
     say(f"config: 8x24 sweep, contraction variant {variant or 'measured'}")

@@ -441,6 +441,103 @@ def test_ring_pi_wide_footprint(strategy, dtype):
     assert rel_l2(N(Pi), ref) <= (1e-10 if dtype == torch.float64 else 1e-5)
 
 
+@pytest.mark.parametrize("geom", ["sphere1440", "cartesian256", "volume256"])
+@pytest.mark.parametrize("strategy", ["zerofill", "deformable"])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_fused_inverse_epilogue(geom, strategy, dtype, monkeypatch):
+    # Row lengths with a compiled-in factorization (1440: the 0.25° longitude; 256) take the inverse
+    # transforms fused with the epilogue: against the oracle, and against the two separate kernels
+    # (CGEF_FUSE=0), for the sphere's local-frame τ, the 2-D and 3-D Cartesian sets, the mask plane of
+    # Deformable and two snapshots.
+    print(f"fused inverse+epilogue {geom} {strategy} {dtype}", flush=True)
+    s_cg, s_o = STRATS[strategy]
+    if geom == "sphere1440":
+        lam, phi = sphere_axes(1440, 36)
+        mask = blob_mask((36, 1440), 44, 0.3)
+        og = oracle_grid_sphere(lam, phi, mask=mask)
+        g = cgef.StructuredGrid(cgef.SphericalGeometry(6.371e6), lam, phi, mask=mask)
+        ker, kind, alpha, ell, eng3 = cgef.GaussianKernel(4.0), O.GAUSSIAN, 4.0, 8.0e5, None
+    elif geom == "cartesian256":
+        xs, ys = cart_axes(256, 1000.0), cart_axes(40, 1000.0)
+        mask = blob_mask((40, 256), 45)
+        og = oracle_grid_cart(xs, ys, periodic=(True, True), mask=mask)
+        g = cgef.StructuredGrid(cgef.CartesianGeometry(), xs, ys, mask=mask, periodic=(True, True))
+        ker, kind, alpha, ell, eng3 = cgef.GaussianKernel(6.0), O.GAUSSIAN, 6.0, 9000.0, None
+    else:
+        xs, ys = cart_axes(256, 1.0), cart_axes(12, 1.0)
+        zs = np.concatenate([[0.0], np.cumsum(0.8 * 1.1 ** np.arange(9))])
+        mask = blob_mask((10, 12, 256), 46)
+        og = oracle_grid_cart(xs, ys, zs, periodic=(True, True, False), mask=mask)
+        g = cgef.StructuredGrid(cgef.CartesianGeometry(), xs, ys, zs, mask=mask, periodic=(True, True, False))
+        ker, kind, alpha, ell, eng3 = cgef.TopHatKernel(), O.TOPHAT, 0.0, 5.0, O.ENGINE_ND3
+    plan = cgef.plan_filter(g, ker, ell, mask_strategy=s_cg, engine="ring")
+    comps = 3 if geom == "volume256" else 2
+    fs = smooth_random_field((2,) + g.field_shape, 49, comps)
+    F = [T(f, dtype) for f in fs]
+    W = F[2] if comps == 3 else None
+    ws = cgef.PiWorkspace(g, (2,), dtype)
+    Pi = torch.empty_like(F[0])
+    monkeypatch.delenv("CGEF_FUSE", raising=False)
+    cgef.compute_pi_(Pi, F[0], F[1], W, g, ker, ell, workspace=ws, filter_plan=plan)
+    got, ub = N(Pi), N(ws.u_filt)
+    monkeypatch.setenv("CGEF_FUSE", "0")
+    P2 = torch.empty_like(Pi)
+    ws2 = cgef.PiWorkspace(g, (2,), dtype)
+    cgef.compute_pi_(P2, F[0], F[1], W, g, ker, ell, workspace=ws2, filter_plan=plan)
+    tol = 1e-13 if dtype == torch.float64 else 1e-6
+    assert rel_l2(got, N(P2)) <= tol
+    assert rel_l2(ub, N(ws2.u_filt)) <= tol
+    for b in range(2):
+        args = [N(f[b]) for f in F]
+        kw = dict(kind=kind, alpha=alpha, ell=ell, strategy=s_o)
+        if eng3 is not None:
+            kw["engine"] = eng3
+        ref, rub, _, _ = O.compute_pi(og, *args, **kw)
+        assert rel_l2(got[b], ref) <= (1e-10 if dtype == torch.float64 else 1e-5)
+        assert rel_l2(ub[b], rub) <= (1e-11 if dtype == torch.float64 else 1e-6)
+
+
+@pytest.mark.parametrize("case", ["sphere", "sphere_ragged", "cartesian_periodic", "cartesian_bounded"])
+@pytest.mark.parametrize("strategy", ["zerofill", "deformable"])
+@pytest.mark.parametrize("nbatch", [1, 3])
+def test_matrix_core_contraction(case, strategy, nbatch, monkeypatch):
+    # The contraction on the matrix cores (kernel variant 3, forced) against the oracle and against the
+    # VALU contraction: row counts that are not multiples of the 16-row tiles or 64-row workgroups, one
+    # band per latitude and a translation-invariant band, wrapped and clipped source rows, a mask
+    # column (Deformable), and 9-30 plane columns (the different column-tile instances).
+    print(f"matrix-core contraction {case} {strategy} nbatch={nbatch}", flush=True)
+    s_cg, s_o = STRATS[strategy]
+    if case == "sphere":
+        og, g, _, _ = sphere_case(True)
+        ker, kind, alpha, ell = cgef.GaussianKernel(4.0), O.GAUSSIAN, 4.0, 6.0e6
+    elif case == "sphere_ragged":
+        og, g, _, _ = sphere_case(True, nlon=72, nlat=70)
+        ker, kind, alpha, ell = cgef.GaussianKernel(4.0), O.GAUSSIAN, 4.0, 4.0e6
+    else:
+        per = case == "cartesian_periodic"
+        ny = 80 if per else 50
+        xs, ys = cart_axes(64, 1000.0), cart_axes(ny, 1000.0)
+        mask = blob_mask((ny, 64), 43)
+        og = oracle_grid_cart(xs, ys, periodic=(True, per), mask=mask)
+        g = cgef.StructuredGrid(cgef.CartesianGeometry(), xs, ys, mask=mask, periodic=(True, per))
+        ker, kind, alpha, ell = cgef.GaussianKernel(6.0), O.GAUSSIAN, 6.0, 15000.0
+    plan = cgef.plan_filter(g, ker, ell, mask_strategy=s_cg, engine="ring")
+    assert plan.info.ring_offsets >= 48
+    fs = smooth_random_field((nbatch,) + g.field_shape, 48, 2)
+    U, V = T(fs[0]), T(fs[1])
+    Pi = torch.empty_like(U)
+    monkeypatch.setenv("CGEF_CONTRACT_VARIANT", "3")
+    cgef.compute_pi_(Pi, U, V, None, g, ker, ell, filter_plan=plan)
+    got = N(Pi)
+    monkeypatch.setenv("CGEF_CONTRACT_VARIANT", "0")
+    P0 = torch.empty_like(U)
+    cgef.compute_pi_(P0, U, V, None, g, ker, ell, filter_plan=plan)
+    assert rel_l2(got, N(P0)) <= 1e-12
+    for b in range(nbatch):
+        ref = O.compute_pi(og, fs[0][b], fs[1][b], kind=kind, alpha=alpha, ell=ell, strategy=s_o)[0]
+        assert rel_l2(got[b], ref) <= 1e-10
+
+
 def test_ring_falls_back_when_the_contraction_slab_exceeds_lds():
     # A footprint spanning more source rows than the contraction can stage in 160 KiB of LDS (5 columns
     # × (256 + dy range) × 16 B): engine = auto takes the banded direct sum of the same footprint,
