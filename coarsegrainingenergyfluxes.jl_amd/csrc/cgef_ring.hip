@@ -344,9 +344,9 @@ __global__ __launch_bounds__(256) void k_ifft_rows(RingFft F, const double2 *__r
 
 __device__ __forceinline__ int lp(int e, int sh) { return e + (e >> sh); }
 
-// F.nsq sequences of F.n points at sm + sq·F.sstr are transformed together by the block's 256 threads
+// F.nsq sequences of F.n points at sm + sq·F.sstr are transformed together by the block's BLK threads
 // (short rows: each stage then has a wave's worth of butterflies or more).
-template <int R, int MB>
+template <int R, int MB, int BLK = 256>
 __device__ __forceinline__ void stage_staged(double2 *sm, int n, int Ns, int sh, int nsq, int sstr,
                                              const double2 *__restrict__ tws, const double2 *__restrict__ tw, bool inv) {
     // the stage parameters pass through an empty asm so that this radix's addresses are formed here:
@@ -360,7 +360,7 @@ __device__ __forceinline__ void stage_staged(double2 *sm, int n, int Ns, int sh,
     int dst[MB], base[MB];
 #pragma unroll
     for (int q = 0; q < MB; q++) {
-        const int p = (int)threadIdx.x + q * 256;
+        const int p = (int)threadIdx.x + q * BLK;
         if (p < tot) {
             const int sq = nsq == 1 ? 0 : p / nb;
             const int j = p - sq * nb;
@@ -386,7 +386,7 @@ __device__ __forceinline__ void stage_staged(double2 *sm, int n, int Ns, int sh,
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < MB; q++) {
-        const int p = (int)threadIdx.x + q * 256;
+        const int p = (int)threadIdx.x + q * BLK;
         if (p < tot) {
 #pragma unroll
             for (int r = 0; r < R; r++) sm[base[q] + lp(dst[q] + r * Ns, sh)] = v[q][r];
@@ -422,7 +422,7 @@ __device__ void fft_staged(double2 *sm, const RingFft &F, const double2 *tw, con
 }
 
 // The same with the factorization fixed at compile time (RingFft::fixed; every stage one butterfly per
-// lane over the block's sequences): only these stages' registers are allocated, where the switch
+// lane over the block's sequences, BLK lanes): only these stages' registers are allocated, where the switch
 // above holds those of its largest radix (96 VGPRs against 67 for 12·12·10 and 30 for 4·4·4·4).
 template <int... Rs>
 struct Radices {};
@@ -437,24 +437,24 @@ struct FixedRadices<RING_FIXED_256> {
     using type = Radices<4, 4, 4, 4>;
 };
 
-template <int Ns, int R, int... Rest>
+template <int BLK, int Ns, int R, int... Rest>
 __device__ __forceinline__ void fixed_stages(double2 *sm, const RingFft &F, const double2 *tw, const double2 *tws,
                                              bool inv, int st) {
-    stage_staged<R, 1>(sm, F.n, Ns, F.lpsh, F.nsq, F.sstr, tws + F.twoff[st], tw, inv);
-    if constexpr (sizeof...(Rest) > 0) fixed_stages<Ns * R, Rest...>(sm, F, tw, tws, inv, st + 1);
+    stage_staged<R, 1, BLK>(sm, F.n, Ns, F.lpsh, F.nsq, F.sstr, tws + F.twoff[st], tw, inv);
+    if constexpr (sizeof...(Rest) > 0) fixed_stages<BLK, Ns * R, Rest...>(sm, F, tw, tws, inv, st + 1);
 }
 
-template <int... Rs>
+template <int BLK, int... Rs>
 __device__ __forceinline__ void fft_fixed(Radices<Rs...>, double2 *sm, const RingFft &F, const double2 *tw,
                                           const double2 *tws, bool inv) {
     __syncthreads();
-    fixed_stages<1, Rs...>(sm, F, tw, tws, inv, 0);
+    fixed_stages<BLK, 1, Rs...>(sm, F, tw, tws, inv, 0);
 }
 
-template <int FIX>
+template <int FIX, int BLK = 256>
 __device__ __forceinline__ void fft_rows_any(double2 *sm, const RingFft &F, const double2 *tw, const double2 *tws, bool inv) {
     if constexpr (FIX == 0) fft_staged(sm, F, tw, tws, inv);
-    else fft_fixed(typename FixedRadices<FIX>::type{}, sm, F, tw, tws, inv);
+    else fft_fixed<BLK>(typename FixedRadices<FIX>::type{}, sm, F, tw, tws, inv);
 }
 
 // Forward, 2·F.nsq real rows per block: rows r0.. of column `col` → XF[m][col][row], m ≤ N/2.
@@ -875,33 +875,63 @@ __global__ __launch_bounds__(256) void k_ring_epilogue(GeomArgs G, const double 
 }
 
 // The inverse row transforms fused with the epilogue (lengths with a compiled-in factorization): a
-// block takes F.nsq consecutive rows of one snapshot and inverse-transforms that snapshot's planes of
-// those rows two at a time (z = X_a + i·X_b gives both real rows), folding each transformed plane into
-// per-point state as soon as it leaves LDS — the numerators of ū and, on the sphere, of the three
-// local-frame components of τ (e_xᵀ F[q qᵀ] e_y is linear in the product planes) — so the filtered
-// planes never return to HBM. The epilogue then forms ū = F[q]/den and τ = F[q_a q_b]/den − ū_a ū_b
-// (Diagnostics.jl:496-527, 596-650) and writes what k_ring_epilogue writes. Plane order per slice:
-// the NC components, the products (xx xy [xz] yy [yz zz]), then the mask when WM (Deformable).
-template <typename T, int FIX, int PPT, int MODE, int WM>
-__global__ __launch_bounds__(256) void k_ifft_epilogue(RingFft F, const double2 *__restrict__ tw,
-                                                       const double2 *__restrict__ tws, const double2 *__restrict__ OUT,
-                                                       GeomArgs G, const double *__restrict__ D_row, int64_t nbatch,
-                                                       double *__restrict__ ubar, T *__restrict__ tau, T *__restrict__ uo,
-                                                       T *__restrict__ vo, T *__restrict__ wo) {
+// block takes NSQ consecutive rows of one snapshot and inverse-transforms that snapshot's planes of
+// those rows two at a time (z = X_a + i·X_b gives both real rows), NPR plane pairs per round as
+// separate sequences of one transform, folding each transformed plane into per-point state as soon
+// as it leaves LDS — the numerators of ū and, on the sphere, of the three local-frame components of τ
+// (e_xᵀ F[q qᵀ] e_y is linear in the product planes) — so the filtered planes never return to HBM.
+// The epilogue then forms ū = F[q]/den and τ = F[q_a q_b]/den − ū_a ū_b (Diagnostics.jl:496-527,
+// 596-650) and writes what k_ring_epilogue writes. Plane order per slice: the NC components, the
+// products (xx xy [xz] yy [yz zz]), then the mask when WM (Deformable). 1440-point rows: one row per
+// block, three pairs per round (2 rounds for 9 or 10 planes), 512 threads (the compiled-in stages use
+// at most 3·144 butterflies, one per thread; every thread shares the per-point work); 256-point rows:
+// four rows per block, one pair per round, 256 threads.
+template <int FIX>
+struct IfeShape;
+template <>
+struct IfeShape<RING_FIXED_1440> {
+    static constexpr int N = 1440, NSQ = 1, NPR = 3, BLK = 512;
+};
+template <>
+struct IfeShape<RING_FIXED_256> {
+    static constexpr int N = 256, NSQ = 4, NPR = 1, BLK = 256;
+};
+
+template <typename T, int FIX, int MODE, int WM>
+__global__ __launch_bounds__(IfeShape<FIX>::BLK) void k_ifft_epilogue(RingFft F, const double2 *__restrict__ tw,
+                                                                      const double2 *__restrict__ tws,
+                                                                      const double2 *__restrict__ OUT, GeomArgs G,
+                                                                      const double *__restrict__ D_row, int64_t nbatch,
+                                                                      double *__restrict__ ubar, T *__restrict__ tau,
+                                                                      T *__restrict__ uo, T *__restrict__ vo,
+                                                                      T *__restrict__ wo) {
+    using S = IfeShape<FIX>;
+    constexpr int N = S::N, NSQ = S::NSQ, NPR = S::NPR, BLK = S::BLK;
+    constexpr int MH = N / 2 + 1;
+    constexpr int PPT = (NSQ * N + BLK - 1) / BLK;           // points per lane
     constexpr int NC = MODE == 0 ? 2 : 3;
     constexpr int NP = MODE == 0 ? 3 : 6;
     constexpr int NF = NC + NP + WM;
-    constexpr int NS = MODE == 1 ? 6 : NC + NP;     // state per point (sphere: ū numerators, 3 local τ sums)
+    constexpr int NPAIR = (NF + 1) / 2;
+    constexpr int NROUND = (NPAIR + NPR - 1) / NPR;
+    constexpr int NS = MODE == 1 ? 6 : NC + NP;              // state per point (sphere: ū numerators, 3 local τ sums)
     extern __shared__ double2 sm[];
-    const int n = F.n, nsq = F.nsq, sh = F.lpsh, sstr = F.sstr;
-    const int64_t nrows = G.n[1] * G.n[2], P = G.npts, Nx = G.n[0], Ny = G.n[1];
-    const int64_t ngrp = (nrows + nsq - 1) / nsq;
+    const int sh = F.lpsh, sstr = F.sstr;
+    const int64_t nrows = G.n[1] * G.n[2], P = G.npts, Ny = G.n[1];
+    const int64_t ngrp = (nrows + NSQ - 1) / NSQ;
     int64_t item;
     if (!xcd_item(ngrp * nbatch, item)) return;
-    const int64_t b = item / ngrp, r0 = (item % ngrp) * nsq;
-    const int mh = n / 2 + 1;
-    const int64_t ncol = nbatch * NF;
-    const double sc = 1.0 / n;
+    const int64_t b = item / ngrp, r0 = (item % ngrp) * NSQ;
+    const int mstride = (int)(nbatch * NF * nrows);          // one mode of the [m][col][row] spectra (< 2^31)
+    const double2 *spec = OUT + (b * NF) * nrows + r0;       // plane 0, row r0, mode 0
+    constexpr double sc = 1.0 / N;
+    double sp_[NSQ], cp_[NSQ];                               // the rows' sinφ, cosφ (sphere)
+#pragma unroll
+    for (int q = 0; q < NSQ; q++) {
+        const int64_t j = (r0 + q) % Ny;
+        sp_[q] = MODE == 1 ? G.trig[2 * G.n[0] + j] : 0.0;
+        cp_[q] = MODE == 1 ? G.trig[2 * G.n[0] + Ny + j] : 0.0;
+    }
     double st[PPT][NS], dn[PPT];
 #pragma unroll
     for (int p = 0; p < PPT; p++) {
@@ -910,70 +940,84 @@ __global__ __launch_bounds__(256) void k_ifft_epilogue(RingFft F, const double2 
         for (int q = 0; q < NS; q++) st[p][q] = 0.0;
     }
 #pragma unroll
-    for (int pr = 0; pr < (NF + 1) / 2; pr++) {
-        const int fa = 2 * pr, fb = 2 * pr + 1;
-        // compose z = X_a + i·X_b per sequence (row) and its conjugate mirror; lane per (mode, row) so a
-        // block's rows of one mode are adjacent in the [m][col][row] spectra
-        for (int e = threadIdx.x; e < nsq * mh; e += 256) {
-            const int m = e / nsq, sq = e - m * nsq;
-            const int64_t row = r0 + sq;
+    for (int rd = 0; rd < NROUND; rd++) {
+        const int npr = NPAIR - rd * NPR < NPR ? NPAIR - rd * NPR : NPR;   // pairs this round
+        const int nseq = npr * NSQ;                                        // sequence = pair-major, then row
+        // compose z = X_a + i·X_b per sequence and its conjugate mirror; lane per (mode, sequence)
+        for (int e = threadIdx.x; e < nseq * MH; e += BLK) {
+            const int m = e / nseq, sq = e - m * nseq;
+            const int k = sq / NSQ, rw = sq - k * NSQ;
+            const int fa = 2 * (rd * NPR + k), fb = fa + 1;
             double2 X = make_double2(0.0, 0.0), Y = X;
-            if (row < nrows) {
-                X = OUT[spec_at(m, b * NF + fa, row, nrows, ncol)];
-                if (fb < NF) Y = OUT[spec_at(m, b * NF + fb, row, nrows, ncol)];
+            if (r0 + rw < nrows) {
+                X = spec[(int64_t)fa * nrows + m * mstride + rw];
+                if (fb < NF) Y = spec[(int64_t)fb * nrows + m * mstride + rw];
             }
             double2 *z = sm + sq * sstr;
             z[lp(m, sh)] = make_double2(X.x - Y.y, X.y + Y.x);
-            if (m > 0 && n - m >= mh) z[lp(n - m, sh)] = make_double2(X.x + Y.y, Y.x - X.y);
+            if (m > 0 && N - m >= MH) z[lp(N - m, sh)] = make_double2(X.x + Y.y, Y.x - X.y);
         }
-        fft_rows_any<FIX>(sm, F, tw, tws, true);
+        RingFft Fr = F;
+        Fr.nsq = nseq;
+        fft_rows_any<FIX, BLK>(sm, Fr, tw, tws, true);
 #pragma unroll
         for (int p = 0; p < PPT; p++) {
-            const int e = (int)threadIdx.x + p * 256;
-            if (e >= nsq * n) continue;
-            const int sq = e / n, i = e - sq * n;
-            const double2 z = sm[sq * sstr + lp(i, sh)];
+            const int e = (int)threadIdx.x + p * BLK;
+            if (e >= NSQ * N) continue;
+            const int rw = e / N, i = e - rw * N;
+            double ee[3] = {0.0, 0.0, 0.0}, en[3] = {0.0, 0.0, 0.0};
+            if (MODE == 1 && (rd * NPR + NPR) * 2 > NC) {
+                // e_east, e_north at this point (a round holding a product plane)
+                const double sl = G.trig[i], cl = G.trig[N + i];
+                ee[0] = -sl;
+                ee[1] = cl;
+                en[0] = -sp_[rw] * cl;
+                en[1] = -sp_[rw] * sl;
+                en[2] = cp_[rw];
+            }
 #pragma unroll
-            for (int h = 0; h < 2; h++) {
-                const int f = h == 0 ? fa : fb;
-                if (f >= NF) continue;
-                const double x = (h == 0 ? z.x : z.y) * sc;
-                if (f < NC) {
-                    st[p][f] = x;
-                } else if (f < NC + NP) {
-                    if constexpr (MODE == 1) {
-                        // e_eastᵀ, e_northᵀ of the planetary product plane (a, b), off-diagonal planes twice
-                        const int64_t j = (r0 + sq) % Ny;
-                        const double sl = G.trig[i], cl = G.trig[Nx + i];
-                        const double sp = G.trig[2 * Nx + j], cp = G.trig[2 * Nx + Ny + j];
-                        const double ee[3] = {-sl, cl, 0.0}, en[3] = {-sp * cl, -sp * sl, cp};
-                        const int k6 = f - NC;
-                        const int a = k6 < 3 ? 0 : (k6 < 5 ? 1 : 2);
-                        const int c = k6 < 3 ? k6 : (k6 < 5 ? k6 - 2 : 2);
-                        const double wee = a == c ? ee[a] * ee[a] : 2.0 * ee[a] * ee[c];
-                        const double wen = a == c ? ee[a] * en[a] : ee[a] * en[c] + ee[c] * en[a];
-                        const double wnn = a == c ? en[a] * en[a] : 2.0 * en[a] * en[c];
-                        st[p][3] += wee * x;
-                        st[p][4] += wen * x;
-                        st[p][5] += wnn * x;
-                    } else {
+            for (int k = 0; k < NPR; k++) {
+                const int pr = rd * NPR + k;
+                if (pr >= NPAIR) continue;
+                const double2 z = sm[(k * NSQ + rw) * sstr + lp(i, sh)];
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const int f = 2 * pr + h;
+                    if (f >= NF) continue;
+                    const double x = (h == 0 ? z.x : z.y) * sc;
+                    if (f < NC) {
                         st[p][f] = x;
+                    } else if (f < NC + NP) {
+                        if constexpr (MODE == 1) {
+                            // planetary product plane (a, c): its share of e_eᵀFe_e, e_eᵀFe_n, e_nᵀFe_n
+                            const int k6 = f - NC;
+                            const int a = k6 < 3 ? 0 : (k6 < 5 ? 1 : 2);
+                            const int c = k6 < 3 ? k6 : (k6 < 5 ? k6 - 2 : 2);
+                            const double wee = a == c ? ee[a] * ee[a] : 2.0 * ee[a] * ee[c];
+                            const double wen = a == c ? ee[a] * en[a] : ee[a] * en[c] + ee[c] * en[a];
+                            const double wnn = a == c ? en[a] * en[a] : 2.0 * en[a] * en[c];
+                            st[p][3] += wee * x;
+                            st[p][4] += wen * x;
+                            st[p][5] += wnn * x;
+                        } else {
+                            st[p][f] = x;
+                        }
+                    } else {
+                        dn[p] = x;   // the filtered mask: Deformable's denominator
                     }
-                } else {
-                    dn[p] = x;   // the filtered mask: Deformable's denominator
                 }
             }
         }
-        __syncthreads();   // every lane has read the transform before the next pair is composed
+        __syncthreads();   // every lane has read the transforms before the next round is composed
     }
 #pragma unroll
     for (int p = 0; p < PPT; p++) {
-        const int e = (int)threadIdx.x + p * 256;
-        if (e >= nsq * n) continue;
-        const int sq = e / n, i = e - sq * n;
-        const int64_t row = r0 + sq;
+        const int e = (int)threadIdx.x + p * BLK;
+        if (e >= NSQ * N) continue;
+        const int rw = e / N, i = e - rw * N;
+        const int64_t row = r0 + rw;
         if (row >= nrows) continue;
-        const int64_t t = row * Nx + i, j = row % Ny;
+        const int64_t t = row * N + i;
         const double den = WM ? dn[p] : D_row[row];
         const bool ok = G.mask[t] != 0 && den > 1e-15;
         const double inv = ok ? 1.0 / den : 0.0;
@@ -981,8 +1025,8 @@ __global__ __launch_bounds__(256) void k_ifft_epilogue(RingFft F, const double2 
         T *tb = tau + b * 6 * P;
         double o0, o1, o2;
         if constexpr (MODE == 1) {
-            const double sl = G.trig[i], cl = G.trig[Nx + i];
-            const double sp = G.trig[2 * Nx + j], cp = G.trig[2 * Nx + Ny + j];
+            const double sl = G.trig[i], cl = G.trig[N + i];
+            const double sp = sp_[rw], cp = cp_[rw];
             const double U0 = st[p][0] * inv, U1 = st[p][1] * inv, U2 = st[p][2] * inv;
             o0 = -sl * U0 + cl * U1;
             o1 = -sp * cl * U0 - sp * sl * U1 + cp * U2;
@@ -1400,44 +1444,45 @@ void ring_prep(const GeomArgs &G, int dtype, const void *u, const void *v, const
 bool ring_inverse_epilogue(const RingPlan &R, const GeomArgs &G, int dtype, const double2 *OUT, const double *D_row,
                            int nf, int withmask, int strategy, double *ubar, void *tau, void *uo, void *vo, void *wo,
                            int64_t nbatch, hipStream_t s) {
-    RingFft F = R.fft;
-    if (R.fft.nseq == 0 || F.fixed == 0) return false;
+    const RingFft &F = R.fft;
+    if (F.nseq == 0 || (F.fixed != RING_FIXED_1440 && F.fixed != RING_FIXED_256)) return false;
     if (withmask && strategy != CGEF_DEFORMABLE) return false;
     const int mode = G.mode;
     if (nf != (mode == 0 ? 5 : 9) + withmask) return false;
+    if (F.fixed == RING_FIXED_1440 && mode != 1) return false;   // 1440 points: the 0.25° longitude
     const int64_t nrows = G.n[1] * G.n[2];
-    if (F.fixed == RING_FIXED_1440) F.nsq = 1;
-    else F.nsq = 4;
-    if (!fixed_fits(F)) return false;
-    const int64_t items = (nrows + F.nsq - 1) / F.nsq * nbatch;
+    if (nbatch * nf * nrows >= ((int64_t)1 << 31) / R.mh) return false;   // 32-bit mode offsets
+    const bool big = F.fixed == RING_FIXED_1440;
+    const int nsq = big ? IfeShape<RING_FIXED_1440>::NSQ : IfeShape<RING_FIXED_256>::NSQ;
+    const int npr = big ? IfeShape<RING_FIXED_1440>::NPR : IfeShape<RING_FIXED_256>::NPR;
+    const int blk = big ? IfeShape<RING_FIXED_1440>::BLK : IfeShape<RING_FIXED_256>::BLK;
+    for (int st = 0; st < F.snst; st++)   // one butterfly per lane in every stage, all of a round's sequences
+        if (F.n / F.srad[st] * nsq * npr > blk) return false;
+    const int64_t items = (nrows + nsq - 1) / nsq * nbatch;
     const dim3 grid((unsigned)(8 * ((items + 7) / 8)));
-    const size_t sh = staged_shmem(F);
-    bool launched = false;
+    const size_t sh = (size_t)nsq * npr * F.sstr * sizeof(double2);
     auto go = [&](auto kern, auto *tp) {
         using TT = std::remove_pointer_t<decltype(tp)>;
-        hipLaunchKernelGGL(kern, grid, dim3(256), sh, s, F, R.tw_d, R.tws_d, OUT, G, D_row, nbatch, ubar, (TT *)tau,
+        set_shmem((const void *)kern, sh);
+        hipLaunchKernelGGL(kern, grid, dim3(blk), sh, s, F, R.tw_d, R.tws_d, OUT, G, D_row, nbatch, ubar, (TT *)tau,
                            (TT *)uo, (TT *)vo, (TT *)wo);
-        launched = true;
     };
-#define CGEF_IFE(TT, FIXV, PPTV, MODEV)                                                              \
-    if (withmask) go(k_ifft_epilogue<TT, FIXV, PPTV, MODEV, 1>, (TT *)nullptr);                     \
-    else go(k_ifft_epilogue<TT, FIXV, PPTV, MODEV, 0>, (TT *)nullptr);
-    if (F.fixed == RING_FIXED_1440) {
-        if (mode != 1) return false;   // 1440-point rows: the longitude of a 0.25° sphere
-        if (dtype == CGEF_F32) { CGEF_IFE(float, RING_FIXED_1440, 6, 1) } else { CGEF_IFE(double, RING_FIXED_1440, 6, 1) }
+#define CGEF_IFE(TT, FIXV, MODEV)                                                                    \
+    if (withmask) go(k_ifft_epilogue<TT, FIXV, MODEV, 1>, (TT *)nullptr);                           \
+    else go(k_ifft_epilogue<TT, FIXV, MODEV, 0>, (TT *)nullptr);
+    if (big) {
+        if (dtype == CGEF_F32) { CGEF_IFE(float, RING_FIXED_1440, 1) } else { CGEF_IFE(double, RING_FIXED_1440, 1) }
+    } else if (dtype == CGEF_F32) {
+        if (mode == 0) { CGEF_IFE(float, RING_FIXED_256, 0) }
+        else if (mode == 1) { CGEF_IFE(float, RING_FIXED_256, 1) }
+        else { CGEF_IFE(float, RING_FIXED_256, 2) }
     } else {
-        if (dtype == CGEF_F32) {
-            if (mode == 0) { CGEF_IFE(float, RING_FIXED_256, 4, 0) }
-            else if (mode == 1) { CGEF_IFE(float, RING_FIXED_256, 4, 1) }
-            else { CGEF_IFE(float, RING_FIXED_256, 4, 2) }
-        } else {
-            if (mode == 0) { CGEF_IFE(double, RING_FIXED_256, 4, 0) }
-            else if (mode == 1) { CGEF_IFE(double, RING_FIXED_256, 4, 1) }
-            else { CGEF_IFE(double, RING_FIXED_256, 4, 2) }
-        }
+        if (mode == 0) { CGEF_IFE(double, RING_FIXED_256, 0) }
+        else if (mode == 1) { CGEF_IFE(double, RING_FIXED_256, 1) }
+        else { CGEF_IFE(double, RING_FIXED_256, 2) }
     }
 #undef CGEF_IFE
-    return launched;
+    return true;
 }
 
 void ring_epilogue(const GeomArgs &G, int dtype, const double *yout, const double *D_row, int nf, int withmask,
