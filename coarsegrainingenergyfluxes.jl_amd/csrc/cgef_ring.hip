@@ -648,16 +648,21 @@ __global__ __launch_bounds__(256, MINW) void k_contract(RingArgs A, const double
 // fall in the staged rows: one A fragment (one coalesced 8-byte load per lane from the lane-ordered
 // weight table) serves NT MFMAs, each with a B fragment read from LDS. Operand layouts (probed on
 // gfx950): lane l holds A[l%16][l/16] and B[l/16][l%16]; the accumulator's value v is D[l/16 + 4v][l%16].
+// The staging is double-buffered: the next pass's spectra and weights are requested into registers
+// before this pass's MFMAs are issued and reach the other LDS buffer after them, so one barrier per
+// pass separates the buffers and the loads' latency runs under the matrix unit's work.
 // Each staged spectrum value feeds up to 16 products in the matrix unit, where the VALU contraction
 // reads it from LDS once per FMA pair; the tile's band edges add (15 + offsets)/offsets of zero work.
 typedef double mfma_f64x4 __attribute__((ext_vector_type(4)));
 constexpr int MFMA_KC = 32;                   // source rows staged per pass (8 four-row steps)
 
 template <int NT>
-__global__ __launch_bounds__(256) void k_contract_mfma(RingArgs A, const double *__restrict__ wt, int nsteps, int wtiles,
+__global__ __launch_bounds__(256, 2) void k_contract_mfma(RingArgs A, const double *__restrict__ wt, int nsteps, int wtiles,
                                                        const double2 *__restrict__ XF, double2 *__restrict__ OUT,
                                                        int64_t ncol) {
     constexpr int LDW = NT * 16 + 2;          // row stride in doubles (≡ 4 banks mod 32: 16-byte stores spread)
+    constexpr int BUF = MFMA_KC * LDW;        // one staging buffer
+    static_assert(MFMA_KC == 32, "staging: a thread owns one row (tid % 32) and NT columns (tid / 32 + 8i)");
     extern __shared__ double lds[];
     const int64_t Ny = A.ny;
     const int64_t nrb = (Ny + 63) / 64;
@@ -684,49 +689,78 @@ __global__ __launch_bounds__(256) void k_contract_mfma(RingArgs A, const double 
     const int brlo = A.py ? 0 : (int)(S0 < 0 ? (-S0 < krows ? -S0 : krows) : 0);
     const int brhi = A.py ? krows : (int)(Ny - S0 < krows ? (Ny - S0 > 0 ? Ny - S0 : 0) : krows);
     const int brlo4 = brlo & ~3;
-    mfma_f64x4 acc[NT];
-#pragma unroll
-    for (int t = 0; t < NT; t++) acc[t] = mfma_f64x4{0.0, 0.0, 0.0, 0.0};
     const double2 *xs = XF + (m * ncol + c0) * A.nrows;
-    for (int base = brlo / MFMA_KC * MFMA_KC; base < brhi; base += MFMA_KC) {
-        // this wave's steps within the staged rows [base, base + KC) of the block's source rows; their
-        // weights are requested before the staging so the loads overlap it
-        int lo = base > 16 * wave ? base : 16 * wave;
+    const int srow = tid & 31, scol = tid >> 5;
+    // this wave's 4-row steps within the staged rows [base, base + KC) of the block's source rows
+    auto steps_of = [&](int base, int &lo, int &hi) {
+        lo = base > 16 * wave ? base : 16 * wave;
         lo = lo > brlo4 ? lo : brlo4;
-        int hi = base + MFMA_KC;
+        hi = base + MFMA_KC;
         hi = hi < 16 * wave + 4 * nsteps ? hi : 16 * wave + 4 * nsteps;
         hi = hi < brhi ? hi : brhi;
-        double a[MFMA_KC / 4];
+    };
+    // the pass's requests: row S0 + base + tid%32 of columns tid/32 + 8i (32 consecutive rows of a column
+    // per half wave: 512 contiguous bytes; zero outside a non-periodic axis and beyond the valid
+    // columns), and this wave's A fragments
+    auto request = [&](int base, double2 (&v)[NT], double (&a)[MFMA_KC / 4]) {
+        int64_t s = S0 + base + srow;
+        bool in = true;
+        if (s < 0 || s >= Ny) {
+            if (A.py) s = ((s % Ny) + Ny) % Ny;
+            else in = false;
+        }
+        const double2 *xr = xs + s;
+#pragma unroll
+        for (int i = 0; i < NT; i++) {
+            const int col = scol + 8 * i;
+            v[i] = (in && col < nvc) ? xr[(int64_t)col * A.nrows] : make_double2(0.0, 0.0);
+        }
+        int lo, hi;
+        steps_of(base, lo, hi);
 #pragma unroll
         for (int q = 0; q < MFMA_KC / 4; q++) {
             const int br = lo + 4 * q;
             a[q] = (active && br < hi) ? wp[(int64_t)((br - 16 * wave) >> 2) * 64] : 0.0;
         }
-        __syncthreads();
-        // stage rows S0 + base .. + KC − 1 of the chunk's columns: lane → (column, row), 32 consecutive
-        // rows of a column per half wave (512 contiguous bytes); zero outside a non-periodic axis and
-        // beyond the valid columns
-        for (int idx = tid; idx < MFMA_KC * NT * 8; idx += 256) {
-            const int row = idx % MFMA_KC, col = idx / MFMA_KC;
-            int64_t s = S0 + base + row;
-            bool in = col < nvc;
-            if (s < 0 || s >= Ny) {
-                if (A.py) s = ((s % Ny) + Ny) % Ny;
-                else in = false;
+    };
+    auto deposit = [&](double *buf, const double2 (&v)[NT]) {
+#pragma unroll
+        for (int i = 0; i < NT; i++) *reinterpret_cast<double2 *>(buf + srow * LDW + 2 * (scol + 8 * i)) = v[i];
+    };
+    mfma_f64x4 acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++) acc[t] = mfma_f64x4{0.0, 0.0, 0.0, 0.0};
+    const int pfirst = brlo / MFMA_KC * MFMA_KC;
+    double2 v[NT];
+    double a[MFMA_KC / 4], an[MFMA_KC / 4];
+    if (pfirst < brhi) {
+        request(pfirst, v, a);
+        deposit(lds, v);
+    }
+    __syncthreads();
+    int cur = 0;
+    for (int base = pfirst; base < brhi; base += MFMA_KC) {
+        const bool more = base + MFMA_KC < brhi;
+        if (more) request(base + MFMA_KC, v, an);
+        if (active) {
+            int lo, hi;
+            steps_of(base, lo, hi);
+            const double *sb = lds + cur * BUF;
+#pragma unroll
+            for (int q = 0; q < MFMA_KC / 4; q++) {
+                const int br = lo + 4 * q;
+                if (br >= hi) break;
+                const double *brow = sb + (br - base + (lane >> 4)) * LDW + (lane & 15);
+#pragma unroll
+                for (int t = 0; t < NT; t++)
+                    acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[q], brow[t * 16], acc[t], 0, 0, 0);
             }
-            const double2 v = in ? xs[(int64_t)col * A.nrows + s] : make_double2(0.0, 0.0);
-            *reinterpret_cast<double2 *>(lds + row * LDW + 2 * col) = v;
         }
+        if (more) deposit(lds + (cur ^ 1) * BUF, v);   // the other buffer: its last reads ended with the previous pass
         __syncthreads();
-        if (!active) continue;
+        cur ^= 1;
 #pragma unroll
-        for (int q = 0; q < MFMA_KC / 4; q++) {
-            const int br = lo + 4 * q;
-            if (br >= hi) break;
-            const double *brow = lds + (br - base + (lane >> 4)) * LDW + (lane & 15);
-#pragma unroll
-            for (int t = 0; t < NT; t++) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[q], brow[t * 16], acc[t], 0, 0, 0);
-        }
+        for (int q = 0; q < MFMA_KC / 4; q++) a[q] = an[q];
     }
     if (!active) return;
     // lane holds D[lane/16 + 4v][lane%16] of each column tile: target row, real column (re/im of a
@@ -738,9 +772,9 @@ __global__ __launch_bounds__(256) void k_contract_mfma(RingArgs A, const double 
         if (c >= nvc) continue;
         double *o = reinterpret_cast<double *>(OUT + (m * ncol + c0 + c) * A.nrows) + (j & 1);
 #pragma unroll
-        for (int v = 0; v < 4; v++) {
-            const int64_t r = tile * 16 + (lane >> 4) + 4 * v;
-            if (r < Ny) o[2 * r] = acc[t][v];
+        for (int v4 = 0; v4 < 4; v4++) {
+            const int64_t r = tile * 16 + (lane >> 4) + 4 * v4;
+            if (r < Ny) o[2 * r] = acc[t][v4];
         }
     }
 }
@@ -1337,8 +1371,9 @@ static void launch_contract_mfma(const RingPlan &R, const double2 *XF, double2 *
     const int64_t ncc = (2 * ncol + nt * 16 - 1) / (nt * 16);
     const int64_t items = R.mh * nrb * ncc;
     const dim3 grid((unsigned)(8 * ((items + 7) / 8)));
-    const size_t sh = (size_t)MFMA_KC * (nt * 16 + 2) * sizeof(double);
+    const size_t sh = 2 * (size_t)MFMA_KC * (nt * 16 + 2) * sizeof(double);   // two staging buffers
     auto go = [&](auto kern) {
+        set_shmem((const void *)kern, sh);
         hipLaunchKernelGGL(kern, grid, dim3(256), sh, s, R.args, R.wt_d, R.wt_steps, R.wt_tiles, XF, OUT, ncol);
     };
     if (nt == 9) go(k_contract_mfma<9>);
