@@ -413,15 +413,28 @@ static void ring_filter_planes(Plan &p, Workspace &w, int64_t ncol, hipStream_t 
     ring_inverse(p.ring, w.rout, w.rxy, ncol, s);
 }
 
+// Longitude ascends from coord[0][0] by 2π/N (to 1e-9 of a step): the condition under which the
+// sphere's east/north weights are the mode shifts the Fourier-space rotation applies.
+static bool lon_ascends_by_ring_step(const Grid &g) {
+    const int64_t N = g.n[0];
+    if (g.geometry != CGEF_SPHERICAL || (int64_t)g.coord[0].size() != N || N < 2) return false;
+    const double step = 2.0 * M_PI / (double)N;
+    for (int64_t i = 0; i < N; i++)
+        if (std::fabs(g.coord[0][i] - g.coord[0][0] - step * (double)i) > 1e-9 * step) return false;
+    return true;
+}
+
 // The contracted spectra of the Π planes (w.rout) to ū and τ: the fused inverse transform + epilogue
-// where the row length allows it (CGEF_FUSE=0 keeps the two kernels), else the inverse into w.rxy and
-// the separate epilogue.
+// where the row length allows it, else the inverse into w.rxy and the separate epilogue. On the sphere
+// the fused kernel rotates to the local frame in Fourier space where it can. CGEF_FUSE=0 keeps the two
+// kernels, CGEF_FUSE=1 the fused kernel with the rotation per point (testing).
 static void ring_pi_inverse_epilogue(Plan &p, Workspace &w, const GeomArgs &G, int dtype, int nf, int wm, void *uo,
                                      void *vo, void *wo, int64_t nbatch, hipStream_t s) {
     const char *fe = std::getenv("CGEF_FUSE");
     const bool fuse = !(fe && fe[0] == '0');
+    const bool spectral = !(fe && fe[0] == '1') && lon_ascends_by_ring_step(*p.grid);
     if (fuse && ring_inverse_epilogue(p.ring, G, dtype, w.rout, p.D64_row_d, nf, wm, p.strategy, w.ubar, w.tau, uo, vo,
-                                      wo, nbatch, s))
+                                      wo, nbatch, spectral, s))
         return;
     ring_inverse(p.ring, w.rout, w.rxy, nf * nbatch, s);
     ring_epilogue(G, dtype, w.rxy, p.D64_row_d, nf, wm, p.strategy, w.ubar, w.tau, uo, vo, wo, nbatch, s);

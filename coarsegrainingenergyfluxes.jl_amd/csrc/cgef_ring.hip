@@ -1096,6 +1096,138 @@ __global__ __launch_bounds__(IfeShape<FIX>::BLK) void k_ifft_epilogue(RingFft F,
     }
 }
 
+// The sphere's fused inverse transform + epilogue with the rotation to the local frame done in Fourier
+// space (1440-point rows, no radial export). The east/north frame weights are trigonometric
+// polynomials in λ of degree ≤ 2: with λ_k = λ₀ + 2πk/N, multiplying a row by cos λ or sin λ mixes
+// modes m ∓ 1 with phase w = e^{iλ₀}, by cos 2λ or sin 2λ modes m ∓ 2 with phase w². So per mode the
+// block forms, from the nine planetary planes, the spectra of the five local numerators
+//     E  = −s·U0 + c·U1,                       N   = −sinφ (c·U0 + s·U1) + cosφ·U2,
+//     Tee = D1 − Q,   Ten = sinφ·R + cosφ (c·Pyz − s·Pxz),
+//     Tnn = sin²φ (D1 + Q) − 2 sinφ cosφ (c·Pxz + s·Pyz) + cos²φ·Pzz,
+// with c, s = cos λ, sin λ, D1 = (Pxx + Pyy)/2, D2 = (Pxx − Pyy)/2, Q = cos 2λ·D2 + sin 2λ·Pxy and
+// R = sin 2λ·D2 − cos 2λ·Pxy (e_east = (−s, c, 0), e_north = (−sinφ c, −sinφ s, cosφ); Diagnostics.jl:
+// 596-650) — and inverse-transforms only those (E + iN, Tee + iTen, Tnn + i·mask): three sequences in
+// one round instead of five pairs in two. Modes beyond 0..N/2 come from X[−m] = X[N−m] = conj X[m].
+// The products and the components are staged through the LDS that the transforms then use.
+template <typename T, int WM>
+__global__ __launch_bounds__(512) void k_ifft_epilogue_sphere(RingFft F, const double2 *__restrict__ tw,
+                                                             const double2 *__restrict__ tws,
+                                                             const double2 *__restrict__ OUT, GeomArgs G,
+                                                             const double *__restrict__ D_row, int64_t nbatch,
+                                                             double *__restrict__ ubar, T *__restrict__ tau,
+                                                             T *__restrict__ uo, T *__restrict__ vo) {
+    constexpr int N = 1440, MH = N / 2 + 1, BLK = 512;
+    constexpr int PPT = (N + BLK - 1) / BLK;                 // points per lane
+    constexpr int MPT = (MH + BLK - 1) / BLK;                // modes per lane
+    constexpr int NF = 9 + WM;
+    constexpr double sc = 1.0 / N;
+    extern __shared__ double2 sm[];
+    const int sh = F.lpsh, sstr = F.sstr;
+    const int64_t nrows = G.n[1] * G.n[2], P = G.npts, Ny = G.n[1];
+    int64_t item;
+    if (!xcd_item(nrows * nbatch, item)) return;
+    const int64_t b = item / nrows, row = item % nrows;
+    const int mstride = (int)(nbatch * NF * nrows);          // one mode of the [m][col][row] spectra (< 2^31)
+    const double2 *spec = OUT + (b * NF) * nrows + row;      // plane f, mode m: spec[f·nrows + m·mstride]
+    const int64_t j = row % Ny;
+    const double sp = G.trig[2 * N + j], cp = G.trig[2 * N + Ny + j];
+    const double s0 = G.trig[0], c0 = G.trig[N];             // w = e^{iλ₀}
+    const double2 w1 = make_double2(c0, s0), w2 = make_double2(c0 * c0 - s0 * s0, 2.0 * s0 * c0);
+    double2 *pl = sm;                                        // staged spectra [plane][MH]
+    auto at = [&](int k, int m) {                            // plane k at any mode in −2..MH+1
+        if (m < 0) return cconj(pl[k * MH - m]);
+        if (m >= MH) return cconj(pl[k * MH + N - m]);
+        return pl[k * MH + m];
+    };
+    auto mi = [](double2 a) { return make_double2(-a.y, a.x); };                   // i·a
+    auto half = [](double2 a) { return make_double2(0.5 * a.x, 0.5 * a.y); };
+    // the products xx xy xz yy yz zz (planes 3..8)
+    for (int e = threadIdx.x; e < 6 * MH; e += BLK) {
+        const int k = e / MH, m = e - k * MH;
+        pl[e] = spec[(int64_t)(3 + k) * nrows + (int64_t)m * mstride];
+    }
+    __syncthreads();
+    double2 Tee[MPT], Ten[MPT], Tnn[MPT];
+#pragma unroll
+    for (int q = 0; q < MPT; q++) {
+        const int m = (int)threadIdx.x + q * BLK;
+        if (m >= MH) continue;
+        // G = D2 − i·Pxy at m − 2, H = D2 + i·Pxy at m + 2: Q = (w²G + w̄²H)/2, R = (w²G − w̄²H)/(2i)
+        const double2 d2l = half(csub(at(0, m - 2), at(3, m - 2))), d2h = half(csub(at(0, m + 2), at(3, m + 2)));
+        const double2 gl = cmul(w2, csub(d2l, mi(at(1, m - 2))));
+        const double2 gh = cmul(cconj(w2), cadd(d2h, mi(at(1, m + 2))));
+        const double2 Q = half(cadd(gl, gh));
+        const double2 dq = half(csub(gl, gh));
+        const double2 R = make_double2(dq.y, -dq.x);                             // dq / i
+        const double2 D1 = half(cadd(at(0, m), at(3, m)));
+        // c·Pxz + s·Pyz and c·Pyz − s·Pxz: (w·(X − iY)[m−1] + w̄·(X + iY)[m+1]) / 2 with (X, Y) = (Pxz, Pyz), (Pyz, −Pxz)
+        const double2 xl = at(2, m - 1), yl = at(4, m - 1), xh = at(2, m + 1), yh = at(4, m + 1);
+        const double2 A1 = half(cadd(cmul(w1, csub(xl, mi(yl))), cmul(cconj(w1), cadd(xh, mi(yh)))));
+        const double2 B1 = half(cadd(cmul(w1, cadd(yl, mi(xl))), cmul(cconj(w1), csub(yh, mi(xh)))));
+        const double2 pzz = at(5, m);
+        Tee[q] = csub(D1, Q);
+        Ten[q] = make_double2(sp * R.x + cp * B1.x, sp * R.y + cp * B1.y);
+        Tnn[q] = make_double2(sp * sp * (D1.x + Q.x) - 2.0 * sp * cp * A1.x + cp * cp * pzz.x,
+                              sp * sp * (D1.y + Q.y) - 2.0 * sp * cp * A1.y + cp * cp * pzz.y);
+    }
+    __syncthreads();
+    // the components U0 U1 U2 (planes 0..2) and, for Deformable, the mask (plane 9)
+    for (int e = threadIdx.x; e < (3 + WM) * MH; e += BLK) {
+        const int k = e / MH, m = e - k * MH;
+        pl[e] = spec[(int64_t)(k < 3 ? k : 9) * nrows + (int64_t)m * mstride];
+    }
+    __syncthreads();
+    double2 Ev[MPT], Nv[MPT], Mv[MPT];
+#pragma unroll
+    for (int q = 0; q < MPT; q++) {
+        const int m = (int)threadIdx.x + q * BLK;
+        if (m >= MH) continue;
+        const double2 u0l = at(0, m - 1), u1l = at(1, m - 1), u0h = at(0, m + 1), u1h = at(1, m + 1);
+        // −s·U0 + c·U1 = (w·(U1 + iU0)[m−1] + w̄·(U1 − iU0)[m+1]) / 2;  c·U0 + s·U1 likewise with (U0, −U1)
+        Ev[q] = half(cadd(cmul(w1, cadd(u1l, mi(u0l))), cmul(cconj(w1), csub(u1h, mi(u0h)))));
+        const double2 cs = half(cadd(cmul(w1, csub(u0l, mi(u1l))), cmul(cconj(w1), cadd(u0h, mi(u1h)))));
+        const double2 u2 = at(2, m);
+        Nv[q] = make_double2(cp * u2.x - sp * cs.x, cp * u2.y - sp * cs.y);
+        Mv[q] = WM ? pl[3 * MH + m] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();   // the staged spectra are read: the sequences may overwrite them
+#pragma unroll
+    for (int q = 0; q < MPT; q++) {
+        const int m = (int)threadIdx.x + q * BLK;
+        if (m >= MH) continue;
+        const double2 X[3] = {Ev[q], Tee[q], Tnn[q]}, Y[3] = {Nv[q], Ten[q], Mv[q]};
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            double2 *z = sm + k * sstr;
+            z[lp(m, sh)] = make_double2(X[k].x - Y[k].y, X[k].y + Y[k].x);
+            if (m > 0 && N - m >= MH) z[lp(N - m, sh)] = make_double2(X[k].x + Y[k].y, Y[k].x - X[k].y);
+        }
+    }
+    RingFft Fr = F;
+    Fr.nsq = 3;
+    fft_rows_any<RING_FIXED_1440, BLK>(sm, Fr, tw, tws, true);
+    double *ub = ubar + b * 3 * P;
+    T *tb = tau + b * 6 * P;
+#pragma unroll
+    for (int p = 0; p < PPT; p++) {
+        const int i = (int)threadIdx.x + p * BLK;
+        if (i >= N) continue;
+        const double2 z0 = sm[lp(i, sh)], z1 = sm[sstr + lp(i, sh)], z2 = sm[2 * sstr + lp(i, sh)];
+        const int64_t t = row * N + i;
+        const double den = WM ? z2.y * sc : D_row[row];
+        const bool ok = G.mask[t] != 0 && den > 1e-15;
+        const double inv = ok ? sc / den : 0.0;
+        const double o0 = z0.x * inv, o1 = z0.y * inv;
+        tb[t] = (T)(z1.x * inv - o0 * o0);
+        tb[P + t] = (T)(z1.y * inv - o0 * o1);
+        tb[2 * P + t] = (T)(z2.x * inv - o1 * o1);
+        ub[t] = o0;
+        ub[P + t] = o1;
+        if (uo) uo[b * P + t] = (T)o0;
+        if (vo) vo[b * P + t] = (T)o1;
+    }
+}
+
 // plain fields for filter_apply: x = f·active (FP64), plus the mask when a Deformable denominator is needed
 __global__ __launch_bounds__(256) void k_ring_prep_fields(GeomArgs G, int dtype, const void *in, int64_t nsl, int withmask,
                                                           double *xin) {
@@ -1478,7 +1610,7 @@ void ring_prep(const GeomArgs &G, int dtype, const void *u, const void *v, const
 // false when the two separate kernels must serve.
 bool ring_inverse_epilogue(const RingPlan &R, const GeomArgs &G, int dtype, const double2 *OUT, const double *D_row,
                            int nf, int withmask, int strategy, double *ubar, void *tau, void *uo, void *vo, void *wo,
-                           int64_t nbatch, hipStream_t s) {
+                           int64_t nbatch, bool spectral_rotation, hipStream_t s) {
     const RingFft &F = R.fft;
     if (F.nseq == 0 || (F.fixed != RING_FIXED_1440 && F.fixed != RING_FIXED_256)) return false;
     if (withmask && strategy != CGEF_DEFORMABLE) return false;
@@ -1488,6 +1620,30 @@ bool ring_inverse_epilogue(const RingPlan &R, const GeomArgs &G, int dtype, cons
     const int64_t nrows = G.n[1] * G.n[2];
     if (nbatch * nf * nrows >= ((int64_t)1 << 31) / R.mh) return false;   // 32-bit mode offsets
     const bool big = F.fixed == RING_FIXED_1440;
+    if (spectral_rotation && big && !wo) {
+        // the sphere's local frame formed in Fourier space: three sequences per row in one round
+        bool fits = (size_t)6 * R.mh <= (size_t)3 * F.sstr;
+        for (int st = 0; st < F.snst; st++) fits = fits && F.n / F.srad[st] * 3 <= 512;
+        if (fits) {
+            const int64_t items = nrows * nbatch;
+            const dim3 grid((unsigned)(8 * ((items + 7) / 8)));
+            const size_t sh = (size_t)3 * F.sstr * sizeof(double2);
+            auto go = [&](auto kern, auto *tp) {
+                using TT = std::remove_pointer_t<decltype(tp)>;
+                set_shmem((const void *)kern, sh);
+                hipLaunchKernelGGL(kern, grid, dim3(512), sh, s, F, R.tw_d, R.tws_d, OUT, G, D_row, nbatch, ubar,
+                                   (TT *)tau, (TT *)uo, (TT *)vo);
+            };
+            if (dtype == CGEF_F32) {
+                if (withmask) go(k_ifft_epilogue_sphere<float, 1>, (float *)nullptr);
+                else go(k_ifft_epilogue_sphere<float, 0>, (float *)nullptr);
+            } else {
+                if (withmask) go(k_ifft_epilogue_sphere<double, 1>, (double *)nullptr);
+                else go(k_ifft_epilogue_sphere<double, 0>, (double *)nullptr);
+            }
+            return true;
+        }
+    }
     const int nsq = big ? IfeShape<RING_FIXED_1440>::NSQ : IfeShape<RING_FIXED_256>::NSQ;
     const int npr = big ? IfeShape<RING_FIXED_1440>::NPR : IfeShape<RING_FIXED_256>::NPR;
     const int blk = big ? IfeShape<RING_FIXED_1440>::BLK : IfeShape<RING_FIXED_256>::BLK;

@@ -44,10 +44,12 @@ void ring_inverse(const RingPlan &R, const double2 *OUT, double *yout, int64_t n
 void ring_prep(const GeomArgs &G, int dtype, const void *u, const void *v, const void *w, int nf, int withmask,
                double *xin, int64_t nbatch, hipStream_t s);
 // inverse row transforms of the contracted spectra fused with the epilogue (false: not served for this
-// length or plane set; use ring_inverse + ring_epilogue)
+// length or plane set; use ring_inverse + ring_epilogue). spectral_rotation: on the 1440-point sphere
+// without a radial export, rotate to the local frame in Fourier space (the caller has checked that
+// longitude ascends by 2π/N from trig[0]), else per point after the transforms.
 bool ring_inverse_epilogue(const RingPlan &R, const GeomArgs &G, int dtype, const double2 *OUT, const double *D_row,
                            int nf, int withmask, int strategy, double *ubar, void *tau, void *uo, void *vo, void *wo,
-                           int64_t nbatch, hipStream_t s);
+                           int64_t nbatch, bool spectral_rotation, hipStream_t s);
 void ring_epilogue(const GeomArgs &G, int dtype, const double *yout, const double *D_row, int nf, int withmask,
                    int strategy, double *ubar, void *tau, void *uo, void *vo, void *wo, int64_t nbatch, hipStream_t s);
 void ring_prep_fields(const GeomArgs &G, int dtype, const void *in, int64_t nsl, int withmask, double *xin, hipStream_t s);

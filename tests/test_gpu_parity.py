@@ -444,15 +444,19 @@ def test_ring_pi_wide_footprint(strategy, dtype):
 @pytest.mark.parametrize("geom", ["sphere1440", "cartesian256", "volume256"])
 @pytest.mark.parametrize("strategy", ["zerofill", "deformable"])
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
-def test_fused_inverse_epilogue(geom, strategy, dtype, monkeypatch):
+@pytest.mark.parametrize("form", ["default", "per_point_rotation"])
+def test_fused_inverse_epilogue(geom, strategy, dtype, form, monkeypatch):
     # Row lengths with a compiled-in factorization (1440: the 0.25° longitude; 256) take the inverse
     # transforms fused with the epilogue: against the oracle, and against the two separate kernels
     # (CGEF_FUSE=0), for the sphere's local-frame τ, the 2-D and 3-D Cartesian sets, the mask plane of
-    # Deformable and two snapshots.
-    print(f"fused inverse+epilogue {geom} {strategy} {dtype}", flush=True)
+    # Deformable and two snapshots. On the sphere the default rotates to the local frame in Fourier
+    # space (longitudes ascending from an arbitrary λ₀, so the mode shifts carry a phase); CGEF_FUSE=1
+    # rotates per point after the transforms.
+    print(f"fused inverse+epilogue {geom} {strategy} {dtype} {form}", flush=True)
     s_cg, s_o = STRATS[strategy]
     if geom == "sphere1440":
         lam, phi = sphere_axes(1440, 36)
+        lam = lam - 2.0
         mask = blob_mask((36, 1440), 44, 0.3)
         og = oracle_grid_sphere(lam, phi, mask=mask)
         g = cgef.StructuredGrid(cgef.SphericalGeometry(6.371e6), lam, phi, mask=mask)
@@ -477,7 +481,10 @@ def test_fused_inverse_epilogue(geom, strategy, dtype, monkeypatch):
     W = F[2] if comps == 3 else None
     ws = cgef.PiWorkspace(g, (2,), dtype)
     Pi = torch.empty_like(F[0])
-    monkeypatch.delenv("CGEF_FUSE", raising=False)
+    if form == "default":
+        monkeypatch.delenv("CGEF_FUSE", raising=False)
+    else:
+        monkeypatch.setenv("CGEF_FUSE", "1")
     cgef.compute_pi_(Pi, F[0], F[1], W, g, ker, ell, workspace=ws, filter_plan=plan)
     got, ub = N(Pi), N(ws.u_filt)
     monkeypatch.setenv("CGEF_FUSE", "0")
