@@ -372,11 +372,14 @@ __device__ __forceinline__ void stage_staged(double2 *sm, int n, int Ns, int sh,
 #pragma unroll
                 for (int r = 1; r < R; r++) v[q][r] = x[lp(j + r * nb, sh)];
             } else {
+                // one table load per butterfly: ω^k, its powers ω^(rk) = ω^(⌈r/2⌉k)·ω^(⌊r/2⌋k) (at most
+                // four products deep for r ≤ 15), in place of R − 1 loads through the texture path
+                double2 wp[R];
+                wp[1] = inv ? cconj(t[0]) : t[0];
 #pragma unroll
-                for (int r = 1; r < R; r++) {
-                    const double2 w = t[r - 1];
-                    v[q][r] = cmul(x[lp(j + r * nb, sh)], inv ? cconj(w) : w);
-                }
+                for (int r = 2; r < R; r++) wp[r] = cmul(wp[(r + 1) / 2], wp[r / 2]);
+#pragma unroll
+                for (int r = 1; r < R; r++) v[q][r] = cmul(x[lp(j + r * nb, sh)], wp[r]);
             }
             dft_small<R>(v[q], tw, n, inv);
             dst[q] = (j - k) * R + k;
