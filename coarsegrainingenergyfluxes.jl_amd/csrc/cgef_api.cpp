@@ -424,18 +424,27 @@ static bool lon_ascends_by_ring_step(const Grid &g) {
     return true;
 }
 
-// The contracted spectra of the Π planes (w.rout) to ū and τ: the fused inverse transform + epilogue
-// where the row length allows it, else the inverse into w.rxy and the separate epilogue. On the sphere
-// the fused kernel rotates to the local frame in Fourier space where it can. CGEF_FUSE=0 keeps the two
-// kernels, CGEF_FUSE=1 the fused kernel with the rotation per point (testing).
-static void ring_pi_inverse_epilogue(Plan &p, Workspace &w, const GeomArgs &G, int dtype, int nf, int wm, void *uo,
-                                     void *vo, void *wo, int64_t nbatch, hipStream_t s) {
+// Whether the Π planes' inverse transform runs fused with the epilogue (the row length allows it and
+// CGEF_FUSE is not 0); the contraction then writes its output as [m][row][col].
+static bool ring_pi_fused(const Plan &p, const GeomArgs &G, int nf, int wm, int64_t nbatch) {
     const char *fe = std::getenv("CGEF_FUSE");
-    const bool fuse = !(fe && fe[0] == '0');
-    const bool spectral = !(fe && fe[0] == '1') && lon_ascends_by_ring_step(*p.grid);
-    if (fuse && ring_inverse_epilogue(p.ring, G, dtype, w.rout, p.D64_row_d, nf, wm, p.strategy, w.ubar, w.tau, uo, vo,
-                                      wo, nbatch, spectral, s))
+    if (fe && fe[0] == '0') return false;
+    return ring_inverse_epilogue_serves(p.ring, G, nf, wm, p.strategy, nbatch);
+}
+
+// The contracted spectra of the Π planes (w.rout) to ū and τ: the fused inverse transform + epilogue
+// when `fused` (from ring_pi_fused, before the contraction), else the inverse into w.rxy and the
+// separate epilogue. On the sphere the fused kernel rotates to the local frame in Fourier space where
+// it can; CGEF_FUSE=1 keeps the rotation per point (testing).
+static void ring_pi_inverse_epilogue(Plan &p, Workspace &w, const GeomArgs &G, int dtype, int nf, int wm, void *uo,
+                                     void *vo, void *wo, int64_t nbatch, bool fused, hipStream_t s) {
+    if (fused) {
+        const char *fe = std::getenv("CGEF_FUSE");
+        const bool spectral = !(fe && fe[0] == '1') && lon_ascends_by_ring_step(*p.grid);
+        ring_inverse_epilogue(p.ring, G, dtype, w.rout, p.D64_row_d, nf, wm, p.strategy, w.ubar, w.tau, uo, vo, wo,
+                              nbatch, spectral, s);
         return;
+    }
     ring_inverse(p.ring, w.rout, w.rxy, nf * nbatch, s);
     ring_epilogue(G, dtype, w.rxy, p.D64_row_d, nf, wm, p.strategy, w.ubar, w.tau, uo, vo, wo, nbatch, s);
 }
@@ -859,10 +868,11 @@ cgef_status cgef_compute_pi(cgef_plan *plan, cgef_workspace *ws, void *Pi, const
             ring_reserve(W, *p, nf * nbatch);
             ring_prep(G, dtype, u, v, w, nf, wm, W.rxy, nbatch, s);
             ring_forward(p->ring, W.rxy, W.rxf, nf * nbatch, s);
+            const bool fused = ring_pi_fused(*p, G, nf, wm, nbatch);
             p->prof_begin(s);
-            ring_contract(p->ring, W.rxf, W.rout, nf * nbatch, s);
+            ring_contract(p->ring, W.rxf, W.rout, nf * nbatch, s, fused);
             p->prof_end(s);
-            ring_pi_inverse_epilogue(*p, W, G, dtype, nf, wm, ubar, vbar, wbar, nbatch, s);
+            ring_pi_inverse_epilogue(*p, W, G, dtype, nf, wm, ubar, vbar, wbar, nbatch, fused, s);
             launch_strain_pi(G, dtype, W.ubar, W.tau, Pi, rho0, nbatch, W.partials, W.npartials, s);
             CGEF_HIP(hipGetLastError());
             if (E_out) fetch_energy(W, nbatch, strain_blocks(G), g.active_area, E_out, s);
@@ -945,10 +955,11 @@ cgef_status cgef_coarse_grain(cgef_plan *const *plans, int64_t nscales, cgef_wor
                     ring_forward(p0.ring, W.rxy, W.rxf, nf * nbatch, st);
                     for (int64_t q = 0; q < nscales; q++) {
                         Plan &p = *reinterpret_cast<Plan *>(plans[q]);
+                        const bool fused = ring_pi_fused(p, G, nf, wm, nbatch);
                         p.prof_begin(st);
-                        ring_contract(p.ring, W.rxf, W.rout, nf * nbatch, st);
+                        ring_contract(p.ring, W.rxf, W.rout, nf * nbatch, st, fused);
                         p.prof_end(st);
-                        ring_pi_inverse_epilogue(p, W, G, dtype, nf, wm, nullptr, nullptr, nullptr, nbatch, st);
+                        ring_pi_inverse_epilogue(p, W, G, dtype, nf, wm, nullptr, nullptr, nullptr, nbatch, fused, st);
                         launch_strain_pi(G, dtype, W.ubar, W.tau, (char *)Pi + (size_t)q * P * es, rho0, nbatch,
                                          W.partials, W.npartials, st, nscales * P);
                         if (E_out) launch_energy_final(nbatch, nblk, W.partials, g.active_area, Eall + q * nbatch, st);

@@ -105,7 +105,7 @@ struct RingArgs {
 // Measured choice of the banded-contraction kernel variant per column count (see ring_contract).
 struct ContractTune {
     std::mutex mu;
-    std::vector<std::pair<int64_t, int>> choice;   // (columns, variant)
+    std::vector<std::pair<int64_t, int>> choice;   // (2·columns + output layout, variant)
 };
 
 struct RingPlan {

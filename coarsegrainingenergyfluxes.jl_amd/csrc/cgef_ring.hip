@@ -553,7 +553,7 @@ __global__ __launch_bounds__(256) void k_ifft_rows_staged(RingFft F, const doubl
 
 template <int CC, int PB, int MINW>
 __global__ __launch_bounds__(256, MINW) void k_contract(RingArgs A, const double2 *__restrict__ XF, double2 *__restrict__ OUT,
-                                                        int64_t ncol) {
+                                                        int64_t ncol, int out_rowmajor) {
     extern __shared__ double2 st[];
     const int64_t Ny = A.ny, Nz = A.nz;
     const int64_t nyb = (Ny + 255) / 256;
@@ -630,6 +630,20 @@ __global__ __launch_bounds__(256, MINW) void k_contract(RingArgs A, const double
             }
         }
     }
+    if (out_rowmajor) {
+        // OUT[m][row][col]: the block's 256 rows × CC columns pass through LDS and are stored row by
+        // row, CC adjacent columns per row, so a wave's stores cover about 13 rows instead of 64
+        __syncthreads();   // every lane has finished reading the staged spectra
+#pragma unroll
+        for (int c = 0; c < CC; c++) st[tid * CC + c] = acc[c];
+        __syncthreads();
+        const int nrv = (int)((Ny - j0) < 256 ? (Ny - j0) : 256);   // valid rows of this block
+        for (int e = tid; e < nrv * CC; e += 256) {
+            const int rr = e / CC, c = e - rr * CC;
+            if (c < nvc) OUT[(m * A.nrows + j0 + rr + Ny * k) * ncol + c0 + c] = st[e];
+        }
+        return;
+    }
     if (!valid) return;
     // fully unrolled with a predicate: a loop bounded by nvc would index acc dynamically and move the
     // accumulators to scratch memory
@@ -662,7 +676,7 @@ constexpr int MFMA_KC = 32;                   // source rows staged per pass (8 
 template <int NT>
 __global__ __launch_bounds__(256, 2) void k_contract_mfma(RingArgs A, const double *__restrict__ wt, int nsteps, int wtiles,
                                                        const double2 *__restrict__ XF, double2 *__restrict__ OUT,
-                                                       int64_t ncol) {
+                                                       int64_t ncol, int out_rowmajor) {
     constexpr int LDW = NT * 16 + 2;          // row stride in doubles (≡ 4 banks mod 32: 16-byte stores spread)
     constexpr int BUF = MFMA_KC * LDW;        // one staging buffer
     static_assert(MFMA_KC == 32, "staging: a thread owns one row (tid % 32) and NT columns (tid / 32 + 8i)");
@@ -767,17 +781,19 @@ __global__ __launch_bounds__(256, 2) void k_contract_mfma(RingArgs A, const doub
     }
     if (!active) return;
     // lane holds D[lane/16 + 4v][lane%16] of each column tile: target row, real column (re/im of a
-    // complex column)
+    // complex column); in the [m][row][col] layout a row's 16 real columns are 128 contiguous bytes
 #pragma unroll
     for (int t = 0; t < NT; t++) {
         const int j = t * 16 + (lane & 15);
         const int c = j >> 1;
         if (c >= nvc) continue;
-        double *o = reinterpret_cast<double *>(OUT + (m * ncol + c0 + c) * A.nrows) + (j & 1);
 #pragma unroll
         for (int v4 = 0; v4 < 4; v4++) {
             const int64_t r = tile * 16 + (lane >> 4) + 4 * v4;
-            if (r < Ny) o[2 * r] = acc[t][v4];
+            if (r >= Ny) continue;
+            double *o = out_rowmajor ? reinterpret_cast<double *>(OUT + (m * A.nrows + r) * ncol + c0 + c)
+                                     : reinterpret_cast<double *>(OUT + (m * ncol + c0 + c) * A.nrows + r);
+            o[j & 1] = acc[t][v4];
         }
     }
 }
@@ -959,8 +975,9 @@ __global__ __launch_bounds__(IfeShape<FIX>::BLK) void k_ifft_epilogue(RingFft F,
     int64_t item;
     if (!xcd_item(ngrp * nbatch, item)) return;
     const int64_t b = item / ngrp, r0 = (item % ngrp) * NSQ;
-    const int mstride = (int)(nbatch * NF * nrows);          // one mode of the [m][col][row] spectra (< 2^31)
-    const double2 *spec = OUT + (b * NF) * nrows + r0;       // plane 0, row r0, mode 0
+    const int64_t ncolt = nbatch * NF;                       // columns of the [m][row][col] spectra
+    const int mstride = (int)(ncolt * nrows);                // one mode (< 2^31)
+    const double2 *spec = OUT + r0 * ncolt + b * NF;         // row r0, plane 0, mode 0
     constexpr double sc = 1.0 / N;
     double sp_[NSQ], cp_[NSQ];                               // the rows' sinφ, cosφ (sphere)
 #pragma unroll
@@ -987,8 +1004,9 @@ __global__ __launch_bounds__(IfeShape<FIX>::BLK) void k_ifft_epilogue(RingFft F,
             const int fa = 2 * (rd * NPR + k), fb = fa + 1;
             double2 X = make_double2(0.0, 0.0), Y = X;
             if (r0 + rw < nrows) {
-                X = spec[(int64_t)fa * nrows + m * mstride + rw];
-                if (fb < NF) Y = spec[(int64_t)fb * nrows + m * mstride + rw];
+                const double2 *x = spec + (int64_t)m * mstride + rw * ncolt;   // planes fa, fb adjacent
+                X = x[fa];
+                if (fb < NF) Y = x[fb];
             }
             double2 *z = sm + sq * sstr;
             z[lp(m, sh)] = make_double2(X.x - Y.y, X.y + Y.x);
@@ -1130,8 +1148,8 @@ __global__ __launch_bounds__(512) void k_ifft_epilogue_sphere(RingFft F, const d
     int64_t item;
     if (!xcd_item(nrows * nbatch, item)) return;
     const int64_t b = item / nrows, row = item % nrows;
-    const int mstride = (int)(nbatch * NF * nrows);          // one mode of the [m][col][row] spectra (< 2^31)
-    const double2 *spec = OUT + (b * NF) * nrows + row;      // plane f, mode m: spec[f·nrows + m·mstride]
+    const int mstride = (int)(nbatch * NF * nrows);          // one mode of the [m][row][col] spectra (< 2^31)
+    const double2 *spec = OUT + row * (nbatch * NF) + b * NF; // plane f, mode m: spec[m·mstride + f]
     const int64_t j = row % Ny;
     const double sp = G.trig[2 * N + j], cp = G.trig[2 * N + Ny + j];
     const double s0 = G.trig[0], c0 = G.trig[N];             // w = e^{iλ₀}
@@ -1144,10 +1162,10 @@ __global__ __launch_bounds__(512) void k_ifft_epilogue_sphere(RingFft F, const d
     };
     auto mi = [](double2 a) { return make_double2(-a.y, a.x); };                   // i·a
     auto half = [](double2 a) { return make_double2(0.5 * a.x, 0.5 * a.y); };
-    // the products xx xy xz yy yz zz (planes 3..8)
+    // the products xx xy xz yy yz zz (planes 3..8: 96 contiguous bytes per mode)
     for (int e = threadIdx.x; e < 6 * MH; e += BLK) {
-        const int k = e / MH, m = e - k * MH;
-        pl[e] = spec[(int64_t)(3 + k) * nrows + (int64_t)m * mstride];
+        const int m = e / 6, k = e - 6 * m;
+        pl[k * MH + m] = spec[(int64_t)m * mstride + 3 + k];
     }
     __syncthreads();
     double2 Tee[MPT], Ten[MPT], Tnn[MPT];
@@ -1176,8 +1194,8 @@ __global__ __launch_bounds__(512) void k_ifft_epilogue_sphere(RingFft F, const d
     __syncthreads();
     // the components U0 U1 U2 (planes 0..2) and, for Deformable, the mask (plane 9)
     for (int e = threadIdx.x; e < (3 + WM) * MH; e += BLK) {
-        const int k = e / MH, m = e - k * MH;
-        pl[e] = spec[(int64_t)(k < 3 ? k : 9) * nrows + (int64_t)m * mstride];
+        const int m = e / (3 + WM), k = e - (3 + WM) * m;
+        pl[k * MH + m] = spec[(int64_t)m * mstride + (k < 3 ? k : 9)];
     }
     __syncthreads();
     double2 Ev[MPT], Nv[MPT], Mv[MPT];
@@ -1500,7 +1518,8 @@ static int mfma_nt(int64_t ncol) {
     return best;
 }
 
-static void launch_contract_mfma(const RingPlan &R, const double2 *XF, double2 *OUT, int64_t ncol, hipStream_t s) {
+static void launch_contract_mfma(const RingPlan &R, const double2 *XF, double2 *OUT, int64_t ncol, int orm,
+                                 hipStream_t s) {
     const int nt = mfma_nt(ncol);
     const int64_t nrb = (R.args.ny + 63) / 64;
     const int64_t ncc = (2 * ncol + nt * 16 - 1) / (nt * 16);
@@ -1509,7 +1528,7 @@ static void launch_contract_mfma(const RingPlan &R, const double2 *XF, double2 *
     const size_t sh = 2 * (size_t)MFMA_KC * (nt * 16 + 2) * sizeof(double);   // two staging buffers
     auto go = [&](auto kern) {
         set_shmem((const void *)kern, sh);
-        hipLaunchKernelGGL(kern, grid, dim3(256), sh, s, R.args, R.wt_d, R.wt_steps, R.wt_tiles, XF, OUT, ncol);
+        hipLaunchKernelGGL(kern, grid, dim3(256), sh, s, R.args, R.wt_d, R.wt_steps, R.wt_tiles, XF, OUT, ncol, orm);
     };
     if (nt == 9) go(k_contract_mfma<9>);
     else if (nt == 5) go(k_contract_mfma<5>);
@@ -1524,7 +1543,7 @@ void ring_build_wtiles(RingPlan &R, hipStream_t s) {
     hipLaunchKernelGGL(k_build_wtiles, dim3((unsigned)blocks), dim3(256), 0, s, R.args, R.wt_steps, R.wt_tiles, R.wt_d);
 }
 
-void ring_contract(const RingPlan &R, const double2 *XF, double2 *OUT, int64_t ncol, hipStream_t s) {
+void ring_contract(const RingPlan &R, const double2 *XF, double2 *OUT, int64_t ncol, hipStream_t s, bool out_rowmajor) {
     constexpr int CC = RING_CONTRACT_CC;
     const int span = 256 + R.args.dy_max - R.args.dy_min;
     const size_t sh = (size_t)CC * span * sizeof(double2);
@@ -1534,12 +1553,14 @@ void ring_contract(const RingPlan &R, const double2 *XF, double2 *OUT, int64_t n
     const size_t lds_blocks = LDS_BYTES_PER_CU / sh;
     const bool has_mfma = R.wt_d != nullptr;
     const int nvar = has_mfma ? 4 : 3;
+    const int orm = out_rowmajor ? 1 : 0;
+    const int64_t key = 2 * ncol + orm;   // the kept variant per column count and output layout
     auto launch = [&](int variant) {
         auto go = [&](auto kern) {
             set_shmem((const void *)kern, sh);
-            hipLaunchKernelGGL(kern, grid, dim3(256), sh, s, R.args, XF, OUT, ncol);
+            hipLaunchKernelGGL(kern, grid, dim3(256), sh, s, R.args, XF, OUT, ncol, orm);
         };
-        if (variant == 3) launch_contract_mfma(R, XF, OUT, ncol, s);
+        if (variant == 3) launch_contract_mfma(R, XF, OUT, ncol, orm, s);
         else if (variant == 0) go(k_contract<CC, 8, 1>);
         else if (variant == 1) go(k_contract<CC, 4, 1>);
         else go(k_contract<CC, 6, 6>);
@@ -1555,7 +1576,7 @@ void ring_contract(const RingPlan &R, const double2 *XF, double2 *OUT, int64_t n
     {
         std::lock_guard<std::mutex> lk(R.tune->mu);
         for (const auto &c : R.tune->choice)
-            if (c.first == ncol) variant = c.second;
+            if (c.first == key) variant = c.second;
     }
     if (variant >= 0) {
         launch(variant);
@@ -1594,7 +1615,7 @@ void ring_contract(const RingPlan &R, const double2 *XF, double2 *OUT, int64_t n
     for (auto &e : ev) (void)hipEventDestroy(e);
     launch(best);
     std::lock_guard<std::mutex> lk(R.tune->mu);
-    R.tune->choice.emplace_back(ncol, best);
+    R.tune->choice.emplace_back(key, best);
 }
 
 void ring_prep(const GeomArgs &G, int dtype, const void *u, const void *v, const void *w, int nf, int withmask,
@@ -1611,9 +1632,8 @@ void ring_prep(const GeomArgs &G, int dtype, const void *u, const void *v, const
 // The fused inverse + epilogue where the row length has a compiled-in factorization that one block's
 // sequences fit (1440 points: one row per block, 6 points per lane; 256 points: 4 rows, 4 per lane);
 // false when the two separate kernels must serve.
-bool ring_inverse_epilogue(const RingPlan &R, const GeomArgs &G, int dtype, const double2 *OUT, const double *D_row,
-                           int nf, int withmask, int strategy, double *ubar, void *tau, void *uo, void *vo, void *wo,
-                           int64_t nbatch, bool spectral_rotation, hipStream_t s) {
+bool ring_inverse_epilogue_serves(const RingPlan &R, const GeomArgs &G, int nf, int withmask, int strategy,
+                                  int64_t nbatch) {
     const RingFft &F = R.fft;
     if (F.nseq == 0 || (F.fixed != RING_FIXED_1440 && F.fixed != RING_FIXED_256)) return false;
     if (withmask && strategy != CGEF_DEFORMABLE) return false;
@@ -1622,6 +1642,23 @@ bool ring_inverse_epilogue(const RingPlan &R, const GeomArgs &G, int dtype, cons
     if (F.fixed == RING_FIXED_1440 && mode != 1) return false;   // 1440 points: the 0.25° longitude
     const int64_t nrows = G.n[1] * G.n[2];
     if (nbatch * nf * nrows >= ((int64_t)1 << 31) / R.mh) return false;   // 32-bit mode offsets
+    const bool big = F.fixed == RING_FIXED_1440;
+    const int nsq = big ? IfeShape<RING_FIXED_1440>::NSQ : IfeShape<RING_FIXED_256>::NSQ;
+    const int npr = big ? IfeShape<RING_FIXED_1440>::NPR : IfeShape<RING_FIXED_256>::NPR;
+    const int blk = big ? IfeShape<RING_FIXED_1440>::BLK : IfeShape<RING_FIXED_256>::BLK;
+    for (int st = 0; st < F.snst; st++)   // one butterfly per lane in every stage, all of a round's sequences
+        if (F.n / F.srad[st] * nsq * npr > blk) return false;
+    return true;
+}
+
+void ring_inverse_epilogue(const RingPlan &R, const GeomArgs &G, int dtype, const double2 *OUT, const double *D_row,
+                           int nf, int withmask, int strategy, double *ubar, void *tau, void *uo, void *vo, void *wo,
+                           int64_t nbatch, bool spectral_rotation, hipStream_t s) {
+    if (!ring_inverse_epilogue_serves(R, G, nf, withmask, strategy, nbatch))
+        fail(CGEF_ERR_UNSUPPORTED, "the fused inverse transform + epilogue does not serve this plan");
+    const RingFft &F = R.fft;
+    const int mode = G.mode;
+    const int64_t nrows = G.n[1] * G.n[2];
     const bool big = F.fixed == RING_FIXED_1440;
     if (spectral_rotation && big && !wo) {
         // the sphere's local frame formed in Fourier space: three sequences per row in one round
@@ -1644,14 +1681,12 @@ bool ring_inverse_epilogue(const RingPlan &R, const GeomArgs &G, int dtype, cons
                 if (withmask) go(k_ifft_epilogue_sphere<double, 1>, (double *)nullptr);
                 else go(k_ifft_epilogue_sphere<double, 0>, (double *)nullptr);
             }
-            return true;
+            return;
         }
     }
     const int nsq = big ? IfeShape<RING_FIXED_1440>::NSQ : IfeShape<RING_FIXED_256>::NSQ;
     const int npr = big ? IfeShape<RING_FIXED_1440>::NPR : IfeShape<RING_FIXED_256>::NPR;
     const int blk = big ? IfeShape<RING_FIXED_1440>::BLK : IfeShape<RING_FIXED_256>::BLK;
-    for (int st = 0; st < F.snst; st++)   // one butterfly per lane in every stage, all of a round's sequences
-        if (F.n / F.srad[st] * nsq * npr > blk) return false;
     const int64_t items = (nrows + nsq - 1) / nsq * nbatch;
     const dim3 grid((unsigned)(8 * ((items + 7) / 8)));
     const size_t sh = (size_t)nsq * npr * F.sstr * sizeof(double2);
@@ -1676,7 +1711,6 @@ bool ring_inverse_epilogue(const RingPlan &R, const GeomArgs &G, int dtype, cons
         else { CGEF_IFE(double, RING_FIXED_256, 2) }
     }
 #undef CGEF_IFE
-    return true;
 }
 
 void ring_epilogue(const GeomArgs &G, int dtype, const double *yout, const double *D_row, int nf, int withmask,

@@ -35,7 +35,9 @@ void free_ring(RingPlan &R);
 void ring_build_what(const RingPlan &R, const Run *runs_d, const int32_t *run_off_d, const double *w64_d, int64_t nruns,
                      hipStream_t s);
 void ring_forward(const RingPlan &R, const double *xin, double2 *XF, int64_t ncol, hipStream_t s);
-void ring_contract(const RingPlan &R, const double2 *XF, double2 *OUT, int64_t ncol, hipStream_t s);
+// out_rowmajor: OUT as [m][row][col] (what the fused inverse + epilogue reads) instead of [m][col][row]
+void ring_contract(const RingPlan &R, const double2 *XF, double2 *OUT, int64_t ncol, hipStream_t s,
+                   bool out_rowmajor = false);
 // fills R.wt_d (allocated by the caller: mh × wt_tiles × wt_steps × 64 doubles) from Ŵ
 void ring_build_wtiles(RingPlan &R, hipStream_t s);
 void ring_inverse(const RingPlan &R, const double2 *OUT, double *yout, int64_t ncol, hipStream_t s);
@@ -43,11 +45,15 @@ void ring_inverse(const RingPlan &R, const double2 *OUT, double *yout, int64_t n
 // `withmask`); the epilogue turns the filtered planes into ū (FP64) and τ, optionally exporting ū in T.
 void ring_prep(const GeomArgs &G, int dtype, const void *u, const void *v, const void *w, int nf, int withmask,
                double *xin, int64_t nbatch, hipStream_t s);
-// inverse row transforms of the contracted spectra fused with the epilogue (false: not served for this
-// length or plane set; use ring_inverse + ring_epilogue). spectral_rotation: on the 1440-point sphere
-// without a radial export, rotate to the local frame in Fourier space (the caller has checked that
-// longitude ascends by 2π/N from trig[0]), else per point after the transforms.
-bool ring_inverse_epilogue(const RingPlan &R, const GeomArgs &G, int dtype, const double2 *OUT, const double *D_row,
+// Whether the inverse row transforms of the Π planes can run fused with the epilogue for this plan,
+// plane set and batch (row lengths with a compiled-in factorization); if not, ring_inverse +
+// ring_epilogue. The fused kernels read the contracted spectra as [m][row][col] (ring_contract with
+// out_rowmajor). spectral_rotation: on the 1440-point sphere without a radial export, rotate to the
+// local frame in Fourier space (the caller has checked that longitude ascends by 2π/N from trig[0]),
+// else per point after the transforms.
+bool ring_inverse_epilogue_serves(const RingPlan &R, const GeomArgs &G, int nf, int withmask, int strategy,
+                                  int64_t nbatch);
+void ring_inverse_epilogue(const RingPlan &R, const GeomArgs &G, int dtype, const double2 *OUT, const double *D_row,
                            int nf, int withmask, int strategy, double *ubar, void *tau, void *uo, void *vo, void *wo,
                            int64_t nbatch, bool spectral_rotation, hipStream_t s);
 void ring_epilogue(const GeomArgs &G, int dtype, const double *yout, const double *D_row, int nf, int withmask,
