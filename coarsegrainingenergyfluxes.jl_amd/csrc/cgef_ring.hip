@@ -1153,6 +1153,13 @@ __global__ __launch_bounds__(512) void k_ifft_epilogue_sphere(RingFft F, const d
     const int64_t j = row % Ny;
     const double sp = G.trig[2 * N + j], cp = G.trig[2 * N + Ny + j];
     const double s0 = G.trig[0], c0 = G.trig[N];             // w = e^{iλ₀}
+    const double drow = WM ? 0.0 : D_row[row];               // what the epilogue reads, requested early
+    bool act[PPT];
+#pragma unroll
+    for (int p = 0; p < PPT; p++) {
+        const int i = (int)threadIdx.x + p * BLK;
+        act[p] = i < N && G.mask[row * N + i] != 0;
+    }
     const double2 w1 = make_double2(c0, s0), w2 = make_double2(c0 * c0 - s0 * s0, 2.0 * s0 * c0);
     double2 *pl = sm;                                        // staged spectra [plane][MH]
     auto at = [&](int k, int m) {                            // plane k at any mode in −2..MH+1
@@ -1162,12 +1169,35 @@ __global__ __launch_bounds__(512) void k_ifft_epilogue_sphere(RingFft F, const d
     };
     auto mi = [](double2 a) { return make_double2(-a.y, a.x); };                   // i·a
     auto half = [](double2 a) { return make_double2(0.5 * a.x, 0.5 * a.y); };
-    // the products xx xy xz yy yz zz (planes 3..8: 96 contiguous bytes per mode)
-    for (int e = threadIdx.x; e < 6 * MH; e += BLK) {
-        const int m = e / 6, k = e - 6 * m;
-        pl[k * MH + m] = spec[(int64_t)m * mstride + 3 + k];
+    // the products xx xy xz yy yz zz (planes 3..8: 96 contiguous bytes per mode), every load of the
+    // lane issued before the first LDS store
+    {
+        constexpr int NA = (6 * MH + BLK - 1) / BLK;
+        double2 v[NA];
+#pragma unroll
+        for (int q = 0; q < NA; q++) {
+            const int e = (int)threadIdx.x + q * BLK;
+            const int m = e / 6, k = e - 6 * m;
+            v[q] = e < 6 * MH ? spec[(int64_t)m * mstride + 3 + k] : make_double2(0.0, 0.0);
+        }
+#pragma unroll
+        for (int q = 0; q < NA; q++) {
+            const int e = (int)threadIdx.x + q * BLK;
+            const int m = e / 6, k = e - 6 * m;
+            if (e < 6 * MH) pl[k * MH + m] = v[q];
+        }
     }
     __syncthreads();
+    // the components U0 U1 U2 (planes 0..2) and, for Deformable, the mask (plane 9): requested now, so
+    // that their latency runs under the products' mixing; they reach LDS once the products are read
+    constexpr int NC4 = 3 + WM, NB = (NC4 * MH + BLK - 1) / BLK;
+    double2 cv[NB];
+#pragma unroll
+    for (int q = 0; q < NB; q++) {
+        const int e = (int)threadIdx.x + q * BLK;
+        const int m = e / NC4, k = e - NC4 * m;
+        cv[q] = e < NC4 * MH ? spec[(int64_t)m * mstride + (k < 3 ? k : 9)] : make_double2(0.0, 0.0);
+    }
     double2 Tee[MPT], Ten[MPT], Tnn[MPT];
 #pragma unroll
     for (int q = 0; q < MPT; q++) {
@@ -1192,10 +1222,11 @@ __global__ __launch_bounds__(512) void k_ifft_epilogue_sphere(RingFft F, const d
                               sp * sp * (D1.y + Q.y) - 2.0 * sp * cp * A1.y + cp * cp * pzz.y);
     }
     __syncthreads();
-    // the components U0 U1 U2 (planes 0..2) and, for Deformable, the mask (plane 9)
-    for (int e = threadIdx.x; e < (3 + WM) * MH; e += BLK) {
-        const int m = e / (3 + WM), k = e - (3 + WM) * m;
-        pl[k * MH + m] = spec[(int64_t)m * mstride + (k < 3 ? k : 9)];
+#pragma unroll
+    for (int q = 0; q < NB; q++) {
+        const int e = (int)threadIdx.x + q * BLK;
+        const int m = e / NC4, k = e - NC4 * m;
+        if (e < NC4 * MH) pl[k * MH + m] = cv[q];
     }
     __syncthreads();
     double2 Ev[MPT], Nv[MPT], Mv[MPT];
@@ -1235,8 +1266,8 @@ __global__ __launch_bounds__(512) void k_ifft_epilogue_sphere(RingFft F, const d
         if (i >= N) continue;
         const double2 z0 = sm[lp(i, sh)], z1 = sm[sstr + lp(i, sh)], z2 = sm[2 * sstr + lp(i, sh)];
         const int64_t t = row * N + i;
-        const double den = WM ? z2.y * sc : D_row[row];
-        const bool ok = G.mask[t] != 0 && den > 1e-15;
+        const double den = WM ? z2.y * sc : drow;
+        const bool ok = act[p] && den > 1e-15;
         const double inv = ok ? sc / den : 0.0;
         const double o0 = z0.x * inv, o1 = z0.y * inv;
         tb[t] = (T)(z1.x * inv - o0 * o0);
