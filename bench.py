@@ -80,6 +80,11 @@ def host_cpu():
 
 
 def timed_steps(step, steps, warmup, plan=None):
+    """Warm-up steps, then `steps` timed between device synchronizations. Python's cyclic garbage
+    collector is run before and held off during the timed steps, as timeit does: a full collection of
+    this process's objects (torch's included) takes about 40 ms and otherwise lands in a random step."""
+    import gc
+
     import torch
     import cgef
     for _ in range(warmup):
@@ -88,12 +93,19 @@ def timed_steps(step, steps, warmup, plan=None):
     L = cgef._lib.load()
     if plan is not None:
         cgef._lib.check(L.cgef_profile_enable(plan.h, 1))
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        step()
-    torch.cuda.synchronize()
-    el = time.perf_counter() - t0
+    gc.collect()
+    gc_was_on = gc.isenabled()
+    gc.disable()
+    try:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step()
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+    finally:
+        if gc_was_on:
+            gc.enable()
     kern_ms = None
     if plan is not None:
         kms, nl = C.c_double(), C.c_int64()

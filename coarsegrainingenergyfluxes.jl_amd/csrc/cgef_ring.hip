@@ -974,7 +974,7 @@ __global__ __launch_bounds__(IfeShape<FIX>::BLK) void k_ifft_epilogue(RingFft F,
     const int64_t ngrp = (nrows + NSQ - 1) / NSQ;
     int64_t item;
     if (!xcd_item(ngrp * nbatch, item)) return;
-    const int64_t b = item / ngrp, r0 = (item % ngrp) * NSQ;
+    const int64_t b = item % nbatch, r0 = (item / nbatch) * NSQ;   // snapshot fastest: rows' planes share lines
     const int64_t ncolt = nbatch * NF;                       // columns of the [m][row][col] spectra
     const int mstride = (int)(ncolt * nrows);                // one mode (< 2^31)
     const double2 *spec = OUT + r0 * ncolt + b * NF;         // row r0, plane 0, mode 0
@@ -1147,7 +1147,9 @@ __global__ __launch_bounds__(512) void k_ifft_epilogue_sphere(RingFft F, const d
     const int64_t nrows = G.n[1] * G.n[2], P = G.npts, Ny = G.n[1];
     int64_t item;
     if (!xcd_item(nrows * nbatch, item)) return;
-    const int64_t b = item / nrows, row = item % nrows;
+    // snapshot fastest: the blocks of one row's snapshots run together on one XCD and share the cache
+    // lines their adjacent 144-byte plane groups straddle
+    const int64_t row = item / nbatch, b = item % nbatch;
     const int mstride = (int)(nbatch * NF * nrows);          // one mode of the [m][row][col] spectra (< 2^31)
     const double2 *spec = OUT + row * (nbatch * NF) + b * NF; // plane f, mode m: spec[m·mstride + f]
     const int64_t j = row % Ny;
