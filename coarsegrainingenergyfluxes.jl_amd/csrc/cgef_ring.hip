@@ -1118,7 +1118,7 @@ __global__ __launch_bounds__(IfeShape<FIX>::BLK) void k_ifft_epilogue(RingFft F,
 }
 
 // The sphere's fused inverse transform + epilogue with the rotation to the local frame done in Fourier
-// space (1440-point rows, no radial export). The east/north frame weights are trigonometric
+// space (1440-point rows). The east/north frame weights are trigonometric
 // polynomials in λ of degree ≤ 2: with λ_k = λ₀ + 2πk/N, multiplying a row by cos λ or sin λ mixes
 // modes m ∓ 1 with phase w = e^{iλ₀}, by cos 2λ or sin 2λ modes m ∓ 2 with phase w². So per mode the
 // block forms, from the nine planetary planes, the spectra of the five local numerators
@@ -1129,14 +1129,18 @@ __global__ __launch_bounds__(IfeShape<FIX>::BLK) void k_ifft_epilogue(RingFft F,
 // R = sin 2λ·D2 − cos 2λ·Pxy (e_east = (−s, c, 0), e_north = (−sinφ c, −sinφ s, cosφ); Diagnostics.jl:
 // 596-650) — and inverse-transforms only those (E + iN, Tee + iTen, Tnn + i·mask): three sequences in
 // one round instead of five pairs in two. Modes beyond 0..N/2 come from X[−m] = X[N−m] = conj X[m].
-// The products and the components are staged through the LDS that the transforms then use.
-template <typename T, int WM>
+// The products and the components are staged through the LDS that the transforms then use. RAD (no
+// mask plane): the radial component cosφ (c·U0 + s·U1) + sinφ·U2 rides in the third sequence's
+// imaginary part and is exported through wo.
+template <typename T, int WM, int RAD>
 __global__ __launch_bounds__(512) void k_ifft_epilogue_sphere(RingFft F, const double2 *__restrict__ tw,
                                                              const double2 *__restrict__ tws,
                                                              const double2 *__restrict__ OUT, GeomArgs G,
                                                              const double *__restrict__ D_row, int64_t nbatch,
                                                              double *__restrict__ ubar, T *__restrict__ tau,
-                                                             T *__restrict__ uo, T *__restrict__ vo) {
+                                                             T *__restrict__ uo, T *__restrict__ vo,
+                                                             T *__restrict__ wo) {
+    static_assert(!(WM && RAD), "the mask plane and the radial component share one sequence slot");
     constexpr int N = 1440, MH = N / 2 + 1, BLK = 512;
     constexpr int PPT = (N + BLK - 1) / BLK;                 // points per lane
     constexpr int MPT = (MH + BLK - 1) / BLK;                // modes per lane
@@ -1242,7 +1246,9 @@ __global__ __launch_bounds__(512) void k_ifft_epilogue_sphere(RingFft F, const d
         const double2 cs = half(cadd(cmul(w1, csub(u0l, mi(u1l))), cmul(cconj(w1), cadd(u0h, mi(u1h)))));
         const double2 u2 = at(2, m);
         Nv[q] = make_double2(cp * u2.x - sp * cs.x, cp * u2.y - sp * cs.y);
-        Mv[q] = WM ? pl[3 * MH + m] : make_double2(0.0, 0.0);
+        if (WM) Mv[q] = pl[3 * MH + m];
+        else if (RAD) Mv[q] = make_double2(cp * cs.x + sp * u2.x, cp * cs.y + sp * u2.y);
+        else Mv[q] = make_double2(0.0, 0.0);
     }
     __syncthreads();   // the staged spectra are read: the sequences may overwrite them
 #pragma unroll
@@ -1279,6 +1285,7 @@ __global__ __launch_bounds__(512) void k_ifft_epilogue_sphere(RingFft F, const d
         ub[P + t] = o1;
         if (uo) uo[b * P + t] = (T)o0;
         if (vo) vo[b * P + t] = (T)o1;
+        if (RAD) wo[b * P + t] = (T)(z2.y * inv);
     }
 }
 
@@ -1693,7 +1700,7 @@ void ring_inverse_epilogue(const RingPlan &R, const GeomArgs &G, int dtype, cons
     const int mode = G.mode;
     const int64_t nrows = G.n[1] * G.n[2];
     const bool big = F.fixed == RING_FIXED_1440;
-    if (spectral_rotation && big && !wo) {
+    if (spectral_rotation && big && !(wo && withmask)) {
         // the sphere's local frame formed in Fourier space: three sequences per row in one round
         bool fits = (size_t)6 * R.mh <= (size_t)3 * F.sstr;
         for (int st = 0; st < F.snst; st++) fits = fits && F.n / F.srad[st] * 3 <= 512;
@@ -1705,14 +1712,16 @@ void ring_inverse_epilogue(const RingPlan &R, const GeomArgs &G, int dtype, cons
                 using TT = std::remove_pointer_t<decltype(tp)>;
                 set_shmem((const void *)kern, sh);
                 hipLaunchKernelGGL(kern, grid, dim3(512), sh, s, F, R.tw_d, R.tws_d, OUT, G, D_row, nbatch, ubar,
-                                   (TT *)tau, (TT *)uo, (TT *)vo);
+                                   (TT *)tau, (TT *)uo, (TT *)vo, (TT *)wo);
             };
             if (dtype == CGEF_F32) {
-                if (withmask) go(k_ifft_epilogue_sphere<float, 1>, (float *)nullptr);
-                else go(k_ifft_epilogue_sphere<float, 0>, (float *)nullptr);
+                if (withmask) go(k_ifft_epilogue_sphere<float, 1, 0>, (float *)nullptr);
+                else if (wo) go(k_ifft_epilogue_sphere<float, 0, 1>, (float *)nullptr);
+                else go(k_ifft_epilogue_sphere<float, 0, 0>, (float *)nullptr);
             } else {
-                if (withmask) go(k_ifft_epilogue_sphere<double, 1>, (double *)nullptr);
-                else go(k_ifft_epilogue_sphere<double, 0>, (double *)nullptr);
+                if (withmask) go(k_ifft_epilogue_sphere<double, 1, 0>, (double *)nullptr);
+                else if (wo) go(k_ifft_epilogue_sphere<double, 0, 1>, (double *)nullptr);
+                else go(k_ifft_epilogue_sphere<double, 0, 0>, (double *)nullptr);
             }
             return;
         }

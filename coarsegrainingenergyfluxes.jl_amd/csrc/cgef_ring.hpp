@@ -48,9 +48,9 @@ void ring_prep(const GeomArgs &G, int dtype, const void *u, const void *v, const
 // Whether the inverse row transforms of the Π planes can run fused with the epilogue for this plan,
 // plane set and batch (row lengths with a compiled-in factorization); if not, ring_inverse +
 // ring_epilogue. The fused kernels read the contracted spectra as [m][row][col] (ring_contract with
-// out_rowmajor). spectral_rotation: on the 1440-point sphere without a radial export, rotate to the
-// local frame in Fourier space (the caller has checked that longitude ascends by 2π/N from trig[0]),
-// else per point after the transforms.
+// out_rowmajor). spectral_rotation: on the 1440-point sphere (except Deformable with a radial
+// export), rotate to the local frame in Fourier space (the caller has checked that longitude ascends
+// by 2π/N from trig[0]), else per point after the transforms.
 bool ring_inverse_epilogue_serves(const RingPlan &R, const GeomArgs &G, int nf, int withmask, int strategy,
                                   int64_t nbatch);
 void ring_inverse_epilogue(const RingPlan &R, const GeomArgs &G, int dtype, const double2 *OUT, const double *D_row,
