@@ -178,6 +178,16 @@ def sweep_headline(args, grid, mask, lam, phi, cpu):
         f"2000 km contraction {kern_ms:.3f} ms")
     names = {0: "k_contract<5,8,1>", 1: "k_contract<5,4,1>", 2: "k_contract<5,6,6>", 3: "k_contract_mfma", -1: None}
     variants = {f"{s / 1e3:.0f}km": names[int(p.query().ring_contract_kernel)] for s, p in zip(scales, plans)}
+    kept = {f"{s / 1e3:.0f}km": round(float(p.query().ring_modes_kept), 4) for s, p in zip(scales, plans)}
+    # the same sweep contracting every Fourier mode (the exact operator FP64 calls run), for comparison
+    os.environ["CGEF_MODE_CUT"] = "0"
+    try:
+        reps = max(3, args.steps // 2)
+        ela, _ = timed_steps(call, reps, 1)
+    finally:
+        os.environ.pop("CGEF_MODE_CUT", None)
+    ms_all_modes = ela * 1e3 / reps
+    log(f"sweep: {ms_all_modes:.2f} ms per call contracting every mode (no mode cut)")
     # the same sweep replayed as one HIP graph (opt-in), timed in this process
     ms_graph = None
     if not args.no_graph:
@@ -242,6 +252,7 @@ def sweep_headline(args, grid, mask, lam, phi, cpu):
     out = {"value": value if parity_ok else None, "ms_per_step": ms, "ms_per_step_graph": ms_graph,
            "parity_ok": parity_ok, "worst_rel_l2": worst, "parity": checks, "roofline": roof,
            "step_compulsory_hbm": step, "contract_kernel_per_scale": variants, "plan_build_s": tplan,
+           "modes_kept_per_scale": kept, "ms_per_step_all_modes": ms_all_modes,
            "engine": plans[0].engine, "direct_sum_equivalent_tflops": direct_flops / (ms * 1e-3) / 1e12,
            "taps_per_step": sum(taps) * NSNAP}
 
@@ -842,8 +853,9 @@ def main():
         "roofline": head["roofline"],
         "cpu_baseline": cpu_base,
     }
-    for k in ("ms_per_step_graph", "parity_ok", "worst_rel_l2", "parity", "step_compulsory_hbm",
-              "contract_kernel_per_scale", "plan_build_s", "direct_sum_equivalent_tflops"):
+    for k in ("ms_per_step_graph", "ms_per_step_all_modes", "modes_kept_per_scale", "parity_ok", "worst_rel_l2",
+              "parity", "step_compulsory_hbm", "contract_kernel_per_scale", "plan_build_s",
+              "direct_sum_equivalent_tflops"):
         out[k] = head[k]
     out.update(ratios)
     out.update(extra)

@@ -79,6 +79,7 @@ class PlanInfo(C.Structure):
         ("ring_modes", C.c_int64),
         ("ring_pi_fields", C.c_int64),
         ("ring_contract_kernel", C.c_int64),
+        ("ring_modes_kept", C.c_double),
     ]
 
 

@@ -404,13 +404,15 @@ static void ring_reserve(Workspace &w, const Plan &p, int64_t ncol) {
     }
 }
 
-// forward transform of w.rxy (ncol planes), banded contraction, inverse back into w.rxy
-static void ring_filter_planes(Plan &p, Workspace &w, int64_t ncol, hipStream_t s) {
+// forward transform of w.rxy (ncol planes), banded contraction, inverse back into w.rxy (FP32 calls
+// under the plan's mode cut)
+static void ring_filter_planes(Plan &p, Workspace &w, int64_t ncol, int dtype, hipStream_t s) {
+    const int32_t *mcut = ring_mode_cut(p.ring, dtype);
     ring_forward(p.ring, w.rxy, w.rxf, ncol, s);
     p.prof_begin(s);
-    ring_contract(p.ring, w.rxf, w.rout, ncol, s);
+    ring_contract(p.ring, w.rxf, w.rout, ncol, s, false, mcut);
     p.prof_end(s);
-    ring_inverse(p.ring, w.rout, w.rxy, ncol, s);
+    ring_inverse(p.ring, w.rout, w.rxy, ncol, s, mcut);
 }
 
 // Longitude ascends from coord[0][0] by 2π/N (to 1e-9 of a step): the condition under which the
@@ -438,14 +440,15 @@ static bool ring_pi_fused(const Plan &p, const GeomArgs &G, int nf, int wm, int6
 // it can; CGEF_FUSE=1 keeps the rotation per point (testing).
 static void ring_pi_inverse_epilogue(Plan &p, Workspace &w, const GeomArgs &G, int dtype, int nf, int wm, void *uo,
                                      void *vo, void *wo, int64_t nbatch, bool fused, hipStream_t s) {
+    const int32_t *mcut = ring_mode_cut(p.ring, dtype);   // what the contraction ran under
     if (fused) {
         const char *fe = std::getenv("CGEF_FUSE");
         const bool spectral = !(fe && fe[0] == '1') && lon_ascends_by_ring_step(*p.grid);
         ring_inverse_epilogue(p.ring, G, dtype, w.rout, p.D64_row_d, nf, wm, p.strategy, w.ubar, w.tau, uo, vo, wo,
-                              nbatch, spectral, s);
+                              nbatch, spectral, s, mcut);
         return;
     }
-    ring_inverse(p.ring, w.rout, w.rxy, nf * nbatch, s);
+    ring_inverse(p.ring, w.rout, w.rxy, nf * nbatch, s, mcut);
     ring_epilogue(G, dtype, w.rxy, p.D64_row_d, nf, wm, p.strategy, w.ubar, w.tau, uo, vo, wo, nbatch, s);
 }
 
@@ -482,7 +485,7 @@ static void ring_filter_apply(Plan &p, Workspace &w, int dtype, const void *in, 
     const int64_t ncol = nsl + withmask;
     ring_reserve(w, p, ncol);
     ring_prep_fields(G, dtype, in, nsl, withmask, w.rxy, s);
-    ring_filter_planes(p, w, ncol, s);
+    ring_filter_planes(p, w, ncol, dtype, s);
     ring_normalize(G, dtype, w.rxy, p.D64_row_d, nsl, withmask, p.strategy, out, s);
 }
 
@@ -767,6 +770,7 @@ cgef_status cgef_plan_query(const cgef_plan *plan, cgef_plan_info *info) {
         info->radius = p->rad;
         info->ring_offsets = p->ring.ok ? p->ring.args.noff : 0;
         info->ring_modes = p->ring.ok ? p->ring.mh : 0;
+        info->ring_modes_kept = p->ring.ok && !p->ring.spectral ? p->ring.modes_kept : 0.0;
         info->ring_pi_fields = !p->ring.ok ? 0 : p->ring.spectral ? 5 : ring_nfields(*p->grid, ring_withmask(*p->grid, p->strategy));
         info->ring_contract_kernel = -1;
         if (p->ring.ok && !p->ring.spectral) {
@@ -870,7 +874,7 @@ cgef_status cgef_compute_pi(cgef_plan *plan, cgef_workspace *ws, void *Pi, const
             ring_forward(p->ring, W.rxy, W.rxf, nf * nbatch, s);
             const bool fused = ring_pi_fused(*p, G, nf, wm, nbatch);
             p->prof_begin(s);
-            ring_contract(p->ring, W.rxf, W.rout, nf * nbatch, s, fused);
+            ring_contract(p->ring, W.rxf, W.rout, nf * nbatch, s, fused, ring_mode_cut(p->ring, dtype));
             p->prof_end(s);
             ring_pi_inverse_epilogue(*p, W, G, dtype, nf, wm, ubar, vbar, wbar, nbatch, fused, s);
             launch_strain_pi(G, dtype, W.ubar, W.tau, Pi, rho0, nbatch, W.partials, W.npartials, s);
@@ -957,7 +961,7 @@ cgef_status cgef_coarse_grain(cgef_plan *const *plans, int64_t nscales, cgef_wor
                         Plan &p = *reinterpret_cast<Plan *>(plans[q]);
                         const bool fused = ring_pi_fused(p, G, nf, wm, nbatch);
                         p.prof_begin(st);
-                        ring_contract(p.ring, W.rxf, W.rout, nf * nbatch, st, fused);
+                        ring_contract(p.ring, W.rxf, W.rout, nf * nbatch, st, fused, ring_mode_cut(p.ring, dtype));
                         p.prof_end(st);
                         ring_pi_inverse_epilogue(p, W, G, dtype, nf, wm, nullptr, nullptr, nullptr, nbatch, fused, st);
                         launch_strain_pi(G, dtype, W.ubar, W.tau, (char *)Pi + (size_t)q * P * es, rho0, nbatch,

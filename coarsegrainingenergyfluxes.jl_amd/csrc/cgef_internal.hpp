@@ -87,6 +87,23 @@ constexpr double RING_WHAT_BUDGET_BYTES = 32.0 * 1024 * 1024 * 1024;
 constexpr int RING_MFMA_MIN_OFFSETS = 48;
 // LDS per CU on gfx950, which is also the most one workgroup may take.
 constexpr size_t LDS_BYTES_PER_CU = 160 * 1024;
+// Fourier-mode cut of the ring engine for FP32 calls: target row r keeps the modes m < M(r), the
+// smallest M for which both tails below hold, with S(m) = Σ_offsets |Ŵ_band(r)(m)|, S0 the row's
+// in-range weight sum Σ_offsets Ŵ_band(r)(0) and c_m = 2 (1 for modes 0 and N/2):
+//   values:  Σ_{m≥M} c_m·S(m)                                ≤ RING_MODE_TOL · S0,
+//   slopes:  Σ_{m≥M} c_m·S(m)·max(1, 4ℓ·sin(πm/N)/Δx_r)      ≤ RING_MODE_SLOPE_TOL · S0.
+// Each |DFT coefficient| of a row is at most N·max|q|, and every x-difference stencil of the strain
+// (centred, one-sided two- and three-node) multiplies mode m by at most 4·sin(πm/N)/Δx. So under
+// ZeroFill, whose denominator is S0, every filtered value moves by at most RING_MODE_TOL·max|q| (the
+// size of the FP32 output's rounding) and every x-difference of one by at most
+// RING_MODE_SLOPE_TOL·max|q|/ℓ (Deformable: both times S0 / filtered mask). The slope condition is what
+// binds near the poles, where Δx = R·cosφ·Δλ is tiny against ℓ. The bounds sit above the floor that
+// the footprint's truncation at G = 1e-10 leaves in every mode (its edge step puts ~1e-12 of S0 into
+// each |Ŵ(m)|; summed over modes about 1e-8 for values, 1e-6 for slopes on the 0.25° sphere), so
+// the cut follows the kernel's own spectral decay (ℓ = 2000 km on that sphere: 5 modes at the poles, 55 at
+// the equator, of 721).
+constexpr double RING_MODE_TOL = 3e-8;
+constexpr double RING_MODE_SLOPE_TOL = 3e-6;
 
 struct RingArgs {
     int64_t ny = 1, nz = 1, nrows = 1;
@@ -105,7 +122,7 @@ struct RingArgs {
 // Measured choice of the banded-contraction kernel variant per column count (see ring_contract).
 struct ContractTune {
     std::mutex mu;
-    std::vector<std::pair<int64_t, int>> choice;   // (2·columns + output layout, variant)
+    std::vector<std::pair<int64_t, int>> choice;   // (4·columns + 2·mode cut + output layout, variant)
 };
 
 struct RingPlan {
@@ -132,6 +149,13 @@ struct RingPlan {
     // A grid with one band (translation invariant along y) stores one tile for all.
     double *wt_d = nullptr;
     int32_t wt_steps = 0, wt_tiles = 0;
+    // The mode cut (RING_MODE_TOL, RING_MODE_SLOPE_TOL), used by FP32 calls; mcut_d is null when every
+    // row keeps every mode. mcut_d: per row (Ny·Nz) the modes kept. citems_d: the (m, 256-row block of one level)
+    // pairs, m · (blocks) + block in ascending m, with m below the cut of some row of the block — the
+    // VALU contraction's work list; mitems_d the same over 64-row blocks for the matrix-core one.
+    int32_t *mcut_d = nullptr, *citems_d = nullptr, *mitems_d = nullptr;
+    int64_t ncitems = 0, nmitems = 0;
+    double modes_kept = 1.0;           // Σ_rows M(r) / (rows · mh)
     RingArgs args;
 };
 

@@ -303,9 +303,11 @@ __global__ __launch_bounds__(256) void k_fft_rows(RingFft F, const double2 *__re
     write_pair_spectra(Z, [](int, int e) { return e; }, n, 1, XF, col, r0, nrows, ncol);
 }
 
-// Inverse: OUT[m][col][row] (Hermitian halves) → yout[col][row][i], normalized by 1/N.
+// Inverse: OUT[m][col][row] (Hermitian halves) → yout[col][row][i], normalized by 1/N. With a mode cut
+// (mcut non-null) row r reads only its modes m < mcut[r]; the others are zero.
 __global__ __launch_bounds__(256) void k_ifft_rows(RingFft F, const double2 *__restrict__ tw, const double2 *__restrict__ OUT,
-                                                   double *__restrict__ yout, int64_t nrows, int64_t ncol) {
+                                                   double *__restrict__ yout, int64_t nrows, int64_t ncol,
+                                                   const int32_t *__restrict__ mcut) {
     extern __shared__ double2 sm[];
     const int n = F.n;
     const int64_t npair = (nrows + 1) / 2;
@@ -315,11 +317,12 @@ __global__ __launch_bounds__(256) void k_ifft_rows(RingFft F, const double2 *__r
     const int64_t r0 = 2 * (item % npair), r1 = r0 + 1;
     const bool has1 = r1 < nrows;
     const int mh = n / 2 + 1;
+    const int mc0 = mcut ? mcut[r0] : mh, mc1 = mcut && has1 ? mcut[r1] : mh;
     for (int m = threadIdx.x; m < n; m += blockDim.x) {
         const bool lo = m < mh;
         const int mm = lo ? m : n - m;
-        double2 X = OUT[spec_at(mm, col, r0, nrows, ncol)];
-        double2 Y = has1 ? OUT[spec_at(mm, col, r1, nrows, ncol)] : make_double2(0.0, 0.0);
+        double2 X = mm < mc0 ? OUT[spec_at(mm, col, r0, nrows, ncol)] : make_double2(0.0, 0.0);
+        double2 Y = has1 && mm < mc1 ? OUT[spec_at(mm, col, r1, nrows, ncol)] : make_double2(0.0, 0.0);
         if (!lo) {
             X = cconj(X);
             Y = cconj(Y);
@@ -496,11 +499,12 @@ __global__ __launch_bounds__(256) void k_fft_rows_staged(RingFft F, const double
 // Inverse, 2·F.nsq rows per block: OUT[m][col][row] (Hermitian halves) → yout[col][row][i] / N. Each
 // stored value is read once, lane per (mode, row) so that the block's rows of one mode are adjacent;
 // each lane pair exchanges its X, Y and writes X + iY at m and the conjugate mirror at N − m.
-// LPT·256 ≥ 2·nsq·(N/2 + 1).
+// LPT·256 ≥ 2·nsq·(N/2 + 1). With a mode cut (mcut non-null) row r reads only its modes m < mcut[r].
 template <int LPT, int FIX>
 __global__ __launch_bounds__(256) void k_ifft_rows_staged(RingFft F, const double2 *__restrict__ tw,
                                                           const double2 *__restrict__ tws, const double2 *__restrict__ OUT,
-                                                          double *__restrict__ yout, int64_t nrows, int64_t ncol) {
+                                                          double *__restrict__ yout, int64_t nrows, int64_t ncol,
+                                                          const int32_t *__restrict__ mcut) {
     extern __shared__ double2 sm[];
     const int n = F.n, nsq = F.nsq, sh = F.lpsh, sstr = F.sstr;
     const int rpb = 2 * nsq;
@@ -514,7 +518,8 @@ __global__ __launch_bounds__(256) void k_ifft_rows_staged(RingFft F, const doubl
     for (int q = 0; q < LPT; q++) {
         const int e = (int)threadIdx.x + q * 256;
         const int m = e / rpb, ro = e - m * rpb;
-        v_[q] = m < mh && r0 + ro < nrows ? OUT[spec_at(m, col, r0 + ro, nrows, ncol)] : make_double2(0.0, 0.0);
+        const bool in = m < mh && r0 + ro < nrows && (!mcut || m < mcut[r0 + ro]);
+        v_[q] = in ? OUT[spec_at(m, col, r0 + ro, nrows, ncol)] : make_double2(0.0, 0.0);
     }
 #pragma unroll
     for (int q = 0; q < LPT; q++) {
@@ -553,18 +558,21 @@ __global__ __launch_bounds__(256) void k_ifft_rows_staged(RingFft F, const doubl
 
 template <int CC, int PB, int MINW>
 __global__ __launch_bounds__(256, MINW) void k_contract(RingArgs A, const double2 *__restrict__ XF, double2 *__restrict__ OUT,
-                                                        int64_t ncol, int out_rowmajor) {
+                                                        int64_t ncol, int out_rowmajor, const int32_t *__restrict__ items,
+                                                        int64_t nitems) {
     extern __shared__ double2 st[];
     const int64_t Ny = A.ny, Nz = A.nz;
     const int64_t nyb = (Ny + 255) / 256;
     const int64_t ncc = (ncol + CC - 1) / CC;
     // items (m, row block, column chunk), column chunk fastest, in contiguous ranges per XCD: the
-    // chunks of one (m, row block) read the same Ŵ slab and so should share an L2
+    // chunks of one (m, row block) read the same Ŵ slab and so should share an L2. With a mode cut the
+    // (m, row block) pairs come from its work list (ascending m), the others are not launched.
     int64_t item;
-    if (!xcd_item((int64_t)A.mh * nyb * Nz * ncc, item)) return;
+    if (!xcd_item((items ? nitems : (int64_t)A.mh * nyb * Nz) * ncc, item)) return;
     const int64_t c0 = (item % ncc) * CC;
-    const int64_t rbk = (item / ncc) % (nyb * Nz);
-    const int64_t m = item / (ncc * nyb * Nz);
+    const int64_t pr = items ? (int64_t)items[item / ncc] : item / ncc;
+    const int64_t rbk = pr % (nyb * Nz);
+    const int64_t m = pr / (nyb * Nz);
     const int64_t k = rbk / nyb, j0 = (rbk % nyb) * 256;
     const int tid = threadIdx.x;
     const int64_t j = j0 + tid;
@@ -676,7 +684,8 @@ constexpr int MFMA_KC = 32;                   // source rows staged per pass (8 
 template <int NT>
 __global__ __launch_bounds__(256, 2) void k_contract_mfma(RingArgs A, const double *__restrict__ wt, int nsteps, int wtiles,
                                                        const double2 *__restrict__ XF, double2 *__restrict__ OUT,
-                                                       int64_t ncol, int out_rowmajor) {
+                                                       int64_t ncol, int out_rowmajor, const int32_t *__restrict__ items,
+                                                       int64_t nitems) {
     constexpr int LDW = NT * 16 + 2;          // row stride in doubles (≡ 4 banks mod 32: 16-byte stores spread)
     constexpr int BUF = MFMA_KC * LDW;        // one staging buffer
     static_assert(MFMA_KC == 32, "staging: a thread owns one row (tid % 32) and NT columns (tid / 32 + 8i)");
@@ -685,12 +694,14 @@ __global__ __launch_bounds__(256, 2) void k_contract_mfma(RingArgs A, const doub
     const int64_t nrb = (Ny + 63) / 64;
     const int64_t ncc = (2 * ncol + NT * 16 - 1) / (NT * 16);
     // items (m, row block, column chunk), column chunk fastest, contiguous ranges per XCD: the row
-    // blocks of one mode re-read overlapping source rows through the same L2
+    // blocks of one mode re-read overlapping source rows through the same L2. With a mode cut the
+    // (m, row block) pairs come from its work list.
     int64_t item;
-    if (!xcd_item((int64_t)A.mh * nrb * ncc, item)) return;
+    if (!xcd_item((items ? nitems : (int64_t)A.mh * nrb) * ncc, item)) return;
     const int64_t cc = item % ncc;
-    const int64_t rb = (item / ncc) % nrb;
-    const int64_t m = item / (ncc * nrb);
+    const int64_t pr = items ? (int64_t)items[item / ncc] : item / ncc;
+    const int64_t rb = pr % nrb;
+    const int64_t m = pr / nrb;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int64_t R0 = rb * 64;
     const int ntile = (int)((Ny - R0) < 64 ? (Ny - R0 + 15) / 16 : 4);   // tiles in this block
@@ -814,6 +825,48 @@ __global__ __launch_bounds__(256) void k_build_wtiles(RingArgs A, int nsteps, in
         if (o >= 0 && o < A.noff && r < A.ny) v = A.what[(m * A.noff + o) * A.nbands + A.band_of_row[r]];
         wt[e] = v;
     }
+}
+
+// The mode cut (RING_MODE_TOL, RING_MODE_SLOPE_TOL), first step: S[m][band] = Σ_offsets |Ŵ[m][offset][band]|,
+// lanes over consecutive bands (coalesced along the [m][offset][band] table).
+__global__ __launch_bounds__(256) void k_mode_sums(RingArgs A, double *__restrict__ S) {
+    const int64_t total = (int64_t)A.mh * A.nbands;
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (int64_t)gridDim.x * 256) {
+        const int64_t m = e / A.nbands, b = e - m * A.nbands;
+        const double *w = A.what + m * A.noff * A.nbands + b;
+        double acc = 0.0;
+        for (int o = 0; o < A.noff; o++) acc += fabs(w[(int64_t)o * A.nbands]);
+        S[e] = acc;
+    }
+}
+
+// Second step, a lane per band: the in-range weight sum S0 = Σ_offsets Ŵ[0][offset][band] and, from the
+// highest mode down, the value tail Σ c_m·S[m] and the slope tail Σ c_m·S[m]·max(1, 4ℓ·sin(πm/N)/Δx)
+// (c_m = 1 for modes 0 and N/2 of the full spectrum, else 2); mband = the smallest M at which both
+// tails are within their bounds (every mode when S0 is not positive). dxb: the band's smallest
+// x spacing.
+__global__ __launch_bounds__(256) void k_mode_cut(RingArgs A, int n, double tol, double slope_tol, double ell,
+                                                  const double *__restrict__ dxb, const double *__restrict__ S,
+                                                  int32_t *__restrict__ mband) {
+    const int64_t b = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (b >= A.nbands) return;
+    double s0 = 0.0;
+    for (int o = 0; o < A.noff; o++) s0 += A.what[(int64_t)o * A.nbands + b];
+    int M = (int)A.mh;
+    if (s0 > 0.0) {
+        const double g = 4.0 * ell / dxb[b];
+        double tv = 0.0, ts = 0.0;
+        int mv = -1, ms = -1;
+        for (int64_t m = A.mh - 1; m >= 0 && (mv < 0 || ms < 0); m--) {
+            const double c = (m == 0 || 2 * m == n ? 1.0 : 2.0) * S[m * A.nbands + b];
+            tv += c;
+            ts += c * fmax(1.0, g * sin(M_PI * (double)m / (double)n));
+            if (mv < 0 && tv > tol * s0) mv = (int)m + 1;
+            if (ms < 0 && ts > slope_tol * s0) ms = (int)m + 1;
+        }
+        M = mv > ms ? mv : ms;
+    }
+    mband[b] = M;
 }
 
 // Ŵ for every (band, offset): the run's weights laid out circularly on a length-N row, FFT'd two
@@ -957,7 +1010,7 @@ __global__ __launch_bounds__(IfeShape<FIX>::BLK) void k_ifft_epilogue(RingFft F,
                                                                       const double *__restrict__ D_row, int64_t nbatch,
                                                                       double *__restrict__ ubar, T *__restrict__ tau,
                                                                       T *__restrict__ uo, T *__restrict__ vo,
-                                                                      T *__restrict__ wo) {
+                                                                      T *__restrict__ wo, const int32_t *__restrict__ mcut) {
     using S = IfeShape<FIX>;
     constexpr int N = S::N, NSQ = S::NSQ, NPR = S::NPR, BLK = S::BLK;
     constexpr int MH = N / 2 + 1;
@@ -1003,7 +1056,7 @@ __global__ __launch_bounds__(IfeShape<FIX>::BLK) void k_ifft_epilogue(RingFft F,
             const int k = sq / NSQ, rw = sq - k * NSQ;
             const int fa = 2 * (rd * NPR + k), fb = fa + 1;
             double2 X = make_double2(0.0, 0.0), Y = X;
-            if (r0 + rw < nrows) {
+            if (r0 + rw < nrows && (!mcut || m < mcut[r0 + rw])) {
                 const double2 *x = spec + (int64_t)m * mstride + rw * ncolt;   // planes fa, fb adjacent
                 X = x[fa];
                 if (fb < NF) Y = x[fb];
@@ -1139,7 +1192,7 @@ __global__ __launch_bounds__(512) void k_ifft_epilogue_sphere(RingFft F, const d
                                                              const double *__restrict__ D_row, int64_t nbatch,
                                                              double *__restrict__ ubar, T *__restrict__ tau,
                                                              T *__restrict__ uo, T *__restrict__ vo,
-                                                             T *__restrict__ wo) {
+                                                             T *__restrict__ wo, const int32_t *__restrict__ mcut) {
     static_assert(!(WM && RAD), "the mask plane and the radial component share one sequence slot");
     constexpr int N = 1440, MH = N / 2 + 1, BLK = 512;
     constexpr int PPT = (N + BLK - 1) / BLK;                 // points per lane
@@ -1160,6 +1213,7 @@ __global__ __launch_bounds__(512) void k_ifft_epilogue_sphere(RingFft F, const d
     const double sp = G.trig[2 * N + j], cp = G.trig[2 * N + Ny + j];
     const double s0 = G.trig[0], c0 = G.trig[N];             // w = e^{iλ₀}
     const double drow = WM ? 0.0 : D_row[row];               // what the epilogue reads, requested early
+    const int mc = mcut ? mcut[row] : MH;                    // modes kept by the cut (the rest are zero)
     bool act[PPT];
 #pragma unroll
     for (int p = 0; p < PPT; p++) {
@@ -1184,7 +1238,7 @@ __global__ __launch_bounds__(512) void k_ifft_epilogue_sphere(RingFft F, const d
         for (int q = 0; q < NA; q++) {
             const int e = (int)threadIdx.x + q * BLK;
             const int m = e / 6, k = e - 6 * m;
-            v[q] = e < 6 * MH ? spec[(int64_t)m * mstride + 3 + k] : make_double2(0.0, 0.0);
+            v[q] = e < 6 * MH && m < mc ? spec[(int64_t)m * mstride + 3 + k] : make_double2(0.0, 0.0);
         }
 #pragma unroll
         for (int q = 0; q < NA; q++) {
@@ -1202,7 +1256,7 @@ __global__ __launch_bounds__(512) void k_ifft_epilogue_sphere(RingFft F, const d
     for (int q = 0; q < NB; q++) {
         const int e = (int)threadIdx.x + q * BLK;
         const int m = e / NC4, k = e - NC4 * m;
-        cv[q] = e < NC4 * MH ? spec[(int64_t)m * mstride + (k < 3 ? k : 9)] : make_double2(0.0, 0.0);
+        cv[q] = e < NC4 * MH && m < mc ? spec[(int64_t)m * mstride + (k < 3 ? k : 9)] : make_double2(0.0, 0.0);
     }
     double2 Tee[MPT], Ten[MPT], Tnn[MPT];
 #pragma unroll
@@ -1495,7 +1549,7 @@ void ring_forward(const RingPlan &R, const double *xin, double2 *XF, int64_t nco
     hipLaunchKernelGGL(k_fft_rows, grid, dim3(256), sh, s, R.fft, R.tw_d, xin, XF, R.args.nrows, ncol);
 }
 
-void ring_inverse(const RingPlan &R, const double2 *OUT, double *yout, int64_t ncol, hipStream_t s) {
+void ring_inverse(const RingPlan &R, const double2 *OUT, double *yout, int64_t ncol, hipStream_t s, const int32_t *mcut) {
     if (R.fft.nseq > 0) {
         const RingFft F = launch_rows_fft(R, ncol);
         const int64_t items = (R.args.nrows + 2 * F.nsq - 1) / (2 * F.nsq) * ncol;
@@ -1503,17 +1557,17 @@ void ring_inverse(const RingPlan &R, const double2 *OUT, double *yout, int64_t n
         const size_t sh = staged_shmem(F);
         if (fixed_fits(F) && F.fixed == RING_FIXED_1440) {
             hipLaunchKernelGGL((k_ifft_rows_staged<9, RING_FIXED_1440>), grid, dim3(256), sh, s, F, R.tw_d, R.tws_d, OUT,
-                               yout, R.args.nrows, ncol);
+                               yout, R.args.nrows, ncol, mcut);
         } else if (fixed_fits(F) && F.fixed == RING_FIXED_256) {  // ≤ 4 row pairs: 8 × 129 stored modes
             hipLaunchKernelGGL((k_ifft_rows_staged<5, RING_FIXED_256>), grid, dim3(256), sh, s, F, R.tw_d, R.tws_d, OUT,
-                               yout, R.args.nrows, ncol);
+                               yout, R.args.nrows, ncol, mcut);
         } else if (F.n <= 2048) {
             hipLaunchKernelGGL((k_ifft_rows_staged<9, 0>), grid, dim3(256), sh, s, F, R.tw_d, R.tws_d, OUT, yout,
-                               R.args.nrows, ncol);
+                               R.args.nrows, ncol, mcut);
         } else {
             set_shmem((const void *)k_ifft_rows_staged<17, 0>, sh);
             hipLaunchKernelGGL((k_ifft_rows_staged<17, 0>), grid, dim3(256), sh, s, F, R.tw_d, R.tws_d, OUT, yout,
-                               R.args.nrows, ncol);
+                               R.args.nrows, ncol, mcut);
         }
         return;
     }
@@ -1521,7 +1575,7 @@ void ring_inverse(const RingPlan &R, const double2 *OUT, double *yout, int64_t n
     const dim3 grid((unsigned)(8 * ((items + 7) / 8)));
     const size_t sh = fft_shmem(R.fft);
     set_shmem((const void *)k_ifft_rows, sh);
-    hipLaunchKernelGGL(k_ifft_rows, grid, dim3(256), sh, s, R.fft, R.tw_d, OUT, yout, R.args.nrows, ncol);
+    hipLaunchKernelGGL(k_ifft_rows, grid, dim3(256), sh, s, R.fft, R.tw_d, OUT, yout, R.args.nrows, ncol, mcut);
 }
 
 // Weight batch depth and occupancy bound of the contraction, from the blocks per CU its LDS slab
@@ -1559,16 +1613,19 @@ static int mfma_nt(int64_t ncol) {
 }
 
 static void launch_contract_mfma(const RingPlan &R, const double2 *XF, double2 *OUT, int64_t ncol, int orm,
-                                 hipStream_t s) {
+                                 bool cut, hipStream_t s) {
     const int nt = mfma_nt(ncol);
     const int64_t nrb = (R.args.ny + 63) / 64;
     const int64_t ncc = (2 * ncol + nt * 16 - 1) / (nt * 16);
-    const int64_t items = R.mh * nrb * ncc;
+    const int32_t *wl = cut ? R.mitems_d : nullptr;
+    const int64_t nwl = cut ? R.nmitems : 0;
+    const int64_t items = (cut ? nwl : R.mh * nrb) * ncc;
     const dim3 grid((unsigned)(8 * ((items + 7) / 8)));
     const size_t sh = 2 * (size_t)MFMA_KC * (nt * 16 + 2) * sizeof(double);   // two staging buffers
     auto go = [&](auto kern) {
         set_shmem((const void *)kern, sh);
-        hipLaunchKernelGGL(kern, grid, dim3(256), sh, s, R.args, R.wt_d, R.wt_steps, R.wt_tiles, XF, OUT, ncol, orm);
+        hipLaunchKernelGGL(kern, grid, dim3(256), sh, s, R.args, R.wt_d, R.wt_steps, R.wt_tiles, XF, OUT, ncol, orm,
+                           wl, nwl);
     };
     if (nt == 9) go(k_contract_mfma<9>);
     else if (nt == 5) go(k_contract_mfma<5>);
@@ -1577,30 +1634,48 @@ static void launch_contract_mfma(const RingPlan &R, const double2 *XF, double2 *
     else go(k_contract_mfma<2>);
 }
 
+void ring_mode_band_cut(const RingPlan &R, double ell, const double *dxb, double *S, int32_t *mband, hipStream_t s) {
+    const RingArgs &A = R.args;
+    const int64_t blocks = std::min<int64_t>((A.mh * A.nbands + 255) / 256, 8 * 1024);
+    hipLaunchKernelGGL(k_mode_sums, dim3((unsigned)blocks), dim3(256), 0, s, A, S);
+    hipLaunchKernelGGL(k_mode_cut, dim3((unsigned)((A.nbands + 255) / 256)), dim3(256), 0, s, A, R.fft.n, RING_MODE_TOL,
+                       RING_MODE_SLOPE_TOL, ell, dxb, (const double *)S, mband);
+}
+
 void ring_build_wtiles(RingPlan &R, hipStream_t s) {
     const int64_t total = R.mh * R.wt_tiles * R.wt_steps * 64;
     const int64_t blocks = std::min<int64_t>((total + 255) / 256, 8 * 1024);
     hipLaunchKernelGGL(k_build_wtiles, dim3((unsigned)blocks), dim3(256), 0, s, R.args, R.wt_steps, R.wt_tiles, R.wt_d);
 }
 
-void ring_contract(const RingPlan &R, const double2 *XF, double2 *OUT, int64_t ncol, hipStream_t s, bool out_rowmajor) {
+const int32_t *ring_mode_cut(const RingPlan &R, int dtype) {
+    if (dtype != CGEF_F32 || !R.mcut_d) return nullptr;
+    const char *e = std::getenv("CGEF_MODE_CUT");
+    return e && e[0] == '0' ? nullptr : R.mcut_d;
+}
+
+void ring_contract(const RingPlan &R, const double2 *XF, double2 *OUT, int64_t ncol, hipStream_t s, bool out_rowmajor,
+                   const int32_t *mcut) {
     constexpr int CC = RING_CONTRACT_CC;
     const int span = 256 + R.args.dy_max - R.args.dy_min;
     const size_t sh = (size_t)CC * span * sizeof(double2);
     const int64_t nyb = (R.args.ny + 255) / 256;
-    const int64_t items = (ncol + CC - 1) / CC * nyb * R.args.nz * R.mh;
+    const bool cut = mcut != nullptr;
+    const int32_t *wl = cut ? R.citems_d : nullptr;
+    const int64_t nwl = cut ? R.ncitems : 0;
+    const int64_t items = (ncol + CC - 1) / CC * (cut ? nwl : nyb * R.args.nz * R.mh);
     const dim3 grid((unsigned)(8 * ((items + 7) / 8)));
     const size_t lds_blocks = LDS_BYTES_PER_CU / sh;
     const bool has_mfma = R.wt_d != nullptr;
     const int nvar = has_mfma ? 4 : 3;
     const int orm = out_rowmajor ? 1 : 0;
-    const int64_t key = 2 * ncol + orm;   // the kept variant per column count and output layout
+    const int64_t key = 4 * ncol + 2 * (cut ? 1 : 0) + orm;   // the kept variant per column count, cut and layout
     auto launch = [&](int variant) {
         auto go = [&](auto kern) {
             set_shmem((const void *)kern, sh);
-            hipLaunchKernelGGL(kern, grid, dim3(256), sh, s, R.args, XF, OUT, ncol, orm);
+            hipLaunchKernelGGL(kern, grid, dim3(256), sh, s, R.args, XF, OUT, ncol, orm, wl, nwl);
         };
-        if (variant == 3) launch_contract_mfma(R, XF, OUT, ncol, orm, s);
+        if (variant == 3) launch_contract_mfma(R, XF, OUT, ncol, orm, cut, s);
         else if (variant == 0) go(k_contract<CC, 8, 1>);
         else if (variant == 1) go(k_contract<CC, 4, 1>);
         else go(k_contract<CC, 6, 6>);
@@ -1693,7 +1768,7 @@ bool ring_inverse_epilogue_serves(const RingPlan &R, const GeomArgs &G, int nf, 
 
 void ring_inverse_epilogue(const RingPlan &R, const GeomArgs &G, int dtype, const double2 *OUT, const double *D_row,
                            int nf, int withmask, int strategy, double *ubar, void *tau, void *uo, void *vo, void *wo,
-                           int64_t nbatch, bool spectral_rotation, hipStream_t s) {
+                           int64_t nbatch, bool spectral_rotation, hipStream_t s, const int32_t *mcut) {
     if (!ring_inverse_epilogue_serves(R, G, nf, withmask, strategy, nbatch))
         fail(CGEF_ERR_UNSUPPORTED, "the fused inverse transform + epilogue does not serve this plan");
     const RingFft &F = R.fft;
@@ -1712,7 +1787,7 @@ void ring_inverse_epilogue(const RingPlan &R, const GeomArgs &G, int dtype, cons
                 using TT = std::remove_pointer_t<decltype(tp)>;
                 set_shmem((const void *)kern, sh);
                 hipLaunchKernelGGL(kern, grid, dim3(512), sh, s, F, R.tw_d, R.tws_d, OUT, G, D_row, nbatch, ubar,
-                                   (TT *)tau, (TT *)uo, (TT *)vo, (TT *)wo);
+                                   (TT *)tau, (TT *)uo, (TT *)vo, (TT *)wo, mcut);
             };
             if (dtype == CGEF_F32) {
                 if (withmask) go(k_ifft_epilogue_sphere<float, 1, 0>, (float *)nullptr);
@@ -1736,7 +1811,7 @@ void ring_inverse_epilogue(const RingPlan &R, const GeomArgs &G, int dtype, cons
         using TT = std::remove_pointer_t<decltype(tp)>;
         set_shmem((const void *)kern, sh);
         hipLaunchKernelGGL(kern, grid, dim3(blk), sh, s, F, R.tw_d, R.tws_d, OUT, G, D_row, nbatch, ubar, (TT *)tau,
-                           (TT *)uo, (TT *)vo, (TT *)wo);
+                           (TT *)uo, (TT *)vo, (TT *)wo, mcut);
     };
 #define CGEF_IFE(TT, FIXV, MODEV)                                                                    \
     if (withmask) go(k_ifft_epilogue<TT, FIXV, MODEV, 1>, (TT *)nullptr);                           \

@@ -35,12 +35,24 @@ void free_ring(RingPlan &R);
 void ring_build_what(const RingPlan &R, const Run *runs_d, const int32_t *run_off_d, const double *w64_d, int64_t nruns,
                      hipStream_t s);
 void ring_forward(const RingPlan &R, const double *xin, double2 *XF, int64_t ncol, hipStream_t s);
-// out_rowmajor: OUT as [m][row][col] (what the fused inverse + epilogue reads) instead of [m][col][row]
+// The mode cut's per-row table for a call in `dtype`: the plan's cut for FP32 calls when it drops any
+// mode (CGEF_MODE_CUT=0 turns it off), none (null) for FP64 calls.
+const int32_t *ring_mode_cut(const RingPlan &R, int dtype);
+// Builds the plan's mode cut (RING_MODE_TOL, RING_MODE_SLOPE_TOL) from Ŵ: mcut_d and the contraction
+// work lists.
+void ring_build_mode_cut(const Plan &p, RingPlan &R, hipStream_t s);
+// Per band the modes kept (dxb: the band's smallest x spacing, device; S: mh × bands scratch; mband:
+// bands).
+void ring_mode_band_cut(const RingPlan &R, double ell, const double *dxb, double *S, int32_t *mband, hipStream_t s);
+// out_rowmajor: OUT as [m][row][col] (what the fused inverse + epilogue reads) instead of [m][col][row].
+// mcut (from ring_mode_cut, may be null): contract only the (mode, row block) pairs of the cut's work
+// lists; the consumers of OUT then read row r's modes m < mcut[r] only.
 void ring_contract(const RingPlan &R, const double2 *XF, double2 *OUT, int64_t ncol, hipStream_t s,
-                   bool out_rowmajor = false);
+                   bool out_rowmajor = false, const int32_t *mcut = nullptr);
 // fills R.wt_d (allocated by the caller: mh × wt_tiles × wt_steps × 64 doubles) from Ŵ
 void ring_build_wtiles(RingPlan &R, hipStream_t s);
-void ring_inverse(const RingPlan &R, const double2 *OUT, double *yout, int64_t ncol, hipStream_t s);
+void ring_inverse(const RingPlan &R, const double2 *OUT, double *yout, int64_t ncol, hipStream_t s,
+                  const int32_t *mcut = nullptr);
 // Π inputs: per slice nf planes (frame velocity components, their products, then the mask when
 // `withmask`); the epilogue turns the filtered planes into ū (FP64) and τ, optionally exporting ū in T.
 void ring_prep(const GeomArgs &G, int dtype, const void *u, const void *v, const void *w, int nf, int withmask,
@@ -55,7 +67,7 @@ bool ring_inverse_epilogue_serves(const RingPlan &R, const GeomArgs &G, int nf, 
                                   int64_t nbatch);
 void ring_inverse_epilogue(const RingPlan &R, const GeomArgs &G, int dtype, const double2 *OUT, const double *D_row,
                            int nf, int withmask, int strategy, double *ubar, void *tau, void *uo, void *vo, void *wo,
-                           int64_t nbatch, bool spectral_rotation, hipStream_t s);
+                           int64_t nbatch, bool spectral_rotation, hipStream_t s, const int32_t *mcut = nullptr);
 void ring_epilogue(const GeomArgs &G, int dtype, const double *yout, const double *D_row, int nf, int withmask,
                    int strategy, double *ubar, void *tau, void *uo, void *vo, void *wo, int64_t nbatch, hipStream_t s);
 void ring_prep_fields(const GeomArgs &G, int dtype, const void *in, int64_t nsl, int withmask, double *xin, hipStream_t s);

@@ -326,7 +326,74 @@ void build_ring(Plan &p, hipStream_t s) {
     CGEF_HIP(hipGetLastError());
     CGEF_HIP(hipStreamSynchronize(s));
     (void)hipFree(run_off_d);
+    ring_build_mode_cut(p, R, s);
     R.ok = true;
+}
+
+// The mode cut (RING_MODE_TOL, RING_MODE_SLOPE_TOL): per band the modes kept, from the device-built Ŵ;
+// per row the band's value; and the contraction work lists over the (mode, row block) pairs some row
+// keeps.
+void ring_build_mode_cut(const Plan &p, RingPlan &R, hipStream_t s) {
+    const RingArgs &A = R.args;
+    const Grid &g = *p.grid;
+    const std::vector<int32_t> &band_of_row = p.band_of_row;
+    // x spacing per row (the ring's uniform step; on the sphere R·cosφ·Δλ), the smallest per band
+    const double step = g.coord[0].size() > 1 ? std::fabs(g.coord[0][1] - g.coord[0][0]) : 1.0;
+    std::vector<double> dxb(A.nbands, HUGE_VAL);
+    for (int64_t r = 0; r < A.nrows; r++) {
+        const int64_t j = r % A.ny;
+        const double dx = g.geometry == CGEF_SPHERICAL ? g.R * std::cos(g.coord[1][j]) * step : step;
+        double &d = dxb[band_of_row[r]];
+        d = std::min(d, std::max(dx, 1e-300));
+    }
+    struct Tmp {
+        void *p = nullptr;
+        ~Tmp() {
+            if (p) (void)hipFree(p);
+        }
+    } S, mb;
+    CGEF_HIP(hipMalloc(&S.p, (size_t)A.mh * A.nbands * sizeof(double)));
+    CGEF_HIP(hipMalloc(&mb.p, (size_t)A.nbands * sizeof(int32_t)));
+    double *dxb_d = up(dxb);
+    ring_mode_band_cut(R, p.scale, dxb_d, (double *)S.p, (int32_t *)mb.p, s);
+    CGEF_HIP(hipStreamSynchronize(s));
+    (void)hipFree(dxb_d);
+    CGEF_HIP(hipGetLastError());
+    std::vector<int32_t> mband(A.nbands);
+    CGEF_HIP(hipMemcpyAsync(mband.data(), mb.p, mband.size() * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    CGEF_HIP(hipStreamSynchronize(s));
+    const int64_t Ny = A.ny, nrows = A.nrows;
+    std::vector<int32_t> mrow(nrows);
+    int64_t kept = 0;
+    for (int64_t r = 0; r < nrows; r++) {
+        mrow[r] = mband[band_of_row[r]];
+        kept += mrow[r];
+    }
+    R.modes_kept = (double)kept / ((double)nrows * (double)A.mh);
+    if (kept == nrows * A.mh) return;   // nothing to drop
+    // per row block (rows of one level), the most modes any of its rows keeps; the work list holds the
+    // (m, block) pairs below that, in ascending m as the full grid orders them
+    auto work_list = [&](int64_t rows_per_block, int64_t &count) {
+        const int64_t nyb = (Ny + rows_per_block - 1) / rows_per_block, nblk = nyb * A.nz;
+        if ((int64_t)A.mh * nblk >= ((int64_t)1 << 31)) fail(CGEF_ERR_UNSUPPORTED, "mode-cut work list exceeds 32 bits");
+        std::vector<int32_t> mblk(nblk, 0), list;
+        for (int64_t r = 0; r < nrows; r++) {
+            const int64_t k = r / Ny, j = r - k * Ny;
+            int32_t &x = mblk[k * nyb + j / rows_per_block];
+            x = std::max(x, mrow[r]);
+        }
+        for (int64_t m = 0; m < A.mh; m++)
+            for (int64_t q = 0; q < nblk; q++)
+                if (m < mblk[q]) list.push_back((int32_t)(m * nblk + q));
+        count = (int64_t)list.size();
+        return up(list);
+    };
+    R.citems_d = work_list(256, R.ncitems);
+    if (R.wt_d) {
+        if (A.nz != 1) fail(CGEF_ERR_UNSUPPORTED, "matrix-core contraction on a 3-D plan");
+        R.mitems_d = work_list(64, R.nmitems);
+    }
+    R.mcut_d = up(mrow);
 }
 
 void free_ring(RingPlan &R) {
@@ -337,6 +404,9 @@ void free_ring(RingPlan &R) {
     if (R.tws_d) (void)hipFree(R.tws_d);
     if (R.what_d) (void)hipFree(R.what_d);
     if (R.wt_d) (void)hipFree(R.wt_d);
+    if (R.mcut_d) (void)hipFree(R.mcut_d);
+    if (R.citems_d) (void)hipFree(R.citems_d);
+    if (R.mitems_d) (void)hipFree(R.mitems_d);
     if (R.dz_val_d) (void)hipFree(R.dz_val_d);
     if (R.dz_off0_d) (void)hipFree(R.dz_off0_d);
     if (R.off_dy_d) (void)hipFree(R.off_dy_d);

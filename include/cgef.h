@@ -90,6 +90,13 @@ typedef struct {
     int64_t ring_contract_kernel; /* ring engine: the contraction kernel kept by measurement for the
                                      widest call so far (0-2 VALU weight-batch variants, 3 matrix
                                      cores), −1 before any measured call */
+    double ring_modes_kept;   /* ring engine: fraction of the (row, mode) pairs FP32 calls contract
+                                 under the mode cut (1 = none dropped; FP64 calls keep every mode):
+                                 row r keeps modes m < M(r), the smallest M at which the weight tail
+                                 2·Σ_{m≥M} Σ_offsets |Ŵ(m)| is ≤ 3e-8 of its weight sum and the same
+                                 tail weighted by max(1, 4ℓ·sin(πm/N)/Δx) ≤ 3e-6 of it, so under
+                                 ZeroFill a filtered value moves by ≤ 3e-8·max|q| and an x-difference
+                                 of one by ≤ 3e-6·max|q|/ℓ; 0 for other engines */
 } cgef_plan_info;
 
 const char *cgef_version(void);
