@@ -12,7 +12,13 @@ The measured engine is the library's default for this grid (the ring engine: the
 footprint evaluated in Fourier space along the periodic longitude, FP64 inside). Every run checks Π
 against the FP64 oracle (the reference's algorithm restated on the CPU): the full field at 50 and
 180 km, and the rows at the poles, ±60° and the equator at ≈656, ≈1053 and 2000 km, for two
-snapshots. The value is reported only when every check is within rel-L2 1e-5 (else null).
+snapshots. The value is reported only when every check is within rel-L2 1e-5 (else null). FP32 calls of
+the ring engine contract only the Fourier modes each row keeps under the library's mode cut (bounded
+at 3e-8·max|q| in every filtered value); the same sweep with every mode is timed beside it.
+
+Roofline: the sweep's dominant kernel, the fused inverse row transform + epilogue, by its bytes per
+launch over its mean launch time (HIP events on the launch stream, in calls outside the timed ones);
+the widest scale's contraction by its FP64 arithmetic as an extra.
 
 Also reported: the same sweep replayed as a HIP graph, the single-scale 0.25° step (ℓ = 200 km) with
 its error table, the fused FP32 direct-sum engine and the plain FP64 GPU direct sum on it, and the
@@ -115,6 +121,30 @@ def timed_steps(step, steps, warmup, plan=None):
     return el, kern_ms
 
 
+def profiled_kernel_ms(step, plans, what, reps=3):
+    """Average launch time (ms) of step `what` (1: the filter step, 2: the ring engine's Π inverse
+    transform + epilogue) over every launch the given plans make in `reps` calls, from the HIP events
+    the library records around those launches on their stream (cgef_profile_enable), outside the timed
+    steps. Also the per-plan averages."""
+    import torch
+    import cgef
+    L = cgef._lib.load()
+    for p in plans:
+        cgef._lib.check(L.cgef_profile_enable(p.h, what))
+    for _ in range(reps):
+        step()
+    torch.cuda.synchronize()
+    tot, n, per = 0.0, 0, []
+    for p in plans:
+        kms, nl = C.c_double(), C.c_int64()
+        cgef._lib.check(L.cgef_profile_read(p.h, C.byref(kms), C.byref(nl)))
+        cgef._lib.check(L.cgef_profile_enable(p.h, 0))
+        tot += kms.value
+        n += nl.value
+        per.append(kms.value / max(nl.value, 1))
+    return tot / max(n, 1), per
+
+
 def rel_l2(got, ref):
     import numpy as np
     got, ref = np.asarray(got, dtype=np.float64), np.asarray(ref, dtype=np.float64)
@@ -126,15 +156,6 @@ def sphere_oracle_grid(lam, phi, mask):
     from oracle import oracle as O
     return O.Grid([lam, phi], spherical=True, R=6.371e6, periodic=[True, False], period=[2 * np.pi, 0.0],
                   mask=mask, uniform=[True, True])
-
-
-def in_range_offsets(noff, ny):
-    """Σ over target rows of the contraction's row offsets whose source row lies on the grid (the
-    footprint's dy range is symmetric, −h..h; rows beyond a pole are clipped, Filtering.jl:705-707)."""
-    import numpy as np
-    h = (noff - 1) // 2
-    j = np.arange(ny)
-    return int(np.sum(np.minimum(j + h, ny - 1) - np.maximum(j - h, 0) + 1))
 
 
 def sweep_headline(args, grid, mask, lam, phi, cpu):
@@ -171,23 +192,28 @@ def sweep_headline(args, grid, mask, lam, phi, cpu):
                                  workspaces=[ws])
 
     # eager (the default): the first call measures the contraction kernel variants per plan
-    el, kern_ms = timed_steps(call, args.steps, args.warmup, plans[-1])
+    el, _ = timed_steps(call, args.steps, args.warmup)
     ms = el * 1e3 / args.steps
     value = npts * ns * NSNAP * args.steps / el
+    # kernel times for the rooflines, from HIP events in separate calls (not the timed ones)
+    ife_ms, ife_per = profiled_kernel_ms(call, plans, 2)
+    kern_ms, _ = profiled_kernel_ms(call, plans[-1:], 1)
     log(f"sweep: {ms:.2f} ms per {NSNAP}x{ns} call (eager) -> {value:.3e} point-scales/s; "
-        f"2000 km contraction {kern_ms:.3f} ms")
+        f"inverse transform + epilogue {ife_ms:.3f} ms per launch (sweep mean), 2000 km contraction {kern_ms:.3f} ms")
     names = {0: "k_contract<5,8,1>", 1: "k_contract<5,4,1>", 2: "k_contract<5,6,6>", 3: "k_contract_mfma", -1: None}
     variants = {f"{s / 1e3:.0f}km": names[int(p.query().ring_contract_kernel)] for s, p in zip(scales, plans)}
     kept = {f"{s / 1e3:.0f}km": round(float(p.query().ring_modes_kept), 4) for s, p in zip(scales, plans)}
     # the same sweep contracting every Fourier mode (the exact operator FP64 calls run), for comparison
-    os.environ["CGEF_MODE_CUT"] = "0"
-    try:
-        reps = max(3, args.steps // 2)
-        ela, _ = timed_steps(call, reps, 1)
-    finally:
-        os.environ.pop("CGEF_MODE_CUT", None)
-    ms_all_modes = ela * 1e3 / reps
-    log(f"sweep: {ms_all_modes:.2f} ms per call contracting every mode (no mode cut)")
+    ms_all_modes = None
+    if not args.no_all_modes:
+        os.environ["CGEF_MODE_CUT"] = "0"
+        try:
+            reps = max(3, args.steps // 2)
+            ela, _ = timed_steps(call, reps, 1)
+        finally:
+            os.environ.pop("CGEF_MODE_CUT", None)
+        ms_all_modes = ela * 1e3 / reps
+        log(f"sweep: {ms_all_modes:.2f} ms per call contracting every mode (no mode cut)")
     # the same sweep replayed as one HIP graph (opt-in), timed in this process
     ms_graph = None
     if not args.no_graph:
@@ -226,23 +252,43 @@ def sweep_headline(args, grid, mask, lam, phi, cpu):
     worst = max(v.get("rel_l2_pi", v.get("rel_l2_pi_rows")) for v in checks.values())
     parity_ok = worst <= PARITY_TOL
 
-    # roofline: the widest scale's contraction (the sweep's largest kernel) by its FP64 arithmetic —
-    # 4 flops (two FMAs on the real and imaginary parts) per (mode, plane column, target row, in-range
-    # row offset) — and by the bytes it must move (Ŵ once, spectra in once and out once)
+    # roofline: the sweep's dominant kernel, the fused inverse row transform + epilogue (one launch per
+    # scale), by the bytes it must move per launch — the contracted spectra of the modes each row keeps
+    # (16 B per (row, kept mode, plane column)) in, the mask in once, ū (2 FP64) and τ (3 FP32) out per
+    # point and snapshot — over its mean launch time across the 24 scales
     info = plans[-1].info
     ncol = info.ring_pi_fields * NSNAP
     mh, noff, ny = info.ring_modes, info.ring_offsets, grid.shape[1]
-    flops = 4.0 * ncol * mh * in_range_offsets(noff, ny)
-    bytes_alg = mh * noff * info.nbands * 8 + 2 * ncol * mh * ny * 16
-    achieved = flops / (kern_ms * 1e-3) / 1e12
+    cuts = [p.mode_cut().astype(np.int64) for p in plans]
+    ife_bytes = [int(c.sum()) * ncol * 16 + npts + NSNAP * npts * (2 * 8 + 3 * 4) for c in cuts]
+    ife_alg = sum(ife_bytes) / ns
+    roof = {"bound": "hbm", "achieved": ife_alg / (ife_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": ife_alg / (ife_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "traffic": measured_traffic(PROFILE_SUMMARY, "k_ifft_epilogue_sweep_mean"),
+            "kernel": "k_ifft_epilogue_sphere (fused inverse row transform + epilogue, mean over the 24 scales' "
+                      f"launches; {ncol} plane columns)",
+            "kernel_ms": ife_ms, "algorithmic_bytes_per_launch": ife_alg,
+            "per_scale_kernel_ms": {f"{s / 1e3:.0f}km": round(t, 4) for s, t in zip(scales, ife_per)}}
+    # the widest scale's contraction by its FP64 arithmetic — 4 flops (two FMAs on the real and imaginary
+    # parts) per (kept mode, plane column, target row, in-range row offset) — and by the bytes it must
+    # move (Ŵ of the kept modes once, their spectra in once and out once)
+    h = (noff - 1) // 2
+    rows = np.arange(ny)
+    inr = np.minimum(rows + h, ny - 1) - np.maximum(rows - h, 0) + 1     # in-range offsets per target row
+    cw = cuts[-1]
+    flops = 4.0 * ncol * float(np.sum(cw * inr))
+    bytes_alg = int(cw.sum()) * noff * 8 + 2 * ncol * int(cw.sum()) * 16
     kname = variants[f"{scales[-1] / 1e3:.0f}km"] or "k_contract"
     pipe = ("FP64 matrix cores (v_mfma_f64_16x16x4f64)" if kname == "k_contract_mfma" else "FP64 vector FMA (v_fma_f64)")
-    roof = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": achieved / FP64_PEAK_TFLOPS, "traffic": measured_traffic(PROFILE_SUMMARY, "k_contract_2000km"),
-            "kernel": f"{kname} (ring contraction, {scales[-1] / 1e3:.0f} km: {noff} row offsets, {ncol} plane columns)",
-            "pipe": pipe + "; the published FP64 peak (78.6 TFLOP/s) is the same for the vector and matrix units",
-            "kernel_ms": kern_ms, "algorithmic_flops_per_launch": flops, "algorithmic_bytes_per_launch": bytes_alg,
-            "hbm_gbs": bytes_alg / (kern_ms * 1e-3) / 1e9, "hbm_frac": bytes_alg / (kern_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    achieved = flops / (kern_ms * 1e-3) / 1e12
+    roof_c = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+              "frac": achieved / FP64_PEAK_TFLOPS, "traffic": measured_traffic(PROFILE_SUMMARY, "k_contract_2000km"),
+              "kernel": f"{kname} (ring contraction, {scales[-1] / 1e3:.0f} km: {noff} row offsets, {ncol} plane "
+                        f"columns, {int(cw.sum())} kept (row, mode) pairs of {ny * mh})",
+              "pipe": pipe + "; the published FP64 peak (78.6 TFLOP/s) is the same for the vector and matrix units",
+              "kernel_ms": kern_ms, "algorithmic_flops_per_launch": flops, "algorithmic_bytes_per_launch": bytes_alg,
+              "hbm_gbs": bytes_alg / (kern_ms * 1e-3) / 1e9,
+              "hbm_frac": bytes_alg / (kern_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
     # the whole step against its compulsory HBM bytes (u, v FP32 and the mask in; Π FP32 out)
     step_bytes = NSNAP * npts * 8 + npts + NSNAP * ns * npts * 4
     step = {"compulsory_bytes": step_bytes, "gbs": step_bytes / (ms * 1e-3) / 1e9,
@@ -251,6 +297,7 @@ def sweep_headline(args, grid, mask, lam, phi, cpu):
     direct_flops = 2.0 * 9.0 * sum(taps) * NSNAP   # the direct sum's algorithmic work per step
     out = {"value": value if parity_ok else None, "ms_per_step": ms, "ms_per_step_graph": ms_graph,
            "parity_ok": parity_ok, "worst_rel_l2": worst, "parity": checks, "roofline": roof,
+           "roofline_contract_widest": roof_c,
            "step_compulsory_hbm": step, "contract_kernel_per_scale": variants, "plan_build_s": tplan,
            "modes_kept_per_scale": kept, "ms_per_step_all_modes": ms_all_modes,
            "engine": plans[0].engine, "direct_sum_equivalent_tflops": direct_flops / (ms * 1e-3) / 1e12,
@@ -313,8 +360,10 @@ def sphere_single_leg(args, grid, mask, lam, phi, cpu, u32, v32):
                        "1 scale, 1 snapshot", "engine": plan.engine, "ms_per_step": el * 1e3 / steps,
            "value": npts * steps / el, "unit": "point-scales/s"}
     if plan.engine == "ring":
-        nf, mh, noff, ny = info.ring_pi_fields, info.ring_modes, info.ring_offsets, grid.shape[1]
-        alg = mh * noff * info.nbands * 8 + 2 * nf * mh * ny * 16
+        # Ŵ and the spectra of the (row, mode) pairs the mode cut keeps (one band per row on the sphere)
+        nf, noff = info.ring_pi_fields, info.ring_offsets
+        kept = int(plan.mode_cut().astype(np.int64).sum())
+        alg = kept * noff * 8 + 2 * nf * kept * 16
         out["roofline_contract"] = {"bound": "hbm", "achieved": alg / (kern_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
                                     "unit": "GB/s", "frac": alg / (kern_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                                     "traffic": measured_traffic(PROFILE_SUMMARY_SINGLE, "k_contract"),
@@ -753,6 +802,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline timings (parity still runs)")
     ap.add_argument("--no-extra", action="store_true", help="skip the fused FP32 and plain FP64 direct sums")
     ap.add_argument("--no-graph", action="store_true", help="skip timing the sweep replayed as a HIP graph")
+    ap.add_argument("--no-all-modes", action="store_true",
+                    help="skip timing the sweep with the mode cut off (every Fourier mode contracted)")
     ap.add_argument("--only", default=None,
                     help="comma-separated extra legs to run after the headline sweep (single, tophat, cartesian, "
                          "cartesian4096, volume, spectral, closure); overrides --no-configs")
@@ -853,9 +904,9 @@ def main():
         "roofline": head["roofline"],
         "cpu_baseline": cpu_base,
     }
-    for k in ("ms_per_step_graph", "ms_per_step_all_modes", "modes_kept_per_scale", "parity_ok", "worst_rel_l2",
-              "parity", "step_compulsory_hbm", "contract_kernel_per_scale", "plan_build_s",
-              "direct_sum_equivalent_tflops"):
+    for k in ("roofline_contract_widest", "ms_per_step_graph", "ms_per_step_all_modes", "modes_kept_per_scale",
+              "parity_ok", "worst_rel_l2", "parity", "step_compulsory_hbm", "contract_kernel_per_scale",
+              "plan_build_s", "direct_sum_equivalent_tflops"):
         out[k] = head[k]
     out.update(ratios)
     out.update(extra)

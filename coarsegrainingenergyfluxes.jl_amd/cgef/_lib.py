@@ -23,7 +23,7 @@ EXPORTED = (
     "cgef_plan_create", "cgef_plan_destroy", "cgef_plan_query", "cgef_workspace_create",
     "cgef_workspace_destroy", "cgef_filter_apply", "cgef_compute_pi", "cgef_coarse_grain", "cgef_derivative",
     "cgef_energy", "cgef_energy_noalloc",
-    "cgef_cumulative_energy", "cgef_profile_enable", "cgef_profile_read", "cgef_spectral_density",
+    "cgef_cumulative_energy", "cgef_profile_enable", "cgef_profile_read", "cgef_plan_mode_cut", "cgef_spectral_density",
     # learned closure (no reference counterpart)
     "cgef_conv5", "cgef_conv5_mfma", "cgef_conv5_pack_mfma", "cgef_conv5_wgrad_scratch", "cgef_conv5_wgrad", "cgef_conv5_pack_weights", "cgef_adam_update",
     "cgef_mse_grad", "cgef_closure_loss_grad", "cgef_closure_crop", "cgef_closure_products", "cgef_closure_targets", "cgef_closure_pi",
@@ -120,6 +120,7 @@ def load():
         "cgef_cumulative_energy": [C.POINTER(vp), i64, vp, vp, vp, vp, i64, i32, dp, vp],
         "cgef_profile_enable": [vp, i32],
         "cgef_profile_read": [vp, dp, C.POINTER(i64)],
+        "cgef_plan_mode_cut": [vp, C.POINTER(C.c_int32), i64],
         "cgef_spectral_density": [i64, dp, dp, dp],
     }
     for name, args in sigs.items():

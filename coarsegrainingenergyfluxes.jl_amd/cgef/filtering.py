@@ -89,6 +89,15 @@ class FilterPlan:
         self.info = info
         return info
 
+    def mode_cut(self):
+        """Per row (Ny·Nz, level-major) the Fourier modes along x that FP32 calls of this ring plan keep
+        (cgef_plan_mode_cut); N/2 + 1 on rows where the cut drops none."""
+        import numpy as np
+        n = int(np.prod(self.grid.field_shape[:-1]))
+        out = np.empty(n, dtype=np.int32)
+        _lib.check(_lib.load().cgef_plan_mode_cut(self.h, out.ctypes.data_as(C.POINTER(C.c_int32)), n))
+        return out
+
     @property
     def mean_taps(self):
         return self.info.mean_taps

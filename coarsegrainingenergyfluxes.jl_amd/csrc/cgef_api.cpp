@@ -77,8 +77,8 @@ Grid::~Grid() {
     for (int d = 0; d < 3; d++) dfree(dcoef_d[d]);
 }
 
-void Plan::prof_begin(hipStream_t s) {
-    if (!profiling) return;
+void Plan::prof_begin(hipStream_t s, int what) {
+    if (profiling != what) return;
     if (prof_used == prof_events.size()) {
         hipEvent_t a, b;
         CGEF_HIP(hipEventCreate(&a));
@@ -88,8 +88,8 @@ void Plan::prof_begin(hipStream_t s) {
     CGEF_HIP(hipEventRecord(prof_events[prof_used].first, s));
 }
 
-void Plan::prof_end(hipStream_t s) {
-    if (!profiling) return;
+void Plan::prof_end(hipStream_t s, int what) {
+    if (profiling != what) return;
     CGEF_HIP(hipEventRecord(prof_events[prof_used].second, s));
     prof_used++;
 }
@@ -441,15 +441,17 @@ static bool ring_pi_fused(const Plan &p, const GeomArgs &G, int nf, int wm, int6
 static void ring_pi_inverse_epilogue(Plan &p, Workspace &w, const GeomArgs &G, int dtype, int nf, int wm, void *uo,
                                      void *vo, void *wo, int64_t nbatch, bool fused, hipStream_t s) {
     const int32_t *mcut = ring_mode_cut(p.ring, dtype);   // what the contraction ran under
+    p.prof_begin(s, 2);
     if (fused) {
         const char *fe = std::getenv("CGEF_FUSE");
         const bool spectral = !(fe && fe[0] == '1') && lon_ascends_by_ring_step(*p.grid);
         ring_inverse_epilogue(p.ring, G, dtype, w.rout, p.D64_row_d, nf, wm, p.strategy, w.ubar, w.tau, uo, vo, wo,
                               nbatch, spectral, s, mcut);
-        return;
+    } else {
+        ring_inverse(p.ring, w.rout, w.rxy, nf * nbatch, s, mcut);
+        ring_epilogue(G, dtype, w.rxy, p.D64_row_d, nf, wm, p.strategy, w.ubar, w.tau, uo, vo, wo, nbatch, s);
     }
-    ring_inverse(p.ring, w.rout, w.rxy, nf * nbatch, s, mcut);
-    ring_epilogue(G, dtype, w.rxy, p.D64_row_d, nf, wm, p.strategy, w.ubar, w.tau, uo, vo, wo, nbatch, s);
+    p.prof_end(s, 2);
 }
 
 // spectral engine: forward row transforms, per-mode column transform × Ĝ and back, row inverse
@@ -1189,7 +1191,9 @@ cgef_status cgef_profile_enable(cgef_plan *plan, int32_t on) {
     return guard([&] {
         Plan *p = reinterpret_cast<Plan *>(plan);
         if (!p) fail(CGEF_ERR_ARGUMENT, "null plan");
-        p->profiling = on != 0;
+        if (on < 0 || on > 2) fail(CGEF_ERR_ARGUMENT, "profile step must be 0, 1 or 2");
+        if (on == 2 && p->engine != CGEF_ENGINE_RING) fail(CGEF_ERR_UNSUPPORTED, "step 2 is the ring engine's");
+        p->profiling = on;
         p->prof_used = 0;
     });
 }
@@ -1207,6 +1211,17 @@ cgef_status cgef_profile_read(cgef_plan *plan, double *total_ms, int64_t *nlaunc
         }
         *total_ms = tot;
         *nlaunches = (int64_t)p->prof_used;
+    });
+}
+
+cgef_status cgef_plan_mode_cut(const cgef_plan *plan, int32_t *modes, int64_t nrows) {
+    return guard([&] {
+        const Plan *p = reinterpret_cast<const Plan *>(plan);
+        if (!p || !modes) fail(CGEF_ERR_ARGUMENT, "null plan or output");
+        if (!p->ring.ok || p->ring.spectral) fail(CGEF_ERR_UNSUPPORTED, "the mode cut belongs to the ring engine");
+        const int64_t n = p->grid->nrows;
+        if (nrows != n) fail(CGEF_ERR_DIMENSION, "expected " + std::to_string(n) + " rows (Ny·Nz)");
+        for (int64_t r = 0; r < n; r++) modes[r] = p->ring.mcut_h.empty() ? (int32_t)p->ring.mh : p->ring.mcut_h[r];
     });
 }
 

@@ -155,6 +155,7 @@ struct RingPlan {
     // VALU contraction's work list; mitems_d the same over 64-row blocks for the matrix-core one.
     int32_t *mcut_d = nullptr, *citems_d = nullptr, *mitems_d = nullptr;
     int64_t ncitems = 0, nmitems = 0;
+    std::vector<int32_t> mcut_h;       // host copy of the per-row modes kept (empty: every mode)
     double modes_kept = 1.0;           // Σ_rows M(r) / (rows · mh)
     RingArgs args;
 };
@@ -253,11 +254,13 @@ struct Plan {
     // prefix-sum top-hat engine
     PrefixPlan pre;
     // optional kernel timing: event pairs around each dominant-kernel launch
-    bool profiling = false;
+    int profiling = 0;            // which step cgef_profile_enable times (0: none)
     std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_events;
     size_t prof_used = 0;
-    void prof_begin(hipStream_t s);
-    void prof_end(hipStream_t s);
+    // brackets a launch of step `what` (1: the filter step, 2: the ring engine's Π inverse transform +
+    // epilogue) with events when that step is being timed
+    void prof_begin(hipStream_t s, int what = 1);
+    void prof_end(hipStream_t s, int what = 1);
     ~Plan();
 };
 

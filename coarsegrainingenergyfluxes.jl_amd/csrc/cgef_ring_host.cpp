@@ -394,6 +394,7 @@ void ring_build_mode_cut(const Plan &p, RingPlan &R, hipStream_t s) {
         R.mitems_d = work_list(64, R.nmitems);
     }
     R.mcut_d = up(mrow);
+    R.mcut_h = std::move(mrow);
 }
 
 void free_ring(RingPlan &R) {

@@ -176,13 +176,19 @@ cgef_status cgef_cumulative_energy(cgef_plan *const *plans, int64_t nscales, cge
                                    const void *u, const void *v, const void *w, int64_t nbatch,
                                    int32_t dtype, double *E_out, void *stream);
 
-/* Kernel timing (no counterpart in the reference; for measurement): while enabled, every launch of
- * the plan's dominant filter step is bracketed by HIP events recorded on the launch stream (banded:
- * the fused filter kernel; ring: the contraction; separable and prefix-sum: their two filter
- * kernels together).
+/* Kernel timing (no counterpart in the reference; for measurement): while enabled (on = 1), every
+ * launch of the plan's dominant filter step is bracketed by HIP events recorded on the launch stream
+ * (banded: the fused filter kernel; ring: the contraction; separable and prefix-sum: their two
+ * filter kernels together); on = 2 (ring plans) times the Π planes' inverse transform + epilogue
+ * instead; on = 0 turns timing off.
  * cgef_profile_read synchronizes on those events and returns the summed kernel time (ms) and the
  * number of launches since enabling; enabling again resets the counters. */
 cgef_status cgef_profile_enable(cgef_plan *plan, int32_t on);
+/* The ring engine's Fourier-mode cut (no counterpart in the reference; for measurement and
+ * inspection): modes[r] (host, nrows = Ny·Nz entries) receives the number of Fourier modes along x
+ * that row r keeps in FP32 calls (N/2 + 1 where the cut drops none; see ring_modes_kept).
+ * CGEF_ERR_UNSUPPORTED for plans of other engines. */
+cgef_status cgef_plan_mode_cut(const cgef_plan *plan, int32_t *modes, int64_t nrows);
 cgef_status cgef_profile_read(cgef_plan *plan, double *total_ms, int64_t *nlaunches);
 
 /* Diagnostics.jl:1319-1333 spectral_density!: host arrays of length n. */

@@ -57,9 +57,16 @@ def test_kept_fraction_falls_with_scale():
     ker = cgef.GaussianKernel(4.0)
     kept = []
     for ell in (50e3, 200e3, 1000e3, 2000e3):
-        info = cgef.plan_filter(g, ker, ell, engine="ring").query()
+        plan = cgef.plan_filter(g, ker, ell, engine="ring")
+        info = plan.query()
         kept.append(info.ring_modes_kept)
-        say(f"  {ell / 1e3:.0f} km: {info.ring_modes_kept:.3f} of the (row, mode) pairs kept")
+        cut = plan.mode_cut()
+        say(f"  {ell / 1e3:.0f} km: {info.ring_modes_kept:.3f} of the (row, mode) pairs kept; "
+            f"rows keep {cut.min()}..{cut.max()} of {info.ring_modes} modes")
+        assert cut.shape == (720,) and cut.min() >= 1 and cut.max() <= info.ring_modes
+        assert np.array_equal(cut, cut[::-1])                 # the grid is symmetric about the equator
+        assert abs(cut.sum() / (720 * info.ring_modes) - info.ring_modes_kept) < 1e-12
+        assert cut[0] < cut[360]                               # the polar rows keep the fewest
     assert kept[0] > 0.9 and kept[-1] < 0.1
     assert all(a >= b for a, b in zip(kept, kept[1:]))
     # a translation-invariant grid keeps the same modes on every row
@@ -67,7 +74,10 @@ def test_kept_fraction_falls_with_scale():
     gc = cgef.StructuredGrid(cgef.CartesianGeometry(), xs, xs, periodic=(True, True))
     assert cgef.plan_filter(gc, cgef.GaussianKernel(6.0), 48.0, engine="ring").query().ring_modes_kept < 0.5
     # engines other than the ring have no cut
-    assert cgef.plan_filter(gc, cgef.GaussianKernel(6.0), 48.0, engine="banded").query().ring_modes_kept == 0.0
+    banded = cgef.plan_filter(gc, cgef.GaussianKernel(6.0), 48.0, engine="banded")
+    assert banded.query().ring_modes_kept == 0.0
+    with pytest.raises(cgef._lib.ArgumentError, match="unsupported"):
+        banded.mode_cut()
 
 
 def _filter_case(kind):
