@@ -138,6 +138,80 @@ __global__ __launch_bounds__(256) void k_conv5(const void *__restrict__ in, cons
     }
 }
 
+// Layers with at most 4 output channels (the 64 → 3 output layer): one 8×8 output tile per block, the
+// input channels split over the four waves (wave w takes channels ≡ w mod 4 of each staged group of
+// 16), lane = output pixel, 4 FP32 accumulators; the waves' partial sums are added in LDS in wave
+// order (deterministic) and wave 0 writes the outputs as k_conv5 does. Four times the blocks of a
+// 16×16 tile and a quarter of the channel loop per lane.
+constexpr int NT8 = 8;                  // output tile edge
+constexpr int NS8 = NT8 + 4;            // staged input tile edge
+constexpr int NCG = 16;                 // input channels staged per pass (4 per wave)
+template <int IN_T, int IN_GELU, int OUT>
+__global__ __launch_bounds__(256) void k_conv5_narrow(const void *__restrict__ in, const uint16_t *__restrict__ w,
+                                                      const float *__restrict__ bias, const uint16_t *__restrict__ zg,
+                                                      void *__restrict__ out, uint16_t *__restrict__ act, int64_t cin,
+                                                      int64_t hin, int64_t win, int64_t cout, int64_t ocs, int64_t hout,
+                                                      int64_t wout, int pad, int tiles_x) {
+    __shared__ float tile[NCG][NS8][NS8 + 1];
+    __shared__ float red[4][64][4];
+    const int t = threadIdx.x, wave = t >> 6, lane = t & 63, tx = lane & (NT8 - 1), ty = lane >> 3;
+    const int64_t ox0 = int64_t(blockIdx.x % tiles_x) * NT8, oy0 = int64_t(blockIdx.x / tiles_x) * NT8;
+    const int64_t img = blockIdx.z;
+    const int64_t iy0 = oy0 - pad, ix0 = ox0 - pad;
+    float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    const int64_t plane = hin * win;
+    for (int64_t c0 = 0; c0 < cin; c0 += NCG) {
+        const int nc = int(cin - c0 < NCG ? cin - c0 : NCG);
+        __syncthreads();
+        for (int e = t; e < NCG * NS8 * NS8; e += 256) {
+            const int c = e / (NS8 * NS8), r = e - c * (NS8 * NS8), yy = r / NS8, xx = r - yy * NS8;
+            const int64_t iy = iy0 + yy, ix = ix0 + xx;
+            float v = 0.0f;
+            if (c < nc && iy >= 0 && iy < hin && ix >= 0 && ix < win) {
+                v = load_as_float<IN_T>(in, (img * cin + c0 + c) * plane + iy * win + ix);
+                if constexpr (IN_GELU) v = gelu(v);
+            }
+            tile[c][yy][xx] = v;
+        }
+        __syncthreads();
+        for (int cc = wave; cc < nc; cc += 4) {
+            const uint32_t *wc = reinterpret_cast<const uint32_t *>(w + ((c0 + cc) * 25) * ocs);
+#pragma unroll
+            for (int k = 0; k < 25; ++k) {
+                const float x = tile[cc][ty + k / 5][tx + k % 5];
+                const uint32_t *wk = wc + k * (ocs >> 1);
+                const uint32_t p0 = wk[0], p1 = wk[1];
+                acc[0] = fmaf(x, __uint_as_float(p0 << 16), acc[0]);
+                acc[1] = fmaf(x, __uint_as_float(p0 & 0xFFFF0000u), acc[1]);
+                acc[2] = fmaf(x, __uint_as_float(p1 << 16), acc[2]);
+                acc[3] = fmaf(x, __uint_as_float(p1 & 0xFFFF0000u), acc[3]);
+            }
+        }
+    }
+#pragma unroll
+    for (int o = 0; o < 4; ++o) red[wave][lane][o] = acc[o];
+    __syncthreads();
+    if (wave != 0) return;
+    const int64_t oy = oy0 + ty, ox = ox0 + tx;
+    if (oy >= hout || ox >= wout) return;
+#pragma unroll
+    for (int o = 0; o < 4; ++o) {
+        const int64_t co = o;
+        if (co >= cout) break;
+        const float sum = ((red[0][lane][o] + red[1][lane][o]) + red[2][lane][o]) + red[3][lane][o];
+        const int64_t idx = ((img * cout + co) * hout + oy) * wout + ox;
+        if constexpr (OUT == 0) {
+            const uint16_t z = f2bf(sum + bias[co]);
+            static_cast<uint16_t *>(out)[idx] = z;
+            if (act) act[idx] = f2bf(gelu(bf2f(z)));
+        } else if constexpr (OUT == 1) {
+            static_cast<float *>(out)[idx] = sum + bias[co];
+        } else {
+            static_cast<float *>(out)[idx] = sum * gelu_grad(bf2f(zg[idx]));
+        }
+    }
+}
+
 // The 64 → 64-channel layers on the matrix cores: an implicit GEMM, output pixels × output channels
 // over K = 25 taps × 64 input channels, with v_mfma_f32_16x16x32_bf16. The block's 20×20 input tile
 // is staged in LDS channel-fastest ([pixel][channel], pixel stride MPS), so the 8 consecutive K values
@@ -269,19 +343,21 @@ __global__ void k_pack_mfma(const float *__restrict__ p, uint16_t *wmf, uint16_t
     if (wmb) wmb[((24 - k) * MCH + i) * MCH + o] = h;
 }
 
-// Weight and bias gradients of a valid 5×5 convolution. Block = 16 output × 16 input channels (one
-// (o, i) pair per lane, its 25 taps in registers) over a range of 16×16 output tiles of one image;
-// each tile stages dZ (16 channels × 256 pixels) and the input (16 channels × 20×20) in LDS, and a
-// lane walks each output row with a 5×5 register window of the input, 6 LDS reads per 25 FMAs.
-// The block's sums go to its own slot of `part` (deterministic; k_wgrad_reduce adds the slots).
-template <int IN_T, int IN_GELU>
+// Weight and bias gradients of a valid 5×5 convolution. Block = OG output × IG input channels (one
+// (o, i) pair per lane, its 25 taps in registers; OG·IG = 256: 16 × 16, or 64 × 4 for an input of at
+// most 4 channels so that no lane idles) over a range of 16×16 output tiles of one image; each tile
+// stages dZ (OG channels × 256 pixels) and the input (IG channels × 20×20) in LDS, and a lane walks
+// each output row with a 5×5 register window of the input, 6 LDS reads per 25 FMAs. The block's sums
+// go to its own slot of `part` (deterministic; k_wgrad_reduce adds the slots).
+template <int OG, int IG, int IN_T, int IN_GELU>
 __global__ __launch_bounds__(256) void k_conv5_wgrad(const float *__restrict__ dz, const void *__restrict__ in,
                                                      float *__restrict__ part, int64_t cin, int64_t cout,
                                                      int64_t hout, int64_t wout, int tiles_x, int ntiles,
                                                      int tiles_per_block, int nig) {
-    __shared__ float ds[WG * DSTR];
-    __shared__ float as[WG * ASTR];
-    const int t = threadIdx.x, ol = t >> 4, il = t & 15;
+    static_assert(OG * IG == 256, "one (o, i) pair per lane");
+    __shared__ float ds[OG * DSTR];
+    __shared__ float as[IG * ASTR];
+    const int t = threadIdx.x, ol = t / IG, il = t % IG;
     const int64_t og = blockIdx.x / nig, ig = blockIdx.x % nig;
     const int64_t img = blockIdx.y;
     const int split = blockIdx.z;
@@ -302,28 +378,35 @@ __global__ __launch_bounds__(256) void k_conv5_wgrad(const float *__restrict__ d
             {
                 const int64_t oy = oy0 + (t >> 4), ox = ox0 + (t & 15);
                 const bool ok = oy < hout && ox < wout;
-                const float *src = dz + (img * cout + og * WG) * oplane + (ok ? oy * wout + ox : 0);
-                float v[WG];
+                const float *src = dz + (img * cout + og * OG) * oplane + (ok ? oy * wout + ox : 0);
+                // 16 planes at a time (their loads in flight together), so 64 output channels do not hold
+                // 64 registers
+                constexpr int CH = OG < 16 ? OG : 16;
+#pragma unroll 1
+                for (int c0 = 0; c0 < OG; c0 += CH) {
+                    float v[CH];
 #pragma unroll
-                for (int c = 0; c < WG; ++c) v[c] = (ok && og * WG + c < cout) ? src[c * oplane] : 0.0f;
+                    for (int c = 0; c < CH; ++c)
+                        v[c] = (ok && og * OG + c0 + c < cout) ? src[(c0 + c) * oplane] : 0.0f;
 #pragma unroll
-                for (int c = 0; c < WG; ++c) ds[c * DSTR + t] = v[c];
+                    for (int c = 0; c < CH; ++c) ds[(c0 + c) * DSTR + t] = v[c];
+                }
             }
-            // input: lane owns pixels r = t, t + 256 of the 20×20 tile, 16 channel planes
+            // input: lane owns pixels r = t, t + 256 of the 20×20 tile, IG channel planes
             const int64_t iplane = hin * win;
             for (int r = t; r < CS * CS; r += 256) {
                 const int yy = r / CS, xx = r - yy * CS;
                 const int64_t iy = oy0 + yy, ix = ox0 + xx;
                 const bool ok = iy < hin && ix < win;
-                const int64_t off = ok ? (img * cin + ig * WG) * iplane + iy * win + ix : 0;
-                float v[WG];
+                const int64_t off = ok ? (img * cin + ig * IG) * iplane + iy * win + ix : 0;
+                float v[IG];
 #pragma unroll
-                for (int c = 0; c < WG; ++c) {
-                    v[c] = (ok && ig * WG + c < cin) ? load_as_float<IN_T>(in, off + c * iplane) : 0.0f;
+                for (int c = 0; c < IG; ++c) {
+                    v[c] = (ok && ig * IG + c < cin) ? load_as_float<IN_T>(in, off + c * iplane) : 0.0f;
                     if constexpr (IN_GELU) v[c] = gelu(v[c]);
                 }
 #pragma unroll
-                for (int c = 0; c < WG; ++c) as[c * ASTR + yy * (CS + 1) + xx] = v[c];
+                for (int c = 0; c < IG; ++c) as[c * ASTR + yy * (CS + 1) + xx] = v[c];
             }
         }
         __syncthreads();
@@ -356,7 +439,7 @@ __global__ __launch_bounds__(256) void k_conv5_wgrad(const float *__restrict__ d
     // slot layout: [slot][cout·cin·25 weights | cout biases]
     const int64_t nw = cout * cin * 25;
     float *slot = part + (img * gridDim.z + split) * (nw + cout);
-    const int64_t co = og * WG + ol, ci = ig * WG + il;
+    const int64_t co = og * OG + ol, ci = ig * IG + il;
     if (co < cout && ci < cin) {
 #pragma unroll
         for (int k = 0; k < 25; ++k) slot[(co * cin + ci) * 25 + k] = acc[k];
@@ -369,21 +452,28 @@ __global__ __launch_bounds__(256) void k_conv5_wgrad(const float *__restrict__ d
 // one kernel row ky (5 taps) and a range of 16 × 8-pixel output tiles of one image; wave w owns input
 // channels 16w…16w+15 for all 64 output channels (4 M tiles) and the 5 taps: 20 accumulator tiles.
 // K runs over the tile's pixels, 32 per MFMA: lane group g supplies 8 consecutive pixels of row
-// g / 2, columns 8·(g % 2)…+7. dZ (staged BF16, [o][pixel]) gives aligned 16-byte fragments; the
-// input ([i][row][col], staged with the GELU applied) is read at the tap's shift, 2 bytes at a time.
-// Bias gradients come from ky = 0 blocks (wave 0 sums its dZ fragments). Sums go to the block's slot
-// of `part` as in k_conv5_wgrad.
+// g / 2, columns 8·(g % 2)…+7. dZ (staged BF16, [o][pixel]) gives aligned 16-byte fragments. The
+// input ([i][row][col], staged with the GELU applied, rows 16-byte aligned) is read as the two aligned
+// 16-byte halves of columns 8·(g % 2)…+15 once per step; the five taps' shifted fragments are
+// selected from those eight dwords (odd shifts joined across dword pairs with v_alignbyte), so a step
+// costs the lane 2 LDS reads for its 5 B fragments instead of 40 two-byte reads. Bias gradients come
+// from ky = 0 blocks (wave 0 sums its dZ fragments). Sums go to the block's slot of `part`, tap-major
+// ([tap][o][i], so a store instruction writes 16 consecutive floats per lane group) — k_wgrad_reduce
+// turns them into [o][i][tap].
 constexpr int WTY = 8;                 // output rows per wgrad tile (tile = WTY × 16 pixels)
 constexpr int DZS = WTY * CT + 8;      // LDS stride of a dZ channel (BF16)
-constexpr int AXS = CT + 4 + 4;        // LDS row pitch of the staged input (BF16; 20 used)
+constexpr int AXP = 24;                // LDS row pitch of the staged input (BF16, 48 B; 20 used)
+constexpr int AXC = WTY * AXP + 8;     // LDS stride of an input channel (400 B: the 16 channels a wave's
+                                       // lanes read start on distinct 16-byte bank groups)
 static_assert((MCH * WTY * 2) % 256 == 0 && (MCH * WTY * 5) % 256 == 0, "wgrad staging trip counts");
+static_assert((AXP * 2) % 16 == 0 && (AXC * 2) % 16 == 0, "staged input rows 16-byte aligned");
 
 template <int IN_T, int IN_GELU>
 __global__ __launch_bounds__(256) void k_conv5_wgrad_mfma(const float *__restrict__ dz, const void *__restrict__ in,
                                                           float *__restrict__ part, int64_t hout, int64_t wout,
                                                           int tiles_x, int ntiles, int tiles_per_block) {
     __shared__ __attribute__((aligned(16))) uint16_t dzs[MCH * DZS];
-    __shared__ __attribute__((aligned(16))) uint16_t axs[MCH * WTY * AXS];
+    __shared__ __attribute__((aligned(16))) uint16_t axs[MCH * AXC];
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63, g = lane >> 4, r16 = lane & 15;
     const int ky = blockIdx.x;
     const int64_t img = blockIdx.y;
@@ -410,8 +500,14 @@ __global__ __launch_bounds__(256) void k_conv5_wgrad_mfma(const float *__restric
                 const int64_t oy = oy0 + rr, oxs = ox0 + 8 * hf;
                 const float *src = dz + (img * MCH + c) * oplane + oy * wout + oxs;
                 float v[8];
+                if (oy < hout && oxs + 8 <= wout && (reinterpret_cast<uintptr_t>(src) & 15) == 0) {
+                    // the segment whole and 16-byte aligned: two vector loads
+                    const float4 p0 = reinterpret_cast<const float4 *>(src)[0], p1 = reinterpret_cast<const float4 *>(src)[1];
+                    v[0] = p0.x, v[1] = p0.y, v[2] = p0.z, v[3] = p0.w, v[4] = p1.x, v[5] = p1.y, v[6] = p1.z, v[7] = p1.w;
+                } else {
 #pragma unroll
-                for (int j = 0; j < 8; ++j) v[j] = (oy < hout && oxs + j < wout) ? src[j] : 0.0f;
+                    for (int j = 0; j < 8; ++j) v[j] = (oy < hout && oxs + j < wout) ? src[j] : 0.0f;
+                }
                 *reinterpret_cast<uint4 *>(&dzs[c * DZS + rr * CT + 8 * hf]) =
                     uint4{pack_bf2(v[0], v[1]), pack_bf2(v[2], v[3]), pack_bf2(v[4], v[5]), pack_bf2(v[6], v[7])};
             }
@@ -424,12 +520,21 @@ __global__ __launch_bounds__(256) void k_conv5_wgrad_mfma(const float *__restric
                 const int64_t iy = oy0 + yy + ky, ixs = ox0 + 4 * q;
                 const int64_t base = (img * MCH + c) * iplane + iy * win + ixs;
                 float v[4];
+                if (IN_T == CGEF_BF16 && iy < hin && ixs + 4 <= win && (base & 3) == 0 &&
+                    (reinterpret_cast<uintptr_t>(in) & 7) == 0) {
+                    // four BF16 values, 8-byte aligned: one load
+                    const uint2 q4 = *reinterpret_cast<const uint2 *>(static_cast<const uint16_t *>(in) + base);
+                    v[0] = __uint_as_float(q4.x << 16), v[1] = __uint_as_float(q4.x & 0xFFFF0000u);
+                    v[2] = __uint_as_float(q4.y << 16), v[3] = __uint_as_float(q4.y & 0xFFFF0000u);
+                } else {
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    v[j] = (iy < hin && ixs + j < win) ? load_as_float<IN_T>(in, base + j) : 0.0f;
-                    if constexpr (IN_GELU) v[j] = gelu(v[j]);
+                    for (int j = 0; j < 4; ++j) v[j] = (iy < hin && ixs + j < win) ? load_as_float<IN_T>(in, base + j) : 0.0f;
                 }
-                *reinterpret_cast<uint2 *>(&axs[(c * WTY + yy) * AXS + 4 * q]) =
+                if constexpr (IN_GELU) {
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) v[j] = gelu(v[j]);
+                }
+                *reinterpret_cast<uint2 *>(&axs[c * AXC + yy * AXP + 4 * q]) =
                     uint2{pack_bf2(v[0], v[1]), pack_bf2(v[2], v[3])};
             }
         }
@@ -447,12 +552,20 @@ __global__ __launch_bounds__(256) void k_conv5_wgrad_mfma(const float *__restric
 #pragma unroll
                     for (int j = 0; j < 8; ++j) bsum[mt] += float(a[mt][j]);
             }
-            const uint16_t *arow = &axs[((wave * 16 + r16) * WTY + row) * AXS + col];
+            // columns col…col+15 of the lane's input row (only col…col+11 are ever selected)
+            const uint16_t *arow = &axs[(wave * 16 + r16) * AXC + row * AXP + col];
+            const uint4 lo = *reinterpret_cast<const uint4 *>(arow), hi = *reinterpret_cast<const uint4 *>(arow + 8);
+            const uint32_t dw[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
 #pragma unroll
             for (int kx = 0; kx < 5; ++kx) {
-                bf16x8_t b;
+                // elements kx…kx+7: dwords kx/2… when kx is even, else the upper half of dword (kx−1)/2 + j
+                // joined with the lower half of the next
+                uint32_t f[4];
 #pragma unroll
-                for (int j = 0; j < 8; ++j) b[j] = __builtin_bit_cast(__bf16, arow[kx + j]);
+                for (int j = 0; j < 4; ++j)
+                    f[j] = (kx & 1) ? __builtin_amdgcn_alignbyte(dw[(kx + 1) / 2 + j], dw[(kx - 1) / 2 + j], 2)
+                                    : dw[kx / 2 + j];
+                const bf16x8_t b = __builtin_bit_cast(bf16x8_t, uint4{f[0], f[1], f[2], f[3]});
 #pragma unroll
                 for (int mt = 0; mt < 4; ++mt)
                     acc[kx][mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mt], b, acc[kx][mt], 0, 0, 0);
@@ -460,7 +573,8 @@ __global__ __launch_bounds__(256) void k_conv5_wgrad_mfma(const float *__restric
         }
     }
 
-    // D[m = 4g + i][n = r16]: output channel 16·mt + 4g + i, input channel 16·wave + r16
+    // D[m = 4g + i][n = r16]: output channel 16·mt + 4g + i, input channel 16·wave + r16; slot layout
+    // [tap][o][i]
     const int64_t nw = int64_t(MCH) * MCH * 25;
     float *slot = part + (img * gridDim.z + split) * (nw + MCH);
     const int ci = wave * 16 + r16;
@@ -471,7 +585,7 @@ __global__ __launch_bounds__(256) void k_conv5_wgrad_mfma(const float *__restric
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int co = mt * 16 + 4 * g + i;
-                slot[(int64_t(co) * MCH + ci) * 25 + ky * 5 + kx] = acc[kx][mt][i];
+                slot[(int64_t(ky * 5 + kx) * MCH + co) * MCH + ci] = acc[kx][mt][i];
             }
     if (wave == 0 && ky == 0) {
         // lanes r16 of the four groups hold partial sums of output channel 16·mt + r16
@@ -485,22 +599,37 @@ __global__ __launch_bounds__(256) void k_conv5_wgrad_mfma(const float *__restric
     }
 }
 
-// Adds the per-block slots: 64 outputs × 4 slot phases per block, the phases combined in LDS.
+// Adds the per-block slots: RDO outputs × RDP slot phases per block (phase p sums slots p, p + RDP, …),
+// the phases combined in LDS in phase order (deterministic). The slots hold [o][i][tap] (VALU kernel)
+// or, with tapmajor, [tap][o][i] (matrix-core kernel: cin input channels); dw is always [o][i][tap].
+// 64 × 4 where the outputs alone give many blocks (coalesced 256-byte rows), 16 × 16 where they do not.
+template <int RDO, int RDP>
 __global__ __launch_bounds__(256) void k_wgrad_reduce(const float *__restrict__ part, int64_t nslots, int64_t nw,
-                                                      int64_t cout, float *__restrict__ dw, float *__restrict__ db) {
-    __shared__ float red[4][64];
-    const int64_t i = int64_t(blockIdx.x) * 64 + (threadIdx.x & 63);
-    const int ph = threadIdx.x >> 6;
+                                                      int64_t cout, float *__restrict__ dw, float *__restrict__ db,
+                                                      int tapmajor, int64_t cin) {
+    __shared__ float red[RDP][RDO];
+    const int o = threadIdx.x % RDO, ph = threadIdx.x / RDO;
+    const int64_t i = int64_t(blockIdx.x) * RDO + o;
     const int64_t len = nw + cout;
     float s = 0.0f;
     if (i < len)
-        for (int64_t p = ph; p < nslots; p += 4) s += part[p * len + i];
-    red[ph][threadIdx.x & 63] = s;
+        for (int64_t p = ph; p < nslots; p += RDP) s += part[p * len + i];
+    red[ph][o] = s;
     __syncthreads();
     if (ph == 0 && i < len) {
-        s = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
-        if (i < nw) dw[i] = s;
-        else db[i - nw] = s;
+        s = 0.0f;
+#pragma unroll
+        for (int q = 0; q < RDP; ++q) s += red[q][o];
+        if (i < nw) {
+            if (tapmajor) {
+                const int64_t tap = i / (cout * cin), oi = i - tap * (cout * cin);
+                dw[oi * 25 + tap] = s;
+            } else {
+                dw[i] = s;
+            }
+        } else {
+            db[i - nw] = s;
+        }
     }
 }
 
@@ -759,8 +888,9 @@ void wgrad_split(bool mfma, int64_t n, int64_t cin, int64_t cout, int64_t hout, 
                  int &ntiles, int &tpb, int &nsplit, int &nig, int &nog) {
     tiles_x = int((wout + CT - 1) / CT);
     ntiles = tiles_x * int((hout + (mfma ? WTY : CT) - 1) / (mfma ? WTY : CT));
-    nig = int((cin + WG - 1) / WG);
-    nog = int((cout + WG - 1) / WG);
+    const int ig = cin <= 4 ? 4 : WG, og = cin <= 4 ? 64 : WG;   // k_conv5_wgrad's block shape (OG × IG)
+    nig = int((cin + ig - 1) / ig);
+    nog = int((cout + og - 1) / og);
     const int64_t base = mfma ? 5 * n : n * nig * nog;
     int64_t want = std::max<int64_t>(1, (mfma ? 1024 : 2048) / std::max<int64_t>(base, 1));
     want = std::min<int64_t>(want, ntiles);
@@ -796,7 +926,10 @@ cgef_status cgef_conv5(const void *in, int32_t in_dtype, int32_t in_gelu, int64_
             out_kind = out_dtype == CGEF_BF16 ? 0 : 1;
         }
         need(!act || out_kind == 0, CGEF_ERR_ARGUMENT, "cgef_conv5: the activation is written with a BF16 output");
-        const int ocb = cout <= 4 ? 4 : (cout <= 16 ? 16 : 64);
+        // output channels per block: 64 lanes' worth of accumulators for wide layers; 16 where the input has
+        // few channels (the 4 → 64 layer, the data gradient through the 64 → 3 layer), which gives four
+        // times the blocks for a short channel loop
+        const int ocb = cout <= 4 ? 4 : (cout <= 16 || cin <= 16 ? 16 : 64);
         need(ocs % ocb == 0 && ocs >= cout, CGEF_ERR_DIMENSION,
              "cgef_conv5: weight layout stride must cover cout in whole output-channel blocks (4, 16 or 64)");
         const int tiles_x = int((wout + CT - 1) / CT);
@@ -804,9 +937,30 @@ cgef_status cgef_conv5(const void *in, int32_t in_dtype, int32_t in_gelu, int64_
         need(ntiles <= 0x7fffffff, CGEF_ERR_DIMENSION, "cgef_conv5: too many tiles");
         dim3 g(unsigned(ntiles), unsigned((cout + ocb - 1) / ocb), unsigned(n));
         hipStream_t st = static_cast<hipStream_t>(stream);
-        if (ocb == 4)
-            conv_in<4>(in_dtype, in_gelu, out_kind, g, st, in, w, bias, zgrad, out, act, cin, hin, win, cout, ocs, hout, wout,
-                       pad, tiles_x);
+        if (ocb == 4 && ntiles * n < 2048) {
+            // few 16×16 tiles (a training batch): 8×8 tiles, the input channels split over the waves
+            // (k_conv5_narrow); large fields keep k_conv5's 16×16 tiles
+            const int tx8 = int((wout + NT8 - 1) / NT8);
+            const int64_t nt8 = int64_t(tx8) * ((hout + NT8 - 1) / NT8);
+            need(nt8 <= 0x7fffffff, CGEF_ERR_DIMENSION, "cgef_conv5: too many tiles");
+            const dim3 g8(unsigned(nt8), 1, unsigned(n));
+            auto go = [&](auto kern) {
+                hipLaunchKernelGGL(kern, g8, dim3(256), 0, st, in, w, bias, zgrad, out, act, cin, hin, win, cout, ocs, hout,
+                                   wout, pad, tx8);
+            };
+#define CGEF_NARROW(IT, IG)                                                  \
+    do {                                                                     \
+        if (out_kind == 0) go(k_conv5_narrow<IT, IG, 0>);                    \
+        else if (out_kind == 1) go(k_conv5_narrow<IT, IG, 1>);               \
+        else go(k_conv5_narrow<IT, IG, 2>);                                  \
+    } while (0)
+            if (in_dtype == CGEF_BF16 && in_gelu) CGEF_NARROW(CGEF_BF16, 1);
+            else if (in_dtype == CGEF_BF16) CGEF_NARROW(CGEF_BF16, 0);
+            else CGEF_NARROW(CGEF_F32, 0);
+#undef CGEF_NARROW
+        } else if (ocb == 4)
+            conv_in<4>(in_dtype, in_gelu, out_kind, g, st, in, w, bias, zgrad, out, act, cin, hin, win, cout, ocs, hout,
+                       wout, pad, tiles_x);
         else if (ocb == 16)
             conv_in<16>(in_dtype, in_gelu, out_kind, g, st, in, w, bias, zgrad, out, act, cin, hin, win, cout, ocs, hout,
                         wout, pad, tiles_x);
@@ -917,19 +1071,28 @@ cgef_status cgef_conv5_wgrad(const float *dz, const void *in, int32_t in_dtype, 
                                    tiles_x, ntiles, tpb);
         } else {
             dim3 g(unsigned(nog * nig), unsigned(n), unsigned(nsplit));
-            if (in_dtype == CGEF_BF16 && in_gelu)
-                hipLaunchKernelGGL((k_conv5_wgrad<CGEF_BF16, 1>), g, dim3(256), 0, st, dz, in, scratch, cin, cout, hout,
-                                   wout, tiles_x, ntiles, tpb, nig);
-            else if (in_dtype == CGEF_BF16)
-                hipLaunchKernelGGL((k_conv5_wgrad<CGEF_BF16, 0>), g, dim3(256), 0, st, dz, in, scratch, cin, cout, hout,
-                                   wout, tiles_x, ntiles, tpb, nig);
-            else
-                hipLaunchKernelGGL((k_conv5_wgrad<CGEF_F32, 0>), g, dim3(256), 0, st, dz, in, scratch, cin, cout, hout,
-                                   wout, tiles_x, ntiles, tpb, nig);
+            auto go = [&](auto kern) {
+                hipLaunchKernelGGL(kern, g, dim3(256), 0, st, dz, in, scratch, cin, cout, hout, wout, tiles_x, ntiles, tpb,
+                                   nig);
+            };
+#define CGEF_WG(OGV, IGV)                                                                        \
+    do {                                                                                         \
+        if (in_dtype == CGEF_BF16 && in_gelu) go(k_conv5_wgrad<OGV, IGV, CGEF_BF16, 1>);         \
+        else if (in_dtype == CGEF_BF16) go(k_conv5_wgrad<OGV, IGV, CGEF_BF16, 0>);               \
+        else go(k_conv5_wgrad<OGV, IGV, CGEF_F32, 0>);                                           \
+    } while (0)
+            if (cin <= 4) CGEF_WG(64, 4);
+            else CGEF_WG(WG, WG);
+#undef CGEF_WG
         }
         check_launch();
-        hipLaunchKernelGGL(k_wgrad_reduce, dim3(unsigned((nw + cout + 63) / 64)), dim3(256), 0, st, scratch, nslots, nw,
-                           cout, dw, db);
+        const int64_t len = nw + cout;
+        if ((len + 63) / 64 >= 1024)
+            hipLaunchKernelGGL((k_wgrad_reduce<64, 4>), dim3(unsigned((len + 63) / 64)), dim3(256), 0, st, scratch, nslots,
+                               nw, cout, dw, db, mfma ? 1 : 0, cin);
+        else
+            hipLaunchKernelGGL((k_wgrad_reduce<16, 16>), dim3(unsigned((len + 15) / 16)), dim3(256), 0, st, scratch,
+                               nslots, nw, cout, dw, db, mfma ? 1 : 0, cin);
         check_launch();
     });
 }

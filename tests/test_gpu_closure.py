@@ -245,3 +245,29 @@ def test_gelu_on_load_matches_stored_activation(mfma):
     CL.conv5_wgrad(gw2, gb2, dz, net.A[0], net.scratch, mfma=mfma)
     assert rel_l2(N(gw), N(gw2)) <= 5e-3
     np.testing.assert_allclose(N(gb), N(gb2), rtol=1e-6, atol=1e-6)
+
+
+@pytest.mark.parametrize("n,h,w", [(2, 40, 37), (1, 748, 748)])
+def test_output_layer_both_tilings_match_fp64_conv(n, h, w):
+    """The 64 → 3 layer runs on 8×8 tiles with the input channels split over the waves when the batch
+    has few 16×16 tiles (n·tiles < 2048: the first case) and on k_conv5's 16×16 tiles otherwise (the
+    second: 1 × 47² tiles); both against torch's FP64 convolution of the same BF16 values."""
+    print(f"closure 64->3 layer n={n} {h}x{w}", flush=True)
+    net = CL.ClosureNet((64, 3), seed=7, device=0, mfma=False)
+    rng = np.random.Generator(np.random.PCG64(70))
+    Ws, _ = net.state()
+    b = [0.1 * rng.standard_normal(3)]
+    net.load_state(Ws, b)
+    x = torch.tensor(rng.standard_normal((n, 64, h, w)), dtype=torch.float32, device=DEV).to(torch.bfloat16)
+    out = torch.empty((n, 3, h - 4, w - 4), dtype=torch.float32, device=DEV)
+    CL.conv5(out, x, net.wf[0], net.ocs_f[0], net.b[0])
+    xd = x.float().cpu().double().numpy()
+    Wd = torch.tensor(np.asarray(Ws[0]), dtype=torch.float32).to(torch.bfloat16).double().numpy().reshape(3, 64, 5, 5)
+    ho, wo = h - 4, w - 4
+    ref = np.zeros((n, 3, ho, wo)) + np.asarray(b[0]).reshape(1, 3, 1, 1)
+    for ky in range(5):           # valid cross-correlation, one tap at a time (a channel contraction each)
+        for kx in range(5):
+            ref += np.einsum("nihw,oi->nohw", xd[:, :, ky:ky + ho, kx:kx + wo], Wd[:, :, ky, kx], optimize=True)
+    err = rel_l2(N(out), ref)
+    print(f"  rel-L2 vs FP64 conv {err:.2e}", flush=True)
+    assert err <= 1e-4
