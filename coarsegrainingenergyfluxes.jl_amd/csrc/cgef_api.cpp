@@ -1,5 +1,6 @@
 // C ABI of libcgef_hip.so (declared in include/cgef.h).
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -623,7 +624,7 @@ cgef_status cgef_grid_create(const cgef_grid_desc *desc, int32_t device, cgef_gr
             }
             if (g->geometry == CGEF_SPHERICAL) {
                 const int64_t Nx = g->n[0], Ny = g->n[1];
-                std::vector<double> tr(2 * Nx + 2 * Ny);
+                std::vector<double> tr(2 * Nx + 4 * Ny);
                 for (int64_t i = 0; i < Nx; i++) {
                     tr[i] = std::sin(g->coord[0][i]);
                     tr[Nx + i] = std::cos(g->coord[0][i]);
@@ -631,6 +632,10 @@ cgef_status cgef_grid_create(const cgef_grid_desc *desc, int32_t device, cgef_gr
                 for (int64_t j = 0; j < Ny; j++) {
                     tr[2 * Nx + j] = std::sin(g->coord[1][j]);
                     tr[2 * Nx + Ny + j] = std::cos(g->coord[1][j]);
+                    // the strain's metric factors of the row: 1/(R cosφ) and tanφ/R (0 at the poles)
+                    const double sp = tr[2 * Nx + j], cp = tr[2 * Nx + Ny + j];
+                    tr[2 * Nx + 2 * Ny + j] = std::fabs(cp) < std::sqrt(DBL_EPSILON) ? 0.0 : 1.0 / (g->R * cp);
+                    tr[2 * Nx + 3 * Ny + j] = std::fabs(cp) > 1e-12 ? sp / (g->R * cp) : 0.0;
                 }
                 g->trig_d = upload(tr);
             }

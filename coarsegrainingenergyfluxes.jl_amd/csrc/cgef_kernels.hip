@@ -465,27 +465,63 @@ __device__ __forceinline__ int64_t xcd_block(int64_t nblk) {
 
 static unsigned xcd_grid(int64_t nblk) { return (unsigned)(8 * ((nblk + 7) / 8)); }
 
+// The same stencil with 32-bit node indices, for the strain of one slice (its points fit 31 bits).
+struct Stencil32 {
+    int q0, q1, q2;
+    double c0, c1, c2;
+};
+
+__device__ __forceinline__ Stencil32 stencil32(const GeomArgs &G, int axis, int kind, int t, int p, int stride) {
+    Stencil32 S{t, t, t, 0.0, 0.0, 0.0};
+    if (kind == STENCIL_NONE) return S;
+    const int n = (int)G.n[axis];
+    const int o0 = kind == STENCIL_BACKWARD3 ? -2 : ((kind == STENCIL_CENTRED || kind == STENCIL_BACKWARD2) ? -1 : 0);
+    const int sb = kind == STENCIL_CENTRED ? 0 : kind == STENCIL_FORWARD3 ? 3 : kind == STENCIL_BACKWARD3 ? 6
+                 : kind == STENCIL_FORWARD2 ? 9 : 11;
+    auto at = [&](int o) {   // t + (wrapped node − p)·stride
+        const int q = p + o;
+        return t + (o + (q < 0 ? n : 0) - (q >= n ? n : 0)) * stride;
+    };
+    const double *w = G.dcoef[axis] + p;  // slot s at w[s·n]
+    S.q0 = at(o0);
+    S.q1 = at(o0 + 1);
+    S.c0 = w[sb * n];
+    S.c1 = w[(sb + 1) * n];
+    if (kind <= STENCIL_BACKWARD3) {
+        S.q2 = at(o0 + 2);
+        S.c2 = w[(sb + 2) * n];
+    }
+    return S;
+}
+
+template <typename F>
+__device__ __forceinline__ double apply_stencil(const Stencil32 &S, F f) {
+    return S.c0 * f(S.q0) + S.c1 * f(S.q1) + S.c2 * f(S.q2);
+}
+
 // Strain S̄ (with the spherical curvature terms, Diagnostics.jl:652-669), contraction
 // (Diagnostics.jl:460-466, 683-698, 848-855) and −S̄:τ at one point t = (i, j, k) of slice b; the point's
 // kinetic energy density ½ |ū|² A (Diagnostics.jl:1125-1134; the radial component is excluded on the
 // sphere) goes to e. Inactive points give 0 for both. The loads at t (ū, τ) do not depend on the
 // point's code and go out with it; only the neighbours' indices and weights follow from the code.
+// Node indices are 32-bit within the slice; the sphere's row factors 1/(R cosφ) and tanφ/R come from
+// the grid's table (one uniform load each instead of two FP64 divisions per point).
 template <typename T, int MODE>
 __device__ __forceinline__ double strain_point(const GeomArgs &G, const double *__restrict__ U, const T *__restrict__ tb,
-                                               int64_t t, int64_t i, int64_t j, int64_t k, double &e) {
-    const int64_t P = G.npts, Nx = G.n[0], Ny = G.n[1];
+                                               int t, int i, int j, int k, double &e) {
+    const int P = (int)G.npts, Nx = (int)G.n[0], Ny = (int)G.n[1];
     const double *V = U + P, *W = U + 2 * P;
     const unsigned code = G.scode[t];
     const double ut = U[t], vt = V[t];
-    auto fu = [&](int64_t q) { return U[q]; };
-    auto fv = [&](int64_t q) { return V[q]; };
-    const Stencil sx = stencil_of(G, 0, code & 7, t, i, 1);
-    const Stencil sy = stencil_of(G, 1, (code >> 3) & 7, t, j, Nx);
+    auto fu = [&](int q) { return U[q]; };
+    auto fv = [&](int q) { return V[q]; };
+    const Stencil32 sx = stencil32(G, 0, code & 7, t, i, 1);
+    const Stencil32 sy = stencil32(G, 1, (code >> 3) & 7, t, j, Nx);
     double cc, v2;
     if (MODE == 2) {
         const double wt = W[t];
-        auto fw = [&](int64_t q) { return W[q]; };
-        const Stencil sz = stencil_of(G, 2, (code >> 6) & 7, t, k, Nx * Ny);
+        auto fw = [&](int q) { return W[q]; };
+        const Stencil32 sz = stencil32(G, 2, (code >> 6) & 7, t, k, Nx * Ny);
         const double Sxx = apply_stencil(sx, fu), Syy = apply_stencil(sy, fv), Szz = apply_stencil(sz, fw);
         const double Sxy = 0.5 * (apply_stencil(sy, fu) + apply_stencil(sx, fv));
         const double Sxz = 0.5 * (apply_stencil(sz, fu) + apply_stencil(sx, fw));
@@ -498,13 +534,12 @@ __device__ __forceinline__ double strain_point(const GeomArgs &G, const double *
         double dvdx = apply_stencil(sx, fv), dvdy = apply_stencil(sy, fv);
         double Sxx, Sxy, Syy;
         if (MODE == 1) {
-            const double sp = G.trig[2 * Nx + j], cp = G.trig[2 * Nx + Ny + j];
-            const double hx = (fabs(cp) < sqrt(DBL_EPSILON)) ? 0.0 : 1.0 / (G.R * cp);
+            const double hx = G.trig[2 * Nx + 2 * Ny + j], tf = G.trig[2 * Nx + 3 * Ny + j];
+            const double invR = 1.0 / G.R;   // uniform: once per wave
             dudx *= hx;
             dvdx *= hx;
-            dudy /= G.R;
-            dvdy /= G.R;
-            const double tf = fabs(cp) > 1e-12 ? sp / (G.R * cp) : 0.0;
+            dudy *= invR;
+            dvdy *= invR;
             Sxx = dudx - vt * tf;
             Syy = dvdy;
             Sxy = 0.5 * (dudy + dvdx + ut * tf);
@@ -533,7 +568,8 @@ __global__ __launch_bounds__(256) void k_strain_pi(GeomArgs G, const double *__r
     const RowChunk c = row_chunk(G, nchunk, item);
     double e = 0.0;
     if (c.i < G.n[0]) {
-        const double out = strain_point<T, MODE>(G, ubar + b * 3 * G.npts, tau + b * 6 * G.npts, c.t, c.i, c.j, c.k, e);
+        const double out = strain_point<T, MODE>(G, ubar + b * 3 * G.npts, tau + b * 6 * G.npts, (int)c.t, (int)c.i,
+                                                  (int)c.j, (int)c.k, e);
         Pi[b * pi_stride + c.t] = (T)(rho0 * out);
     }
     if (partials) {
@@ -546,6 +582,7 @@ int64_t strain_blocks(const GeomArgs &G) { return row_chunks(G) * G.n[1] * G.n[2
 
 void launch_strain_pi(const GeomArgs &G, int dtype, const double *ubar, const void *tau, void *Pi, double rho0,
                       int64_t nbatch, double *partials, int64_t npartials, hipStream_t s, int64_t pi_stride) {
+    if (G.npts * 6 >= ((int64_t)1 << 31)) fail(CGEF_ERR_DIMENSION, "the strain kernel indexes a slice in 32 bits");
     const int64_t nblk = strain_blocks(G);
     double *pp = (partials && nblk * nbatch <= npartials) ? partials : nullptr;
     dim3 grid(xcd_grid(nblk), (unsigned)nbatch);

@@ -23,7 +23,8 @@ struct GeomArgs {
     const uint16_t *scode; // derivative stencil code per cell (STENCIL_* kind per axis, STENCIL_ACTIVE)
     const double *coord[3];
     const double *width[3];
-    const double *trig;    // spherical: sinλ[Nx], cosλ[Nx], sinφ[Ny], cosφ[Ny]
+    const double *trig;    // spherical: sinλ[Nx], cosλ[Nx], sinφ[Ny], cosφ[Ny], then per row the strain's
+                           // 1/(R cosφ) [Ny] and tanφ/R [Ny] (0 within √ε, resp. 1e-12, of a pole)
     const double *dcoef[3];  // derivative stencil weights, DCOEF_SLOTS arrays of n[axis] each
     double active_area;
 };
