@@ -559,7 +559,7 @@ __global__ __launch_bounds__(256) void k_ifft_rows_staged(RingFft F, const doubl
 template <int CC, int PB, int MINW>
 __global__ __launch_bounds__(256, MINW) void k_contract(RingArgs A, const double2 *__restrict__ XF, double2 *__restrict__ OUT,
                                                         int64_t ncol, int out_rowmajor, const int32_t *__restrict__ items,
-                                                        int64_t nitems) {
+                                                        int64_t nitems, const int32_t *__restrict__ mcut) {
     extern __shared__ double2 st[];
     const int64_t Ny = A.ny, Nz = A.nz;
     const int64_t nyb = (Ny + 255) / 256;
@@ -577,6 +577,8 @@ __global__ __launch_bounds__(256, MINW) void k_contract(RingArgs A, const double
     const int tid = threadIdx.x;
     const int64_t j = j0 + tid;
     const bool valid = j < Ny;
+    // rows whose cut lies at or below m take part in the staging only: their mode m is never read
+    const bool act = valid && (!mcut || m < mcut[j + Ny * k]);
     const int band = valid ? A.band_of_row[j + Ny * k] : 0;
     const int span = 256 + A.dy_max - A.dy_min;
     const int nvc = (int)((ncol - c0) < CC ? (ncol - c0) : CC);  // valid columns in this chunk
@@ -596,7 +598,7 @@ __global__ __launch_bounds__(256, MINW) void k_contract(RingArgs A, const double
         const double *wrow = A.what + (mrow + o0) * A.nbands + band;
         double wn[PB];
 #pragma unroll
-        for (int u = 0; u < PB; u++) wn[u] = (valid && u < ng) ? wrow[(int64_t)u * A.nbands] : 0.0;
+        for (int u = 0; u < PB; u++) wn[u] = (act && u < ng) ? wrow[(int64_t)u * A.nbands] : 0.0;
         __syncthreads();
         // staging: a lane takes source-row offsets s = tid, tid + 256, … and loads the CC columns of
         // each together (CC independent loads in flight, no division by the span)
@@ -616,7 +618,7 @@ __global__ __launch_bounds__(256, MINW) void k_contract(RingArgs A, const double
             for (int c = 0; c < CC; c++) st[c * span + s] = v[c];
         }
         __syncthreads();
-        if (!valid) continue;
+        if (!act) continue;
         const double2 *src = st + tid + e0;
         for (int p0 = 0; p0 < ng; p0 += PB) {
             double w[PB];
@@ -648,11 +650,12 @@ __global__ __launch_bounds__(256, MINW) void k_contract(RingArgs A, const double
         const int nrv = (int)((Ny - j0) < 256 ? (Ny - j0) : 256);   // valid rows of this block
         for (int e = tid; e < nrv * CC; e += 256) {
             const int rr = e / CC, c = e - rr * CC;
-            if (c < nvc) OUT[(m * A.nrows + j0 + rr + Ny * k) * ncol + c0 + c] = st[e];
+            const int64_t row = j0 + rr + Ny * k;
+            if (c < nvc && (!mcut || m < mcut[row])) OUT[(m * A.nrows + row) * ncol + c0 + c] = st[e];
         }
         return;
     }
-    if (!valid) return;
+    if (!act) return;
     // fully unrolled with a predicate: a loop bounded by nvc would index acc dynamically and move the
     // accumulators to scratch memory
     const int64_t r = j + Ny * k;
@@ -1673,7 +1676,7 @@ void ring_contract(const RingPlan &R, const double2 *XF, double2 *OUT, int64_t n
     auto launch = [&](int variant) {
         auto go = [&](auto kern) {
             set_shmem((const void *)kern, sh);
-            hipLaunchKernelGGL(kern, grid, dim3(256), sh, s, R.args, XF, OUT, ncol, orm, wl, nwl);
+            hipLaunchKernelGGL(kern, grid, dim3(256), sh, s, R.args, XF, OUT, ncol, orm, wl, nwl, mcut);
         };
         if (variant == 3) launch_contract_mfma(R, XF, OUT, ncol, orm, cut, s);
         else if (variant == 0) go(k_contract<CC, 8, 1>);
