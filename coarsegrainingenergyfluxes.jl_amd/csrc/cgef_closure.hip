@@ -247,26 +247,57 @@ __global__ __launch_bounds__(256) void k_conv5_mfma(const void *__restrict__ in,
     uint4 wr0 = wsrc[0], wr1 = wsrc[1];
     reinterpret_cast<uint4 *>(wdst0)[0] = wr0;
     reinterpret_cast<uint4 *>(wdst0)[1] = wr1;
-    // staging: a lane owns pixels r = t, t + 256 of the 20×20 tile and walks the 64 channel planes,
-    // 8 channels per 16-byte LDS store (all address arithmetic hoisted out of the channel loop)
+    // staging: a lane takes (channel pair, tile row) items — lane t: pair t % 32, rows t / 32, + 8, + 16 —
+    // loads the pair's two 20-element row segments as five 4-element groups (8-byte loads for BF16,
+    // 16-byte for FP32) when the input and the tile origin are 4-element aligned — each group then
+    // lies wholly inside or wholly outside the input row — else element by element, and writes each
+    // pixel's two channels as one 4-byte LDS store (consecutive lanes: consecutive channel pairs)
     {
         const int64_t plane = hin * win;
-        for (int r = t; r < CS * CS; r += 256) {
-            const int yy = r / CS, xx = r - yy * CS;
-            const int64_t iy = iy0 + yy, ix = ix0 + xx;
-            const bool ok = iy >= 0 && iy < hin && ix >= 0 && ix < win;
-            const int64_t off = ok ? img * MCH * plane + iy * win + ix : 0;
-            uint4 *dst = reinterpret_cast<uint4 *>(xs + r * MPS);
+        const int cp = t & 31;
+        const bool vec_ok = (reinterpret_cast<uintptr_t>(in) & (IN_T == CGEF_BF16 ? 7 : 15)) == 0 &&
+                            (win & 3) == 0 && (ix0 & 3) == 0;
+        for (int yy = t >> 5; yy < CS; yy += 8) {
+            const int64_t iy = iy0 + yy;
+            float v[2][CS];
+            const bool row_in = iy >= 0 && iy < hin;
 #pragma unroll
-            for (int cg = 0; cg < MCH / 8; ++cg) {
-                float v[8];
+            for (int h = 0; h < 2; ++h) {
+                const int64_t base = (img * MCH + 2 * cp + h) * plane + iy * win + ix0;
+                if (vec_ok) {
 #pragma unroll
-                for (int j = 0; j < 8; ++j) {
-                    v[j] = ok ? load_as_float<IN_T>(in, off + (cg * 8 + j) * plane) : 0.0f;
-                    if constexpr (IN_GELU) v[j] = gelu(v[j]);
+                    for (int q = 0; q < CS / 4; ++q) {
+                        const bool ok = row_in && ix0 + 4 * q >= 0 && ix0 + 4 * q + 4 <= win;
+                        if constexpr (IN_T == CGEF_BF16) {
+                            const uint2 w4 =
+                                ok ? reinterpret_cast<const uint2 *>(static_cast<const uint16_t *>(in) + base)[q]
+                                   : uint2{0u, 0u};
+                            v[h][4 * q] = __uint_as_float(w4.x << 16);
+                            v[h][4 * q + 1] = __uint_as_float(w4.x & 0xFFFF0000u);
+                            v[h][4 * q + 2] = __uint_as_float(w4.y << 16);
+                            v[h][4 * q + 3] = __uint_as_float(w4.y & 0xFFFF0000u);
+                        } else {
+                            const float4 f4 =
+                                ok ? reinterpret_cast<const float4 *>(static_cast<const float *>(in) + base)[q]
+                                   : float4{0.0f, 0.0f, 0.0f, 0.0f};
+                            v[h][4 * q] = f4.x, v[h][4 * q + 1] = f4.y, v[h][4 * q + 2] = f4.z, v[h][4 * q + 3] = f4.w;
+                        }
+                    }
+                } else {
+#pragma unroll
+                    for (int xx = 0; xx < CS; ++xx) {
+                        const int64_t ix = ix0 + xx;
+                        v[h][xx] = row_in && ix >= 0 && ix < win ? load_as_float<IN_T>(in, base + xx) : 0.0f;
+                    }
                 }
-                dst[cg] = uint4{pack_bf2(v[0], v[1]), pack_bf2(v[2], v[3]), pack_bf2(v[4], v[5]), pack_bf2(v[6], v[7])};
+                if constexpr (IN_GELU) {
+#pragma unroll
+                    for (int xx = 0; xx < CS; ++xx) v[h][xx] = gelu(v[h][xx]);
+                }
             }
+            uint32_t *dst = reinterpret_cast<uint32_t *>(xs + (yy * CS) * MPS) + cp;
+#pragma unroll
+            for (int xx = 0; xx < CS; ++xx) dst[xx * (MPS / 2)] = pack_bf2(v[0][xx], v[1][xx]);
         }
     }
     __syncthreads();
@@ -284,6 +315,9 @@ __global__ __launch_bounds__(256) void k_conv5_mfma(const void *__restrict__ in,
             wr0 = wsrc[(tap + 1) * (MCH * MCH / 8)];
             wr1 = wsrc[(tap + 1) * (MCH * MCH / 8) + 1];
         }
+        // keep the two loads above the products: left to itself the scheduler sinks them next to
+        // their LDS stores, and every tap then waits out the full load latency
+        __builtin_amdgcn_sched_barrier(0);
         const uint16_t *wt = &wsm[tap & 1][r16 * MPS + g * 8];
 #pragma unroll
         for (int half = 0; half < 2; ++half) {

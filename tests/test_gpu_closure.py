@@ -38,7 +38,10 @@ def _net_and_input(channels, n, h, w, seed, mfma=True):
     return net, xb, x, params
 
 
-CASES = [((4, 64, 64, 64, 3), 2, 40, 37, True), ((4, 64, 64, 64, 3), 2, 40, 37, False), ((4, 8, 20, 3), 3, 23, 50, True)]
+# 40 × 37 stages the matrix-core layers' inputs element by element (odd width); 44 × 48 keeps every
+# layer's width a multiple of 4, so they stage 4-element groups, edge tiles partly outside the input
+CASES = [((4, 64, 64, 64, 3), 2, 40, 37, True), ((4, 64, 64, 64, 3), 2, 44, 48, True),
+         ((4, 64, 64, 64, 3), 2, 40, 37, False), ((4, 8, 20, 3), 3, 23, 50, True)]
 
 
 @pytest.mark.parametrize("channels,n,h,w,mfma", CASES)
