@@ -20,6 +20,7 @@
 #include <cmath>
 #include <cstdint>
 #include <string>
+#include <type_traits>
 
 #include "cgef_internal.hpp"
 
@@ -233,11 +234,16 @@ __global__ __launch_bounds__(256) void k_conv5_mfma(const void *__restrict__ in,
                                                     const float *__restrict__ bias, const uint16_t *__restrict__ zg,
                                                     void *__restrict__ out, uint16_t *__restrict__ act, int64_t hin,
                                                     int64_t win, int64_t hout,
-                                                    int64_t wout, int pad, int tiles_x) {
+                                                    int64_t wout, int pad, int tiles_x, int64_t ntiles) {
     __shared__ __attribute__((aligned(16))) uint16_t xs[CS * CS * MPS];
     __shared__ __attribute__((aligned(16))) uint16_t wsm[2][MCH * MPS];
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63, g = lane >> 4, r16 = lane & 15;
-    const int64_t ox0 = int64_t(blockIdx.x % tiles_x) * CT, oy0 = int64_t(blockIdx.x / tiles_x) * CT;
+    // Workgroups go round-robin to the 8 XCDs (gridDim.x is a multiple of 8): XCD k takes the k-th
+    // contiguous range of tiles, so tiles that share input halo rows and output cache lines (the
+    // 16-pixel rows of neighbouring tiles) meet in one L2.
+    const int64_t chunk = gridDim.x / 8, tile = int64_t(blockIdx.x % 8) * chunk + blockIdx.x / 8;
+    if (tile >= ntiles) return;
+    const int64_t ox0 = (tile % tiles_x) * CT, oy0 = (tile / tiles_x) * CT;
     const int64_t img = blockIdx.z;
     const int64_t iy0 = oy0 - pad, ix0 = ox0 - pad;
     // each lane moves 16 weights (32 bytes) of a tap's 64 × 64 block: row t / 4, columns 16·(t % 4)…
@@ -245,59 +251,80 @@ __global__ __launch_bounds__(256) void k_conv5_mfma(const void *__restrict__ in,
     uint16_t *wdst0 = &wsm[0][(t >> 2) * MPS + (t & 3) * 16];
     uint16_t *wdst1 = &wsm[1][(t >> 2) * MPS + (t & 3) * 16];
     uint4 wr0 = wsrc[0], wr1 = wsrc[1];
+    float bv[4];   // the bias of the lane's four output channels, requested once
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) bv[nt] = OUT == 2 ? 0.0f : bias[nt * 16 + r16];
     reinterpret_cast<uint4 *>(wdst0)[0] = wr0;
     reinterpret_cast<uint4 *>(wdst0)[1] = wr1;
-    // staging: a lane takes (channel pair, tile row) items — lane t: pair t % 32, rows t / 32, + 8, + 16 —
-    // loads the pair's two 20-element row segments as five 4-element groups (8-byte loads for BF16,
-    // 16-byte for FP32) when the input and the tile origin are 4-element aligned — each group then
-    // lies wholly inside or wholly outside the input row — else element by element, and writes each
-    // pixel's two channels as one 4-byte LDS store (consecutive lanes: consecutive channel pairs)
+    wr0 = wsrc[MCH * MCH / 8];   // tap 1's weights, stored to LDS during tap 0
+    wr1 = wsrc[MCH * MCH / 8 + 1];
+    // staging of the 20×20 input tile, channel-fastest. With the input and the tile origin 4-element
+    // aligned (each 4-element group of a row then lies wholly inside or wholly outside the input),
+    // work items are (channel pair, tile row, 4-pixel group), group fastest, so that consecutive lanes
+    // read consecutive 8-byte (BF16) or 16-byte (FP32) pieces of one row; an item's two channels go to
+    // LDS as one 4-byte store per pixel. A lane requests a batch of items before storing any of them.
+    // Otherwise a lane takes (channel pair, row) items and loads element by element.
     {
         const int64_t plane = hin * win;
-        const int cp = t & 31;
         const bool vec_ok = (reinterpret_cast<uintptr_t>(in) & (IN_T == CGEF_BF16 ? 7 : 15)) == 0 &&
                             (win & 3) == 0 && (ix0 & 3) == 0;
-        for (int yy = t >> 5; yy < CS; yy += 8) {
-            const int64_t iy = iy0 + yy;
-            float v[2][CS];
-            const bool row_in = iy >= 0 && iy < hin;
+        auto store_px = [&](int yy, int xx, int cp, float v0, float v1) {
+            if constexpr (IN_GELU) v0 = gelu(v0), v1 = gelu(v1);
+            reinterpret_cast<uint32_t *>(xs + (yy * CS + xx) * MPS)[cp] = pack_bf2(v0, v1);
+        };
+        if (vec_ok) {
+            using Raw = typename std::conditional<IN_T == CGEF_BF16, uint2, float4>::type;
+            constexpr int NG = CS / 4, NITEM = (MCH / 2) * CS * NG, NI = (NITEM + 255) / 256;
+            // items per batch: BF16 13 (every item in flight), FP32 7, GELU-on-load 1
+            constexpr int NB = IN_GELU ? 1 : (IN_T == CGEF_BF16 ? NI : (NI + 1) / 2);
+#pragma unroll 1
+            for (int i0 = 0; i0 < NI; i0 += NB) {
+                Raw r[NB][2];
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const int64_t base = (img * MCH + 2 * cp + h) * plane + iy * win + ix0;
-                if (vec_ok) {
-#pragma unroll
-                    for (int q = 0; q < CS / 4; ++q) {
-                        const bool ok = row_in && ix0 + 4 * q >= 0 && ix0 + 4 * q + 4 <= win;
-                        if constexpr (IN_T == CGEF_BF16) {
-                            const uint2 w4 =
-                                ok ? reinterpret_cast<const uint2 *>(static_cast<const uint16_t *>(in) + base)[q]
-                                   : uint2{0u, 0u};
-                            v[h][4 * q] = __uint_as_float(w4.x << 16);
-                            v[h][4 * q + 1] = __uint_as_float(w4.x & 0xFFFF0000u);
-                            v[h][4 * q + 2] = __uint_as_float(w4.y << 16);
-                            v[h][4 * q + 3] = __uint_as_float(w4.y & 0xFFFF0000u);
-                        } else {
-                            const float4 f4 =
-                                ok ? reinterpret_cast<const float4 *>(static_cast<const float *>(in) + base)[q]
-                                   : float4{0.0f, 0.0f, 0.0f, 0.0f};
-                            v[h][4 * q] = f4.x, v[h][4 * q + 1] = f4.y, v[h][4 * q + 2] = f4.z, v[h][4 * q + 3] = f4.w;
-                        }
-                    }
-                } else {
-#pragma unroll
-                    for (int xx = 0; xx < CS; ++xx) {
-                        const int64_t ix = ix0 + xx;
-                        v[h][xx] = row_in && ix >= 0 && ix < win ? load_as_float<IN_T>(in, base + xx) : 0.0f;
-                    }
+                for (int b = 0; b < NB; ++b) {
+                    const int idx = t + 256 * (i0 + b);
+                    const int q = idx % NG, rest = idx / NG, cp = rest % (MCH / 2), yy = rest / (MCH / 2);
+                    const int64_t iy = iy0 + yy, ix = ix0 + 4 * q;
+                    const bool ok = i0 + b < NI && idx < NITEM && iy >= 0 && iy < hin && ix >= 0 && ix + 4 <= win;
+                    const Raw *src = reinterpret_cast<const Raw *>(static_cast<const char *>(in) +
+                        ((img * MCH + 2 * cp) * plane + iy * win + ix) * (IN_T == CGEF_BF16 ? 2 : 4));
+                    r[b][0] = ok ? src[0] : Raw{};
+                    r[b][1] = ok ? src[plane / 4] : Raw{};
                 }
-                if constexpr (IN_GELU) {
 #pragma unroll
-                    for (int xx = 0; xx < CS; ++xx) v[h][xx] = gelu(v[h][xx]);
+                for (int b = 0; b < NB; ++b) {
+                    const int idx = t + 256 * (i0 + b);
+                    if (i0 + b >= NI || idx >= NITEM) continue;
+                    const int q = idx % NG, rest = idx / NG, cp = rest % (MCH / 2), yy = rest / (MCH / 2);
+                    if constexpr (IN_T == CGEF_BF16) {
+                        const uint2 w0 = r[b][0], w1 = r[b][1];
+                        store_px(yy, 4 * q, cp, __uint_as_float(w0.x << 16), __uint_as_float(w1.x << 16));
+                        store_px(yy, 4 * q + 1, cp, __uint_as_float(w0.x & 0xFFFF0000u), __uint_as_float(w1.x & 0xFFFF0000u));
+                        store_px(yy, 4 * q + 2, cp, __uint_as_float(w0.y << 16), __uint_as_float(w1.y << 16));
+                        store_px(yy, 4 * q + 3, cp, __uint_as_float(w0.y & 0xFFFF0000u), __uint_as_float(w1.y & 0xFFFF0000u));
+                    } else {
+                        const float4 f0 = r[b][0], f1 = r[b][1];
+                        store_px(yy, 4 * q, cp, f0.x, f1.x);
+                        store_px(yy, 4 * q + 1, cp, f0.y, f1.y);
+                        store_px(yy, 4 * q + 2, cp, f0.z, f1.z);
+                        store_px(yy, 4 * q + 3, cp, f0.w, f1.w);
+                    }
                 }
             }
-            uint32_t *dst = reinterpret_cast<uint32_t *>(xs + (yy * CS) * MPS) + cp;
-#pragma unroll
-            for (int xx = 0; xx < CS; ++xx) dst[xx * (MPS / 2)] = pack_bf2(v[0][xx], v[1][xx]);
+        } else {
+            const int cp = t & 31;
+            for (int yy = t >> 5; yy < CS; yy += 8) {
+                const int64_t iy = iy0 + yy;
+                const bool row_in = iy >= 0 && iy < hin;
+                const int64_t base = (img * MCH + 2 * cp) * plane + iy * win + ix0;
+#pragma unroll 4
+                for (int xx = 0; xx < CS; ++xx) {
+                    const int64_t ix = ix0 + xx;
+                    const bool ok = row_in && ix >= 0 && ix < win;
+                    store_px(yy, xx, cp, ok ? load_as_float<IN_T>(in, base + xx) : 0.0f,
+                             ok ? load_as_float<IN_T>(in, base + plane + xx) : 0.0f);
+                }
+            }
         }
     }
     __syncthreads();
@@ -307,60 +334,121 @@ __global__ __launch_bounds__(256) void k_conv5_mfma(const void *__restrict__ in,
     for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
         for (int nt = 0; nt < 4; ++nt) acc[mt][nt] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+    // Software-pipelined over the 50 (tap, channel half) steps: the fragments of the next step are
+    // requested from LDS before the current step's 16 products. Tap t + 1's weights (loaded into
+    // registers during tap t − 1) are stored to LDS slab (t + 1) & 1 between the halves of tap t —
+    // that slab was last read before the barrier that ended tap t − 1 — and the barrier ending tap t
+    // publishes them; only the first weight fragments of each tap are requested after a barrier.
     const uint16_t *xw = xs + (wave * 4 * CS + r16) * MPS + g * 8;
-    for (int tap = 0; tap < 25; ++tap) {
+    const uint16_t *wf = &wsm[0][r16 * MPS + g * 8];
+    auto frag_a = [&](bf16x8_t(&a)[4], int tap, int half) {
         const int ky = tap / 5, kx = tap - 5 * ky;
-        const bool more = tap + 1 < 25;
-        if (more) {   // next tap's weights in flight during this tap's products
-            wr0 = wsrc[(tap + 1) * (MCH * MCH / 8)];
-            wr1 = wsrc[(tap + 1) * (MCH * MCH / 8) + 1];
-        }
-        // keep the two loads above the products: left to itself the scheduler sinks them next to
-        // their LDS stores, and every tap then waits out the full load latency
-        __builtin_amdgcn_sched_barrier(0);
-        const uint16_t *wt = &wsm[tap & 1][r16 * MPS + g * 8];
 #pragma unroll
-        for (int half = 0; half < 2; ++half) {
-            bf16x8_t a[4], b[4];
+        for (int mt = 0; mt < 4; ++mt)
+            a[mt] = *reinterpret_cast<const bf16x8_t *>(xw + ((mt + ky) * CS + kx) * MPS + half * 32);
+    };
+    auto frag_b = [&](bf16x8_t(&b)[4], int tap, int half) {
+        const uint16_t *wt = wf + (tap & 1) * (MCH * MPS);
 #pragma unroll
-            for (int mt = 0; mt < 4; ++mt)
-                a[mt] = *reinterpret_cast<const bf16x8_t *>(xw + ((mt + ky) * CS + kx) * MPS + half * 32);
+        for (int nt = 0; nt < 4; ++nt) b[nt] = *reinterpret_cast<const bf16x8_t *>(wt + nt * 16 * MPS + half * 32);
+    };
+    auto products = [&](const bf16x8_t(&a)[4], const bf16x8_t(&b)[4]) {
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
             for (int nt = 0; nt < 4; ++nt)
-                b[nt] = *reinterpret_cast<const bf16x8_t *>(wt + nt * 16 * MPS + half * 32);
-#pragma unroll
-            for (int mt = 0; mt < 4; ++mt)
-#pragma unroll
-                for (int nt = 0; nt < 4; ++nt)
-                    acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mt], b[nt], acc[mt][nt], 0, 0, 0);
-        }
+                acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mt], b[nt], acc[mt][nt], 0, 0, 0);
+    };
+    bf16x8_t a0[4], b0[4], a1[4], b1[4];
+    frag_a(a0, 0, 0);
+    frag_b(b0, 0, 0);
+    for (int tap = 0; tap < 25; ++tap) {
+        const bool more = tap + 1 < 25;
+        frag_a(a1, tap, 1);
+        frag_b(b1, tap, 1);
+        __builtin_amdgcn_sched_barrier(0);   // the requests above stay ahead of the products below
+        products(a0, b0);
+        __builtin_amdgcn_sched_barrier(0);   // the weight stores wait for their loads: after the products
         if (more) {
             uint16_t *wd = (tap & 1) ? wdst0 : wdst1;
             reinterpret_cast<uint4 *>(wd)[0] = wr0;
             reinterpret_cast<uint4 *>(wd)[1] = wr1;
+            frag_a(a0, tap + 1, 0);
         }
-        __syncthreads();
+        __builtin_amdgcn_sched_barrier(0);
+        products(a1, b1);
+        if (more) {
+            __syncthreads();
+            frag_b(b0, tap + 1, 0);
+            if (tap + 2 < 25) {   // tap + 2's weights in flight through tap + 1's first half
+                wr0 = wsrc[(tap + 2) * (MCH * MCH / 8)];
+                wr1 = wsrc[(tap + 2) * (MCH * MCH / 8) + 1];
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
     }
 
+    // Epilogue: lane (g, r16) holds pixels 4g … 4g + 3 of output row py for channel nt·16 + r16, i.e.
+    // four consecutive outputs: one 8-byte (BF16) or 16-byte (FP32) store each when the row length
+    // keeps them aligned and inside the output, element by element otherwise.
+    const bool vec_out = (wout & 3) == 0 && ((reinterpret_cast<uintptr_t>(out) | reinterpret_cast<uintptr_t>(act) |
+                                              reinterpret_cast<uintptr_t>(zg)) & 15) == 0;
+    const int64_t px0 = ox0 + 4 * g;
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt) {
         const int64_t py = oy0 + wave * 4 + mt;
         if (py >= hout) continue;
+        float gz[4][4];   // OUT 2: GELU'(Z) of the four pixels, requested for all channels first
+        if constexpr (OUT == 2) {
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) {
+                const int64_t idx0 = ((img * MCH + nt * 16 + r16) * hout + py) * wout + px0;
+                if (vec_out && px0 + 4 <= wout) {
+                    const uint2 z4 = *reinterpret_cast<const uint2 *>(zg + idx0);
+                    gz[nt][0] = __uint_as_float(z4.x << 16), gz[nt][1] = __uint_as_float(z4.x & 0xFFFF0000u);
+                    gz[nt][2] = __uint_as_float(z4.y << 16), gz[nt][3] = __uint_as_float(z4.y & 0xFFFF0000u);
+                } else {
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) gz[nt][i] = px0 + i < wout ? bf2f(zg[idx0 + i]) : 0.0f;
+                }
+            }
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) gz[nt][i] = gelu_grad(gz[nt][i]);
+        }
 #pragma unroll
         for (int nt = 0; nt < 4; ++nt) {
-            const int64_t co = nt * 16 + r16;
+            const int64_t idx0 = ((img * MCH + nt * 16 + r16) * hout + py) * wout + px0;
+            float v[4];
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const int64_t px = ox0 + 4 * g + i;
-                if (px >= wout) continue;
-                const int64_t idx = ((img * MCH + co) * hout + py) * wout + px;
-                const float v = acc[mt][nt][i];
+                if constexpr (OUT == 2) v[i] = acc[mt][nt][i] * gz[nt][i];
+                else v[i] = acc[mt][nt][i] + bv[nt];
+            }
+            if (vec_out && px0 + 4 <= wout) {
                 if constexpr (OUT == 0) {
-                    const uint16_t z = f2bf(v + bias[co]);
-                    static_cast<uint16_t *>(out)[idx] = z;
-                    if (act) act[idx] = f2bf(gelu(bf2f(z)));
-                } else if constexpr (OUT == 1) static_cast<float *>(out)[idx] = v + bias[co];
-                else static_cast<float *>(out)[idx] = v * gelu_grad(bf2f(zg[idx]));
+                    uint16_t z[4];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) z[i] = f2bf(v[i]);
+                    *reinterpret_cast<uint2 *>(static_cast<uint16_t *>(out) + idx0) =
+                        uint2{uint32_t(z[0]) | (uint32_t(z[1]) << 16), uint32_t(z[2]) | (uint32_t(z[3]) << 16)};
+                    if (act)
+                        *reinterpret_cast<uint2 *>(act + idx0) =
+                            uint2{pack_bf2(gelu(bf2f(z[0])), gelu(bf2f(z[1]))), pack_bf2(gelu(bf2f(z[2])), gelu(bf2f(z[3])))};
+                } else {
+                    *reinterpret_cast<float4 *>(static_cast<float *>(out) + idx0) = float4{v[0], v[1], v[2], v[3]};
+                }
+            } else {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    if (px0 + i >= wout) continue;
+                    if constexpr (OUT == 0) {
+                        const uint16_t z = f2bf(v[i]);
+                        static_cast<uint16_t *>(out)[idx0 + i] = z;
+                        if (act) act[idx0 + i] = f2bf(gelu(bf2f(z)));
+                    } else static_cast<float *>(out)[idx0 + i] = v[i];
+                }
             }
         }
     }
@@ -1030,12 +1118,12 @@ cgef_status cgef_conv5_mfma(const void *in, int32_t in_dtype, int32_t in_gelu, i
         need(!act || out_kind == 0, CGEF_ERR_ARGUMENT, "cgef_conv5_mfma: the activation is written with a BF16 output");
         const int tiles_x = int((wout + CT - 1) / CT);
         const int64_t ntiles = int64_t(tiles_x) * ((hout + CT - 1) / CT);
-        need(ntiles <= 0x7fffffff, CGEF_ERR_DIMENSION, "cgef_conv5_mfma: too many tiles");
-        dim3 g(unsigned(ntiles), 1, unsigned(n));
+        need(ntiles <= 0x7ffffff0, CGEF_ERR_DIMENSION, "cgef_conv5_mfma: too many tiles");
+        dim3 g(unsigned((ntiles + 7) / 8 * 8), 1, unsigned(n));
         hipStream_t st = static_cast<hipStream_t>(stream);
 #define CGEF_MFMA_LAUNCH(IT, IG, OK)                                                                                  \
     hipLaunchKernelGGL((k_conv5_mfma<IT, IG, OK>), g, dim3(256), 0, st, in, wm, bias, zgrad, out, act, hin, win, hout, \
-                       wout, pad, tiles_x)
+                       wout, pad, tiles_x, ntiles)
         if (in_dtype == CGEF_BF16 && in_gelu) {
             if (out_kind == 0) CGEF_MFMA_LAUNCH(CGEF_BF16, 1, 0);
             else if (out_kind == 1) CGEF_MFMA_LAUNCH(CGEF_BF16, 1, 1);
