@@ -36,22 +36,36 @@ constexpr int DSTR = CT * CT + 1;        // LDS channel stride of the wgrad dZ t
 
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(uint32_t(h) << 16); }
 
-// Round to nearest even (finite inputs).
-__device__ __forceinline__ uint16_t f2bf(float f) {
-    uint32_t u = __float_as_uint(f);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return uint16_t(u >> 16);
-}
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
 
+// FP32 → BF16, round to nearest even: one v_cvt_pk_bf16_f32 per pair.
 __device__ __forceinline__ uint32_t pack_bf2(float a, float b) {
-    return uint32_t(f2bf(a)) | (uint32_t(f2bf(b)) << 16);
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2_t{a, b}), bf16x2_t));
+}
+__device__ __forceinline__ uint16_t f2bf(float f) { return uint16_t(pack_bf2(f, 0.0f)); }
+
+// erf for the GELU: Abramowitz & Stegun 7.1.26 on |x| (absolute error at most 1.5e-7, below the
+// BF16 activations' resolution and the FP32 gradients' tolerance), extended as an odd function;
+// also returns exp(−x²), which the GELU derivative reuses.
+__device__ __forceinline__ float erf_as(float x, float &ex2) {
+    const float ax = fabsf(x);
+    const float t = __builtin_amdgcn_rcpf(1.0f + 0.3275911f * ax);
+    const float poly =
+        t * (0.254829592f + t * (-0.284496736f + t * (1.421413741f + t * (-1.453152027f + t * 1.061405429f))));
+    ex2 = __expf(-ax * ax);
+    return copysignf(1.0f - poly * ex2, x);
 }
 
-// GELU(x) = x·Φ(x) with the exact normal CDF, and its derivative Φ(x) + x·φ(x).
-__device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.7071067811865476f)); }
+// GELU(x) = x·Φ(x) with the normal CDF Φ(x) = (1 + erf(x/√2)) / 2, and its derivative Φ(x) + x·φ(x).
+__device__ __forceinline__ float gelu(float x) {
+    float e;
+    return 0.5f * x * (1.0f + erf_as(x * 0.7071067811865476f, e));
+}
 __device__ __forceinline__ float gelu_grad(float x) {
-    const float cdf = 0.5f * (1.0f + erff(x * 0.7071067811865476f));
-    return cdf + x * 0.3989422804014327f * expf(-0.5f * x * x);
+    float e;   // exp(−x²/2)
+    const float cdf = 0.5f * (1.0f + erf_as(x * 0.7071067811865476f, e));
+    return cdf + x * 0.3989422804014327f * e;
 }
 
 template <int T>
@@ -230,86 +244,143 @@ typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
 
 template <int IN_T, int IN_GELU, int OUT>
-__global__ __launch_bounds__(256) void k_conv5_mfma(const void *__restrict__ in, const uint16_t *__restrict__ wm,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_conv5_mfma(const void *__restrict__ in, const uint16_t *__restrict__ wm,
                                                     const float *__restrict__ bias, const uint16_t *__restrict__ zg,
                                                     void *__restrict__ out, uint16_t *__restrict__ act, int64_t hin,
                                                     int64_t win, int64_t hout,
-                                                    int64_t wout, int pad, int tiles_x, int64_t ntiles) {
+                                                    int64_t wout, int pad, int tiles_x, int64_t ntiles, int64_t nimg) {
     __shared__ __attribute__((aligned(16))) uint16_t xs[CS * CS * MPS];
     __shared__ __attribute__((aligned(16))) uint16_t wsm[2][MCH * MPS];
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63, g = lane >> 4, r16 = lane & 15;
-    // Workgroups go round-robin to the 8 XCDs (gridDim.x is a multiple of 8): XCD k takes the k-th
-    // contiguous range of tiles, so tiles that share input halo rows and output cache lines (the
-    // 16-pixel rows of neighbouring tiles) meet in one L2.
-    const int64_t chunk = gridDim.x / 8, tile = int64_t(blockIdx.x % 8) * chunk + blockIdx.x / 8;
-    if (tile >= ntiles) return;
-    const int64_t ox0 = (tile % tiles_x) * CT, oy0 = (tile / tiles_x) * CT;
-    const int64_t img = blockIdx.z;
-    const int64_t iy0 = oy0 - pad, ix0 = ox0 - pad;
+    // Persistent over (image, tile) items. Workgroups go round-robin to the 8 XCDs (gridDim.x is a
+    // multiple of 8): XCD k owns the k-th contiguous range of items and its workgroups step through
+    // it together, so tiles that share input halo rows and output cache lines (the 16-pixel rows of
+    // neighbouring tiles) meet in one L2.
+    const int64_t total = ntiles * nimg, chunk = (total + 7) / 8, nb = gridDim.x / 8;
+    const int64_t lo = int64_t(blockIdx.x % 8) * chunk, hi = lo + chunk < total ? lo + chunk : total;
+    int64_t item = lo + blockIdx.x / 8;
+    if (item >= hi) return;
     // each lane moves 16 weights (32 bytes) of a tap's 64 × 64 block: row t / 4, columns 16·(t % 4)…
-    const uint4 *wsrc = reinterpret_cast<const uint4 *>(wm) + 2 * t;
+    const uint4 *wsrc0 = reinterpret_cast<const uint4 *>(wm) + 2 * t;
     uint16_t *wdst0 = &wsm[0][(t >> 2) * MPS + (t & 3) * 16];
     uint16_t *wdst1 = &wsm[1][(t >> 2) * MPS + (t & 3) * 16];
-    uint4 wr0 = wsrc[0], wr1 = wsrc[1];
-    float bv[4];   // the bias of the lane's four output channels, requested once
+    uint4 wr0, wr1;
+    float bv[4];   // the bias of the lane's four output channels
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt) bv[nt] = OUT == 2 ? 0.0f : bias[nt * 16 + r16];
-    reinterpret_cast<uint4 *>(wdst0)[0] = wr0;
-    reinterpret_cast<uint4 *>(wdst0)[1] = wr1;
-    wr0 = wsrc[MCH * MCH / 8];   // tap 1's weights, stored to LDS during tap 0
-    wr1 = wsrc[MCH * MCH / 8 + 1];
-    // staging of the 20×20 input tile, channel-fastest. With the input and the tile origin 4-element
-    // aligned (each 4-element group of a row then lies wholly inside or wholly outside the input),
-    // work items are (channel pair, tile row, 4-pixel group), group fastest, so that consecutive lanes
-    // read consecutive 8-byte (BF16) or 16-byte (FP32) pieces of one row; an item's two channels go to
-    // LDS as one 4-byte store per pixel. A lane requests a batch of items before storing any of them.
+
+    // Input staging, channel-fastest. With the input 4-element aligned (pad is 0 or 4, so every
+    // tile origin is too, and each 4-element group of a row lies wholly inside or wholly outside the
+    // input), work items are (channel pair, tile row, 4-pixel group), group fastest, so consecutive
+    // lanes read consecutive 8-byte (BF16) or 16-byte (FP32) pieces of one row; an item's two
+    // channels go to LDS as one 4-byte store per pixel. The first NPF items of the next tile are
+    // requested into registers during the current tile's last taps; the rest at the tile's start.
     // Otherwise a lane takes (channel pair, row) items and loads element by element.
-    {
-        const int64_t plane = hin * win;
-        const bool vec_ok = (reinterpret_cast<uintptr_t>(in) & (IN_T == CGEF_BF16 ? 7 : 15)) == 0 &&
-                            (win & 3) == 0 && (ix0 & 3) == 0;
-        auto store_px = [&](int yy, int xx, int cp, float v0, float v1) {
-            if constexpr (IN_GELU) v0 = gelu(v0), v1 = gelu(v1);
-            reinterpret_cast<uint32_t *>(xs + (yy * CS + xx) * MPS)[cp] = pack_bf2(v0, v1);
-        };
+    using Raw = typename std::conditional<IN_T == CGEF_BF16, uint2, float4>::type;
+    constexpr int NG = CS / 4, NITEM = (MCH / 2) * CS * NG, NI = (NITEM + 255) / 256;
+    // prefetched items: as many as the register budget of two waves per SIMD leaves
+    constexpr int NPF = IN_GELU ? 0 : (IN_T == CGEF_BF16 ? (OUT == 2 ? 5 : NI) : (OUT == 2 ? 3 : 5));
+    constexpr int NB = IN_GELU ? 1 : (IN_T == CGEF_BF16 ? (NI - NPF > 0 ? NI - NPF : 1) : 5);   // batch of the items left
+    const int64_t plane = hin * win;
+    const bool vec_ok = (reinterpret_cast<uintptr_t>(in) & (IN_T == CGEF_BF16 ? 7 : 15)) == 0 && (win & 3) == 0 &&
+                        (pad & 3) == 0;
+    auto store_px = [&](int yy, int xx, int cp, float v0, float v1) {
+        if constexpr (IN_GELU) v0 = gelu(v0), v1 = gelu(v1);
+        reinterpret_cast<uint32_t *>(xs + (yy * CS + xx) * MPS)[cp] = pack_bf2(v0, v1);
+    };
+    // lane index for the item arithmetic, made opaque per tile (below) so that the compiler does not
+    // hoist every item's offsets out of the tile loop into registers
+    int tq = t;
+    auto load_item = [&](int i, int64_t img, int64_t iy0, int64_t ix0, Raw(&r)[2]) {
+        const int idx = tq + 256 * i;
+        const int q = idx % NG, rest = idx / NG, cp = rest % (MCH / 2), yy = rest / (MCH / 2);
+        const int64_t iy = iy0 + yy, ix = ix0 + 4 * q;
+        const bool ok = i < NI && idx < NITEM && iy >= 0 && iy < hin && ix >= 0 && ix + 4 <= win;
+        const Raw *src = reinterpret_cast<const Raw *>(static_cast<const char *>(in) +
+                                                       ((img * MCH + 2 * cp) * plane + iy * win + ix) *
+                                                           (IN_T == CGEF_BF16 ? 2 : 4));
+        r[0] = ok ? src[0] : Raw{};
+        r[1] = ok ? src[plane / 4] : Raw{};
+    };
+    auto store_item = [&](int i, const Raw(&r)[2]) {
+        const int idx = tq + 256 * i;
+        if (i >= NI || idx >= NITEM) return;
+        const int q = idx % NG, rest = idx / NG, cp = rest % (MCH / 2), yy = rest / (MCH / 2);
+        if constexpr (IN_T == CGEF_BF16 && !IN_GELU) {   // BF16 in, BF16 out: the halves regrouped
+            const uint2 w0 = r[0], w1 = r[1];
+            uint32_t *d = reinterpret_cast<uint32_t *>(xs + (yy * CS + 4 * q) * MPS) + cp;
+            d[0] = (w0.x & 0xFFFFu) | (w1.x << 16);
+            d[MPS / 2] = (w0.x >> 16) | (w1.x & 0xFFFF0000u);
+            d[MPS] = (w0.y & 0xFFFFu) | (w1.y << 16);
+            d[3 * MPS / 2] = (w0.y >> 16) | (w1.y & 0xFFFF0000u);
+        } else if constexpr (IN_T == CGEF_BF16) {
+            const uint2 w0 = r[0], w1 = r[1];
+            store_px(yy, 4 * q, cp, __uint_as_float(w0.x << 16), __uint_as_float(w1.x << 16));
+            store_px(yy, 4 * q + 1, cp, __uint_as_float(w0.x & 0xFFFF0000u), __uint_as_float(w1.x & 0xFFFF0000u));
+            store_px(yy, 4 * q + 2, cp, __uint_as_float(w0.y << 16), __uint_as_float(w1.y << 16));
+            store_px(yy, 4 * q + 3, cp, __uint_as_float(w0.y & 0xFFFF0000u), __uint_as_float(w1.y & 0xFFFF0000u));
+        } else {
+            const float4 f0 = r[0], f1 = r[1];
+            store_px(yy, 4 * q, cp, f0.x, f1.x);
+            store_px(yy, 4 * q + 1, cp, f0.y, f1.y);
+            store_px(yy, 4 * q + 2, cp, f0.z, f1.z);
+            store_px(yy, 4 * q + 3, cp, f0.w, f1.w);
+        }
+    };
+    auto item_origin = [&](int64_t it, int64_t &img, int64_t &oy0, int64_t &ox0) {
+        img = it / ntiles;
+        const int64_t tile = it - img * ntiles;
+        ox0 = (tile % tiles_x) * CT, oy0 = (tile / tiles_x) * CT;
+    };
+    Raw pf[NPF > 0 ? NPF : 1][2];
+    int64_t img, oy0, ox0;
+    item_origin(item, img, oy0, ox0);
+    if (vec_ok) {
+#pragma unroll
+        for (int i = 0; i < NPF; ++i) load_item(i, img, oy0 - pad, ox0 - pad, pf[i]);
+    }
+
+    const uint16_t *xw = xs + (wave * 4 * CS + r16) * MPS + g * 8;
+    const uint16_t *wf = &wsm[0][r16 * MPS + g * 8];
+    auto frag_a = [&](bf16x8_t(&a)[4], int tap, int half) {
+        const int ky = tap / 5, kx = tap - 5 * ky;
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt)
+            a[mt] = *reinterpret_cast<const bf16x8_t *>(xw + ((mt + ky) * CS + kx) * MPS + half * 32);
+    };
+    auto frag_b = [&](bf16x8_t(&b)[4], int tap, int half) {
+        const uint16_t *wt = wf + (tap & 1) * (MCH * MPS);
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) b[nt] = *reinterpret_cast<const bf16x8_t *>(wt + nt * 16 * MPS + half * 32);
+    };
+    f32x4_t acc[4][4];
+    auto products = [&](const bf16x8_t(&a)[4], const bf16x8_t(&b)[4]) {
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt)
+                acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mt], b[nt], acc[mt][nt], 0, 0, 0);
+    };
+
+    for (;;) {
+        const int64_t iy0 = oy0 - pad, ix0 = ox0 - pad;
+        // Opaque copy of the weight pointer per tile: otherwise the per-tap weight addresses, being
+        // the same for every tile, are hoisted out of this loop and held in ~100 registers.
+        const uint4 *wsrc = wsrc0;
+        asm volatile("" : "+v"(wsrc));
+        tq = t;
+        asm volatile("" : "+v"(tq));
+        __syncthreads();   // the previous tile's products have read xs and both weight slabs
         if (vec_ok) {
-            using Raw = typename std::conditional<IN_T == CGEF_BF16, uint2, float4>::type;
-            constexpr int NG = CS / 4, NITEM = (MCH / 2) * CS * NG, NI = (NITEM + 255) / 256;
-            // items per batch: BF16 13 (every item in flight), FP32 7, GELU-on-load 1
-            constexpr int NB = IN_GELU ? 1 : (IN_T == CGEF_BF16 ? NI : (NI + 1) / 2);
+#pragma unroll
+            for (int i = 0; i < NPF; ++i) store_item(i, pf[i]);
 #pragma unroll 1
-            for (int i0 = 0; i0 < NI; i0 += NB) {
-                Raw r[NB][2];
+            for (int i0 = NPF; i0 < NI; i0 += NB) {
+                Raw r[NB > 0 ? NB : 1][2];
 #pragma unroll
-                for (int b = 0; b < NB; ++b) {
-                    const int idx = t + 256 * (i0 + b);
-                    const int q = idx % NG, rest = idx / NG, cp = rest % (MCH / 2), yy = rest / (MCH / 2);
-                    const int64_t iy = iy0 + yy, ix = ix0 + 4 * q;
-                    const bool ok = i0 + b < NI && idx < NITEM && iy >= 0 && iy < hin && ix >= 0 && ix + 4 <= win;
-                    const Raw *src = reinterpret_cast<const Raw *>(static_cast<const char *>(in) +
-                        ((img * MCH + 2 * cp) * plane + iy * win + ix) * (IN_T == CGEF_BF16 ? 2 : 4));
-                    r[b][0] = ok ? src[0] : Raw{};
-                    r[b][1] = ok ? src[plane / 4] : Raw{};
-                }
+                for (int b = 0; b < NB; ++b) load_item(i0 + b, img, iy0, ix0, r[b]);
 #pragma unroll
-                for (int b = 0; b < NB; ++b) {
-                    const int idx = t + 256 * (i0 + b);
-                    if (i0 + b >= NI || idx >= NITEM) continue;
-                    const int q = idx % NG, rest = idx / NG, cp = rest % (MCH / 2), yy = rest / (MCH / 2);
-                    if constexpr (IN_T == CGEF_BF16) {
-                        const uint2 w0 = r[b][0], w1 = r[b][1];
-                        store_px(yy, 4 * q, cp, __uint_as_float(w0.x << 16), __uint_as_float(w1.x << 16));
-                        store_px(yy, 4 * q + 1, cp, __uint_as_float(w0.x & 0xFFFF0000u), __uint_as_float(w1.x & 0xFFFF0000u));
-                        store_px(yy, 4 * q + 2, cp, __uint_as_float(w0.y << 16), __uint_as_float(w1.y << 16));
-                        store_px(yy, 4 * q + 3, cp, __uint_as_float(w0.y & 0xFFFF0000u), __uint_as_float(w1.y & 0xFFFF0000u));
-                    } else {
-                        const float4 f0 = r[b][0], f1 = r[b][1];
-                        store_px(yy, 4 * q, cp, f0.x, f1.x);
-                        store_px(yy, 4 * q + 1, cp, f0.y, f1.y);
-                        store_px(yy, 4 * q + 2, cp, f0.z, f1.z);
-                        store_px(yy, 4 * q + 3, cp, f0.w, f1.w);
-                    }
-                }
+                for (int b = 0; b < NB; ++b) store_item(i0 + b, r[b]);
             }
         } else {
             const int cp = t & 31;
@@ -326,131 +397,130 @@ __global__ __launch_bounds__(256) void k_conv5_mfma(const void *__restrict__ in,
                 }
             }
         }
-    }
-    __syncthreads();
+        wr0 = wsrc[0];
+        wr1 = wsrc[1];
+        reinterpret_cast<uint4 *>(wdst0)[0] = wr0;
+        reinterpret_cast<uint4 *>(wdst0)[1] = wr1;
+        wr0 = wsrc[MCH * MCH / 8];   // tap 1's weights, stored to LDS during tap 0
+        wr1 = wsrc[MCH * MCH / 8 + 1];
+        __syncthreads();
+        const int64_t next = item + nb;
+        int64_t nimg_, noy0, nox0;
+        item_origin(next < hi ? next : item, nimg_, noy0, nox0);
 
-    f32x4_t acc[4][4];
-#pragma unroll
-    for (int mt = 0; mt < 4; ++mt)
-#pragma unroll
-        for (int nt = 0; nt < 4; ++nt) acc[mt][nt] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
-    // Software-pipelined over the 50 (tap, channel half) steps: the fragments of the next step are
-    // requested from LDS before the current step's 16 products. Tap t + 1's weights (loaded into
-    // registers during tap t − 1) are stored to LDS slab (t + 1) & 1 between the halves of tap t —
-    // that slab was last read before the barrier that ended tap t − 1 — and the barrier ending tap t
-    // publishes them; only the first weight fragments of each tap are requested after a barrier.
-    const uint16_t *xw = xs + (wave * 4 * CS + r16) * MPS + g * 8;
-    const uint16_t *wf = &wsm[0][r16 * MPS + g * 8];
-    auto frag_a = [&](bf16x8_t(&a)[4], int tap, int half) {
-        const int ky = tap / 5, kx = tap - 5 * ky;
-#pragma unroll
-        for (int mt = 0; mt < 4; ++mt)
-            a[mt] = *reinterpret_cast<const bf16x8_t *>(xw + ((mt + ky) * CS + kx) * MPS + half * 32);
-    };
-    auto frag_b = [&](bf16x8_t(&b)[4], int tap, int half) {
-        const uint16_t *wt = wf + (tap & 1) * (MCH * MPS);
-#pragma unroll
-        for (int nt = 0; nt < 4; ++nt) b[nt] = *reinterpret_cast<const bf16x8_t *>(wt + nt * 16 * MPS + half * 32);
-    };
-    auto products = [&](const bf16x8_t(&a)[4], const bf16x8_t(&b)[4]) {
 #pragma unroll
         for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
-            for (int nt = 0; nt < 4; ++nt)
-                acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mt], b[nt], acc[mt][nt], 0, 0, 0);
-    };
-    bf16x8_t a0[4], b0[4], a1[4], b1[4];
-    frag_a(a0, 0, 0);
-    frag_b(b0, 0, 0);
-    for (int tap = 0; tap < 25; ++tap) {
-        const bool more = tap + 1 < 25;
-        frag_a(a1, tap, 1);
-        frag_b(b1, tap, 1);
-        __builtin_amdgcn_sched_barrier(0);   // the requests above stay ahead of the products below
-        products(a0, b0);
-        __builtin_amdgcn_sched_barrier(0);   // the weight stores wait for their loads: after the products
-        if (more) {
-            uint16_t *wd = (tap & 1) ? wdst0 : wdst1;
-            reinterpret_cast<uint4 *>(wd)[0] = wr0;
-            reinterpret_cast<uint4 *>(wd)[1] = wr1;
-            frag_a(a0, tap + 1, 0);
-        }
-        __builtin_amdgcn_sched_barrier(0);
-        products(a1, b1);
-        if (more) {
-            __syncthreads();
-            frag_b(b0, tap + 1, 0);
-            if (tap + 2 < 25) {   // tap + 2's weights in flight through tap + 1's first half
-                wr0 = wsrc[(tap + 2) * (MCH * MCH / 8)];
-                wr1 = wsrc[(tap + 2) * (MCH * MCH / 8) + 1];
+            for (int nt = 0; nt < 4; ++nt) acc[mt][nt] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+        // Software-pipelined over the 50 (tap, channel half) steps: the fragments of the next step
+        // are requested from LDS before the current step's 16 products. Tap t + 1's weights are
+        // stored to LDS slab (t + 1) & 1 between the halves of tap t — that slab was last read before
+        // the barrier that ended tap t − 1 — and the barrier ending tap t publishes them; only the
+        // first weight fragments of each tap are requested after a barrier. Tap t + 2's weights are
+        // requested after that barrier; the next tile's input after tap 24's weights are stored.
+        bf16x8_t a0[4], b0[4], a1[4], b1[4];
+        frag_a(a0, 0, 0);
+        frag_b(b0, 0, 0);
+        for (int tap = 0; tap < 25; ++tap) {
+            const bool more = tap + 1 < 25;
+            frag_a(a1, tap, 1);
+            frag_b(b1, tap, 1);
+            __builtin_amdgcn_sched_barrier(0);   // the requests above stay ahead of the products below
+            products(a0, b0);
+            __builtin_amdgcn_sched_barrier(0);   // the weight stores wait for their loads: after the products
+            if (more) {
+                uint16_t *wd = (tap & 1) ? wdst0 : wdst1;
+                reinterpret_cast<uint4 *>(wd)[0] = wr0;
+                reinterpret_cast<uint4 *>(wd)[1] = wr1;
+                frag_a(a0, tap + 1, 0);
+            }
+            if (tap == 23 && vec_ok && next < hi) {
+                tq = t;   // opaque again: the item addresses are formed here, not held through the taps
+                asm volatile("" : "+v"(tq));
+#pragma unroll
+                for (int i = 0; i < NPF; ++i) load_item(i, nimg_, noy0 - pad, nox0 - pad, pf[i]);
             }
             __builtin_amdgcn_sched_barrier(0);
+            products(a1, b1);
+            if (more) {
+                __syncthreads();
+                frag_b(b0, tap + 1, 0);
+                if (tap + 2 < 25) {
+                    wr0 = wsrc[(tap + 2) * (MCH * MCH / 8)];
+                    wr1 = wsrc[(tap + 2) * (MCH * MCH / 8) + 1];
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
         }
-    }
 
-    // Epilogue: lane (g, r16) holds pixels 4g … 4g + 3 of output row py for channel nt·16 + r16, i.e.
-    // four consecutive outputs: one 8-byte (BF16) or 16-byte (FP32) store each when the row length
-    // keeps them aligned and inside the output, element by element otherwise.
-    const bool vec_out = (wout & 3) == 0 && ((reinterpret_cast<uintptr_t>(out) | reinterpret_cast<uintptr_t>(act) |
-                                              reinterpret_cast<uintptr_t>(zg)) & 15) == 0;
-    const int64_t px0 = ox0 + 4 * g;
-#pragma unroll
-    for (int mt = 0; mt < 4; ++mt) {
-        const int64_t py = oy0 + wave * 4 + mt;
-        if (py >= hout) continue;
-        float gz[4][4];   // OUT 2: GELU'(Z) of the four pixels, requested for all channels first
-        if constexpr (OUT == 2) {
-#pragma unroll
+        // Epilogue: lane (g, r16) holds pixels 4g … 4g + 3 of output row py for channel nt·16 + r16, i.e.
+        // four consecutive outputs: one 8-byte (BF16) or 16-byte (FP32) store each when the row length
+        // keeps them aligned and inside the output, element by element otherwise.
+        const bool vec_out = (wout & 3) == 0 && ((reinterpret_cast<uintptr_t>(out) | reinterpret_cast<uintptr_t>(act) |
+                                                  reinterpret_cast<uintptr_t>(zg)) & 15) == 0;
+        const int64_t px0 = ox0 + 4 * g;
+    #pragma unroll
+        for (int mt = 0; mt < 4; ++mt) {
+            const int64_t py = oy0 + wave * 4 + mt;
+            if (py >= hout) continue;
+            float gz[4][4];   // OUT 2: GELU'(Z) of the four pixels, requested for all channels first
+            if constexpr (OUT == 2) {
+    #pragma unroll
+                for (int nt = 0; nt < 4; ++nt) {
+                    const int64_t idx0 = ((img * MCH + nt * 16 + r16) * hout + py) * wout + px0;
+                    if (vec_out && px0 + 4 <= wout) {
+                        const uint2 z4 = *reinterpret_cast<const uint2 *>(zg + idx0);
+                        gz[nt][0] = __uint_as_float(z4.x << 16), gz[nt][1] = __uint_as_float(z4.x & 0xFFFF0000u);
+                        gz[nt][2] = __uint_as_float(z4.y << 16), gz[nt][3] = __uint_as_float(z4.y & 0xFFFF0000u);
+                    } else {
+    #pragma unroll
+                        for (int i = 0; i < 4; ++i) gz[nt][i] = px0 + i < wout ? bf2f(zg[idx0 + i]) : 0.0f;
+                    }
+                }
+    #pragma unroll
+                for (int nt = 0; nt < 4; ++nt)
+    #pragma unroll
+                    for (int i = 0; i < 4; ++i) gz[nt][i] = gelu_grad(gz[nt][i]);
+            }
+    #pragma unroll
             for (int nt = 0; nt < 4; ++nt) {
                 const int64_t idx0 = ((img * MCH + nt * 16 + r16) * hout + py) * wout + px0;
-                if (vec_out && px0 + 4 <= wout) {
-                    const uint2 z4 = *reinterpret_cast<const uint2 *>(zg + idx0);
-                    gz[nt][0] = __uint_as_float(z4.x << 16), gz[nt][1] = __uint_as_float(z4.x & 0xFFFF0000u);
-                    gz[nt][2] = __uint_as_float(z4.y << 16), gz[nt][3] = __uint_as_float(z4.y & 0xFFFF0000u);
-                } else {
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) gz[nt][i] = px0 + i < wout ? bf2f(zg[idx0 + i]) : 0.0f;
-                }
-            }
-#pragma unroll
-            for (int nt = 0; nt < 4; ++nt)
-#pragma unroll
-                for (int i = 0; i < 4; ++i) gz[nt][i] = gelu_grad(gz[nt][i]);
-        }
-#pragma unroll
-        for (int nt = 0; nt < 4; ++nt) {
-            const int64_t idx0 = ((img * MCH + nt * 16 + r16) * hout + py) * wout + px0;
-            float v[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                if constexpr (OUT == 2) v[i] = acc[mt][nt][i] * gz[nt][i];
-                else v[i] = acc[mt][nt][i] + bv[nt];
-            }
-            if (vec_out && px0 + 4 <= wout) {
-                if constexpr (OUT == 0) {
-                    uint16_t z[4];
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) z[i] = f2bf(v[i]);
-                    *reinterpret_cast<uint2 *>(static_cast<uint16_t *>(out) + idx0) =
-                        uint2{uint32_t(z[0]) | (uint32_t(z[1]) << 16), uint32_t(z[2]) | (uint32_t(z[3]) << 16)};
-                    if (act)
-                        *reinterpret_cast<uint2 *>(act + idx0) =
-                            uint2{pack_bf2(gelu(bf2f(z[0])), gelu(bf2f(z[1]))), pack_bf2(gelu(bf2f(z[2])), gelu(bf2f(z[3])))};
-                } else {
-                    *reinterpret_cast<float4 *>(static_cast<float *>(out) + idx0) = float4{v[0], v[1], v[2], v[3]};
-                }
-            } else {
-#pragma unroll
+                float v[4];
+    #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    if (px0 + i >= wout) continue;
-                    if constexpr (OUT == 0) {
-                        const uint16_t z = f2bf(v[i]);
-                        static_cast<uint16_t *>(out)[idx0 + i] = z;
-                        if (act) act[idx0 + i] = f2bf(gelu(bf2f(z)));
-                    } else static_cast<float *>(out)[idx0 + i] = v[i];
+                    if constexpr (OUT == 2) v[i] = acc[mt][nt][i] * gz[nt][i];
+                    else v[i] = acc[mt][nt][i] + bv[nt];
                 }
+                if (vec_out && px0 + 4 <= wout) {
+                    if constexpr (OUT == 0) {
+                        uint16_t z[4];
+    #pragma unroll
+                        for (int i = 0; i < 4; ++i) z[i] = f2bf(v[i]);
+                        *reinterpret_cast<uint2 *>(static_cast<uint16_t *>(out) + idx0) =
+                            uint2{uint32_t(z[0]) | (uint32_t(z[1]) << 16), uint32_t(z[2]) | (uint32_t(z[3]) << 16)};
+                        if (act)
+                            *reinterpret_cast<uint2 *>(act + idx0) =
+                                uint2{pack_bf2(gelu(bf2f(z[0])), gelu(bf2f(z[1]))), pack_bf2(gelu(bf2f(z[2])), gelu(bf2f(z[3])))};
+                    } else {
+                        *reinterpret_cast<float4 *>(static_cast<float *>(out) + idx0) = float4{v[0], v[1], v[2], v[3]};
+                    }
+                } else {
+    #pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        if (px0 + i >= wout) continue;
+                        if constexpr (OUT == 0) {
+                            const uint16_t z = f2bf(v[i]);
+                            static_cast<uint16_t *>(out)[idx0 + i] = z;
+                            if (act) act[idx0 + i] = f2bf(gelu(bf2f(z)));
+                        } else static_cast<float *>(out)[idx0 + i] = v[i];
+                    }
+                }
+                __builtin_amdgcn_sched_barrier(0);   // one (row, channel group) at a time: bounded temporaries
             }
         }
+        item = next;
+        if (item >= hi) break;
+        img = nimg_, oy0 = noy0, ox0 = nox0;
     }
 }
 
@@ -966,6 +1036,20 @@ unsigned blocks_for(int64_t n, int b = 256) { return unsigned((n + b - 1) / b); 
 
 void check_launch() { CGEF_HIP(hipGetLastError()); }
 
+// Compute units of the current device (cached per device).
+int device_cus() {
+    static int cus[64] = {};
+    int dev = 0;
+    CGEF_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return 256;
+    if (!cus[dev]) {
+        int c = 0;
+        CGEF_HIP(hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev));
+        cus[dev] = c > 0 ? c : 256;
+    }
+    return cus[dev];
+}
+
 template <int OCB, int IN_T, int IN_GELU, int OUT>
 void launch_conv(dim3 g, hipStream_t st, const void *in, const uint16_t *w, const float *bias, const uint16_t *zg,
                  void *out, uint16_t *act, int64_t cin, int64_t hin, int64_t win, int64_t cout, int64_t ocs,
@@ -1118,12 +1202,13 @@ cgef_status cgef_conv5_mfma(const void *in, int32_t in_dtype, int32_t in_gelu, i
         need(!act || out_kind == 0, CGEF_ERR_ARGUMENT, "cgef_conv5_mfma: the activation is written with a BF16 output");
         const int tiles_x = int((wout + CT - 1) / CT);
         const int64_t ntiles = int64_t(tiles_x) * ((hout + CT - 1) / CT);
-        need(ntiles <= 0x7ffffff0, CGEF_ERR_DIMENSION, "cgef_conv5_mfma: too many tiles");
-        dim3 g(unsigned((ntiles + 7) / 8 * 8), 1, unsigned(n));
+        // persistent workgroups: two per CU (the LDS of one is 76 KB), in 8 equal XCD groups
+        const int64_t items = ntiles * n, per_xcd = std::min<int64_t>((items + 7) / 8, 2 * std::max(1, device_cus() / 8));
+        dim3 g(unsigned(8 * per_xcd), 1, 1);
         hipStream_t st = static_cast<hipStream_t>(stream);
 #define CGEF_MFMA_LAUNCH(IT, IG, OK)                                                                                  \
     hipLaunchKernelGGL((k_conv5_mfma<IT, IG, OK>), g, dim3(256), 0, st, in, wm, bias, zgrad, out, act, hin, win, hout, \
-                       wout, pad, tiles_x, ntiles)
+                       wout, pad, tiles_x, ntiles, n)
         if (in_dtype == CGEF_BF16 && in_gelu) {
             if (out_kind == 0) CGEF_MFMA_LAUNCH(CGEF_BF16, 1, 0);
             else if (out_kind == 1) CGEF_MFMA_LAUNCH(CGEF_BF16, 1, 1);
