@@ -939,13 +939,14 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_sep_pi_row(GeomArgs G, const double *__restrict__ gx, int lx,
                                                     const T *__restrict__ u, const T *__restrict__ v,
                                                     double *__restrict__ rp, int64_t nchunk) {
-    extern __shared__ double srow[];
+    extern __shared__ double srow[];   // u and v of the segment and its halo, then the 2·lx + 1 taps
     const int64_t b = blockIdx.y;
     const int64_t j = (int64_t)blockIdx.x / nchunk;
     const int64_t i0 = ((int64_t)blockIdx.x - j * nchunk) * 256;
     const int64_t Nx = G.n[0], P = G.npts, rowbase = j * Nx;
     const int L = 256 + 2 * lx;
-    double *su = srow, *sv = srow + L;
+    double *su = srow, *sv = srow + L, *sg = srow + 2 * L;
+    for (int e = threadIdx.x; e <= 2 * lx; e += 256) sg[e] = gx[e];
     const T *ub = u + b * P, *vb = v + b * P;
     for (int sidx = threadIdx.x; sidx < L; sidx += 256) {
         int64_t ii = i0 - lx + sidx;
@@ -963,7 +964,7 @@ __global__ __launch_bounds__(256) void k_sep_pi_row(GeomArgs G, const double *__
     if (i >= Nx) return;
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, a4 = 0.0;
     for (int d = 0; d <= 2 * lx; d++) {
-        const double g = gx[d], x = su[threadIdx.x + d], y = sv[threadIdx.x + d];
+        const double g = sg[d], x = su[threadIdx.x + d], y = sv[threadIdx.x + d];
         a0 += g * x;
         a1 += g * y;
         a2 += g * (x * x);
@@ -979,45 +980,65 @@ __global__ __launch_bounds__(256) void k_sep_pi_row(GeomArgs G, const double *__
     o[4 * P + t] = a4;
 }
 
-template <typename T, int TJ>
-__global__ __launch_bounds__(256) void k_sep_pi_col(GeomArgs G, const double *__restrict__ gy, int ly,
+// Column pass of the separable Π on 64-column × 64-row tiles (512 threads: lane = column, wave = an
+// 8-row band). The five planes pass through LDS one at a time — the tile's 64 + 2·ly rows of that
+// plane, staged with coalesced 512-byte row pieces, so each element is read from memory once per tile
+// instead of once per 8-row band — and each lane sums its 8 outputs from LDS, the taps in a register
+// window that shifts by one per row (taps outside the kernel are 0, and a row beyond a non-periodic
+// edge reads as 0: adding +0 leaves a sum unchanged, so this equals skipping them).
+constexpr int SEPC_W = 64, SEPC_H = 64, SEPC_TJ = 8;
+template <typename T>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_sep_pi_col(GeomArgs G, const double *__restrict__ gy, int ly,
                                                     const double *__restrict__ rp, const double *__restrict__ px,
                                                     const double *__restrict__ py, const double *__restrict__ invden,
                                                     double *__restrict__ ubar, T *__restrict__ tau, T *__restrict__ uo,
-                                                    T *__restrict__ vo, T *__restrict__ wo, int64_t nchunk) {
+                                                    T *__restrict__ vo, T *__restrict__ wo, int64_t ntx) {
+    extern __shared__ double smem[];   // the 2·ly + 1 taps, then one plane's 64 + 2·ly rows × 64 columns
+    const int taps = 2 * ly + 1, R = SEPC_H + 2 * ly;
+    double *sg = smem, *st = smem + taps;
     const int64_t b = blockIdx.y;
-    const int64_t jt = (int64_t)blockIdx.x / nchunk;
-    const int64_t i = ((int64_t)blockIdx.x - jt * nchunk) * 256 + threadIdx.x;
+    const int64_t tx = (int64_t)blockIdx.x % ntx, ty = (int64_t)blockIdx.x / ntx;
     const int64_t Nx = G.n[0], Ny = G.n[1], P = G.npts;
-    if (i >= Nx) return;
-    const int64_t j0 = jt * TJ;
+    const int c = threadIdx.x & (SEPC_W - 1), band = threadIdx.x / SEPC_W;
+    const int64_t i = tx * SEPC_W + c, j0 = ty * SEPC_H + band * SEPC_TJ;
     const double *r = rp + b * 5 * P;
-    double F[TJ][5];
+    for (int e = threadIdx.x; e < taps; e += 512) sg[e] = gy[e];
+    double F[SEPC_TJ][5];
 #pragma unroll
-    for (int q = 0; q < TJ; q++)
+    for (int q = 0; q < SEPC_TJ; q++)
 #pragma unroll
         for (int f = 0; f < 5; f++) F[q][f] = 0.0;
-    for (int64_t jj = j0 - ly; jj < j0 + TJ + ly; jj++) {
-        int64_t jw = jj;
-        if (jw < 0 || jw >= Ny) {
-            if (!G.periodic[1]) continue;
-            jw = wrapi(jw, Ny);
+#pragma unroll
+    for (int f = 0; f < 5; f++) {
+        if (f) __syncthreads();   // the previous plane's readers are done
+        // lane: column c, rows band, band + 8, …: the row tests are wave-uniform branches
+#pragma unroll 4
+        for (int k = band; k < R; k += 512 / SEPC_W) {
+            int64_t jw = ty * SEPC_H - ly + k;
+            bool in = i < Nx;
+            if (jw < 0 || jw >= Ny) {
+                if (G.periodic[1]) jw = wrapi(jw, Ny);
+                else in = false;
+            }
+            st[k * SEPC_W + c] = in ? r[f * P + jw * Nx + i] : 0.0;
         }
-        const int64_t qi = i + Nx * jw;
-        double x[5];
+        __syncthreads();
+        double g[SEPC_TJ];
 #pragma unroll
-        for (int f = 0; f < 5; f++) x[f] = r[f * P + qi];
+        for (int q = 0; q < SEPC_TJ; q++) g[q] = 0.0;
+        const double *col = st + band * SEPC_TJ * SEPC_W + c;
+        for (int k = 0; k < taps + SEPC_TJ - 1; k++) {
 #pragma unroll
-        for (int q = 0; q < TJ; q++) {
-            const int64_t d = jj - (j0 + q);
-            if (d < -ly || d > ly) continue;
-            const double g = gy[d + ly];
+            for (int q = SEPC_TJ - 1; q > 0; q--) g[q] = g[q - 1];
+            g[0] = k < taps ? sg[k] : 0.0;
+            const double x = col[k * SEPC_W];
 #pragma unroll
-            for (int f = 0; f < 5; f++) F[q][f] += g * x[f];
+            for (int q = 0; q < SEPC_TJ; q++) F[q][f] += g[q] * x;
         }
     }
+    if (i >= Nx) return;
 #pragma unroll
-    for (int q = 0; q < TJ; q++) {
+    for (int q = 0; q < SEPC_TJ; q++) {
         const int64_t j = j0 + q;
         if (j >= Ny) break;
         const int64_t t = i + Nx * j;
@@ -1038,40 +1059,30 @@ __global__ __launch_bounds__(256) void k_sep_pi_col(GeomArgs G, const double *__
 void launch_separable_pi(const GeomArgs &G, const Plan &p, int dtype, const void *u, const void *v, int64_t nbatch,
                          double *rp, double *ubar, void *tau, void *uo, void *vo, void *wo, hipStream_t s) {
     const int64_t nchunk = (G.n[0] + 255) / 256;
-    // 8 output rows per lane where that still leaves the chip ≥ 1024 workgroups, else one
-    const bool tall = nchunk * ((G.n[1] + 7) / 8) * nbatch >= 1024;
-    const int TJ = tall ? 8 : 1;
     const int lx = p.sep_lim[0], ly = p.sep_lim[1];
-    const size_t sh = (size_t)2 * (256 + 2 * lx) * sizeof(double);
-    if (sh > 160 * 1024) fail(CGEF_ERR_UNSUPPORTED, "separable row half-width too large for the row pass");
-    if (sh > 64 * 1024) {
-        const void *fn = dtype == CGEF_F32 ? (const void *)k_sep_pi_row<float> : (const void *)k_sep_pi_row<double>;
-        CGEF_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
-    }
+    const size_t sh = (size_t)(2 * (256 + 2 * lx) + 2 * lx + 1) * sizeof(double);
+    const size_t shc = (size_t)(2 * ly + 1 + (SEPC_H + 2 * ly) * SEPC_W) * sizeof(double);
+    if (sh > 160 * 1024 || shc > 160 * 1024)
+        fail(CGEF_ERR_UNSUPPORTED, "separable half-width too large for the LDS of the row and column passes");
+    const void *fr = dtype == CGEF_F32 ? (const void *)k_sep_pi_row<float> : (const void *)k_sep_pi_row<double>;
+    const void *fc = dtype == CGEF_F32 ? (const void *)k_sep_pi_col<float> : (const void *)k_sep_pi_col<double>;
+    if (sh > 64 * 1024) CGEF_HIP(hipFuncSetAttribute(fr, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+    if (shc > 64 * 1024) CGEF_HIP(hipFuncSetAttribute(fc, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shc));
+    const int64_t ntx = (G.n[0] + SEPC_W - 1) / SEPC_W, nty = (G.n[1] + SEPC_H - 1) / SEPC_H;
     const dim3 gr((unsigned)(nchunk * G.n[1]), (unsigned)nbatch);
-    const dim3 gc((unsigned)(nchunk * ((G.n[1] + TJ - 1) / TJ)), (unsigned)nbatch);
+    const dim3 gc((unsigned)(ntx * nty), (unsigned)nbatch);
     if (dtype == CGEF_F32) {
         hipLaunchKernelGGL(k_sep_pi_row<float>, gr, dim3(256), sh, s, G, p.sep_g_d[0], lx, (const float *)u,
                            (const float *)v, rp, nchunk);
-        if (tall)
-            hipLaunchKernelGGL((k_sep_pi_col<float, 8>), gc, dim3(256), 0, s, G, p.sep_g_d[1], ly, rp, p.sep_prof_d[0],
-                               p.sep_prof_d[1], p.sep_invden_d, ubar, (float *)tau, (float *)uo, (float *)vo,
-                               (float *)wo, nchunk);
-        else
-            hipLaunchKernelGGL((k_sep_pi_col<float, 1>), gc, dim3(256), 0, s, G, p.sep_g_d[1], ly, rp, p.sep_prof_d[0],
-                               p.sep_prof_d[1], p.sep_invden_d, ubar, (float *)tau, (float *)uo, (float *)vo,
-                               (float *)wo, nchunk);
+        hipLaunchKernelGGL(k_sep_pi_col<float>, gc, dim3(512), shc, s, G, p.sep_g_d[1], ly, rp, p.sep_prof_d[0],
+                           p.sep_prof_d[1], p.sep_invden_d, ubar, (float *)tau, (float *)uo, (float *)vo, (float *)wo,
+                           ntx);
     } else {
         hipLaunchKernelGGL(k_sep_pi_row<double>, gr, dim3(256), sh, s, G, p.sep_g_d[0], lx, (const double *)u,
                            (const double *)v, rp, nchunk);
-        if (tall)
-            hipLaunchKernelGGL((k_sep_pi_col<double, 8>), gc, dim3(256), 0, s, G, p.sep_g_d[1], ly, rp,
-                               p.sep_prof_d[0], p.sep_prof_d[1], p.sep_invden_d, ubar, (double *)tau, (double *)uo,
-                               (double *)vo, (double *)wo, nchunk);
-        else
-            hipLaunchKernelGGL((k_sep_pi_col<double, 1>), gc, dim3(256), 0, s, G, p.sep_g_d[1], ly, rp,
-                               p.sep_prof_d[0], p.sep_prof_d[1], p.sep_invden_d, ubar, (double *)tau, (double *)uo,
-                               (double *)vo, (double *)wo, nchunk);
+        hipLaunchKernelGGL(k_sep_pi_col<double>, gc, dim3(512), shc, s, G, p.sep_g_d[1], ly, rp, p.sep_prof_d[0],
+                           p.sep_prof_d[1], p.sep_invden_d, ubar, (double *)tau, (double *)uo, (double *)vo,
+                           (double *)wo, ntx);
     }
 }
 
