@@ -663,13 +663,20 @@ static_assert((AXP * 2) % 16 == 0 && (AXC * 2) % 16 == 0, "staged input rows 16-
 template <int IN_T, int IN_GELU>
 __global__ __launch_bounds__(256) void k_conv5_wgrad_mfma(const float *__restrict__ dz, const void *__restrict__ in,
                                                           float *__restrict__ part, int64_t hout, int64_t wout,
-                                                          int tiles_x, int ntiles, int tiles_per_block) {
+                                                          int tiles_x, int ntiles, int tiles_per_block, int64_t nimg,
+                                                          int nsplit) {
     __shared__ __attribute__((aligned(16))) uint16_t dzs[MCH * DZS];
     __shared__ __attribute__((aligned(16))) uint16_t axs[MCH * AXC];
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63, g = lane >> 4, r16 = lane & 15;
-    const int ky = blockIdx.x;
-    const int64_t img = blockIdx.y;
-    const int split = blockIdx.z;
+    // (kernel row, image, tile range) items, kernel row fastest, in 8 contiguous ranges: workgroups go
+    // round-robin to the XCDs (gridDim.x is a multiple of 8), so the five kernel-row blocks that stage
+    // the same dZ tiles run together on one XCD and share them through its L2
+    const int64_t nitems = 5 * nimg * nsplit, chunk = gridDim.x / 8;
+    const int64_t item = int64_t(blockIdx.x % 8) * chunk + blockIdx.x / 8;
+    if (item >= nitems) return;
+    const int ky = int(item % 5);
+    const int64_t img = (item / 5) % nimg;
+    const int split = int(item / (5 * nimg));
     const int64_t hin = hout + 4, win = wout + 4;
     f32x4_t acc[5][4];
 #pragma unroll
@@ -768,7 +775,7 @@ __global__ __launch_bounds__(256) void k_conv5_wgrad_mfma(const float *__restric
     // D[m = 4g + i][n = r16]: output channel 16·mt + 4g + i, input channel 16·wave + r16; slot layout
     // [tap][o][i]
     const int64_t nw = int64_t(MCH) * MCH * 25;
-    float *slot = part + (img * gridDim.z + split) * (nw + MCH);
+    float *slot = part + (img * nsplit + split) * (nw + MCH);
     const int ci = wave * 16 + r16;
 #pragma unroll
     for (int kx = 0; kx < 5; ++kx)
@@ -1266,16 +1273,18 @@ cgef_status cgef_conv5_wgrad(const float *dz, const void *in, int32_t in_dtype, 
              "cgef_conv5_wgrad: scratch smaller than cgef_conv5_wgrad_scratch");
         hipStream_t st = static_cast<hipStream_t>(stream);
         if (mfma) {
-            dim3 g(5, unsigned(n), unsigned(nsplit));
+            const int64_t nitems = 5 * n * nsplit;
+            need(nitems <= 0x7ffffff0, CGEF_ERR_DIMENSION, "cgef_conv5_wgrad: too many blocks");
+            dim3 g(unsigned((nitems + 7) / 8 * 8));
             if (in_dtype == CGEF_BF16 && in_gelu)
                 hipLaunchKernelGGL((k_conv5_wgrad_mfma<CGEF_BF16, 1>), g, dim3(256), 0, st, dz, in, scratch, hout, wout,
-                                   tiles_x, ntiles, tpb);
+                                   tiles_x, ntiles, tpb, n, nsplit);
             else if (in_dtype == CGEF_BF16)
                 hipLaunchKernelGGL((k_conv5_wgrad_mfma<CGEF_BF16, 0>), g, dim3(256), 0, st, dz, in, scratch, hout, wout,
-                                   tiles_x, ntiles, tpb);
+                                   tiles_x, ntiles, tpb, n, nsplit);
             else
                 hipLaunchKernelGGL((k_conv5_wgrad_mfma<CGEF_F32, 0>), g, dim3(256), 0, st, dz, in, scratch, hout, wout,
-                                   tiles_x, ntiles, tpb);
+                                   tiles_x, ntiles, tpb, n, nsplit);
         } else {
             dim3 g(unsigned(nog * nig), unsigned(n), unsigned(nsplit));
             auto go = [&](auto kern) {
