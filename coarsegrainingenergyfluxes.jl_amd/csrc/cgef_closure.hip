@@ -75,6 +75,17 @@ __device__ __forceinline__ float load_as_float(const void *p, int64_t i) {
     else return float(static_cast<const double *>(p)[i]);
 }
 
+// Work item of a workgroup in a 1-D grid whose length is a multiple of 8. Workgroups are dispatched
+// round-robin over the 8 XCDs, each with its own L2; XCD k takes the k-th contiguous range of the
+// items, so items that are close in the item order (and read the same input tiles, halo rows or
+// gradient tiles) share one L2.
+__device__ __forceinline__ bool xcd_item(int64_t total, int64_t &item) {
+    const int64_t chunk = gridDim.x / 8;
+    item = int64_t(blockIdx.x % 8) * chunk + blockIdx.x / 8;
+    return item < total;
+}
+
+
 // One 16×16 output tile × OCB output channels per block; lane = output pixel, OCB FP32 accumulators.
 // Input channels pass through LDS ICC at a time (zero outside the input: pad 4 gives the full
 // convolution used by the data gradient). OUT: 0 BF16 Z + bias, 1 FP32 + bias, 2 FP32 × GELU'(zg).
@@ -84,12 +95,14 @@ __global__ __launch_bounds__(256) void k_conv5(const void *__restrict__ in, cons
                                                void *__restrict__ out, uint16_t *__restrict__ act, int64_t cin,
                                                int64_t hin, int64_t win,
                                                int64_t cout, int64_t ocs, int64_t hout, int64_t wout, int pad,
-                                               int tiles_x) {
+                                               int tiles_x, int64_t ntiles, int nob, int64_t nimg) {
     constexpr int KUNROLL = OCB >= 64 ? 1 : 25;
     __shared__ float tile[ICC][CS][CS + 1];
     const int t = threadIdx.x, tx = t & (CT - 1), ty = t >> 4;
-    const int64_t ox0 = int64_t(blockIdx.x % tiles_x) * CT, oy0 = int64_t(blockIdx.x / tiles_x) * CT;
-    const int64_t ob = blockIdx.y, img = blockIdx.z;
+    int64_t item;   // (output-channel block, tile, image), channel block fastest: the blocks of one tile together
+    if (!xcd_item(nob * ntiles * nimg, item)) return;
+    const int64_t ob = item % nob, tl = (item / nob) % ntiles, img = item / (nob * ntiles);
+    const int64_t ox0 = (tl % tiles_x) * CT, oy0 = (tl / tiles_x) * CT;
     const int64_t iy0 = oy0 - pad, ix0 = ox0 - pad;
     float acc[OCB];
 #pragma unroll
@@ -166,12 +179,15 @@ __global__ __launch_bounds__(256) void k_conv5_narrow(const void *__restrict__ i
                                                       const float *__restrict__ bias, const uint16_t *__restrict__ zg,
                                                       void *__restrict__ out, uint16_t *__restrict__ act, int64_t cin,
                                                       int64_t hin, int64_t win, int64_t cout, int64_t ocs, int64_t hout,
-                                                      int64_t wout, int pad, int tiles_x) {
+                                                      int64_t wout, int pad, int tiles_x, int64_t ntiles,
+                                                      int64_t nimg) {
     __shared__ float tile[NCG][NS8][NS8 + 1];
     __shared__ float red[4][64][4];
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63, tx = lane & (NT8 - 1), ty = lane >> 3;
-    const int64_t ox0 = int64_t(blockIdx.x % tiles_x) * NT8, oy0 = int64_t(blockIdx.x / tiles_x) * NT8;
-    const int64_t img = blockIdx.z;
+    int64_t item;   // (tile, image), tile fastest
+    if (!xcd_item(ntiles * nimg, item)) return;
+    const int64_t tl = item % ntiles, img = item / ntiles;
+    const int64_t ox0 = (tl % tiles_x) * NT8, oy0 = (tl / tiles_x) * NT8;
     const int64_t iy0 = oy0 - pad, ix0 = ox0 - pad;
     float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     const int64_t plane = hin * win;
@@ -545,14 +561,17 @@ template <int OG, int IG, int IN_T, int IN_GELU>
 __global__ __launch_bounds__(256) void k_conv5_wgrad(const float *__restrict__ dz, const void *__restrict__ in,
                                                      float *__restrict__ part, int64_t cin, int64_t cout,
                                                      int64_t hout, int64_t wout, int tiles_x, int ntiles,
-                                                     int tiles_per_block, int nig) {
+                                                     int tiles_per_block, int nig, int nog, int64_t nimg, int nsplit) {
     static_assert(OG * IG == 256, "one (o, i) pair per lane");
     __shared__ float ds[OG * DSTR];
     __shared__ float as[IG * ASTR];
     const int t = threadIdx.x, ol = t / IG, il = t % IG;
-    const int64_t og = blockIdx.x / nig, ig = blockIdx.x % nig;
-    const int64_t img = blockIdx.y;
-    const int split = blockIdx.z;
+    int64_t item;   // (channel-pair block, image, tile range), channel blocks fastest
+    const int64_t nblk = int64_t(nog) * nig;
+    if (!xcd_item(nblk * nimg * nsplit, item)) return;
+    const int64_t og = (item % nblk) / nig, ig = (item % nblk) % nig;
+    const int64_t img = (item / nblk) % nimg;
+    const int split = int(item / (nblk * nimg));
     const int64_t hin = hout + 4, win = wout + 4;
     float acc[25];
 #pragma unroll
@@ -630,7 +649,7 @@ __global__ __launch_bounds__(256) void k_conv5_wgrad(const float *__restrict__ d
 
     // slot layout: [slot][cout·cin·25 weights | cout biases]
     const int64_t nw = cout * cin * 25;
-    float *slot = part + (img * gridDim.z + split) * (nw + cout);
+    float *slot = part + (img * nsplit + split) * (nw + cout);
     const int64_t co = og * OG + ol, ci = ig * IG + il;
     if (co < cout && ci < cin) {
 #pragma unroll
@@ -668,12 +687,10 @@ __global__ __launch_bounds__(256) void k_conv5_wgrad_mfma(const float *__restric
     __shared__ __attribute__((aligned(16))) uint16_t dzs[MCH * DZS];
     __shared__ __attribute__((aligned(16))) uint16_t axs[MCH * AXC];
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63, g = lane >> 4, r16 = lane & 15;
-    // (kernel row, image, tile range) items, kernel row fastest, in 8 contiguous ranges: workgroups go
-    // round-robin to the XCDs (gridDim.x is a multiple of 8), so the five kernel-row blocks that stage
+    // (kernel row, image, tile range) items, kernel row fastest: the five kernel-row blocks that stage
     // the same dZ tiles run together on one XCD and share them through its L2
-    const int64_t nitems = 5 * nimg * nsplit, chunk = gridDim.x / 8;
-    const int64_t item = int64_t(blockIdx.x % 8) * chunk + blockIdx.x / 8;
-    if (item >= nitems) return;
+    int64_t item;
+    if (!xcd_item(5 * nimg * nsplit, item)) return;
     const int ky = int(item % 5);
     const int64_t img = (item / 5) % nimg;
     const int split = int(item / (5 * nimg));
@@ -1061,8 +1078,11 @@ template <int OCB, int IN_T, int IN_GELU, int OUT>
 void launch_conv(dim3 g, hipStream_t st, const void *in, const uint16_t *w, const float *bias, const uint16_t *zg,
                  void *out, uint16_t *act, int64_t cin, int64_t hin, int64_t win, int64_t cout, int64_t ocs,
                  int64_t hout, int64_t wout, int pad, int tiles_x) {
-    hipLaunchKernelGGL((k_conv5<OCB, IN_T, IN_GELU, OUT>), g, dim3(256), 0, st, in, w, bias, zg, out, act, cin, hin,
-                       win, cout, ocs, hout, wout, pad, tiles_x);
+    // g carries (tiles, output-channel blocks, images); the kernel takes them as 1-D XCD-ranged items
+    const int64_t items = int64_t(g.x) * g.y * g.z;
+    hipLaunchKernelGGL((k_conv5<OCB, IN_T, IN_GELU, OUT>), dim3(unsigned((items + 7) / 8 * 8)), dim3(256), 0, st, in,
+                       w, bias, zg, out, act, cin, hin, win, cout, ocs, hout, wout, pad, tiles_x, int64_t(g.x),
+                       int(g.y), int64_t(g.z));
 }
 
 template <int OCB, int IN_T, int IN_GELU>
@@ -1147,7 +1167,7 @@ cgef_status cgef_conv5(const void *in, int32_t in_dtype, int32_t in_gelu, int64_
              "cgef_conv5: weight layout stride must cover cout in whole output-channel blocks (4, 16 or 64)");
         const int tiles_x = int((wout + CT - 1) / CT);
         const int64_t ntiles = int64_t(tiles_x) * ((hout + CT - 1) / CT);
-        need(ntiles <= 0x7fffffff, CGEF_ERR_DIMENSION, "cgef_conv5: too many tiles");
+        need(ntiles * ((cout + ocb - 1) / ocb) * n <= 0x7ffffff0, CGEF_ERR_DIMENSION, "cgef_conv5: too many tiles");
         dim3 g(unsigned(ntiles), unsigned((cout + ocb - 1) / ocb), unsigned(n));
         hipStream_t st = static_cast<hipStream_t>(stream);
         if (ocb == 4 && ntiles * n < 2048) {
@@ -1156,10 +1176,11 @@ cgef_status cgef_conv5(const void *in, int32_t in_dtype, int32_t in_gelu, int64_
             const int tx8 = int((wout + NT8 - 1) / NT8);
             const int64_t nt8 = int64_t(tx8) * ((hout + NT8 - 1) / NT8);
             need(nt8 <= 0x7fffffff, CGEF_ERR_DIMENSION, "cgef_conv5: too many tiles");
-            const dim3 g8(unsigned(nt8), 1, unsigned(n));
+            need(nt8 * n <= 0x7ffffff0, CGEF_ERR_DIMENSION, "cgef_conv5: too many tiles");
+            const dim3 g8(unsigned((nt8 * n + 7) / 8 * 8));
             auto go = [&](auto kern) {
                 hipLaunchKernelGGL(kern, g8, dim3(256), 0, st, in, w, bias, zgrad, out, act, cin, hin, win, cout, ocs, hout,
-                                   wout, pad, tx8);
+                                   wout, pad, tx8, nt8, n);
             };
 #define CGEF_NARROW(IT, IG)                                                  \
     do {                                                                     \
@@ -1286,10 +1307,12 @@ cgef_status cgef_conv5_wgrad(const float *dz, const void *in, int32_t in_dtype, 
                 hipLaunchKernelGGL((k_conv5_wgrad_mfma<CGEF_F32, 0>), g, dim3(256), 0, st, dz, in, scratch, hout, wout,
                                    tiles_x, ntiles, tpb, n, nsplit);
         } else {
-            dim3 g(unsigned(nog * nig), unsigned(n), unsigned(nsplit));
+            const int64_t nitems = int64_t(nog) * nig * n * nsplit;
+            need(nitems <= 0x7ffffff0, CGEF_ERR_DIMENSION, "cgef_conv5_wgrad: too many blocks");
+            dim3 g(unsigned((nitems + 7) / 8 * 8));
             auto go = [&](auto kern) {
                 hipLaunchKernelGGL(kern, g, dim3(256), 0, st, dz, in, scratch, cin, cout, hout, wout, tiles_x, ntiles, tpb,
-                                   nig);
+                                   nig, nog, n, nsplit);
             };
 #define CGEF_WG(OGV, IGV)                                                                        \
     do {                                                                                         \
