@@ -25,7 +25,8 @@ EXPORTED = (
     "cgef_energy", "cgef_energy_noalloc",
     "cgef_cumulative_energy", "cgef_profile_enable", "cgef_profile_read", "cgef_plan_mode_cut", "cgef_spectral_density",
     # learned closure (no reference counterpart)
-    "cgef_conv5", "cgef_conv5_mfma", "cgef_conv5_pack_mfma", "cgef_conv5_wgrad_scratch", "cgef_conv5_wgrad", "cgef_conv5_pack_weights", "cgef_adam_update",
+    "cgef_conv5", "cgef_conv5_mfma", "cgef_conv5_pack_mfma", "cgef_conv5_mfma_ex", "cgef_conv5_pack_mfma_ex",
+    "cgef_conv5_wgrad_scratch", "cgef_conv5_wgrad", "cgef_conv5_pack_weights", "cgef_adam_update",
     "cgef_mse_grad", "cgef_closure_loss_grad", "cgef_closure_crop", "cgef_closure_products", "cgef_closure_targets", "cgef_closure_pi",
     "cgef_closure_gradient_model",
 )

@@ -57,6 +57,8 @@ def _L():
             "cgef_conv5": [vp, i32, i32, i64, i64, i64, i64, vp, i64, vp, i64, i32, vp, i32, vp, vp, vp],
             "cgef_conv5_mfma": [vp, i32, i32, i64, i64, i64, vp, vp, i32, vp, i32, vp, vp, vp],
             "cgef_conv5_pack_mfma": [vp, vp, vp, vp],
+            "cgef_conv5_mfma_ex": [vp, i32, i32, i64, i64, i64, i64, vp, vp, i64, i32, vp, i32, vp, vp, vp],
+            "cgef_conv5_pack_mfma_ex": [vp, i64, i64, vp, vp, vp],
             "cgef_conv5_wgrad_scratch": [i64, i64, i64, i64, i64, i32, C.POINTER(i64)],
             "cgef_conv5_wgrad": [vp, vp, i32, i32, i64, i64, i64, i64, i64, i32, vp, vp, vp, i64, vp],
             "cgef_conv5_pack_weights": [vp, i64, i64, vp, i64, vp, i64, vp],
@@ -113,16 +115,28 @@ def conv5(out, x, wf, ocs, bias, *, in_gelu=False, pad=0, zgrad=None, act=None):
     return out
 
 
+def mfma_pads(cin, cout):
+    """(input-channel pad, output-channel pad) of a matrix-core layer: 32 or 64, 16 or 64."""
+    return (32 if cin <= 32 else 64), (16 if cout <= 16 else 64)
+
+
+def mfma_layout_size(cin, cout):
+    """Entries of the forward matrix-core layout of a cin → cout layer (cgef_conv5_pack_mfma_ex)."""
+    ci, co = mfma_pads(cin, cout)
+    return 25 * ci * co
+
+
 def conv5_mfma(out, x, wm, bias, *, in_gelu=False, pad=0, zgrad=None, act=None):
-    """A 64 → 64-channel 5×5 layer on the matrix cores (cgef_conv5_mfma); wm is the forward or the
-    backward-data matrix-core layout (cgef_conv5_pack_mfma)."""
+    """A 5×5 layer of 1 to 64 input and output channels on the matrix cores (cgef_conv5_mfma_ex); wm
+    is the forward layout (pad 0) or the backward-data layout (pad 4) of cgef_conv5_pack_mfma_ex."""
     n, cin, h, w = x.shape
+    cout = out.shape[1]
     ho, wo = h + 2 * pad - 4, w + 2 * pad - 4
-    if cin != 64 or tuple(out.shape) != (n, 64, ho, wo):
-        raise _lib.DimensionMismatch(f"matrix-core layer needs 64 channels in and out; got x {tuple(x.shape)}, "
+    if not (1 <= cin <= 64 and 1 <= cout <= 64) or tuple(out.shape) != (n, cout, ho, wo):
+        raise _lib.DimensionMismatch(f"matrix-core layer takes 1 to 64 channels in and out; got x {tuple(x.shape)}, "
                                      f"out {tuple(out.shape)}")
-    if wm.numel() != 64 * 64 * 25:
-        raise _lib.DimensionMismatch("matrix-core weight layout has 64·64·25 entries")
+    if wm.numel() != mfma_layout_size(cin, cout):
+        raise _lib.DimensionMismatch(f"matrix-core weight layout has {mfma_layout_size(cin, cout)} entries")
     if zgrad is not None and tuple(zgrad.shape) != tuple(out.shape):
         raise _lib.DimensionMismatch("zgrad must be shaped as out")
     for t, nm in ((out, "out"), (x, "x"), (wm, "weights")):
@@ -131,8 +145,8 @@ def conv5_mfma(out, x, wm, bias, *, in_gelu=False, pad=0, zgrad=None, act=None):
     out_dt = BF16 if out.dtype == torch.bfloat16 else _lib.F32
     if act is not None and (tuple(act.shape) != tuple(out.shape) or act.dtype != torch.bfloat16):
         raise _lib.DimensionMismatch("act must be a BF16 tensor shaped as out")
-    _lib.check(_L().cgef_conv5_mfma(_p(x), in_dt, int(in_gelu), n, h, w, _p(wm), _p(bias), pad, _p(out), out_dt,
-                                    _p(zgrad), _p(act), _stream(x.device.index)))
+    _lib.check(_L().cgef_conv5_mfma_ex(_p(x), in_dt, int(in_gelu), n, cin, h, w, _p(wm), _p(bias), cout, pad, _p(out),
+                                       out_dt, _p(zgrad), _p(act), _stream(x.device.index)))
     return out
 
 
@@ -161,7 +175,8 @@ class ClosureNet:
     """conv5 → GELU → conv5 → GELU → conv5 → GELU → conv5 with `channels` = (4, 64, 64, 64, 3):
     FP32 master weights and Adam moments, BF16 weight layouts and activations, FP32 accumulation.
     The input must carry HALO·L cells of context on each side of the predicted window. With
-    mfma=True the 64 → 64-channel layers (forward and data gradient) run on the matrix cores."""
+    mfma=True every layer of at most 64 channels in and out runs its forward and data gradient on the
+    matrix cores (channels padded to the MFMA tiles), and the 64 → 64 layers their weight gradients."""
 
     def __init__(self, channels=(4, 64, 64, 64, 3), *, seed=0, device=None, lr=1e-3, betas=(0.9, 0.999),
                  eps=1e-8, mfma=True):
@@ -174,7 +189,7 @@ class ClosureNet:
         rng = np.random.Generator(np.random.PCG64(seed))
         dev = f"cuda:{self.device}"
         self.W, self.b, self.mW, self.vW, self.mb, self.vb = [], [], [], [], [], []
-        self.wf, self.wb, self.ocs_f, self.ocs_b, self.wmf, self.wmb = [], [], [], [], [], []
+        self.wf, self.wb, self.ocs_f, self.ocs_b, self.wmf, self.wmb, self.wgmm = [], [], [], [], [], [], []
         for l in range(self.L):
             ci, co = self.channels[l], self.channels[l + 1]
             gain = math.sqrt(2.0) if l < self.L - 1 else 1.0          # He init before GELU
@@ -190,9 +205,13 @@ class ClosureNet:
             self.ocs_b.append(ob)
             self.wf.append(torch.zeros(ci * 25 * of, dtype=torch.bfloat16, device=dev))
             self.wb.append(torch.zeros(co * 25 * ob, dtype=torch.bfloat16, device=dev))
-            mm = self.mfma and ci == 64 and co == 64
-            self.wmf.append(torch.zeros(64 * 64 * 25, dtype=torch.bfloat16, device=dev) if mm else None)
-            self.wmb.append(torch.zeros(64 * 64 * 25, dtype=torch.bfloat16, device=dev) if mm else None)
+            mm = self.mfma and ci <= 64 and co <= 64
+            self.wmf.append(torch.zeros(mfma_layout_size(ci, co), dtype=torch.bfloat16, device=dev) if mm else None)
+            # the data gradient through layer l (l ≥ 1) is a co → ci layer; from at most 4 channels (the
+            # 64 → 3 output layer) the VALU kernel is faster: one K slice per tap would be ≥ 7/8 padding
+            self.wmb.append(torch.zeros(mfma_layout_size(co, ci), dtype=torch.bfloat16, device=dev)
+                            if mm and l > 0 and co > 4 else None)
+            self.wgmm.append(self.mfma and ci == 64 and co == 64)   # matrix-core weight gradient
         self.pack()
         self._shape = None
         self.gW = [torch.zeros_like(w) for w in self.W]
@@ -213,7 +232,9 @@ class ClosureNet:
 
     def _pack_mfma(self, l):
         if self.wmf[l] is not None:
-            _lib.check(_L().cgef_conv5_pack_mfma(_p(self.W[l]), _p(self.wmf[l]), _p(self.wmb[l]), _stream(self.device)))
+            co, ci = self.channels[l + 1], self.channels[l]
+            _lib.check(_L().cgef_conv5_pack_mfma_ex(_p(self.W[l]), co, ci, _p(self.wmf[l]), _p(self.wmb[l]),
+                                                    _stream(self.device)))
 
     def load_state(self, weights, biases):
         for l in range(self.L):
@@ -242,7 +263,7 @@ class ClosureNet:
         for l in range(self.L):
             ho, wo = h - 4 * (l + 1), w - 4 * (l + 1)
             need = max(need, wgrad_scratch_floats(n, self.channels[l], self.channels[l + 1], ho, wo,
-                                                  self.wmf[l] is not None))
+                                                  self.wgmm[l]))
         self.scratch = torch.empty(need, dtype=torch.float32, device=dev)
         self._shape = (n, h, w)
 
@@ -289,7 +310,7 @@ class ClosureNet:
             self.dZ[-1].copy_(dy)
         for l in range(self.L - 1, -1, -1):
             a = self.x if l == 0 else self.A[l - 1]
-            conv5_wgrad(self.gW[l], self.gb[l], self.dZ[l], a, self.scratch, mfma=self.wmf[l] is not None)
+            conv5_wgrad(self.gW[l], self.gb[l], self.dZ[l], a, self.scratch, mfma=self.wgmm[l])
             if l > 0 and self.wmb[l] is not None:
                 conv5_mfma(self.dZ[l - 1], self.dZ[l], self.wmb[l], None, pad=4, zgrad=self.Z[l - 1])
             elif l > 0:

@@ -259,14 +259,20 @@ constexpr int MPS = MCH + 8;
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
 
-template <int IN_T, int IN_GELU, int OUT>
+template <int IN_T, int IN_GELU, int OUT, int CP, int NT, int EXACT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_conv5_mfma(const void *__restrict__ in, const uint16_t *__restrict__ wm,
                                                     const float *__restrict__ bias, const uint16_t *__restrict__ zg,
-                                                    void *__restrict__ out, uint16_t *__restrict__ act, int64_t hin,
-                                                    int64_t win, int64_t hout,
+                                                    void *__restrict__ out, uint16_t *__restrict__ act, int64_t cin_arg,
+                                                    int64_t cout_arg, int64_t hin, int64_t win, int64_t hout,
                                                     int64_t wout, int pad, int tiles_x, int64_t ntiles, int64_t nimg) {
-    __shared__ __attribute__((aligned(16))) uint16_t xs[CS * CS * MPS];
-    __shared__ __attribute__((aligned(16))) uint16_t wsm[2][MCH * MPS];
+    static_assert(CP == 32 || CP == 64, "input channels padded to one or two 32-wide K slices per tap");
+    constexpr int KS = CP + 8;     // LDS pixel stride of the input tile and row stride of a weight slab
+    constexpr int NO = 16 * NT;    // output channels, padded to whole 16-wide N tiles
+    // EXACT: the channel counts are the padded ones (the 64 → 64 layers), known at compile time
+    const int64_t cin = EXACT ? CP : cin_arg, cout = EXACT ? NO : cout_arg;
+    constexpr int H = CP / 32;     // K slices per tap
+    __shared__ __attribute__((aligned(16))) uint16_t xs[CS * CS * KS];
+    __shared__ __attribute__((aligned(16))) uint16_t wsm[2][NO * KS];
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63, g = lane >> 4, r16 = lane & 15;
     // Persistent over (image, tile) items. Workgroups go round-robin to the 8 XCDs (gridDim.x is a
     // multiple of 8): XCD k owns the k-th contiguous range of items and its workgroups step through
@@ -276,24 +282,31 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     const int64_t lo = int64_t(blockIdx.x % 8) * chunk, hi = lo + chunk < total ? lo + chunk : total;
     int64_t item = lo + blockIdx.x / 8;
     if (item >= hi) return;
-    // each lane moves 16 weights (32 bytes) of a tap's 64 × 64 block: row t / 4, columns 16·(t % 4)…
-    const uint4 *wsrc0 = reinterpret_cast<const uint4 *>(wm) + 2 * t;
-    uint16_t *wdst0 = &wsm[0][(t >> 2) * MPS + (t & 3) * 16];
-    uint16_t *wdst1 = &wsm[1][(t >> 2) * MPS + (t & 3) * 16];
-    uint4 wr0, wr1;
-    float bv[4];   // the bias of the lane's four output channels
+    // a tap's NO × CP weights are NW16 16-byte pieces; lane t moves pieces t and t + 256 (where they
+    // exist): piece c is output row c / (CP / 8), input channels 8·(c % (CP / 8))…
+    constexpr int NW16 = NO * CP / 8;
+    const bool w0ok = t < NW16, w1ok = t + 256 < NW16;
+    const uint4 *wsrc0 = reinterpret_cast<const uint4 *>(wm) + t;
+    const int wo0 = (t / (CP / 8)) * KS + (t % (CP / 8)) * 8;
+    const int wo1 = ((t + 256) / (CP / 8)) * KS + ((t + 256) % (CP / 8)) * 8;
+    uint4 wr0 = uint4{0u, 0u, 0u, 0u}, wr1 = uint4{0u, 0u, 0u, 0u};
+    auto store_w = [&](int slab) {
+        if (w0ok) *reinterpret_cast<uint4 *>(&wsm[slab][wo0]) = wr0;
+        if (w1ok) *reinterpret_cast<uint4 *>(&wsm[slab][wo1]) = wr1;
+    };
+    float bv[NT];   // the bias of the lane's output channels
 #pragma unroll
-    for (int nt = 0; nt < 4; ++nt) bv[nt] = OUT == 2 ? 0.0f : bias[nt * 16 + r16];
+    for (int nt = 0; nt < NT; ++nt) bv[nt] = OUT == 2 || nt * 16 + r16 >= cout ? 0.0f : bias[nt * 16 + r16];
 
-    // Input staging, channel-fastest. With the input 4-element aligned (pad is 0 or 4, so every
-    // tile origin is too, and each 4-element group of a row lies wholly inside or wholly outside the
-    // input), work items are (channel pair, tile row, 4-pixel group), group fastest, so consecutive
-    // lanes read consecutive 8-byte (BF16) or 16-byte (FP32) pieces of one row; an item's two
-    // channels go to LDS as one 4-byte store per pixel. The first NPF items of the next tile are
-    // requested into registers during the current tile's last taps; the rest at the tile's start.
-    // Otherwise a lane takes (channel pair, row) items and loads element by element.
+    // Input staging, channel-fastest, channels cin … CP − 1 zero. With the input 4-element aligned
+    // (pad is 0 or 4, so every tile origin is too, and each 4-element group of a row lies wholly
+    // inside or wholly outside the input), work items are (channel pair, tile row, 4-pixel group),
+    // group fastest, so consecutive lanes read consecutive 8-byte (BF16) or 16-byte (FP32) pieces of
+    // one row; an item's two channels go to LDS as one 4-byte store per pixel. The first NPF items of
+    // the next tile are requested into registers during the current tile's last taps; the rest at
+    // the tile's start. Otherwise a lane takes (channel pair, row) items and loads element by element.
     using Raw = typename std::conditional<IN_T == CGEF_BF16, uint2, float4>::type;
-    constexpr int NG = CS / 4, NITEM = (MCH / 2) * CS * NG, NI = (NITEM + 255) / 256;
+    constexpr int NG = CS / 4, NITEM = (CP / 2) * CS * NG, NI = (NITEM + 255) / 256;
     // prefetched items: as many as the register budget of two waves per SIMD leaves
     constexpr int NPF = IN_GELU ? 0 : (IN_T == CGEF_BF16 ? (OUT == 2 ? 5 : NI) : (OUT == 2 ? 3 : 5));
     constexpr int NB = IN_GELU ? 1 : (IN_T == CGEF_BF16 ? (NI - NPF > 0 ? NI - NPF : 1) : 5);   // batch of the items left
@@ -302,33 +315,33 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
                         (pad & 3) == 0;
     auto store_px = [&](int yy, int xx, int cp, float v0, float v1) {
         if constexpr (IN_GELU) v0 = gelu(v0), v1 = gelu(v1);
-        reinterpret_cast<uint32_t *>(xs + (yy * CS + xx) * MPS)[cp] = pack_bf2(v0, v1);
+        reinterpret_cast<uint32_t *>(xs + (yy * CS + xx) * KS)[cp] = pack_bf2(v0, v1);
     };
     // lane index for the item arithmetic, made opaque per tile (below) so that the compiler does not
     // hoist every item's offsets out of the tile loop into registers
     int tq = t;
     auto load_item = [&](int i, int64_t img, int64_t iy0, int64_t ix0, Raw(&r)[2]) {
         const int idx = tq + 256 * i;
-        const int q = idx % NG, rest = idx / NG, cp = rest % (MCH / 2), yy = rest / (MCH / 2);
+        const int q = idx % NG, rest = idx / NG, cp = rest % (CP / 2), yy = rest / (CP / 2);
         const int64_t iy = iy0 + yy, ix = ix0 + 4 * q;
         const bool ok = i < NI && idx < NITEM && iy >= 0 && iy < hin && ix >= 0 && ix + 4 <= win;
         const Raw *src = reinterpret_cast<const Raw *>(static_cast<const char *>(in) +
-                                                       ((img * MCH + 2 * cp) * plane + iy * win + ix) *
+                                                       ((img * cin + 2 * cp) * plane + iy * win + ix) *
                                                            (IN_T == CGEF_BF16 ? 2 : 4));
-        r[0] = ok ? src[0] : Raw{};
-        r[1] = ok ? src[plane / 4] : Raw{};
+        r[0] = ok && 2 * cp < cin ? src[0] : Raw{};
+        r[1] = ok && 2 * cp + 1 < cin ? src[plane / 4] : Raw{};
     };
     auto store_item = [&](int i, const Raw(&r)[2]) {
         const int idx = tq + 256 * i;
         if (i >= NI || idx >= NITEM) return;
-        const int q = idx % NG, rest = idx / NG, cp = rest % (MCH / 2), yy = rest / (MCH / 2);
+        const int q = idx % NG, rest = idx / NG, cp = rest % (CP / 2), yy = rest / (CP / 2);
         if constexpr (IN_T == CGEF_BF16 && !IN_GELU) {   // BF16 in, BF16 out: the halves regrouped
             const uint2 w0 = r[0], w1 = r[1];
-            uint32_t *d = reinterpret_cast<uint32_t *>(xs + (yy * CS + 4 * q) * MPS) + cp;
+            uint32_t *d = reinterpret_cast<uint32_t *>(xs + (yy * CS + 4 * q) * KS) + cp;
             d[0] = (w0.x & 0xFFFFu) | (w1.x << 16);
-            d[MPS / 2] = (w0.x >> 16) | (w1.x & 0xFFFF0000u);
-            d[MPS] = (w0.y & 0xFFFFu) | (w1.y << 16);
-            d[3 * MPS / 2] = (w0.y >> 16) | (w1.y & 0xFFFF0000u);
+            d[KS / 2] = (w0.x >> 16) | (w1.x & 0xFFFF0000u);
+            d[KS] = (w0.y & 0xFFFFu) | (w1.y << 16);
+            d[3 * KS / 2] = (w0.y >> 16) | (w1.y & 0xFFFF0000u);
         } else if constexpr (IN_T == CGEF_BF16) {
             const uint2 w0 = r[0], w1 = r[1];
             store_px(yy, 4 * q, cp, __uint_as_float(w0.x << 16), __uint_as_float(w1.x << 16));
@@ -356,25 +369,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         for (int i = 0; i < NPF; ++i) load_item(i, img, oy0 - pad, ox0 - pad, pf[i]);
     }
 
-    const uint16_t *xw = xs + (wave * 4 * CS + r16) * MPS + g * 8;
-    const uint16_t *wf = &wsm[0][r16 * MPS + g * 8];
+    const uint16_t *xw = xs + (wave * 4 * CS + r16) * KS + g * 8;
+    const uint16_t *wf = &wsm[0][r16 * KS + g * 8];
     auto frag_a = [&](bf16x8_t(&a)[4], int tap, int half) {
         const int ky = tap / 5, kx = tap - 5 * ky;
 #pragma unroll
         for (int mt = 0; mt < 4; ++mt)
-            a[mt] = *reinterpret_cast<const bf16x8_t *>(xw + ((mt + ky) * CS + kx) * MPS + half * 32);
+            a[mt] = *reinterpret_cast<const bf16x8_t *>(xw + ((mt + ky) * CS + kx) * KS + half * 32);
     };
-    auto frag_b = [&](bf16x8_t(&b)[4], int tap, int half) {
-        const uint16_t *wt = wf + (tap & 1) * (MCH * MPS);
+    auto frag_b = [&](bf16x8_t(&b)[NT], int tap, int half) {
+        const uint16_t *wt = wf + (tap & 1) * (NO * KS);
 #pragma unroll
-        for (int nt = 0; nt < 4; ++nt) b[nt] = *reinterpret_cast<const bf16x8_t *>(wt + nt * 16 * MPS + half * 32);
+        for (int nt = 0; nt < NT; ++nt) b[nt] = *reinterpret_cast<const bf16x8_t *>(wt + nt * 16 * KS + half * 32);
     };
-    f32x4_t acc[4][4];
-    auto products = [&](const bf16x8_t(&a)[4], const bf16x8_t(&b)[4]) {
+    f32x4_t acc[4][NT];
+    auto products = [&](const bf16x8_t(&a)[4], const bf16x8_t(&b)[NT]) {
 #pragma unroll
         for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
-            for (int nt = 0; nt < 4; ++nt)
+            for (int nt = 0; nt < NT; ++nt)
                 acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mt], b[nt], acc[mt][nt], 0, 0, 0);
     };
 
@@ -399,26 +412,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
                 for (int b = 0; b < NB; ++b) store_item(i0 + b, r[b]);
             }
         } else {
-            const int cp = t & 31;
-            for (int yy = t >> 5; yy < CS; yy += 8) {
+            const int cp = t % (CP / 2);
+            for (int yy = t / (CP / 2); yy < CS; yy += 256 / (CP / 2)) {
                 const int64_t iy = iy0 + yy;
                 const bool row_in = iy >= 0 && iy < hin;
-                const int64_t base = (img * MCH + 2 * cp) * plane + iy * win + ix0;
+                const int64_t base = (img * cin + 2 * cp) * plane + iy * win + ix0;
+                const bool c0 = 2 * cp < cin, c1 = 2 * cp + 1 < cin;
 #pragma unroll 4
                 for (int xx = 0; xx < CS; ++xx) {
                     const int64_t ix = ix0 + xx;
                     const bool ok = row_in && ix >= 0 && ix < win;
-                    store_px(yy, xx, cp, ok ? load_as_float<IN_T>(in, base + xx) : 0.0f,
-                             ok ? load_as_float<IN_T>(in, base + plane + xx) : 0.0f);
+                    store_px(yy, xx, cp, ok && c0 ? load_as_float<IN_T>(in, base + xx) : 0.0f,
+                             ok && c1 ? load_as_float<IN_T>(in, base + plane + xx) : 0.0f);
                 }
             }
         }
-        wr0 = wsrc[0];
-        wr1 = wsrc[1];
-        reinterpret_cast<uint4 *>(wdst0)[0] = wr0;
-        reinterpret_cast<uint4 *>(wdst0)[1] = wr1;
-        wr0 = wsrc[MCH * MCH / 8];   // tap 1's weights, stored to LDS during tap 0
-        wr1 = wsrc[MCH * MCH / 8 + 1];
+        if (w0ok) wr0 = wsrc[0];
+        if (w1ok) wr1 = wsrc[256];
+        store_w(0);
+        if (w0ok) wr0 = wsrc[NW16];   // tap 1's weights, stored to LDS during tap 0
+        if (w1ok) wr1 = wsrc[NW16 + 256];
         __syncthreads();
         const int64_t next = item + nb;
         int64_t nimg_, noy0, nox0;
@@ -427,28 +440,30 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 #pragma unroll
         for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
-            for (int nt = 0; nt < 4; ++nt) acc[mt][nt] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
-        // Software-pipelined over the 50 (tap, channel half) steps: the fragments of the next step
-        // are requested from LDS before the current step's 16 products. Tap t + 1's weights are
-        // stored to LDS slab (t + 1) & 1 between the halves of tap t — that slab was last read before
-        // the barrier that ended tap t − 1 — and the barrier ending tap t publishes them; only the
-        // first weight fragments of each tap are requested after a barrier. Tap t + 2's weights are
+            for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+        // Software-pipelined over the (tap, K slice) steps: the fragments of the next step are
+        // requested from LDS before the current step's products. Tap t + 1's weights are stored to
+        // LDS slab (t + 1) & 1 after tap t's first products — that slab was last read before the
+        // barrier that ended tap t − 1 — and the barrier ending tap t publishes them; only the first
+        // weight fragments of each tap are requested after a barrier. Tap t + 2's weights are
         // requested after that barrier; the next tile's input after tap 24's weights are stored.
-        bf16x8_t a0[4], b0[4], a1[4], b1[4];
+        bf16x8_t a0[4], b0[NT], a1[4], b1[NT];
         frag_a(a0, 0, 0);
         frag_b(b0, 0, 0);
         for (int tap = 0; tap < 25; ++tap) {
             const bool more = tap + 1 < 25;
-            frag_a(a1, tap, 1);
-            frag_b(b1, tap, 1);
+            if constexpr (H == 2) {
+                frag_a(a1, tap, 1);
+                frag_b(b1, tap, 1);
+            } else if (more) {
+                frag_a(a1, tap + 1, 0);
+            }
             __builtin_amdgcn_sched_barrier(0);   // the requests above stay ahead of the products below
             products(a0, b0);
             __builtin_amdgcn_sched_barrier(0);   // the weight stores wait for their loads: after the products
             if (more) {
-                uint16_t *wd = (tap & 1) ? wdst0 : wdst1;
-                reinterpret_cast<uint4 *>(wd)[0] = wr0;
-                reinterpret_cast<uint4 *>(wd)[1] = wr1;
-                frag_a(a0, tap + 1, 0);
+                store_w((tap + 1) & 1);
+                if constexpr (H == 2) frag_a(a0, tap + 1, 0);
             }
             if (tap == 23 && vec_ok && next < hi) {
                 tq = t;   // opaque again: the item addresses are formed here, not held through the taps
@@ -457,13 +472,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
                 for (int i = 0; i < NPF; ++i) load_item(i, nimg_, noy0 - pad, nox0 - pad, pf[i]);
             }
             __builtin_amdgcn_sched_barrier(0);
-            products(a1, b1);
+            if constexpr (H == 2) products(a1, b1);
             if (more) {
                 __syncthreads();
                 frag_b(b0, tap + 1, 0);
+                if constexpr (H == 1) {
+#pragma unroll
+                    for (int mt = 0; mt < 4; ++mt) a0[mt] = a1[mt];
+                }
                 if (tap + 2 < 25) {
-                    wr0 = wsrc[(tap + 2) * (MCH * MCH / 8)];
-                    wr1 = wsrc[(tap + 2) * (MCH * MCH / 8) + 1];
+                    if (w0ok) wr0 = wsrc[(tap + 2) * NW16];
+                    if (w1ok) wr1 = wsrc[(tap + 2) * NW16 + 256];
                 }
                 __builtin_amdgcn_sched_barrier(0);
             }
@@ -475,34 +494,40 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         const bool vec_out = (wout & 3) == 0 && ((reinterpret_cast<uintptr_t>(out) | reinterpret_cast<uintptr_t>(act) |
                                                   reinterpret_cast<uintptr_t>(zg)) & 15) == 0;
         const int64_t px0 = ox0 + 4 * g;
-    #pragma unroll
+#pragma unroll
         for (int mt = 0; mt < 4; ++mt) {
             const int64_t py = oy0 + wave * 4 + mt;
             if (py >= hout) continue;
-            float gz[4][4];   // OUT 2: GELU'(Z) of the four pixels, requested for all channels first
+            float gz[NT][4];   // OUT 2: GELU'(Z) of the four pixels, requested for all channels first
             if constexpr (OUT == 2) {
-    #pragma unroll
-                for (int nt = 0; nt < 4; ++nt) {
-                    const int64_t idx0 = ((img * MCH + nt * 16 + r16) * hout + py) * wout + px0;
-                    if (vec_out && px0 + 4 <= wout) {
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt) {
+                    const int64_t co = nt * 16 + r16;
+                    const int64_t idx0 = ((img * cout + co) * hout + py) * wout + px0;
+                    if (co >= cout) {
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) gz[nt][i] = 0.0f;
+                    } else if (vec_out && px0 + 4 <= wout) {
                         const uint2 z4 = *reinterpret_cast<const uint2 *>(zg + idx0);
                         gz[nt][0] = __uint_as_float(z4.x << 16), gz[nt][1] = __uint_as_float(z4.x & 0xFFFF0000u);
                         gz[nt][2] = __uint_as_float(z4.y << 16), gz[nt][3] = __uint_as_float(z4.y & 0xFFFF0000u);
                     } else {
-    #pragma unroll
+#pragma unroll
                         for (int i = 0; i < 4; ++i) gz[nt][i] = px0 + i < wout ? bf2f(zg[idx0 + i]) : 0.0f;
                     }
                 }
-    #pragma unroll
-                for (int nt = 0; nt < 4; ++nt)
-    #pragma unroll
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
                     for (int i = 0; i < 4; ++i) gz[nt][i] = gelu_grad(gz[nt][i]);
             }
-    #pragma unroll
-            for (int nt = 0; nt < 4; ++nt) {
-                const int64_t idx0 = ((img * MCH + nt * 16 + r16) * hout + py) * wout + px0;
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) {
+                const int64_t co = nt * 16 + r16;
+                if (co >= cout) continue;
+                const int64_t idx0 = ((img * cout + co) * hout + py) * wout + px0;
                 float v[4];
-    #pragma unroll
+#pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     if constexpr (OUT == 2) v[i] = acc[mt][nt][i] * gz[nt][i];
                     else v[i] = acc[mt][nt][i] + bv[nt];
@@ -510,7 +535,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
                 if (vec_out && px0 + 4 <= wout) {
                     if constexpr (OUT == 0) {
                         uint16_t z[4];
-    #pragma unroll
+#pragma unroll
                         for (int i = 0; i < 4; ++i) z[i] = f2bf(v[i]);
                         *reinterpret_cast<uint2 *>(static_cast<uint16_t *>(out) + idx0) =
                             uint2{uint32_t(z[0]) | (uint32_t(z[1]) << 16), uint32_t(z[2]) | (uint32_t(z[3]) << 16)};
@@ -521,7 +546,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
                         *reinterpret_cast<float4 *>(static_cast<float *>(out) + idx0) = float4{v[0], v[1], v[2], v[3]};
                     }
                 } else {
-    #pragma unroll
+#pragma unroll
                     for (int i = 0; i < 4; ++i) {
                         if (px0 + i >= wout) continue;
                         if constexpr (OUT == 0) {
@@ -542,13 +567,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 
 // BF16 matrix-core layouts of a 64 × 64 × 25 master weight tensor: forward wmf[k][o][i] and
 // backward-data wmb[24 − k][i][o].
-__global__ void k_pack_mfma(const float *__restrict__ p, uint16_t *wmf, uint16_t *wmb) {
+__global__ void k_pack_mfma(const float *__restrict__ p, int64_t cout, int64_t cin, uint16_t *wmf, int fo, int fi,
+                            uint16_t *wmb, int bo, int bi) {
     const int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (idx >= MCH * MCH * 25) return;
-    const int64_t k = idx % 25, oi = idx / 25, i = oi % MCH, o = oi / MCH;
-    const uint16_t h = f2bf(p[idx]);
-    if (wmf) wmf[(k * MCH + o) * MCH + i] = h;
-    if (wmb) wmb[((24 - k) * MCH + i) * MCH + o] = h;
+    if (wmf && idx < 25 * fo * fi) {   // [k][o][i], zero beyond cout × cin
+        const int64_t k = idx / (fo * fi), o = (idx / fi) % fo, i = idx % fi;
+        wmf[idx] = o < cout && i < cin ? f2bf(p[(o * cin + i) * 25 + k]) : 0;
+    }
+    if (wmb && idx < 25 * bo * bi) {   // [24 − k][i][o]: the data gradient's input is dZ (cout channels)
+        const int64_t kk = idx / (bo * bi), i = (idx / bi) % bo, o = idx % bi;
+        wmb[idx] = o < cout && i < cin ? f2bf(p[(o * cin + i) * 25 + (24 - kk)]) : 0;
+    }
 }
 
 // Weight and bias gradients of a valid 5×5 convolution. Block = OG output × IG input channels (one
@@ -1060,6 +1089,18 @@ unsigned blocks_for(int64_t n, int b = 256) { return unsigned((n + b - 1) / b); 
 
 void check_launch() { CGEF_HIP(hipGetLastError()); }
 
+// Matrix-core layer shapes: input channels padded to one or two 32-wide K slices per tap, output
+// channels to one or four 16-wide N tiles.
+int mfma_cin_pad(int64_t cin) { return cin <= 32 ? 32 : 64; }
+int mfma_cout_pad(int64_t cout) { return cout <= 16 ? 16 : 64; }
+
+// Workgroups of 256 threads of kernel fn that one CU holds at once (at least 1).
+int resident_blocks(const void *fn) {
+    int nb = 0;
+    CGEF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, 256, 0));
+    return nb > 0 ? nb : 1;
+}
+
 // Compute units of the current device (cached per device).
 int device_cus() {
     static int cus[64] = {};
@@ -1205,15 +1246,20 @@ cgef_status cgef_conv5(const void *in, int32_t in_dtype, int32_t in_gelu, int64_
     });
 }
 
-cgef_status cgef_conv5_mfma(const void *in, int32_t in_dtype, int32_t in_gelu, int64_t n, int64_t hin, int64_t win,
-                            const uint16_t *wm, const float *bias, int32_t pad, void *out, int32_t out_dtype,
-                            const uint16_t *zgrad, uint16_t *act, void *stream) {
+cgef_status cgef_conv5_mfma_ex(const void *in, int32_t in_dtype, int32_t in_gelu, int64_t n, int64_t cin, int64_t hin,
+                               int64_t win, const uint16_t *wm, const float *bias, int64_t cout, int32_t pad, void *out,
+                               int32_t out_dtype, const uint16_t *zgrad, uint16_t *act, void *stream) {
     return guarded([&] {
         need(in && wm && out, CGEF_ERR_ARGUMENT, "cgef_conv5_mfma: null pointer");
         need(in_dtype == CGEF_BF16 || (in_dtype == CGEF_F32 && !in_gelu), CGEF_ERR_ARGUMENT,
              "cgef_conv5_mfma: input must be BF16 (optionally GELU-applied) or FP32");
         need(pad == 0 || pad == 4, CGEF_ERR_ARGUMENT, "cgef_conv5_mfma: pad must be 0 (valid) or 4 (full)");
         need(n >= 1 && n <= 65535, CGEF_ERR_DIMENSION, "cgef_conv5_mfma: bad n");
+        need(cin >= 1 && cin <= MCH && cout >= 1 && cout <= MCH, CGEF_ERR_DIMENSION,
+             "cgef_conv5_mfma: 1 to 64 input and output channels");
+        const int cp = mfma_cin_pad(cin), no = mfma_cout_pad(cout);
+        need(!in_gelu || (cp == 64 && no == 64), CGEF_ERR_UNSUPPORTED,
+             "cgef_conv5_mfma: GELU on load only for 33-64 input and output channels");
         const int64_t hout = hin + 2 * pad - 4, wout = win + 2 * pad - 4;
         need(hin >= 1 && win >= 1 && hout >= 1 && wout >= 1, CGEF_ERR_DIMENSION,
              "cgef_conv5_mfma: input smaller than 5x5");
@@ -1230,38 +1276,70 @@ cgef_status cgef_conv5_mfma(const void *in, int32_t in_dtype, int32_t in_gelu, i
         need(!act || out_kind == 0, CGEF_ERR_ARGUMENT, "cgef_conv5_mfma: the activation is written with a BF16 output");
         const int tiles_x = int((wout + CT - 1) / CT);
         const int64_t ntiles = int64_t(tiles_x) * ((hout + CT - 1) / CT);
-        // persistent workgroups: two per CU (the LDS of one is 76 KB), in 8 equal XCD groups
-        const int64_t items = ntiles * n, per_xcd = std::min<int64_t>((items + 7) / 8, 2 * std::max(1, device_cus() / 8));
-        dim3 g(unsigned(8 * per_xcd), 1, 1);
+        const int64_t items = ntiles * n;
         hipStream_t st = static_cast<hipStream_t>(stream);
-#define CGEF_MFMA_LAUNCH(IT, IG, OK)                                                                                  \
-    hipLaunchKernelGGL((k_conv5_mfma<IT, IG, OK>), g, dim3(256), 0, st, in, wm, bias, zgrad, out, act, hin, win, hout, \
-                       wout, pad, tiles_x, ntiles, n)
-        if (in_dtype == CGEF_BF16 && in_gelu) {
-            if (out_kind == 0) CGEF_MFMA_LAUNCH(CGEF_BF16, 1, 0);
-            else if (out_kind == 1) CGEF_MFMA_LAUNCH(CGEF_BF16, 1, 1);
-            else CGEF_MFMA_LAUNCH(CGEF_BF16, 1, 2);
-        } else if (in_dtype == CGEF_BF16) {
-            if (out_kind == 0) CGEF_MFMA_LAUNCH(CGEF_BF16, 0, 0);
-            else if (out_kind == 1) CGEF_MFMA_LAUNCH(CGEF_BF16, 0, 1);
-            else CGEF_MFMA_LAUNCH(CGEF_BF16, 0, 2);
-        } else {
-            if (out_kind == 0) CGEF_MFMA_LAUNCH(CGEF_F32, 0, 0);
-            else if (out_kind == 1) CGEF_MFMA_LAUNCH(CGEF_F32, 0, 1);
-            else CGEF_MFMA_LAUNCH(CGEF_F32, 0, 2);
-        }
-#undef CGEF_MFMA_LAUNCH
+        // persistent workgroups: as many per CU as the kernel's LDS and registers allow, in 8 equal XCD groups
+        auto launch = [&](const void *fn, auto kern) {
+            const int per_cu = resident_blocks(fn);
+            const int64_t per_xcd = std::min<int64_t>((items + 7) / 8, int64_t(per_cu) * std::max(1, device_cus() / 8));
+            hipLaunchKernelGGL(kern, dim3(unsigned(8 * per_xcd)), dim3(256), 0, st, in, wm, bias, zgrad, out, act, cin,
+                               cout, hin, win, hout, wout, pad, tiles_x, ntiles, n);
+        };
+#define CGEF_MFMA_K(IT, IG, OK, CPV, NTV)                                                                            \
+    do {                                                                                                             \
+        if (CPV == 64 && NTV == 4 && cin == 64 && cout == 64)                                                        \
+            launch((const void *)k_conv5_mfma<IT, IG, OK, CPV, NTV, (CPV == 64 && NTV == 4)>,                       \
+                   k_conv5_mfma<IT, IG, OK, CPV, NTV, (CPV == 64 && NTV == 4)>);                                     \
+        else                                                                                                         \
+            launch((const void *)k_conv5_mfma<IT, IG, OK, CPV, NTV, 0>, k_conv5_mfma<IT, IG, OK, CPV, NTV, 0>);     \
+    } while (0)
+#define CGEF_MFMA_OUT(IT, IG, CPV, NTV)                                              \
+    do {                                                                             \
+        if (out_kind == 0) CGEF_MFMA_K(IT, IG, 0, CPV, NTV);                         \
+        else if (out_kind == 1) CGEF_MFMA_K(IT, IG, 1, CPV, NTV);                    \
+        else CGEF_MFMA_K(IT, IG, 2, CPV, NTV);                                       \
+    } while (0)
+#define CGEF_MFMA_IN(CPV, NTV)                                                       \
+    do {                                                                             \
+        if (in_dtype == CGEF_BF16) CGEF_MFMA_OUT(CGEF_BF16, 0, CPV, NTV);            \
+        else CGEF_MFMA_OUT(CGEF_F32, 0, CPV, NTV);                                   \
+    } while (0)
+        if (cp == 64 && no == 64) {
+            if (in_dtype == CGEF_BF16 && in_gelu) CGEF_MFMA_OUT(CGEF_BF16, 1, 64, 4);
+            else CGEF_MFMA_IN(64, 4);
+        } else if (cp == 64) CGEF_MFMA_IN(64, 1);
+        else if (no == 64) CGEF_MFMA_IN(32, 4);
+        else CGEF_MFMA_IN(32, 1);
+#undef CGEF_MFMA_IN
+#undef CGEF_MFMA_OUT
+#undef CGEF_MFMA_K
+        check_launch();
+    });
+}
+
+cgef_status cgef_conv5_mfma(const void *in, int32_t in_dtype, int32_t in_gelu, int64_t n, int64_t hin, int64_t win,
+                            const uint16_t *wm, const float *bias, int32_t pad, void *out, int32_t out_dtype,
+                            const uint16_t *zgrad, uint16_t *act, void *stream) {
+    return cgef_conv5_mfma_ex(in, in_dtype, in_gelu, n, MCH, hin, win, wm, bias, MCH, pad, out, out_dtype, zgrad, act,
+                              stream);
+}
+
+cgef_status cgef_conv5_pack_mfma_ex(const float *w, int64_t cout, int64_t cin, uint16_t *wmf, uint16_t *wmb,
+                                    void *stream) {
+    return guarded([&] {
+        need(w && (wmf || wmb), CGEF_ERR_ARGUMENT, "cgef_conv5_pack_mfma: null pointer");
+        need(cin >= 1 && cin <= MCH && cout >= 1 && cout <= MCH, CGEF_ERR_DIMENSION,
+             "cgef_conv5_pack_mfma: 1 to 64 input and output channels");
+        const int fo = mfma_cout_pad(cout), fi = mfma_cin_pad(cin), bo = mfma_cout_pad(cin), bi = mfma_cin_pad(cout);
+        const int64_t count = 25 * std::max(fo * fi, bo * bi);
+        hipLaunchKernelGGL(k_pack_mfma, dim3(blocks_for(count)), dim3(256), 0, static_cast<hipStream_t>(stream), w, cout,
+                           cin, wmf, fo, fi, wmb, bo, bi);
         check_launch();
     });
 }
 
 cgef_status cgef_conv5_pack_mfma(const float *w, uint16_t *wmf, uint16_t *wmb, void *stream) {
-    return guarded([&] {
-        need(w && (wmf || wmb), CGEF_ERR_ARGUMENT, "cgef_conv5_pack_mfma: null pointer");
-        hipLaunchKernelGGL(k_pack_mfma, dim3(blocks_for(MCH * MCH * 25)), dim3(256), 0,
-                           static_cast<hipStream_t>(stream), w, wmf, wmb);
-        check_launch();
-    });
+    return cgef_conv5_pack_mfma_ex(w, MCH, MCH, wmf, wmb, stream);
 }
 
 cgef_status cgef_conv5_wgrad_scratch(int64_t n, int64_t cin, int64_t cout, int64_t hout, int64_t wout, int32_t mfma,

@@ -226,6 +226,23 @@ cgef_status cgef_conv5_mfma(const void *in, int32_t in_dtype, int32_t in_gelu, i
  * backward-data wmb[24 − k][i][o] (either may be NULL), 64·64·25 entries each. */
 cgef_status cgef_conv5_pack_mfma(const float *w, uint16_t *wmf, uint16_t *wmb, void *stream);
 
+/* cgef_conv5_mfma for 1 to 64 input and output channels (the end layers of the closure network:
+ * 4 → 64, 64 → 3 and the data gradient through the latter). On the matrix cores the input channels
+ * are padded to cin_pad = 32 (cin ≤ 32) or 64 and the output channels to cout_pad = 16 (cout ≤ 16)
+ * or 64, with zeros; wm is the layout of cgef_conv5_pack_mfma_ex for this (cin, cout) — its forward
+ * layout for pad 0, its backward-data layout (a layer cout → cin) for pad 4. in_gelu only with 33 to
+ * 64 channels on both sides. Other arguments as in cgef_conv5. */
+cgef_status cgef_conv5_mfma_ex(const void *in, int32_t in_dtype, int32_t in_gelu, int64_t n, int64_t cin, int64_t hin,
+                               int64_t win, const uint16_t *wm, const float *bias, int64_t cout, int32_t pad, void *out,
+                               int32_t out_dtype, const uint16_t *zgrad, uint16_t *act, void *stream);
+
+/* BF16 matrix-core layouts of a cout × cin master weight tensor W[o][i][25] (1 to 64 each):
+ * forward wmf[k][o][i] of 25 · pad16(cout) · pad32(cin) entries and backward-data wmb[24 − k][i][o]
+ * of 25 · pad16(cin) · pad32(cout) entries, where pad16(c) = 16 for c ≤ 16 else 64 and pad32(c) = 32
+ * for c ≤ 32 else 64; padding entries are zero. Either output may be NULL. */
+cgef_status cgef_conv5_pack_mfma_ex(const float *w, int64_t cout, int64_t cin, uint16_t *wmf, uint16_t *wmb,
+                                    void *stream);
+
 /* Weight and bias gradients of a valid 5×5 layer: dw[o][i][k] = Σ dz[n][o][p]·A[n][i][p+k],
  * db[o] = Σ dz[n][o][p], with A the layer input (n, cin, hout+4, wout+4) as in cgef_conv5 and dz
  * FP32 (n, cout, hout, wout). mfma = 1 (64 → 64 layers only) runs the GEMMs on the matrix cores with
