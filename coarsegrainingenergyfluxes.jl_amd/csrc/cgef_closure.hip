@@ -581,8 +581,8 @@ __global__ void k_pack_mfma(const float *__restrict__ p, int64_t cout, int64_t c
 }
 
 // Weight and bias gradients of a valid 5×5 convolution. Block = OG output × IG input channels (one
-// (o, i) pair per lane, its 25 taps in registers; OG·IG = 256: 16 × 16, or 64 × 4 for an input of at
-// most 4 channels so that no lane idles) over a range of 16×16 output tiles of one image; each tile
+// (o, i) pair per lane, its 25 taps in registers; OG·IG = 256: 16 × 16, 64 × 4 for an input of at
+// most 4 channels so that no lane idles, 8 × 32 for at most 8 output channels) over a range of 16×16 output tiles of one image; each tile
 // stages dZ (OG channels × 256 pixels) and the input (IG channels × 20×20) in LDS, and a lane walks
 // each output row with a 5×5 register window of the input, 6 LDS reads per 25 FMAs. The block's sums
 // go to its own slot of `part` (deterministic; k_wgrad_reduce adds the slots).
@@ -1162,7 +1162,9 @@ void wgrad_split(bool mfma, int64_t n, int64_t cin, int64_t cout, int64_t hout, 
                  int &ntiles, int &tpb, int &nsplit, int &nig, int &nog) {
     tiles_x = int((wout + CT - 1) / CT);
     ntiles = tiles_x * int((hout + (mfma ? WTY : CT) - 1) / (mfma ? WTY : CT));
-    const int ig = cin <= 4 ? 4 : WG, og = cin <= 4 ? 64 : WG;   // k_conv5_wgrad's block shape (OG × IG)
+    // k_conv5_wgrad's block shape (OG × IG): 64 × 4 for a ≤ 4-channel input, 8 × 32 for ≤ 8 output
+    // channels (the 64 → 3 layer: 3 of 8 lane rows busy instead of 3 of 16), else 16 × 16
+    const int ig = cin <= 4 ? 4 : (cout <= 8 ? 32 : WG), og = cin <= 4 ? 64 : (cout <= 8 ? 8 : WG);
     nig = int((cin + ig - 1) / ig);
     nog = int((cout + og - 1) / og);
     const int64_t base = mfma ? 5 * n : n * nig * nog;
@@ -1399,6 +1401,7 @@ cgef_status cgef_conv5_wgrad(const float *dz, const void *in, int32_t in_dtype, 
         else go(k_conv5_wgrad<OGV, IGV, CGEF_F32, 0>);                                           \
     } while (0)
             if (cin <= 4) CGEF_WG(64, 4);
+            else if (cout <= 8) CGEF_WG(8, 32);
             else CGEF_WG(WG, WG);
 #undef CGEF_WG
         }
