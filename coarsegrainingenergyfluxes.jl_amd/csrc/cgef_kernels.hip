@@ -1056,33 +1056,103 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     }
 }
 
+// Column pass for grids with too few 64 × 64 tiles to fill the chip: one lane per column walking
+// TJ output rows (each read row feeds every output row within reach), the planes read from memory.
+template <typename T, int TJ>
+__global__ __launch_bounds__(256) void k_sep_pi_col_lanes(GeomArgs G, const double *__restrict__ gy, int ly,
+                                                          const double *__restrict__ rp, const double *__restrict__ px,
+                                                          const double *__restrict__ py,
+                                                          const double *__restrict__ invden, double *__restrict__ ubar,
+                                                          T *__restrict__ tau, T *__restrict__ uo, T *__restrict__ vo,
+                                                          T *__restrict__ wo, int64_t nchunk) {
+    const int64_t b = blockIdx.y;
+    const int64_t jt = (int64_t)blockIdx.x / nchunk;
+    const int64_t i = ((int64_t)blockIdx.x - jt * nchunk) * 256 + threadIdx.x;
+    const int64_t Nx = G.n[0], Ny = G.n[1], P = G.npts;
+    if (i >= Nx) return;
+    const int64_t j0 = jt * TJ;
+    const double *r = rp + b * 5 * P;
+    double F[TJ][5];
+#pragma unroll
+    for (int q = 0; q < TJ; q++)
+#pragma unroll
+        for (int f = 0; f < 5; f++) F[q][f] = 0.0;
+    for (int64_t jj = j0 - ly; jj < j0 + TJ + ly; jj++) {
+        int64_t jw = jj;
+        if (jw < 0 || jw >= Ny) {
+            if (!G.periodic[1]) continue;
+            jw = wrapi(jw, Ny);
+        }
+        const int64_t qi = i + Nx * jw;
+        double x[5];
+#pragma unroll
+        for (int f = 0; f < 5; f++) x[f] = r[f * P + qi];
+#pragma unroll
+        for (int q = 0; q < TJ; q++) {
+            const int64_t d = jj - (j0 + q);
+            if (d < -ly || d > ly) continue;
+            const double g = gy[d + ly];
+#pragma unroll
+            for (int f = 0; f < 5; f++) F[q][f] += g * x[f];
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < TJ; q++) {
+        const int64_t j = j0 + q;
+        if (j >= Ny) break;
+        const int64_t t = i + Nx * j;
+        bool ok;
+        double inv;
+        if (invden) {
+            inv = invden[t];
+            ok = G.mask[t] != 0 && inv > 0.0;
+        } else {
+            const double den = px[i] * py[j];
+            ok = G.mask[t] != 0 && den > 1e-15;
+            inv = ok ? 1.0 / den : 0.0;
+        }
+        pi_point_epilogue_inv<T>(G, t, i, j, F[q], ok, inv, b, ubar, tau, uo, vo, wo);
+    }
+}
+
 void launch_separable_pi(const GeomArgs &G, const Plan &p, int dtype, const void *u, const void *v, int64_t nbatch,
                          double *rp, double *ubar, void *tau, void *uo, void *vo, void *wo, hipStream_t s) {
     const int64_t nchunk = (G.n[0] + 255) / 256;
     const int lx = p.sep_lim[0], ly = p.sep_lim[1];
     const size_t sh = (size_t)(2 * (256 + 2 * lx) + 2 * lx + 1) * sizeof(double);
     const size_t shc = (size_t)(2 * ly + 1 + (SEPC_H + 2 * ly) * SEPC_W) * sizeof(double);
-    if (sh > 160 * 1024 || shc > 160 * 1024)
-        fail(CGEF_ERR_UNSUPPORTED, "separable half-width too large for the LDS of the row and column passes");
+    if (sh > 160 * 1024) fail(CGEF_ERR_UNSUPPORTED, "separable row half-width too large for the row pass");
+    const int64_t ntx = (G.n[0] + SEPC_W - 1) / SEPC_W, nty = (G.n[1] + SEPC_H - 1) / SEPC_H;
+    // the tiled column pass where its 64 × 64 tiles fill the chip and the tile fits in LDS; else a lane
+    // per column, 8 output rows per lane where that still leaves ≥ 1024 workgroups, else one
+    const bool tiled = ntx * nty * nbatch >= 1024 && shc <= 160 * 1024;
     const void *fr = dtype == CGEF_F32 ? (const void *)k_sep_pi_row<float> : (const void *)k_sep_pi_row<double>;
     const void *fc = dtype == CGEF_F32 ? (const void *)k_sep_pi_col<float> : (const void *)k_sep_pi_col<double>;
     if (sh > 64 * 1024) CGEF_HIP(hipFuncSetAttribute(fr, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
-    if (shc > 64 * 1024) CGEF_HIP(hipFuncSetAttribute(fc, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shc));
-    const int64_t ntx = (G.n[0] + SEPC_W - 1) / SEPC_W, nty = (G.n[1] + SEPC_H - 1) / SEPC_H;
+    if (tiled && shc > 64 * 1024) CGEF_HIP(hipFuncSetAttribute(fc, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shc));
     const dim3 gr((unsigned)(nchunk * G.n[1]), (unsigned)nbatch);
     const dim3 gc((unsigned)(ntx * nty), (unsigned)nbatch);
+    const bool tall = nchunk * ((G.n[1] + 7) / 8) * nbatch >= 1024;
+    const int TJ = tall ? 8 : 1;
+    const dim3 gl((unsigned)(nchunk * ((G.n[1] + TJ - 1) / TJ)), (unsigned)nbatch);
+    auto col = [&](auto tiled_k, auto lanes8_k, auto lanes1_k, auto *tauT) {
+        using TT = std::remove_pointer_t<decltype(tauT)>;
+        if (tiled)
+            hipLaunchKernelGGL(tiled_k, gc, dim3(512), shc, s, G, p.sep_g_d[1], ly, rp, p.sep_prof_d[0],
+                               p.sep_prof_d[1], p.sep_invden_d, ubar, (TT *)tau, (TT *)uo, (TT *)vo, (TT *)wo, ntx);
+        else
+            hipLaunchKernelGGL(tall ? lanes8_k : lanes1_k, gl, dim3(256), 0, s, G, p.sep_g_d[1], ly, rp,
+                               p.sep_prof_d[0], p.sep_prof_d[1], p.sep_invden_d, ubar, (TT *)tau, (TT *)uo, (TT *)vo,
+                               (TT *)wo, nchunk);
+    };
     if (dtype == CGEF_F32) {
         hipLaunchKernelGGL(k_sep_pi_row<float>, gr, dim3(256), sh, s, G, p.sep_g_d[0], lx, (const float *)u,
                            (const float *)v, rp, nchunk);
-        hipLaunchKernelGGL(k_sep_pi_col<float>, gc, dim3(512), shc, s, G, p.sep_g_d[1], ly, rp, p.sep_prof_d[0],
-                           p.sep_prof_d[1], p.sep_invden_d, ubar, (float *)tau, (float *)uo, (float *)vo, (float *)wo,
-                           ntx);
+        col(k_sep_pi_col<float>, k_sep_pi_col_lanes<float, 8>, k_sep_pi_col_lanes<float, 1>, (float *)nullptr);
     } else {
         hipLaunchKernelGGL(k_sep_pi_row<double>, gr, dim3(256), sh, s, G, p.sep_g_d[0], lx, (const double *)u,
                            (const double *)v, rp, nchunk);
-        hipLaunchKernelGGL(k_sep_pi_col<double>, gc, dim3(512), shc, s, G, p.sep_g_d[1], ly, rp, p.sep_prof_d[0],
-                           p.sep_prof_d[1], p.sep_invden_d, ubar, (double *)tau, (double *)uo, (double *)vo,
-                           (double *)wo, ntx);
+        col(k_sep_pi_col<double>, k_sep_pi_col_lanes<double, 8>, k_sep_pi_col_lanes<double, 1>, (double *)nullptr);
     }
 }
 
