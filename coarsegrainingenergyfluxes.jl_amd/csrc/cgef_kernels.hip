@@ -987,6 +987,7 @@ __global__ __launch_bounds__(256) void k_sep_pi_row(GeomArgs G, const double *__
 // window that shifts by one per row (taps outside the kernel are 0, and a row beyond a non-periodic
 // edge reads as 0: adding +0 leaves a sum unchanged, so this equals skipping them).
 constexpr int SEPC_W = 64, SEPC_H = 64, SEPC_TJ = 8;
+constexpr int SEPC_AH = 32;   // rows per workgroup of the column pass with direct global → LDS loads
 template <typename T>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_sep_pi_col(GeomArgs G, const double *__restrict__ gy, int ly,
                                                     const double *__restrict__ rp, const double *__restrict__ px,
@@ -1027,6 +1028,91 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
 #pragma unroll
         for (int q = 0; q < SEPC_TJ; q++) g[q] = 0.0;
         const double *col = st + band * SEPC_TJ * SEPC_W + c;
+        for (int k = 0; k < taps + SEPC_TJ - 1; k++) {
+#pragma unroll
+            for (int q = SEPC_TJ - 1; q > 0; q--) g[q] = g[q - 1];
+            g[0] = k < taps ? sg[k] : 0.0;
+            const double x = col[k * SEPC_W];
+#pragma unroll
+            for (int q = 0; q < SEPC_TJ; q++) F[q][f] += g[q] * x;
+        }
+    }
+    if (i >= Nx) return;
+#pragma unroll
+    for (int q = 0; q < SEPC_TJ; q++) {
+        const int64_t j = j0 + q;
+        if (j >= Ny) break;
+        const int64_t t = i + Nx * j;
+        bool ok;
+        double inv;
+        if (invden) {
+            inv = invden[t];
+            ok = G.mask[t] != 0 && inv > 0.0;
+        } else {
+            const double den = px[i] * py[j];
+            ok = G.mask[t] != 0 && den > 1e-15;
+            inv = ok ? 1.0 / den : 0.0;
+        }
+        pi_point_epilogue_inv<T>(G, t, i, j, F[q], ok, inv, b, ubar, tau, uo, vo, wo);
+    }
+}
+
+// The same tiled column pass with the planes streamed into two LDS buffers by direct global → LDS
+// loads (global_load_lds_dwordx4: lane l's 16 bytes land at the wave's LDS base + 16·l, no registers
+// on the way), so plane f + 1 is in flight while plane f is summed. A wave's load covers two 64-column
+// rows; a row beyond a non-periodic edge is written as zeros by its lanes instead. Needs an even Nx
+// (16-byte aligned pairs); columns beyond Nx read the row's last pair (their outputs are dropped).
+template <typename T, int TH>
+__global__ __launch_bounds__(TH * 8) __attribute__((amdgpu_waves_per_eu(2, 4))) void k_sep_pi_col_async(
+    GeomArgs G, const double *__restrict__ gy, int ly, const double *__restrict__ rp, const double *__restrict__ px,
+    const double *__restrict__ py, const double *__restrict__ invden, double *__restrict__ ubar, T *__restrict__ tau,
+    T *__restrict__ uo, T *__restrict__ vo, T *__restrict__ wo, int64_t ntx) {
+    extern __shared__ __attribute__((aligned(16))) double smem[];   // taps, then two planes' 64 + 2·ly rows × 64
+    constexpr int NT = TH * 8, NW = NT / 64;   // threads, waves: one 8-row band per wave
+    const int taps = 2 * ly + 1, R = TH + 2 * ly, tpad = (taps + 1) & ~1;
+    double *sg = smem, *buf0 = smem + tpad, *buf1 = buf0 + R * SEPC_W;
+    const int64_t b = blockIdx.y;
+    const int64_t tx = (int64_t)blockIdx.x % ntx, ty = (int64_t)blockIdx.x / ntx;
+    const int64_t Nx = G.n[0], Ny = G.n[1], P = G.npts;
+    const int c = threadIdx.x & (SEPC_W - 1), band = threadIdx.x / SEPC_W, wave = threadIdx.x >> 6;
+    const int64_t i = tx * SEPC_W + c, j0 = ty * TH + band * SEPC_TJ;
+    const double *r = rp + b * 5 * P;
+    for (int e = threadIdx.x; e < taps; e += NT) sg[e] = gy[e];
+    // lane l of a wave: row 2·pair + l / 32 of the pair, columns 2·(l % 32) and + 1
+    const int lane = threadIdx.x & 63, half = lane >> 5, cc = 2 * (lane & 31);
+    const int64_t ic = tx * SEPC_W + cc < Nx ? tx * SEPC_W + cc : Nx - 2;
+    auto stage = [&](int f, double *buf) {
+        for (int pr = wave; pr < R / 2; pr += NW) {
+            const int k = 2 * pr + half;
+            int64_t jw = ty * TH - ly + k;
+            bool in = true;
+            if (jw < 0 || jw >= Ny) {
+                if (G.periodic[1]) jw = wrapi(jw, Ny);
+                else in = false;
+            }
+            if (in)
+                __builtin_amdgcn_global_load_lds((const void *)(r + f * P + jw * Nx + ic),
+                                                 (void __attribute__((address_space(3))) *)(buf + 2 * pr * SEPC_W), 16,
+                                                 0, 0);
+            else
+                *reinterpret_cast<double2 *>(buf + k * SEPC_W + cc) = make_double2(0.0, 0.0);
+        }
+    };
+    double F[SEPC_TJ][5];
+#pragma unroll
+    for (int q = 0; q < SEPC_TJ; q++)
+#pragma unroll
+        for (int f = 0; f < 5; f++) F[q][f] = 0.0;
+    stage(0, buf0);
+#pragma unroll
+    for (int f = 0; f < 5; f++) {
+        __builtin_amdgcn_s_waitcnt(0);   // this wave's part of plane f has landed
+        __syncthreads();                 // everyone's; and plane f − 1's readers are done with the other buffer
+        if (f < 4) stage(f + 1, (f & 1) ? buf0 : buf1);
+        double g[SEPC_TJ];
+#pragma unroll
+        for (int q = 0; q < SEPC_TJ; q++) g[q] = 0.0;
+        const double *col = ((f & 1) ? buf1 : buf0) + band * SEPC_TJ * SEPC_W + c;
         for (int k = 0; k < taps + SEPC_TJ - 1; k++) {
 #pragma unroll
             for (int q = SEPC_TJ - 1; q > 0; q--) g[q] = g[q - 1];
@@ -1126,6 +1212,10 @@ void launch_separable_pi(const GeomArgs &G, const Plan &p, int dtype, const void
     // the tiled column pass where its 64 × 64 tiles fill the chip and the tile fits in LDS; else a lane
     // per column, 8 output rows per lane where that still leaves ≥ 1024 workgroups, else one
     const bool tiled = ntx * nty * nbatch >= 1024 && shc <= 160 * 1024;
+    // two plane buffers filled by direct global → LDS loads where they fit and Nx is even
+    const size_t sha = (size_t)(((2 * ly + 2) & ~1) + 2 * (SEPC_AH + 2 * ly) * SEPC_W) * sizeof(double);
+    const bool async = tiled && sha <= 160 * 1024 && G.n[0] % 2 == 0 && G.n[0] >= 2 &&
+                       (reinterpret_cast<uintptr_t>(rp) & 15) == 0;
     const void *fr = dtype == CGEF_F32 ? (const void *)k_sep_pi_row<float> : (const void *)k_sep_pi_row<double>;
     const void *fc = dtype == CGEF_F32 ? (const void *)k_sep_pi_col<float> : (const void *)k_sep_pi_col<double>;
     if (sh > 64 * 1024) CGEF_HIP(hipFuncSetAttribute(fr, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
@@ -1135,9 +1225,15 @@ void launch_separable_pi(const GeomArgs &G, const Plan &p, int dtype, const void
     const bool tall = nchunk * ((G.n[1] + 7) / 8) * nbatch >= 1024;
     const int TJ = tall ? 8 : 1;
     const dim3 gl((unsigned)(nchunk * ((G.n[1] + TJ - 1) / TJ)), (unsigned)nbatch);
-    auto col = [&](auto tiled_k, auto lanes8_k, auto lanes1_k, auto *tauT) {
+    auto col = [&](auto async_k, auto tiled_k, auto lanes8_k, auto lanes1_k, auto *tauT) {
         using TT = std::remove_pointer_t<decltype(tauT)>;
-        if (tiled)
+        if (async) {
+            if (sha > 64 * 1024)
+                CGEF_HIP(hipFuncSetAttribute((const void *)async_k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sha));
+            const dim3 ga((unsigned)(ntx * ((G.n[1] + SEPC_AH - 1) / SEPC_AH)), (unsigned)nbatch);
+            hipLaunchKernelGGL(async_k, ga, dim3(SEPC_AH * 8), sha, s, G, p.sep_g_d[1], ly, rp, p.sep_prof_d[0],
+                               p.sep_prof_d[1], p.sep_invden_d, ubar, (TT *)tau, (TT *)uo, (TT *)vo, (TT *)wo, ntx);
+        } else if (tiled)
             hipLaunchKernelGGL(tiled_k, gc, dim3(512), shc, s, G, p.sep_g_d[1], ly, rp, p.sep_prof_d[0],
                                p.sep_prof_d[1], p.sep_invden_d, ubar, (TT *)tau, (TT *)uo, (TT *)vo, (TT *)wo, ntx);
         else
@@ -1148,11 +1244,13 @@ void launch_separable_pi(const GeomArgs &G, const Plan &p, int dtype, const void
     if (dtype == CGEF_F32) {
         hipLaunchKernelGGL(k_sep_pi_row<float>, gr, dim3(256), sh, s, G, p.sep_g_d[0], lx, (const float *)u,
                            (const float *)v, rp, nchunk);
-        col(k_sep_pi_col<float>, k_sep_pi_col_lanes<float, 8>, k_sep_pi_col_lanes<float, 1>, (float *)nullptr);
+        col(k_sep_pi_col_async<float, SEPC_AH>, k_sep_pi_col<float>, k_sep_pi_col_lanes<float, 8>, k_sep_pi_col_lanes<float, 1>,
+            (float *)nullptr);
     } else {
         hipLaunchKernelGGL(k_sep_pi_row<double>, gr, dim3(256), sh, s, G, p.sep_g_d[0], lx, (const double *)u,
                            (const double *)v, rp, nchunk);
-        col(k_sep_pi_col<double>, k_sep_pi_col_lanes<double, 8>, k_sep_pi_col_lanes<double, 1>, (double *)nullptr);
+        col(k_sep_pi_col_async<double, SEPC_AH>, k_sep_pi_col<double>, k_sep_pi_col_lanes<double, 8>,
+            k_sep_pi_col_lanes<double, 1>, (double *)nullptr);
     }
 }
 
