@@ -288,7 +288,7 @@ static void ws_reserve(Workspace &w, int64_t nbatch, int dtype) {
     CGEF_HIP(hipMalloc(&w.ubar, (size_t)P * nbatch * 3 * sizeof(double)));
     CGEF_HIP(hipMalloc(&w.tau, (size_t)P * nbatch * 6 * es));
     CGEF_HIP(hipMalloc(&w.cmean, (size_t)nbatch * 3 * sizeof(double)));
-    const int64_t nblk = (g.n[0] + 255) / 256 * g.n[1] * g.n[2];  // strain_blocks
+    const int64_t nblk = (g.n[0] + 255) / 256 * g.n[1] * g.n[2];  // energy_blocks ≥ strain_blocks
     w.npartials = nbatch * std::max<int64_t>(nblk, 3 * 1024);
     CGEF_HIP(hipMalloc(&w.partials, (size_t)w.npartials * sizeof(double)));
     CGEF_HIP(hipMalloc(&w.E_dev, (size_t)nbatch * sizeof(double)));
@@ -1113,7 +1113,7 @@ cgef_status cgef_energy(const cgef_grid *grid, const void *ubar, const void *vba
         Workspace &W = resolve_ws(*g, nullptr, nbatch, dtype);
         const GeomArgs G = geom_args(*g);
         launch_energy_fields(G, dtype, ubar, vbar, wbar, nbatch, W.partials, W.npartials, s);
-        fetch_energy(W, nbatch, strain_blocks(G), g->active_area, E_out, s);
+        fetch_energy(W, nbatch, energy_blocks(G), g->active_area, E_out, s);
     });
 }
 
@@ -1157,7 +1157,7 @@ cgef_status cgef_cumulative_energy(cgef_plan *const *plans, int64_t nscales, cge
                 ring_inverse(p.ring, W.rxf, W.rxy, nsl, s);
                 ring_spectral_normalize(G, dtype, W.rxy, p.ring.invrenorm_d, nsl, fb, s);
                 launch_energy_fields(G, dtype, uf, vf, wf, nbatch, W.partials, W.npartials, s);
-                fetch_energy(W, nbatch, strain_blocks(G), g.active_area, E.data(), s);
+                fetch_energy(W, nbatch, energy_blocks(G), g.active_area, E.data(), s);
                 for (int64_t b = 0; b < nbatch; b++) E_out[q + nscales * b] = E[b];
             }
             return;
@@ -1186,7 +1186,7 @@ cgef_status cgef_cumulative_energy(cgef_plan *const *plans, int64_t nscales, cge
             filt(v, vf);
             if (w) filt(w, wf);
             launch_energy_fields(G, dtype, uf, vf, wf, nbatch, W.partials, W.npartials, s);
-            fetch_energy(W, nbatch, strain_blocks(G), g.active_area, E.data(), s);
+            fetch_energy(W, nbatch, energy_blocks(G), g.active_area, E.data(), s);
             for (int64_t b = 0; b < nbatch; b++) E_out[q + nscales * b] = E[b];
         }
     });

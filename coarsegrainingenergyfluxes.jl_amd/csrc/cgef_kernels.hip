@@ -471,26 +471,33 @@ struct Stencil32 {
     double c0, c1, c2;
 };
 
+// Branch-free: every kind loads three weights from valid slots and the unused ones are replaced by 0
+// (nodes by t), so the loads of several points can be issued together. The weights and nodes it
+// returns are those of the branching form above.
 __device__ __forceinline__ Stencil32 stencil32(const GeomArgs &G, int axis, int kind, int t, int p, int stride) {
-    Stencil32 S{t, t, t, 0.0, 0.0, 0.0};
-    if (kind == STENCIL_NONE) return S;
     const int n = (int)G.n[axis];
-    const int o0 = kind == STENCIL_BACKWARD3 ? -2 : ((kind == STENCIL_CENTRED || kind == STENCIL_BACKWARD2) ? -1 : 0);
-    const int sb = kind == STENCIL_CENTRED ? 0 : kind == STENCIL_FORWARD3 ? 3 : kind == STENCIL_BACKWARD3 ? 6
-                 : kind == STENCIL_FORWARD2 ? 9 : 11;
+    const bool none = kind == STENCIL_NONE, three = !none && kind <= STENCIL_BACKWARD3;
+    // per kind (4-bit fields, kind 0 lowest): first node offset + 2 (centred −1, backward 3-node −2,
+    // backward 2-node −1, else 0) and first weight slot (none: 11, a valid slot whose weights are dropped)
+    const int o0 = (int)((0x120212u >> (4 * kind)) & 15u) - 2;
+    const int sb = (int)((0xB9630Bu >> (4 * kind)) & 15u);
     auto at = [&](int o) {   // t + (wrapped node − p)·stride
         const int q = p + o;
         return t + (o + (q < 0 ? n : 0) - (q >= n ? n : 0)) * stride;
     };
     const double *w = G.dcoef[axis] + p;  // slot s at w[s·n]
-    S.q0 = at(o0);
-    S.q1 = at(o0 + 1);
-    S.c0 = w[sb * n];
-    S.c1 = w[(sb + 1) * n];
-    if (kind <= STENCIL_BACKWARD3) {
-        S.q2 = at(o0 + 2);
-        S.c2 = w[(sb + 2) * n];
-    }
+    const double c0 = w[sb * n], c1 = w[(sb + 1) * n], c2 = w[(three ? sb + 2 : sb) * n];
+    // masks rather than conditional expressions, which the compiler turns into branches around loads
+    const int mi = none ? 0 : -1, m3 = three ? -1 : 0;
+    const uint64_t mw = none ? 0ull : ~0ull, mw3 = three ? ~0ull : 0ull;
+    auto keep = [](double c, uint64_t m) { return __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, c) & m); };
+    Stencil32 S;
+    S.q0 = t + ((at(o0) - t) & mi);
+    S.q1 = t + ((at(o0 + 1) - t) & mi);
+    S.q2 = t + ((at(o0 + 2) - t) & m3);
+    S.c0 = keep(c0, mw);
+    S.c1 = keep(c1, mw);
+    S.c2 = keep(c2, mw3);
     return S;
 }
 
@@ -506,6 +513,32 @@ __device__ __forceinline__ double apply_stencil(const Stencil32 &S, F f) {
 // point's code and go out with it; only the neighbours' indices and weights follow from the code.
 // Node indices are 32-bit within the slice; the sphere's row factors 1/(R cosφ) and tanφ/R come from
 // the grid's table (one uniform load each instead of two FP64 divisions per point).
+// S̄:τ in 2-D from the four derivatives of ū (frame components; on the sphere scaled by the row's metric
+// factors and with the curvature terms, Diagnostics.jl:652-669) and τ's xx, xy, yy at the point.
+template <int MODE>
+__device__ __forceinline__ double strain_contract_2d(const GeomArgs &G, int j, double dudx, double dudy, double dvdx,
+                                                     double dvdy, double ut, double vt, double txx, double txy,
+                                                     double tyy) {
+    double Sxx, Sxy, Syy;
+    if (MODE == 1) {
+        const int Nx = (int)G.n[0], Ny = (int)G.n[1];
+        const double hx = G.trig[2 * Nx + 2 * Ny + j], tf = G.trig[2 * Nx + 3 * Ny + j];
+        const double invR = 1.0 / G.R;   // uniform: once per wave
+        dudx *= hx;
+        dvdx *= hx;
+        dudy *= invR;
+        dvdy *= invR;
+        Sxx = dudx - vt * tf;
+        Syy = dvdy;
+        Sxy = 0.5 * (dudy + dvdx + ut * tf);
+    } else {
+        Sxx = dudx;
+        Syy = dvdy;
+        Sxy = 0.5 * (dudy + dvdx);
+    }
+    return Sxx * txx + 2.0 * Sxy * txy + Syy * tyy;
+}
+
 template <typename T, int MODE>
 __device__ __forceinline__ double strain_point(const GeomArgs &G, const double *__restrict__ U, const T *__restrict__ tb,
                                                int t, int i, int j, int k, double &e) {
@@ -530,55 +563,198 @@ __device__ __forceinline__ double strain_point(const GeomArgs &G, const double *
              2.0 * Sxz * (double)tb[2 * P + t] + 2.0 * Syz * (double)tb[4 * P + t] + Szz * (double)tb[5 * P + t];
         v2 = ut * ut + vt * vt + wt * wt;
     } else {
-        double dudx = apply_stencil(sx, fu), dudy = apply_stencil(sy, fu);
-        double dvdx = apply_stencil(sx, fv), dvdy = apply_stencil(sy, fv);
-        double Sxx, Sxy, Syy;
-        if (MODE == 1) {
-            const double hx = G.trig[2 * Nx + 2 * Ny + j], tf = G.trig[2 * Nx + 3 * Ny + j];
-            const double invR = 1.0 / G.R;   // uniform: once per wave
-            dudx *= hx;
-            dvdx *= hx;
-            dudy *= invR;
-            dvdy *= invR;
-            Sxx = dudx - vt * tf;
-            Syy = dvdy;
-            Sxy = 0.5 * (dudy + dvdx + ut * tf);
-        } else {
-            Sxx = dudx;
-            Syy = dvdy;
-            Sxy = 0.5 * (dudy + dvdx);
-        }
-        cc = Sxx * (double)tb[t] + 2.0 * Sxy * (double)tb[P + t] + Syy * (double)tb[2 * P + t];
+        cc = strain_contract_2d<MODE>(G, j, apply_stencil(sx, fu), apply_stencil(sy, fu), apply_stencil(sx, fv),
+                                      apply_stencil(sy, fv), ut, vt, (double)tb[t], (double)tb[P + t],
+                                      (double)tb[2 * P + t]);
         v2 = ut * ut + vt * vt;
     }
     const bool act = (code & STENCIL_ACTIVE) != 0;
-    e = act ? 0.5 * v2 * cell_area(G, i, j, k) : 0.0;
+    // cell_area for this MODE, the same products in the same order, without its branch on the mode
+    double area;
+    if (MODE == 1) area = G.R * G.R * G.trig[2 * Nx + Ny + j] * G.width[0][i] * G.width[1][j];
+    else if (MODE == 2) area = G.width[0][i] * G.width[1][j] * G.width[2][k];
+    else area = G.width[0][i] * G.width[1][j];
+    e = act ? 0.5 * v2 * area : 0.0;
     return act ? -cc : 0.0;
 }
 
-// Π = −ρ₀ S̄:τ, one point per lane, and per-block partial energies.
+// Π = −ρ₀ S̄:τ and per-block partial energies. A block covers 256 consecutive x indices (lane c) of
+// STRAIN_RJ consecutive rows j of one level k, 512 threads: thread (c, h) takes rows h, h + 2, ….
+constexpr int STRAIN_RJ = 8;
+constexpr int STRAIN_NT = 512;
+
+// Per-point loads (3-D volumes, odd row lengths): each point gathers its stencil nodes and weights.
 template <typename T, int MODE>
-__global__ __launch_bounds__(256) void k_strain_pi(GeomArgs G, const double *__restrict__ ubar, const T *__restrict__ tau,
-                                                   T *__restrict__ Pi, double rho0, double *partials, int64_t nblk,
-                                                   int64_t pi_stride, int64_t nchunk) {
-    __shared__ double sh[4];
+__global__ __launch_bounds__(STRAIN_NT) void k_strain_pi(GeomArgs G, const double *__restrict__ ubar,
+                                                         const T *__restrict__ tau, T *__restrict__ Pi, double rho0,
+                                                         double *partials, int nblk, int64_t pi_stride, int nchunk,
+                                                         int nrb) {
+    __shared__ double sh[STRAIN_NT / 64];
     const int64_t b = blockIdx.y;
-    const int64_t item = xcd_block(nblk);
+    const int item = (int)xcd_block(nblk);
     if (item >= nblk) return;  // whole block: padding of the XCD-ordered launch
-    const RowChunk c = row_chunk(G, nchunk, item);
+    const int rk = item / nchunk;   // level · nrb + row block
+    const int i = (item - rk * nchunk) * 256 + (int)(threadIdx.x & 255);
+    const int k = rk / nrb, j0 = (rk - k * nrb) * STRAIN_RJ;
+    const int Nx = (int)G.n[0], Ny = (int)G.n[1];
+    const double *U = ubar + b * 3 * G.npts;
+    const T *tb = tau + b * 6 * G.npts;
+    T *Pb = Pi + b * pi_stride;
     double e = 0.0;
-    if (c.i < G.n[0]) {
-        const double out = strain_point<T, MODE>(G, ubar + b * 3 * G.npts, tau + b * 6 * G.npts, (int)c.t, (int)c.i,
-                                                  (int)c.j, (int)c.k, e);
-        Pi[b * pi_stride + c.t] = (T)(rho0 * out);
-    }
+    if (i < Nx)
+        for (int q = (int)(threadIdx.x >> 8); q < STRAIN_RJ && j0 + q < Ny; q += 2) {
+            const int j = j0 + q, t = i + Nx * (j + Ny * k);
+            double eq;
+            const double out = strain_point<T, MODE>(G, U, tb, t, i, j, k, eq);
+            Pb[t] = (T)(rho0 * out);
+            e += eq;
+        }
     if (partials) {
-        const double s = block_sum<256>(e, sh);
+        const double s = block_sum<STRAIN_NT>(e, sh);
         if (threadIdx.x == 0) partials[b * nblk + item] = s;
     }
 }
 
-int64_t strain_blocks(const GeomArgs &G) { return row_chunks(G) * G.n[1] * G.n[2]; }
+// 2-D with an even row length: ū, v̄ of the block's rows j0 − 2 … j0 + STRAIN_RJ + 1 and columns
+// i0 − 2 … i0 + 257 (wrapped on periodic axes, clamped otherwise: a stencil never reaches a clamped
+// node) and the block's derivative weights are staged in LDS by 16-byte loads, so a point reads from
+// memory only its stencil code and τ; per-point gathers of nodes and weights had the texture
+// addresser busy for most of the kernel. Same nodes, weights and arithmetic as k_strain_pi.
+constexpr int STRAIN_SR = STRAIN_RJ + 4, STRAIN_SC = 260;
+template <typename T, int MODE>
+__global__ __launch_bounds__(STRAIN_NT) void k_strain_pi_tile(GeomArgs G, const double *__restrict__ ubar,
+                                                              const T *__restrict__ tau, T *__restrict__ Pi,
+                                                              double rho0, double *partials, int nblk,
+                                                              int64_t pi_stride, int nchunk, int nrb) {
+    __shared__ __attribute__((aligned(16))) double su[STRAIN_SR * STRAIN_SC], sv[STRAIN_SR * STRAIN_SC];
+    __shared__ double sxw[DCOEF_SLOTS * 256], syw[STRAIN_RJ * DCOEF_SLOTS];
+    __shared__ double sh[STRAIN_NT / 64];
+    const int64_t b = blockIdx.y;
+    const int item = (int)xcd_block(nblk);
+    if (item >= nblk) return;  // whole block: padding of the XCD-ordered launch
+    const int rk = item / nchunk;
+    const int i0 = (item - rk * nchunk) * 256;
+    const int k = rk / nrb, j0 = (rk - k * nrb) * STRAIN_RJ;
+    const int Nx = (int)G.n[0], Ny = (int)G.n[1], P = (int)G.npts, lev = Nx * Ny * k;
+    const double *U = ubar + b * 3 * G.npts, *V = U + P;
+    const int tid = (int)threadIdx.x;
+    // Every staging load is issued before the first LDS store (a load-store loop waited out each
+    // load's latency in turn): a thread's index runs past the end as repeats of the last element,
+    // which store the same value to the same place.
+    constexpr int NPAIR = STRAIN_SR * (STRAIN_SC / 2), NPS = (2 * NPAIR + STRAIN_NT - 1) / STRAIN_NT;
+    constexpr int NW = DCOEF_SLOTS * 256, NWS = (NW + STRAIN_NT - 1) / STRAIN_NT;
+    double2 xs[NPS];
+    double ws[NWS];
+#pragma unroll
+    for (int n = 0; n < NPS; n++) {
+        const int e = min(tid + n * STRAIN_NT, 2 * NPAIR - 1);
+        const int f = e >= NPAIR, ee = e - f * NPAIR;
+        const int r = ee / (STRAIN_SC / 2), pc = ee - r * (STRAIN_SC / 2);
+        int jr = j0 - 2 + r, gc = i0 - 2 + 2 * pc;   // gc even, Nx even: a pair never straddles the wrap
+        if (G.periodic[1]) jr = (int)wrapi(jr, Ny);
+        else jr = min(max(jr, 0), Ny - 1);
+        if (G.periodic[0]) gc = (int)wrapi(gc, Nx);
+        else gc = min(max(gc, 0), Nx - 2);
+        xs[n] = *reinterpret_cast<const double2 *>((f ? V : U) + lev + jr * Nx + gc);
+    }
+#pragma unroll
+    for (int n = 0; n < NWS; n++) {
+        const int e = min(tid + n * STRAIN_NT, NW - 1);
+        ws[n] = G.dcoef[0][(e >> 8) * Nx + min(i0 + (e & 255), Nx - 1)];
+    }
+    // the thread's points: column c, rows h, h + 2, …; their codes and τ requested with the staging
+    // (a column past the row's end or a row past the last repeats the last and is not stored)
+    const int c = tid & 255, i = i0 + c, h = tid >> 8;
+    const T *tb = tau + b * 6 * G.npts;
+    constexpr int NQ = STRAIN_RJ / 2;
+    unsigned codes[NQ];
+    double txx[NQ], txy[NQ], tyy[NQ];
+#pragma unroll
+    for (int n = 0; n < NQ; n++) {
+        const int t = lev + min(j0 + h + 2 * n, Ny - 1) * Nx + min(i, Nx - 1);
+        codes[n] = G.scode[t];
+        txx[n] = (double)tb[t];
+        txy[n] = (double)tb[P + t];
+        tyy[n] = (double)tb[2 * P + t];
+    }
+    const int qy = min(tid, STRAIN_RJ * DCOEF_SLOTS - 1) / DCOEF_SLOTS;
+    const double wy = G.dcoef[1][(min(tid, STRAIN_RJ * DCOEF_SLOTS - 1) - qy * DCOEF_SLOTS) * Ny + min(j0 + qy, Ny - 1)];
+#pragma unroll
+    for (int n = 0; n < NPS; n++) {
+        const int e = min(tid + n * STRAIN_NT, 2 * NPAIR - 1);
+        const int f = e >= NPAIR, ee = e - f * NPAIR;
+        const int r = ee / (STRAIN_SC / 2), pc = ee - r * (STRAIN_SC / 2);
+        *reinterpret_cast<double2 *>((f ? sv : su) + r * STRAIN_SC + 2 * pc) = xs[n];
+    }
+#pragma unroll
+    for (int n = 0; n < NWS; n++) sxw[min(tid + n * STRAIN_NT, NW - 1)] = ws[n];
+    syw[min(tid, STRAIN_RJ * DCOEF_SLOTS - 1)] = wy;
+#pragma unroll
+    for (int n = 0; n < NQ; n++)   // keeps the compiler from sinking these loads below the barrier
+        asm volatile("" ::"v"(codes[n]), "v"(txx[n]), "v"(txy[n]), "v"(tyy[n]));
+    __syncthreads();
+    T *Pb = Pi + b * pi_stride;
+    double e = 0.0;
+    if (i < Nx) {
+        const double wxi = G.width[0][i];
+#pragma unroll
+        for (int n = 0; n < NQ; n++) {
+            const int q = h + 2 * n, j = min(j0 + q, Ny - 1), t = lev + j * Nx + i;
+            const unsigned code = codes[n];
+            const double *pu = su + (q + 2) * STRAIN_SC + c + 2, *pv = sv + (q + 2) * STRAIN_SC + c + 2;
+            // the stencil of stencil32, as LDS offsets from the point and slots of the staged weights
+            auto stencil = [&](int kind, int &o0, int &o1, int &o2, int &s0, int &s1, int &s2, uint64_t &m01,
+                               uint64_t &m2) {
+                const bool none = kind == STENCIL_NONE, three = !none && kind <= STENCIL_BACKWARD3;
+                const int oo = (int)((0x120212u >> (4 * kind)) & 15u) - 2;
+                const int sb = (int)((0xB9630Bu >> (4 * kind)) & 15u);
+                const int mi = none ? 0 : -1, m3 = three ? -1 : 0;
+                o0 = oo & mi;
+                o1 = (oo + 1) & mi;
+                o2 = (oo + 2) & m3;
+                s0 = sb;
+                s1 = sb + 1;
+                s2 = three ? sb + 2 : sb;
+                m01 = none ? 0ull : ~0ull;
+                m2 = three ? ~0ull : 0ull;
+            };
+            auto keep = [](double w, uint64_t m) {
+                return __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, w) & m);
+            };
+            int x0, x1, x2, xs0, xs1, xs2, y0, y1, y2, ys0, ys1, ys2;
+            uint64_t xm, xm2, ym, ym2;
+            stencil(code & 7, x0, x1, x2, xs0, xs1, xs2, xm, xm2);
+            stencil((code >> 3) & 7, y0, y1, y2, ys0, ys1, ys2, ym, ym2);
+            const double cx0 = keep(sxw[xs0 * 256 + c], xm), cx1 = keep(sxw[xs1 * 256 + c], xm),
+                         cx2 = keep(sxw[xs2 * 256 + c], xm2);
+            const double *yw = syw + q * DCOEF_SLOTS;
+            const double cy0 = keep(yw[ys0], ym), cy1 = keep(yw[ys1], ym), cy2 = keep(yw[ys2], ym2);
+            const double ut = pu[0], vt = pv[0];
+            const double dudx = cx0 * pu[x0] + cx1 * pu[x1] + cx2 * pu[x2];
+            const double dudy = cy0 * pu[y0 * STRAIN_SC] + cy1 * pu[y1 * STRAIN_SC] + cy2 * pu[y2 * STRAIN_SC];
+            const double dvdx = cx0 * pv[x0] + cx1 * pv[x1] + cx2 * pv[x2];
+            const double dvdy = cy0 * pv[y0 * STRAIN_SC] + cy1 * pv[y1 * STRAIN_SC] + cy2 * pv[y2 * STRAIN_SC];
+            const double cc = strain_contract_2d<MODE>(G, j, dudx, dudy, dvdx, dvdy, ut, vt, txx[n], txy[n], tyy[n]);
+            const double v2 = ut * ut + vt * vt;
+            const bool act = (code & STENCIL_ACTIVE) != 0;
+            double area;
+            if (MODE == 1) area = G.R * G.R * G.trig[2 * Nx + Ny + j] * wxi * G.width[1][j];
+            else area = wxi * G.width[1][j];
+            if (j0 + q < Ny) {
+                Pb[t] = (T)(rho0 * (act ? -cc : 0.0));
+                e += act ? 0.5 * v2 * area : 0.0;
+            }
+        }
+    }
+    if (partials) {
+        const double s = block_sum<STRAIN_NT>(e, sh);
+        if (threadIdx.x == 0) partials[b * nblk + item] = s;
+    }
+}
+
+int64_t strain_blocks(const GeomArgs &G) { return row_chunks(G) * ((G.n[1] + STRAIN_RJ - 1) / STRAIN_RJ) * G.n[2]; }
+
+int64_t energy_blocks(const GeomArgs &G) { return row_chunks(G) * G.n[1] * G.n[2]; }
 
 void launch_strain_pi(const GeomArgs &G, int dtype, const double *ubar, const void *tau, void *Pi, double rho0,
                       int64_t nbatch, double *partials, int64_t npartials, hipStream_t s, int64_t pi_stride) {
@@ -587,16 +763,22 @@ void launch_strain_pi(const GeomArgs &G, int dtype, const double *ubar, const vo
     double *pp = (partials && nblk * nbatch <= npartials) ? partials : nullptr;
     dim3 grid(xcd_grid(nblk), (unsigned)nbatch);
     const int64_t ps = pi_stride > 0 ? pi_stride : G.npts;
-    const int64_t nch = row_chunks(G);
-#define CGEF_SP(TT, MM) hipLaunchKernelGGL((k_strain_pi<TT, MM>), grid, dim3(256), 0, s, G, ubar, (const TT *)tau, (TT *)Pi, rho0, pp, nblk, ps, nch)
+    const int nch = (int)row_chunks(G), nrb = (int)((G.n[1] + STRAIN_RJ - 1) / STRAIN_RJ);
+    // the LDS-staged tile kernel for 2-D slices with 16-byte aligned pairs of an even row length
+    const bool tile = G.mode != 2 && G.n[2] == 1 && G.n[0] % 2 == 0 && (reinterpret_cast<uintptr_t>(ubar) & 15) == 0;
+#define CGEF_SP(KK, TT, MM) hipLaunchKernelGGL((KK<TT, MM>), grid, dim3(STRAIN_NT), 0, s, G, ubar, (const TT *)tau, (TT *)Pi, rho0, pp, (int)nblk, ps, nch, nrb)
     if (dtype == CGEF_F32) {
-        if (G.mode == 0) CGEF_SP(float, 0);
-        else if (G.mode == 1) CGEF_SP(float, 1);
-        else CGEF_SP(float, 2);
+        if (G.mode == 0 && tile) CGEF_SP(k_strain_pi_tile, float, 0);
+        else if (G.mode == 1 && tile) CGEF_SP(k_strain_pi_tile, float, 1);
+        else if (G.mode == 0) CGEF_SP(k_strain_pi, float, 0);
+        else if (G.mode == 1) CGEF_SP(k_strain_pi, float, 1);
+        else CGEF_SP(k_strain_pi, float, 2);
     } else {
-        if (G.mode == 0) CGEF_SP(double, 0);
-        else if (G.mode == 1) CGEF_SP(double, 1);
-        else CGEF_SP(double, 2);
+        if (G.mode == 0 && tile) CGEF_SP(k_strain_pi_tile, double, 0);
+        else if (G.mode == 1 && tile) CGEF_SP(k_strain_pi_tile, double, 1);
+        else if (G.mode == 0) CGEF_SP(k_strain_pi, double, 0);
+        else if (G.mode == 1) CGEF_SP(k_strain_pi, double, 1);
+        else CGEF_SP(k_strain_pi, double, 2);
     }
 #undef CGEF_SP
 }
@@ -640,7 +822,7 @@ __global__ __launch_bounds__(256) void k_energy_fields(GeomArgs G, const T *ub, 
 
 void launch_energy_fields(const GeomArgs &G, int dtype, const void *ub, const void *vb, const void *wb, int64_t nbatch,
                           double *partials, int64_t npartials, hipStream_t s) {
-    const int64_t nblk = strain_blocks(G);
+    const int64_t nblk = energy_blocks(G);
     (void)npartials;
     dim3 grid(xcd_grid(nblk), (unsigned)nbatch);
     if (dtype == CGEF_F32)

@@ -57,7 +57,8 @@ void launch_strain_pi(const GeomArgs &G, int dtype, const double *ubar, const vo
                       int64_t pi_stride = 0);
 void launch_energy_final(int64_t nbatch, int64_t nblocks, const double *partials, double active_area,
                          double *E_dev, hipStream_t s);
-int64_t strain_blocks(const GeomArgs &G);
+int64_t strain_blocks(const GeomArgs &G);   // partial sums written by the strain / Π kernel per slice
+int64_t energy_blocks(const GeomArgs &G);   // … and by the energy-of-fields kernel (at least as many)
 void launch_filter_lin(const GeomArgs &G, const BandArgs &B, int dtype, bool masked, bool edge_x,
                        const void *in, void *out, int64_t nslices, hipStream_t s);
 void launch_baseline_filter(const GeomArgs &G, const Plan &p, const double *in, double *out, int64_t nslices,

@@ -54,7 +54,7 @@ extern "C" cgef_status cgef_energy_noalloc(const cgef_grid *grid, const void *ub
         if (dtype != CGEF_F32 && dtype != CGEF_F64) fail(CGEF_ERR_ARGUMENT, "dtype must be CGEF_F32 or CGEF_F64");
         if (!(g->active_area > 0)) fail(CGEF_ERR_ARGUMENT, "grid has no active cells (all masked out)");
         const GeomArgs G = geom_args(*g);
-        const int64_t nblk = strain_blocks(G);
+        const int64_t nblk = energy_blocks(G);
         if (nbatch > NOALLOC_SLICES || nblk > NOALLOC_PARTIALS / nbatch)
             fail(CGEF_ERR_DIMENSION,
                  "cgef_energy_noalloc holds at most " + std::to_string(NOALLOC_SLICES) + " slices and " +

@@ -320,6 +320,27 @@ def test_pi_3d_fp32():
     assert rel_l2(N(Pi), ref) <= 1e-5
 
 
+@pytest.mark.parametrize("masked", [False, True])
+def test_pi_3d_contraction_variants_same_bits(masked, monkeypatch):
+    """The three VALU contraction variants of a multi-plane (dz) footprint (8 or 4 columns per
+    block at one wave per SIMD, 6 columns at six) give the same bits, and the oracle's Π."""
+    og, g = volume_case(True, masked)
+    u, v, w = smooth_random_field(g.field_shape, 18, 3)
+    ker = cgef.TopHatKernel()
+    plan = cgef.plan_filter(g, ker, 6.0, engine="ring")
+    outs = []
+    for variant in ("0", "1", "2"):
+        print(f"3-D contraction variant {variant} masked={masked}", flush=True)
+        monkeypatch.setenv("CGEF_CONTRACT_VARIANT", variant)
+        Pi = torch.empty(g.field_shape, dtype=torch.float64, device=DEV)
+        cgef.compute_pi_(Pi, T(u), T(v), T(w), g, ker, 6.0, filter_plan=plan)
+        outs.append(N(Pi))
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])
+    ref, _, _, _ = O.compute_pi(og, u, v, w, kind=O.TOPHAT, alpha=0.0, ell=6.0, engine=O.ENGINE_ND3)
+    assert rel_l2(outs[-1], ref) <= 1e-10
+
+
 def test_derivative_z_stretched():
     og, g = volume_case(True, True)
     f = smooth_random_field(g.field_shape, 17, 1)[0]
