@@ -617,17 +617,18 @@ __global__ __launch_bounds__(STRAIN_NT) void k_strain_pi(GeomArgs G, const doubl
 
 // 2-D with an even row length: ū, v̄ of the block's rows j0 − 2 … j0 + STRAIN_RJ + 1 and columns
 // i0 − 2 … i0 + 257 (wrapped on periodic axes, clamped otherwise: a stencil never reaches a clamped
-// node) and the block's derivative weights are staged in LDS by 16-byte loads, so a point reads from
-// memory only its stencil code and τ; per-point gathers of nodes and weights had the texture
-// addresser busy for most of the kernel. Same nodes, weights and arithmetic as k_strain_pi.
+// node) are staged in LDS by 16-byte loads with the y weights of its rows; a lane holds its column's
+// centred x weights. A point reads from memory only its stencil code and τ (and the x weights of a
+// non-centred kind); per-point gathers of nodes and weights had the texture addresser busy for most of
+// the kernel. Same nodes, weights and arithmetic as k_strain_pi.
 constexpr int STRAIN_SR = STRAIN_RJ + 4, STRAIN_SC = 260;
 template <typename T, int MODE>
-__global__ __launch_bounds__(STRAIN_NT) void k_strain_pi_tile(GeomArgs G, const double *__restrict__ ubar,
+__global__ __launch_bounds__(STRAIN_NT) __attribute__((amdgpu_waves_per_eu(6, 6))) void k_strain_pi_tile(GeomArgs G, const double *__restrict__ ubar,
                                                               const T *__restrict__ tau, T *__restrict__ Pi,
                                                               double rho0, double *partials, int nblk,
                                                               int64_t pi_stride, int nchunk, int nrb) {
     __shared__ __attribute__((aligned(16))) double su[STRAIN_SR * STRAIN_SC], sv[STRAIN_SR * STRAIN_SC];
-    __shared__ double sxw[DCOEF_SLOTS * 256], syw[STRAIN_RJ * DCOEF_SLOTS];
+    __shared__ double syw[STRAIN_RJ * DCOEF_SLOTS];
     __shared__ double sh[STRAIN_NT / 64];
     const int64_t b = blockIdx.y;
     const int item = (int)xcd_block(nblk);
@@ -642,9 +643,7 @@ __global__ __launch_bounds__(STRAIN_NT) void k_strain_pi_tile(GeomArgs G, const 
     // load's latency in turn): a thread's index runs past the end as repeats of the last element,
     // which store the same value to the same place.
     constexpr int NPAIR = STRAIN_SR * (STRAIN_SC / 2), NPS = (2 * NPAIR + STRAIN_NT - 1) / STRAIN_NT;
-    constexpr int NW = DCOEF_SLOTS * 256, NWS = (NW + STRAIN_NT - 1) / STRAIN_NT;
     double2 xs[NPS];
-    double ws[NWS];
 #pragma unroll
     for (int n = 0; n < NPS; n++) {
         const int e = min(tid + n * STRAIN_NT, 2 * NPAIR - 1);
@@ -657,25 +656,23 @@ __global__ __launch_bounds__(STRAIN_NT) void k_strain_pi_tile(GeomArgs G, const 
         else gc = min(max(gc, 0), Nx - 2);
         xs[n] = *reinterpret_cast<const double2 *>((f ? V : U) + lev + jr * Nx + gc);
     }
-#pragma unroll
-    for (int n = 0; n < NWS; n++) {
-        const int e = min(tid + n * STRAIN_NT, NW - 1);
-        ws[n] = G.dcoef[0][(e >> 8) * Nx + min(i0 + (e & 255), Nx - 1)];
-    }
     // the thread's points: column c, rows h, h + 2, …; their codes and τ requested with the staging
     // (a column past the row's end or a row past the last repeats the last and is not stored)
-    const int c = tid & 255, i = i0 + c, h = tid >> 8;
+    const int c = tid & 255, i = i0 + c, h = tid >> 8, ic = min(i, Nx - 1);
     const T *tb = tau + b * 6 * G.npts;
+    // the column's centred x weights (slots 0-2); other kinds read theirs when they occur
+    const double *wxc = G.dcoef[0] + ic;
+    const double cxc0 = wxc[0], cxc1 = wxc[Nx], cxc2 = wxc[2 * Nx];
     constexpr int NQ = STRAIN_RJ / 2;
     unsigned codes[NQ];
-    double txx[NQ], txy[NQ], tyy[NQ];
+    T txx[NQ], txy[NQ], tyy[NQ];   // held in the storage type until used
 #pragma unroll
     for (int n = 0; n < NQ; n++) {
-        const int t = lev + min(j0 + h + 2 * n, Ny - 1) * Nx + min(i, Nx - 1);
+        const int t = lev + min(j0 + h + 2 * n, Ny - 1) * Nx + ic;
         codes[n] = G.scode[t];
-        txx[n] = (double)tb[t];
-        txy[n] = (double)tb[P + t];
-        tyy[n] = (double)tb[2 * P + t];
+        txx[n] = tb[t];
+        txy[n] = tb[P + t];
+        tyy[n] = tb[2 * P + t];
     }
     const int qy = min(tid, STRAIN_RJ * DCOEF_SLOTS - 1) / DCOEF_SLOTS;
     const double wy = G.dcoef[1][(min(tid, STRAIN_RJ * DCOEF_SLOTS - 1) - qy * DCOEF_SLOTS) * Ny + min(j0 + qy, Ny - 1)];
@@ -686,12 +683,10 @@ __global__ __launch_bounds__(STRAIN_NT) void k_strain_pi_tile(GeomArgs G, const 
         const int r = ee / (STRAIN_SC / 2), pc = ee - r * (STRAIN_SC / 2);
         *reinterpret_cast<double2 *>((f ? sv : su) + r * STRAIN_SC + 2 * pc) = xs[n];
     }
-#pragma unroll
-    for (int n = 0; n < NWS; n++) sxw[min(tid + n * STRAIN_NT, NW - 1)] = ws[n];
     syw[min(tid, STRAIN_RJ * DCOEF_SLOTS - 1)] = wy;
 #pragma unroll
     for (int n = 0; n < NQ; n++)   // keeps the compiler from sinking these loads below the barrier
-        asm volatile("" ::"v"(codes[n]), "v"(txx[n]), "v"(txy[n]), "v"(tyy[n]));
+        asm volatile("" ::"v"(codes[n]), "v"(txx[n]), "v"(txy[n]), "v"(tyy[n]), "v"(cxc0), "v"(cxc1), "v"(cxc2));
     __syncthreads();
     T *Pb = Pi + b * pi_stride;
     double e = 0.0;
@@ -725,8 +720,12 @@ __global__ __launch_bounds__(STRAIN_NT) void k_strain_pi_tile(GeomArgs G, const 
             uint64_t xm, xm2, ym, ym2;
             stencil(code & 7, x0, x1, x2, xs0, xs1, xs2, xm, xm2);
             stencil((code >> 3) & 7, y0, y1, y2, ys0, ys1, ys2, ym, ym2);
-            const double cx0 = keep(sxw[xs0 * 256 + c], xm), cx1 = keep(sxw[xs1 * 256 + c], xm),
-                         cx2 = keep(sxw[xs2 * 256 + c], xm2);
+            double cx0 = cxc0, cx1 = cxc1, cx2 = cxc2;
+            if ((code & 7) != STENCIL_CENTRED) {
+                cx0 = keep(wxc[xs0 * Nx], xm);
+                cx1 = keep(wxc[xs1 * Nx], xm);
+                cx2 = keep(wxc[xs2 * Nx], xm2);
+            }
             const double *yw = syw + q * DCOEF_SLOTS;
             const double cy0 = keep(yw[ys0], ym), cy1 = keep(yw[ys1], ym), cy2 = keep(yw[ys2], ym2);
             const double ut = pu[0], vt = pv[0];
@@ -734,7 +733,8 @@ __global__ __launch_bounds__(STRAIN_NT) void k_strain_pi_tile(GeomArgs G, const 
             const double dudy = cy0 * pu[y0 * STRAIN_SC] + cy1 * pu[y1 * STRAIN_SC] + cy2 * pu[y2 * STRAIN_SC];
             const double dvdx = cx0 * pv[x0] + cx1 * pv[x1] + cx2 * pv[x2];
             const double dvdy = cy0 * pv[y0 * STRAIN_SC] + cy1 * pv[y1 * STRAIN_SC] + cy2 * pv[y2 * STRAIN_SC];
-            const double cc = strain_contract_2d<MODE>(G, j, dudx, dudy, dvdx, dvdy, ut, vt, txx[n], txy[n], tyy[n]);
+            const double cc = strain_contract_2d<MODE>(G, j, dudx, dudy, dvdx, dvdy, ut, vt, (double)txx[n],
+                                                       (double)txy[n], (double)tyy[n]);
             const double v2 = ut * ut + vt * vt;
             const bool act = (code & STENCIL_ACTIVE) != 0;
             double area;
