@@ -620,10 +620,11 @@ __global__ __launch_bounds__(STRAIN_NT) void k_strain_pi(GeomArgs G, const doubl
 // node) are staged in LDS by 16-byte loads with the y weights of its rows; a lane holds its column's
 // centred x weights. A point reads from memory only its stencil code and τ (and the x weights of a
 // non-centred kind); per-point gathers of nodes and weights had the texture addresser busy for most of
-// the kernel. Same nodes, weights and arithmetic as k_strain_pi.
+// the kernel. Same nodes, weights and arithmetic as k_strain_pi. Six waves per SIMD (three blocks per
+// CU by LDS) for FP32 τ; FP64 τ needs the registers of four.
 constexpr int STRAIN_SR = STRAIN_RJ + 4, STRAIN_SC = 260;
 template <typename T, int MODE>
-__global__ __launch_bounds__(STRAIN_NT) __attribute__((amdgpu_waves_per_eu(6, 6))) void k_strain_pi_tile(GeomArgs G, const double *__restrict__ ubar,
+__global__ __launch_bounds__(STRAIN_NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 6 : 4, sizeof(T) == 4 ? 6 : 4))) void k_strain_pi_tile(GeomArgs G, const double *__restrict__ ubar,
                                                               const T *__restrict__ tau, T *__restrict__ Pi,
                                                               double rho0, double *partials, int nblk,
                                                               int64_t pi_stride, int nchunk, int nrb) {
@@ -686,7 +687,7 @@ __global__ __launch_bounds__(STRAIN_NT) __attribute__((amdgpu_waves_per_eu(6, 6)
     syw[min(tid, STRAIN_RJ * DCOEF_SLOTS - 1)] = wy;
 #pragma unroll
     for (int n = 0; n < NQ; n++)   // keeps the compiler from sinking these loads below the barrier
-        asm volatile("" ::"v"(codes[n]), "v"(txx[n]), "v"(txy[n]), "v"(tyy[n]), "v"(cxc0), "v"(cxc1), "v"(cxc2));
+        asm volatile("" ::"v"(codes[n]), "v"(txx[n]), "v"(txy[n]), "v"(tyy[n]));
     __syncthreads();
     T *Pb = Pi + b * pi_stride;
     double e = 0.0;
