@@ -624,12 +624,14 @@ __global__ __launch_bounds__(STRAIN_NT) void k_strain_pi(GeomArgs G, const doubl
 // CU by LDS) for FP32 τ; FP64 τ needs the registers of four.
 constexpr int STRAIN_SR = STRAIN_RJ + 4, STRAIN_SC = 260;
 template <typename T, int MODE>
-__global__ __launch_bounds__(STRAIN_NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 6 : 4, sizeof(T) == 4 ? 6 : 4))) void k_strain_pi_tile(GeomArgs G, const double *__restrict__ ubar,
+__global__ __launch_bounds__(STRAIN_NT) __attribute__((amdgpu_waves_per_eu(MODE != 2 && sizeof(T) == 4 ? 6 : 4, MODE != 2 && sizeof(T) == 4 ? 6 : 4))) void k_strain_pi_tile(GeomArgs G, const double *__restrict__ ubar,
                                                               const T *__restrict__ tau, T *__restrict__ Pi,
                                                               double rho0, double *partials, int nblk,
                                                               int64_t pi_stride, int nchunk, int nrb) {
-    __shared__ __attribute__((aligned(16))) double su[STRAIN_SR * STRAIN_SC], sv[STRAIN_SR * STRAIN_SC];
-    __shared__ double syw[STRAIN_RJ * DCOEF_SLOTS];
+    constexpr int NF = MODE == 2 ? 3 : 2, SFL = STRAIN_SR * STRAIN_SC;   // staged fields: ū, v̄ (, w̄)
+    __shared__ __attribute__((aligned(16))) double sfl[NF * SFL];
+    __shared__ double syw[STRAIN_RJ * DCOEF_SLOTS], szw[DCOEF_SLOTS];
+    double *su = sfl, *sv = sfl + SFL;
     __shared__ double sh[STRAIN_NT / 64];
     const int64_t b = blockIdx.y;
     const int item = (int)xcd_block(nblk);
@@ -638,24 +640,24 @@ __global__ __launch_bounds__(STRAIN_NT) __attribute__((amdgpu_waves_per_eu(sizeo
     const int i0 = (item - rk * nchunk) * 256;
     const int k = rk / nrb, j0 = (rk - k * nrb) * STRAIN_RJ;
     const int Nx = (int)G.n[0], Ny = (int)G.n[1], P = (int)G.npts, lev = Nx * Ny * k;
-    const double *U = ubar + b * 3 * G.npts, *V = U + P;
+    const double *U = ubar + b * 3 * G.npts;
     const int tid = (int)threadIdx.x;
     // Every staging load is issued before the first LDS store (a load-store loop waited out each
     // load's latency in turn): a thread's index runs past the end as repeats of the last element,
     // which store the same value to the same place.
-    constexpr int NPAIR = STRAIN_SR * (STRAIN_SC / 2), NPS = (2 * NPAIR + STRAIN_NT - 1) / STRAIN_NT;
+    constexpr int NPAIR = STRAIN_SR * (STRAIN_SC / 2), NPS = (NF * NPAIR + STRAIN_NT - 1) / STRAIN_NT;
     double2 xs[NPS];
 #pragma unroll
     for (int n = 0; n < NPS; n++) {
-        const int e = min(tid + n * STRAIN_NT, 2 * NPAIR - 1);
-        const int f = e >= NPAIR, ee = e - f * NPAIR;
+        const int e = min(tid + n * STRAIN_NT, NF * NPAIR - 1);
+        const int f = e / NPAIR, ee = e - f * NPAIR;
         const int r = ee / (STRAIN_SC / 2), pc = ee - r * (STRAIN_SC / 2);
         int jr = j0 - 2 + r, gc = i0 - 2 + 2 * pc;   // gc even, Nx even: a pair never straddles the wrap
         if (G.periodic[1]) jr = (int)wrapi(jr, Ny);
         else jr = min(max(jr, 0), Ny - 1);
         if (G.periodic[0]) gc = (int)wrapi(gc, Nx);
         else gc = min(max(gc, 0), Nx - 2);
-        xs[n] = *reinterpret_cast<const double2 *>((f ? V : U) + lev + jr * Nx + gc);
+        xs[n] = *reinterpret_cast<const double2 *>(U + f * P + lev + jr * Nx + gc);
     }
     // the thread's points: column c, rows h, h + 2, …; their codes and τ requested with the staging
     // (a column past the row's end or a row past the last repeats the last and is not stored)
@@ -671,23 +673,26 @@ __global__ __launch_bounds__(STRAIN_NT) __attribute__((amdgpu_waves_per_eu(sizeo
     for (int n = 0; n < NQ; n++) {
         const int t = lev + min(j0 + h + 2 * n, Ny - 1) * Nx + ic;
         codes[n] = G.scode[t];
-        txx[n] = tb[t];
-        txy[n] = tb[P + t];
-        tyy[n] = tb[2 * P + t];
+        if (MODE != 2) {   // 3-D: τ's six components are read with the z nodes, after the barrier
+            txx[n] = tb[t];
+            txy[n] = tb[P + t];
+            tyy[n] = tb[2 * P + t];
+        }
     }
     const int qy = min(tid, STRAIN_RJ * DCOEF_SLOTS - 1) / DCOEF_SLOTS;
     const double wy = G.dcoef[1][(min(tid, STRAIN_RJ * DCOEF_SLOTS - 1) - qy * DCOEF_SLOTS) * Ny + min(j0 + qy, Ny - 1)];
 #pragma unroll
     for (int n = 0; n < NPS; n++) {
-        const int e = min(tid + n * STRAIN_NT, 2 * NPAIR - 1);
-        const int f = e >= NPAIR, ee = e - f * NPAIR;
+        const int e = min(tid + n * STRAIN_NT, NF * NPAIR - 1);
+        const int f = e / NPAIR, ee = e - f * NPAIR;
         const int r = ee / (STRAIN_SC / 2), pc = ee - r * (STRAIN_SC / 2);
-        *reinterpret_cast<double2 *>((f ? sv : su) + r * STRAIN_SC + 2 * pc) = xs[n];
+        *reinterpret_cast<double2 *>(sfl + f * SFL + r * STRAIN_SC + 2 * pc) = xs[n];
     }
+    if (MODE == 2 && tid < DCOEF_SLOTS) szw[tid] = G.dcoef[2][tid * G.n[2] + k];   // the level's z weights
     syw[min(tid, STRAIN_RJ * DCOEF_SLOTS - 1)] = wy;
 #pragma unroll
     for (int n = 0; n < NQ; n++)   // keeps the compiler from sinking these loads below the barrier
-        asm volatile("" ::"v"(codes[n]), "v"(txx[n]), "v"(txy[n]), "v"(tyy[n]));
+        if (MODE != 2) asm volatile("" ::"v"(codes[n]), "v"(txx[n]), "v"(txy[n]), "v"(tyy[n]));
     __syncthreads();
     T *Pb = Pi + b * pi_stride;
     double e = 0.0;
@@ -734,13 +739,45 @@ __global__ __launch_bounds__(STRAIN_NT) __attribute__((amdgpu_waves_per_eu(sizeo
             const double dudy = cy0 * pu[y0 * STRAIN_SC] + cy1 * pu[y1 * STRAIN_SC] + cy2 * pu[y2 * STRAIN_SC];
             const double dvdx = cx0 * pv[x0] + cx1 * pv[x1] + cx2 * pv[x2];
             const double dvdy = cy0 * pv[y0 * STRAIN_SC] + cy1 * pv[y1 * STRAIN_SC] + cy2 * pv[y2 * STRAIN_SC];
-            const double cc = strain_contract_2d<MODE>(G, j, dudx, dudy, dvdx, dvdy, ut, vt, (double)txx[n],
-                                                       (double)txy[n], (double)tyy[n]);
-            const double v2 = ut * ut + vt * vt;
+            double cc, v2, area;
+            if constexpr (MODE == 2) {
+                // the z nodes of the point's column from memory (levels k + o, wrapped on a periodic axis),
+                // the level's z weights from LDS; the arithmetic of strain_point's 3-D branch
+                const double *pw = pu + 2 * SFL;
+                const int Nz = (int)G.n[2], NxNy = Nx * Ny;
+                int z0, z1, z2, zs0, zs1, zs2;
+                uint64_t zm, zm2;
+                stencil((code >> 6) & 7, z0, z1, z2, zs0, zs1, zs2, zm, zm2);
+                auto lvl = [&](int o) {   // the point's index at level k + o
+                    int kw = k + o;
+                    if (G.periodic[2]) kw = (int)((unsigned)(kw + 2 * Nz) % (unsigned)Nz);
+                    return i + Nx * j + NxNy * kw;
+                };
+                const int q0 = lvl(z0), q1 = lvl(z1), q2 = lvl(z2);
+                const double cz0 = keep(szw[zs0], zm), cz1 = keep(szw[zs1], zm), cz2 = keep(szw[zs2], zm2);
+                const double *Vp = U + P, *Wp = U + 2 * P;
+                const double wt = pw[0];
+                const double dwdx = cx0 * pw[x0] + cx1 * pw[x1] + cx2 * pw[x2];
+                const double dwdy = cy0 * pw[y0 * STRAIN_SC] + cy1 * pw[y1 * STRAIN_SC] + cy2 * pw[y2 * STRAIN_SC];
+                const double dudz = cz0 * U[q0] + cz1 * U[q1] + cz2 * U[q2];
+                const double dvdz = cz0 * Vp[q0] + cz1 * Vp[q1] + cz2 * Vp[q2];
+                const double dwdz = cz0 * Wp[q0] + cz1 * Wp[q1] + cz2 * Wp[q2];
+                const double Sxx = dudx, Syy = dvdy, Szz = dwdz;
+                const double Sxy = 0.5 * (dudy + dvdx);
+                const double Sxz = 0.5 * (dudz + dwdx);
+                const double Syz = 0.5 * (dvdz + dwdy);
+                cc = Sxx * (double)tb[t] + 2.0 * Sxy * (double)tb[P + t] + Syy * (double)tb[3 * P + t] +
+                     2.0 * Sxz * (double)tb[2 * P + t] + 2.0 * Syz * (double)tb[4 * P + t] + Szz * (double)tb[5 * P + t];
+                v2 = ut * ut + vt * vt + wt * wt;
+                area = wxi * G.width[1][j] * G.width[2][k];
+            } else {
+                cc = strain_contract_2d<MODE>(G, j, dudx, dudy, dvdx, dvdy, ut, vt, (double)txx[n], (double)txy[n],
+                                              (double)tyy[n]);
+                v2 = ut * ut + vt * vt;
+                if (MODE == 1) area = G.R * G.R * G.trig[2 * Nx + Ny + j] * wxi * G.width[1][j];
+                else area = wxi * G.width[1][j];
+            }
             const bool act = (code & STENCIL_ACTIVE) != 0;
-            double area;
-            if (MODE == 1) area = G.R * G.R * G.trig[2 * Nx + Ny + j] * wxi * G.width[1][j];
-            else area = wxi * G.width[1][j];
             if (j0 + q < Ny) {
                 Pb[t] = (T)(rho0 * (act ? -cc : 0.0));
                 e += act ? 0.5 * v2 * area : 0.0;
@@ -766,17 +803,19 @@ void launch_strain_pi(const GeomArgs &G, int dtype, const double *ubar, const vo
     const int64_t ps = pi_stride > 0 ? pi_stride : G.npts;
     const int nch = (int)row_chunks(G), nrb = (int)((G.n[1] + STRAIN_RJ - 1) / STRAIN_RJ);
     // the LDS-staged tile kernel for 2-D slices with 16-byte aligned pairs of an even row length
-    const bool tile = G.mode != 2 && G.n[2] == 1 && G.n[0] % 2 == 0 && (reinterpret_cast<uintptr_t>(ubar) & 15) == 0;
+    const bool tile = (G.mode == 2 || G.n[2] == 1) && G.n[0] % 2 == 0 && (reinterpret_cast<uintptr_t>(ubar) & 15) == 0;
 #define CGEF_SP(KK, TT, MM) hipLaunchKernelGGL((KK<TT, MM>), grid, dim3(STRAIN_NT), 0, s, G, ubar, (const TT *)tau, (TT *)Pi, rho0, pp, (int)nblk, ps, nch, nrb)
     if (dtype == CGEF_F32) {
         if (G.mode == 0 && tile) CGEF_SP(k_strain_pi_tile, float, 0);
         else if (G.mode == 1 && tile) CGEF_SP(k_strain_pi_tile, float, 1);
+        else if (G.mode == 2 && tile) CGEF_SP(k_strain_pi_tile, float, 2);
         else if (G.mode == 0) CGEF_SP(k_strain_pi, float, 0);
         else if (G.mode == 1) CGEF_SP(k_strain_pi, float, 1);
         else CGEF_SP(k_strain_pi, float, 2);
     } else {
         if (G.mode == 0 && tile) CGEF_SP(k_strain_pi_tile, double, 0);
         else if (G.mode == 1 && tile) CGEF_SP(k_strain_pi_tile, double, 1);
+        else if (G.mode == 2 && tile) CGEF_SP(k_strain_pi_tile, double, 2);
         else if (G.mode == 0) CGEF_SP(k_strain_pi, double, 0);
         else if (G.mode == 1) CGEF_SP(k_strain_pi, double, 1);
         else CGEF_SP(k_strain_pi, double, 2);
