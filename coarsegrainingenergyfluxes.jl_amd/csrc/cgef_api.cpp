@@ -949,8 +949,11 @@ cgef_status cgef_coarse_grain(cgef_plan *const *plans, int64_t nscales, cgef_wor
         const GeomArgs G = geom_args(g);
         const int64_t P = g.npts;
         const size_t es = esize(dtype);
-        double *Eall = (double *)ws_fbuf(W, (size_t)nscales * nbatch * sizeof(double));
         const int64_t nblk = strain_blocks(G);
+        // E(ℓ) per scale and slice, then each scale's partial sums in a slice of their own: one energy
+        // reduction after the last scale instead of one launch per scale
+        double *Eall = (double *)ws_fbuf(W, (size_t)nscales * nbatch * (1 + nblk) * sizeof(double));
+        double *Pall = Eall + nscales * nbatch;
         if (all_ring || all_spec) {
             Plan &p0 = *reinterpret_cast<Plan *>(plans[0]);
             int wm = 0;
@@ -972,8 +975,7 @@ cgef_status cgef_coarse_grain(cgef_plan *const *plans, int64_t nscales, cgef_wor
                         p.prof_end(st);
                         ring_pi_inverse_epilogue(p, W, G, dtype, nf, wm, nullptr, nullptr, nullptr, nbatch, fused, st);
                         launch_strain_pi(G, dtype, W.ubar, W.tau, (char *)Pi + (size_t)q * P * es, rho0, nbatch,
-                                         W.partials, W.npartials, st, nscales * P);
-                        if (E_out) launch_energy_final(nbatch, nblk, W.partials, g.active_area, Eall + q * nbatch, st);
+                                         Pall + q * nbatch * nblk, nbatch * nblk, st, nscales * P);
                     }
                 } else {
                     // the spectrum of the masked inputs is taken once and reused by every scale
@@ -990,19 +992,19 @@ cgef_status cgef_coarse_grain(cgef_plan *const *plans, int64_t nscales, cgef_wor
                         ring_spectral_epilogue(G, dtype, W.rxy, p.ring.invrenorm_d, W.ubar, W.tau, nullptr, nullptr,
                                                nullptr, nbatch, st);
                         launch_strain_pi(G, dtype, W.ubar, W.tau, (char *)Pi + (size_t)q * P * es, rho0, nbatch,
-                                         W.partials, W.npartials, st, nscales * P);
-                        if (E_out) launch_energy_final(nbatch, nblk, W.partials, g.active_area, Eall + q * nbatch, st);
+                                         Pall + q * nbatch * nblk, nbatch * nblk, st, nscales * P);
                     }
                 }
+                if (E_out) launch_energy_final(nscales * nbatch, nblk, Pall, g.active_area, Eall, st);
                 CGEF_HIP(hipGetLastError());
             };
             // Optionally the sweep as one HIP graph (CGEF_GRAPH=1): the first sweep with a given set
             // of arguments runs eagerly (and settles the contraction's measured variants), the second
             // is captured on the workspace's own stream and launched, later ones replay it. A graph
             // holds no memset node. Off by default: on MI355X the replayed 8-snapshot × 24-scale
-            // sweep on the 0.25° sphere measured 45.4 ms against 33.2 ms launched eagerly (its
-            // 145 launches run 0.1–2.6 ms each, so launch overhead is not what limits it). Plans
-            // being profiled always run eagerly. Calls that share a workspace must not overlap (as
+            // sweep on the 0.25° sphere measures no faster than launched eagerly (12.44–12.60 ms
+            // against 12.29–12.44 ms; its launches run 5 µs–0.3 ms each). Plans being profiled
+            // always run eagerly. Calls that share a workspace must not overlap (as
             // for every entry point using it).
             bool profiling = false;
             for (int64_t q = 0; q < nscales; q++) profiling = profiling || reinterpret_cast<Plan *>(plans[q])->profiling;
