@@ -1170,16 +1170,35 @@ __global__ __launch_bounds__(256) void k_sep_pi_row(GeomArgs G, const double *__
     double *su = srow, *sv = srow + L, *sg = srow + 2 * L;
     for (int e = threadIdx.x; e <= 2 * lx; e += 256) sg[e] = gx[e];
     const T *ub = u + b * P, *vb = v + b * P;
-    for (int sidx = threadIdx.x; sidx < L; sidx += 256) {
-        int64_t ii = i0 - lx + sidx;
-        bool in = true;
-        if (ii < 0 || ii >= Nx) {
-            if (G.periodic[0]) ii = wrapi(ii, Nx);
-            else in = false;
+    // two elements per lane per trip, both requested before either is stored (one trip for L ≤ 512);
+    // the wrap is 32-bit and branch-free (ii + wadd ≥ 0), so no branch separates the loads
+    const int nx = (int)Nx, wadd = (lx / nx + 1) * nx;
+    const bool per = G.periodic[0] != 0;
+    for (int s0 = 0; s0 < L; s0 += 512) {
+        T xu[2], xv[2];
+        bool ac[2];
+#pragma unroll
+        for (int m = 0; m < 2; m++) {
+            const int sidx = s0 + (int)threadIdx.x + 256 * m;
+            const int ii = (int)i0 - lx + sidx;
+            int iw = (int)((unsigned)(ii + wadd) % (unsigned)nx);
+            asm volatile("" : "+v"(iw));   // computed unconditionally
+            const bool inr = ii >= 0 && ii < nx;
+            const bool in = sidx < L && (inr || per);
+            const int64_t q = rowbase + (inr ? ii : (per ? iw : 0));   // a valid index either way
+            const bool mk = G.mask[q] != 0;                                 // read unconditionally
+            ac[m] = in && mk;
+            xu[m] = ub[q];
+            xv[m] = vb[q];
         }
-        const bool act = in && G.mask[rowbase + ii] != 0;
-        su[sidx] = act ? (double)ub[rowbase + ii] : 0.0;
-        sv[sidx] = act ? (double)vb[rowbase + ii] : 0.0;
+#pragma unroll
+        for (int m = 0; m < 2; m++) {
+            const int sidx = s0 + (int)threadIdx.x + 256 * m;
+            if (sidx < L) {
+                su[sidx] = ac[m] ? (double)xu[m] : 0.0;
+                sv[sidx] = ac[m] ? (double)xv[m] : 0.0;
+            }
+        }
     }
     __syncthreads();
     const int64_t i = i0 + threadIdx.x;
